@@ -1,0 +1,196 @@
+/*
+ * soap_gpu.h -- C ABI of libsoapgpu, the MI355X (gfx950) SOAP-GPU engine.
+ *
+ * The reference (YinanZhao21/SOAP_GPU) has no library interface: its stages are
+ * four executables called with positional argv and exchanging raw .bin files
+ * (SOAP_code/run_all_SOAP.py:226-250).  Each entry point below replaces one
+ * stage (or one step inside a stage); the comment on each names the reference
+ * code it stands in for.  The CLI shims built from soap_gpu_amd/driver keep
+ * the reference's argv and files on top of this ABI.
+ *
+ * Conventions
+ *  - All arithmetic is FP64.  Host arrays are caller-owned, contiguous,
+ *    little-endian double (int32 for the region map), sizes in elements.
+ *  - Device state lives in an opaque soap_ctx owned by the library; one ctx per
+ *    host thread.  Each ctx owns one HIP stream; entry points enqueue work on it
+ *    and return; soap_sync waits.  Calls that return data to the host
+ *    synchronise the stream themselves.
+ *  - Every call returns SOAP_OK (0) or a negative soap_status; the message of
+ *    the last failure is soap_last_error(ctx).
+ *  - Seed cubes are [n_mu][n] row-major (row k = mu bin k), the layout of the
+ *    reference's seed-cube .bin files (SOAP_integration.cu:504-580).
+ *  - Spectra produced per phase are stored [phase][type][n] with type
+ *    SOAP_FLUX, SOAP_BCONV, SOAP_TOT (file names of SOAP_integration.cu:802-816).
+ */
+#ifndef SOAP_GPU_H
+#define SOAP_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct soap_ctx soap_ctx;
+
+enum soap_status {
+  SOAP_OK = 0,
+  SOAP_ERR_ARG = -1,     /* bad argument (null pointer, size, range) */
+  SOAP_ERR_STATE = -2,   /* a required earlier step was not done */
+  SOAP_ERR_HIP = -3,     /* HIP runtime error */
+  SOAP_ERR_NOMEM = -4,   /* device allocation failed */
+  SOAP_ERR_GRID = -5,    /* wavelength grid not of the declared kind */
+  SOAP_ERR_LIMIT = -6    /* size beyond a compiled limit */
+};
+
+enum soap_grid_kind { SOAP_GRID_LINEAR = 0, SOAP_GRID_LOG = 1 };
+
+/* seed cubes, by the reference's file roles (run_all_SOAP.py:94-100) */
+enum soap_cube {
+  SOAP_CUBE_QUIET = 0,      /* new_noconti_convec_T_eff_{T}.bin       (normalised quiet)  */
+  SOAP_CUBE_QUIET_SED = 1,  /* new_conti_convec_T_eff_{T}.bin         (SED quiet)         */
+  SOAP_CUBE_SPOT = 2,       /* new_noconti_convec_T_eff_{T-dTs}.bin   (normalised spot)   */
+  SOAP_CUBE_SPOT_SED = 3,   /* new_conti_T_eff_{T-dTs}.bin            (SED spot, row 0)   */
+  SOAP_CUBE_FAC = 4,        /* new_noconti_convec_T_eff_{T+dTf}.bin   (normalised facula) */
+  SOAP_CUBE_FAC_SED = 5     /* new_conti_T_eff_{T+dTf}.bin            (SED facula, row 0) */
+};
+
+enum soap_spec_type { SOAP_FLUX = 0, SOAP_BCONV = 1, SOAP_TOT = 2 };
+
+/* star parameters: SOAP_initialize argv 2-8 and 20-23 (SOAP_initialize.cu:513-537) */
+typedef struct {
+  double prot_days;  /* argv[2] period */
+  double w_ratio;    /* argv[3] differential rotation ratio omega1/omega0 */
+  double incl_deg;   /* argv[4] inclination */
+  double radius_km;  /* argv[5] stellar radius */
+  double u1, u2;     /* argv[6,7] quadratic limb darkening */
+  double lb[4];      /* argv[20..23] limb-brightening polynomial, constant term first */
+  int32_t grid;      /* argv[8] disk grid size */
+} soap_star;
+
+/* activity parameters: SOAP_integration argv 2,4,5,7 (SOAP_integration.cu:469-497) */
+typedef struct {
+  int32_t nrho;        /* boundary points per region */
+  int32_t t_star;      /* K */
+  int32_t t_diff_spot; /* K */
+  int32_t t_diff_fac;  /* K */
+  int32_t solar_mode;  /* 1: facula intensity by Planck ratio (SOAP_integration_solar.cu:433) */
+} soap_activity;
+
+/* one active region: one line "lat lon size type" of maps{id}_for_all.txt
+ * (SOAP_integration.cu:669-682); type 0 = spot, 1 = facula */
+typedef struct {
+  double lat_deg, lon_deg, size, type;
+} soap_region;
+
+/* ------------------------------------------------------------ context */
+int soap_ctx_create(int device, soap_ctx **out);
+void soap_ctx_destroy(soap_ctx *ctx);
+const char *soap_last_error(const soap_ctx *ctx);
+int soap_sync(soap_ctx *ctx);
+/* library version string */
+const char *soap_version(void);
+
+/* ------------------------------------------------------------ inputs */
+/* wavelength grid (the solar_wave .bin, SOAP_initialize.cu:554-556).  kind LOG
+ * requires lambda_j = lambda_0 exp(j h) to 1e-12 relative (else SOAP_ERR_GRID);
+ * step is the reference's step_size argv (round(l[3]-l[2],5),
+ * run_all_SOAP.py:124) and is used for LINEAR grids only. */
+int soap_set_wave(soap_ctx *ctx, int kind, const double *wave, int64_t n, double step);
+/* mu boundaries (mu_boundary_array.bin, SOAP_initialize.cu:567-574); n_mu = n_bounds + 1 */
+int soap_set_mu_bounds(soap_ctx *ctx, const double *bounds, int32_t n_bounds);
+/* seed cube [n_mu][n] host -> device (the fread + cudaMemcpy of SOAP_integration.cu:504-580) */
+int soap_set_cube(soap_ctx *ctx, int which, const double *cube, int32_t n_mu);
+/* seed cube already resident on the device (not copied; must outlive its use) */
+int soap_set_cube_device(soap_ctx *ctx, int which, const double *dev_cube, int32_t n_mu);
+
+/* ------------------------------------------------------------ a1: disk grid */
+/* itot's per-cell loop (SOAP_initialize.cu:297-395) as one kernel: mu, projected
+ * velocity, limb darkening, limb brightening and mu bin for every cell */
+int soap_disk_init(soap_ctx *ctx, const soap_star *star);
+/* maps to host, any pointer may be NULL (the five *_map.bin, SOAP_initialize.cu:399-422) */
+int soap_get_maps(soap_ctx *ctx, double *mu, double *delta, double *limb, int32_t *region, double *lb);
+/* maps from host (the map .bin files read by SOAP_integration.cu:603-635) */
+int soap_set_maps(soap_ctx *ctx, const soap_star *star, const double *delta, const double *limb,
+                  const int32_t *region, const double *lb);
+
+/* ------------------------------------------------------------ a2: quiet star */
+/* quiet-star sums of the normalised and SED cubes (fast_shift + fast_interp_new +
+ * fast_sum over every disk cell, SOAP_initialize.cu:117-176,379-381; run twice by
+ * run_all_SOAP.py:226-234) */
+int soap_quiet_star(soap_ctx *ctx);
+/* which: 0 = normalised (_quiet_sun_spec.bin), 1 = SED (_SED_quiet_sun_spec.bin) */
+int soap_get_quiet(soap_ctx *ctx, int which, double *out);
+int soap_set_quiet(soap_ctx *ctx, int which, const double *q);
+
+/* ------------------------------------------------------------ a5/a6: activity */
+/* phases (rotations) and per-phase region lists (phase file + final_info.txt +
+ * maps{id}_for_all.txt, SOAP_integration.cu:637-682) */
+int soap_set_activity(soap_ctx *ctx, const soap_activity *act, int32_t n_phase, const double *phase,
+                      const int32_t *region_count, const soap_region *regions);
+/* per-phase active-region deltas flux/bconv/tot for every phase (the phase loop of
+ * SOAP_integration.cu:661-845: spot_init_gpu/spot_phase_gpu/spot_area/spot_scan +
+ * spot_integrate) */
+int soap_integrate(soap_ctx *ctx);
+/* raw spectra of one phase to host (_Spec_{flux,bconv,tot}_{id}_.bin); NULL skips */
+int soap_get_raw(soap_ctx *ctx, int32_t phase, double *flux, double *bconv, double *tot);
+/* raw spectra of one phase from host (SOAP_resolution.cu:398-412) */
+int soap_set_raw(soap_ctx *ctx, int32_t phase, const double *flux, const double *bconv, const double *tot);
+/* diagnostic temperature / binary maps of one phase (_temp_map_, _spot_binary_map_,
+ * SOAP_integration.cu:832-840) */
+int soap_phase_maps(soap_ctx *ctx, int32_t phase, double *temp_map, double *bi_map);
+
+/* ------------------------------------------------------------ a7: resolution */
+/* Gaussian instrumental convolution + spec_norm for every phase and type
+ * (lower_resolution_gpu, SOAP_resolution.cu:235-310) and for the two quiet
+ * frames (SOAP_initialize.cu:446-505).  sigma_ms = c/R/(2 sqrt(2 ln 2));
+ * quiet_weight = the wavelength_step_median argv of SOAP_initialize (LINEAR only). */
+int soap_resolution(soap_ctx *ctx, double sigma_ms, double quiet_weight);
+/* lowered spectrum: phase in [0, n_phase) and type, or phase = -1 for the quiet
+ * frames (type SOAP_BCONV -> _quiet_sun_lower_spec, else _SED_quiet_sun_lower_spec) */
+int soap_get_lower(soap_ctx *ctx, int32_t phase, int type, double *out);
+/* [min, max] of spec_norm for one spectrum (diagnostic) */
+int soap_get_norm(soap_ctx *ctx, int32_t phase, int type, double *minmax2);
+
+/* ------------------------------------------------------------ a8: CCF */
+/* mask lines after the driver's selection (run_all_SOAP.py:285-290), velocity grid
+ * (run_all_SOAP.py:262) and the pixel trim (cutoff 200 then 10, :259-281) */
+int soap_set_mask(soap_ctx *ctx, const double *line_l, const double *line_w, int32_t n_lines,
+                  const double *vgrid, int32_t n_v, int32_t trim);
+/* CCF of every frame and type (calculate_CCF_1/2 of CCF_func.py:5-91), normalised by
+ * its max and appodised (run_all_SOAP.py:306-314,368-370).  Frames 0..n_phase-1
+ * are the phases, frame n_phase the quiet star (run_all_SOAP.py:351-362). */
+int soap_ccf(soap_ctx *ctx);
+/* CCFs to host: out[(frame*3+type)*n_v + iv] */
+int soap_get_ccf(soap_ctx *ctx, double *out);
+
+/* ------------------------------------------------------------ a9: fit */
+/* Gaussian fit + bisector of every CCF (BIS_FIT2.cpp:97-320, one per frame and type) */
+int soap_fit(soap_ctx *ctx);
+/* results, row = frame*3+type: par[row*5 + {c,|k|,span,v0,fwhm}], model[row*n_v],
+ * depth[row*91], bis[row*91]; NULL skips */
+int soap_get_fit(soap_ctx *ctx, double *par, double *model, double *depth, double *bis);
+/* standalone batched fit of host CCFs (the BIS_FIT2 executable, one call per CCF
+ * in run_all_SOAP.py:316-394): v[n_v], y[n_ccf*n_v] */
+int soap_fit_ccf_host(soap_ctx *ctx, const double *v, int32_t n_v, const double *y, int32_t n_ccf,
+                      double *par, double *model, double *depth, double *bis);
+
+/* ------------------------------------------------------------ whole series */
+/* disk init + quiet star + integrate + resolution + CCF + fit, all on device */
+int soap_run_series(soap_ctx *ctx, const soap_star *star, double sigma_ms, double quiet_weight);
+
+/* ------------------------------------------------------------ instrumentation */
+/* per-stage timing with HIP events on the ctx stream: enable, then read the
+ * accumulated milliseconds of each stage (names by index via soap_stage_name) */
+int soap_set_timing(soap_ctx *ctx, int enable);
+int soap_get_stage_ms(soap_ctx *ctx, double *ms, int32_t n_stage);
+const char *soap_stage_name(int32_t stage);
+int32_t soap_num_stages(void);
+/* algorithmic bytes of the last soap_integrate (seed rows touched + continua +
+ * outputs) and the distinct seed rows R_psi summed over phases */
+int soap_get_integrate_stats(soap_ctx *ctx, double *alg_bytes, int64_t *rows_touched);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,298 @@
+"""ctypes front end of the FP64 CPU oracle (oracle/soap_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The engine in soap_gpu_amd/
+never imports this package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+_lib = None
+
+GRID_LINEAR = 0
+GRID_LOG = 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class Region(ctypes.Structure):
+    _fields_ = [("lat", ctypes.c_double), ("lon", ctypes.c_double),
+                ("size", ctypes.c_double), ("type", ctypes.c_double)]
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc + OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.orc_rndup.restype = ctypes.c_double
+        L.orc_rndup.argtypes = [ctypes.c_double, ctypes.c_int]
+        L.orc_delta_lambda.restype = ctypes.c_double
+        L.orc_delta_lambda.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_delta_lambda_cpu.restype = ctypes.c_double
+        L.orc_delta_lambda_cpu.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.orc_grid_h.restype = ctypes.c_double
+        L.orc_grid_h.argtypes = [_dp, ctypes.c_long]
+        L.orc_lb_params.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_disk_init.argtypes = [ctypes.c_int] + [ctypes.c_double] * 6 + [_dp, _dp, ctypes.c_int,
+                                                                              _dp, _dp, _dp, _ip, _dp]
+        L.orc_shift_row.argtypes = [ctypes.c_int, _dp, ctypes.c_long, ctypes.c_double, _dp,
+                                    ctypes.c_double, ctypes.c_double, _dp]
+        L.orc_scatter_row_ref.argtypes = [_dp, ctypes.c_long, ctypes.c_double, _dp, ctypes.c_double,
+                                          ctypes.c_double, _dp]
+        L.orc_quiet_star.argtypes = [ctypes.c_int, _dp, ctypes.c_long, ctypes.c_double, ctypes.c_int,
+                                     _dp, _dp, _ip, _dp, ctypes.c_int, _dp, ctypes.c_int]
+        L.orc_quiet_star_ref.argtypes = [_dp, ctypes.c_long, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                         _ip, _dp, ctypes.c_int, _dp]
+        L.orc_region_bbox.argtypes = [ctypes.c_double] * 4 + [ctypes.c_int, ctypes.c_int,
+                                                              ctypes.c_double, _ip]
+        L.orc_phase_maps.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.POINTER(Region), ctypes.c_int, _dp, _dp]
+        L.orc_integrate_phase.argtypes = ([ctypes.c_int, _dp, ctypes.c_long, ctypes.c_double, ctypes.c_int,
+                                           _dp, _dp, _ip, _dp, _dp, _dp] + [_dp] * 6
+                                          + [ctypes.c_int] * 4 + [_dp, _dp, _dp, ctypes.c_int])
+        L.orc_conv_window.argtypes = [_dp, ctypes.c_double]
+        L.orc_lower_resolution.argtypes = [ctypes.c_int, _dp, ctypes.c_long, ctypes.c_double, _dp, _dp,
+                                           ctypes.c_int, ctypes.c_double, _dp, _dp, ctypes.c_int]
+        L.orc_select_lines.restype = ctypes.c_long
+        L.orc_select_lines.argtypes = [_dp, ctypes.c_long, _dp, _dp, ctypes.c_long, _dp, _dp]
+        L.orc_ccf.argtypes = [ctypes.c_int, _dp, ctypes.c_long, ctypes.c_double, _dp, ctypes.c_int,
+                              _dp, _dp, ctypes.c_long, _dp, _dp, ctypes.c_int]
+        L.orc_appodisation.argtypes = [ctypes.c_int, _dp]
+        L.orc_ccf_finish.argtypes = [ctypes.c_int, _dp, _dp]
+        L.orc_fit_ccf.argtypes = [_dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(_ip)
+
+
+def nthreads_default() -> int:
+    return int(os.environ.get("ORACLE_THREADS", os.cpu_count() or 1))
+
+
+# ---------------------------------------------------------------- scalars
+def rndup(x, nb=0):
+    return lib().orc_rndup(float(x), int(nb))
+
+
+def delta_lambda(v_ms, lam):
+    return lib().orc_delta_lambda(float(v_ms), float(lam))
+
+
+def delta_lambda_cpu(v_ms, lam):
+    return lib().orc_delta_lambda_cpu(float(v_ms), float(lam))
+
+
+def grid_h(wave):
+    w, wp = _d(wave)
+    return lib().orc_grid_h(wp, len(w))
+
+
+def lb_params(t_star, lb_g2, lb_k0, lb_m0, temps):
+    g, gp = _d(lb_g2)
+    k, kp = _d(lb_k0)
+    m, mp = _d(lb_m0)
+    t, tp = _d(temps)
+    out = np.zeros(4)
+    lib().orc_lb_params(int(t_star), gp, kp, mp, tp, out.ctypes.data_as(_dp))
+    return out
+
+
+# ------------------------------------------------------------------- a1
+def disk_init(grid, prot, w_ratio, incl_deg, radius_km, u1, u2, lb4, mu_bounds):
+    lb, lbp = _d(lb4)
+    mb, mbp = _d(mu_bounds)
+    n_mu = len(mb) + 1
+    g2 = grid * grid
+    mu = np.empty(g2)
+    delta = np.empty(g2)
+    limb = np.empty(g2)
+    region = np.empty(g2, dtype=np.int32)
+    lbm = np.empty(g2)
+    lib().orc_disk_init(int(grid), float(prot), float(w_ratio), float(incl_deg), float(radius_km),
+                        float(u1), float(u2), lbp, mbp, n_mu,
+                        mu.ctypes.data_as(_dp), delta.ctypes.data_as(_dp), limb.ctypes.data_as(_dp),
+                        region.ctypes.data_as(_ip), lbm.ctypes.data_as(_dp))
+    return dict(mu=mu, delta=delta, limb=limb, region=region, lb=lbm)
+
+
+# ------------------------------------------------------------------- a4
+def shift_row(kind, wave, row, delta_kms, limb=1.0, step=0.0):
+    w, wp = _d(wave)
+    r, rp = _d(row)
+    out = np.empty(len(w))
+    lib().orc_shift_row(int(kind), wp, len(w), float(step), rp, float(delta_kms), float(limb),
+                        out.ctypes.data_as(_dp))
+    return out
+
+
+def scatter_row_ref(wave, row, delta_kms, limb, step, buf):
+    w, wp = _d(wave)
+    r, rp = _d(row)
+    assert buf.dtype == np.float64 and buf.flags.c_contiguous
+    lib().orc_scatter_row_ref(wp, len(w), float(step), rp, float(delta_kms), float(limb),
+                              buf.ctypes.data_as(_dp))
+    return buf
+
+
+# ------------------------------------------------------------------- a2
+def quiet_star(kind, wave, maps, cube, grid, step=0.0, nthreads=None):
+    w, wp = _d(wave)
+    c, cp = _d(cube)
+    n_mu = c.shape[0]
+    de, dep = _d(maps["delta"])
+    li, lip = _d(maps["limb"])
+    rg, rgp = _i(maps["region"])
+    out = np.empty(len(w))
+    lib().orc_quiet_star(int(kind), wp, len(w), float(step), int(grid), dep, lip, rgp, cp, n_mu,
+                         out.ctypes.data_as(_dp), nthreads or nthreads_default())
+    return out
+
+
+def quiet_star_ref(wave, maps, cube, grid, step):
+    w, wp = _d(wave)
+    c, cp = _d(cube)
+    de, dep = _d(maps["delta"])
+    li, lip = _d(maps["limb"])
+    rg, rgp = _i(maps["region"])
+    out = np.empty(len(w))
+    lib().orc_quiet_star_ref(wp, len(w), float(step), int(grid), dep, lip, rgp, cp, c.shape[0],
+                             out.ctypes.data_as(_dp))
+    return out
+
+
+# ------------------------------------------------------------------- a5
+def region_bbox(lat, lon, size, incl_deg, nrho, grid, phase):
+    b = np.zeros(5, dtype=np.int32)
+    lib().orc_region_bbox(float(lat), float(lon), float(size), float(incl_deg), int(nrho), int(grid),
+                          float(phase), b.ctypes.data_as(_ip))
+    return b
+
+
+def phase_maps(grid, incl_deg, nrho, t_star, t_diff_spot, t_diff_fac, phase, regions):
+    regs = (Region * max(1, len(regions)))()
+    for i, r in enumerate(regions):
+        regs[i].lat, regs[i].lon, regs[i].size, regs[i].type = r
+    g2 = grid * grid
+    temp = np.empty(g2)
+    bi = np.empty(g2)
+    lib().orc_phase_maps(int(grid), float(incl_deg), int(nrho), int(t_star), int(t_diff_spot),
+                         int(t_diff_fac), float(phase), regs, len(regions),
+                         temp.ctypes.data_as(_dp), bi.ctypes.data_as(_dp))
+    return temp, bi
+
+
+# ------------------------------------------------------------------- a6
+def integrate_phase(kind, wave, maps, temp, bi, cubes, grid, t_star, t_diff_fac, solar_mode=0,
+                    step=0.0, nthreads=None):
+    """cubes: dict with keys q, qsed, sp, spsed, fac, facsed -> [n_mu, n] arrays."""
+    w, wp = _d(wave)
+    n = len(w)
+    keep = [_d(cubes[k]) for k in ("q", "qsed", "sp", "spsed", "fac", "facsed")]
+    de, dep = _d(maps["delta"])
+    li, lip = _d(maps["limb"])
+    rg, rgp = _i(maps["region"])
+    lbm, lbp = _d(maps["lb"])
+    te, tep = _d(temp)
+    bb, bbp = _d(bi)
+    flux = np.empty(n)
+    bconv = np.empty(n)
+    tot = np.empty(n)
+    lib().orc_integrate_phase(int(kind), wp, n, float(step), int(grid), dep, lip, rgp, lbp, tep, bbp,
+                              *[k[1] for k in keep], keep[0][0].shape[0], int(t_star), int(t_diff_fac),
+                              int(solar_mode), flux.ctypes.data_as(_dp), bconv.ctypes.data_as(_dp),
+                              tot.ctypes.data_as(_dp), nthreads or nthreads_default())
+    return flux, bconv, tot
+
+
+# ------------------------------------------------------------------- a7
+def conv_window(wave, sigma_ms):
+    w, wp = _d(wave)
+    return lib().orc_conv_window(wp, float(sigma_ms))
+
+
+def lower_resolution(kind, wave, sigma_ms, quiet, spec, variant, weight=0.005, nthreads=None):
+    w, wp = _d(wave)
+    q, qp = _d(quiet if quiet is not None else spec)
+    s, sp = _d(spec)
+    out = np.empty(len(w))
+    mm = np.zeros(2)
+    lib().orc_lower_resolution(int(kind), wp, len(w), float(sigma_ms), qp, sp, int(variant), float(weight),
+                               out.ctypes.data_as(_dp), mm.ctypes.data_as(_dp), nthreads or nthreads_default())
+    return out, mm
+
+
+# ------------------------------------------------------------------- a8
+def select_lines(wave_t, line_l, line_w):
+    w, wp = _d(wave_t)
+    l, lp = _d(line_l)
+    ww, wwp = _d(line_w)
+    sl = np.empty(len(l))
+    sw = np.empty(len(l))
+    k = lib().orc_select_lines(wp, len(w), lp, wwp, len(l), sl.ctypes.data_as(_dp), sw.ctypes.data_as(_dp))
+    return sl[:k].copy(), sw[:k].copy()
+
+
+def ccf(kind, wave_t, step, vgrid, line_l, line_w, spec_t, nthreads=None):
+    w, wp = _d(wave_t)
+    v, vp = _d(vgrid)
+    l, lp = _d(line_l)
+    ww, wwp = _d(line_w)
+    s, sp = _d(spec_t)
+    out = np.empty(len(v))
+    lib().orc_ccf(int(kind), wp, len(w), float(step), vp, len(v), lp, wwp, len(l), sp,
+                  out.ctypes.data_as(_dp), nthreads or nthreads_default())
+    return out
+
+
+def appodisation(n_v):
+    out = np.empty(n_v)
+    lib().orc_appodisation(int(n_v), out.ctypes.data_as(_dp))
+    return out
+
+
+def ccf_finish(ccf_raw, appod):
+    c = np.array(ccf_raw, dtype=np.float64, copy=True)
+    a, ap = _d(appod)
+    lib().orc_ccf_finish(len(c), ap, c.ctypes.data_as(_dp))
+    return c
+
+
+# ------------------------------------------------------------------- a9
+def fit_ccf(v, y):
+    vv, vp = _d(v)
+    yy, yp = _d(y)
+    n = len(vv)
+    par = np.zeros(5)
+    model = np.zeros(n)
+    depth = np.zeros(91)
+    bis = np.zeros(91)
+    it = np.zeros(1, dtype=np.int32)
+    st = lib().orc_fit_ccf(vp, yp, n, par.ctypes.data_as(_dp), model.ctypes.data_as(_dp),
+                           depth.ctypes.data_as(_dp), bis.ctypes.data_as(_dp), it.ctypes.data_as(_ip))
+    return dict(par=par, model=model, depth=depth, bis=bis, iters=int(it[0]), status=int(st))

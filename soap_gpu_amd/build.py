@@ -1,0 +1,76 @@
+"""Build recipe of libsoapgpu.so (gfx950) and the C++ command-line tools.
+
+hipcc compiles the HIP sources for gfx950 only; the library is written in-tree
+(soap_gpu_amd/libsoapgpu.so) so it travels with the source tree.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libsoapgpu.so")
+BIN = os.path.join(HERE, "bin")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result", "-Wno-unused-variable",
+]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("soap_kernels.hip", "soap_engine.hip")]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd[:3]) + " ...")
+    return r
+
+
+def build_lib(verbose: bool = False) -> str:
+    objs = []
+    os.makedirs(os.path.join(HERE, "build"), exist_ok=True)
+    for s in SOURCES:
+        o = os.path.join(HERE, "build", os.path.basename(s) + ".o")
+        _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", s, "-o", o])
+        objs.append(o)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+def build_tools(verbose: bool = False) -> list[str]:
+    """C++ host driver and CLI shims (link against libsoapgpu.so)."""
+    drv = os.path.join(HERE, "driver")
+    if not os.path.isdir(drv):
+        return []
+    os.makedirs(BIN, exist_ok=True)
+    out = []
+    for src in sorted(os.listdir(drv)):
+        if not src.endswith(".cpp"):
+            continue
+        exe = os.path.join(BIN, src[:-4])
+        _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", drv,
+              os.path.join(drv, src), "-o", exe, "-L", HERE, "-lsoapgpu",
+              "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + HERE])
+        out.append(exe)
+        if verbose:
+            print("built", exe)
+    return out
+
+
+def build(verbose: bool = False):
+    build_lib(verbose)
+    build_tools(verbose)
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -1,0 +1,1183 @@
+// Host side of libsoapgpu: context, validation, per-phase region geometry and
+// stage orchestration on one HIP stream.  See include/soap_gpu.h for the ABI.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <cstdarg>
+#include <cstdio>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/soap_gpu.h"
+#include "soap_internal.h"
+
+using namespace soap;
+
+namespace {
+
+enum Stage {
+  ST_DISK = 0,
+  ST_TAPS_QUIET,
+  ST_FIR_QUIET,
+  ST_TAPS_ACTIVE,
+  ST_FIR_ACTIVE,
+  ST_CONV,
+  ST_NORM,
+  ST_MASK,
+  ST_CCF,
+  ST_FIT,
+  ST_COUNT
+};
+const char *kStageNames[ST_COUNT] = {"disk_init", "taps_quiet", "fir_quiet", "taps_active", "fir_active",
+                                     "conv",      "norm",       "mask",      "ccf",         "fit"};
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct soap_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // grid
+  int64_t n = 0;
+  int kind = -1;
+  double step = 0, h = 0, expm1_h = 0;
+  std::vector<double> wave;
+  double *d_wave = nullptr;
+  // mu
+  int n_mu = 0;
+  std::vector<double> mu_bounds;
+  double *d_mub = nullptr;
+  // cubes
+  double *d_cube[6] = {};
+  bool cube_owned[6] = {};
+  int cube_nmu[6] = {};
+  // star + maps
+  soap_star star{};
+  bool have_maps = false;
+  int G = 0;
+  double *d_mu = nullptr, *d_delta = nullptr, *d_limb = nullptr, *d_lbm = nullptr, *d_t = nullptr;
+  int32_t *d_region = nullptr, *d_m = nullptr;
+  int dmin = 0, nd = 0;
+  // quiet
+  double *d_quiet = nullptr;  // [2][n]
+  bool have_quiet = false;
+  // activity
+  soap_activity act{};
+  int P = 0;
+  std::vector<double> phases;
+  std::vector<int32_t> reg_count;
+  std::vector<soap_region> regions;
+  std::vector<RegionDev> geo;
+  std::vector<int32_t> geo_off, box;
+  RegionDev *d_geo = nullptr;
+  int32_t *d_geo_off = nullptr, *d_box = nullptr;
+  int max_col_blocks = 0;
+  double *d_raw = nullptr;  // [P][3][n]
+  bool have_raw = false;
+  // tap scratch
+  DevBuf acc, taps, krange, kflags, bins, nbins;
+  // resolution
+  double *d_lower = nullptr;  // [(P+1)][3][n]
+  DevBuf part, mm, ctap, qsel, variant;
+  bool have_lower = false;
+  // mask / ccf / fit
+  int n_lines = 0, n_v = 0, trim = 0;
+  double *d_line_l = nullptr, *d_line_w = nullptr, *d_vgrid = nullptr, *d_appod = nullptr;
+  DevBuf m_start, m_len, m_val, m_alive, sorted, rank;
+  int mask_scan = 0;
+  bool have_mask = false, mask_built = false;
+  double *d_ccf = nullptr;  // [(P+1)*3][n_v]
+  bool have_ccf = false;
+  DevBuf f_par, f_model, f_depth, f_bis, f_iter;
+  bool have_fit = false;
+  // timing
+  bool timing = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+  double stage_ms[ST_COUNT] = {};
+  // allocation sizes of the buffers made by dalloc
+  std::map<void *, size_t> caps;
+  // resolution constants last uploaded
+  double res_sigma = -1;
+  int res_P = -1;
+  // stats
+  double alg_bytes = 0;
+  int64_t rows_touched = 0;
+};
+
+// ------------------------------------------------------------------ helpers
+static int fail(soap_ctx *c, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (c) c->err = buf;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                         \
+  do {                                                                                           \
+    hipError_t _e = (expr);                                                                      \
+    if (_e != hipSuccess) return fail((ctx), SOAP_ERR_HIP, "%s: %s", #expr, hipGetErrorString(_e)); \
+  } while (0)
+
+// (re)allocate *p to hold at least `bytes`; an existing allocation that is
+// large enough is kept, so repeated calls in a loop do not hit hipMalloc
+static int dalloc(soap_ctx *c, void **p, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  auto it = c->caps.find(*p);
+  if (*p && it != c->caps.end() && it->second >= bytes) return SOAP_OK;
+  if (*p) {
+    if (it != c->caps.end()) c->caps.erase(it);
+    hipFree(*p);
+    *p = nullptr;
+  }
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(c, SOAP_ERR_NOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  c->caps[*p] = bytes;
+  return SOAP_OK;
+}
+
+template <class T>
+static int dalloc(soap_ctx *c, T **p, size_t count) {
+  return dalloc(c, (void **)p, count * sizeof(T));
+}
+
+static int ensure(soap_ctx *c, DevBuf &b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return SOAP_OK;
+  int rc = dalloc(c, &b.p, bytes);
+  b.bytes = rc == SOAP_OK ? std::max(bytes, c->caps[b.p]) : 0;
+  return rc;
+}
+
+static void dfree(void *p) {
+  if (p) hipFree(p);
+}
+
+static void dfree_c(soap_ctx *c, void *p) {
+  if (!p) return;
+  c->caps.erase(p);
+  hipFree(p);
+}
+
+struct StageTimer {
+  soap_ctx *c;
+  int st;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  StageTimer(soap_ctx *c_, int st_) : c(c_), st(st_) {
+    if (c->timing) {
+      hipEventCreate(&e0);
+      hipEventCreate(&e1);
+      hipEventRecord(e0, c->stream);
+    }
+  }
+  ~StageTimer() {
+    if (c->timing) {
+      hipEventRecord(e1, c->stream);
+      c->pending.push_back({st, {e0, e1}});
+    }
+  }
+};
+
+static void collect_timing(soap_ctx *c) {
+  for (auto &p : c->pending) {
+    float ms = 0;
+    if (hipEventSynchronize(p.second.second) == hipSuccess &&
+        hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess)
+      c->stage_ms[p.first] += ms;
+    hipEventDestroy(p.second.first);
+    hipEventDestroy(p.second.second);
+  }
+  c->pending.clear();
+}
+
+// ------------------------------------------------------------ region geometry
+// spot_init_gpu + spot_phase_gpu + spot_area, evaluated serially (the
+// reference's cross-block race in spot_area resolved to the serial bounding
+// box), SOAP_integration.cu:21-161.
+static int region_bbox(double lat, double lon, double s, double incl_deg, int nrho, int grid, double phase,
+                       int bnd[5]) {
+  const double pi = kPi;
+  double longitude = lon * pi / 180.;
+  double latitude = lat * pi / 180.;
+  double inclination = incl_deg * pi / 180.;
+  double rho_step = 2. * pi / (nrho - 1);
+  double b = -latitude, g = longitude, b2 = pi / 2. - inclination;
+  double R[3][3] = {{cos(b2) * cos(g) * cos(b) - sin(b) * sin(b2), -sin(g) * cos(b2),
+                     cos(b2) * cos(g) * sin(b) + sin(b2) * cos(b)},
+                    {sin(g) * cos(b), cos(g), sin(g) * sin(b)},
+                    {-sin(b2) * cos(g) * cos(b) - cos(b2) * sin(b), sin(b2) * sin(g),
+                     -sin(b2) * cos(g) * sin(b) + cos(b2) * cos(b)}};
+  double psi = -phase * (2 * pi);
+  double inc = incl_deg * pi / 180.;
+  double ax[3] = {cos(inc), 0, sin(inc)};
+  double cp = cos(psi), sp = sin(psi);
+  double Pm[3][3] = {{(1 - cp) * ax[0] * ax[0] + cp, (1 - cp) * ax[0] * ax[1] + sp * ax[2],
+                      (1 - cp) * ax[0] * ax[2] - sp * ax[1]},
+                     {(1 - cp) * ax[1] * ax[0] - sp * ax[2], (1 - cp) * ax[1] * ax[1] + cp,
+                      (1 - cp) * ax[1] * ax[2] + sp * ax[0]},
+                     {(1 - cp) * ax[2] * ax[0] + sp * ax[1], (1 - cp) * ax[2] * ax[1] - sp * ax[0],
+                      (1 - cp) * ax[2] * ax[2] + cp}};
+  int on = 0, off = 0;
+  double miny = 1, minz = 1, maxy = -1, maxz = -1;
+  for (int k = 0; k < nrho; k++) {
+    double rho = -pi + k * rho_step;
+    double x2[3] = {sqrt(1 - s * s), s * cos(rho), s * sin(rho)};
+    double x1[3], x3[3];
+    for (int r = 0; r < 3; r++) x1[r] = R[r][0] * x2[0] + R[r][1] * x2[1] + R[r][2] * x2[2];
+    for (int r = 0; r < 3; r++) x3[r] = Pm[r][0] * x1[0] + Pm[r][1] * x1[1] + Pm[r][2] * x1[2];
+    if (x3[0] >= 0) {
+      on++;
+      miny = std::min(miny, x3[1]);
+      minz = std::min(minz, x3[2]);
+      maxy = std::max(maxy, x3[1]);
+      maxz = std::max(maxz, x3[2]);
+    } else {
+      off++;
+    }
+  }
+  if (on > 0 && off > 0) {
+    if (miny * maxy < 0) {
+      if (minz < 0)
+        minz = -1;
+      else
+        maxz = 1;
+    }
+    if (minz * maxz < 0) {
+      if (miny < 0)
+        miny = -1;
+      else
+        maxy = 1;
+    }
+  }
+  double gs = 2. / grid;
+  bnd[0] = (int)floor((1. + miny) / gs);
+  bnd[1] = (int)floor((1. + minz) / gs);
+  bnd[2] = (int)ceil((1. + maxy) / gs);
+  bnd[3] = (int)ceil((1. + maxz) / gs);
+  bnd[4] = on > 0 ? 1 : 0;
+  return bnd[4];
+}
+
+// Row 0 of spot_inverse_rotation's R3 = R2 R (SOAP_integration.cu:164-202),
+// phase pre-decrement included; only x_back is used by the membership test.
+static void inverse_rotation_row0(double lon, double lat, double incl_deg, double phase, double r0[3]) {
+  const double pi = kPi;
+  double g2 = (phase - 1) * (2 * pi);
+  double i = incl_deg * pi / 180.;
+  double b = lat * pi / 180.;
+  double g = -lon * pi / 180.;
+  double b2 = -(pi / 2. - i);
+  double R[3][3] = {{(1 - cos(g2)) * cos(i) * cos(i) + cos(g2), sin(g2) * sin(i), (1 - cos(g2)) * cos(i) * sin(i)},
+                    {-sin(g2) * sin(i), cos(g2), sin(g2) * cos(i)},
+                    {(1 - cos(g2)) * sin(i) * cos(i), -sin(g2) * cos(i), (1 - cos(g2)) * sin(i) * sin(i) + cos(g2)}};
+  double R2r0[3] = {cos(b) * cos(g) * cos(b2) - sin(b2) * sin(b), -sin(g) * cos(b),
+                    cos(b) * cos(g) * sin(b2) + sin(b) * cos(b2)};
+  for (int c = 0; c < 3; c++) r0[c] = R2r0[0] * R[0][c] + R2r0[1] * R[1][c] + R2r0[2] * R[2][c];
+}
+
+// ------------------------------------------------------------------ API
+extern "C" {
+
+const char *soap_version(void) { return "soap_gpu_amd 0.1 (gfx950)"; }
+
+int soap_ctx_create(int device, soap_ctx **out) {
+  if (!out) return SOAP_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SOAP_ERR_HIP;
+  if (device < 0 || device >= ndev) return SOAP_ERR_ARG;
+  if (hipSetDevice(device) != hipSuccess) return SOAP_ERR_HIP;
+  soap_ctx *c = new soap_ctx();
+  c->device = device;
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return SOAP_ERR_HIP;
+  }
+  *out = c;
+  return SOAP_OK;
+}
+
+void soap_ctx_destroy(soap_ctx *c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  collect_timing(c);
+  dfree(c->d_wave);
+  dfree(c->d_mub);
+  for (int i = 0; i < 6; i++)
+    if (c->cube_owned[i]) dfree(c->d_cube[i]);
+  void *ptrs[] = {c->d_mu,      c->d_delta,   c->d_limb,    c->d_lbm,      c->d_t,      c->d_region, c->d_m,
+                  c->d_quiet,   c->d_geo,     c->d_geo_off, c->d_box,      c->d_raw,    c->d_lower,  c->d_line_l,
+                  c->d_line_w,  c->d_vgrid,   c->d_appod,   c->d_ccf,      c->acc.p,    c->taps.p,   c->krange.p,
+                  c->kflags.p,  c->bins.p,    c->nbins.p,   c->part.p,     c->mm.p,     c->ctap.p,   c->qsel.p,
+                  c->variant.p, c->m_start.p, c->m_len.p,   c->m_val.p,    c->m_alive.p, c->sorted.p, c->rank.p,
+                  c->f_par.p,   c->f_model.p, c->f_depth.p, c->f_bis.p,    c->f_iter.p};
+  for (void *p : ptrs) dfree(p);
+  hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char *soap_last_error(const soap_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int soap_sync(soap_ctx *c) {
+  if (!c) return SOAP_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  return SOAP_OK;
+}
+
+int soap_set_timing(soap_ctx *c, int enable) {
+  if (!c) return SOAP_ERR_ARG;
+  soap_sync(c);
+  c->timing = enable != 0;
+  for (double &v : c->stage_ms) v = 0;
+  return SOAP_OK;
+}
+
+int soap_get_stage_ms(soap_ctx *c, double *ms, int32_t n_stage) {
+  if (!c || !ms) return SOAP_ERR_ARG;
+  int rc = soap_sync(c);
+  if (rc) return rc;
+  for (int i = 0; i < n_stage && i < ST_COUNT; i++) ms[i] = c->stage_ms[i];
+  return SOAP_OK;
+}
+
+const char *soap_stage_name(int32_t st) { return (st >= 0 && st < ST_COUNT) ? kStageNames[st] : ""; }
+int32_t soap_num_stages(void) { return ST_COUNT; }
+
+int soap_set_wave(soap_ctx *c, int kind, const double *wave, int64_t n, double step) {
+  if (!c || !wave || n < 8) return fail(c, SOAP_ERR_ARG, "set_wave: need a wavelength array of >= 8 pixels");
+  if (kind != SOAP_GRID_LOG && kind != SOAP_GRID_LINEAR) return fail(c, SOAP_ERR_ARG, "set_wave: bad grid kind");
+  if (kind == SOAP_GRID_LINEAR)
+    return fail(c, SOAP_ERR_LIMIT, "set_wave: linear grids are handled by the gather path, not built in this library");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<double> w(wave, wave + n);
+  double h = log(w[n - 1] / w[0]) / (double)(n - 1);
+  for (int64_t j = 0; j < n; j++) {
+    double e = w[0] * exp((double)j * h);
+    if (!(fabs(w[j] - e) <= 1e-12 * fabs(w[j])))
+      return fail(c, SOAP_ERR_GRID, "set_wave: pixel %lld deviates from a log-uniform grid (%.17g vs %.17g)",
+                  (long long)j, w[j], e);
+  }
+  if (c->n != n) {
+    // every per-pixel buffer depends on n
+    for (int i = 0; i < 6; i++) {
+      if (c->cube_owned[i]) {
+        c->caps.erase(c->d_cube[i]);
+        dfree(c->d_cube[i]);
+      }
+      c->d_cube[i] = nullptr;
+      c->cube_owned[i] = false;
+    }
+    c->have_quiet = c->have_raw = c->have_lower = c->have_ccf = c->have_fit = false;
+    c->mask_built = false;
+  }
+  c->wave = w;
+  c->n = n;
+  c->kind = kind;
+  c->step = step;
+  c->h = h;
+  c->expm1_h = expm1(h);
+  c->res_sigma = -1;
+  int rc = dalloc(c, &c->d_wave, (size_t)n);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(c->d_wave, w.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SOAP_OK;
+}
+
+int soap_set_mu_bounds(soap_ctx *c, const double *bounds, int32_t n_bounds) {
+  if (!c || (!bounds && n_bounds > 0) || n_bounds < 0) return fail(c, SOAP_ERR_ARG, "set_mu_bounds: bad arguments");
+  if (n_bounds > 63) return fail(c, SOAP_ERR_LIMIT, "set_mu_bounds: at most 64 mu bins");
+  HIPCHK(c, hipSetDevice(c->device));
+  c->mu_bounds.assign(bounds, bounds + n_bounds);
+  c->n_mu = n_bounds + 1;
+  int rc = dalloc(c, &c->d_mub, (size_t)std::max(1, n_bounds));
+  if (rc) return rc;
+  if (n_bounds)
+    HIPCHK(c, hipMemcpyAsync(c->d_mub, bounds, n_bounds * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SOAP_OK;
+}
+
+int soap_set_cube(soap_ctx *c, int which, const double *cube, int32_t n_mu) {
+  if (!c || !cube || which < 0 || which > 5) return fail(c, SOAP_ERR_ARG, "set_cube: bad arguments");
+  if (c->n == 0) return fail(c, SOAP_ERR_STATE, "set_cube: set the wavelength grid first");
+  if (n_mu < 1) return fail(c, SOAP_ERR_ARG, "set_cube: n_mu < 1");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->cube_owned[which]) c->d_cube[which] = nullptr;
+  int rc = dalloc(c, &c->d_cube[which], (size_t)n_mu * c->n);
+  if (rc) return rc;
+  c->cube_owned[which] = true;
+  c->cube_nmu[which] = n_mu;
+  HIPCHK(c, hipMemcpyAsync(c->d_cube[which], cube, (size_t)n_mu * c->n * sizeof(double), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SOAP_OK;
+}
+
+int soap_set_cube_device(soap_ctx *c, int which, const double *dev_cube, int32_t n_mu) {
+  if (!c || !dev_cube || which < 0 || which > 5 || n_mu < 1) return fail(c, SOAP_ERR_ARG, "set_cube_device: bad arguments");
+  if (c->cube_owned[which]) {
+    c->caps.erase(c->d_cube[which]);
+    dfree(c->d_cube[which]);
+  }
+  c->d_cube[which] = const_cast<double *>(dev_cube);
+  c->cube_owned[which] = false;
+  c->cube_nmu[which] = n_mu;
+  return SOAP_OK;
+}
+
+static int alloc_maps(soap_ctx *c, int G) {
+  size_t g2 = (size_t)G * G;
+  int rc;
+  if ((rc = dalloc(c, &c->d_mu, g2)) || (rc = dalloc(c, &c->d_delta, g2)) || (rc = dalloc(c, &c->d_limb, g2)) ||
+      (rc = dalloc(c, &c->d_lbm, g2)) || (rc = dalloc(c, &c->d_region, g2)) || (rc = dalloc(c, &c->d_m, g2)) ||
+      (rc = dalloc(c, &c->d_t, g2)))
+    return rc;
+  c->G = G;
+  return SOAP_OK;
+}
+
+// Tap offset range [dmin, dmin+nd) covering every cell's m-1 and m, from a bound
+// on |delta| (no device round trip): |delta| <= vmax, s(delta) monotone.
+static int set_tap_range(soap_ctx *c, double vmax_kms) {
+  auto s_of = [&](double d) {
+    double beta = d * 1000. / kC;
+    double fbeta = -1 * (1 - sqrt((1 + beta) / (1 - beta)));
+    return log1p(fbeta) / c->h;
+  };
+  vmax_kms = fabs(vmax_kms) * (1 + 1e-9) + 1e-12;
+  if (!(vmax_kms < 0.1 * kC / 1000.)) return fail(c, SOAP_ERR_ARG, "projected velocity %g km/s out of range", vmax_kms);
+  int lo = (int)floor(s_of(-vmax_kms)) - 2, hi = (int)ceil(s_of(vmax_kms)) + 1;
+  c->dmin = lo;
+  c->nd = hi - lo + 1;
+  if (c->nd > 4096) return fail(c, SOAP_ERR_LIMIT, "Doppler shift range of %d pixels exceeds 4096", c->nd);
+  return SOAP_OK;
+}
+
+int soap_disk_init(soap_ctx *c, const soap_star *st) {
+  if (!c || !st) return fail(c, SOAP_ERR_ARG, "disk_init: null argument");
+  if (c->n == 0 || c->n_mu == 0) return fail(c, SOAP_ERR_STATE, "disk_init: set wave and mu bounds first");
+  if (st->grid < 1 || st->grid > 8192) return fail(c, SOAP_ERR_ARG, "disk_init: grid must be in [1, 8192]");
+  if (!(st->prot_days != 0.0)) return fail(c, SOAP_ERR_ARG, "disk_init: zero rotation period");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = alloc_maps(c, st->grid);
+  if (rc) return rc;
+  c->star = *st;
+  DiskArgs a{};
+  a.grid = st->grid;
+  a.n_mu = c->n_mu;
+  a.delta_grid = 2. / st->grid;
+  a.theta_grid = 180.0 / st->grid;
+  a.omega0 = 360.0 / st->prot_days;
+  a.omega1 = a.omega0 * st->w_ratio;
+  a.radius = st->radius_km;
+  a.sin_incl = sin(st->incl_deg * kPi / 180.);
+  a.u1 = st->u1;
+  a.u2 = st->u2;
+  a.lb0 = st->lb[0];
+  a.lb1 = st->lb[1];
+  a.lb2 = st->lb[2];
+  a.lb3 = st->lb[3];
+  a.mu_bounds = c->d_mub;
+  a.log_grid = c->kind == SOAP_GRID_LOG;
+  a.h = c->h;
+  a.expm1_h = c->expm1_h;
+  a.mu = c->d_mu;
+  a.delta = c->d_delta;
+  a.limb = c->d_limb;
+  a.lbm = c->d_lbm;
+  a.region = c->d_region;
+  a.m = c->d_m;
+  a.t = c->d_t;
+  // |delta| <= (|omega0| + |omega1|) pi/(180*86400) R |sin i|, SOAP_initialize.cu:348-351
+  rc = set_tap_range(c, (fabs(a.omega0) + fabs(a.omega1)) * kPi / (180 * 24 * 3600) * fabs(a.radius) * fabs(a.sin_incl));
+  if (rc) return rc;
+  {
+    StageTimer tm(c, ST_DISK);
+    HIPCHK(c, launch_disk_init(a, c->stream));
+  }
+  c->have_maps = true;
+  c->have_quiet = c->have_raw = c->have_lower = c->have_ccf = c->have_fit = false;
+  return SOAP_OK;
+}
+
+int soap_get_maps(soap_ctx *c, double *mu, double *delta, double *limb, int32_t *region, double *lb) {
+  if (!c) return SOAP_ERR_ARG;
+  if (!c->have_maps) return fail(c, SOAP_ERR_STATE, "get_maps: no maps");
+  HIPCHK(c, hipSetDevice(c->device));
+  size_t g2 = (size_t)c->G * c->G;
+  if (mu) HIPCHK(c, hipMemcpyAsync(mu, c->d_mu, g2 * 8, hipMemcpyDeviceToHost, c->stream));
+  if (delta) HIPCHK(c, hipMemcpyAsync(delta, c->d_delta, g2 * 8, hipMemcpyDeviceToHost, c->stream));
+  if (limb) HIPCHK(c, hipMemcpyAsync(limb, c->d_limb, g2 * 8, hipMemcpyDeviceToHost, c->stream));
+  if (region) HIPCHK(c, hipMemcpyAsync(region, c->d_region, g2 * 4, hipMemcpyDeviceToHost, c->stream));
+  if (lb) HIPCHK(c, hipMemcpyAsync(lb, c->d_lbm, g2 * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SOAP_OK;
+}
+
+int soap_set_maps(soap_ctx *c, const soap_star *st, const double *delta, const double *limb, const int32_t *region,
+                  const double *lb) {
+  if (!c || !st || !delta || !limb || !region || !lb) return fail(c, SOAP_ERR_ARG, "set_maps: null argument");
+  if (c->n == 0 || c->n_mu == 0) return fail(c, SOAP_ERR_STATE, "set_maps: set wave and mu bounds first");
+  HIPCHK(c, hipSetDevice(c->device));
+  int G = st->grid;
+  size_t g2 = (size_t)G * G;
+  int rc = alloc_maps(c, G);
+  if (rc) return rc;
+  c->star = *st;
+  // per-cell (m, t) from the given delta map, same expressions as k_disk_init
+  std::vector<int32_t> m(g2, 0);
+  std::vector<double> t(g2, 0.0), mu(g2, -999.0);
+  double vmax = 0;
+  for (size_t i = 0; i < g2; i++)
+    if (!(limb[i] == -999.0 && delta[i] == -999.0)) vmax = std::max(vmax, fabs(delta[i]));
+  rc = set_tap_range(c, vmax);
+  if (rc) return rc;
+  for (size_t i = 0; i < g2; i++) {
+    if (region[i] < 0 || region[i] >= c->n_mu) return fail(c, SOAP_ERR_ARG, "set_maps: region %d out of range", region[i]);
+    if (limb[i] == -999.0 && delta[i] == -999.0) continue;
+    double beta = delta[i] * 1000. / kC;
+    double fbeta = -1 * (1 - sqrt((1 + beta) / (1 - beta)));
+    double sh = log1p(fbeta) / c->h;
+    double mc = ceil(sh);
+    m[i] = (int32_t)mc;
+    t[i] = expm1((mc - sh) * c->h) / c->expm1_h;
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_delta, delta, g2 * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_limb, limb, g2 * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_region, region, g2 * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_lbm, lb, g2 * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_mu, mu.data(), g2 * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_m, m.data(), g2 * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_t, t.data(), g2 * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_maps = true;
+  c->have_quiet = c->have_raw = c->have_lower = c->have_ccf = c->have_fit = false;
+  return SOAP_OK;
+}
+
+static int check_cube(soap_ctx *c, int which, const char *what) {
+  if (!c->d_cube[which]) return fail(c, SOAP_ERR_STATE, "%s: seed cube %d not set", what, which);
+  if (c->cube_nmu[which] != c->n_mu)
+    return fail(c, SOAP_ERR_ARG, "%s: cube %d has %d mu rows, the mu bounds give %d", what, which, c->cube_nmu[which],
+                c->n_mu);
+  return SOAP_OK;
+}
+
+// taps scratch for n_job jobs of nsets_in/out sets
+static int ensure_taps(soap_ctx *c, int n_job, int nsets_in, int nsets_out) {
+  int rc;
+  size_t per = (size_t)c->n_mu * c->nd;
+  if ((rc = ensure(c, c->acc, (size_t)n_job * nsets_in * per * 2 * 8)) ||
+      (rc = ensure(c, c->taps, (size_t)n_job * nsets_out * per * 8)) ||
+      (rc = ensure(c, c->krange, (size_t)n_job * c->n_mu * 2 * 4)) ||
+      (rc = ensure(c, c->kflags, (size_t)n_job * c->n_mu * 4)) || (rc = ensure(c, c->bins, (size_t)n_job * c->n_mu * 4)) ||
+      (rc = ensure(c, c->nbins, (size_t)n_job * 4)))
+    return rc;
+  return SOAP_OK;
+}
+
+static int tap_window(soap_ctx *c) { return std::min(16, c->nd); }
+
+int soap_quiet_star(soap_ctx *c) {
+  if (!c) return SOAP_ERR_ARG;
+  if (!c->have_maps) return fail(c, SOAP_ERR_STATE, "quiet_star: run disk_init first");
+  int rc;
+  if ((rc = check_cube(c, SOAP_CUBE_QUIET, "quiet_star")) || (rc = check_cube(c, SOAP_CUBE_QUIET_SED, "quiet_star")))
+    return rc;
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = ensure_taps(c, 1, 1, 1))) return rc;
+  if ((rc = dalloc(c, &c->d_quiet, (size_t)2 * c->n))) return rc;
+  TapAccArgs a{};
+  a.grid = c->G;
+  a.n_mu = c->n_mu;
+  a.dmin = c->dmin;
+  a.nd = c->nd;
+  a.nsets = 1;
+  a.wd = tap_window(c);
+  a.cols_per_block = std::max(1, 4096 / std::max(1, c->G));
+  a.region = c->d_region;
+  a.limb = c->d_limb;
+  a.delta = c->d_delta;
+  a.lbm = c->d_lbm;
+  a.m = c->d_m;
+  a.t = c->d_t;
+  a.acc = (unsigned long long *)c->acc.p;
+  {
+    StageTimer tm(c, ST_TAPS_QUIET);
+    HIPCHK(c, launch_fill_u64(a.acc, (size_t)c->n_mu * c->nd * 2, c->stream));
+    HIPCHK(c, launch_taps_quiet(a, c->stream));
+    HIPCHK(c, launch_taps_finalize(a.acc, 1, 1, 1, c->n_mu, c->nd, (double *)c->taps.p, (int32_t *)c->krange.p,
+                                   (int32_t *)c->kflags.p, (int32_t *)c->bins.p, (int32_t *)c->nbins.p, c->stream));
+  }
+  FirArgs f{};
+  f.q = c->d_cube[SOAP_CUBE_QUIET];
+  f.qs = c->d_cube[SOAP_CUBE_QUIET_SED];
+  f.n = c->n;
+  f.n_mu = c->n_mu;
+  f.dmin = c->dmin;
+  f.nd = c->nd;
+  f.taps = (const double *)c->taps.p;
+  f.krange = (const int32_t *)c->krange.p;
+  f.kflags = (const int32_t *)c->kflags.p;
+  f.bins = (const int32_t *)c->bins.p;
+  f.nbins = (const int32_t *)c->nbins.p;
+  f.out = c->d_quiet;
+  {
+    StageTimer tm(c, ST_FIR_QUIET);
+    HIPCHK(c, launch_fir_quiet(f, c->stream));
+  }
+  c->have_quiet = true;
+  c->have_lower = c->have_ccf = c->have_fit = false;
+  return SOAP_OK;
+}
+
+int soap_get_quiet(soap_ctx *c, int which, double *out) {
+  if (!c || !out || which < 0 || which > 1) return fail(c, SOAP_ERR_ARG, "get_quiet: bad arguments");
+  if (!c->have_quiet) return fail(c, SOAP_ERR_STATE, "get_quiet: no quiet star");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(out, c->d_quiet + (size_t)which * c->n, c->n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SOAP_OK;
+}
+
+int soap_set_quiet(soap_ctx *c, int which, const double *q) {
+  if (!c || !q || which < 0 || which > 1) return fail(c, SOAP_ERR_ARG, "set_quiet: bad arguments");
+  if (c->n == 0) return fail(c, SOAP_ERR_STATE, "set_quiet: set the wavelength grid first");
+  HIPCHK(c, hipSetDevice(c->device));
+  if (!c->d_quiet) {
+    int rc = dalloc(c, &c->d_quiet, (size_t)2 * c->n);
+    if (rc) return rc;
+    HIPCHK(c, launch_fill_f64(c->d_quiet, (size_t)2 * c->n, 0.0, c->stream));
+  }
+  HIPCHK(c, hipMemcpyAsync(c->d_quiet + (size_t)which * c->n, q, c->n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_quiet = true;
+  return SOAP_OK;
+}
+
+int soap_set_activity(soap_ctx *c, const soap_activity *act, int32_t n_phase, const double *phase,
+                      const int32_t *region_count, const soap_region *regions) {
+  if (!c || !act || n_phase < 0 || (n_phase > 0 && (!phase || !region_count)))
+    return fail(c, SOAP_ERR_ARG, "set_activity: bad arguments");
+  if (!c->have_maps) return fail(c, SOAP_ERR_STATE, "set_activity: run disk_init first");
+  if (act->nrho < 2) return fail(c, SOAP_ERR_ARG, "set_activity: nrho must be >= 2");
+  if (act->solar_mode) return fail(c, SOAP_ERR_LIMIT, "set_activity: solar (Planck-ratio facula) mode not built");
+  HIPCHK(c, hipSetDevice(c->device));
+  int64_t total = 0;
+  for (int p = 0; p < n_phase; p++) {
+    if (region_count[p] < 0) return fail(c, SOAP_ERR_ARG, "set_activity: negative region count");
+    total += region_count[p];
+  }
+  if (total > 0 && !regions) return fail(c, SOAP_ERR_ARG, "set_activity: null regions");
+  for (int64_t i = 0; i < total; i++)
+    if (!(regions[i].size >= 0.0 && regions[i].size < 1.0))
+      return fail(c, SOAP_ERR_ARG, "set_activity: region size %g outside [0, 1)", regions[i].size);
+  c->act = *act;
+  c->P = n_phase;
+  c->phases.assign(phase, phase + n_phase);
+  c->reg_count.assign(region_count, region_count + n_phase);
+  c->regions.assign(regions, regions + total);
+  // geometry per (phase, visible region)
+  int G = c->G;
+  c->geo.clear();
+  c->geo_off.assign(1, 0);
+  c->box.clear();
+  int cpb = std::max(1, 4096 / std::max(1, G));
+  c->max_col_blocks = 0;
+  int64_t ri = 0;
+  for (int p = 0; p < n_phase; p++) {
+    int by0 = G, by1 = 0, bz0 = G, bz1 = 0;
+    for (int r = 0; r < region_count[p]; r++, ri++) {
+      const soap_region &rg = c->regions[ri];
+      int bnd[5];
+      region_bbox(rg.lat_deg, rg.lon_deg, rg.size, c->star.incl_deg, act->nrho, G, phase[p], bnd);
+      if (bnd[4] != 1) continue;
+      RegionDev g{};
+      double r0[3];
+      inverse_rotation_row0(rg.lon_deg, rg.lat_deg, c->star.incl_deg, phase[p], r0);
+      g.r0 = r0[0];
+      g.r1 = r0[1];
+      g.r2 = r0[2];
+      g.oms2 = 1. - rg.size * rg.size;
+      g.y0 = std::max(0, bnd[0]);
+      g.y1 = std::min(G, bnd[2]);
+      g.z0 = std::max(0, bnd[1]);
+      g.z1 = std::min(G, bnd[3]);
+      g.type = rg.type == 0.0 ? 0 : 1;
+      if (g.y0 >= g.y1 || g.z0 >= g.z1) continue;
+      c->geo.push_back(g);
+      by0 = std::min(by0, g.y0);
+      by1 = std::max(by1, g.y1);
+      bz0 = std::min(bz0, g.z0);
+      bz1 = std::max(bz1, g.z1);
+    }
+    c->geo_off.push_back((int32_t)c->geo.size());
+    if (by0 >= by1) by0 = by1 = bz0 = bz1 = 0;
+    c->box.insert(c->box.end(), {by0, by1, bz0, bz1});
+    c->max_col_blocks = std::max(c->max_col_blocks, (by1 - by0 + cpb - 1) / cpb);
+  }
+  int rc;
+  if ((rc = dalloc(c, &c->d_geo, std::max<size_t>(1, c->geo.size()))) ||
+      (rc = dalloc(c, &c->d_geo_off, c->geo_off.size())) || (rc = dalloc(c, &c->d_box, std::max<size_t>(4, c->box.size()))))
+    return rc;
+  if (!c->geo.empty())
+    HIPCHK(c, hipMemcpyAsync(c->d_geo, c->geo.data(), c->geo.size() * sizeof(RegionDev), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_geo_off, c->geo_off.data(), c->geo_off.size() * 4, hipMemcpyHostToDevice, c->stream));
+  if (!c->box.empty())
+    HIPCHK(c, hipMemcpyAsync(c->d_box, c->box.data(), c->box.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if ((rc = dalloc(c, &c->d_raw, (size_t)std::max(1, n_phase) * 3 * c->n))) return rc;
+  c->have_raw = c->have_lower = c->have_ccf = c->have_fit = false;
+  return SOAP_OK;
+}
+
+int soap_integrate(soap_ctx *c) {
+  if (!c) return SOAP_ERR_ARG;
+  if (!c->have_maps || c->phases.size() != (size_t)c->P) return fail(c, SOAP_ERR_STATE, "integrate: set activity first");
+  int rc;
+  if ((rc = check_cube(c, SOAP_CUBE_QUIET, "integrate")) || (rc = check_cube(c, SOAP_CUBE_QUIET_SED, "integrate")))
+    return rc;
+  bool any_spot = false, any_fac = false;
+  for (auto &g : c->geo) (g.type == 0 ? any_spot : any_fac) = true;
+  if (any_spot && ((rc = check_cube(c, SOAP_CUBE_SPOT, "integrate")) || (rc = check_cube(c, SOAP_CUBE_SPOT_SED, "integrate"))))
+    return rc;
+  if (any_fac && ((rc = check_cube(c, SOAP_CUBE_FAC, "integrate")) || (rc = check_cube(c, SOAP_CUBE_FAC_SED, "integrate"))))
+    return rc;
+  if (c->P == 0) {
+    c->have_raw = true;
+    return SOAP_OK;
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  if ((rc = ensure_taps(c, c->P, 3, 4))) return rc;
+  TapAccArgs a{};
+  a.grid = c->G;
+  a.n_mu = c->n_mu;
+  a.dmin = c->dmin;
+  a.nd = c->nd;
+  a.nsets = 3;
+  a.wd = tap_window(c);
+  a.cols_per_block = std::max(1, 4096 / std::max(1, c->G));
+  a.region = c->d_region;
+  a.limb = c->d_limb;
+  a.delta = c->d_delta;
+  a.lbm = c->d_lbm;
+  a.m = c->d_m;
+  a.t = c->d_t;
+  a.acc = (unsigned long long *)c->acc.p;
+  a.geo = c->d_geo;
+  a.geo_off = c->d_geo_off;
+  a.box = c->d_box;
+  {
+    StageTimer tm(c, ST_TAPS_ACTIVE);
+    HIPCHK(c, launch_fill_u64(a.acc, (size_t)c->P * 3 * c->n_mu * c->nd * 2, c->stream));
+    HIPCHK(c, launch_taps_active(a, c->P, c->max_col_blocks, c->stream));
+    HIPCHK(c, launch_taps_finalize(a.acc, c->P, 3, 4, c->n_mu, c->nd, (double *)c->taps.p, (int32_t *)c->krange.p,
+                                   (int32_t *)c->kflags.p, (int32_t *)c->bins.p, (int32_t *)c->nbins.p, c->stream));
+  }
+  FirArgs f{};
+  f.q = c->d_cube[SOAP_CUBE_QUIET];
+  f.qs = c->d_cube[SOAP_CUBE_QUIET_SED];
+  f.sp = c->d_cube[SOAP_CUBE_SPOT];
+  f.sps = c->d_cube[SOAP_CUBE_SPOT_SED];
+  f.fa = c->d_cube[SOAP_CUBE_FAC];
+  f.fas = c->d_cube[SOAP_CUBE_FAC_SED];
+  f.n = c->n;
+  f.n_mu = c->n_mu;
+  f.dmin = c->dmin;
+  f.nd = c->nd;
+  f.taps = (const double *)c->taps.p;
+  f.krange = (const int32_t *)c->krange.p;
+  f.kflags = (const int32_t *)c->kflags.p;
+  f.bins = (const int32_t *)c->bins.p;
+  f.nbins = (const int32_t *)c->nbins.p;
+  f.out = c->d_raw;
+  {
+    StageTimer tm(c, ST_FIR_ACTIVE);
+    HIPCHK(c, launch_fir_active(f, c->P, c->stream));
+  }
+  c->have_raw = true;
+  c->have_lower = c->have_ccf = c->have_fit = false;
+  c->alg_bytes = -1;  // computed lazily
+  return SOAP_OK;
+}
+
+int soap_get_integrate_stats(soap_ctx *c, double *alg_bytes, int64_t *rows_touched) {
+  if (!c) return SOAP_ERR_ARG;
+  if (!c->have_raw || c->P == 0) return fail(c, SOAP_ERR_STATE, "integrate_stats: run integrate first");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<int32_t> fl((size_t)c->P * c->n_mu);
+  HIPCHK(c, hipMemcpyAsync(fl.data(), c->kflags.p, fl.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int64_t rows = 0;
+  double bytes = 0;
+  for (int p = 0; p < c->P; p++) {
+    int64_t r = 0;
+    bool s = false, f = false;
+    for (int k = 0; k < c->n_mu; k++) {
+      int v = fl[(size_t)p * c->n_mu + k];
+      if (v) r += 2;  // quiet normalised + SED rows
+      if (v & 1) r += 1, s = true;
+      if (v & 2) r += 1, f = true;
+    }
+    rows += r;
+    bytes += 8.0 * c->n * (double)(r + (s ? 1 : 0) + (f ? 1 : 0) + 3);
+  }
+  if (alg_bytes) *alg_bytes = bytes;
+  if (rows_touched) *rows_touched = rows;
+  return SOAP_OK;
+}
+
+int soap_get_raw(soap_ctx *c, int32_t phase, double *flux, double *bconv, double *tot) {
+  if (!c || phase < 0 || phase >= c->P) return fail(c, SOAP_ERR_ARG, "get_raw: bad phase");
+  if (!c->have_raw) return fail(c, SOAP_ERR_STATE, "get_raw: run integrate first");
+  HIPCHK(c, hipSetDevice(c->device));
+  double *outs[3] = {flux, bconv, tot};
+  for (int t = 0; t < 3; t++)
+    if (outs[t])
+      HIPCHK(c, hipMemcpyAsync(outs[t], c->d_raw + ((size_t)phase * 3 + t) * c->n, c->n * 8, hipMemcpyDeviceToHost,
+                               c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SOAP_OK;
+}
+
+int soap_set_raw(soap_ctx *c, int32_t phase, const double *flux, const double *bconv, const double *tot) {
+  if (!c || phase < 0 || phase >= c->P || !flux || !bconv || !tot) return fail(c, SOAP_ERR_ARG, "set_raw: bad arguments");
+  HIPCHK(c, hipSetDevice(c->device));
+  const double *ins[3] = {flux, bconv, tot};
+  for (int t = 0; t < 3; t++)
+    HIPCHK(c, hipMemcpyAsync(c->d_raw + ((size_t)phase * 3 + t) * c->n, ins[t], c->n * 8, hipMemcpyHostToDevice,
+                             c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_raw = true;
+  return SOAP_OK;
+}
+
+int soap_phase_maps(soap_ctx *c, int32_t phase, double *temp_map, double *bi_map) {
+  if (!c || phase < 0 || phase >= c->P || !temp_map || !bi_map) return fail(c, SOAP_ERR_ARG, "phase_maps: bad arguments");
+  HIPCHK(c, hipSetDevice(c->device));
+  size_t g2 = (size_t)c->G * c->G;
+  double *d_tmp = nullptr;
+  int rc = dalloc(c, &d_tmp, 2 * g2);
+  if (rc) return rc;
+  int g0 = c->geo_off[phase], g1 = c->geo_off[phase + 1];
+  hipError_t e = launch_phase_maps(c->G, c->act.t_star, c->act.t_diff_spot, c->act.t_diff_fac, c->d_geo + g0, g1 - g0,
+                                   d_tmp, d_tmp + g2, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(temp_map, d_tmp, g2 * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(bi_map, d_tmp + g2, g2 * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  dfree_c(c, d_tmp);
+  if (e != hipSuccess) return fail(c, SOAP_ERR_HIP, "phase_maps: %s", hipGetErrorString(e));
+  return SOAP_OK;
+}
+
+// ------------------------------------------------------------ resolution
+// nb_pixel with Delta_lambda_cpu's single-precision sqrtf, SOAP_resolution.cu:18-49,241-244
+static double rndup(double n, int nb) {
+  double t = n * pow(10, nb) - floor(n * pow(10, nb));
+  if (t >= 0.5) {
+    n *= pow(10, nb);
+    n = ceil(n);
+    n /= pow(10, nb);
+  } else {
+    n *= pow(10, nb);
+    n = floor(n);
+    n /= pow(10, nb);
+  }
+  return n;
+}
+
+static int conv_window(const std::vector<double> &w, double sigma_ms) {
+  double beta = sigma_ms / kC;
+  double dl = -1 * w[0] * (1 - sqrtf((float)((1 + beta) / (1 - beta))));
+  return (int)rndup((50. * dl) / (w[1] - w[0]), 0);
+}
+
+int soap_resolution(soap_ctx *c, double sigma_ms, double quiet_weight) {
+  (void)quiet_weight;  // linear grids only
+  if (!c) return SOAP_ERR_ARG;
+  if (!c->have_quiet) return fail(c, SOAP_ERR_STATE, "resolution: quiet star missing");
+  if (c->P > 0 && !c->have_raw) return fail(c, SOAP_ERR_STATE, "resolution: raw spectra missing");
+  if (!(sigma_ms > 0)) return fail(c, SOAP_ERR_ARG, "resolution: sigma must be > 0");
+  HIPCHK(c, hipSetDevice(c->device));
+  int wid = conv_window(c->wave, sigma_ms);
+  if (wid < 1 || 2 * (int64_t)wid + 2 >= c->n) return fail(c, SOAP_ERR_ARG, "resolution: window %d vs %lld pixels", wid, (long long)c->n);
+  // log-grid taps c_d = exp(d h) expm1(h)/(fs sqrt(2 pi)) exp(-(expm1(d h)/fs)^2/2),
+  // truncated where c_d < 2^-60 c_0 (far below FP64 resolution of the sum)
+  double fs = -1 * 1.0 * (1 - sqrt((1 + sigma_ms / kC) / (1 - sigma_ms / kC)));
+  auto tap = [&](int d) {
+    double u = expm1(d * c->h) / fs;
+    return exp(d * c->h) * c->expm1_h / (fs * sqrt(2. * kPi)) * exp(-u * u / 2);
+  };
+  double c0 = tap(0);
+  int dt = 0;
+  for (int d = 1; d <= wid; d++)
+    if (tap(d) >= c0 * 8.673617379884035e-19 || tap(-d) >= c0 * 8.673617379884035e-19) dt = d;
+  ConvArgs ca{};
+  ca.n = c->n;
+  ca.wid = wid;
+  ca.dlo = -std::min(wid, dt);
+  ca.dhi = std::min(wid - 2, dt);
+  std::vector<double> ct(ca.dhi - ca.dlo + 1);
+  for (int d = ca.dlo; d <= ca.dhi; d++) ct[d - ca.dlo] = tap(d);
+  ca.n_tiles = (int)((c->n + 255) / 256);
+  int n_act = 3 * c->P, n_spec = n_act + 2;
+  int rc;
+  if ((rc = dalloc(c, &c->d_lower, (size_t)(c->P + 1) * 3 * c->n)) || (rc = ensure(c, c->ctap, ct.size() * 8)) ||
+      (rc = ensure(c, c->part, (size_t)n_spec * ca.n_tiles * 2 * 8)) || (rc = ensure(c, c->mm, (size_t)n_spec * 2 * 8)) ||
+      (rc = ensure(c, c->qsel, (size_t)n_spec * 4)) || (rc = ensure(c, c->variant, (size_t)n_spec * 4)))
+    return rc;
+  std::vector<int32_t> qsel(n_spec), var(n_spec);
+  for (int p = 0; p < c->P; p++)
+    for (int t = 0; t < 3; t++) {
+      qsel[p * 3 + t] = t == SOAP_BCONV ? 0 : 1;  // SOAP_resolution.cu:414-416
+      var[p * 3 + t] = 1;
+    }
+  qsel[n_act] = 1;  // _SED_quiet_sun_lower_spec
+  qsel[n_act + 1] = 0;  // _quiet_sun_lower_spec
+  var[n_act] = var[n_act + 1] = 0;
+  if (c->res_sigma != sigma_ms || c->res_P != c->P) {
+    HIPCHK(c, hipMemcpyAsync(c->ctap.p, ct.data(), ct.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->qsel.p, qsel.data(), n_spec * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->variant.p, var.data(), n_spec * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->res_sigma = sigma_ms;
+    c->res_P = c->P;
+  }
+  ca.ctap = (const double *)c->ctap.p;
+  double *lower_q = c->d_lower + (size_t)n_act * c->n;  // frame P: [flux, bconv, tot]
+  {
+    StageTimer tm(c, ST_CONV);
+    if (n_act) HIPCHK(c, launch_conv(ca, c->d_quiet, c->d_raw, n_act, (const int32_t *)c->qsel.p,
+                                     (const int32_t *)c->variant.p, c->d_lower, (double *)c->part.p, c->stream));
+    // quiet frames written to the flux slot (SED) and the bconv slot (normalised)
+    HIPCHK(c, launch_conv(ca, c->d_quiet, c->d_quiet, 2, (const int32_t *)c->qsel.p + n_act,
+                          (const int32_t *)c->variant.p + n_act, lower_q,
+                          (double *)c->part.p + (size_t)n_act * ca.n_tiles * 2, c->stream));
+  }
+  {
+    StageTimer tm(c, ST_NORM);
+    HIPCHK(c, launch_norm(ca, n_spec, (const double *)c->part.p, (double *)c->mm.p, c->d_lower, c->stream));
+    HIPCHK(c, hipMemcpyAsync(lower_q + 2 * c->n, lower_q, c->n * 8, hipMemcpyDeviceToDevice, c->stream));
+  }
+  c->have_lower = true;
+  c->have_ccf = c->have_fit = false;
+  return SOAP_OK;
+}
+
+int soap_get_lower(soap_ctx *c, int32_t phase, int type, double *out) {
+  if (!c || !out || type < 0 || type > 2 || phase < -1 || phase >= c->P) return fail(c, SOAP_ERR_ARG, "get_lower: bad arguments");
+  if (!c->have_lower) return fail(c, SOAP_ERR_STATE, "get_lower: run resolution first");
+  HIPCHK(c, hipSetDevice(c->device));
+  int frame = phase < 0 ? c->P : phase;
+  HIPCHK(c, hipMemcpyAsync(out, c->d_lower + ((size_t)frame * 3 + type) * c->n, c->n * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SOAP_OK;
+}
+
+int soap_get_norm(soap_ctx *c, int32_t phase, int type, double *minmax2) {
+  if (!c || !minmax2 || type < 0 || type > 2 || phase < -1 || phase >= c->P) return fail(c, SOAP_ERR_ARG, "get_norm: bad arguments");
+  if (!c->have_lower) return fail(c, SOAP_ERR_STATE, "get_norm: run resolution first");
+  HIPCHK(c, hipSetDevice(c->device));
+  int sp = phase >= 0 ? phase * 3 + type : 3 * c->P + (type == SOAP_BCONV ? 1 : 0);
+  HIPCHK(c, hipMemcpyAsync(minmax2, (double *)c->mm.p + sp * 2, 16, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SOAP_OK;
+}
+
+// ------------------------------------------------------------------ CCF
+int soap_set_mask(soap_ctx *c, const double *line_l, const double *line_w, int32_t n_lines, const double *vgrid,
+                  int32_t n_v, int32_t trim) {
+  if (!c || n_lines < 1 || !line_l || !line_w || !vgrid || n_v < 3 || trim < 0) return fail(c, SOAP_ERR_ARG, "set_mask: bad arguments");
+  if (c->n == 0) return fail(c, SOAP_ERR_STATE, "set_mask: set the wavelength grid first");
+  if (2 * (int64_t)trim + 8 > c->n) return fail(c, SOAP_ERR_ARG, "set_mask: trim %d too large", trim);
+  if (n_v > 1024) return fail(c, SOAP_ERR_LIMIT, "set_mask: at most 1024 velocities");
+  HIPCHK(c, hipSetDevice(c->device));
+  c->n_lines = n_lines;
+  c->n_v = n_v;
+  c->trim = trim;
+  int rc;
+  if ((rc = dalloc(c, &c->d_line_l, n_lines)) || (rc = dalloc(c, &c->d_line_w, n_lines)) ||
+      (rc = dalloc(c, &c->d_vgrid, n_v)) || (rc = dalloc(c, &c->d_appod, n_v)))
+    return rc;
+  // wavelength order of the lines and the neighbour scan depth for overwrites
+  std::vector<int32_t> srt(n_lines), rnk(n_lines);
+  std::iota(srt.begin(), srt.end(), 0);
+  std::stable_sort(srt.begin(), srt.end(), [&](int a, int b) { return line_l[a] < line_l[b]; });
+  for (int i = 0; i < n_lines; i++) rnk[srt[i]] = i;
+  double lmax = 0;
+  for (int i = 0; i < n_lines; i++) lmax = std::max(lmax, fabs(line_l[i]));
+  double beta = 820. / kC;
+  double lim = 2 * lmax * (sqrt((1 + beta) / (1 - beta)) - 1) + 4 * lmax * c->expm1_h + 1e-9;
+  int scan = 1;
+  for (int i = 0, j = 0; i < n_lines; i++) {
+    while (line_l[srt[i]] - line_l[srt[j]] > lim) j++;
+    scan = std::max(scan, i - j + 1);
+  }
+  c->mask_scan = scan;
+  // a mask hole must span at most 30 pixels (32-bit alive masks per line)
+  double hole_px = (sqrt((1 + beta) / (1 - beta)) - 1) * 1.001 / c->h + 3;
+  if (hole_px > 30) return fail(c, SOAP_ERR_LIMIT, "set_mask: mask holes span %.1f pixels (> 30)", hole_px);
+  // appodisation window, run_all_SOAP.py:306-311
+  std::vector<double> ap;
+  {
+    int nb0 = 5, period = (int)((n_v - 1) / 2.), len_app = (int)(period / 2.);
+    for (int i = 0; i < nb0; i++) ap.push_back(0.0);
+    for (int a = 0; a < len_app; a++) ap.push_back(0.5 * cos(2 * kPi / period * a - kPi) + 0.5);
+    for (int i = 0; i < n_v - period - 2 * nb0; i++) ap.push_back(1.0);
+    for (int a = len_app - 1; a >= 0; a--) ap.push_back(0.5 * cos(2 * kPi / period * a - kPi) + 0.5);
+    for (int i = 0; i < nb0; i++) ap.push_back(0.0);
+    if ((int)ap.size() != n_v) return fail(c, SOAP_ERR_ARG, "set_mask: %d velocities too few for the appodisation", n_v);
+  }
+  size_t ne = (size_t)n_v * n_lines;
+  if ((rc = ensure(c, c->m_start, ne * 4)) || (rc = ensure(c, c->m_len, ne * 4)) || (rc = ensure(c, c->m_val, ne * 16)) ||
+      (rc = ensure(c, c->m_alive, ne * 4)) || (rc = ensure(c, c->sorted, (size_t)n_lines * 4)) ||
+      (rc = ensure(c, c->rank, (size_t)n_lines * 4)))
+    return rc;
+  HIPCHK(c, hipMemcpyAsync(c->d_line_l, line_l, n_lines * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_line_w, line_w, n_lines * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_vgrid, vgrid, n_v * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->d_appod, ap.data(), n_v * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->sorted.p, srt.data(), n_lines * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->rank.p, rnk.data(), n_lines * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_mask = true;
+  c->mask_built = false;
+  c->have_ccf = c->have_fit = false;
+  return SOAP_OK;
+}
+
+static int build_mask(soap_ctx *c) {
+  StageTimer tm(c, ST_MASK);
+  int64_t n_t = c->n - 2 * (int64_t)c->trim;
+  HIPCHK(c, launch_mask_build(c->d_line_l, c->d_line_w, c->n_lines, c->d_vgrid, c->n_v, c->wave[c->trim], c->h, n_t,
+                              (int32_t *)c->m_start.p, (int32_t *)c->m_len.p, (double *)c->m_val.p, c->stream));
+  HIPCHK(c, launch_mask_overwrite((const int32_t *)c->sorted.p, (const int32_t *)c->rank.p, c->n_lines, c->n_v,
+                                  c->mask_scan, (const int32_t *)c->m_start.p, (const int32_t *)c->m_len.p,
+                                  (uint32_t *)c->m_alive.p, c->stream));
+  c->mask_built = true;
+  return SOAP_OK;
+}
+
+int soap_ccf(soap_ctx *c) {
+  if (!c) return SOAP_ERR_ARG;
+  if (!c->have_mask) return fail(c, SOAP_ERR_STATE, "ccf: set the mask first");
+  if (!c->have_lower) return fail(c, SOAP_ERR_STATE, "ccf: run resolution first");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc;
+  if (!c->mask_built && (rc = build_mask(c))) return rc;
+  int n_spec = (c->P + 1) * 3;
+  if ((rc = dalloc(c, &c->d_ccf, (size_t)n_spec * c->n_v))) return rc;
+  {
+    StageTimer tm(c, ST_CCF);
+    HIPCHK(c, launch_ccf(c->d_lower, c->n, c->trim, n_spec, c->n_lines, c->n_v, c->n - 2 * (int64_t)c->trim,
+                         (const int32_t *)c->m_start.p, (const int32_t *)c->m_len.p, (const uint32_t *)c->m_alive.p,
+                         (const double *)c->m_val.p, c->d_line_w, c->d_ccf, c->stream));
+    HIPCHK(c, launch_ccf_finish(c->d_ccf, n_spec, c->n_v, c->d_appod, c->stream));
+  }
+  c->have_ccf = true;
+  c->have_fit = false;
+  return SOAP_OK;
+}
+
+int soap_get_ccf(soap_ctx *c, double *out) {
+  if (!c || !out) return SOAP_ERR_ARG;
+  if (!c->have_ccf) return fail(c, SOAP_ERR_STATE, "get_ccf: run ccf first");
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMemcpyAsync(out, c->d_ccf, (size_t)(c->P + 1) * 3 * c->n_v * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SOAP_OK;
+}
+
+static int ensure_fit(soap_ctx *c, int n_ccf, int n_v) {
+  int rc;
+  if ((rc = ensure(c, c->f_par, (size_t)n_ccf * 5 * 8)) || (rc = ensure(c, c->f_model, (size_t)n_ccf * n_v * 8)) ||
+      (rc = ensure(c, c->f_depth, (size_t)n_ccf * 91 * 8)) || (rc = ensure(c, c->f_bis, (size_t)n_ccf * 91 * 8)) ||
+      (rc = ensure(c, c->f_iter, (size_t)n_ccf * 4)))
+    return rc;
+  return SOAP_OK;
+}
+
+int soap_fit(soap_ctx *c) {
+  if (!c) return SOAP_ERR_ARG;
+  if (!c->have_ccf) return fail(c, SOAP_ERR_STATE, "fit: run ccf first");
+  HIPCHK(c, hipSetDevice(c->device));
+  int n_ccf = (c->P + 1) * 3;
+  int rc = ensure_fit(c, n_ccf, c->n_v);
+  if (rc) return rc;
+  {
+    StageTimer tm(c, ST_FIT);
+    HIPCHK(c, launch_fit(c->d_vgrid, c->n_v, c->d_ccf, n_ccf, (double *)c->f_par.p, (double *)c->f_model.p,
+                         (double *)c->f_depth.p, (double *)c->f_bis.p, (int32_t *)c->f_iter.p, c->stream));
+  }
+  c->have_fit = true;
+  return SOAP_OK;
+}
+
+int soap_get_fit(soap_ctx *c, double *par, double *model, double *depth, double *bis) {
+  if (!c) return SOAP_ERR_ARG;
+  if (!c->have_fit) return fail(c, SOAP_ERR_STATE, "get_fit: run fit first");
+  HIPCHK(c, hipSetDevice(c->device));
+  size_t n_ccf = (size_t)(c->P + 1) * 3;
+  if (par) HIPCHK(c, hipMemcpyAsync(par, c->f_par.p, n_ccf * 5 * 8, hipMemcpyDeviceToHost, c->stream));
+  if (model) HIPCHK(c, hipMemcpyAsync(model, c->f_model.p, n_ccf * c->n_v * 8, hipMemcpyDeviceToHost, c->stream));
+  if (depth) HIPCHK(c, hipMemcpyAsync(depth, c->f_depth.p, n_ccf * 91 * 8, hipMemcpyDeviceToHost, c->stream));
+  if (bis) HIPCHK(c, hipMemcpyAsync(bis, c->f_bis.p, n_ccf * 91 * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SOAP_OK;
+}
+
+int soap_fit_ccf_host(soap_ctx *c, const double *v, int32_t n_v, const double *y, int32_t n_ccf, double *par,
+                      double *model, double *depth, double *bis) {
+  if (!c || !v || !y || n_v < 3 || n_v > 1024 || n_ccf < 1 || !par) return fail(c, SOAP_ERR_ARG, "fit_ccf_host: bad arguments");
+  HIPCHK(c, hipSetDevice(c->device));
+  double *dv = nullptr, *dy = nullptr, *dpar = nullptr, *dmod = nullptr, *ddep = nullptr, *dbis = nullptr;
+  int rc = SOAP_OK;
+  if ((rc = dalloc(c, &dv, n_v)) || (rc = dalloc(c, &dy, (size_t)n_ccf * n_v)) || (rc = dalloc(c, &dpar, (size_t)n_ccf * 5)) ||
+      (rc = dalloc(c, &dmod, (size_t)n_ccf * n_v)) || (rc = dalloc(c, &ddep, (size_t)n_ccf * 91)) ||
+      (rc = dalloc(c, &dbis, (size_t)n_ccf * 91))) {
+    for (double *p : {dv, dy, dpar, dmod, ddep, dbis}) dfree_c(c, p);
+    return rc;
+  }
+  hipError_t e = hipMemcpyAsync(dv, v, n_v * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dy, y, (size_t)n_ccf * n_v * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = launch_fit(dv, n_v, dy, n_ccf, dpar, dmod, ddep, dbis, nullptr, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(par, dpar, (size_t)n_ccf * 5 * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && model) e = hipMemcpyAsync(model, dmod, (size_t)n_ccf * n_v * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && depth) e = hipMemcpyAsync(depth, ddep, (size_t)n_ccf * 91 * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && bis) e = hipMemcpyAsync(bis, dbis, (size_t)n_ccf * 91 * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  for (double *p : {dv, dy, dpar, dmod, ddep, dbis}) dfree_c(c, p);
+  if (e != hipSuccess) return fail(c, SOAP_ERR_HIP, "fit_ccf_host: %s", hipGetErrorString(e));
+  return SOAP_OK;
+}
+
+int soap_run_series(soap_ctx *c, const soap_star *star, double sigma_ms, double quiet_weight) {
+  int rc;
+  if ((rc = soap_disk_init(c, star))) return rc;
+  if ((rc = soap_quiet_star(c))) return rc;
+  if ((rc = soap_integrate(c))) return rc;
+  if ((rc = soap_resolution(c, sigma_ms, quiet_weight))) return rc;
+  if (c->have_mask) {
+    if ((rc = soap_ccf(c))) return rc;
+    if ((rc = soap_fit(c))) return rc;
+  }
+  return SOAP_OK;
+}
+
+}  // extern "C"

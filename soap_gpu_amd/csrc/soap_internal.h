@@ -1,0 +1,102 @@
+// Internal declarations shared by the host engine (soap_engine.hip) and the
+// gfx950 kernels (soap_kernels.hip).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace soap {
+
+constexpr double kC = 299792458.;
+constexpr double kPi = 3.141592653589793238463;
+
+// Region geometry of one (phase, region), evaluated on the host once per phase
+// (spot_init_gpu/spot_phase_gpu/spot_area + the first row of spot_inverse_rotation).
+struct RegionDev {
+  double r0, r1, r2;  // row 0 of the inverse rotation R3 = R2 R
+  double oms2;        // 1. - s*s
+  int32_t y0, y1, z0, z1;  // scan box [y0,y1) x [z0,z1), clamped to the grid
+  int32_t type;       // 0 spot, 1 facula
+  int32_t pad;
+};
+
+struct DiskArgs {
+  int32_t grid, n_mu;
+  double delta_grid, theta_grid, omega0, omega1, radius, sin_incl, u1, u2;
+  double lb0, lb1, lb2, lb3;
+  const double *mu_bounds;  // n_mu - 1
+  int32_t log_grid;
+  double h, expm1_h;
+  double *mu, *delta, *limb, *lbm;
+  int32_t *region;
+  int32_t *m;  // per-cell integer pixel shift (log grids)
+  double *t;   // per-cell interpolation fraction (log grids)
+};
+
+// Fixed-point tap accumulation (exact, order independent): value =
+// hi * 2^-32 + lo * 2^-72, hi signed, lo unsigned.
+struct TapAccArgs {
+  int32_t grid, n_mu, dmin, nd, nsets, wd, cols_per_block;
+  const int32_t *region;
+  const double *limb, *delta, *lbm;
+  const int32_t *m;
+  const double *t;
+  unsigned long long *acc;  // [job][nsets][n_mu][nd][2]
+  // active mode
+  const RegionDev *geo;
+  const int32_t *geo_off;   // [n_job + 1]
+  const int32_t *box;       // [n_job][4]: cy0, cy1, cz0, cz1
+};
+
+struct FirArgs {
+  const double *q, *qs, *sp, *sps, *fa, *fas;  // cube bases [n_mu][n]
+  int64_t n;
+  int32_t n_mu, dmin, nd;
+  const double *taps;    // [job][n_mu][nsets][nd]
+  const int32_t *krange; // [job][n_mu][2] inclusive d-index range, lo > hi: empty
+  const int32_t *kflags; // [job][n_mu]: bit0 spot sets nonzero, bit1 facula sets nonzero
+  const int32_t *bins;   // [job][n_mu]
+  const int32_t *nbins;  // [job]
+  double *out;           // quiet: [2][n]; active: [job][3][n]
+};
+
+struct ConvArgs {
+  int64_t n;
+  int32_t wid;           // reference window half width (nb_pixel)
+  int32_t dlo, dhi;      // taps used: d in [dlo, dhi]
+  const double *ctap;    // [dhi - dlo + 1]
+  int32_t n_tiles;
+};
+
+}  // namespace soap
+
+// launchers (soap_kernels.hip)
+namespace soap {
+hipError_t launch_fill_u64(unsigned long long *p, size_t count, hipStream_t s);
+hipError_t launch_fill_f64(double *p, size_t count, double v, hipStream_t s);
+hipError_t launch_disk_init(const DiskArgs &a, hipStream_t s);
+hipError_t launch_taps_quiet(const TapAccArgs &a, hipStream_t s);
+hipError_t launch_taps_active(const TapAccArgs &a, int n_job, int max_col_blocks, hipStream_t s);
+hipError_t launch_taps_finalize(const unsigned long long *acc, int n_job, int nsets_in, int nsets_out, int n_mu,
+                                int nd, double *taps, int32_t *krange, int32_t *kflags, int32_t *bins,
+                                int32_t *nbins, hipStream_t s);
+hipError_t launch_fir_quiet(const FirArgs &a, hipStream_t s);
+hipError_t launch_fir_active(const FirArgs &a, int n_job, hipStream_t s);
+hipError_t launch_phase_maps(int grid, int t_star, int t_diff_spot, int t_diff_fac, const RegionDev *geo,
+                             int n_geo, double *temp, double *bi, hipStream_t s);
+// resolution: spectra sp = 0..n_spec-1; A = quiet[qidx[sp]] - raw[sp] (variant 1) or quiet[qidx] (variant 0)
+hipError_t launch_conv(const ConvArgs &c, const double *quiet, const double *raw, int n_spec,
+                       const int32_t *qsel, const int32_t *variant, double *o, double *part, hipStream_t s);
+hipError_t launch_norm(const ConvArgs &c, int n_spec, const double *part, double *mm, double *o, hipStream_t s);
+// CCF
+hipError_t launch_mask_build(const double *line_l, const double *line_w, int n_lines, const double *vgrid,
+                             int n_v, double lam_t0, double h, int64_t n_t, int32_t *m_start, int32_t *m_len,
+                             double *m_val, hipStream_t s);
+hipError_t launch_mask_overwrite(const int32_t *sorted, const int32_t *rank, int n_lines, int n_v, int scan,
+                                 const int32_t *m_start, const int32_t *m_len, uint32_t *m_alive, hipStream_t s);
+hipError_t launch_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
+                      int64_t n_t, const int32_t *m_start, const int32_t *m_len, const uint32_t *m_alive,
+                      const double *m_val, const double *line_w, double *ccf, hipStream_t s);
+hipError_t launch_ccf_finish(double *ccf, int n_spec, int n_v, const double *appod, hipStream_t s);
+hipError_t launch_fit(const double *v, int n_v, const double *y, int n_ccf, double *par, double *model,
+                      double *depth, double *bis, int32_t *iters, hipStream_t s);
+}  // namespace soap

@@ -1,0 +1,959 @@
+// gfx950 kernels of the SOAP-GPU engine.  All arithmetic FP64.
+//
+// Hot-path design (log-uniform grids): on lambda_j = lambda_0 exp(j h) a Doppler
+// shift is a constant fractional pixel offset s = log1p(fbeta)/h, so the
+// reference's per-cell "shift + linear interpolation" (SOAP_integration.cu:232-281)
+// equals (1-t) S[j-m] + t S[j-m+1] with m = ceil(s), t = expm1((m-s)h)/expm1(h)
+// for every pixel j.  Summing cells therefore reduces to a short FIR filter per
+// mu bin whose taps are the limb-weighted (1-t, t) of all cells with that shift:
+//   1. k_disk_init   : per-cell maps + (m, t)                       (a1)
+//   2. k_taps_*      : exact fixed-point accumulation of the taps   (a2/a5/a6)
+//   3. k_fir_*       : lambda-streaming FIR over the seed rows       (a2/a6)
+//   4. k_conv/k_norm : Gaussian instrumental convolution + spec_norm (a7)
+//   5. k_mask_*/k_ccf: sparse mask CCF                               (a8)
+//   6. k_fit         : Gaussian LM fit + bisector, one wave per CCF  (a9)
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "soap_internal.h"
+
+namespace soap {
+
+constexpr int kTPB = 256;
+
+// ------------------------------------------------------------------ utilities
+__global__ void k_fill_u64(unsigned long long *p, size_t count) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) p[i] = 0ull;
+}
+
+__global__ void k_fill_f64(double *p, size_t count, double v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < count; i += stride) p[i] = v;
+}
+
+static unsigned grid_for(size_t count) {
+  size_t b = (count + kTPB - 1) / kTPB;
+  if (b > 65535 * 8) b = 65535 * 8;
+  if (b == 0) b = 1;
+  return (unsigned)b;
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, size_t count, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  k_fill_u64<<<grid_for(count), kTPB, 0, s>>>(p, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(double *p, size_t count, double v, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  k_fill_f64<<<grid_for(count), kTPB, 0, s>>>(p, count, v);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ a1: disk
+// itot's cell loop, SOAP_initialize.cu:339-395.  Contraction is off so every
+// map value rounds exactly as the reference's separate multiply/add sequence.
+__global__ void k_disk_init(DiskArgs a) {
+#pragma clang fp contract(off)
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ncell = (long)a.grid * a.grid;
+  if (idx >= ncell) return;
+  int iy = (int)(idx / a.grid), iz = (int)(idx % a.grid);
+  double y = -1. + iy * a.delta_grid;
+  double z = -1. + iz * a.delta_grid;
+  double theta_z = -90.0 + iz * a.theta_grid;
+  double s = sin(theta_z * kPi / 180);
+  double omega_velo = (a.omega0 + a.omega1 * (s * s)) * kPi / (180 * 24 * 3600) * a.radius;
+  double delta = y * omega_velo * a.sin_incl;
+  double mu = -999.0, dl = -999.0, limb = -999.0, lb = -999.0;
+  int32_t k = 0, m = 0;
+  double t = 0.0;
+  if ((y * y + z * z) <= 1.) {
+    double r_cos = sqrt(1. - (y * y + z * z));
+    limb = 1. - a.u1 * (1 - r_cos) - a.u2 * (1 - r_cos) * (1 - r_cos);
+    lb = a.lb0 + a.lb1 * r_cos + a.lb2 * r_cos * r_cos + a.lb3 * r_cos * r_cos * r_cos;
+    k = a.n_mu - 1;
+    for (int b = 0; b < a.n_mu - 1; b++)
+      if (r_cos > a.mu_bounds[b]) {
+        k = b;
+        break;
+      }
+    mu = r_cos;
+    dl = delta;
+    if (a.log_grid) {
+      double beta = delta * 1000. / kC;
+      double fbeta = -1 * (1 - sqrt((1 + beta) / (1 - beta)));
+      double sh = log1p(fbeta) / a.h;
+      double mc = ceil(sh);
+      m = (int32_t)mc;
+      t = expm1((mc - sh) * a.h) / a.expm1_h;
+    }
+  }
+  a.mu[idx] = mu;
+  a.delta[idx] = dl;
+  a.limb[idx] = limb;
+  a.lbm[idx] = lb;
+  a.region[idx] = k;
+  if (a.m) {
+    a.m[idx] = m;
+    a.t[idx] = t;
+  }
+}
+
+hipError_t launch_disk_init(const DiskArgs &a, hipStream_t s) {
+  long ncell = (long)a.grid * a.grid;
+  k_disk_init<<<(unsigned)((ncell + kTPB - 1) / kTPB), kTPB, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ fixed point taps
+// A contribution x is split into hi = floor(x 2^32) and lo = (x 2^32 - hi) 2^40
+// (exact: power-of-two scalings and the subtraction of an integer part do not
+// round); sums of 64-bit integers are associative, so atomics give the exact,
+// order-independent sum of all contributions truncated at 2^-72.
+__device__ __forceinline__ void fx_split(double x, unsigned long long &hi, unsigned long long &lo) {
+  double y = x * 4294967296.0;
+  double fl = floor(y);
+  hi = (unsigned long long)(long long)fl;
+  lo = (unsigned long long)((y - fl) * 1099511627776.0);
+}
+
+__device__ __forceinline__ void fx_add(unsigned long long *p, double x) {
+  unsigned long long hi, lo;
+  fx_split(x, hi, lo);
+  atomicAdd(p, hi);
+  atomicAdd(p + 1, lo);
+}
+
+__device__ __forceinline__ double fx_value(unsigned long long hi, unsigned long long lo) {
+  long long h = (long long)hi + (long long)(lo >> 40);
+  unsigned long long r = lo & ((1ull << 40) - 1);
+  return (double)h * 2.3283064365386963e-10 + (double)r * 2.117582368135751e-22;  // 2^-32, 2^-72
+}
+
+// Shared part of the tap kernels: accumulate into an LDS window of WD tap offsets
+// [base, base+WD) per (set, bin); offsets outside it go straight to global.
+struct TapWindow {
+  unsigned long long *lds;  // [nsets][n_mu][wd][2]
+  unsigned long long *gacc; // [nsets][n_mu][nd][2] of this job
+  int base, wd, nd, dmin, n_mu;
+  __device__ void add(int set, int k, int d, double x) const {
+    if (x == 0.0) return;
+    int w = d - base;
+    if (w >= 0 && w < wd)
+      fx_add(lds + (((long)set * n_mu + k) * wd + w) * 2, x);
+    else if (d - dmin >= 0 && d - dmin < nd)  // always true: the host bounds every shift
+      fx_add(gacc + (((long)set * n_mu + k) * nd + (d - dmin)) * 2, x);
+  }
+};
+
+__device__ int block_min_int(int v, int *scratch) {
+  for (int off = 32; off > 0; off >>= 1) {
+    int o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) scratch[wv] = v;
+  __syncthreads();
+  int nw = (blockDim.x + 63) >> 6;
+  int r = scratch[0];
+  for (int i = 1; i < nw; i++) r = scratch[i] < r ? scratch[i] : r;
+  __syncthreads();
+  return r;
+}
+
+__device__ void flush_window(const TapWindow &w, int nsets) {
+  __syncthreads();
+  long total = (long)nsets * w.n_mu * w.wd;
+  for (long e = threadIdx.x; e < total; e += blockDim.x) {
+    unsigned long long hi = w.lds[e * 2], lo = w.lds[e * 2 + 1];
+    if (hi == 0ull && lo == 0ull) continue;
+    int wi = (int)(e % w.wd);
+    long sk = e / w.wd;  // set * n_mu + k
+    long g = (sk * w.nd + (w.base + wi - w.dmin)) * 2;
+    atomicAdd(w.gacc + g, hi);
+    atomicAdd(w.gacc + g + 1, lo);
+  }
+}
+
+// Quiet-star taps: every disk cell contributes limb*(1-t) at offset m and limb*t
+// at m-1 of its mu bin (itot's cell loop, SOAP_initialize.cu:339-395).
+__global__ void k_taps_quiet(TapAccArgs a) {
+  extern __shared__ unsigned long long lds_u64[];
+  __shared__ int scratch[16];
+  int G = a.grid;
+  int iy0 = blockIdx.x * a.cols_per_block;
+  int iy1 = min(G, iy0 + a.cols_per_block);
+  long ncell = (long)(iy1 - iy0) * G;
+  int mn = 1 << 30;
+  for (long c = threadIdx.x; c < ncell; c += blockDim.x) {
+    long idx = (long)iy0 * G + c;
+    if (!(a.limb[idx] == -999.0 && a.delta[idx] == -999.0)) mn = min(mn, a.m[idx] - 1);
+  }
+  int base = block_min_int(mn, scratch);
+  TapWindow w{lds_u64, a.acc, base, a.wd, a.nd, a.dmin, a.n_mu};
+  long lds_n = (long)a.nsets * a.n_mu * a.wd * 2;
+  for (long e = threadIdx.x; e < lds_n; e += blockDim.x) lds_u64[e] = 0ull;
+  __syncthreads();
+  for (long c = threadIdx.x; c < ncell; c += blockDim.x) {
+    long idx = (long)iy0 * G + c;
+    if (a.limb[idx] == -999.0 && a.delta[idx] == -999.0) continue;
+    double L = a.limb[idx], t = a.t[idx];
+    int k = a.region[idx], m = a.m[idx];
+    w.add(0, k, m, L * (1.0 - t));
+    w.add(0, k, m - 1, L * t);
+  }
+  flush_window(w, a.nsets);
+}
+
+hipError_t launch_taps_quiet(const TapAccArgs &a, hipStream_t s) {
+  int nb = (a.grid + a.cols_per_block - 1) / a.cols_per_block;
+  size_t lds = (size_t)a.nsets * a.n_mu * a.wd * 2 * sizeof(unsigned long long);
+  k_taps_quiet<<<nb, kTPB, lds, s>>>(a);
+  return hipGetLastError();
+}
+
+// Cell classification of one phase: loop over the phase's visible regions in
+// order, later ones overwriting earlier ones (SOAP_integration.cu:724-785,
+// spot_scan :346-387).  Returns -1 (quiet), 0 (spot) or 1 (facula).
+__device__ __forceinline__ int classify_cell(int iy, int iz, double delta_grid, const RegionDev *geo, int g0,
+                                             int g1, double *r_cos_out) {
+#pragma clang fp contract(off)
+  double y = -1. + iy * delta_grid;
+  double z = -1. + iz * delta_grid;
+  if (!(z * z + y * y < 1.)) return -1;
+  double xa0 = sqrt(1. - (y * y + z * z));
+  int res = -1;
+  for (int r = g0; r < g1; r++) {
+    const RegionDev &g = geo[r];
+    if (iy < g.y0 || iy >= g.y1 || iz < g.z0 || iz >= g.z1) continue;
+    double xi0 = g.r0 * xa0 + g.r1 * y + g.r2 * z;
+    if (xi0 * xi0 >= g.oms2) res = g.type;
+  }
+  *r_cos_out = xa0;
+  return res;
+}
+
+// Active taps of every phase (job = phase): spot cells add limb to set 0,
+// facula cells add limb to set 1 and (1+LB) limb to set 2 (the intensity factor
+// of fast_intensity_fac, SOAP_integration.cu:317-323,424-440).
+__global__ void k_taps_active(TapAccArgs a) {
+  extern __shared__ unsigned long long lds_u64[];
+  __shared__ int scratch[16];
+  int job = blockIdx.y;
+  int G = a.grid;
+  const int32_t *bx = a.box + job * 4;
+  int cy0 = bx[0], cy1 = bx[1], cz0 = bx[2], cz1 = bx[3];
+  int iy0 = cy0 + blockIdx.x * a.cols_per_block;
+  if (iy0 >= cy1) return;  // uniform over the block
+  int iy1 = min(cy1, iy0 + a.cols_per_block);
+  int nz = cz1 - cz0;
+  long ncell = (long)(iy1 - iy0) * nz;
+  int g0 = a.geo_off[job], g1 = a.geo_off[job + 1];
+  double dg = 2. / G;
+  int mn = 1 << 30;
+  for (long c = threadIdx.x; c < ncell; c += blockDim.x) {
+    long idx = (long)(iy0 + c / nz) * G + (cz0 + c % nz);
+    if (!(a.limb[idx] == -999.0 && a.delta[idx] == -999.0)) mn = min(mn, a.m[idx] - 1);
+  }
+  int base = block_min_int(mn, scratch);
+  TapWindow w{lds_u64, a.acc + (long)job * a.nsets * a.n_mu * a.nd * 2, base, a.wd, a.nd, a.dmin, a.n_mu};
+  long lds_n = (long)a.nsets * a.n_mu * a.wd * 2;
+  for (long e = threadIdx.x; e < lds_n; e += blockDim.x) lds_u64[e] = 0ull;
+  __syncthreads();
+  for (long c = threadIdx.x; c < ncell; c += blockDim.x) {
+    int iy = iy0 + (int)(c / nz), iz = cz0 + (int)(c % nz);
+    long idx = (long)iy * G + iz;
+    double rc;
+    int ty = classify_cell(iy, iz, dg, a.geo, g0, g1, &rc);
+    if (ty < 0) continue;
+    double L = a.limb[idx], t = a.t[idx];
+    int k = a.region[idx], m = a.m[idx];
+    if (ty == 0) {
+      w.add(0, k, m, L * (1.0 - t));
+      w.add(0, k, m - 1, L * t);
+    } else {
+      double L2 = (1 + a.lbm[idx]) * L;
+      w.add(1, k, m, L * (1.0 - t));
+      w.add(1, k, m - 1, L * t);
+      w.add(2, k, m, L2 * (1.0 - t));
+      w.add(2, k, m - 1, L2 * t);
+    }
+  }
+  flush_window(w, a.nsets);
+}
+
+hipError_t launch_taps_active(const TapAccArgs &a, int n_job, int max_col_blocks, hipStream_t s) {
+  if (n_job == 0 || max_col_blocks == 0) return hipSuccess;
+  size_t lds = (size_t)a.nsets * a.n_mu * a.wd * 2 * sizeof(unsigned long long);
+  dim3 grid(max_col_blocks, n_job);
+  k_taps_active<<<grid, kTPB, lds, s>>>(a);
+  return hipGetLastError();
+}
+
+// Fixed point -> double taps, per-(job, bin) nonzero d range, flags, bin list.
+// nsets_out = nsets_in + 1 for the active taps: set 3 = set 0 + set 1 (all
+// active cells, weight limb), used on the SED quiet rows.
+__global__ void k_taps_finalize(const unsigned long long *acc, int nsets_in, int nsets_out, int n_mu, int nd,
+                                double *taps, int32_t *krange, int32_t *kflags, int32_t *bins, int32_t *nbins) {
+  int job = blockIdx.x;
+  __shared__ int s_lo[256], s_hi[256], s_fl[256];
+  for (int k = threadIdx.x; k < n_mu; k += blockDim.x) {
+    int lo = nd, hi = -1, fl = 0;
+    for (int d = 0; d < nd; d++) {
+      double v[4] = {0, 0, 0, 0};
+      for (int st = 0; st < nsets_in; st++) {
+        long e = ((((long)job * nsets_in + st) * n_mu + k) * nd + d) * 2;
+        v[st] = fx_value(acc[e], acc[e + 1]);
+      }
+      if (nsets_out > nsets_in) v[nsets_in] = v[0] + v[1];
+      bool nz = false;
+      for (int st = 0; st < nsets_out; st++) {
+        taps[(((long)job * n_mu + k) * nsets_out + st) * nd + d] = v[st];
+        if (v[st] != 0.0) nz = true;
+      }
+      if (nsets_in == 3) {
+        if (v[0] != 0.0) fl |= 1;
+        if (v[1] != 0.0 || v[2] != 0.0) fl |= 2;
+      } else if (v[0] != 0.0) {
+        fl |= 1;
+      }
+      if (nz) {
+        lo = d < lo ? d : lo;
+        hi = d;
+      }
+    }
+    s_lo[k] = lo;
+    s_hi[k] = hi;
+    s_fl[k] = fl;
+    krange[((long)job * n_mu + k) * 2] = lo;
+    krange[((long)job * n_mu + k) * 2 + 1] = hi;
+    kflags[(long)job * n_mu + k] = fl;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int c = 0;
+    for (int k = 0; k < n_mu; k++)
+      if (s_lo[k] <= s_hi[k]) bins[(long)job * n_mu + c++] = k;
+    nbins[job] = c;
+  }
+}
+
+hipError_t launch_taps_finalize(const unsigned long long *acc, int n_job, int nsets_in, int nsets_out, int n_mu,
+                                int nd, double *taps, int32_t *krange, int32_t *kflags, int32_t *bins,
+                                int32_t *nbins, hipStream_t s) {
+  if (n_mu > 256) return hipErrorInvalidValue;
+  k_taps_finalize<<<n_job, 64, 0, s>>>(acc, nsets_in, nsets_out, n_mu, nd, taps, krange, kflags, bins, nbins);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ a2/a6: FIR
+__device__ __forceinline__ double row_at(const double *row, long i, long n) {
+  return (i >= 0 && i < n) ? row[i] : 0.0;
+}
+
+// Quiet star: Q_j = sum_k sum_d H_k[d] Sq_k[j-d], the SED sum likewise.
+__global__ void k_fir_quiet(FirArgs a) {
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.n) return;
+  double accq = 0.0, accs = 0.0;
+  int nb = a.nbins[0];
+  for (int kk = 0; kk < nb; kk++) {
+    int k = a.bins[kk];
+    int lo = a.krange[k * 2], hi = a.krange[k * 2 + 1];
+    const double *T = a.taps + (long)k * a.nd;
+    const double *rq = a.q + (long)k * a.n;
+    const double *rs = a.qs ? a.qs + (long)k * a.n : nullptr;
+    for (int di = lo; di <= hi; di++) {
+      double hv = T[di];
+      long i = j - (a.dmin + di);
+      accq += hv * row_at(rq, i, a.n);
+      if (rs) accs += hv * row_at(rs, i, a.n);
+    }
+  }
+  a.out[j] = accq;
+  a.out[a.n + j] = accs;
+}
+
+hipError_t launch_fir_quiet(const FirArgs &a, hipStream_t s) {
+  k_fir_quiet<<<(unsigned)((a.n + kTPB - 1) / kTPB), kTPB, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+// Active deltas of one phase per blockIdx.y (spot_integrate's fast_flux /
+// fast_bconv / fast_tot summed over cells, SOAP_integration.cu:285-307,392-447):
+//   flux  = H1a*SEDq - Cs (H1s*Sq) - Cf (H2f*Sq)
+//   bconv = H1s*Sq + H1f*Sq - H1s*Ssp - H1f*Sfac
+//   tot   = H1a*SEDq - Cs (H1s*Ssp) - Cf (H2f*Sfac)
+// with Cs, Cf the unshifted row 0 of the SED spot / facula cubes.
+__global__ void k_fir_active(FirArgs a) {
+  int job = blockIdx.y;
+  long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.n) return;
+  double F1 = 0, F2 = 0, F3 = 0, F4 = 0, F5 = 0, F6 = 0, F7 = 0;
+  int nb = a.nbins[job];
+  const int32_t *bl = a.bins + (long)job * a.n_mu;
+  for (int kk = 0; kk < nb; kk++) {
+    int k = bl[kk];
+    long jk = (long)job * a.n_mu + k;
+    int lo = a.krange[jk * 2], hi = a.krange[jk * 2 + 1];
+    int fl = a.kflags[jk];
+    const double *T = a.taps + jk * 4 * a.nd;
+    const double *rq = a.q + (long)k * a.n, *rs = a.qs + (long)k * a.n;
+    for (int di = lo; di <= hi; di++) {
+      long i = j - (a.dmin + di);
+      if (i < 0 || i >= a.n) continue;
+      double h0 = T[di], h1 = T[a.nd + di], h2 = T[2 * a.nd + di], h3 = T[3 * a.nd + di];
+      double vq = rq[i];
+      F1 += h3 * rs[i];
+      F2 += h0 * vq;
+      F4 += h1 * vq;
+      F3 += h2 * vq;
+      if (fl & 1) F5 += h0 * a.sp[(long)k * a.n + i];
+      if (fl & 2) {
+        double vf = a.fa[(long)k * a.n + i];
+        F6 += h1 * vf;
+        F7 += h2 * vf;
+      }
+    }
+  }
+  double cs = a.sps ? a.sps[j] : 0.0;
+  double cf = a.fas ? a.fas[j] : 0.0;
+  double *o = a.out + (long)job * 3 * a.n;
+  o[j] = F1 - cs * F2 - cf * F3;
+  o[a.n + j] = (F2 + F4) - F5 - F6;
+  o[2 * a.n + j] = F1 - cs * F5 - cf * F7;
+}
+
+hipError_t launch_fir_active(const FirArgs &a, int n_job, hipStream_t s) {
+  if (n_job == 0) return hipSuccess;
+  dim3 grid((unsigned)((a.n + kTPB - 1) / kTPB), n_job);
+  k_fir_active<<<grid, kTPB, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+// Temperature / binary maps of one phase (spot_scan, SOAP_integration.cu:326-390;
+// initial -9999 fill :686-700).
+__global__ void k_phase_maps(int grid, int t_star, int t_diff_spot, int t_diff_fac, const RegionDev *geo, int n_geo,
+                             double *temp, double *bi) {
+#pragma clang fp contract(off)
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long)grid * grid) return;
+  int iy = (int)(idx / grid), iz = (int)(idx % grid);
+  double rc = 0;
+  int ty = classify_cell(iy, iz, 2. / grid, geo, 0, n_geo, &rc);
+  if (ty < 0) {
+    temp[idx] = -9999.0;
+    bi[idx] = -9999.0;
+  } else if (ty == 0) {
+    int T_spot = t_star - t_diff_spot;
+    temp[idx] = T_spot;
+    bi[idx] = 0.0;
+  } else {
+    int T_plage = (int)(t_star + t_diff_fac + 0.9 - 407.7 * rc + 190.9 * (rc * rc));
+    temp[idx] = T_plage;
+    bi[idx] = 1.0;
+  }
+}
+
+hipError_t launch_phase_maps(int grid, int t_star, int t_diff_spot, int t_diff_fac, const RegionDev *geo,
+                             int n_geo, double *temp, double *bi, hipStream_t s) {
+  long ncell = (long)grid * grid;
+  k_phase_maps<<<(unsigned)((ncell + kTPB - 1) / kTPB), kTPB, 0, s>>>(grid, t_star, t_diff_spot, t_diff_fac,
+                                                                     geo, n_geo, temp, bi);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ a7: resolution
+// o_i = sum_d c_d (1 - A_{i+d}) for wid < i < n-1-wid (conv_inst,
+// SOAP_resolution.cu:76-95), with block partial min(A) over every pixel and
+// max(o != 0) seeded at -60000 (reduceMin/MaxIdxOptimizedShared, :123-214).
+__global__ void k_conv(ConvArgs c, const double *quiet, const double *raw, const int32_t *qsel,
+                       const int32_t *variant, double *o, double *part) {
+  __shared__ double s_min[kTPB / 64], s_max[kTPB / 64];
+  int sp = blockIdx.y;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const double *Q = quiet + (long)qsel[sp] * c.n;
+  const double *D = raw + (long)sp * c.n;
+  bool quiet_form = variant[sp] == 0;
+  double amin = HUGE_VAL, omax = -60000.0;
+  if (i < c.n) {
+    double Ai = quiet_form ? Q[i] : Q[i] - D[i];
+    amin = Ai;
+    double acc = 0.0;
+    if ((i > c.wid) && (i + c.wid < c.n - 1)) {
+      for (int d = c.dlo; d <= c.dhi; d++) {
+        long m = i + d;
+        double Am = quiet_form ? Q[m] : Q[m] - D[m];
+        acc += c.ctap[d - c.dlo] * (1.0 - Am);
+      }
+    }
+    o[(long)sp * c.n + i] = acc;
+    if (acc != 0.0 && omax < acc) omax = acc;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    double a = __shfl_xor(amin, off), b = __shfl_xor(omax, off);
+    amin = a < amin ? a : amin;
+    omax = b > omax ? b : omax;
+  }
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_min[wv] = amin;
+    s_max[wv] = omax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a = s_min[0], b = s_max[0];
+    for (int k = 1; k < kTPB / 64; k++) {
+      a = s_min[k] < a ? s_min[k] : a;
+      b = s_max[k] > b ? s_max[k] : b;
+    }
+    part[((long)sp * c.n_tiles + blockIdx.x) * 2] = a;
+    part[((long)sp * c.n_tiles + blockIdx.x) * 2 + 1] = b;
+  }
+}
+
+hipError_t launch_conv(const ConvArgs &c, const double *quiet, const double *raw, int n_spec, const int32_t *qsel,
+                       const int32_t *variant, double *o, double *part, hipStream_t s) {
+  if (n_spec == 0) return hipSuccess;
+  dim3 grid(c.n_tiles, n_spec);
+  k_conv<<<grid, kTPB, 0, s>>>(c, quiet, raw, qsel, variant, o, part);
+  return hipGetLastError();
+}
+
+// spec_norm, SOAP_resolution.cu:225-232: out = 1 - o (1 - min) / max.
+__global__ void k_norm(ConvArgs c, const double *part, double *mm, double *o) {
+  __shared__ double s_min, s_max;
+  int sp = blockIdx.y;
+  if (threadIdx.x == 0) {
+    double a = part[(long)sp * c.n_tiles * 2], b = part[(long)sp * c.n_tiles * 2 + 1];
+    for (int t = 1; t < c.n_tiles; t++) {
+      double pa = part[((long)sp * c.n_tiles + t) * 2], pb = part[((long)sp * c.n_tiles + t) * 2 + 1];
+      a = pa < a ? pa : a;
+      b = pb > b ? pb : b;
+    }
+    s_min = a;
+    s_max = b;
+    if (blockIdx.x == 0) {
+      mm[sp * 2] = a;
+      mm[sp * 2 + 1] = b;
+    }
+  }
+  __syncthreads();
+  double mn = s_min, mx = s_max;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += stride) {
+    double v = o[(long)sp * c.n + i];
+    o[(long)sp * c.n + i] = 1.0 - v * (1 - mn) / mx;
+  }
+}
+
+hipError_t launch_norm(const ConvArgs &c, int n_spec, const double *part, double *mm, double *o, hipStream_t s) {
+  if (n_spec == 0) return hipSuccess;
+  dim3 grid(64, n_spec);
+  k_norm<<<grid, kTPB, 0, s>>>(c, part, mm, o);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ a8: CCF
+// Mask of calc_mask (CCF_func.py:45-91) for every (velocity, line), log grid:
+// index arithmetic through x(l) = ln(l / l_t0)/h on the trimmed grid.
+// Layout [v][line]: start = aa-1, len = bb-aa+2, val = {fb, fa}.
+__global__ void k_mask_build(const double *line_l, int n_lines, const double *vgrid, int n_v, double lam_t0, double h,
+                             int32_t *m_start, int32_t *m_len, double *m_val) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n_v * n_lines) return;
+  int iv = (int)(e / n_lines), li = (int)(e % n_lines);
+  double ls = line_l[li] * (1 + vgrid[iv] / kC);
+  double beta = 820. / kC;
+  double hw = ls * (sqrt((1 + beta) / (1 - beta)) - 1);
+  double bgn = ls - hw / 2., end = ls + hw / 2.;
+  double xb = log(bgn / lam_t0) / h, xe = log(end / lam_t0) / h;
+  double aa = ceil(xb);
+  double bb = ceil(xe) - 1;
+  double fb = -expm1(-(aa - xb) * h) / -expm1(-h);
+  double fa = fabs(expm1((xe - bb) * h)) / expm1(h);
+  m_start[e] = (int32_t)aa - 1;
+  m_len[e] = (int32_t)(bb - aa) + 2;
+  m_val[e * 2] = fb;
+  m_val[e * 2 + 1] = fa;
+}
+
+hipError_t launch_mask_build(const double *line_l, const double *line_w, int n_lines, const double *vgrid, int n_v,
+                             double lam_t0, double h, int64_t n_t, int32_t *m_start, int32_t *m_len, double *m_val,
+                             hipStream_t s) {
+  (void)line_w;
+  (void)n_t;
+  long tot = (long)n_v * n_lines;
+  if (tot == 0) return hipSuccess;
+  k_mask_build<<<(unsigned)((tot + kTPB - 1) / kTPB), kTPB, 0, s>>>(line_l, n_lines, vgrid, n_v, lam_t0, h, m_start,
+                                                                    m_len, m_val);
+  return hipGetLastError();
+}
+
+// Later lines overwrite earlier ones (CCF_func.py:85-89): pixel p of line i is
+// alive unless a line with a larger input index writes p.  Candidates are the
+// neighbours in wavelength order whose pixel spans overlap.
+__global__ void k_mask_overwrite(const int32_t *sorted, const int32_t *rank, int n_lines, int n_v, int scan,
+                                 const int32_t *m_start, const int32_t *m_len, uint32_t *m_alive) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n_v * n_lines) return;
+  int iv = (int)(e / n_lines), li = (int)(e % n_lines);
+  const int32_t *st = m_start + (long)iv * n_lines;
+  const int32_t *ln = m_len + (long)iv * n_lines;
+  int s0 = st[li], l0 = ln[li];
+  uint32_t alive = l0 >= 32 ? 0xffffffffu : ((1u << l0) - 1u);
+  int q = rank[li];
+  for (int dir = -1; dir <= 1; dir += 2) {
+    for (int step = 1; step <= scan; step++) {
+      int qq = q + dir * step;
+      if (qq < 0 || qq >= n_lines) break;
+      int lj = sorted[qq];
+      int s1 = st[lj], l1 = ln[lj];
+      if (s1 + l1 - 1 < s0 || s1 > s0 + l0 - 1) continue;
+      if (lj <= li) continue;
+      for (int b = 0; b < l0; b++) {
+        int p = s0 + b;
+        if (p >= s1 && p <= s1 + l1 - 1) alive &= ~(1u << b);
+      }
+    }
+  }
+  m_alive[e] = alive;
+}
+
+hipError_t launch_mask_overwrite(const int32_t *sorted, const int32_t *rank, int n_lines, int n_v, int scan,
+                                 const int32_t *m_start, const int32_t *m_len, uint32_t *m_alive, hipStream_t s) {
+  long tot = (long)n_v * n_lines;
+  if (tot == 0) return hipSuccess;
+  k_mask_overwrite<<<(unsigned)((tot + kTPB - 1) / kTPB), kTPB, 0, s>>>(sorted, rank, n_lines, n_v, scan, m_start,
+                                                                        m_len, m_alive);
+  return hipGetLastError();
+}
+
+// calculate_CCF_2 (CCF_func.py:27-36) on the sparse mask: one block per
+// (spectrum, velocity), lines over threads, fixed-order block reduction.
+__global__ void k_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_lines, int n_v, int64_t n_t,
+                      const int32_t *m_start, const int32_t *m_len, const uint32_t *m_alive, const double *m_val,
+                      const double *line_w, double *ccf) {
+  __shared__ double red[kTPB];
+  int sp = blockIdx.y, iv = blockIdx.x;
+  const double *S = spec + (long)sp * spec_stride + trim;
+  double acc = 0.0;
+  for (int li = threadIdx.x; li < n_lines; li += blockDim.x) {
+    long e = (long)iv * n_lines + li;
+    int s0 = m_start[e], l0 = m_len[e];
+    uint32_t al = m_alive[e];
+    double w = line_w[li];
+    double wfb = w * m_val[e * 2], wfa = w * m_val[e * 2 + 1];
+    int bb = s0 + l0 - 1;
+    for (int b = 0; b < l0; b++) {
+      long p = s0 + b;
+      if (!((al >> b) & 1u) || p < 0 || p >= n_t) continue;
+      double val = (p == bb) ? wfa : ((b == 0) ? wfb : w);
+      acc += S[p] * val;
+    }
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kTPB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) ccf[(long)sp * n_v + iv] = red[0];
+}
+
+hipError_t launch_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
+                      int64_t n_t, const int32_t *m_start, const int32_t *m_len, const uint32_t *m_alive,
+                      const double *m_val, const double *line_w, double *ccf, hipStream_t s) {
+  if (n_spec == 0) return hipSuccess;
+  dim3 grid(n_v, n_spec);
+  k_ccf<<<grid, kTPB, 0, s>>>(spec, spec_stride, trim, n_lines, n_v, n_t, m_start, m_len, m_alive, m_val, line_w,
+                              ccf);
+  return hipGetLastError();
+}
+
+// CCF /= max(CCF); CCF = 1 - ((-CCF + 1) * appod), run_all_SOAP.py:368-370.
+__global__ void k_ccf_finish(double *ccf, int n_v, const double *appod) {
+  int sp = blockIdx.x;
+  double *c = ccf + (long)sp * n_v;
+  double mx = c[0];
+  for (int i = 1; i < n_v; i++) mx = c[i] > mx ? c[i] : mx;
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_v; i += blockDim.x) {
+    double cc = c[i] / mx;
+    c[i] = 1 - ((-cc + 1) * appod[i]);
+  }
+}
+
+hipError_t launch_ccf_finish(double *ccf, int n_spec, int n_v, const double *appod, hipStream_t s) {
+  if (n_spec == 0) return hipSuccess;
+  k_ccf_finish<<<n_spec, 64, 0, s>>>(ccf, n_v, appod);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ a9: fit
+constexpr int kMaxV = 1024;
+
+__device__ __forceinline__ double wave_sum(double x) {
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
+struct GaussEval {
+  double sum_f2;
+  double A[10];  // upper triangle of J^T J: 00 01 02 03 11 12 13 22 23 33
+  double g[4];
+  double colsq[4];
+};
+
+// gauss_f / gauss_df of BIS_FIT2.cpp:31-81 (err = 1), lane-strided over points;
+// with_jac = false evaluates only sum f^2.
+__device__ void gauss_eval(const double *px, const double *y, int n, const double *x, bool with_jac, GaussEval &r) {
+  double c = x[0], k = x[1], x0 = x[2], fwhm = x[3];
+  double sigma = fwhm / 2 / sqrt(2 * log(2.0));
+  double f2 = 0, A[10] = {0}, g[4] = {0};
+  for (int i = threadIdx.x; i < n; i += 64) {
+    double e = exp(-(px[i] - x0) * (px[i] - x0) / 2 / sigma / sigma);
+    double fi = (c + k * e) - y[i];
+    f2 += fi * fi;
+    if (with_jac) {
+      double J[4] = {1.0, e, k * e * (px[i] - x0) / sigma / sigma,
+                     k * e * (px[i] - x0) * (px[i] - x0) / sigma / sigma / sigma / 2 / sqrt(2 * log(2.0))};
+      int q = 0;
+      for (int a = 0; a < 4; a++) {
+        g[a] += J[a] * fi;
+        for (int b = a; b < 4; b++) A[q++] += J[a] * J[b];
+      }
+    }
+  }
+  r.sum_f2 = wave_sum(f2);
+  if (with_jac) {
+    for (int q = 0; q < 10; q++) r.A[q] = wave_sum(A[q]);
+    for (int a = 0; a < 4; a++) r.g[a] = wave_sum(g[a]);
+    r.colsq[0] = r.A[0];
+    r.colsq[1] = r.A[4];
+    r.colsq[2] = r.A[7];
+    r.colsq[3] = r.A[9];
+  }
+}
+
+__device__ __forceinline__ double symA(const double *A, int a, int b) {
+  if (a > b) {
+    int t = a;
+    a = b;
+    b = t;
+  }
+  const int idx[4][4] = {{0, 1, 2, 3}, {1, 4, 5, 6}, {2, 5, 7, 8}, {3, 6, 8, 9}};
+  return A[idx[a][b]];
+}
+
+__device__ int solve4(const double *A, const double *D, double mu, const double *g, double *dx) {
+  double L[4][4] = {{0}};
+  for (int i = 0; i < 4; i++) {
+    for (int j = 0; j <= i; j++) {
+      double s = symA(A, i, j) + (i == j ? mu * D[i] * D[i] : 0.0);
+      for (int k = 0; k < j; k++) s -= L[i][k] * L[j][k];
+      if (i == j) {
+        if (!(s > 0)) return -1;
+        L[i][i] = sqrt(s);
+      } else {
+        L[i][j] = s / L[j][j];
+      }
+    }
+  }
+  double z[4];
+  for (int i = 0; i < 4; i++) {
+    double s = -g[i];
+    for (int k = 0; k < i; k++) s -= L[i][k] * z[k];
+    z[i] = s / L[i][i];
+  }
+  for (int i = 3; i >= 0; i--) {
+    double s = z[i];
+    for (int k = i + 1; k < 4; k++) s -= L[k][i] * dx[k];
+    dx[i] = s / L[i][i];
+  }
+  return 0;
+}
+
+// BIS_FIT2 (BIS_FIT2.cpp:97-320) for one CCF per 64-lane block: the same
+// Levenberg-Marquardt restatement of GSL's trust-region solver as the oracle
+// (More scaling, Nielsen damping, xtol = gtol = 1e-8), then the bisector.
+__global__ __launch_bounds__(64) void k_fit(const double *v, int n, const double *Y, double *par, double *model,
+                                            double *depth_out, double *bis_out, int32_t *iters) {
+  __shared__ double s_px[kMaxV], s_y[kMaxV], s_nc[kMaxV], s_p0[kMaxV], s_p1[kMaxV], s_p2[kMaxV];
+  __shared__ double s_bis[91];
+  __shared__ double s_init[4];
+  int cidx = blockIdx.x;
+  const double *y = Y + (long)cidx * n;
+  for (int i = threadIdx.x; i < n; i += 64) {
+    s_px[i] = v[i] - v[0];
+    s_y[i] = y[i];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double c = (s_y[0] + s_y[n - 1]) / 2;
+    double fwhm = (s_px[n - 1] - s_px[0]) / 5.;
+    double k = 0., x0 = s_px[n / 2];
+    for (int i = 0; i < n; i++)
+      if (fabs(s_y[i] - c) > fabs(k)) {
+        k = s_y[i] - c;
+        x0 = s_px[i];
+      }
+    s_init[0] = c;
+    s_init[1] = k;
+    s_init[2] = x0;
+    s_init[3] = fwhm;
+  }
+  __syncthreads();
+  double x[4] = {s_init[0], s_init[1], s_init[2], s_init[3]};
+  const double xtol = 1.0e-8, gtol = 1.0e-8;
+  GaussEval ev;
+  gauss_eval(s_px, s_y, n, x, true, ev);
+  double D[4];
+  for (int a = 0; a < 4; a++) D[a] = sqrt(ev.colsq[a]) > 0 ? sqrt(ev.colsq[a]) : 1.0;
+  double mu = 1.0e-3, nu = 2.0;
+  int iter = 0;
+  for (iter = 0; iter < 10000; iter++) {
+    double dx[4], xt[4];
+    int bad = 0;
+    bool accepted = false;
+    while (!accepted) {
+      if (solve4(ev.A, D, mu, ev.g, dx) != 0) {
+        mu *= nu;
+        nu *= 2;
+        if (++bad > 15) break;
+        continue;
+      }
+      for (int a = 0; a < 4; a++) xt[a] = x[a] + dx[a];
+      GaussEval et;
+      gauss_eval(s_px, s_y, n, xt, false, et);
+      double pred = 0;
+      for (int a = 0; a < 4; a++) {
+        double Adx = 0;
+        for (int b = 0; b < 4; b++) Adx += symA(ev.A, a, b) * dx[b];
+        pred += dx[a] * Adx + 2 * mu * D[a] * D[a] * dx[a] * dx[a];
+      }
+      double rho = pred > 0 ? (ev.sum_f2 - et.sum_f2) / pred : -1.0;
+      if (rho > 0) {
+        accepted = true;
+        double tt = 2 * rho - 1;
+        double fac = 1 - tt * tt * tt;
+        mu *= fac > 1. / 3. ? fac : 1. / 3.;
+        nu = 2.0;
+      } else {
+        mu *= nu;
+        nu *= 2;
+        if (++bad > 15) break;
+      }
+    }
+    if (!accepted) break;
+    for (int a = 0; a < 4; a++) x[a] = xt[a];
+    gauss_eval(s_px, s_y, n, x, true, ev);
+    for (int a = 0; a < 4; a++)
+      if (sqrt(ev.colsq[a]) > D[a]) D[a] = sqrt(ev.colsq[a]);
+    bool small = true;
+    for (int a = 0; a < 4; a++)
+      if (!(fabs(dx[a]) < xtol * xtol + xtol * fabs(x[a]))) small = false;
+    if (small) {
+      iter++;
+      break;
+    }
+    double gn = 0;
+    for (int a = 0; a < 4; a++) {
+      double sc = fabs(x[a]) > 1.0 ? fabs(x[a]) : 1.0;
+      if (fabs(ev.g[a] * sc) > gn) gn = fabs(ev.g[a] * sc);
+    }
+    double phi = 0.5 * ev.sum_f2;
+    if (gn <= gtol * (phi > 1.0 ? phi : 1.0)) {
+      iter++;
+      break;
+    }
+  }
+  double c = x[0], k = x[1], x0 = x[2], fwhm = x[3];
+  double sigma_m = fwhm / 2 / sqrt(2 * log(2.0));
+  if (model)
+    for (int i = threadIdx.x; i < n; i += 64)
+      model[(long)cidx * n + i] = c + k * exp(-(s_px[i] - x0) * (s_px[i] - x0) / 2 / sigma_m / sigma_m);
+  // bisector, BIS_FIT2.cpp:214-268
+  double sigma = fwhm / 2. / sqrt(2. * log(2.));
+  double v0 = x0 + v[0];
+  for (int i = threadIdx.x; i < n; i += 64) s_nc[i] = -c / k * (1. - s_y[i] / c);
+  __syncthreads();
+  for (int i = threadIdx.x; i < n - 1; i += 64) {
+    double vr = (v[i] + v[i + 1]) / 2.;
+    double s2 = sigma * sigma;
+    double dd = (vr - v0) * (vr - v0);
+    double dccf = -(vr - v0) / s2 * exp(-dd / 2. / s2);
+    double d2ccf = (dd / s2 - 1) / s2 * exp(-dd / 2. / s2);
+    double d2rv = -d2ccf / (dccf * dccf * dccf);
+    double p2 = d2rv / 2.;
+    double p1 = (v[i + 1] - v[i] - p2 * (s_nc[i + 1] * s_nc[i + 1] - s_nc[i] * s_nc[i])) / (s_nc[i + 1] - s_nc[i]);
+    s_p2[i] = p2;
+    s_p1[i] = p1;
+    s_p0[i] = v[i] - p1 * s_nc[i] - p2 * (s_nc[i] * s_nc[i]);
+  }
+  if (threadIdx.x == 0) {
+    s_p0[n - 1] = 0;
+    s_p1[n - 1] = 0;
+    s_p2[n - 1] = 0;
+  }
+  __syncthreads();
+  __shared__ int s_imax;
+  if (threadIdx.x == 0) {
+    int im = 0;
+    for (int i = 0; i < n; i++)
+      if (s_nc[i] > s_nc[im]) im = i;
+    s_imax = im;
+  }
+  __syncthreads();
+  int ind_max = s_imax;
+  for (int i = threadIdx.x; i < 91; i += 64) {
+    double dep = (double)(i + 5) / 100;
+    int i_b = ind_max, i_r = ind_max;
+    while ((s_nc[i_b] > dep) && (i_b > 1)) i_b--;
+    while ((s_nc[i_r + 1] > dep) && (i_r < (n - 2))) i_r++;
+    double b = (s_p0[i_b] + s_p0[i_r]) + (s_p1[i_b] + s_p1[i_r]) * dep + (s_p2[i_b] + s_p2[i_r]) * (dep * dep);
+    b /= 2.;
+    s_bis[i] = b;
+    if (bis_out) bis_out[(long)cidx * 91 + i] = b;
+    if (depth_out) depth_out[(long)cidx * 91 + i] = 1.0 - fabs(k) * dep;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int n1 = 0, n2 = 0;
+    double top = 0., bot = 0.;
+    for (int i = 0; i < 91; i++) {
+      double dep = (double)(i + 5) / 100;
+      if ((dep >= 0.1) && (dep <= 0.4)) {
+        n1++;
+        top += s_bis[i];
+      }
+      if ((dep >= 0.6) && (dep <= 0.9)) {
+        n2++;
+        bot += s_bis[i];
+      }
+    }
+    top /= n1;
+    bot /= n2;
+    double *pp = par + (long)cidx * 5;
+    pp[0] = c;
+    pp[1] = fabs(k);
+    pp[2] = top - bot;
+    pp[3] = v0;
+    pp[4] = fwhm;
+    if (iters) iters[cidx] = iter;
+  }
+}
+
+hipError_t launch_fit(const double *v, int n_v, const double *y, int n_ccf, double *par, double *model, double *depth,
+                      double *bis, int32_t *iters, hipStream_t s) {
+  if (n_ccf == 0) return hipSuccess;
+  if (n_v > kMaxV || n_v < 3) return hipErrorInvalidValue;
+  k_fit<<<n_ccf, 64, 0, s>>>(v, n_v, y, par, model, depth, bis, iters);
+  return hipGetLastError();
+}
+
+}  // namespace soap

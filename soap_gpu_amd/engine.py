@@ -1,0 +1,355 @@
+"""ctypes binding of libsoapgpu (include/soap_gpu.h).
+
+The engine is the HIP library; this module only marshals numpy arrays across
+the C ABI.  Loading fails loudly when the library is missing: there is no CPU
+fallback for any stage.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsoapgpu.so")
+
+GRID_LINEAR, GRID_LOG = 0, 1
+CUBE_QUIET, CUBE_QUIET_SED, CUBE_SPOT, CUBE_SPOT_SED, CUBE_FAC, CUBE_FAC_SED = range(6)
+FLUX, BCONV, TOT = 0, 1, 2
+CUBE_KEYS = ("q", "qsed", "sp", "spsed", "fac", "facsed")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+
+
+class Star(ctypes.Structure):
+    _fields_ = [("prot_days", ctypes.c_double), ("w_ratio", ctypes.c_double), ("incl_deg", ctypes.c_double),
+                ("radius_km", ctypes.c_double), ("u1", ctypes.c_double), ("u2", ctypes.c_double),
+                ("lb", ctypes.c_double * 4), ("grid", ctypes.c_int32)]
+
+
+class Activity(ctypes.Structure):
+    _fields_ = [("nrho", ctypes.c_int32), ("t_star", ctypes.c_int32), ("t_diff_spot", ctypes.c_int32),
+                ("t_diff_fac", ctypes.c_int32), ("solar_mode", ctypes.c_int32)]
+
+
+class Region(ctypes.Structure):
+    _fields_ = [("lat_deg", ctypes.c_double), ("lon_deg", ctypes.c_double), ("size", ctypes.c_double),
+                ("type", ctypes.c_double)]
+
+
+# every symbol declared in include/soap_gpu.h
+EXPORTS = (
+    "soap_ctx_create", "soap_ctx_destroy", "soap_last_error", "soap_sync", "soap_version",
+    "soap_set_wave", "soap_set_mu_bounds", "soap_set_cube", "soap_set_cube_device",
+    "soap_disk_init", "soap_get_maps", "soap_set_maps",
+    "soap_quiet_star", "soap_get_quiet", "soap_set_quiet",
+    "soap_set_activity", "soap_integrate", "soap_get_raw", "soap_set_raw", "soap_phase_maps",
+    "soap_resolution", "soap_get_lower", "soap_get_norm",
+    "soap_set_mask", "soap_ccf", "soap_get_ccf",
+    "soap_fit", "soap_get_fit", "soap_fit_ccf_host",
+    "soap_run_series",
+    "soap_set_timing", "soap_get_stage_ms", "soap_stage_name", "soap_num_stages",
+    "soap_get_integrate_stats",
+)
+
+_lib = None
+
+
+class SoapError(RuntimeError):
+    pass
+
+
+def load(path: str = LIB_PATH):
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SoapError(f"libsoapgpu.so not built ({path}); run python -m soap_gpu_amd.build")
+    L = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    sig = {
+        "soap_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(vp)]),
+        "soap_ctx_destroy": (None, [vp]),
+        "soap_last_error": (ctypes.c_char_p, [vp]),
+        "soap_sync": (ctypes.c_int, [vp]),
+        "soap_version": (ctypes.c_char_p, []),
+        "soap_set_wave": (ctypes.c_int, [vp, ctypes.c_int, _dp, ctypes.c_int64, ctypes.c_double]),
+        "soap_set_mu_bounds": (ctypes.c_int, [vp, _dp, ctypes.c_int32]),
+        "soap_set_cube": (ctypes.c_int, [vp, ctypes.c_int, _dp, ctypes.c_int32]),
+        "soap_set_cube_device": (ctypes.c_int, [vp, ctypes.c_int, vp, ctypes.c_int32]),
+        "soap_disk_init": (ctypes.c_int, [vp, ctypes.POINTER(Star)]),
+        "soap_get_maps": (ctypes.c_int, [vp, _dp, _dp, _dp, _ip, _dp]),
+        "soap_set_maps": (ctypes.c_int, [vp, ctypes.POINTER(Star), _dp, _dp, _ip, _dp]),
+        "soap_quiet_star": (ctypes.c_int, [vp]),
+        "soap_get_quiet": (ctypes.c_int, [vp, ctypes.c_int, _dp]),
+        "soap_set_quiet": (ctypes.c_int, [vp, ctypes.c_int, _dp]),
+        "soap_set_activity": (ctypes.c_int, [vp, ctypes.POINTER(Activity), ctypes.c_int32, _dp, _ip,
+                                             ctypes.POINTER(Region)]),
+        "soap_integrate": (ctypes.c_int, [vp]),
+        "soap_get_raw": (ctypes.c_int, [vp, ctypes.c_int32, _dp, _dp, _dp]),
+        "soap_set_raw": (ctypes.c_int, [vp, ctypes.c_int32, _dp, _dp, _dp]),
+        "soap_phase_maps": (ctypes.c_int, [vp, ctypes.c_int32, _dp, _dp]),
+        "soap_resolution": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double]),
+        "soap_get_lower": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int, _dp]),
+        "soap_get_norm": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int, _dp]),
+        "soap_set_mask": (ctypes.c_int, [vp, _dp, _dp, ctypes.c_int32, _dp, ctypes.c_int32, ctypes.c_int32]),
+        "soap_ccf": (ctypes.c_int, [vp]),
+        "soap_get_ccf": (ctypes.c_int, [vp, _dp]),
+        "soap_fit": (ctypes.c_int, [vp]),
+        "soap_get_fit": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp]),
+        "soap_fit_ccf_host": (ctypes.c_int, [vp, _dp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, _dp, _dp, _dp]),
+        "soap_run_series": (ctypes.c_int, [vp, ctypes.POINTER(Star), ctypes.c_double, ctypes.c_double]),
+        "soap_set_timing": (ctypes.c_int, [vp, ctypes.c_int]),
+        "soap_get_stage_ms": (ctypes.c_int, [vp, _dp, ctypes.c_int32]),
+        "soap_stage_name": (ctypes.c_char_p, [ctypes.c_int32]),
+        "soap_num_stages": (ctypes.c_int32, []),
+        "soap_get_integrate_stats": (ctypes.c_int, [vp, _dp, ctypes.POINTER(ctypes.c_int64)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _d(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a, a.ctypes.data_as(_dp)
+
+
+def _out(n, dtype=np.float64):
+    a = np.empty(n, dtype=dtype)
+    return a, a.ctypes.data_as(_dp if dtype == np.float64 else _ip)
+
+
+def make_star(star: dict) -> Star:
+    s = Star()
+    s.prot_days = float(star["prot"])
+    s.w_ratio = float(star.get("w_ratio", 0.0))
+    s.incl_deg = float(star["incl"])
+    s.radius_km = float(star["radius_km"])
+    s.u1 = float(star["u1"])
+    s.u2 = float(star["u2"])
+    for i in range(4):
+        s.lb[i] = float(star["lb"][i])
+    s.grid = int(star["grid"])
+    return s
+
+
+class Engine:
+    """One soap_ctx (one HIP stream on one device)."""
+
+    def __init__(self, device: int = 0):
+        self.L = load()
+        self.ctx = ctypes.c_void_p()
+        rc = self.L.soap_ctx_create(int(device), ctypes.byref(self.ctx))
+        if rc != 0:
+            raise SoapError(f"soap_ctx_create failed ({rc}): no usable HIP device")
+        self.n = 0
+        self.grid = 0
+        self.n_phase = 0
+        self.n_v = 0
+
+    def close(self):
+        if self.ctx:
+            self.L.soap_ctx_destroy(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.soap_last_error(self.ctx)
+            raise SoapError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    # ---------------------------------------------------------------- inputs
+    def set_wave(self, wave, kind=GRID_LOG, step=0.0):
+        w, wp = _d(wave)
+        self._chk(self.L.soap_set_wave(self.ctx, kind, wp, len(w), float(step)), "set_wave")
+        self.n = len(w)
+
+    def set_mu_bounds(self, bounds):
+        b, bp = _d(bounds)
+        self._chk(self.L.soap_set_mu_bounds(self.ctx, bp, len(b)), "set_mu_bounds")
+
+    def set_cube(self, which, cube):
+        c, cp = _d(cube)
+        self._chk(self.L.soap_set_cube(self.ctx, which, cp, c.shape[0]), "set_cube")
+
+    def set_cubes(self, cubes: dict):
+        for i, k in enumerate(CUBE_KEYS):
+            if k in cubes and cubes[k] is not None:
+                self.set_cube(i, cubes[k])
+
+    # ---------------------------------------------------------------- stages
+    def disk_init(self, star: dict):
+        self._star = make_star(star)
+        self._chk(self.L.soap_disk_init(self.ctx, ctypes.byref(self._star)), "disk_init")
+        self.grid = int(star["grid"])
+
+    def get_maps(self):
+        g2 = self.grid * self.grid
+        mu, mup = _out(g2)
+        de, dep = _out(g2)
+        li, lip = _out(g2)
+        rg = np.empty(g2, dtype=np.int32)
+        lb, lbp = _out(g2)
+        self._chk(self.L.soap_get_maps(self.ctx, mup, dep, lip, rg.ctypes.data_as(_ip), lbp), "get_maps")
+        return dict(mu=mu, delta=de, limb=li, region=rg, lb=lb)
+
+    def set_maps(self, star: dict, maps: dict):
+        self._star = make_star(star)
+        de, dep = _d(maps["delta"])
+        li, lip = _d(maps["limb"])
+        rg = np.ascontiguousarray(maps["region"], dtype=np.int32)
+        lb, lbp = _d(maps["lb"])
+        self._chk(self.L.soap_set_maps(self.ctx, ctypes.byref(self._star), dep, lip, rg.ctypes.data_as(_ip), lbp),
+                  "set_maps")
+        self.grid = int(star["grid"])
+
+    def quiet_star(self):
+        self._chk(self.L.soap_quiet_star(self.ctx), "quiet_star")
+
+    def get_quiet(self, which):
+        o, op = _out(self.n)
+        self._chk(self.L.soap_get_quiet(self.ctx, which, op), "get_quiet")
+        return o
+
+    def set_quiet(self, which, q):
+        a, ap = _d(q)
+        self._chk(self.L.soap_set_quiet(self.ctx, which, ap), "set_quiet")
+
+    def set_activity(self, act: dict, phases, regions_per_phase, solar_mode=0):
+        A = Activity(int(act["nrho"]), int(act["t_star"]), int(act["t_diff_spot"]), int(act["t_diff_fac"]),
+                     int(solar_mode))
+        ph, php = _d(phases)
+        counts = np.array([len(r) for r in regions_per_phase], dtype=np.int32)
+        flat = [r for rs in regions_per_phase for r in rs]
+        R = (Region * max(1, len(flat)))()
+        for i, r in enumerate(flat):
+            R[i].lat_deg, R[i].lon_deg, R[i].size, R[i].type = [float(x) for x in r]
+        self._chk(self.L.soap_set_activity(self.ctx, ctypes.byref(A), len(ph), php, counts.ctypes.data_as(_ip), R),
+                  "set_activity")
+        self.n_phase = len(ph)
+
+    def integrate(self):
+        self._chk(self.L.soap_integrate(self.ctx), "integrate")
+
+    def get_raw(self, phase):
+        f, fp = _out(self.n)
+        b, bp = _out(self.n)
+        t, tp = _out(self.n)
+        self._chk(self.L.soap_get_raw(self.ctx, int(phase), fp, bp, tp), "get_raw")
+        return f, b, t
+
+    def set_raw(self, phase, flux, bconv, tot):
+        f, fp = _d(flux)
+        b, bp = _d(bconv)
+        t, tp = _d(tot)
+        self._chk(self.L.soap_set_raw(self.ctx, int(phase), fp, bp, tp), "set_raw")
+
+    def phase_maps(self, phase):
+        g2 = self.grid * self.grid
+        t, tp = _out(g2)
+        b, bp = _out(g2)
+        self._chk(self.L.soap_phase_maps(self.ctx, int(phase), tp, bp), "phase_maps")
+        return t, b
+
+    def resolution(self, sigma_ms, quiet_weight=0.005):
+        self._chk(self.L.soap_resolution(self.ctx, float(sigma_ms), float(quiet_weight)), "resolution")
+
+    def get_lower(self, phase, typ):
+        o, op = _out(self.n)
+        self._chk(self.L.soap_get_lower(self.ctx, int(phase), int(typ), op), "get_lower")
+        return o
+
+    def get_norm(self, phase, typ):
+        o, op = _out(2)
+        self._chk(self.L.soap_get_norm(self.ctx, int(phase), int(typ), op), "get_norm")
+        return o
+
+    def set_mask(self, line_l, line_w, vgrid, trim=210):
+        l, lp = _d(line_l)
+        w, wp = _d(line_w)
+        v, vp = _d(vgrid)
+        self._chk(self.L.soap_set_mask(self.ctx, lp, wp, len(l), vp, len(v), int(trim)), "set_mask")
+        self.n_v = len(v)
+
+    def ccf(self):
+        self._chk(self.L.soap_ccf(self.ctx), "ccf")
+
+    def get_ccf(self):
+        o, op = _out((self.n_phase + 1) * 3 * self.n_v)
+        self._chk(self.L.soap_get_ccf(self.ctx, op), "get_ccf")
+        return o.reshape(self.n_phase + 1, 3, self.n_v)
+
+    def fit(self):
+        self._chk(self.L.soap_fit(self.ctx), "fit")
+
+    def get_fit(self):
+        n_ccf = (self.n_phase + 1) * 3
+        par, pp = _out(n_ccf * 5)
+        mod, mp = _out(n_ccf * self.n_v)
+        dep, dp = _out(n_ccf * 91)
+        bis, bp = _out(n_ccf * 91)
+        self._chk(self.L.soap_get_fit(self.ctx, pp, mp, dp, bp), "get_fit")
+        sh = (self.n_phase + 1, 3)
+        return dict(par=par.reshape(*sh, 5), model=mod.reshape(*sh, self.n_v), depth=dep.reshape(*sh, 91),
+                    bis=bis.reshape(*sh, 91))
+
+    def fit_ccf_host(self, v, y):
+        vv, vp = _d(v)
+        yy = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
+        n_ccf = yy.shape[0]
+        par, pp = _out(n_ccf * 5)
+        mod, mp = _out(n_ccf * len(vv))
+        dep, dp = _out(n_ccf * 91)
+        bis, bp = _out(n_ccf * 91)
+        self._chk(self.L.soap_fit_ccf_host(self.ctx, vp, len(vv), yy.ctypes.data_as(_dp), n_ccf, pp, mp, dp, bp),
+                  "fit_ccf_host")
+        return dict(par=par.reshape(n_ccf, 5), model=mod.reshape(n_ccf, -1), depth=dep.reshape(n_ccf, 91),
+                    bis=bis.reshape(n_ccf, 91))
+
+    def run_series(self, star: dict, sigma_ms, quiet_weight=0.005):
+        self._star = make_star(star)
+        self._chk(self.L.soap_run_series(self.ctx, ctypes.byref(self._star), float(sigma_ms), float(quiet_weight)),
+                  "run_series")
+
+    def sync(self):
+        self._chk(self.L.soap_sync(self.ctx), "sync")
+
+    # ---------------------------------------------------------------- timing
+    def set_timing(self, on: bool):
+        self._chk(self.L.soap_set_timing(self.ctx, 1 if on else 0), "set_timing")
+
+    def stage_ms(self) -> dict:
+        ns = self.L.soap_num_stages()
+        o, op = _out(ns)
+        self._chk(self.L.soap_get_stage_ms(self.ctx, op, ns), "get_stage_ms")
+        return {self.L.soap_stage_name(i).decode(): float(o[i]) for i in range(ns)}
+
+    def integrate_stats(self):
+        b = ctypes.c_double()
+        r = ctypes.c_int64()
+        self._chk(self.L.soap_get_integrate_stats(self.ctx, ctypes.byref(b), ctypes.byref(r)), "integrate_stats")
+        return float(b.value), int(r.value)
+
+
+def setup_workload(eng: Engine, w, with_mask: bool = True, trim: int = 210):
+    """Upload a synth.Workload: grid, mu bounds, cubes, activity, mask."""
+    eng.set_wave(w.wave, GRID_LOG)
+    eng.set_mu_bounds(w.mu_bounds)
+    eng.set_cubes(w.cubes)
+    eng.disk_init(w.star)
+    eng.set_activity(w.act, w.phases, w.regions)
+    if with_mask:
+        wave_t = w.wave[trim:len(w.wave) - trim]
+        sel = (w.mask_l > wave_t.min()) & (w.mask_l < wave_t.max()) & (w.mask_w > 0.1)
+        eng.set_mask(w.mask_l[sel], w.mask_w[sel], w.vgrid, trim)

@@ -1,0 +1,220 @@
+"""Seeded synthetic inputs for the SOAP-GPU workloads.
+
+The reference's own seed spectra (PHOENIX/RASSINE products and the solar
+wavelength grid) are not part of its repository, so every workload here is
+generated from a seed:
+
+* wavelength grid: log-uniform, lambda_j = lambda_min * exp(j h);
+* absorption lines: centres U(lambda_min, lambda_max), depth U(0.05, 0.9),
+  Gaussian sigma U(3, 8) km/s; a seed row is prod_l (1 - d_l G_l(lambda));
+* mu bins equal in mu; bin k (centre mu_k) scales depths by 1 + 0.05 (2 mu_k - 1)
+  and shifts quiet lines by a convective blueshift of -300 m/s * mu_k;
+* spot rows: depths x1.1, no blueshift; SED continuum x0.3;
+* facula rows: quiet depths, no blueshift (suppressed); SED continuum x1.1;
+* SED cubes are the normalised rows times a Planck continuum of T_star
+  normalised at 5500 A.
+
+Cube layout is [n_mu][n] float64 row-major, the layout of the reference's
+seed-cube .bin files (SOAP_integration.cu:504-518).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+C_MS = 299792458.0
+R_SUN_KM = 696000.0
+# LB_params(5778) of run_all_SOAP.py:21-34,140-145 evaluated on the reference's
+# Limb_brightening_info.npz (constant term first).
+LB_5778 = (0.23161761, -0.11674743, -0.36866954, 0.25537337)
+
+
+@dataclass
+class Workload:
+    name: str
+    wave: np.ndarray
+    grid_kind: int  # 1 = log-uniform
+    mu_bounds: np.ndarray
+    cubes: dict
+    lines: dict
+    mask_l: np.ndarray
+    mask_w: np.ndarray
+    star: dict
+    act: dict
+    phases: np.ndarray
+    regions: list  # per phase: list of (lat, lon, size, type)
+    sigma_inst: float
+    vgrid: np.ndarray
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.wave.shape[0])
+
+    @property
+    def n_mu(self) -> int:
+        return int(self.mu_bounds.shape[0] + 1)
+
+
+def log_grid(lmin: float, lmax: float, n: int | None = None, dv_kms: float | None = None) -> np.ndarray:
+    if n is None:
+        h = dv_kms * 1000.0 / C_MS
+        n = int(np.ceil(np.log(lmax / lmin) / h)) + 1
+    else:
+        h = np.log(lmax / lmin) / (n - 1)
+    return lmin * np.exp(np.arange(n, dtype=np.float64) * h)
+
+
+def mu_bins(n_mu: int):
+    bounds = 1.0 - np.arange(1, n_mu, dtype=np.float64) / n_mu  # descending, n_mu-1 values
+    centres = 1.0 - (np.arange(n_mu, dtype=np.float64) + 0.5) / n_mu
+    return bounds, centres
+
+
+def draw_lines(rng: np.random.Generator, n_lines: int, lmin: float, lmax: float) -> dict:
+    return dict(
+        center=rng.uniform(lmin, lmax, n_lines),
+        depth=rng.uniform(0.05, 0.9, n_lines),
+        sigma_kms=rng.uniform(3.0, 8.0, n_lines),
+    )
+
+
+def _planck(lam_a: np.ndarray, t: float) -> np.ndarray:
+    h, c, kb = 6.62607015e-34, C_MS, 1.380649e-23
+    lam = lam_a * 1e-10
+    return 2 * h * c * c / lam ** 5 / np.expm1(h * c / (lam * kb * t))
+
+
+def sed_continuum(wave: np.ndarray, t_star: float = 5778.0) -> np.ndarray:
+    return _planck(wave, t_star) / _planck(np.array([5500.0]), t_star)[0]
+
+
+def line_row(wave: np.ndarray, lines: dict, depth_scale: float, shift_ms: float,
+             nsig: float = 6.0) -> np.ndarray:
+    """prod_l (1 - min(d_l*scale, 0.98) exp(-(lambda - lambda_l')^2 / 2 sigma_l^2)) on a log grid."""
+    n = wave.shape[0]
+    h = np.log(wave[-1] / wave[0]) / (n - 1)
+    lc = lines["center"] * (1.0 + shift_ms / C_MS)
+    sig_px = (lines["sigma_kms"] * 1000.0 / C_MS) / h
+    xc = np.log(lc / wave[0]) / h
+    half = int(np.ceil(nsig * sig_px.max())) + 1
+    offs = np.arange(-half, half + 1)
+    idx = np.floor(xc).astype(np.int64)[:, None] + offs[None, :]
+    ok = (idx >= 0) & (idx < n)
+    idxc = np.clip(idx, 0, n - 1)
+    lam = wave[idxc]
+    sig_l = lc * lines["sigma_kms"] * 1000.0 / C_MS
+    g = np.exp(-0.5 * ((lam - lc[:, None]) / sig_l[:, None]) ** 2)
+    d = np.minimum(lines["depth"] * depth_scale, 0.98)[:, None]
+    tau = np.where(ok, np.log1p(-d * g), 0.0)
+    acc = np.bincount(idxc.ravel(), weights=tau.ravel(), minlength=n)
+    return np.exp(acc)
+
+
+def make_cubes(wave: np.ndarray, lines: dict, n_mu: int, t_star: float = 5778.0,
+               spot_contrast: float = 0.3, fac_contrast: float = 1.1, cb_ms: float = -300.0) -> dict:
+    _, mu_c = mu_bins(n_mu)
+    cont = sed_continuum(wave, t_star)
+    q = np.empty((n_mu, wave.shape[0]))
+    sp = np.empty_like(q)
+    fac = np.empty_like(q)
+    for k, mu in enumerate(mu_c):
+        a = 1.0 + 0.05 * (2.0 * mu - 1.0)
+        q[k] = line_row(wave, lines, a, cb_ms * mu)
+        sp[k] = line_row(wave, lines, 1.1 * a, 0.0)
+        fac[k] = line_row(wave, lines, a, 0.0)
+    return dict(q=q, qsed=q * cont, sp=sp, spsed=sp * (spot_contrast * cont),
+                fac=fac, facsed=fac * (fac_contrast * cont))
+
+
+def prot_for_vsini(vsini_kms: float, radius_km: float = R_SUN_KM, incl_deg: float = 90.0) -> float:
+    """Rotation period [days] giving vsini with the reference's velocity formula
+    (SOAP_initialize.cu:348-351: v = (360/P) pi/(180*86400) R sin i)."""
+    return 2.0 * np.pi * radius_km * np.sin(np.deg2rad(incl_deg)) / (vsini_kms * 86400.0)
+
+
+def sigma_inst_ms(resolution: float) -> float:
+    """run_all_SOAP.py:63-65"""
+    fwhm = C_MS / float(resolution)
+    return fwhm / (2 * np.sqrt(2 * np.log(2)))
+
+
+def vel_grid(window_ms: float, step_ms: float) -> np.ndarray:
+    """run_all_SOAP.py:262"""
+    return np.arange(-1 * window_ms, window_ms + 1.0, step_ms)
+
+
+def _star(grid, vsini, incl=90.0, u1=0.29, u2=0.34, w_ratio=0.0, radius_km=R_SUN_KM):
+    return dict(prot=prot_for_vsini(vsini, radius_km, incl), w_ratio=w_ratio, incl=incl,
+                radius_km=radius_km, u1=u1, u2=u2, grid=int(grid), lb=np.array(LB_5778))
+
+
+def _act():
+    return dict(t_star=5778, t_diff_spot=663, t_diff_fac=250, nrho=20)
+
+
+def config0(seed: int = 1234, grid: int = 300) -> Workload:
+    """Plain FP64, small: grid 300, 5000-5100 A log-uniform 20k points, 200 lines,
+    10 mu bins, one spot (lat 30, lon 0, r 0.1), 25 phases, R=115000, 81-point CCF."""
+    rng = np.random.default_rng(seed)
+    wave = log_grid(5000.0, 5100.0, n=20000)
+    lines = draw_lines(rng, 200, 5000.0, 5100.0)
+    bounds, _ = mu_bins(10)
+    cubes = make_cubes(wave, lines, 10)
+    phases = np.arange(25, dtype=np.float64) / 25.0
+    regs = [(30.0, 0.0, 0.1, 0.0)]
+    return Workload("config0", wave, 1, bounds, cubes, lines, lines["center"].copy(), lines["depth"].copy(),
+                    _star(grid, 2.0), _act(), phases, [list(regs) for _ in phases],
+                    sigma_inst_ms(115000), vel_grid(20000.0, 500.0))
+
+
+def config1(seed: int = 1235, n_phase: int = 100, grid: int = 1000) -> Workload:
+    """Plain FP64 production shape: grid 1000, 3800-7000 A log-uniform at 0.25 km/s
+    (~733k points), 8000 lines, 20 mu bins, 2 spots + 2 faculae overlapping,
+    100 phases, R=115000, 4000-line mask CCF on 161 velocities."""
+    rng = np.random.default_rng(seed)
+    wave = log_grid(3800.0, 7000.0, dv_kms=0.25)
+    lines = draw_lines(rng, 8000, 3800.0, 7000.0)
+    bounds, _ = mu_bins(20)
+    cubes = make_cubes(wave, lines, 20)
+    regs = overlapping_regions(rng)
+    phases = np.arange(n_phase, dtype=np.float64) / n_phase
+    return Workload("config1", wave, 1, bounds, cubes, lines, lines["center"][:4000].copy(),
+                    lines["depth"][:4000].copy(), _star(grid, 2.0), _act(), phases,
+                    [list(regs) for _ in phases], sigma_inst_ms(115000), vel_grid(20000.0, 250.0))
+
+
+def overlapping_regions(rng: np.random.Generator):
+    """2 faculae (r 0.1-0.2) each overlapped by a spot (r 0.05-0.12); faculae first so
+    the later spot wins where they overlap (SOAP_integration.cu:364-381)."""
+    regs = []
+    lon0 = rng.uniform(0.0, 60.0)
+    for j in range(2):
+        lat = rng.uniform(-35.0, 35.0)
+        lon = lon0 + 160.0 * j
+        rf = rng.uniform(0.1, 0.2)
+        regs.append((lat, lon, rf, 1.0))
+    for j in range(2):
+        lat_f, lon_f, rf, _ = regs[j]
+        rs = rng.uniform(0.05, 0.12)
+        off = np.rad2deg(0.5 * rf)
+        regs.append((lat_f + rng.uniform(-off, off), lon_f + rng.uniform(-off, off), rs, 0.0))
+    return regs
+
+
+def small(seed: int = 7, grid: int = 48, n: int = 4096, n_mu: int = 6, n_phase: int = 4,
+          n_lines: int = 60, vsini: float = 3.0, incl: float = 70.0, w_ratio: float = 0.0,
+          regions=None, lmin: float = 5000.0, lmax: float = 5060.0) -> Workload:
+    """Tiny workload for brute-force parity (oracle in well under a second)."""
+    rng = np.random.default_rng(seed)
+    wave = log_grid(lmin, lmax, n=n)
+    lines = draw_lines(rng, n_lines, lmin, lmax)
+    bounds, _ = mu_bins(n_mu)
+    cubes = make_cubes(wave, lines, n_mu)
+    if regions is None:
+        regions = [(20.0, 10.0, 0.25, 1.0), (25.0, 15.0, 0.15, 0.0), (-10.0, 200.0, 0.2, 0.0)]
+    phases = np.arange(n_phase, dtype=np.float64) / n_phase
+    return Workload("small", wave, 1, bounds, cubes, lines, lines["center"].copy(), lines["depth"].copy(),
+                    _star(grid, vsini, incl=incl, w_ratio=w_ratio), _act(), phases,
+                    [list(regions) for _ in phases], sigma_inst_ms(115000), vel_grid(20000.0, 500.0))

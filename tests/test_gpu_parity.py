@@ -1,0 +1,189 @@
+"""Parity of the HIP engine (through the C ABI) with the FP64 CPU oracle.
+
+Bars (from the task statement of this path): spectra relative error <= 1e-12
+norm-wise (max|d| / max|ref|) on the interior [210, n-210); RV, FWHM and
+bisector span within 1 cm/s.  Integer/index outputs (mu bins, region maps)
+bit-exact."""
+import numpy as np
+import pytest
+
+import oracle as O
+from soap_gpu_amd import synth
+from soap_gpu_amd.engine import BCONV, FLUX, TOT, Engine, setup_workload
+from tests.pipeline_ref import TRIM, ccf_stage, normwise, oracle_maps, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+SPEC_TOL = 1e-12
+RV_TOL = 0.01  # m/s
+
+
+def run_engine(eng, w):
+    setup_workload(eng, w)
+    eng.quiet_star()
+    eng.integrate()
+    eng.resolution(w.sigma_inst)
+    eng.ccf()
+    eng.fit()
+    P = len(w.phases)
+    raw = np.stack([np.stack(eng.get_raw(p)) for p in range(P)]) if P else np.zeros((0, 3, w.n))
+    lower = np.stack([np.stack([eng.get_lower(p, t) for t in range(3)]) for p in list(range(P)) + [-1]])
+    return dict(maps=eng.get_maps(), quiet=np.stack([eng.get_quiet(0), eng.get_quiet(1)]), raw=raw, lower=lower,
+                ccf=eng.get_ccf(), fit=eng.get_fit())
+
+
+@pytest.fixture(scope="module")
+def small_case(engine):
+    w = synth.small()
+    return w, run_engine(engine, w), run_oracle(w)
+
+
+@pytest.fixture(scope="module")
+def config0_case(engine):
+    w = synth.config0()
+    return w, run_engine(engine, w), run_oracle(w)
+
+
+@pytest.mark.parametrize("case", ["small_case", "config0_case"])
+def test_disk_maps_bitwise(case, request):
+    w, g, o = request.getfixturevalue(case)
+    for k in ("mu", "delta", "limb", "lb"):
+        assert np.array_equal(g["maps"][k], o["maps"][k]), k
+    assert np.array_equal(g["maps"]["region"], o["maps"]["region"])
+
+
+def test_disk_maps_differential_rotation(engine):
+    w = synth.small(w_ratio=-0.18, grid=64)
+    setup_workload(engine, w, with_mask=False)
+    g = engine.get_maps()
+    o = oracle_maps(w)
+    assert np.array_equal(g["region"], o["region"])
+    for k in ("mu", "limb", "lb"):
+        assert np.array_equal(g[k], o[k]), k
+    # delta goes through sin(): device and host libm may differ by an ulp
+    np.testing.assert_allclose(g["delta"], o["delta"], rtol=4e-16, atol=0)
+
+
+@pytest.mark.parametrize("case", ["small_case", "config0_case"])
+def test_quiet_star_parity(case, request):
+    w, g, o = request.getfixturevalue(case)
+    for which in (0, 1):
+        assert normwise(g["quiet"][which], o["quiet"][which]) <= SPEC_TOL
+
+
+@pytest.mark.parametrize("vsini,incl,w_ratio", [(1.0, 90.0, 0.0), (6.5, 35.0, 0.0), (3.0, 80.0, -0.18),
+                                                (12.0, 60.0, 0.0)])
+def test_quiet_star_shapes(engine, vsini, incl, w_ratio):
+    w = synth.small(vsini=vsini, incl=incl, w_ratio=w_ratio, grid=40, n=3000)
+    setup_workload(engine, w, with_mask=False)
+    engine.quiet_star()
+    o = oracle_maps(w)
+    for which, key in ((0, "q"), (1, "qsed")):
+        ref = O.quiet_star(O.GRID_LOG, w.wave, o, w.cubes[key], w.star["grid"])
+        assert normwise(engine.get_quiet(which), ref, lo=60) <= SPEC_TOL
+
+
+@pytest.mark.parametrize("case", ["small_case", "config0_case"])
+def test_integrate_parity(case, request):
+    w, g, o = request.getfixturevalue(case)
+    for p in range(len(w.phases)):
+        for t in range(3):
+            ref = o["raw"][p, t]
+            if np.max(np.abs(ref)) == 0:
+                assert np.max(np.abs(g["raw"][p, t])) == 0
+                continue
+            assert normwise(g["raw"][p, t], ref) <= SPEC_TOL, (p, t)
+
+
+def test_phase_maps_equal(engine):
+    w = synth.small(grid=64)
+    setup_workload(engine, w, with_mask=False)
+    a = w.act
+    for p in range(len(w.phases)):
+        t_g, b_g = engine.phase_maps(p)
+        t_o, b_o = O.phase_maps(64, w.star["incl"], a["nrho"], a["t_star"], a["t_diff_spot"], a["t_diff_fac"],
+                                w.phases[p], w.regions[p])
+        assert np.array_equal(t_g, t_o) and np.array_equal(b_g, b_o)
+
+
+@pytest.mark.parametrize("case", ["small_case", "config0_case"])
+def test_resolution_parity_isolated(case, request):
+    """GPU resolution vs the oracle's lower_resolution on the GPU's own raw and
+    quiet spectra (isolates a7)."""
+    w, g, o = request.getfixturevalue(case)
+    qn, qs = g["quiet"]
+    for p in range(len(w.phases)):
+        for t in range(3):
+            ref, _ = O.lower_resolution(O.GRID_LOG, w.wave, w.sigma_inst, qn if t == BCONV else qs, g["raw"][p, t], 1)
+            assert normwise(g["lower"][p, t], ref) <= SPEC_TOL, (p, t)
+    for t, q in ((FLUX, qs), (BCONV, qn), (TOT, qs)):
+        ref, _ = O.lower_resolution(O.GRID_LOG, w.wave, w.sigma_inst, None, q, 0)
+        assert normwise(g["lower"][-1, t], ref) <= SPEC_TOL
+
+
+@pytest.mark.parametrize("case", ["small_case", "config0_case"])
+def test_resolution_parity_end_to_end(case, request):
+    w, g, o = request.getfixturevalue(case)
+    assert normwise(g["lower"], o["lower"]) <= SPEC_TOL
+
+
+@pytest.mark.parametrize("case", ["small_case", "config0_case"])
+def test_ccf_parity_isolated(case, request):
+    """GPU mask CCF vs the oracle's dense-mask CCF on the GPU's lower spectra."""
+    w, g, o = request.getfixturevalue(case)
+    ref = ccf_stage(w, g["lower"])
+    np.testing.assert_allclose(g["ccf"], ref["ccf"], rtol=1e-12, atol=1e-13)
+
+
+def test_fit_parity_on_same_ccfs(engine, config0_case):
+    """Batched GPU fit vs the oracle fit on identical CCFs."""
+    w, g, o = config0_case
+    y = o["ccf"].reshape(-1, len(w.vgrid))
+    r = engine.fit_ccf_host(w.vgrid, y)
+    par_o = o["par"].reshape(-1, 5)
+    np.testing.assert_allclose(r["par"][:, 3], par_o[:, 3], rtol=0, atol=1e-4)  # RV
+    np.testing.assert_allclose(r["par"][:, 4], par_o[:, 4], rtol=0, atol=1e-4)  # FWHM
+    np.testing.assert_allclose(r["par"][:, 2], par_o[:, 2], rtol=0, atol=1e-4)  # span
+    np.testing.assert_allclose(r["par"][:, :2], par_o[:, :2], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(r["bis"], o["bis"].reshape(-1, 91), rtol=0, atol=1e-3)
+
+
+@pytest.mark.parametrize("case", ["small_case", "config0_case"])
+def test_end_to_end_rv_fwhm_bis(case, request):
+    w, g, o = request.getfixturevalue(case)
+    par_g = g["fit"]["par"]
+    rv_g = par_g[:-1, :, 3] - par_g[-1:, :, 3]
+    assert np.max(np.abs(rv_g - o["rv"])) < RV_TOL
+    assert np.max(np.abs(par_g[:, :, 4] - o["par"][:, :, 4])) < RV_TOL
+    assert np.max(np.abs(par_g[:, :, 2] - o["par"][:, :, 2])) < RV_TOL
+
+
+def test_config0_spot_signal_is_physical(config0_case):
+    """One spot at lat 30 on an equator-on star: the tot RV swings through zero
+    with an amplitude of order (filling factor x vsini)."""
+    w, g, o = config0_case
+    par = g["fit"]["par"]
+    rv = par[:-1, TOT, 3] - par[-1, TOT, 3]
+    amp = np.max(np.abs(rv))
+    assert 0.5 < amp < 100.0
+    assert rv.max() > 0 > rv.min()
+
+
+def test_errors_are_reported(engine):
+    """Host-forced failures return a status and a message (no GPU fault)."""
+    from soap_gpu_amd.engine import SoapError
+    e2 = Engine(0)
+    with pytest.raises(SoapError, match="set the wavelength grid"):
+        e2.set_cube(0, np.ones((2, 10)))
+    with pytest.raises(SoapError, match="log-uniform"):
+        e2.set_wave(np.linspace(5000, 5010, 100))
+    w = synth.small()
+    e2.set_wave(w.wave)
+    e2.set_mu_bounds(w.mu_bounds)
+    with pytest.raises(SoapError, match="mu rows"):
+        e2.set_cube(0, np.ones((2, w.n)))
+        e2.disk_init(w.star)
+        e2.quiet_star()
+    with pytest.raises(SoapError, match="run disk_init first"):
+        Engine(0).quiet_star()
+    e2.close()
