@@ -113,11 +113,12 @@ def main():
     st = eng.stage_ms()
     eng.set_timing(False)
     per = {k: v / args.steps for k, v in st.items()}
-    alg_bytes, rows = eng.integrate_stats()
+    alg_bytes, phase_bytes, rows = eng.integrate_stats()
     fir_ms = per["fir_active"]
     achieved = alg_bytes / (fir_ms * 1e-3) / 1e9
     log("[bench] stage ms/step: " + ", ".join(f"{k}={v:.3f}" for k, v in per.items()))
-    log(f"[bench] fir_active: {alg_bytes / 1e6:.1f} MB algorithmic, R_psi summed over phases = {rows}")
+    log(f"[bench] fir_active: {alg_bytes / 1e6:.1f} MB minimum traffic (union of rows + outputs), "
+        f"{phase_bytes / 1e6:.1f} MB counted per phase, sum R_psi = {rows}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -144,7 +145,8 @@ def main():
                    "grid": args.grid, "n_lambda": w.n, "phases": len(w.phases)},
         "roofline": {"kernel": "k_fir_active", "bound": "hbm", "achieved": round(achieved, 1),
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": None, "alg_bytes_per_launch": alg_bytes, "launch_ms": round(fir_ms, 4)},
+                     "traffic": None, "alg_bytes_per_launch": alg_bytes, "bytes_per_phase_sum": phase_bytes,
+                     "rows_sum": rows, "launch_ms": round(fir_ms, 4)},
         "stage_ms": {k: round(v, 4) for k, v in per.items()},
         "cpu_baseline": cpu,
     }

@@ -186,9 +186,10 @@ int soap_set_timing(soap_ctx *ctx, int enable);
 int soap_get_stage_ms(soap_ctx *ctx, double *ms, int32_t n_stage);
 const char *soap_stage_name(int32_t stage);
 int32_t soap_num_stages(void);
-/* algorithmic bytes of the last soap_integrate (seed rows touched + continua +
- * outputs) and the distinct seed rows R_psi summed over phases */
-int soap_get_integrate_stats(soap_ctx *ctx, double *alg_bytes, int64_t *rows_touched);
+/* HBM bytes of the last soap_integrate: bytes_min = seed rows touched by any
+ * phase (read once) + continua + 3 outputs per phase; bytes_phase_sum = the same
+ * counted per phase, sum_psi 8 n (R_psi + continua + 3); rows_sum = sum_psi R_psi */
+int soap_get_integrate_stats(soap_ctx *ctx, double *bytes_min, double *bytes_phase_sum, int64_t *rows_sum);
 
 #ifdef __cplusplus
 }

@@ -32,19 +32,20 @@ def _run(cmd):
     return r
 
 
-def build_lib(verbose: bool = False) -> str:
+def build_lib(verbose: bool = False, out: str = LIB, defines=()) -> str:
     objs = []
-    os.makedirs(os.path.join(HERE, "build"), exist_ok=True)
+    bdir = os.path.join(HERE, "build", os.path.basename(out) + ".d")
+    os.makedirs(bdir, exist_ok=True)
     for s in SOURCES:
-        o = os.path.join(HERE, "build", os.path.basename(s) + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", s, "-o", o])
+        o = os.path.join(bdir, os.path.basename(s) + ".o")
+        _run([HIPCC, *HIP_FLAGS, *[f"-D{d}" for d in defines], "-I", os.path.join(ROOT, "include"), "-c", s, "-o", o])
         objs.append(o)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print("built", LIB)
-    return LIB
+        print("built", out)
+    return out
 
 
 def build_tools(verbose: bool = False) -> list[str]:

@@ -85,14 +85,17 @@ struct soap_ctx {
   bool have_raw = false;
   // tap scratch
   DevBuf acc, taps, krange, kflags, bins, nbins;
+
   // resolution
   double *d_lower = nullptr;  // [(P+1)][3][n]
+  double *d_qc = nullptr;     // [2][n] quiet convolution terms
   DevBuf part, mm, ctap, qsel, variant;
+  int res_dlo = 0, res_ntp = 0;
   bool have_lower = false;
   // mask / ccf / fit
   int n_lines = 0, n_v = 0, trim = 0;
-  double *d_line_l = nullptr, *d_line_w = nullptr, *d_vgrid = nullptr, *d_appod = nullptr;
-  DevBuf m_start, m_len, m_val, m_alive, sorted, rank;
+  double *d_line_l = nullptr, *d_line_w = nullptr, *d_vgrid = nullptr, *d_appod = nullptr, *d_w_sorted = nullptr;
+  DevBuf m_start, m_len, m_val, m_alive, sorted, rank, ccf_part;
   int mask_scan = 0;
   bool have_mask = false, mask_built = false;
   double *d_ccf = nullptr;  // [(P+1)*3][n_v]
@@ -105,9 +108,8 @@ struct soap_ctx {
   double stage_ms[ST_COUNT] = {};
   // allocation sizes of the buffers made by dalloc
   std::map<void *, size_t> caps;
-  // resolution constants last uploaded
+  // resolution taps last uploaded
   double res_sigma = -1;
-  int res_P = -1;
   // stats
   double alg_bytes = 0;
   int64_t rows_touched = 0;
@@ -320,11 +322,11 @@ void soap_ctx_destroy(soap_ctx *c) {
   for (int i = 0; i < 6; i++)
     if (c->cube_owned[i]) dfree(c->d_cube[i]);
   void *ptrs[] = {c->d_mu,      c->d_delta,   c->d_limb,    c->d_lbm,      c->d_t,      c->d_region, c->d_m,
-                  c->d_quiet,   c->d_geo,     c->d_geo_off, c->d_box,      c->d_raw,    c->d_lower,  c->d_line_l,
-                  c->d_line_w,  c->d_vgrid,   c->d_appod,   c->d_ccf,      c->acc.p,    c->taps.p,   c->krange.p,
+                  c->d_quiet,   c->d_qc,      c->d_geo,     c->d_geo_off, c->d_box,      c->d_raw,    c->d_lower,  c->d_line_l,
+                  c->d_line_w,  c->d_w_sorted, c->d_vgrid,   c->d_appod,   c->d_ccf,      c->acc.p,    c->taps.p,   c->krange.p,
                   c->kflags.p,  c->bins.p,    c->nbins.p,   c->part.p,     c->mm.p,     c->ctap.p,   c->qsel.p,
                   c->variant.p, c->m_start.p, c->m_len.p,   c->m_val.p,    c->m_alive.p, c->sorted.p, c->rank.p,
-                  c->f_par.p,   c->f_model.p, c->f_depth.p, c->f_bis.p,    c->f_iter.p};
+                  c->f_par.p,   c->f_model.p, c->f_depth.p, c->f_bis.p,    c->f_iter.p, c->ccf_part.p};
   for (void *p : ptrs) dfree(p);
   hipStreamDestroy(c->stream);
   delete c;
@@ -640,7 +642,7 @@ int soap_quiet_star(soap_ctx *c) {
   f.out = c->d_quiet;
   {
     StageTimer tm(c, ST_FIR_QUIET);
-    HIPCHK(c, launch_fir_quiet(f, c->stream));
+    HIPCHK(c, launch_fir_quiet_lds(f, c->stream));
   }
   c->have_quiet = true;
   c->have_lower = c->have_ccf = c->have_fit = false;
@@ -809,7 +811,9 @@ int soap_integrate(soap_ctx *c) {
   f.out = c->d_raw;
   {
     StageTimer tm(c, ST_FIR_ACTIVE);
-    HIPCHK(c, launch_fir_active(f, c->P, c->stream));
+    hipError_t e = launch_fir_active_grp(f, c->P, c->stream);
+    if (e == hipErrorInvalidValue) e = launch_fir_active(f, c->P, c->stream);  // very wide shift ranges
+    HIPCHK(c, e);
   }
   c->have_raw = true;
   c->have_lower = c->have_ccf = c->have_fit = false;
@@ -817,7 +821,7 @@ int soap_integrate(soap_ctx *c) {
   return SOAP_OK;
 }
 
-int soap_get_integrate_stats(soap_ctx *c, double *alg_bytes, int64_t *rows_touched) {
+int soap_get_integrate_stats(soap_ctx *c, double *bytes_min, double *bytes_phase_sum, int64_t *rows_sum) {
   if (!c) return SOAP_ERR_ARG;
   if (!c->have_raw || c->P == 0) return fail(c, SOAP_ERR_STATE, "integrate_stats: run integrate first");
   HIPCHK(c, hipSetDevice(c->device));
@@ -825,21 +829,29 @@ int soap_get_integrate_stats(soap_ctx *c, double *alg_bytes, int64_t *rows_touch
   HIPCHK(c, hipMemcpyAsync(fl.data(), c->kflags.p, fl.size() * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   int64_t rows = 0;
-  double bytes = 0;
+  double per_phase = 0;
+  std::vector<int> uni(c->n_mu, 0);
+  bool us = false, uf = false;
   for (int p = 0; p < c->P; p++) {
     int64_t r = 0;
     bool s = false, f = false;
     for (int k = 0; k < c->n_mu; k++) {
       int v = fl[(size_t)p * c->n_mu + k];
+      uni[k] |= v ? (4 | v) : 0;
       if (v) r += 2;  // quiet normalised + SED rows
       if (v & 1) r += 1, s = true;
       if (v & 2) r += 1, f = true;
     }
+    us |= s;
+    uf |= f;
     rows += r;
-    bytes += 8.0 * c->n * (double)(r + (s ? 1 : 0) + (f ? 1 : 0) + 3);
+    per_phase += 8.0 * c->n * (double)(r + (s ? 1 : 0) + (f ? 1 : 0) + 3);
   }
-  if (alg_bytes) *alg_bytes = bytes;
-  if (rows_touched) *rows_touched = rows;
+  int64_t ru = 0;
+  for (int k = 0; k < c->n_mu; k++) ru += (uni[k] & 4 ? 2 : 0) + (uni[k] & 1 ? 1 : 0) + (uni[k] & 2 ? 1 : 0);
+  if (bytes_min) *bytes_min = 8.0 * c->n * (double)(ru + (us ? 1 : 0) + (uf ? 1 : 0) + 3 * (int64_t)c->P);
+  if (bytes_phase_sum) *bytes_phase_sum = per_phase;
+  if (rows_sum) *rows_sum = rows;
   return SOAP_OK;
 }
 
@@ -909,71 +921,61 @@ static int conv_window(const std::vector<double> &w, double sigma_ms) {
 }
 
 int soap_resolution(soap_ctx *c, double sigma_ms, double quiet_weight) {
-  (void)quiet_weight;  // linear grids only
+  (void)quiet_weight;  // weight of the quiet-star form on linear grids only
   if (!c) return SOAP_ERR_ARG;
   if (!c->have_quiet) return fail(c, SOAP_ERR_STATE, "resolution: quiet star missing");
   if (c->P > 0 && !c->have_raw) return fail(c, SOAP_ERR_STATE, "resolution: raw spectra missing");
   if (!(sigma_ms > 0)) return fail(c, SOAP_ERR_ARG, "resolution: sigma must be > 0");
   HIPCHK(c, hipSetDevice(c->device));
   int wid = conv_window(c->wave, sigma_ms);
-  if (wid < 1 || 2 * (int64_t)wid + 2 >= c->n) return fail(c, SOAP_ERR_ARG, "resolution: window %d vs %lld pixels", wid, (long long)c->n);
-  // log-grid taps c_d = exp(d h) expm1(h)/(fs sqrt(2 pi)) exp(-(expm1(d h)/fs)^2/2),
-  // truncated where c_d < 2^-60 c_0 (far below FP64 resolution of the sum)
-  double fs = -1 * 1.0 * (1 - sqrt((1 + sigma_ms / kC) / (1 - sigma_ms / kC)));
-  auto tap = [&](int d) {
-    double u = expm1(d * c->h) / fs;
-    return exp(d * c->h) * c->expm1_h / (fs * sqrt(2. * kPi)) * exp(-u * u / 2);
-  };
-  double c0 = tap(0);
-  int dt = 0;
-  for (int d = 1; d <= wid; d++)
-    if (tap(d) >= c0 * 8.673617379884035e-19 || tap(-d) >= c0 * 8.673617379884035e-19) dt = d;
+  if (wid < 1 || 2 * (int64_t)wid + 2 >= c->n)
+    return fail(c, SOAP_ERR_ARG, "resolution: window %d vs %lld pixels", wid, (long long)c->n);
   ConvArgs ca{};
   ca.n = c->n;
   ca.wid = wid;
-  ca.dlo = -std::min(wid, dt);
-  ca.dhi = std::min(wid - 2, dt);
-  std::vector<double> ct(ca.dhi - ca.dlo + 1);
-  for (int d = ca.dlo; d <= ca.dhi; d++) ct[d - ca.dlo] = tap(d);
-  ca.n_tiles = (int)((c->n + 255) / 256);
-  int n_act = 3 * c->P, n_spec = n_act + 2;
+  ca.n_tiles = (int)((c->n + conv_tile() - 1) / conv_tile());
+  int n_spec = 3 * (c->P + 1);
   int rc;
-  if ((rc = dalloc(c, &c->d_lower, (size_t)(c->P + 1) * 3 * c->n)) || (rc = ensure(c, c->ctap, ct.size() * 8)) ||
-      (rc = ensure(c, c->part, (size_t)n_spec * ca.n_tiles * 2 * 8)) || (rc = ensure(c, c->mm, (size_t)n_spec * 2 * 8)) ||
-      (rc = ensure(c, c->qsel, (size_t)n_spec * 4)) || (rc = ensure(c, c->variant, (size_t)n_spec * 4)))
+  if ((rc = dalloc(c, &c->d_lower, (size_t)n_spec * c->n)) || (rc = dalloc(c, &c->d_qc, (size_t)2 * c->n)) ||
+      (rc = ensure(c, c->part, (size_t)n_spec * ca.n_tiles * 2 * 8)) || (rc = ensure(c, c->mm, (size_t)n_spec * 2 * 8)))
     return rc;
-  std::vector<int32_t> qsel(n_spec), var(n_spec);
-  for (int p = 0; p < c->P; p++)
-    for (int t = 0; t < 3; t++) {
-      qsel[p * 3 + t] = t == SOAP_BCONV ? 0 : 1;  // SOAP_resolution.cu:414-416
-      var[p * 3 + t] = 1;
-    }
-  qsel[n_act] = 1;  // _SED_quiet_sun_lower_spec
-  qsel[n_act + 1] = 0;  // _quiet_sun_lower_spec
-  var[n_act] = var[n_act + 1] = 0;
-  if (c->res_sigma != sigma_ms || c->res_P != c->P) {
+  if (c->res_sigma != sigma_ms) {
+    // log-grid taps c_d = exp(d h) expm1(h)/(fs sqrt(2 pi)) exp(-(expm1(d h)/fs)^2/2)
+    // (generalisation of conv_inst's 0.005 A pixel), used where c_d >= 2^-60 c_0
+    double fs = -1 * 1.0 * (1 - sqrt((1 + sigma_ms / kC) / (1 - sigma_ms / kC)));
+    auto tap = [&](int d) {
+      double u = expm1(d * c->h) / fs;
+      return exp(d * c->h) * c->expm1_h / (fs * sqrt(2. * kPi)) * exp(-u * u / 2);
+    };
+    double c0 = tap(0);
+    int dt = 0;
+    for (int d = 1; d <= wid; d++)
+      if (tap(d) >= c0 * 8.673617379884035e-19 || tap(-d) >= c0 * 8.673617379884035e-19) dt = d;
+    int dlo = -std::min(wid, dt), dhi = std::min(wid - 2, dt);
+    int ch = conv_chunk();
+    int ntp = ((dhi - dlo + 1) + ch - 1) / ch * ch;
+    std::vector<double> ct(ntp, 0.0);
+    for (int d = dlo; d <= dhi; d++) ct[d - dlo] = tap(d);
+    if ((rc = ensure(c, c->ctap, ct.size() * 8))) return rc;
     HIPCHK(c, hipMemcpyAsync(c->ctap.p, ct.data(), ct.size() * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->qsel.p, qsel.data(), n_spec * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->variant.p, var.data(), n_spec * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->res_sigma = sigma_ms;
-    c->res_P = c->P;
+    c->res_dlo = dlo;
+    c->res_ntp = ntp;
   }
+  ca.dlo = c->res_dlo;
+  ca.ntp = c->res_ntp;
   ca.ctap = (const double *)c->ctap.p;
-  double *lower_q = c->d_lower + (size_t)n_act * c->n;  // frame P: [flux, bconv, tot]
+  double *part = (double *)c->part.p;
   {
     StageTimer tm(c, ST_CONV);
-    if (n_act) HIPCHK(c, launch_conv(ca, c->d_quiet, c->d_raw, n_act, (const int32_t *)c->qsel.p,
-                                     (const int32_t *)c->variant.p, c->d_lower, (double *)c->part.p, c->stream));
-    // quiet frames written to the flux slot (SED) and the bconv slot (normalised)
-    HIPCHK(c, launch_conv(ca, c->d_quiet, c->d_quiet, 2, (const int32_t *)c->qsel.p + n_act,
-                          (const int32_t *)c->variant.p + n_act, lower_q,
-                          (double *)c->part.p + (size_t)n_act * ca.n_tiles * 2, c->stream));
+    HIPCHK(c, launch_conv_quiet(ca, c->d_quiet, c->d_qc, c->d_lower + (size_t)3 * c->P * c->n,
+                                part + (size_t)3 * c->P * ca.n_tiles * 2, c->stream));
+    HIPCHK(c, launch_conv_active(ca, c->d_quiet, c->d_qc, c->d_raw, c->P, c->d_lower, part, c->stream));
   }
   {
     StageTimer tm(c, ST_NORM);
-    HIPCHK(c, launch_norm(ca, n_spec, (const double *)c->part.p, (double *)c->mm.p, c->d_lower, c->stream));
-    HIPCHK(c, hipMemcpyAsync(lower_q + 2 * c->n, lower_q, c->n * 8, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, launch_norm(c->n, n_spec, ca.n_tiles, part, (double *)c->mm.p, c->d_lower, c->stream));
   }
   c->have_lower = true;
   c->have_ccf = c->have_fit = false;
@@ -994,7 +996,7 @@ int soap_get_norm(soap_ctx *c, int32_t phase, int type, double *minmax2) {
   if (!c || !minmax2 || type < 0 || type > 2 || phase < -1 || phase >= c->P) return fail(c, SOAP_ERR_ARG, "get_norm: bad arguments");
   if (!c->have_lower) return fail(c, SOAP_ERR_STATE, "get_norm: run resolution first");
   HIPCHK(c, hipSetDevice(c->device));
-  int sp = phase >= 0 ? phase * 3 + type : 3 * c->P + (type == SOAP_BCONV ? 1 : 0);
+  int sp = (phase >= 0 ? phase : c->P) * 3 + type;
   HIPCHK(c, hipMemcpyAsync(minmax2, (double *)c->mm.p + sp * 2, 16, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SOAP_OK;
@@ -1013,7 +1015,8 @@ int soap_set_mask(soap_ctx *c, const double *line_l, const double *line_w, int32
   c->trim = trim;
   int rc;
   if ((rc = dalloc(c, &c->d_line_l, n_lines)) || (rc = dalloc(c, &c->d_line_w, n_lines)) ||
-      (rc = dalloc(c, &c->d_vgrid, n_v)) || (rc = dalloc(c, &c->d_appod, n_v)))
+      (rc = dalloc(c, &c->d_vgrid, n_v)) || (rc = dalloc(c, &c->d_appod, n_v)) ||
+      (rc = dalloc(c, &c->d_w_sorted, n_lines)))
     return rc;
   // wavelength order of the lines and the neighbour scan depth for overwrites
   std::vector<int32_t> srt(n_lines), rnk(n_lines);
@@ -1030,9 +1033,9 @@ int soap_set_mask(soap_ctx *c, const double *line_l, const double *line_w, int32
     scan = std::max(scan, i - j + 1);
   }
   c->mask_scan = scan;
-  // a mask hole must span at most 30 pixels (32-bit alive masks per line)
+  // a mask hole must span at most 22 pixels (24-bit alive masks per line)
   double hole_px = (sqrt((1 + beta) / (1 - beta)) - 1) * 1.001 / c->h + 3;
-  if (hole_px > 30) return fail(c, SOAP_ERR_LIMIT, "set_mask: mask holes span %.1f pixels (> 30)", hole_px);
+  if (hole_px > 22) return fail(c, SOAP_ERR_LIMIT, "set_mask: mask holes span %.1f pixels (> 22)", hole_px);
   // appodisation window, run_all_SOAP.py:306-311
   std::vector<double> ap;
   {
@@ -1055,6 +1058,9 @@ int soap_set_mask(soap_ctx *c, const double *line_l, const double *line_w, int32
   HIPCHK(c, hipMemcpyAsync(c->d_appod, ap.data(), n_v * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->sorted.p, srt.data(), n_lines * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->rank.p, rnk.data(), n_lines * 4, hipMemcpyHostToDevice, c->stream));
+  std::vector<double> ws(n_lines);
+  for (int q = 0; q < n_lines; q++) ws[q] = line_w[srt[q]];
+  HIPCHK(c, hipMemcpyAsync(c->d_w_sorted, ws.data(), n_lines * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->have_mask = true;
   c->mask_built = false;
@@ -1065,11 +1071,13 @@ int soap_set_mask(soap_ctx *c, const double *line_l, const double *line_w, int32
 static int build_mask(soap_ctx *c) {
   StageTimer tm(c, ST_MASK);
   int64_t n_t = c->n - 2 * (int64_t)c->trim;
-  HIPCHK(c, launch_mask_build(c->d_line_l, c->d_line_w, c->n_lines, c->d_vgrid, c->n_v, c->wave[c->trim], c->h, n_t,
-                              (int32_t *)c->m_start.p, (int32_t *)c->m_len.p, (double *)c->m_val.p, c->stream));
-  HIPCHK(c, launch_mask_overwrite((const int32_t *)c->sorted.p, (const int32_t *)c->rank.p, c->n_lines, c->n_v,
-                                  c->mask_scan, (const int32_t *)c->m_start.p, (const int32_t *)c->m_len.p,
-                                  (uint32_t *)c->m_alive.p, c->stream));
+  (void)n_t;
+  HIPCHK(c, launch_mask_build(c->d_line_l, c->d_line_w, (const int32_t *)c->sorted.p, c->n_lines, c->d_vgrid, c->n_v,
+                              c->wave[c->trim], c->h, (int32_t *)c->m_start.p, (int32_t *)c->m_len.p,
+                              (double2 *)c->m_val.p, c->stream));
+  HIPCHK(c, launch_mask_overwrite((const int32_t *)c->sorted.p, c->n_lines, c->n_v, c->mask_scan,
+                                  (const int32_t *)c->m_start.p, (const int32_t *)c->m_len.p, (uint32_t *)c->m_alive.p,
+                                  c->stream));
   c->mask_built = true;
   return SOAP_OK;
 }
@@ -1082,13 +1090,14 @@ int soap_ccf(soap_ctx *c) {
   int rc;
   if (!c->mask_built && (rc = build_mask(c))) return rc;
   int n_spec = (c->P + 1) * 3;
-  if ((rc = dalloc(c, &c->d_ccf, (size_t)n_spec * c->n_v))) return rc;
+  if ((rc = dalloc(c, &c->d_ccf, (size_t)n_spec * c->n_v)) ||
+      (rc = ensure(c, c->ccf_part, (size_t)ccf_chunks(c->n_lines) * n_spec * c->n_v * 8)))
+    return rc;
   {
     StageTimer tm(c, ST_CCF);
     HIPCHK(c, launch_ccf(c->d_lower, c->n, c->trim, n_spec, c->n_lines, c->n_v, c->n - 2 * (int64_t)c->trim,
-                         (const int32_t *)c->m_start.p, (const int32_t *)c->m_len.p, (const uint32_t *)c->m_alive.p,
-                         (const double *)c->m_val.p, c->d_line_w, c->d_ccf, c->stream));
-    HIPCHK(c, launch_ccf_finish(c->d_ccf, n_spec, c->n_v, c->d_appod, c->stream));
+                         (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p,
+                         c->d_w_sorted, c->d_appod, (double *)c->ccf_part.p, c->d_ccf, c->stream));
   }
   c->have_ccf = true;
   c->have_fit = false;

@@ -51,7 +51,7 @@ struct FirArgs {
   const double *q, *qs, *sp, *sps, *fa, *fas;  // cube bases [n_mu][n]
   int64_t n;
   int32_t n_mu, dmin, nd;
-  const double *taps;    // [job][n_mu][nsets][nd]
+  const double *taps;    // quiet: [n_mu][nd]; active: [job][n_mu][nd][4] (sets contiguous)
   const int32_t *krange; // [job][n_mu][2] inclusive d-index range, lo > hi: empty
   const int32_t *kflags; // [job][n_mu]: bit0 spot sets nonzero, bit1 facula sets nonzero
   const int32_t *bins;   // [job][n_mu]
@@ -62,9 +62,10 @@ struct FirArgs {
 struct ConvArgs {
   int64_t n;
   int32_t wid;           // reference window half width (nb_pixel)
-  int32_t dlo, dhi;      // taps used: d in [dlo, dhi]
-  const double *ctap;    // [dhi - dlo + 1]
-  int32_t n_tiles;
+  int32_t dlo;           // first tap offset
+  int32_t ntp;           // taps used (zero padded to a multiple of conv_chunk())
+  const double *ctap;    // [ntp]: c_{dlo + k}
+  int32_t n_tiles;       // ceil(n / conv_tile())
 };
 
 }  // namespace soap
@@ -81,22 +82,34 @@ hipError_t launch_taps_finalize(const unsigned long long *acc, int n_job, int ns
                                 int32_t *nbins, hipStream_t s);
 hipError_t launch_fir_quiet(const FirArgs &a, hipStream_t s);
 hipError_t launch_fir_active(const FirArgs &a, int n_job, hipStream_t s);
+hipError_t launch_fir_quiet_lds(const FirArgs &a, hipStream_t s);
+// phase-group form (bin outer, phases inner, LDS-staged rows and taps)
+hipError_t launch_fir_active_grp(const FirArgs &a, int n_job, hipStream_t s);
 hipError_t launch_phase_maps(int grid, int t_star, int t_diff_spot, int t_diff_fac, const RegionDev *geo,
                              int n_geo, double *temp, double *bi, hipStream_t s);
-// resolution: spectra sp = 0..n_spec-1; A = quiet[qidx[sp]] - raw[sp] (variant 1) or quiet[qidx] (variant 0)
-hipError_t launch_conv(const ConvArgs &c, const double *quiet, const double *raw, int n_spec,
-                       const int32_t *qsel, const int32_t *variant, double *o, double *part, hipStream_t s);
-hipError_t launch_norm(const ConvArgs &c, int n_spec, const double *part, double *mm, double *o, hipStream_t s);
+// resolution
+int conv_tile();
+int conv_chunk();
+// quiet terms Qc (two spectra) + the quiet-star frame (3 slots of the lower layout)
+hipError_t launch_conv_quiet(const ConvArgs &c, const double *quiet, double *qc, double *frame, double *part,
+                             hipStream_t s);
+hipError_t launch_conv_active(const ConvArgs &c, const double *quiet, const double *qc, const double *raw,
+                              int n_phase, double *o_out, double *part, hipStream_t s);
+// spectra sp = 0..n_spec-1 stored contiguously at o + sp*n, partials part[sp][n_tiles][2]
+hipError_t launch_norm(int64_t n, int n_spec, int n_tiles, const double *part, double *mm, double *o, hipStream_t s);
 // CCF
-hipError_t launch_mask_build(const double *line_l, const double *line_w, int n_lines, const double *vgrid,
-                             int n_v, double lam_t0, double h, int64_t n_t, int32_t *m_start, int32_t *m_len,
-                             double *m_val, hipStream_t s);
-hipError_t launch_mask_overwrite(const int32_t *sorted, const int32_t *rank, int n_lines, int n_v, int scan,
-                                 const int32_t *m_start, const int32_t *m_len, uint32_t *m_alive, hipStream_t s);
+// mask tables [q][v], q = position in wavelength order (sorted[q] = line index)
+hipError_t launch_mask_build(const double *line_l, const double *line_w, const int32_t *sorted, int n_lines,
+                             const double *vgrid, int n_v, double lam_t0, double h, int32_t *m_start, int32_t *m_len,
+                             double2 *m_wv, hipStream_t s);
+hipError_t launch_mask_overwrite(const int32_t *sorted, int n_lines, int n_v, int scan, const int32_t *m_start,
+                                 const int32_t *m_len, uint32_t *m_pk, hipStream_t s);
+int ccf_chunks(int n_lines);
+// CCF of n_spec spectra (spec + sp*spec_stride + trim), normalised and appodised;
+// part: ccf_chunks(n_lines) * n_spec * n_v doubles; w_sorted[q] = weight of line sorted[q]
 hipError_t launch_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
-                      int64_t n_t, const int32_t *m_start, const int32_t *m_len, const uint32_t *m_alive,
-                      const double *m_val, const double *line_w, double *ccf, hipStream_t s);
-hipError_t launch_ccf_finish(double *ccf, int n_spec, int n_v, const double *appod, hipStream_t s);
+                      int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
+                      const double *w_sorted, const double *appod, double *part, double *ccf, hipStream_t s);
 hipError_t launch_fit(const double *v, int n_v, const double *y, int n_ccf, double *par, double *model,
                       double *depth, double *bis, int32_t *iters, hipStream_t s);
 }  // namespace soap

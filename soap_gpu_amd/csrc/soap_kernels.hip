@@ -312,7 +312,8 @@ __global__ void k_taps_finalize(const unsigned long long *acc, int nsets_in, int
       if (nsets_out > nsets_in) v[nsets_in] = v[0] + v[1];
       bool nz = false;
       for (int st = 0; st < nsets_out; st++) {
-        taps[(((long)job * n_mu + k) * nsets_out + st) * nd + d] = v[st];
+        // quiet (1 set): [k][nd]; active (4 sets): [k][nd][4], sets contiguous
+        taps[(((long)job * n_mu + k) * nd + d) * nsets_out + st] = v[st];
         if (v[st] != 0.0) nz = true;
       }
       if (nsets_in == 3) {
@@ -406,7 +407,7 @@ __global__ void k_fir_active(FirArgs a) {
     for (int di = lo; di <= hi; di++) {
       long i = j - (a.dmin + di);
       if (i < 0 || i >= a.n) continue;
-      double h0 = T[di], h1 = T[a.nd + di], h2 = T[2 * a.nd + di], h3 = T[3 * a.nd + di];
+      double h0 = T[di * 4], h1 = T[di * 4 + 1], h2 = T[di * 4 + 2], h3 = T[di * 4 + 3];
       double vq = rq[i];
       F1 += h3 * rs[i];
       F2 += h0 * vq;
@@ -432,6 +433,194 @@ hipError_t launch_fir_active(const FirArgs &a, int n_job, hipStream_t s) {
   if (n_job == 0) return hipSuccess;
   dim3 grid((unsigned)((a.n + kTPB - 1) / kTPB), n_job);
   k_fir_active<<<grid, kTPB, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+// Phase-group FIR (bin outer, phases inner).  Block = 256-pixel tile x group of
+// kFirG phases; every pixel's 7 partial sums of the kFirG phases live in
+// registers.  For each mu bin touched by any phase of the group, the block
+// stages that bin's seed-row segments (quiet, quiet SED, and spot / facula rows
+// when present) and the group's taps in LDS; each staged row value then feeds
+// the FMAs of all phases of the group.  Blocks are numbered so the groups of a
+// tile run back to back on one XCD (b % 8), keeping the tile's rows in its L2.
+#ifndef SOAP_FIR_G
+#define SOAP_FIR_G 4
+#endif
+constexpr int kFirG = SOAP_FIR_G;  // phases per block
+constexpr int kFirTile = kTPB;
+
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__global__ __launch_bounds__(kTPB) void k_fir_active_grp(FirArgs a, int n_job, int n_tiles, int n_groups) {
+  extern __shared__ double rows[];  // [4][row_len]: quiet, quiet SED, spot, facula rows of one bin
+  __shared__ int s_fl[kFirG][64];
+  __shared__ int s_bins[64], s_ufl[64], s_ulo[64], s_uhi[64];
+  __shared__ int s_nb;
+  long b = blockIdx.x;
+  long xcd = b % 8, slot = b / 8;
+  long tile = (slot / n_groups) * 8 + xcd;
+  int grp = (int)(slot % n_groups);
+  if (tile >= n_tiles) return;  // uniform over the block
+  int p0 = grp * kFirG;
+  int ng = min(kFirG, n_job - p0);
+  int nd = a.nd, n_mu = a.n_mu;
+  int row_len = kFirTile + nd - 1;
+  long j0 = tile * kFirTile;
+  long src0 = j0 - a.dmin - nd + 1;
+  // group metadata: per-phase flags, union flags and union tap range per bin
+  for (int e = threadIdx.x; e < kFirG * 64; e += blockDim.x) {
+    int g = e / 64, k = e % 64;
+    int fl = 0;
+    if (g < ng && k < n_mu) {
+      long jk = (long)(p0 + g) * n_mu + k;
+      fl = a.kflags[jk];
+      if (a.krange[jk * 2] > a.krange[jk * 2 + 1]) fl = 0;
+    }
+    s_fl[g][k] = fl;
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    int k = threadIdx.x;
+    int ufl = 0, ulo = nd, uhi = -1;
+    for (int g = 0; g < ng; g++)
+      if (s_fl[g][k]) {
+        long jk = (long)(p0 + g) * n_mu + k;
+        ufl |= s_fl[g][k] | 4;
+        ulo = min(ulo, a.krange[jk * 2]);
+        uhi = max(uhi, a.krange[jk * 2 + 1]);
+      }
+    s_ufl[k] = ufl;
+    s_ulo[k] = ulo;
+    s_uhi[k] = uhi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int c = 0;
+    for (int k = 0; k < n_mu; k++)
+      if (s_ufl[k]) s_bins[c++] = k;
+    s_nb = c;
+  }
+  __syncthreads();
+  double F[kFirG][7];
+#pragma unroll
+  for (int g = 0; g < kFirG; g++)
+#pragma unroll
+    for (int f = 0; f < 7; f++) F[g][f] = 0.0;
+  const int nb = uni(s_nb);
+  for (int kk = 0; kk < nb; kk++) {
+    const int k = uni(s_bins[kk]);
+    const int ufl = uni(s_ufl[k]), ulo = uni(s_ulo[k]), uhi = uni(s_uhi[k]);
+    if (kk > 0) __syncthreads();
+    // rows present: quiet, quiet SED, then spot (if any spot cell) and facula (if any)
+    const int has_sp = ufl & 1, has_fa = (ufl >> 1) & 1;
+    const int nrow = 2 + has_sp + has_fa;
+    const double *src[4] = {a.q + (long)k * a.n, a.qs + (long)k * a.n,
+                            has_sp ? a.sp + (long)k * a.n : a.fa + (long)k * a.n, a.fa + (long)k * a.n};
+    for (int e = threadIdx.x; e < nrow * row_len; e += blockDim.x) {
+      int r = e / row_len, x = e - r * row_len;
+      long i = src0 + x;
+      rows[e] = (i >= 0 && i < a.n) ? src[r][i] : 0.0;
+    }
+    __syncthreads();
+    int flg[kFirG];
+#pragma unroll
+    for (int g = 0; g < kFirG; g++) flg[g] = uni(s_fl[g][k]);
+    // taps of phase g at offset di: a.taps[(((p0+g)*n_mu + k)*nd + di)*4 + set], uniform addresses
+    const double *Tk = a.taps + ((long)p0 * n_mu + k) * nd * 4;
+    const long gstride = (long)n_mu * nd * 4;
+    const double *rq = rows, *rs = rows + row_len, *rp = rows + 2 * row_len, *rf = rows + (2 + has_sp) * row_len;
+    for (int di = ulo; di <= uhi; di++) {
+      int e = threadIdx.x + nd - 1 - di;
+      double vq = rq[e], vs = rs[e];
+      double vp = (ufl & 1) ? rp[e] : 0.0;
+      double vf = (ufl & 2) ? rf[e] : 0.0;
+#pragma unroll
+      for (int g = 0; g < kFirG; g++) {
+        const int fl = flg[g];
+        if (!fl) continue;
+        const double *T = Tk + g * gstride + di * 4;
+        F[g][0] = fma(T[3], vs, F[g][0]);
+        if (fl & 1) {
+          F[g][1] = fma(T[0], vq, F[g][1]);
+          F[g][4] = fma(T[0], vp, F[g][4]);
+        }
+        if (fl & 2) {
+          F[g][2] = fma(T[2], vq, F[g][2]);
+          F[g][3] = fma(T[1], vq, F[g][3]);
+          F[g][5] = fma(T[1], vf, F[g][5]);
+          F[g][6] = fma(T[2], vf, F[g][6]);
+        }
+      }
+    }
+  }
+  long j = j0 + threadIdx.x;
+  if (j < a.n) {
+    double cs = a.sps ? a.sps[j] : 0.0;
+    double cf = a.fas ? a.fas[j] : 0.0;
+#pragma unroll
+    for (int g = 0; g < kFirG; g++) {
+      if (g >= ng) break;
+      double *o = a.out + (long)(p0 + g) * 3 * a.n;
+      // F: 0 = H1a*SEDq, 1 = H1s*Sq, 2 = H2f*Sq, 3 = H1f*Sq, 4 = H1s*Ssp, 5 = H1f*Sfac, 6 = H2f*Sfac
+      o[j] = F[g][0] - cs * F[g][1] - cf * F[g][2];
+      o[a.n + j] = (F[g][1] + F[g][3]) - F[g][4] - F[g][5];
+      o[2 * a.n + j] = F[g][0] - cs * F[g][4] - cf * F[g][6];
+    }
+  }
+}
+
+hipError_t launch_fir_active_grp(const FirArgs &a, int n_job, hipStream_t s) {
+  if (n_job == 0) return hipSuccess;
+  if (a.n_mu > 64) return hipErrorInvalidValue;
+  int n_tiles = (int)((a.n + kFirTile - 1) / kFirTile);
+  int n_groups = (n_job + kFirG - 1) / kFirG;
+  long tiles8 = ((long)n_tiles + 7) / 8 * 8;
+  long blocks = tiles8 * n_groups;
+  size_t lds = (size_t)4 * (kFirTile + a.nd - 1) * sizeof(double);
+  if (lds > 60000) return hipErrorInvalidValue;
+  k_fir_active_grp<<<(unsigned)blocks, kTPB, lds, s>>>(a, n_job, n_tiles, n_groups);
+  return hipGetLastError();
+}
+
+// Quiet star with LDS staging per mu bin: 256 pixels per block, the two rows of
+// a bin staged together, one pixel per thread.
+__global__ __launch_bounds__(kTPB) void k_fir_quiet_lds(FirArgs a) {
+  extern __shared__ double rows[];
+  long j0 = (long)blockIdx.x * kTPB;
+  int row_len = kTPB + a.nd - 1;
+  long src0 = j0 - a.dmin - a.nd + 1;
+  long j = j0 + threadIdx.x;
+  double accq = 0.0, accs = 0.0;
+  int nb = a.nbins[0];
+  for (int kk = 0; kk < nb; kk++) {
+    int k = a.bins[kk];
+    int lo = a.krange[k * 2], hi = a.krange[k * 2 + 1];
+    const double *T = a.taps + (long)k * a.nd;
+    __syncthreads();
+    for (int e = threadIdx.x; e < row_len; e += blockDim.x) {
+      long i = src0 + e;
+      bool in = i >= 0 && i < a.n;
+      rows[e] = in ? a.q[(long)k * a.n + i] : 0.0;
+      rows[row_len + e] = in ? a.qs[(long)k * a.n + i] : 0.0;
+    }
+    __syncthreads();
+    for (int di = lo; di <= hi; di++) {
+      int e = threadIdx.x + a.nd - 1 - di;
+      double hv = T[di];
+      accq = fma(hv, rows[e], accq);
+      accs = fma(hv, rows[row_len + e], accs);
+    }
+  }
+  if (j < a.n) {
+    a.out[j] = accq;
+    a.out[a.n + j] = accs;
+  }
+}
+
+hipError_t launch_fir_quiet_lds(const FirArgs &a, hipStream_t s) {
+  size_t lds = (size_t)2 * (kTPB + a.nd - 1) * sizeof(double);
+  if (lds > 65536) return launch_fir_quiet(a, s);
+  k_fir_quiet_lds<<<(unsigned)((a.n + kTPB - 1) / kTPB), kTPB, lds, s>>>(a);
   return hipGetLastError();
 }
 
@@ -468,32 +657,50 @@ hipError_t launch_phase_maps(int grid, int t_star, int t_diff_spot, int t_diff_f
 }
 
 // ------------------------------------------------------------ a7: resolution
-// o_i = sum_d c_d (1 - A_{i+d}) for wid < i < n-1-wid (conv_inst,
-// SOAP_resolution.cu:76-95), with block partial min(A) over every pixel and
-// max(o != 0) seeded at -60000 (reduceMin/MaxIdxOptimizedShared, :123-214).
-__global__ void k_conv(ConvArgs c, const double *quiet, const double *raw, const int32_t *qsel,
-                       const int32_t *variant, double *o, double *part) {
-  __shared__ double s_min[kTPB / 64], s_max[kTPB / 64];
-  int sp = blockIdx.y;
-  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const double *Q = quiet + (long)qsel[sp] * c.n;
-  const double *D = raw + (long)sp * c.n;
-  bool quiet_form = variant[sp] == 0;
-  double amin = HUGE_VAL, omax = -60000.0;
-  if (i < c.n) {
-    double Ai = quiet_form ? Q[i] : Q[i] - D[i];
-    amin = Ai;
-    double acc = 0.0;
-    if ((i > c.wid) && (i + c.wid < c.n - 1)) {
-      for (int d = c.dlo; d <= c.dhi; d++) {
-        long m = i + d;
-        double Am = quiet_form ? Q[m] : Q[m] - D[m];
-        acc += c.ctap[d - c.dlo] * (1.0 - Am);
-      }
-    }
-    o[(long)sp * c.n + i] = acc;
-    if (acc != 0.0 && omax < acc) omax = acc;
+// conv_inst (SOAP_resolution.cu:76-95) is linear in the spectrum, so with
+// A = Q - D:  o_i = sum_d c_d (1 - Q_{i+d}) + sum_d c_d D_{i+d}.  The quiet
+// term Qc = sum_d c_d (1 - Q_{i+d}) is computed once per quiet spectrum and is
+// itself the quiet-star frame (A = Q, SOAP_initialize.cu:78-96); every active
+// spectrum then needs one FIR of its delta.  Outputs exist for wid < i < n-1-wid
+// only (zero elsewhere).  Tiles: kConvB outputs per thread (odd, so the per-lane
+// LDS stride is bank-conflict free for 64-bit reads), taps in register chunks of
+// kConvCH (the host pads the tap vector with zeros to a multiple of kConvCH).
+constexpr int kConvB = 9;
+constexpr int kConvCH = 9;
+constexpr int kConvTile = kTPB * kConvB;
+constexpr int kConvMaxTaps = 4095;
+
+// stage x[i0 + dlo + e], e in [0, kConvTile + ntp - 1), into LDS; g(x) = 1 - x for
+// the quiet term, x otherwise; zero outside [0, n)
+template <bool ONE_MINUS>
+__device__ __forceinline__ void conv_stage(double *xs, const double *X, long i0, int dlo, int span, long n) {
+  for (int e = threadIdx.x; e < span; e += blockDim.x) {
+    long m = i0 + dlo + e;
+    double v = (m >= 0 && m < n) ? X[m] : 0.0;
+    xs[e] = ONE_MINUS ? 1.0 - v : v;
   }
+}
+
+// acc[b] = sum_k ctap[k] xs[tid*B + b + k], k in [0, ntp); ctap in LDS (broadcast)
+__device__ __forceinline__ void conv_accumulate(const double *xs, const double *ctap, int ntp, double acc[kConvB]) {
+#pragma unroll
+  for (int b = 0; b < kConvB; b++) acc[b] = 0.0;
+  const double *base = xs + threadIdx.x * kConvB;
+  for (int k0 = 0; k0 < ntp; k0 += kConvCH) {
+    double win[kConvCH + kConvB - 1];
+#pragma unroll
+    for (int e = 0; e < kConvCH + kConvB - 1; e++) win[e] = base[k0 + e];
+#pragma unroll
+    for (int k = 0; k < kConvCH; k++) {
+      double cv = ctap[k0 + k];
+#pragma unroll
+      for (int b = 0; b < kConvB; b++) acc[b] = fma(cv, win[b + k], acc[b]);
+    }
+  }
+}
+
+__device__ __forceinline__ void block_minmax_store(double amin, double omax, double *part_slot) {
+  __shared__ double s_min[kTPB / 64], s_max[kTPB / 64];
   for (int off = 32; off > 0; off >>= 1) {
     double a = __shfl_xor(amin, off), b = __shfl_xor(omax, off);
     amin = a < amin ? a : amin;
@@ -511,62 +718,177 @@ __global__ void k_conv(ConvArgs c, const double *quiet, const double *raw, const
       a = s_min[k] < a ? s_min[k] : a;
       b = s_max[k] > b ? s_max[k] : b;
     }
-    part[((long)sp * c.n_tiles + blockIdx.x) * 2] = a;
-    part[((long)sp * c.n_tiles + blockIdx.x) * 2 + 1] = b;
+    part_slot[0] = a;
+    part_slot[1] = b;
+  }
+  __syncthreads();
+}
+
+// Quiet terms Qc[q] (q = 0 normalised, 1 SED) and the quiet-star frame: Qc is
+// written to qc[q] and to the frame slots of lower-spectrum layout
+// [flux, bconv, tot] = [SED, normalised, SED] (run_all_SOAP.py:359-362), with the
+// partial min(Q) and max(o != 0) (seed -60000) of each tile per slot.
+__global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *quiet, double *qc, double *frame,
+                                                     double *part) {
+  extern __shared__ double xs[];
+  int q = blockIdx.y;
+  long i0 = (long)blockIdx.x * kConvTile;
+  int span = kConvTile + c.ntp - 1;
+  const double *Q = quiet + (long)q * c.n;
+  double *ct = xs + span;
+  for (int e = threadIdx.x; e < c.ntp; e += blockDim.x) ct[e] = c.ctap[e];
+  conv_stage<true>(xs, Q, i0, c.dlo, span, c.n);
+  __syncthreads();
+  double acc[kConvB];
+  conv_accumulate(xs, ct, c.ntp, acc);
+  __syncthreads();
+  // results through LDS so the global stores and loads below are coalesced
+#pragma unroll
+  for (int b = 0; b < kConvB; b++) xs[threadIdx.x * kConvB + b] = acc[b];
+  __syncthreads();
+  double amin = HUGE_VAL, omax = -60000.0;
+  int s0 = q == 0 ? 1 : 0;  // slot of this quiet spectrum; SED also fills slot 2
+  for (int e = threadIdx.x; e < kConvTile; e += blockDim.x) {
+    long i = i0 + e;
+    if (i >= c.n) break;
+    double o = ((i > c.wid) && (i + c.wid < c.n - 1)) ? xs[e] : 0.0;
+    qc[(long)q * c.n + i] = o;
+    frame[(long)s0 * c.n + i] = o;
+    if (q == 1) frame[2L * c.n + i] = o;
+    double Ai = Q[i];
+    amin = Ai < amin ? Ai : amin;
+    if (o != 0.0 && omax < o) omax = o;
+  }
+  block_minmax_store(amin, omax, part + ((long)s0 * c.n_tiles + blockIdx.x) * 2);
+  if (q == 1 && threadIdx.x == 0) {
+    part[(2L * c.n_tiles + blockIdx.x) * 2] = part[((long)s0 * c.n_tiles + blockIdx.x) * 2];
+    part[(2L * c.n_tiles + blockIdx.x) * 2 + 1] = part[((long)s0 * c.n_tiles + blockIdx.x) * 2 + 1];
   }
 }
 
-hipError_t launch_conv(const ConvArgs &c, const double *quiet, const double *raw, int n_spec, const int32_t *qsel,
-                       const int32_t *variant, double *o, double *part, hipStream_t s) {
-  if (n_spec == 0) return hipSuccess;
-  dim3 grid(c.n_tiles, n_spec);
-  k_conv<<<grid, kTPB, 0, s>>>(c, quiet, raw, qsel, variant, o, part);
+// Active spectra of phase p = blockIdx.y, types 0..2 (flux, bconv, tot):
+// o = Qc[sel] + sum_d c_d D_{i+d}; partial min(Q[sel] - D) over every pixel and
+// max(o != 0).  sel = 0 for bconv, 1 otherwise (SOAP_resolution.cu:414-416).
+__global__ __launch_bounds__(kTPB) void k_conv_active(ConvArgs c, const double *quiet, const double *qc,
+                                                      const double *raw, double *o_out, double *part) {
+  extern __shared__ double xs[];
+  int p = blockIdx.y;
+  long i0 = (long)blockIdx.x * kConvTile;
+  int span = kConvTile + c.ntp - 1;
+  double *ct = xs + span;                // taps
+  double *ob = ct + c.ntp;               // per-tile results
+  for (int e = threadIdx.x; e < c.ntp; e += blockDim.x) ct[e] = c.ctap[e];
+  for (int t = 0; t < 3; t++) {
+    int sel = t == 1 ? 0 : 1;
+    long sp = (long)p * 3 + t;
+    const double *D = raw + sp * c.n;
+    conv_stage<false>(xs, D, i0, c.dlo, span, c.n);
+    __syncthreads();
+    double acc[kConvB];
+    conv_accumulate(xs, ct, c.ntp, acc);
+#pragma unroll
+    for (int b = 0; b < kConvB; b++) ob[threadIdx.x * kConvB + b] = acc[b];
+    __syncthreads();
+    double amin = HUGE_VAL, omax = -60000.0;
+    const double *Q = quiet + (long)sel * c.n;
+    const double *QC = qc + (long)sel * c.n;
+    for (int e = threadIdx.x; e < kConvTile; e += blockDim.x) {
+      long i = i0 + e;
+      if (i >= c.n) break;
+      double o = ((i > c.wid) && (i + c.wid < c.n - 1)) ? QC[i] + ob[e] : 0.0;
+      o_out[sp * c.n + i] = o;
+      double Ai = Q[i] - xs[e - c.dlo];
+      amin = Ai < amin ? Ai : amin;
+      if (o != 0.0 && omax < o) omax = o;
+    }
+    block_minmax_store(amin, omax, part + (sp * c.n_tiles + blockIdx.x) * 2);
+  }
+}
+
+hipError_t launch_conv_quiet(const ConvArgs &c, const double *quiet, double *qc, double *frame, double *part,
+                             hipStream_t s) {
+  if (c.ntp % kConvCH || c.ntp > kConvMaxTaps) return hipErrorInvalidValue;
+  size_t lds = (size_t)(kConvTile + 2 * c.ntp - 1) * sizeof(double);
+  dim3 grid(c.n_tiles, 2);
+  k_conv_quiet<<<grid, kTPB, lds, s>>>(c, quiet, qc, frame, part);
   return hipGetLastError();
 }
 
-// spec_norm, SOAP_resolution.cu:225-232: out = 1 - o (1 - min) / max.
-__global__ void k_norm(ConvArgs c, const double *part, double *mm, double *o) {
-  __shared__ double s_min, s_max;
-  int sp = blockIdx.y;
-  if (threadIdx.x == 0) {
-    double a = part[(long)sp * c.n_tiles * 2], b = part[(long)sp * c.n_tiles * 2 + 1];
-    for (int t = 1; t < c.n_tiles; t++) {
-      double pa = part[((long)sp * c.n_tiles + t) * 2], pb = part[((long)sp * c.n_tiles + t) * 2 + 1];
-      a = pa < a ? pa : a;
-      b = pb > b ? pb : b;
-    }
-    s_min = a;
-    s_max = b;
-    if (blockIdx.x == 0) {
-      mm[sp * 2] = a;
-      mm[sp * 2 + 1] = b;
-    }
+hipError_t launch_conv_active(const ConvArgs &c, const double *quiet, const double *qc, const double *raw, int n_phase,
+                              double *o_out, double *part, hipStream_t s) {
+  if (n_phase == 0) return hipSuccess;
+  if (c.ntp % kConvCH || c.ntp > kConvMaxTaps) return hipErrorInvalidValue;
+  size_t lds = (size_t)(2 * kConvTile + 2 * c.ntp - 1) * sizeof(double);
+  dim3 grid(c.n_tiles, n_phase);
+  k_conv_active<<<grid, kTPB, lds, s>>>(c, quiet, qc, raw, o_out, part);
+  return hipGetLastError();
+}
+
+int conv_tile() { return kConvTile; }
+int conv_chunk() { return kConvCH; }
+
+// min / max of each spectrum over its tile partials (exact: order independent)
+__global__ __launch_bounds__(kTPB) void k_minmax_reduce(const double *part, int n_tiles, double *mm) {
+  __shared__ double s_min[kTPB], s_max[kTPB];
+  int sp = blockIdx.x;
+  double a = HUGE_VAL, b = -HUGE_VAL;
+  for (int t = threadIdx.x; t < n_tiles; t += blockDim.x) {
+    double pa = part[((long)sp * n_tiles + t) * 2], pb = part[((long)sp * n_tiles + t) * 2 + 1];
+    a = pa < a ? pa : a;
+    b = pb > b ? pb : b;
   }
+  s_min[threadIdx.x] = a;
+  s_max[threadIdx.x] = b;
   __syncthreads();
-  double mn = s_min, mx = s_max;
-  long stride = (long)gridDim.x * blockDim.x;
-  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < c.n; i += stride) {
-    double v = o[(long)sp * c.n + i];
-    o[(long)sp * c.n + i] = 1.0 - v * (1 - mn) / mx;
+  for (int off = kTPB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      s_min[threadIdx.x] = s_min[threadIdx.x + off] < s_min[threadIdx.x] ? s_min[threadIdx.x + off] : s_min[threadIdx.x];
+      s_max[threadIdx.x] = s_max[threadIdx.x + off] > s_max[threadIdx.x] ? s_max[threadIdx.x + off] : s_max[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    mm[sp * 2] = s_min[0];
+    mm[sp * 2 + 1] = s_max[0];
   }
 }
 
-hipError_t launch_norm(const ConvArgs &c, int n_spec, const double *part, double *mm, double *o, hipStream_t s) {
+// spec_norm, SOAP_resolution.cu:225-232: out = 1 - o (1 - min) / max, in place;
+// spectra of frame f live at o + (f*3 + t)*n, their min/max at mm[sp_of(f,t)]
+__global__ void k_norm(long n, int n_spec, const double *mm, double *o) {
+  long total = (long)n_spec * n;
+  long stride = (long)gridDim.x * blockDim.x;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+    long sp = e / n;
+    double mn = mm[sp * 2], mx = mm[sp * 2 + 1];
+    o[e] = 1.0 - o[e] * (1 - mn) / mx;
+  }
+}
+
+hipError_t launch_norm(int64_t n, int n_spec, int n_tiles, const double *part, double *mm, double *o, hipStream_t s) {
   if (n_spec == 0) return hipSuccess;
-  dim3 grid(64, n_spec);
-  k_norm<<<grid, kTPB, 0, s>>>(c, part, mm, o);
+  k_minmax_reduce<<<n_spec, kTPB, 0, s>>>(part, n_tiles, mm);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  long total = (long)n_spec * n;
+  long blocks = (total + kTPB - 1) / kTPB;
+  if (blocks > 256 * 64) blocks = 256 * 64;
+  k_norm<<<(unsigned)blocks, kTPB, 0, s>>>(n, n_spec, mm, o);
   return hipGetLastError();
 }
 
 // ------------------------------------------------------------------ a8: CCF
 // Mask of calc_mask (CCF_func.py:45-91) for every (velocity, line), log grid:
-// index arithmetic through x(l) = ln(l / l_t0)/h on the trimmed grid.
-// Layout [v][line]: start = aa-1, len = bb-aa+2, val = {fb, fa}.
-__global__ void k_mask_build(const double *line_l, int n_lines, const double *vgrid, int n_v, double lam_t0, double h,
-                             int32_t *m_start, int32_t *m_len, double *m_val) {
+// index arithmetic through x(l) = ln(l / l_t0)/h on the trimmed grid.  Tables
+// are [q][v] with q the line's position in wavelength order (line = sorted[q]):
+// start = aa-1, len = bb-aa+2, wv = {w fb, w fa}.
+__global__ void k_mask_build(const double *line_l, const double *line_w, const int32_t *sorted, int n_lines,
+                             const double *vgrid, int n_v, double lam_t0, double h, int32_t *m_start, int32_t *m_len,
+                             double2 *m_wv) {
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)n_v * n_lines) return;
-  int iv = (int)(e / n_lines), li = (int)(e % n_lines);
+  int q = (int)(e / n_v), iv = (int)(e % n_v);
+  int li = sorted[q];
   double ls = line_l[li] * (1 + vgrid[iv] / kC);
   double beta = 820. / kC;
   double hw = ls * (sqrt((1 + beta) / (1 - beta)) - 1);
@@ -576,121 +898,147 @@ __global__ void k_mask_build(const double *line_l, int n_lines, const double *vg
   double bb = ceil(xe) - 1;
   double fb = -expm1(-(aa - xb) * h) / -expm1(-h);
   double fa = fabs(expm1((xe - bb) * h)) / expm1(h);
+  double w = line_w[li];
   m_start[e] = (int32_t)aa - 1;
   m_len[e] = (int32_t)(bb - aa) + 2;
-  m_val[e * 2] = fb;
-  m_val[e * 2 + 1] = fa;
+  m_wv[e] = make_double2(w * fb, w * fa);
 }
 
-hipError_t launch_mask_build(const double *line_l, const double *line_w, int n_lines, const double *vgrid, int n_v,
-                             double lam_t0, double h, int64_t n_t, int32_t *m_start, int32_t *m_len, double *m_val,
-                             hipStream_t s) {
-  (void)line_w;
-  (void)n_t;
+hipError_t launch_mask_build(const double *line_l, const double *line_w, const int32_t *sorted, int n_lines,
+                             const double *vgrid, int n_v, double lam_t0, double h, int32_t *m_start, int32_t *m_len,
+                             double2 *m_wv, hipStream_t s) {
   long tot = (long)n_v * n_lines;
   if (tot == 0) return hipSuccess;
-  k_mask_build<<<(unsigned)((tot + kTPB - 1) / kTPB), kTPB, 0, s>>>(line_l, n_lines, vgrid, n_v, lam_t0, h, m_start,
-                                                                    m_len, m_val);
+  k_mask_build<<<(unsigned)((tot + kTPB - 1) / kTPB), kTPB, 0, s>>>(line_l, line_w, sorted, n_lines, vgrid, n_v,
+                                                                    lam_t0, h, m_start, m_len, m_wv);
   return hipGetLastError();
 }
 
-// Later lines overwrite earlier ones (CCF_func.py:85-89): pixel p of line i is
+// Later lines overwrite earlier ones (CCF_func.py:85-89): pixel p of a line is
 // alive unless a line with a larger input index writes p.  Candidates are the
-// neighbours in wavelength order whose pixel spans overlap.
-__global__ void k_mask_overwrite(const int32_t *sorted, const int32_t *rank, int n_lines, int n_v, int scan,
-                                 const int32_t *m_start, const int32_t *m_len, uint32_t *m_alive) {
+// wavelength-order neighbours whose pixel spans overlap.  Output
+// m_pk = (alive << 8) | len (len <= 24).
+__global__ void k_mask_overwrite(const int32_t *sorted, int n_lines, int n_v, int scan, const int32_t *m_start,
+                                 const int32_t *m_len, uint32_t *m_pk) {
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (long)n_v * n_lines) return;
-  int iv = (int)(e / n_lines), li = (int)(e % n_lines);
-  const int32_t *st = m_start + (long)iv * n_lines;
-  const int32_t *ln = m_len + (long)iv * n_lines;
-  int s0 = st[li], l0 = ln[li];
-  uint32_t alive = l0 >= 32 ? 0xffffffffu : ((1u << l0) - 1u);
-  int q = rank[li];
+  int q = (int)(e / n_v), iv = (int)(e % n_v);
+  const int32_t *st = m_start + iv;  // element q' at st[q' * n_v]
+  const int32_t *ln = m_len + iv;
+  int s0 = st[(long)q * n_v], l0 = ln[(long)q * n_v];
+  int li = sorted[q];
+  uint32_t alive = (1u << l0) - 1u;
   for (int dir = -1; dir <= 1; dir += 2) {
     for (int step = 1; step <= scan; step++) {
       int qq = q + dir * step;
       if (qq < 0 || qq >= n_lines) break;
-      int lj = sorted[qq];
-      int s1 = st[lj], l1 = ln[lj];
+      int s1 = st[(long)qq * n_v], l1 = ln[(long)qq * n_v];
       if (s1 + l1 - 1 < s0 || s1 > s0 + l0 - 1) continue;
-      if (lj <= li) continue;
+      if (sorted[qq] <= li) continue;
       for (int b = 0; b < l0; b++) {
         int p = s0 + b;
         if (p >= s1 && p <= s1 + l1 - 1) alive &= ~(1u << b);
       }
     }
   }
-  m_alive[e] = alive;
+  m_pk[e] = (alive << 8) | (uint32_t)l0;
 }
 
-hipError_t launch_mask_overwrite(const int32_t *sorted, const int32_t *rank, int n_lines, int n_v, int scan,
-                                 const int32_t *m_start, const int32_t *m_len, uint32_t *m_alive, hipStream_t s) {
+hipError_t launch_mask_overwrite(const int32_t *sorted, int n_lines, int n_v, int scan, const int32_t *m_start,
+                                 const int32_t *m_len, uint32_t *m_pk, hipStream_t s) {
   long tot = (long)n_v * n_lines;
   if (tot == 0) return hipSuccess;
-  k_mask_overwrite<<<(unsigned)((tot + kTPB - 1) / kTPB), kTPB, 0, s>>>(sorted, rank, n_lines, n_v, scan, m_start,
-                                                                        m_len, m_alive);
+  k_mask_overwrite<<<(unsigned)((tot + kTPB - 1) / kTPB), kTPB, 0, s>>>(sorted, n_lines, n_v, scan, m_start, m_len,
+                                                                        m_pk);
   return hipGetLastError();
 }
 
-// calculate_CCF_2 (CCF_func.py:27-36) on the sparse mask: one block per
-// (spectrum, velocity), lines over threads, fixed-order block reduction.
-__global__ void k_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_lines, int n_v, int64_t n_t,
-                      const int32_t *m_start, const int32_t *m_len, const uint32_t *m_alive, const double *m_val,
-                      const double *line_w, double *ccf) {
-  __shared__ double red[kTPB];
-  int sp = blockIdx.y, iv = blockIdx.x;
-  const double *S = spec + (long)sp * spec_stride + trim;
-  double acc = 0.0;
-  for (int li = threadIdx.x; li < n_lines; li += blockDim.x) {
-    long e = (long)iv * n_lines + li;
-    int s0 = m_start[e], l0 = m_len[e];
-    uint32_t al = m_alive[e];
-    double w = line_w[li];
-    double wfb = w * m_val[e * 2], wfa = w * m_val[e * 2 + 1];
-    int bb = s0 + l0 - 1;
-    for (int b = 0; b < l0; b++) {
-      long p = s0 + b;
-      if (!((al >> b) & 1u) || p < 0 || p >= n_t) continue;
-      double val = (p == bb) ? wfa : ((b == 0) ? wfb : w);
-      acc += S[p] * val;
+// calculate_CCF_2 (CCF_func.py:27-36) on the sparse mask.  Block = (chunk of
+// kCcfLines lines in wavelength order, chunk of kCcfSpec spectra); thread = one
+// velocity, summing its lines in order for all spectra of the chunk (each mask
+// entry is read once per block).  Partials per line chunk are summed in order
+// by k_ccf_reduce, which also normalises by the maximum and appodises
+// (run_all_SOAP.py:368-370).
+constexpr int kCcfLines = 32;
+constexpr int kCcfSpec = 8;
+
+__global__ void k_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
+                      int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
+                      const double *w_sorted, double *part) {
+  int chunk = blockIdx.x;
+  int s0 = blockIdx.y * kCcfSpec;
+  int ns = min(kCcfSpec, n_spec - s0);
+  int q0 = chunk * kCcfLines, q1 = min(n_lines, q0 + kCcfLines);
+  const double *S[kCcfSpec];
+#pragma unroll
+  for (int k = 0; k < kCcfSpec; k++) S[k] = spec + (long)(s0 + (k < ns ? k : 0)) * spec_stride + trim;
+  for (int iv = threadIdx.x; iv < n_v; iv += blockDim.x) {
+    double acc[kCcfSpec];
+#pragma unroll
+    for (int k = 0; k < kCcfSpec; k++) acc[k] = 0.0;
+    for (int q = q0; q < q1; q++) {
+      long e = (long)q * n_v + iv;
+      int st = m_start[e];
+      uint32_t pk = m_pk[e];
+      double2 wv = m_wv[e];
+      double w = w_sorted[q];
+      int ln = (int)(pk & 0xffu);
+      uint32_t al = pk >> 8;
+      for (int b = 0; b < ln; b++) {
+        long pp = st + b;
+        if (!((al >> b) & 1u) || pp < 0 || pp >= n_t) continue;
+        double val = (b == ln - 1) ? wv.y : ((b == 0) ? wv.x : w);
+#pragma unroll
+        for (int k = 0; k < kCcfSpec; k++) acc[k] = fma(S[k][pp], val, acc[k]);
+      }
     }
+    for (int k = 0; k < ns; k++) part[((long)chunk * n_spec + s0 + k) * n_v + iv] = acc[k];
   }
-  red[threadIdx.x] = acc;
+}
+
+__global__ void k_ccf_reduce(const double *part, int n_chunks, int n_spec, int n_v, const double *appod, double *ccf) {
+  __shared__ double s_red[1024];
+  int sp = blockIdx.x;
+  double v = -HUGE_VAL;
+  for (int iv = threadIdx.x; iv < n_v; iv += blockDim.x) {
+    double a = 0.0;
+    for (int c = 0; c < n_chunks; c++) a += part[((long)c * n_spec + sp) * n_v + iv];
+    ccf[(long)sp * n_v + iv] = a;
+  }
   __syncthreads();
-  for (int off = kTPB / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+  // max over velocities (np.max), then normalise and appodise
+  for (int iv = threadIdx.x; iv < n_v; iv += blockDim.x) v = fmax(v, ccf[(long)sp * n_v + iv]);
+  s_red[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) s_red[threadIdx.x] = fmax(s_red[threadIdx.x], s_red[threadIdx.x + off]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) ccf[(long)sp * n_v + iv] = red[0];
-}
-
-hipError_t launch_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
-                      int64_t n_t, const int32_t *m_start, const int32_t *m_len, const uint32_t *m_alive,
-                      const double *m_val, const double *line_w, double *ccf, hipStream_t s) {
-  if (n_spec == 0) return hipSuccess;
-  dim3 grid(n_v, n_spec);
-  k_ccf<<<grid, kTPB, 0, s>>>(spec, spec_stride, trim, n_lines, n_v, n_t, m_start, m_len, m_alive, m_val, line_w,
-                              ccf);
-  return hipGetLastError();
-}
-
-// CCF /= max(CCF); CCF = 1 - ((-CCF + 1) * appod), run_all_SOAP.py:368-370.
-__global__ void k_ccf_finish(double *ccf, int n_v, const double *appod) {
-  int sp = blockIdx.x;
-  double *c = ccf + (long)sp * n_v;
-  double mx = c[0];
-  for (int i = 1; i < n_v; i++) mx = c[i] > mx ? c[i] : mx;
-  __syncthreads();
-  for (int i = threadIdx.x; i < n_v; i += blockDim.x) {
-    double cc = c[i] / mx;
-    c[i] = 1 - ((-cc + 1) * appod[i]);
+  double mx = s_red[0];
+  for (int iv = threadIdx.x; iv < n_v; iv += blockDim.x) {
+    double cc = ccf[(long)sp * n_v + iv] / mx;
+    ccf[(long)sp * n_v + iv] = 1 - ((-cc + 1) * appod[iv]);
   }
 }
 
-hipError_t launch_ccf_finish(double *ccf, int n_spec, int n_v, const double *appod, hipStream_t s) {
+int ccf_chunks(int n_lines) { return (n_lines + kCcfLines - 1) / kCcfLines; }
+
+hipError_t launch_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
+                      int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
+                      const double *w_sorted, const double *appod, double *part, double *ccf, hipStream_t s) {
   if (n_spec == 0) return hipSuccess;
-  k_ccf_finish<<<n_spec, 64, 0, s>>>(ccf, n_v, appod);
+  if (n_v > 1024) return hipErrorInvalidValue;
+  int nc = ccf_chunks(n_lines);
+  int threads = ((n_v + 63) / 64) * 64;
+  dim3 grid(nc, (n_spec + kCcfSpec - 1) / kCcfSpec);
+  k_ccf<<<grid, threads, 0, s>>>(spec, spec_stride, trim, n_spec, n_lines, n_v, n_t, m_start, m_pk, m_wv, w_sorted,
+                                 part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  int rt = 1;
+  while (rt < n_v && rt < 1024) rt <<= 1;
+  rt = rt < 64 ? 64 : rt;
+  k_ccf_reduce<<<n_spec, rt, 0, s>>>(part, nc, n_spec, n_v, appod, ccf);
   return hipGetLastError();
 }
 

@@ -105,7 +105,7 @@ def load(path: str = LIB_PATH):
         "soap_get_stage_ms": (ctypes.c_int, [vp, _dp, ctypes.c_int32]),
         "soap_stage_name": (ctypes.c_char_p, [ctypes.c_int32]),
         "soap_num_stages": (ctypes.c_int32, []),
-        "soap_get_integrate_stats": (ctypes.c_int, [vp, _dp, ctypes.POINTER(ctypes.c_int64)]),
+        "soap_get_integrate_stats": (ctypes.c_int, [vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -336,10 +336,13 @@ class Engine:
         return {self.L.soap_stage_name(i).decode(): float(o[i]) for i in range(ns)}
 
     def integrate_stats(self):
+        """(bytes_min, bytes_phase_sum, rows_sum) of the last integrate."""
         b = ctypes.c_double()
+        bp = ctypes.c_double()
         r = ctypes.c_int64()
-        self._chk(self.L.soap_get_integrate_stats(self.ctx, ctypes.byref(b), ctypes.byref(r)), "integrate_stats")
-        return float(b.value), int(r.value)
+        self._chk(self.L.soap_get_integrate_stats(self.ctx, ctypes.byref(b), ctypes.byref(bp), ctypes.byref(r)),
+                  "integrate_stats")
+        return float(b.value), float(bp.value), int(r.value)
 
 
 def setup_workload(eng: Engine, w, with_mask: bool = True, trim: int = 210):
