@@ -134,6 +134,9 @@ int soap_set_activity(soap_ctx *ctx, const soap_activity *act, int32_t n_phase, 
 int soap_integrate(soap_ctx *ctx);
 /* raw spectra of one phase to host (_Spec_{flux,bconv,tot}_{id}_.bin); NULL skips */
 int soap_get_raw(soap_ctx *ctx, int32_t phase, double *flux, double *bconv, double *tot);
+/* declare n_phase spectrum slots without region geometry, for callers that
+ * bring their own raw spectra (the phase loop of SOAP_resolution.cu:379-428) */
+int soap_alloc_phases(soap_ctx *ctx, int32_t n_phase);
 /* raw spectra of one phase from host (SOAP_resolution.cu:398-412) */
 int soap_set_raw(soap_ctx *ctx, int32_t phase, const double *flux, const double *bconv, const double *tot);
 /* diagnostic temperature / binary maps of one phase (_temp_map_, _spot_binary_map_,

@@ -48,18 +48,25 @@ def build_lib(verbose: bool = False, out: str = LIB, defines=()) -> str:
     return out
 
 
+# executables keep the names of the reference's stages (its os.system calls)
+TOOLS = {
+    "SOAP_initialize": ("cli_initialize.cpp", []),
+    "SOAP_integration": ("cli_integration.cpp", []),
+    "SOAP_integration_solar": ("cli_integration.cpp", ["-DSOAP_SOLAR_MODE=1"]),
+    "SOAP_resolution": ("cli_resolution.cpp", []),
+    "BIS_FIT2": ("cli_bisfit.cpp", []),
+    "soap_run": ("soap_run.cpp", []),
+}
+
+
 def build_tools(verbose: bool = False) -> list[str]:
-    """C++ host driver and CLI shims (link against libsoapgpu.so)."""
+    """C++ host driver and CLI shims (g++, linked against libsoapgpu.so)."""
     drv = os.path.join(HERE, "driver")
-    if not os.path.isdir(drv):
-        return []
     os.makedirs(BIN, exist_ok=True)
     out = []
-    for src in sorted(os.listdir(drv)):
-        if not src.endswith(".cpp"):
-            continue
-        exe = os.path.join(BIN, src[:-4])
-        _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", drv,
+    for exe_name, (src, extra) in TOOLS.items():
+        exe = os.path.join(BIN, exe_name)
+        _run(["g++", "-O2", "-std=c++17", "-Wall", *extra, "-I", os.path.join(ROOT, "include"), "-I", drv,
               os.path.join(drv, src), "-o", exe, "-L", HERE, "-lsoapgpu",
               "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + HERE])
         out.append(exe)

@@ -81,6 +81,9 @@ struct soap_ctx {
   RegionDev *d_geo = nullptr;
   int32_t *d_geo_off = nullptr, *d_box = nullptr;
   int max_col_blocks = 0;
+  // solar mode: Lagrange nodes over the facula temperature range
+  int solar_k = 0, t_norm_fac = 0;
+  double solar_t[kMaxSolarNodes] = {}, solar_w[kMaxSolarNodes] = {};
   double *d_raw = nullptr;  // [P][3][n]
   bool have_raw = false;
   // tap scratch
@@ -594,7 +597,12 @@ static int ensure_taps(soap_ctx *c, int n_job, int nsets_in, int nsets_out) {
   return SOAP_OK;
 }
 
-static int tap_window(soap_ctx *c) { return std::min(16, c->nd); }
+// LDS window of tap offsets per (set, bin), kept within 64 KiB of LDS
+static int tap_window(soap_ctx *c, int nsets) {
+  int wd = std::min(16, c->nd);
+  while (wd > 1 && (size_t)nsets * c->n_mu * wd * 16 > 65536) wd--;
+  return wd;
+}
 
 int soap_quiet_star(soap_ctx *c) {
   if (!c) return SOAP_ERR_ARG;
@@ -611,7 +619,7 @@ int soap_quiet_star(soap_ctx *c) {
   a.dmin = c->dmin;
   a.nd = c->nd;
   a.nsets = 1;
-  a.wd = tap_window(c);
+  a.wd = tap_window(c, 1);
   a.cols_per_block = std::max(1, 4096 / std::max(1, c->G));
   a.region = c->d_region;
   a.limb = c->d_limb;
@@ -673,13 +681,68 @@ int soap_set_quiet(soap_ctx *c, int which, const double *q) {
   return SOAP_OK;
 }
 
+// Solar mode (SOAP_integration_solar.cu:424-433): facula intensity
+// B(lambda, T_cell)/B(lambda, T_norm) C_fac(lambda), T_norm = int(t_sum + 0.9 - 216.8),
+// with T_cell = int(t_sum + 0.9 - 407.7 mu + 190.9 mu^2) in [t_sum - 216, t_sum].
+// The kernels use the Lagrange interpolant of the ratio in T on K Chebyshev
+// nodes; K is the smallest of 6, 8, 10, 12 whose interpolant matches the ratio
+// to 2e-14 relative at every integer T of the range, at 9 wavelengths across the
+// grid (the error is largest at the shortest wavelength).
+static double planck_ratio(double lam, double t, int t_norm) {
+  double hc = kPlanckH * kC;
+  return std::expm1(hc / (lam * kBoltzK * (double)t_norm * 1e-10)) / std::expm1(hc / (lam * kBoltzK * t * 1e-10));
+}
+
+static int solar_nodes(soap_ctx *c, int t_sum) {
+  int t_norm = (int)(t_sum + 0.9 - 216.8);
+  double lo = t_sum - 216, hi = t_sum;
+  if (lo < 500) return fail(c, SOAP_ERR_ARG, "set_activity: facula temperature range [%g, %g] K is not physical", lo, hi);
+  for (int K = 6; K <= kMaxSolarNodes; K += 2) {
+    double x[kMaxSolarNodes], w[kMaxSolarNodes];
+    for (int q = 0; q < K; q++) x[q] = 0.5 * (lo + hi) + 0.5 * (hi - lo) * std::cos((2 * q + 1) * kPi / (2 * K));
+    for (int q = 0; q < K; q++) {
+      w[q] = 1.0;
+      for (int r = 0; r < K; r++)
+        if (r != q) w[q] /= x[q] - x[r];
+    }
+    double err = 0;
+    for (int sidx = 0; sidx < 9; sidx++) {
+      double lam = c->wave[(size_t)((c->n - 1) * sidx / 8)];
+      double R[kMaxSolarNodes];
+      for (int q = 0; q < K; q++) R[q] = planck_ratio(lam, x[q], t_norm);
+      for (int T = t_sum - 216; T <= t_sum; T++) {
+        double v = 0;
+        for (int q = 0; q < K; q++) {
+          double l = w[q];
+          for (int r = 0; r < K; r++)
+            if (r != q) l *= T - x[r];
+          v += R[q] * l;
+        }
+        double ex = planck_ratio(lam, T, t_norm);
+        err = std::max(err, std::fabs(v - ex) / std::fabs(ex));
+      }
+    }
+    if (err <= 2e-14) {
+      c->solar_k = K;
+      c->t_norm_fac = t_norm;
+      std::copy(x, x + K, c->solar_t);
+      std::copy(w, w + K, c->solar_w);
+      return SOAP_OK;
+    }
+  }
+  return fail(c, SOAP_ERR_LIMIT, "set_activity: Planck-ratio interpolation over [%g, %g] K misses 2e-14 at %d nodes", lo,
+              hi, kMaxSolarNodes);
+}
+
 int soap_set_activity(soap_ctx *c, const soap_activity *act, int32_t n_phase, const double *phase,
                       const int32_t *region_count, const soap_region *regions) {
   if (!c || !act || n_phase < 0 || (n_phase > 0 && (!phase || !region_count)))
     return fail(c, SOAP_ERR_ARG, "set_activity: bad arguments");
   if (!c->have_maps) return fail(c, SOAP_ERR_STATE, "set_activity: run disk_init first");
   if (act->nrho < 2) return fail(c, SOAP_ERR_ARG, "set_activity: nrho must be >= 2");
-  if (act->solar_mode) return fail(c, SOAP_ERR_LIMIT, "set_activity: solar (Planck-ratio facula) mode not built");
+  int rc;
+  if (act->solar_mode && (rc = solar_nodes(c, act->t_star + act->t_diff_fac))) return rc;
+  if (!act->solar_mode) c->solar_k = 0;
   HIPCHK(c, hipSetDevice(c->device));
   int64_t total = 0;
   for (int p = 0; p < n_phase; p++) {
@@ -734,7 +797,6 @@ int soap_set_activity(soap_ctx *c, const soap_activity *act, int32_t n_phase, co
     c->box.insert(c->box.end(), {by0, by1, bz0, bz1});
     c->max_col_blocks = std::max(c->max_col_blocks, (by1 - by0 + cpb - 1) / cpb);
   }
-  int rc;
   if ((rc = dalloc(c, &c->d_geo, std::max<size_t>(1, c->geo.size()))) ||
       (rc = dalloc(c, &c->d_geo_off, c->geo_off.size())) || (rc = dalloc(c, &c->d_box, std::max<size_t>(4, c->box.size()))))
     return rc;
@@ -766,14 +828,24 @@ int soap_integrate(soap_ctx *c) {
     return SOAP_OK;
   }
   HIPCHK(c, hipSetDevice(c->device));
-  if ((rc = ensure_taps(c, c->P, 3, 4))) return rc;
+  // sets: H1s, H1f, H2f ((1+LB) limb) -> + H1a; solar: H1s, H1f, L_0..L_{K-1} -> + H1a
+  const bool solar = c->act.solar_mode != 0;
+  const int nsets = solar ? 2 + c->solar_k : 3;
+  if (solar && c->solar_k == 0) return fail(c, SOAP_ERR_STATE, "integrate: solar nodes missing");
+  if ((rc = ensure_taps(c, c->P, nsets, nsets + 1))) return rc;
   TapAccArgs a{};
   a.grid = c->G;
   a.n_mu = c->n_mu;
   a.dmin = c->dmin;
   a.nd = c->nd;
-  a.nsets = 3;
-  a.wd = tap_window(c);
+  a.nsets = nsets;
+  a.wd = tap_window(c, nsets);
+  if (solar) {
+    a.solar_nodes = c->solar_k;
+    a.t_sum = c->act.t_star + c->act.t_diff_fac;
+    std::copy(c->solar_t, c->solar_t + c->solar_k, a.node_t);
+    std::copy(c->solar_w, c->solar_w + c->solar_k, a.node_w);
+  }
   a.cols_per_block = std::max(1, 4096 / std::max(1, c->G));
   a.region = c->d_region;
   a.limb = c->d_limb;
@@ -787,9 +859,9 @@ int soap_integrate(soap_ctx *c) {
   a.box = c->d_box;
   {
     StageTimer tm(c, ST_TAPS_ACTIVE);
-    HIPCHK(c, launch_fill_u64(a.acc, (size_t)c->P * 3 * c->n_mu * c->nd * 2, c->stream));
+    HIPCHK(c, launch_fill_u64(a.acc, (size_t)c->P * nsets * c->n_mu * c->nd * 2, c->stream));
     HIPCHK(c, launch_taps_active(a, c->P, c->max_col_blocks, c->stream));
-    HIPCHK(c, launch_taps_finalize(a.acc, c->P, 3, 4, c->n_mu, c->nd, (double *)c->taps.p, (int32_t *)c->krange.p,
+    HIPCHK(c, launch_taps_finalize(a.acc, c->P, nsets, nsets + 1, c->n_mu, c->nd, (double *)c->taps.p, (int32_t *)c->krange.p,
                                    (int32_t *)c->kflags.p, (int32_t *)c->bins.p, (int32_t *)c->nbins.p, c->stream));
   }
   FirArgs f{};
@@ -811,9 +883,21 @@ int soap_integrate(soap_ctx *c) {
   f.out = c->d_raw;
   {
     StageTimer tm(c, ST_FIR_ACTIVE);
-    hipError_t e = launch_fir_active_grp(f, c->P, c->stream);
-    if (e == hipErrorInvalidValue) e = launch_fir_active(f, c->P, c->stream);  // very wide shift ranges
-    HIPCHK(c, e);
+    if (solar) {
+      SolarFir sf{};
+      sf.wave = c->d_wave;
+      sf.k = c->solar_k;
+      sf.t_norm = c->t_norm_fac;
+      std::copy(c->solar_t, c->solar_t + c->solar_k, sf.node_t);
+      hipError_t e = launch_fir_solar(f, sf, c->P, c->stream);
+      if (e == hipErrorInvalidValue)
+        return fail(c, SOAP_ERR_LIMIT, "integrate: solar mode needs n_mu <= 64 and a shift range of < 7000 pixels");
+      HIPCHK(c, e);
+    } else {
+      hipError_t e = launch_fir_active_grp(f, c->P, c->stream);
+      if (e == hipErrorInvalidValue) e = launch_fir_active(f, c->P, c->stream);  // very wide shift ranges
+      HIPCHK(c, e);
+    }
   }
   c->have_raw = true;
   c->have_lower = c->have_ccf = c->have_fit = false;
@@ -865,6 +949,24 @@ int soap_get_raw(soap_ctx *c, int32_t phase, double *flux, double *bconv, double
       HIPCHK(c, hipMemcpyAsync(outs[t], c->d_raw + ((size_t)phase * 3 + t) * c->n, c->n * 8, hipMemcpyDeviceToHost,
                                c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SOAP_OK;
+}
+
+int soap_alloc_phases(soap_ctx *c, int32_t n_phase) {
+  if (!c || n_phase < 0) return fail(c, SOAP_ERR_ARG, "alloc_phases: bad arguments");
+  if (c->n == 0) return fail(c, SOAP_ERR_STATE, "alloc_phases: set the wavelength grid first");
+  HIPCHK(c, hipSetDevice(c->device));
+  int rc = dalloc(c, &c->d_raw, (size_t)std::max(1, n_phase) * 3 * c->n);
+  if (rc) return rc;
+  HIPCHK(c, launch_fill_f64(c->d_raw, (size_t)std::max(1, n_phase) * 3 * c->n, 0.0, c->stream));
+  c->P = n_phase;
+  c->phases.assign(n_phase, 0.0);
+  c->reg_count.assign(n_phase, 0);
+  c->regions.clear();
+  c->geo.clear();
+  c->geo_off.assign(n_phase + 1, 0);
+  c->have_raw = true;
+  c->have_lower = c->have_ccf = c->have_fit = false;
   return SOAP_OK;
 }
 

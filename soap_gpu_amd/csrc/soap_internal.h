@@ -8,6 +8,10 @@ namespace soap {
 
 constexpr double kC = 299792458.;
 constexpr double kPi = 3.141592653589793238463;
+constexpr int kMaxSolarNodes = 12;
+// loi_Planck constants (SOAP_integration.cu:205-212)
+constexpr double kPlanckH = 6.62606896e-34;
+constexpr double kBoltzK = 1.380e-23;
 
 // Region geometry of one (phase, region), evaluated on the host once per phase
 // (spot_init_gpu/spot_phase_gpu/spot_area + the first row of spot_inverse_rotation).
@@ -45,6 +49,11 @@ struct TapAccArgs {
   const RegionDev *geo;
   const int32_t *geo_off;   // [n_job + 1]
   const int32_t *box;       // [n_job][4]: cy0, cy1, cz0, cz1
+  // solar mode (facula intensity by Planck ratio): facula cells also add
+  // limb * l_q(T_cell) to set 2+q, l_q the Lagrange basis on node_t
+  int32_t solar_nodes;      // K, 0 = off
+  int32_t t_sum;            // t_star + t_diff_fac
+  double node_t[kMaxSolarNodes], node_w[kMaxSolarNodes];  // nodes, barycentric weights
 };
 
 struct FirArgs {
@@ -57,6 +66,15 @@ struct FirArgs {
   const int32_t *bins;   // [job][n_mu]
   const int32_t *nbins;  // [job]
   double *out;           // quiet: [2][n]; active: [job][3][n]
+};
+
+// Solar-mode epilogue: the Planck ratio B(lambda, T)/B(lambda, t_norm) at the
+// K temperature nodes multiplies the Lagrange-set FIR sums.
+struct SolarFir {
+  const double *wave;  // [n]
+  int32_t k;           // number of nodes
+  int32_t t_norm;
+  double node_t[kMaxSolarNodes];
 };
 
 struct ConvArgs {
@@ -85,6 +103,8 @@ hipError_t launch_fir_active(const FirArgs &a, int n_job, hipStream_t s);
 hipError_t launch_fir_quiet_lds(const FirArgs &a, hipStream_t s);
 // phase-group form (bin outer, phases inner, LDS-staged rows and taps)
 hipError_t launch_fir_active_grp(const FirArgs &a, int n_job, hipStream_t s);
+// solar mode: taps [job][n_mu][nd][K+3] = H1s, H1f, L_0..L_{K-1}, H1a
+hipError_t launch_fir_solar(const FirArgs &a, const SolarFir &sf, int n_job, hipStream_t s);
 hipError_t launch_phase_maps(int grid, int t_star, int t_diff_spot, int t_diff_fac, const RegionDev *geo,
                              int n_geo, double *temp, double *bi, hipStream_t s);
 // resolution

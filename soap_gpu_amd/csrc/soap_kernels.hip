@@ -237,9 +237,21 @@ __device__ __forceinline__ int classify_cell(int iy, int iz, double delta_grid, 
   return res;
 }
 
+// Facula temperature of a cell at r_cos (spot_scan, SOAP_integration.cu:376).
+__device__ __forceinline__ int plage_temp(int t_sum, double rc) {
+#pragma clang fp contract(off)
+  return (int)(t_sum + 0.9 - 407.7 * rc + 190.9 * (rc * rc));
+}
+
 // Active taps of every phase (job = phase): spot cells add limb to set 0,
 // facula cells add limb to set 1 and (1+LB) limb to set 2 (the intensity factor
 // of fast_intensity_fac, SOAP_integration.cu:317-323,424-440).
+// Solar mode (SOAP_integration_solar.cu:424-433): the facula intensity is
+// B(lambda, T_cell)/B(lambda, T_norm) C_fac(lambda), a smooth function of T_cell
+// over [t_sum - 216, t_sum].  It is replaced by its Lagrange interpolant on K
+// Chebyshev nodes T_q (the host checks the interpolation error), so facula cells
+// add limb to set 1 and limb * l_q(T_cell) to set 2+q; the FIR epilogue then
+// weights set 2+q by B(lambda, T_q)/B(lambda, T_norm).
 __global__ void k_taps_active(TapAccArgs a) {
   extern __shared__ unsigned long long lds_u64[];
   __shared__ int scratch[16];
@@ -275,12 +287,24 @@ __global__ void k_taps_active(TapAccArgs a) {
     if (ty == 0) {
       w.add(0, k, m, L * (1.0 - t));
       w.add(0, k, m - 1, L * t);
-    } else {
+    } else if (a.solar_nodes == 0) {
       double L2 = (1 + a.lbm[idx]) * L;
       w.add(1, k, m, L * (1.0 - t));
       w.add(1, k, m - 1, L * t);
       w.add(2, k, m, L2 * (1.0 - t));
       w.add(2, k, m - 1, L2 * t);
+    } else {
+      w.add(1, k, m, L * (1.0 - t));
+      w.add(1, k, m - 1, L * t);
+      double T = (double)plage_temp(a.t_sum, rc);
+      for (int q = 0; q < a.solar_nodes; q++) {
+        double l = a.node_w[q];
+        for (int r = 0; r < a.solar_nodes; r++)
+          if (r != q) l *= T - a.node_t[r];
+        double Lq = L * l;
+        w.add(2 + q, k, m, Lq * (1.0 - t));
+        w.add(2 + q, k, m - 1, Lq * t);
+      }
     }
   }
   flush_window(w, a.nsets);
@@ -295,8 +319,9 @@ hipError_t launch_taps_active(const TapAccArgs &a, int n_job, int max_col_blocks
 }
 
 // Fixed point -> double taps, per-(job, bin) nonzero d range, flags, bin list.
-// nsets_out = nsets_in + 1 for the active taps: set 3 = set 0 + set 1 (all
-// active cells, weight limb), used on the SED quiet rows.
+// Active taps get one more output set than they accumulate: the last one is
+// set 0 + set 1 (all active cells, weight limb), used on the SED quiet rows.
+// Flags: bit 0 the spot set is nonzero, bit 1 a facula set is nonzero.
 __global__ void k_taps_finalize(const unsigned long long *acc, int nsets_in, int nsets_out, int n_mu, int nd,
                                 double *taps, int32_t *krange, int32_t *kflags, int32_t *bins, int32_t *nbins) {
   int job = blockIdx.x;
@@ -304,24 +329,20 @@ __global__ void k_taps_finalize(const unsigned long long *acc, int nsets_in, int
   for (int k = threadIdx.x; k < n_mu; k += blockDim.x) {
     int lo = nd, hi = -1, fl = 0;
     for (int d = 0; d < nd; d++) {
-      double v[4] = {0, 0, 0, 0};
+      bool nz = false;
+      double v01 = 0.0;
+      double *out = taps + (((long)job * n_mu + k) * nd + d) * nsets_out;
       for (int st = 0; st < nsets_in; st++) {
         long e = ((((long)job * nsets_in + st) * n_mu + k) * nd + d) * 2;
-        v[st] = fx_value(acc[e], acc[e + 1]);
+        double v = fx_value(acc[e], acc[e + 1]);
+        out[st] = v;
+        if (st < 2) v01 += v;
+        if (v != 0.0) {
+          nz = true;
+          fl |= (st == 0 || nsets_in == 1) ? 1 : 2;
+        }
       }
-      if (nsets_out > nsets_in) v[nsets_in] = v[0] + v[1];
-      bool nz = false;
-      for (int st = 0; st < nsets_out; st++) {
-        // quiet (1 set): [k][nd]; active (4 sets): [k][nd][4], sets contiguous
-        taps[(((long)job * n_mu + k) * nd + d) * nsets_out + st] = v[st];
-        if (v[st] != 0.0) nz = true;
-      }
-      if (nsets_in == 3) {
-        if (v[0] != 0.0) fl |= 1;
-        if (v[1] != 0.0 || v[2] != 0.0) fl |= 2;
-      } else if (v[0] != 0.0) {
-        fl |= 1;
-      }
+      if (nsets_out > nsets_in) out[nsets_in] = v01;
       if (nz) {
         lo = d < lo ? d : lo;
         hi = d;
@@ -582,6 +603,120 @@ hipError_t launch_fir_active_grp(const FirArgs &a, int n_job, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Solar-mode active deltas, one phase per block (taps [job][n_mu][nd][K+3]:
+// H1s, H1f, L_0..L_{K-1}, H1a).  With R_q = B(lambda, T_q)/B(lambda, T_norm):
+//   flux  = H1a*SEDq - Cs (H1s*Sq) - Cf sum_q R_q (L_q*Sq)
+//   bconv = H1s*Sq + H1f*Sq - H1s*Ssp - H1f*Sfac
+//   tot   = H1a*SEDq - Cs (H1s*Ssp) - Cf sum_q R_q (L_q*Sfac)
+// Rows are staged in LDS per mu bin as in k_fir_active_grp; blocks of one tile
+// run on one XCD.
+template <int K>
+__global__ __launch_bounds__(kTPB) void k_fir_solar(FirArgs a, SolarFir sf, int n_job, int n_tiles) {
+  constexpr int NS = K + 3;
+  extern __shared__ double rows[];
+  __shared__ int s_bins[64];
+  __shared__ int s_nb;
+  long b = blockIdx.x;
+  long xcd = b % 8, slot = b / 8;
+  long tile = (slot / n_job) * 8 + xcd;
+  int job = (int)(slot % n_job);
+  if (tile >= n_tiles) return;  // uniform over the block
+  int nd = a.nd, n_mu = a.n_mu;
+  int row_len = kFirTile + nd - 1;
+  long j0 = tile * kFirTile;
+  long src0 = j0 - a.dmin - nd + 1;
+  const long jb = (long)job * n_mu;
+  if (threadIdx.x == 0) {
+    int c = 0;
+    for (int k = 0; k < n_mu; k++)
+      if (a.kflags[jb + k] && a.krange[(jb + k) * 2] <= a.krange[(jb + k) * 2 + 1]) s_bins[c++] = k;
+    s_nb = c;
+  }
+  __syncthreads();
+  double F0 = 0, F1 = 0, F4 = 0, Fq = 0, Ff = 0, Gq[K], Gf[K];
+#pragma unroll
+  for (int q = 0; q < K; q++) Gq[q] = Gf[q] = 0.0;
+  const int nb = uni(s_nb);
+  for (int kk = 0; kk < nb; kk++) {
+    const int k = uni(s_bins[kk]);
+    const int fl = uni(a.kflags[jb + k]);
+    const int lo = uni(a.krange[(jb + k) * 2]), hi = uni(a.krange[(jb + k) * 2 + 1]);
+    if (kk > 0) __syncthreads();
+    const int has_sp = fl & 1, has_fa = (fl >> 1) & 1;
+    const int nrow = 2 + has_sp + has_fa;
+    const double *src[4] = {a.q + (long)k * a.n, a.qs + (long)k * a.n,
+                            has_sp ? a.sp + (long)k * a.n : a.fa + (long)k * a.n, a.fa + (long)k * a.n};
+    for (int e = threadIdx.x; e < nrow * row_len; e += blockDim.x) {
+      int r = e / row_len, x = e - r * row_len;
+      long i = src0 + x;
+      rows[e] = (i >= 0 && i < a.n) ? src[r][i] : 0.0;
+    }
+    __syncthreads();
+    const double *Tk = a.taps + (jb + k) * nd * NS;
+    const double *rq = rows, *rs = rows + row_len, *rp = rows + 2 * row_len, *rf = rows + (2 + has_sp) * row_len;
+    for (int di = lo; di <= hi; di++) {
+      int e = threadIdx.x + nd - 1 - di;
+      const double *T = Tk + di * NS;
+      double vq = rq[e];
+      F0 = fma(T[NS - 1], rs[e], F0);
+      if (has_sp) {
+        F1 = fma(T[0], vq, F1);
+        F4 = fma(T[0], rp[e], F4);
+      }
+      if (has_fa) {
+        double vf = rf[e];
+        Fq = fma(T[1], vq, Fq);
+        Ff = fma(T[1], vf, Ff);
+#pragma unroll
+        for (int q = 0; q < K; q++) {
+          Gq[q] = fma(T[2 + q], vq, Gq[q]);
+          Gf[q] = fma(T[2 + q], vf, Gf[q]);
+        }
+      }
+    }
+  }
+  long j = j0 + threadIdx.x;
+  if (j < a.n) {
+    double cs = a.sps ? a.sps[j] : 0.0;
+    double cf = a.fas ? a.fas[j] : 0.0;
+    double sq = 0.0, sfa = 0.0;
+    if (cf != 0.0) {
+      // B(T)/B(Tn) = (exp(x_n) - 1)/(exp(x) - 1), x = h c/(lambda k T scale)
+      double lam = sf.wave[j];
+      double hc = kPlanckH * kC;
+      double en = expm1(hc / (lam * kBoltzK * (double)sf.t_norm * 1e-10));
+#pragma unroll
+      for (int q = 0; q < K; q++) {
+        double R = en / expm1(hc / (lam * kBoltzK * sf.node_t[q] * 1e-10));
+        sq = fma(R, Gq[q], sq);
+        sfa = fma(R, Gf[q], sfa);
+      }
+    }
+    double *o = a.out + (long)job * 3 * a.n;
+    o[j] = F0 - cs * F1 - cf * sq;
+    o[a.n + j] = (F1 + Fq) - F4 - Ff;
+    o[2 * a.n + j] = F0 - cs * F4 - cf * sfa;
+  }
+}
+
+hipError_t launch_fir_solar(const FirArgs &a, const SolarFir &sf, int n_job, hipStream_t s) {
+  if (n_job == 0) return hipSuccess;
+  if (a.n_mu > 64) return hipErrorInvalidValue;
+  int n_tiles = (int)((a.n + kFirTile - 1) / kFirTile);
+  long tiles8 = ((long)n_tiles + 7) / 8 * 8;
+  long blocks = tiles8 * n_job;
+  size_t lds = (size_t)4 * (kFirTile + a.nd - 1) * sizeof(double);
+  if (lds > 60000 || blocks > 0x7fffffffL) return hipErrorInvalidValue;
+  switch (sf.k) {
+    case 6: k_fir_solar<6><<<(unsigned)blocks, kTPB, lds, s>>>(a, sf, n_job, n_tiles); break;
+    case 8: k_fir_solar<8><<<(unsigned)blocks, kTPB, lds, s>>>(a, sf, n_job, n_tiles); break;
+    case 10: k_fir_solar<10><<<(unsigned)blocks, kTPB, lds, s>>>(a, sf, n_job, n_tiles); break;
+    case 12: k_fir_solar<12><<<(unsigned)blocks, kTPB, lds, s>>>(a, sf, n_job, n_tiles); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // Quiet star with LDS staging per mu bin: 256 pixels per block, the two rows of
 // a bin staged together, one pixel per thread.
 __global__ __launch_bounds__(kTPB) void k_fir_quiet_lds(FirArgs a) {
@@ -642,8 +777,7 @@ __global__ void k_phase_maps(int grid, int t_star, int t_diff_spot, int t_diff_f
     temp[idx] = T_spot;
     bi[idx] = 0.0;
   } else {
-    int T_plage = (int)(t_star + t_diff_fac + 0.9 - 407.7 * rc + 190.9 * (rc * rc));
-    temp[idx] = T_plage;
+    temp[idx] = plage_temp(t_star + t_diff_fac, rc);
     bi[idx] = 1.0;
   }
 }
@@ -853,15 +987,20 @@ __global__ __launch_bounds__(kTPB) void k_minmax_reduce(const double *part, int 
   }
 }
 
-// spec_norm, SOAP_resolution.cu:225-232: out = 1 - o (1 - min) / max, in place;
-// spectra of frame f live at o + (f*3 + t)*n, their min/max at mm[sp_of(f,t)]
-__global__ void k_norm(long n, int n_spec, const double *mm, double *o) {
-  long total = (long)n_spec * n;
-  long stride = (long)gridDim.x * blockDim.x;
-  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
-    long sp = e / n;
-    double mn = mm[sp * 2], mx = mm[sp * 2 + 1];
-    o[e] = 1.0 - o[e] * (1 - mn) / mx;
+// spec_norm, SOAP_resolution.cu:225-232: out = 1 - o (1 - min) / max, in place.
+// Spectrum sp = blockIdx.y at o + sp*n; two pixels per thread per step.
+__global__ void k_norm(long n, const double *mm, double *o) {
+  long sp = blockIdx.y;
+  double mn = mm[sp * 2], mx = mm[sp * 2 + 1];
+  double *x = o + sp * n;
+  long stride = (long)gridDim.x * blockDim.x * 2;
+  for (long i = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 2; i < n; i += stride) {
+    double v0 = x[i];
+    x[i] = 1.0 - v0 * (1 - mn) / mx;
+    if (i + 1 < n) {
+      double v1 = x[i + 1];
+      x[i + 1] = 1.0 - v1 * (1 - mn) / mx;
+    }
   }
 }
 
@@ -870,10 +1009,10 @@ hipError_t launch_norm(int64_t n, int n_spec, int n_tiles, const double *part, d
   k_minmax_reduce<<<n_spec, kTPB, 0, s>>>(part, n_tiles, mm);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  long total = (long)n_spec * n;
-  long blocks = (total + kTPB - 1) / kTPB;
-  if (blocks > 256 * 64) blocks = 256 * 64;
-  k_norm<<<(unsigned)blocks, kTPB, 0, s>>>(n, n_spec, mm, o);
+  long bx = (n + 2L * kTPB - 1) / (2L * kTPB);
+  if (bx > 64) bx = 64;
+  dim3 grid((unsigned)bx, n_spec);
+  k_norm<<<grid, kTPB, 0, s>>>(n, mm, o);
   return hipGetLastError();
 }
 

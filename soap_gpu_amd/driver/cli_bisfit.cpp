@@ -1,0 +1,33 @@
+// BIS_FIT2 drop-in (SOAP_code/BIS_FIT2.cpp:97-320): argv[1] = number of CCF
+// points; reads CCF_data.txt ("v ccf") and CCF_error.txt from the working
+// directory, writes ccf_model.bin, ccf_parameter.bin (c, |k|, span, v0, fwhm),
+// ccf_depth.bin and ccf_bis.bin.  The fit runs on the GPU (soap_fit_ccf_host).
+#include <iostream>
+
+#include "soap_host.hpp"
+
+using namespace soaphost;
+
+int main(int argc, char **argv) {
+  if (argc != 2) die("usage: BIS_FIT2 <n>");
+  int n = std::atoi(argv[1]);
+  std::vector<double> d = read_numbers("CCF_data.txt"), e = read_numbers("CCF_error.txt");
+  if ((int)d.size() < 2 * n || (int)e.size() < n) die("CCF_data.txt / CCF_error.txt shorter than n");
+  std::vector<double> v(n), y(n);
+  for (int i = 0; i < n; i++) {
+    v[i] = d[2 * i];
+    y[i] = d[2 * i + 1];
+    if (e[i] != 1.0) die("CCF_error.txt: only unit errors are supported (the reference driver writes ones)");
+  }
+  soap_ctx *ctx = nullptr;
+  if (soap_ctx_create(0, &ctx) != SOAP_OK) die("no HIP device");
+  std::vector<double> par(5), model(n), depth(91), bis(91);
+  check(ctx, soap_fit_ccf_host(ctx, v.data(), n, y.data(), 1, par.data(), model.data(), depth.data(), bis.data()),
+        "fit_ccf_host");
+  soap_ctx_destroy(ctx);
+  write_bin("ccf_model.bin", model.data(), n);
+  write_bin("ccf_parameter.bin", par.data(), 5);
+  write_bin("ccf_depth.bin", depth.data(), 91);
+  write_bin("ccf_bis.bin", bis.data(), 91);
+  return 0;
+}

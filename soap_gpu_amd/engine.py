@@ -45,7 +45,7 @@ EXPORTS = (
     "soap_set_wave", "soap_set_mu_bounds", "soap_set_cube", "soap_set_cube_device",
     "soap_disk_init", "soap_get_maps", "soap_set_maps",
     "soap_quiet_star", "soap_get_quiet", "soap_set_quiet",
-    "soap_set_activity", "soap_integrate", "soap_get_raw", "soap_set_raw", "soap_phase_maps",
+    "soap_set_activity", "soap_integrate", "soap_get_raw", "soap_set_raw", "soap_alloc_phases", "soap_phase_maps",
     "soap_resolution", "soap_get_lower", "soap_get_norm",
     "soap_set_mask", "soap_ccf", "soap_get_ccf",
     "soap_fit", "soap_get_fit", "soap_fit_ccf_host",
@@ -90,6 +90,7 @@ def load(path: str = LIB_PATH):
         "soap_integrate": (ctypes.c_int, [vp]),
         "soap_get_raw": (ctypes.c_int, [vp, ctypes.c_int32, _dp, _dp, _dp]),
         "soap_set_raw": (ctypes.c_int, [vp, ctypes.c_int32, _dp, _dp, _dp]),
+        "soap_alloc_phases": (ctypes.c_int, [vp, ctypes.c_int32]),
         "soap_phase_maps": (ctypes.c_int, [vp, ctypes.c_int32, _dp, _dp]),
         "soap_resolution": (ctypes.c_int, [vp, ctypes.c_double, ctypes.c_double]),
         "soap_get_lower": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int, _dp]),
@@ -254,6 +255,10 @@ class Engine:
         b, bp = _d(bconv)
         t, tp = _d(tot)
         self._chk(self.L.soap_set_raw(self.ctx, int(phase), fp, bp, tp), "set_raw")
+
+    def alloc_phases(self, n_phase):
+        self._chk(self.L.soap_alloc_phases(self.ctx, int(n_phase)), "alloc_phases")
+        self.n_phase = int(n_phase)
 
     def phase_maps(self, phase):
         g2 = self.grid * self.grid

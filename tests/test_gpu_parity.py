@@ -95,6 +95,30 @@ def test_integrate_parity(case, request):
             assert normwise(g["raw"][p, t], ref) <= SPEC_TOL, (p, t)
 
 
+@pytest.mark.parametrize("t_star", [5778, 3700, 9000])
+def test_integrate_solar_mode(engine, t_star):
+    """Solar mode: facula intensity by the Planck ratio of the cell's plage
+    temperature (SOAP_integration_solar.cu:424-433), against the oracle."""
+    w = synth.small(grid=64)
+    w.act = dict(w.act, t_star=t_star)
+    setup_workload(engine, w, with_mask=False)
+    engine.set_activity(w.act, w.phases, w.regions, solar_mode=1)
+    engine.integrate()
+    maps = oracle_maps(w)
+    a = w.act
+    for p in range(len(w.phases)):
+        temp, bi = O.phase_maps(64, w.star["incl"], a["nrho"], a["t_star"], a["t_diff_spot"], a["t_diff_fac"],
+                                w.phases[p], w.regions[p])
+        ref = O.integrate_phase(O.GRID_LOG, w.wave, maps, temp, bi, w.cubes, 64, a["t_star"], a["t_diff_fac"],
+                                solar_mode=1)
+        got = engine.get_raw(p)
+        for t in range(3):
+            if np.max(np.abs(ref[t])) == 0:
+                assert np.max(np.abs(got[t])) == 0
+                continue
+            assert normwise(got[t], ref[t]) <= SPEC_TOL, (p, t)
+
+
 def test_phase_maps_equal(engine):
     w = synth.small(grid=64)
     setup_workload(engine, w, with_mask=False)

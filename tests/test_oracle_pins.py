@@ -255,12 +255,19 @@ def test_overlapping_regions_last_wins():
 
 
 # ------------------------------------------------------------------ a6: integrate
-def _naive_integrate(w, maps, temp, bi, grid):
+def _planck(lam, temp):
+    """loi_Planck with the constants of SOAP_integration.cu:205-212."""
+    h, c, kb = 6.62606896e-34, 299792458.0, 1.380e-23
+    return 2 * h * c ** 2 / (lam * 1e-10) ** 5 / (np.exp(h * c / (lam * 1e-10 * kb * temp)) - 1)
+
+
+def _naive_integrate(w, maps, temp, bi, grid, solar=False):
     n = w.n
     flux = np.zeros(n)
     bconv = np.zeros(n)
     tot = np.zeros(n)
     cs, cf = w.cubes["spsed"][0], w.cubes["facsed"][0]
+    t_norm = int(w.act["t_star"] + w.act["t_diff_fac"] + 0.9 - 216.8)
     for idx in np.nonzero((bi == 0.0) | (bi == 1.0))[0]:
         sh = _shifted_axis(w.wave, maps["delta"][idx])
         k, L = maps["region"][idx], maps["limb"][idx]
@@ -268,6 +275,8 @@ def _naive_integrate(w, maps, temp, bi, grid):
         q, qs = it("q"), it("qsed")
         if bi[idx] == 0.0:
             I, a = cs, it("sp")
+        elif solar:
+            I, a = _planck(w.wave, temp[idx]) / _planck(w.wave, t_norm) * cf, it("fac")
         else:
             I, a = (1 + maps["lb"][idx]) * cf, it("fac")
         flux += qs - I * q
@@ -289,6 +298,26 @@ def test_integrate_bruteforce():
     sl = slice(40, w.n - 40)
     for g, r in zip(got, ref):
         assert np.max(np.abs(g[sl] - r[sl])) / np.max(np.abs(r[sl])) < 1e-10
+
+
+def test_integrate_bruteforce_solar():
+    """Solar mode (SOAP_integration_solar.cu:424-433): facula intensity by the
+    Planck ratio at the cell's plage temperature, against the numpy restatement;
+    it differs from the (1+LB) form."""
+    w = synth.small(grid=24, n=2500, n_mu=4)
+    s, a = w.star, w.act
+    maps = O.disk_init(24, s["prot"], 0.0, s["incl"], s["radius_km"], s["u1"], s["u2"], s["lb"], w.mu_bounds)
+    temp, bi = O.phase_maps(24, s["incl"], a["nrho"], a["t_star"], a["t_diff_spot"], a["t_diff_fac"], 0.0,
+                            w.regions[0])
+    got = O.integrate_phase(O.GRID_LOG, w.wave, maps, temp, bi, w.cubes, 24, a["t_star"], a["t_diff_fac"],
+                            solar_mode=1)
+    ref = _naive_integrate(w, maps, temp, bi, 24, solar=True)
+    plain = O.integrate_phase(O.GRID_LOG, w.wave, maps, temp, bi, w.cubes, 24, a["t_star"], a["t_diff_fac"])
+    sl = slice(40, w.n - 40)
+    for g, r in zip(got, ref):
+        assert np.max(np.abs(g[sl] - r[sl])) / np.max(np.abs(r[sl])) < 1e-10
+    assert np.array_equal(got[1], plain[1])  # bconv has no intensity factor
+    assert np.max(np.abs(got[0] - plain[0])) > 1e-3 * np.max(np.abs(plain[0]))
 
 
 def test_integrate_spot_equals_quiet_seeds():
