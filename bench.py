@@ -1,17 +1,16 @@
 """Benchmark: phase-spectra/s of the full SOAP-GPU forward series on one MI355X.
 
 Workload (synthetic, seeded): grid 1000x1000, lambda 3800-7000 A log-uniform at
-0.25 km/s (732,588 pixels), 8000 lines, 20 mu bins, 2 spots + 2 faculae
+0.25 km/s (732,585 pixels), 8000 lines, 20 mu bins, 2 spots + 2 faculae
 (overlapping), 100 phases, R=115000 resolution, 4000-line mask CCF on 161
 velocities, FP64 throughout.
 
-One step = the whole series on device (inputs resident in HBM): disk grid,
-quiet star (normalised + SED), active-region deltas of all 100 phases,
-instrumental convolution + normalisation of 3x100 + 2 spectra, mask CCF of
-303 spectra and the Gaussian fit + bisector of 303 CCFs.  value = 100 x steps
-/ wall time.
+One step = the whole series on device, inputs resident in HBM: disk grid, quiet
+star (normalised + SED), active-region deltas of all 100 phases, instrumental
+convolution + normalisation of 3x100 + 3 spectra, mask CCF of 303 spectra and
+the Gaussian fit + bisector of 303 CCFs.  value = 100 x steps / wall time.
 
-Prints one JSON line (rank 0, N=1).
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -26,42 +25,89 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_HBM_GBS = 8000.0
+PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+PEAK_FP64_TFS = 78.6       # MI355X FP64 (vector and matrix)
+KERNELS = {"fir_active": "k_fir_active_grp", "conv": "k_conv_active", "norm": "k_norm", "ccf": "k_ccf"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_series_1000grid_733k.json")
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(w, eng_maps, frac: float = 0.04, nthreads: int | None = None):
-    """Oracle (FP64 C, OpenMP) on a bounded sample of the same workload: phase 0's
-    active-region integration + resolution of its three spectra on a contiguous
-    lambda sub-range of `frac` of the pixels (quiet star, CCF and fit excluded),
-    scaled to the full grid.  Returns phase-spectra/s."""
+def cpu_baseline(w, frac_active: float, frac_quiet: float, nthreads: int):
+    """FP64 oracle (C, OpenMP) on a bounded sample of the same workload, scaled to
+    one phase-spectrum: phase 0's integration and the resolution of its three
+    spectra over a contiguous `frac_active` of the pixels (scaled by 1/frac), the
+    quiet star (normalised + SED) over `frac_quiet` of the pixels (scaled, then
+    shared by the series' phases), and the CCF + fit of three full spectra."""
     import oracle as O
+
     n = w.n
-    m = int(n * frac)
-    j0 = (n - m) // 2
-    sl = slice(j0, j0 + m)
-    sub = {k: np.ascontiguousarray(v[:, sl]) for k, v in w.cubes.items()}
-    wave = np.ascontiguousarray(w.wave[sl])
-    nt = nthreads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    a, s = w.act, w.star
-    maps = O.disk_init(s["grid"], s["prot"], 0.0, s["incl"], s["radius_km"], s["u1"], s["u2"], s["lb"], w.mu_bounds)
+    trim = 210  # cutoff 200 then 10 (run_all_SOAP.py:259,266)
+    s, a = w.star, w.act
+
+    def sub(frac):
+        m = int(n * frac)
+        j0 = (n - m) // 2
+        sl = slice(j0, j0 + m)
+        return m, np.ascontiguousarray(w.wave[sl]), {k: np.ascontiguousarray(v[:, sl]) for k, v in w.cubes.items()}
+
+    maps = O.disk_init(s["grid"], s["prot"], s.get("w_ratio", 0.0), s["incl"], s["radius_km"], s["u1"], s["u2"],
+                       s["lb"], w.mu_bounds)
+    # active: integration + resolution of 3 spectra
+    m_a, wave_a, cubes_a = sub(frac_active)
     t0 = time.perf_counter()
     temp, bi = O.phase_maps(s["grid"], s["incl"], a["nrho"], a["t_star"], a["t_diff_spot"], a["t_diff_fac"],
                             w.phases[0], w.regions[0])
-    raw = O.integrate_phase(O.GRID_LOG, wave, maps, temp, bi, sub, s["grid"], a["t_star"], a["t_diff_fac"],
-                            nthreads=nt)
-    q = np.full(m, 6.6e5)
+    raw = O.integrate_phase(O.GRID_LOG, wave_a, maps, temp, bi, cubes_a, s["grid"], a["t_star"], a["t_diff_fac"],
+                            nthreads=nthreads)
+    t_int = time.perf_counter() - t0
+    q_sub = np.full(m_a, 6.6e5)  # the quiet spectrum's values do not change the work
+    t0 = time.perf_counter()
     for t in range(3):
-        O.lower_resolution(O.GRID_LOG, wave, w.sigma_inst, q, raw[t], 1, nthreads=nt)
-    dt = time.perf_counter() - t0
-    per_phase = dt / frac
-    return dict(value=1.0 / per_phase, unit="phase-spectra/s", cores=nt, kind="port",
-                sample=f"oracle on phase 0: integration + resolution of 3 spectra over {m} of {n} pixels "
-                       f"({frac:.0%}), {dt:.2f} s measured, scaled x{1 / frac:.1f}; quiet star, CCF, fit excluded",
-                seconds=dt)
+        O.lower_resolution(O.GRID_LOG, wave_a, w.sigma_inst, q_sub, raw[t], 1, nthreads=nthreads)
+    t_res = time.perf_counter() - t0
+    # quiet star, both cubes
+    m_q, wave_q, cubes_q = sub(frac_quiet)
+    t0 = time.perf_counter()
+    for key in ("q", "qsed"):
+        O.quiet_star(O.GRID_LOG, wave_q, maps, cubes_q[key], s["grid"], nthreads=nthreads)
+    t_quiet = time.perf_counter() - t0
+    # CCF + fit of three full spectra (a synthetic normalised spectrum of the grid)
+    spec = 1.0 - 0.5 * (1.0 - w.cubes["q"][0] / np.max(w.cubes["q"][0]))
+    wave_t = w.wave[trim:n - trim]
+    sl_, sw_ = O.select_lines(wave_t, w.mask_l, w.mask_w)
+    h = O.grid_h(w.wave)
+    ap = O.appodisation(len(w.vgrid))
+    t0 = time.perf_counter()
+    for _ in range(3):
+        c = O.ccf_finish(O.ccf(O.GRID_LOG, wave_t, h, w.vgrid, sl_, sw_, spec[trim:n - trim], nthreads=nthreads), ap)
+        O.fit_ccf(w.vgrid, c)
+    t_ccf = time.perf_counter() - t0
+    per_phase = (t_int + t_res) / frac_active + t_quiet / frac_quiet / len(w.phases) + t_ccf
+    measured = t_int + t_res + t_quiet + t_ccf
+    return dict(value=round(1.0 / per_phase, 5), unit="phase-spectra/s", cores=nthreads, kind="port",
+                sample=(f"oracle, {nthreads} threads: phase 0 integration + resolution of 3 spectra over "
+                        f"{m_a} of {n} pixels (x{1 / frac_active:.1f}: {t_int + t_res:.2f} s), quiet star "
+                        f"(normalised + SED) over {m_q} pixels (x{1 / frac_quiet:.1f}, / {len(w.phases)} phases: "
+                        f"{t_quiet:.2f} s), CCF + fit of 3 full spectra ({t_ccf:.2f} s); {measured:.1f} s measured, "
+                        f"{per_phase:.2f} s per phase-spectrum"),
+                seconds=measured)
+
+
+def load_traffic(kernel: str, workload: str):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel calibration)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = d.get("kernels", {}).get(kernel)
+    if not k or d.get("workload") != workload:
+        return None
+    return k.get("traffic_bytes")
 
 
 def main():
@@ -72,11 +118,12 @@ def main():
     ap.add_argument("--phases", type=int, default=100)
     ap.add_argument("--grid", type=int, default=1000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-frac", type=float, default=0.04)
+    ap.add_argument("--cpu-frac", type=float, default=0.25, help="pixel fraction of the active-phase CPU sample")
+    ap.add_argument("--cpu-frac-quiet", type=float, default=0.08, help="pixel fraction of the quiet-star CPU sample")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU only (phases shard across GPUs by design, "
-              "but no multi-GPU run is part of it); use --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the series shards by phase across GPUs by design, but this build "
+              "runs and reports a single MI355X only; use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     from soap_gpu_amd import synth
@@ -84,6 +131,9 @@ def main():
 
     t0 = time.time()
     w = synth.config1(n_phase=args.phases, grid=args.grid)
+    workload = (f"SOAP series: grid {args.grid}x{args.grid}, {w.n}-pt log lambda 3800-7000 A, 8000 lines, "
+                f"20 mu bins, 2 spots + 2 faculae overlapping, {len(w.phases)} phases, R=115000, "
+                f"{len(w.mask_l)}-line mask, {len(w.vgrid)} velocities")
     log(f"[bench] workload built in {time.time() - t0:.1f} s: n={w.n} grid={args.grid} phases={len(w.phases)}")
     eng = Engine(0)
     setup_workload(eng, w)
@@ -106,25 +156,49 @@ def main():
     value = len(w.phases) * args.steps / dt
     log(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step")
 
-    # per-stage HIP-event timing over the same steps, on the engine's stream
+    # per-stage HIP-event timing over the same number of steps, on the engine's stream
     eng.set_timing(True)
     for _ in range(args.steps):
         step()
     st = eng.stage_ms()
     eng.set_timing(False)
     per = {k: v / args.steps for k, v in st.items()}
-    alg_bytes, phase_bytes, rows = eng.integrate_stats()
-    fir_ms = per["fir_active"]
-    achieved = alg_bytes / (fir_ms * 1e-3) / 1e9
     log("[bench] stage ms/step: " + ", ".join(f"{k}={v:.3f}" for k, v in per.items()))
-    log(f"[bench] fir_active: {alg_bytes / 1e6:.1f} MB minimum traffic (union of rows + outputs), "
-        f"{phase_bytes / 1e6:.1f} MB counted per phase, sum R_psi = {rows}")
+
+    kern = {}
+    for stage, kname in KERNELS.items():
+        b, f = eng.stage_work(stage)
+        ms = per[stage]
+        gbs = b / (ms * 1e-3) / 1e9
+        tfs = f / (ms * 1e-3) / 1e12
+        kern[stage] = dict(kernel=kname, ms=round(ms, 4), bytes=b, flops=f, GBs=round(gbs, 1),
+                           TFs=round(tfs, 2), hbm_frac=round(gbs / PEAK_HBM_GBS, 4),
+                           fp64_frac=round(tfs / PEAK_FP64_TFS, 4))
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    k = kern[dom]
+    compute_bound = k["flops"] / (PEAK_FP64_TFS * 1e12) > k["bytes"] / (PEAK_HBM_GBS * 1e9)
+    roof = {"kernel": k["kernel"], "bound": "mfma" if compute_bound else "hbm",
+            "achieved": k["TFs"] if compute_bound else k["GBs"],
+            "peak": PEAK_FP64_TFS if compute_bound else PEAK_HBM_GBS,
+            "unit": "TFLOP/s" if compute_bound else "GB/s",
+            "frac": k["fp64_frac"] if compute_bound else k["hbm_frac"],
+            "traffic": load_traffic(k["kernel"], workload),
+            "alg_bytes_per_launch": k["bytes"], "alg_flops_per_launch": k["flops"], "launch_ms": k["ms"],
+            "note": "FP64 vector FMA kernel; MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s"
+                    if compute_bound else "HBM-bound kernel"}
+    _, phase_bytes, rows = eng.integrate_stats()
+    series_bytes = 8.0 * w.n * (rows + 10 * len(w.phases))  # sum_psi 8 N (R_psi + 2 + 2 + 6)
+    series = {"bytes_per_step": series_bytes, "rows_sum": rows,
+              "GBs": round(series_bytes / (ms_step * 1e-3) / 1e9, 1),
+              "hbm_frac": round(series_bytes / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+    log(f"[bench] dominant kernel {k['kernel']}: {roof['achieved']} {roof['unit']} ({roof['frac']:.1%} of peak); "
+        f"series {series['GBs']} GB/s algorithmic ({series['hbm_frac']:.1%} of HBM peak)")
 
     cpu = None
     if not args.no_cpu_baseline:
-        maps = None
-        cpu = cpu_baseline(w, maps, frac=args.cpu_frac)
-        log(f"[bench] cpu baseline {cpu['value']:.4f} phase-spectra/s ({cpu['seconds']:.1f} s)")
+        nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cpu = cpu_baseline(w, args.cpu_frac, args.cpu_frac_quiet, nt)
+        log(f"[bench] cpu baseline {cpu['value']} phase-spectra/s ({cpu['seconds']:.1f} s measured)")
         cpu.pop("seconds")
 
     out = {
@@ -139,17 +213,17 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded lines, log-uniform grid; no reference seed spectra exist)",
-        "config": {"workload": "config1: grid 1000, 732588-pt log lambda 3800-7000 A, 8000 lines, 20 mu bins, "
-                               "2 spots + 2 faculae, 100 phases, R=115000, 4000-line mask, 161 velocities",
-                   "grid": args.grid, "n_lambda": w.n, "phases": len(w.phases)},
-        "roofline": {"kernel": "k_fir_active", "bound": "hbm", "achieved": round(achieved, 1),
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": None, "alg_bytes_per_launch": alg_bytes, "bytes_per_phase_sum": phase_bytes,
-                     "rows_sum": rows, "launch_ms": round(fir_ms, 4)},
+        "data": "synthetic (seeded Gaussian-line seed cubes on a log-uniform grid; the reference ships no seed spectra)",
+        "config": {"workload": workload, "grid": args.grid, "n_lambda": w.n, "phases": len(w.phases),
+                   "mu_bins": w.n_mu, "mask_lines": int(len(w.mask_l)), "velocities": int(len(w.vgrid))},
+        "roofline": roof,
+        "series_hbm": series,
+        "kernels": kern,
         "stage_ms": {k: round(v, 4) for k, v in per.items()},
         "cpu_baseline": cpu,
     }
+    if cpu:
+        out["vs_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(out), flush=True)
     eng.close()
 

@@ -193,6 +193,12 @@ int32_t soap_num_stages(void);
  * phase (read once) + continua + 3 outputs per phase; bytes_phase_sum = the same
  * counted per phase, sum_psi 8 n (R_psi + continua + 3); rows_sum = sum_psi R_psi */
 int soap_get_integrate_stats(soap_ctx *ctx, double *bytes_min, double *bytes_phase_sum, int64_t *rows_sum);
+/* Algorithmic work of one run of a stage (index as soap_stage_name) on the
+ * current inputs: HBM bytes that must move (each input read once, each output
+ * written once) and FP64 flops (2 per multiply-add the algorithm needs, padding
+ * and skipped zero taps excluded).  Stages without a model return 0 / 0.
+ * No counterpart in the reference (measurement support). */
+int soap_get_stage_work(soap_ctx *ctx, int32_t stage, double *bytes, double *flops);
 
 #ifdef __cplusplus
 }

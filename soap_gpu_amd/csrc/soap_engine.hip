@@ -8,6 +8,7 @@
 #include <map>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -25,15 +26,19 @@ enum Stage {
   ST_FIR_QUIET,
   ST_TAPS_ACTIVE,
   ST_FIR_ACTIVE,
+  ST_CONV_QUIET,
   ST_CONV,
+  ST_MINMAX,
   ST_NORM,
   ST_MASK,
   ST_CCF,
+  ST_CCF_REDUCE,
   ST_FIT,
   ST_COUNT
 };
-const char *kStageNames[ST_COUNT] = {"disk_init", "taps_quiet", "fir_quiet", "taps_active", "fir_active",
-                                     "conv",      "norm",       "mask",      "ccf",         "fit"};
+const char *kStageNames[ST_COUNT] = {"disk_init", "taps_quiet", "fir_quiet",  "taps_active", "fir_active",
+                                     "conv_quiet", "conv",       "minmax",     "norm",        "mask",
+                                     "ccf",        "ccf_reduce", "fit"};
 
 struct DevBuf {
   void *p = nullptr;
@@ -91,9 +96,8 @@ struct soap_ctx {
 
   // resolution
   double *d_lower = nullptr;  // [(P+1)][3][n]
-  double *d_qc = nullptr;     // [2][n] quiet convolution terms
   DevBuf part, mm, ctap, qsel, variant;
-  int res_dlo = 0, res_ntp = 0;
+  int res_dlo = 0, res_ntp = 0, res_wid = -1;
   bool have_lower = false;
   // mask / ccf / fit
   int n_lines = 0, n_v = 0, trim = 0;
@@ -325,7 +329,7 @@ void soap_ctx_destroy(soap_ctx *c) {
   for (int i = 0; i < 6; i++)
     if (c->cube_owned[i]) dfree(c->d_cube[i]);
   void *ptrs[] = {c->d_mu,      c->d_delta,   c->d_limb,    c->d_lbm,      c->d_t,      c->d_region, c->d_m,
-                  c->d_quiet,   c->d_qc,      c->d_geo,     c->d_geo_off, c->d_box,      c->d_raw,    c->d_lower,  c->d_line_l,
+                  c->d_quiet,   c->d_geo,     c->d_geo_off, c->d_box,      c->d_raw,    c->d_lower,  c->d_line_l,
                   c->d_line_w,  c->d_w_sorted, c->d_vgrid,   c->d_appod,   c->d_ccf,      c->acc.p,    c->taps.p,   c->krange.p,
                   c->kflags.p,  c->bins.p,    c->nbins.p,   c->part.p,     c->mm.p,     c->ctap.p,   c->qsel.p,
                   c->variant.p, c->m_start.p, c->m_len.p,   c->m_val.p,    c->m_alive.p, c->sorted.p, c->rank.p,
@@ -939,6 +943,68 @@ int soap_get_integrate_stats(soap_ctx *c, double *bytes_min, double *bytes_phase
   return SOAP_OK;
 }
 
+int soap_get_stage_work(soap_ctx *c, int32_t stage, double *bytes, double *flops) {
+  if (!c || !bytes || !flops || stage < 0 || stage >= ST_COUNT) return fail(c, SOAP_ERR_ARG, "stage_work: bad arguments");
+  HIPCHK(c, hipSetDevice(c->device));
+  double n = (double)c->n;
+  *bytes = 0;
+  *flops = 0;
+  switch (stage) {
+    case ST_FIR_ACTIVE: {
+      if (!c->have_raw || c->P == 0) return fail(c, SOAP_ERR_STATE, "stage_work: run integrate first");
+      int rc = soap_get_integrate_stats(c, bytes, nullptr, nullptr);
+      if (rc) return rc;
+      size_t nk = (size_t)c->P * c->n_mu;
+      std::vector<int32_t> fl(nk), kr(nk * 2);
+      HIPCHK(c, hipMemcpyAsync(fl.data(), c->kflags.p, nk * 4, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(kr.data(), c->krange.p, nk * 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      // products per tap: H1a*SEDq always; spot: 2; facula: 4 ((1+LB) form) or 2 + 2K (solar)
+      int fac_prod = c->act.solar_mode ? 2 + 2 * c->solar_k : 4;
+      double fma = 0;
+      for (size_t e = 0; e < nk; e++) {
+        if (!fl[e] || kr[2 * e] > kr[2 * e + 1]) continue;
+        int prod = 1 + ((fl[e] & 1) ? 2 : 0) + ((fl[e] & 2) ? fac_prod : 0);
+        fma += (double)(kr[2 * e + 1] - kr[2 * e] + 1) * prod;
+      }
+      *flops = 2.0 * fma * n;
+      return SOAP_OK;
+    }
+    case ST_CONV: {
+      if (!c->have_lower) return fail(c, SOAP_ERR_STATE, "stage_work: run resolution first");
+      double taps = 0;  // nonzero taps of the window
+      std::vector<double> ct(c->res_ntp);
+      HIPCHK(c, hipMemcpyAsync(ct.data(), c->ctap.p, ct.size() * 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      for (double v : ct) taps += v != 0.0;
+      double spectra = 3.0 * c->P + 2;  // active spectra + the two quiet spectra
+      // reads: raw 3P + quiet 2; writes: 3P + 3 frame slots
+      *bytes = 8 * n * (3.0 * c->P + 2) + 8 * n * (3.0 * c->P + 3);
+      *flops = 2 * taps * n * spectra;
+      return SOAP_OK;
+    }
+    case ST_NORM:
+      if (!c->have_lower) return fail(c, SOAP_ERR_STATE, "stage_work: run resolution first");
+      *bytes = 2 * 8 * n * (3.0 * (c->P + 1));
+      return SOAP_OK;
+    case ST_CCF: {
+      if (!c->have_ccf) return fail(c, SOAP_ERR_STATE, "stage_work: run ccf first");
+      size_t ne = (size_t)c->n_v * c->n_lines;
+      std::vector<int32_t> len(ne);
+      HIPCHK(c, hipMemcpyAsync(len.data(), c->m_len.p, ne * 4, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      double entries = 0;
+      for (int32_t v : len) entries += v > 0 ? v : 0;
+      double n_spec = 3.0 * (c->P + 1);
+      *bytes = 8 * (n - 2.0 * c->trim) * n_spec;  // every trimmed spectrum read once
+      *flops = 2 * entries * n_spec;
+      return SOAP_OK;
+    }
+    default:
+      return SOAP_OK;
+  }
+}
+
 int soap_get_raw(soap_ctx *c, int32_t phase, double *flux, double *bconv, double *tot) {
   if (!c || phase < 0 || phase >= c->P) return fail(c, SOAP_ERR_ARG, "get_raw: bad phase");
   if (!c->have_raw) return fail(c, SOAP_ERR_STATE, "get_raw: run integrate first");
@@ -1032,16 +1098,8 @@ int soap_resolution(soap_ctx *c, double sigma_ms, double quiet_weight) {
   int wid = conv_window(c->wave, sigma_ms);
   if (wid < 1 || 2 * (int64_t)wid + 2 >= c->n)
     return fail(c, SOAP_ERR_ARG, "resolution: window %d vs %lld pixels", wid, (long long)c->n);
-  ConvArgs ca{};
-  ca.n = c->n;
-  ca.wid = wid;
-  ca.n_tiles = (int)((c->n + conv_tile() - 1) / conv_tile());
-  int n_spec = 3 * (c->P + 1);
   int rc;
-  if ((rc = dalloc(c, &c->d_lower, (size_t)n_spec * c->n)) || (rc = dalloc(c, &c->d_qc, (size_t)2 * c->n)) ||
-      (rc = ensure(c, c->part, (size_t)n_spec * ca.n_tiles * 2 * 8)) || (rc = ensure(c, c->mm, (size_t)n_spec * 2 * 8)))
-    return rc;
-  if (c->res_sigma != sigma_ms) {
+  if (c->res_sigma != sigma_ms || c->res_wid != wid) {
     // log-grid taps c_d = exp(d h) expm1(h)/(fs sqrt(2 pi)) exp(-(expm1(d h)/fs)^2/2)
     // (generalisation of conv_inst's 0.005 A pixel), used where c_d >= 2^-60 c_0
     double fs = -1 * 1.0 * (1 - sqrt((1 + sigma_ms / kC) / (1 - sigma_ms / kC)));
@@ -1062,22 +1120,38 @@ int soap_resolution(soap_ctx *c, double sigma_ms, double quiet_weight) {
     HIPCHK(c, hipMemcpyAsync(c->ctap.p, ct.data(), ct.size() * 8, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->res_sigma = sigma_ms;
+    c->res_wid = wid;
     c->res_dlo = dlo;
     c->res_ntp = ntp;
   }
+  ConvArgs ca{};
+  ca.n = c->n;
+  ca.wid = wid;
   ca.dlo = c->res_dlo;
   ca.ntp = c->res_ntp;
   ca.ctap = (const double *)c->ctap.p;
+  ca.n_tiles = (int)((c->n + conv_tile() - 1) / conv_tile());
+  int n_spec = 3 * (c->P + 1);
+  if ((rc = dalloc(c, &c->d_lower, (size_t)n_spec * c->n)) ||
+      (rc = ensure(c, c->part, (size_t)n_spec * ca.n_tiles * 2 * 8)) || (rc = ensure(c, c->mm, (size_t)n_spec * 2 * 8)))
+    return rc;
   double *part = (double *)c->part.p;
   {
-    StageTimer tm(c, ST_CONV);
-    HIPCHK(c, launch_conv_quiet(ca, c->d_quiet, c->d_qc, c->d_lower + (size_t)3 * c->P * c->n,
+    StageTimer tm(c, ST_CONV_QUIET);
+    HIPCHK(c, launch_conv_quiet(ca, c->d_quiet, c->d_lower + (size_t)3 * c->P * c->n,
                                 part + (size_t)3 * c->P * ca.n_tiles * 2, c->stream));
-    HIPCHK(c, launch_conv_active(ca, c->d_quiet, c->d_qc, c->d_raw, c->P, c->d_lower, part, c->stream));
+  }
+  {
+    StageTimer tm(c, ST_CONV);
+    HIPCHK(c, launch_conv_active(ca, c->d_quiet, c->d_raw, c->P, c->d_lower, part, c->stream));
+  }
+  {
+    StageTimer tm(c, ST_MINMAX);
+    HIPCHK(c, launch_minmax(n_spec, ca.n_tiles, part, (double *)c->mm.p, c->stream));
   }
   {
     StageTimer tm(c, ST_NORM);
-    HIPCHK(c, launch_norm(c->n, n_spec, ca.n_tiles, part, (double *)c->mm.p, c->d_lower, c->stream));
+    HIPCHK(c, launch_norm(c->n, n_spec, (double *)c->mm.p, c->d_lower, c->stream));
   }
   c->have_lower = true;
   c->have_ccf = c->have_fit = false;
@@ -1199,7 +1273,12 @@ int soap_ccf(soap_ctx *c) {
     StageTimer tm(c, ST_CCF);
     HIPCHK(c, launch_ccf(c->d_lower, c->n, c->trim, n_spec, c->n_lines, c->n_v, c->n - 2 * (int64_t)c->trim,
                          (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p,
-                         c->d_w_sorted, c->d_appod, (double *)c->ccf_part.p, c->d_ccf, c->stream));
+                         c->d_w_sorted, (double *)c->ccf_part.p, c->stream));
+  }
+  {
+    StageTimer tm(c, ST_CCF_REDUCE);
+    HIPCHK(c, launch_ccf_reduce((const double *)c->ccf_part.p, ccf_chunks(c->n_lines), n_spec, c->n_v, c->d_appod,
+                                c->d_ccf, c->stream));
   }
   c->have_ccf = true;
   c->have_fit = false;

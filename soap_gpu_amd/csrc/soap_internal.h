@@ -110,13 +110,14 @@ hipError_t launch_phase_maps(int grid, int t_star, int t_diff_spot, int t_diff_f
 // resolution
 int conv_tile();
 int conv_chunk();
-// quiet terms Qc (two spectra) + the quiet-star frame (3 slots of the lower layout)
-hipError_t launch_conv_quiet(const ConvArgs &c, const double *quiet, double *qc, double *frame, double *part,
-                             hipStream_t s);
-hipError_t launch_conv_active(const ConvArgs &c, const double *quiet, const double *qc, const double *raw,
-                              int n_phase, double *o_out, double *part, hipStream_t s);
+// the quiet-star frame (3 slots of the lower layout) from the two quiet spectra
+hipError_t launch_conv_quiet(const ConvArgs &c, const double *quiet, double *frame, double *part, hipStream_t s);
+// 3 spectra per phase from A = quiet[sel] - raw
+hipError_t launch_conv_active(const ConvArgs &c, const double *quiet, const double *raw, int n_phase, double *o_out,
+                              double *part, hipStream_t s);
 // spectra sp = 0..n_spec-1 stored contiguously at o + sp*n, partials part[sp][n_tiles][2]
-hipError_t launch_norm(int64_t n, int n_spec, int n_tiles, const double *part, double *mm, double *o, hipStream_t s);
+hipError_t launch_minmax(int n_spec, int n_tiles, const double *part, double *mm, hipStream_t s);
+hipError_t launch_norm(int64_t n, int n_spec, const double *mm, double *o, hipStream_t s);
 // CCF
 // mask tables [q][v], q = position in wavelength order (sorted[q] = line index)
 hipError_t launch_mask_build(const double *line_l, const double *line_w, const int32_t *sorted, int n_lines,
@@ -129,7 +130,9 @@ int ccf_chunks(int n_lines);
 // part: ccf_chunks(n_lines) * n_spec * n_v doubles; w_sorted[q] = weight of line sorted[q]
 hipError_t launch_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
                       int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
-                      const double *w_sorted, const double *appod, double *part, double *ccf, hipStream_t s);
+                      const double *w_sorted, double *part, hipStream_t s);
+hipError_t launch_ccf_reduce(const double *part, int nc, int n_spec, int n_v, const double *appod, double *ccf,
+                             hipStream_t s);
 hipError_t launch_fit(const double *v, int n_v, const double *y, int n_ccf, double *par, double *model,
                       double *depth, double *bis, int32_t *iters, hipStream_t s);
 }  // namespace soap

@@ -799,7 +799,10 @@ hipError_t launch_phase_maps(int grid, int t_star, int t_diff_spot, int t_diff_f
 // only (zero elsewhere).  Tiles: kConvB outputs per thread (odd, so the per-lane
 // LDS stride is bank-conflict free for 64-bit reads), taps in register chunks of
 // kConvCH (the host pads the tap vector with zeros to a multiple of kConvCH).
-constexpr int kConvB = 9;
+#ifndef SOAP_CONV_B
+#define SOAP_CONV_B 11
+#endif
+constexpr int kConvB = SOAP_CONV_B;  // outputs per thread (odd: conflict-free LDS stride)
 constexpr int kConvCH = 9;
 constexpr int kConvTile = kTPB * kConvB;
 constexpr int kConvMaxTaps = 4095;
@@ -815,7 +818,8 @@ __device__ __forceinline__ void conv_stage(double *xs, const double *X, long i0,
   }
 }
 
-// acc[b] = sum_k ctap[k] xs[tid*B + b + k], k in [0, ntp); ctap in LDS (broadcast)
+// acc[b] = sum_k ctap[k] xs[tid*B + b + k], k in [0, ntp); the taps and the
+// window are both in LDS (the taps are broadcast reads)
 __device__ __forceinline__ void conv_accumulate(const double *xs, const double *ctap, int ntp, double acc[kConvB]) {
 #pragma unroll
   for (int b = 0; b < kConvB; b++) acc[b] = 0.0;
@@ -858,18 +862,18 @@ __device__ __forceinline__ void block_minmax_store(double amin, double omax, dou
   __syncthreads();
 }
 
-// Quiet terms Qc[q] (q = 0 normalised, 1 SED) and the quiet-star frame: Qc is
-// written to qc[q] and to the frame slots of lower-spectrum layout
-// [flux, bconv, tot] = [SED, normalised, SED] (run_all_SOAP.py:359-362), with the
-// partial min(Q) and max(o != 0) (seed -60000) of each tile per slot.
-__global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *quiet, double *qc, double *frame,
-                                                     double *part) {
+// Quiet-star frame (the quiet form of lower_resolution_gpu, SOAP_initialize.cu:
+// 446-505, A = Q): o = sum_d c_d (1 - Q_{i+d}) for q = 0 (normalised) and 1 (SED),
+// written to the frame slots of the lower-spectrum layout [flux, bconv, tot] =
+// [SED, normalised, SED] (run_all_SOAP.py:359-362), with the partial min(Q) and
+// max(o != 0) (seed -60000) of each tile per slot.
+__global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *quiet, double *frame, double *part) {
   extern __shared__ double xs[];
   int q = blockIdx.y;
   long i0 = (long)blockIdx.x * kConvTile;
   int span = kConvTile + c.ntp - 1;
   const double *Q = quiet + (long)q * c.n;
-  double *ct = xs + span;
+  double *ct = xs + span + (span & 1);  // taps, 16-byte aligned
   for (int e = threadIdx.x; e < c.ntp; e += blockDim.x) ct[e] = c.ctap[e];
   conv_stage<true>(xs, Q, i0, c.dlo, span, c.n);
   __syncthreads();
@@ -886,7 +890,6 @@ __global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *q
     long i = i0 + e;
     if (i >= c.n) break;
     double o = ((i > c.wid) && (i + c.wid < c.n - 1)) ? xs[e] : 0.0;
-    qc[(long)q * c.n + i] = o;
     frame[(long)s0 * c.n + i] = o;
     if (q == 1) frame[2L * c.n + i] = o;
     double Ai = Q[i];
@@ -903,58 +906,80 @@ __global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *q
 // Active spectra of phase p = blockIdx.y, types 0..2 (flux, bconv, tot):
 // o = Qc[sel] + sum_d c_d D_{i+d}; partial min(Q[sel] - D) over every pixel and
 // max(o != 0).  sel = 0 for bconv, 1 otherwise (SOAP_resolution.cu:414-416).
-__global__ __launch_bounds__(kTPB) void k_conv_active(ConvArgs c, const double *quiet, const double *qc,
-                                                      const double *raw, double *o_out, double *part) {
+// Active spectra of phase p, types 0..2 (flux, bconv, tot), as in
+// lower_resolution_gpu (SOAP_resolution.cu:235-310): A = Q[sel] - D is staged as
+// 1 - A, o = sum_d c_d (1 - A_{i+d}) for wid < i < n-1-wid (0 elsewhere), with
+// the partial min(A) over the tile's pixels and max(o != 0) (seed -60000).
+// sel = 0 for bconv, 1 otherwise (SOAP_resolution.cu:414-416).  Blocks are
+// numbered so that all phases of one lambda tile run back to back on one XCD
+// (block b -> XCD b % 8): the tile's quiet rows stay in that XCD's L2.
+__global__ __launch_bounds__(kTPB) void k_conv_active(ConvArgs c, const double *quiet, const double *raw,
+                                                      double *o_out, double *part, int n_phase) {
   extern __shared__ double xs[];
-  int p = blockIdx.y;
-  long i0 = (long)blockIdx.x * kConvTile;
+  long b = blockIdx.x;
+  long slot = b / 8;
+  int tile = (int)((slot / n_phase) * 8 + b % 8);
+  int p = (int)(slot % n_phase);
+  if (tile >= c.n_tiles) return;  // uniform over the block
+  long i0 = (long)tile * kConvTile;
   int span = kConvTile + c.ntp - 1;
-  double *ct = xs + span;                // taps
-  double *ob = ct + c.ntp;               // per-tile results
+  double *ct = xs + span + (span & 1);  // taps, 16-byte aligned
   for (int e = threadIdx.x; e < c.ntp; e += blockDim.x) ct[e] = c.ctap[e];
+#pragma unroll 1
   for (int t = 0; t < 3; t++) {
     int sel = t == 1 ? 0 : 1;
     long sp = (long)p * 3 + t;
     const double *D = raw + sp * c.n;
-    conv_stage<false>(xs, D, i0, c.dlo, span, c.n);
+    const double *Q = quiet + (long)sel * c.n;
+    double amin = HUGE_VAL, omax = -60000.0;
+    for (int e = threadIdx.x; e < span; e += blockDim.x) {
+      long m = i0 + c.dlo + e;
+      double v = 1.0;
+      if (m >= 0 && m < c.n) {
+        double A = Q[m] - D[m];
+        v = 1.0 - A;
+        if (m >= i0 && m < i0 + kConvTile) amin = A < amin ? A : amin;
+      }
+      xs[e] = v;
+    }
     __syncthreads();
     double acc[kConvB];
     conv_accumulate(xs, ct, c.ntp, acc);
-#pragma unroll
-    for (int b = 0; b < kConvB; b++) ob[threadIdx.x * kConvB + b] = acc[b];
     __syncthreads();
-    double amin = HUGE_VAL, omax = -60000.0;
-    const double *Q = quiet + (long)sel * c.n;
-    const double *QC = qc + (long)sel * c.n;
+#pragma unroll
+    for (int k = 0; k < kConvB; k++) xs[threadIdx.x * kConvB + k] = acc[k];
+    __syncthreads();
     for (int e = threadIdx.x; e < kConvTile; e += blockDim.x) {
       long i = i0 + e;
       if (i >= c.n) break;
-      double o = ((i > c.wid) && (i + c.wid < c.n - 1)) ? QC[i] + ob[e] : 0.0;
+      double o = ((i > c.wid) && (i + c.wid < c.n - 1)) ? xs[e] : 0.0;
       o_out[sp * c.n + i] = o;
-      double Ai = Q[i] - xs[e - c.dlo];
-      amin = Ai < amin ? Ai : amin;
       if (o != 0.0 && omax < o) omax = o;
     }
-    block_minmax_store(amin, omax, part + (sp * c.n_tiles + blockIdx.x) * 2);
+    block_minmax_store(amin, omax, part + (sp * c.n_tiles + tile) * 2);  // ends with a barrier
   }
 }
 
-hipError_t launch_conv_quiet(const ConvArgs &c, const double *quiet, double *qc, double *frame, double *part,
-                             hipStream_t s) {
+static size_t conv_lds(const ConvArgs &c) { return (size_t)(kConvTile + 2 * c.ntp + 1) * sizeof(double); }
+
+hipError_t launch_conv_quiet(const ConvArgs &c, const double *quiet, double *frame, double *part, hipStream_t s) {
   if (c.ntp % kConvCH || c.ntp > kConvMaxTaps) return hipErrorInvalidValue;
-  size_t lds = (size_t)(kConvTile + 2 * c.ntp - 1) * sizeof(double);
+  size_t lds = conv_lds(c);
+  if (lds > 65536) return hipErrorInvalidValue;
   dim3 grid(c.n_tiles, 2);
-  k_conv_quiet<<<grid, kTPB, lds, s>>>(c, quiet, qc, frame, part);
+  k_conv_quiet<<<grid, kTPB, lds, s>>>(c, quiet, frame, part);
   return hipGetLastError();
 }
 
-hipError_t launch_conv_active(const ConvArgs &c, const double *quiet, const double *qc, const double *raw, int n_phase,
-                              double *o_out, double *part, hipStream_t s) {
+hipError_t launch_conv_active(const ConvArgs &c, const double *quiet, const double *raw, int n_phase, double *o_out,
+                              double *part, hipStream_t s) {
   if (n_phase == 0) return hipSuccess;
   if (c.ntp % kConvCH || c.ntp > kConvMaxTaps) return hipErrorInvalidValue;
-  size_t lds = (size_t)(2 * kConvTile + 2 * c.ntp - 1) * sizeof(double);
-  dim3 grid(c.n_tiles, n_phase);
-  k_conv_active<<<grid, kTPB, lds, s>>>(c, quiet, qc, raw, o_out, part);
+  size_t lds = conv_lds(c);
+  long tiles8 = ((long)c.n_tiles + 7) / 8 * 8;
+  long blocks = tiles8 * n_phase;
+  if (blocks > 0x7fffffffL || lds > 65536) return hipErrorInvalidValue;
+  k_conv_active<<<(unsigned)blocks, kTPB, lds, s>>>(c, quiet, raw, o_out, part, n_phase);
   return hipGetLastError();
 }
 
@@ -988,29 +1013,36 @@ __global__ __launch_bounds__(kTPB) void k_minmax_reduce(const double *part, int 
 }
 
 // spec_norm, SOAP_resolution.cu:225-232: out = 1 - o (1 - min) / max, in place.
-// Spectrum sp = blockIdx.y at o + sp*n; two pixels per thread per step.
-__global__ void k_norm(long n, const double *mm, double *o) {
+// Spectrum sp = blockIdx.y at o + sp*n; a block covers kNormPer*kTPB pixels with
+// coalesced loads issued before the stores.
+constexpr int kNormPer = 8;
+__global__ __launch_bounds__(kTPB) void k_norm(long n, const double *mm, double *o) {
   long sp = blockIdx.y;
   double mn = mm[sp * 2], mx = mm[sp * 2 + 1];
   double *x = o + sp * n;
-  long stride = (long)gridDim.x * blockDim.x * 2;
-  for (long i = ((long)blockIdx.x * blockDim.x + threadIdx.x) * 2; i < n; i += stride) {
-    double v0 = x[i];
-    x[i] = 1.0 - v0 * (1 - mn) / mx;
-    if (i + 1 < n) {
-      double v1 = x[i + 1];
-      x[i + 1] = 1.0 - v1 * (1 - mn) / mx;
-    }
+  long base = (long)blockIdx.x * (kTPB * kNormPer) + threadIdx.x;
+  double v[kNormPer];
+#pragma unroll
+  for (int k = 0; k < kNormPer; k++) {
+    long i = base + (long)k * kTPB;
+    v[k] = i < n ? x[i] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < kNormPer; k++) {
+    long i = base + (long)k * kTPB;
+    if (i < n) x[i] = 1.0 - v[k] * (1 - mn) / mx;
   }
 }
 
-hipError_t launch_norm(int64_t n, int n_spec, int n_tiles, const double *part, double *mm, double *o, hipStream_t s) {
+hipError_t launch_minmax(int n_spec, int n_tiles, const double *part, double *mm, hipStream_t s) {
   if (n_spec == 0) return hipSuccess;
   k_minmax_reduce<<<n_spec, kTPB, 0, s>>>(part, n_tiles, mm);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  long bx = (n + 2L * kTPB - 1) / (2L * kTPB);
-  if (bx > 64) bx = 64;
+  return hipGetLastError();
+}
+
+hipError_t launch_norm(int64_t n, int n_spec, const double *mm, double *o, hipStream_t s) {
+  if (n_spec == 0) return hipSuccess;
+  long bx = (n + (long)kTPB * kNormPer - 1) / ((long)kTPB * kNormPer);
   dim3 grid((unsigned)bx, n_spec);
   k_norm<<<grid, kTPB, 0, s>>>(n, mm, o);
   return hipGetLastError();
@@ -1164,7 +1196,7 @@ int ccf_chunks(int n_lines) { return (n_lines + kCcfLines - 1) / kCcfLines; }
 
 hipError_t launch_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
                       int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
-                      const double *w_sorted, const double *appod, double *part, double *ccf, hipStream_t s) {
+                      const double *w_sorted, double *part, hipStream_t s) {
   if (n_spec == 0) return hipSuccess;
   if (n_v > 1024) return hipErrorInvalidValue;
   int nc = ccf_chunks(n_lines);
@@ -1172,8 +1204,13 @@ hipError_t launch_ccf(const double *spec, int64_t spec_stride, int64_t trim, int
   dim3 grid(nc, (n_spec + kCcfSpec - 1) / kCcfSpec);
   k_ccf<<<grid, threads, 0, s>>>(spec, spec_stride, trim, n_spec, n_lines, n_v, n_t, m_start, m_pk, m_wv, w_sorted,
                                  part);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t launch_ccf_reduce(const double *part, int nc, int n_spec, int n_v, const double *appod, double *ccf,
+                             hipStream_t s) {
+  if (n_spec == 0) return hipSuccess;
+  if (n_v > 1024) return hipErrorInvalidValue;
   int rt = 1;
   while (rt < n_v && rt < 1024) rt <<= 1;
   rt = rt < 64 ? 64 : rt;

@@ -52,6 +52,7 @@ EXPORTS = (
     "soap_run_series",
     "soap_set_timing", "soap_get_stage_ms", "soap_stage_name", "soap_num_stages",
     "soap_get_integrate_stats",
+    "soap_get_stage_work",
 )
 
 _lib = None
@@ -107,6 +108,7 @@ def load(path: str = LIB_PATH):
         "soap_stage_name": (ctypes.c_char_p, [ctypes.c_int32]),
         "soap_num_stages": (ctypes.c_int32, []),
         "soap_get_integrate_stats": (ctypes.c_int, [vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64)]),
+        "soap_get_stage_work": (ctypes.c_int, [vp, ctypes.c_int32, _dp, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -339,6 +341,14 @@ class Engine:
         o, op = _out(ns)
         self._chk(self.L.soap_get_stage_ms(self.ctx, op, ns), "get_stage_ms")
         return {self.L.soap_stage_name(i).decode(): float(o[i]) for i in range(ns)}
+
+    def stage_work(self, name):
+        """(bytes, flops) of one run of the named stage on the current inputs."""
+        names = [self.L.soap_stage_name(i).decode() for i in range(self.L.soap_num_stages())]
+        b, f = ctypes.c_double(), ctypes.c_double()
+        self._chk(self.L.soap_get_stage_work(self.ctx, names.index(name), ctypes.byref(b), ctypes.byref(f)),
+                  "stage_work")
+        return b.value, f.value
 
     def integrate_stats(self):
         """(bytes_min, bytes_phase_sum, rows_sum) of the last integrate."""

@@ -1,0 +1,123 @@
+"""Measurement helpers for rocprofv3 runs of the series benchmark.
+
+    python -m soap_gpu_amd.profiling copy
+        a plain streaming kernel of known size (1 GiB read + 1 GiB written per
+        dispatch, 3 dispatches) to calibrate the FETCH_SIZE / WRITE_SIZE counters;
+    python -m soap_gpu_amd.profiling summarize OUT_PREFIX WORKLOAD \\
+            --fetch DIR --write DIR --copy-fetch DIR --copy-write DIR
+        averages the per-dispatch counters of each kernel from the CSV output of
+        `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE --output-format csv`,
+        scales each by (known bytes / counter reading) of the copy kernel and
+        writes OUT_PREFIX.json (read by bench.py) and OUT_PREFIX.md.
+"""
+from __future__ import annotations
+
+import csv
+import glob
+import json
+import os
+import sys
+
+COPY_BYTES = 1 << 30
+COPY_KERNEL_HINT = "mulfunctor"
+
+
+def copy_kernel(reps: int = 3):
+    import torch
+    n = COPY_BYTES // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(reps):
+        torch.mul(x, 1.0, out=y)
+    torch.cuda.synchronize()
+    print(f"copy kernel: {reps} dispatches of {COPY_BYTES} bytes read + {COPY_BYTES} bytes written", flush=True)
+
+
+def _counter_rows(d):
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {d}")
+    rows = []
+    for f in files:
+        with open(f) as fh:
+            rows += list(csv.DictReader(fh))
+    return rows
+
+
+def per_kernel(d, counter):
+    """{kernel name: mean counter value per dispatch (KiB)}"""
+    acc = {}
+    for r in _counter_rows(d):
+        if r.get("Counter_Name") != counter:
+            continue
+        name = r.get("Kernel_Name", "")
+        key = (name, r.get("Dispatch_Id"))
+        acc.setdefault(name, {}).setdefault(key, 0.0)
+        acc[name][key] += float(r["Counter_Value"])
+    return {k: sum(v.values()) / len(v) for k, v in acc.items()}
+
+
+def _short(name):
+    for k in ("k_fir_active_grp", "k_conv_active", "k_norm", "k_ccf_reduce", "k_ccf", "k_fir_quiet_lds",
+              "k_conv_quiet", "k_taps_active", "k_taps_quiet", "k_minmax_reduce", "k_fit", "k_mask_build",
+              "k_mask_overwrite", "k_disk_init", "k_taps_finalize", "k_fill_u64"):
+        if k + "E" in name or name.endswith(k) or ("soap::" + k) in name or ("4soap" + str(len(k)) + k) in name:
+            return k
+    return name
+
+
+def summarize(out_prefix, workload, fetch, write, copy_fetch, copy_write):
+    cf = {k: v for k, v in per_kernel(copy_fetch, "FETCH_SIZE").items() if COPY_KERNEL_HINT in k.lower()}
+    cw = {k: v for k, v in per_kernel(copy_write, "WRITE_SIZE").items() if COPY_KERNEL_HINT in k.lower()}
+    if not cf or not cw:
+        raise SystemExit("copy kernel not found in the calibration runs")
+    known_kib = COPY_BYTES / 1024.0
+    f_fetch = known_kib / max(cf.values())
+    f_write = known_kib / max(cw.values())
+    fe = per_kernel(fetch, "FETCH_SIZE")
+    wr = per_kernel(write, "WRITE_SIZE")
+    kern = {}
+    for name in sorted(set(fe) | set(wr)):
+        s = _short(name)
+        fb = fe.get(name, 0.0) * 1024 * f_fetch
+        wb = wr.get(name, 0.0) * 1024 * f_write
+        kern[s] = {"fetch_kib_raw": round(fe.get(name, 0.0), 1), "write_kib_raw": round(wr.get(name, 0.0), 1),
+                   "fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb}
+    summary = {"workload": workload, "copy_factor_fetch": f_fetch, "copy_factor_write": f_write,
+               "copy_kernel": list(cf)[0], "kernels": kern}
+    with open(out_prefix + ".json", "w") as f:
+        json.dump(summary, f, indent=1)
+    lines = [f"# HBM traffic per dispatch (rocprofv3 PMC), {workload}", "",
+             "Counters: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), one rocprofv3 pass each,",
+             f"scaled by the copy calibration (1 GiB read + 1 GiB written): fetch x{f_fetch:.4f}, "
+             f"write x{f_write:.4f}.", "",
+             "| kernel | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM bytes (scaled) |", "|---|---:|---:|---:|"]
+    for k, v in sorted(kern.items(), key=lambda kv: -kv[1]["traffic_bytes"]):
+        lines.append(f"| {k} | {v['fetch_kib_raw']:.1f} | {v['write_kib_raw']:.1f} | {v['traffic_bytes']:.4g} |")
+    with open(out_prefix + ".md", "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def main(argv):
+    if not argv:
+        raise SystemExit(__doc__)
+    if argv[0] == "copy":
+        copy_kernel()
+    elif argv[0] == "summarize":
+        import argparse
+        ap = argparse.ArgumentParser()
+        ap.add_argument("out_prefix")
+        ap.add_argument("workload")
+        ap.add_argument("--fetch", required=True)
+        ap.add_argument("--write", required=True)
+        ap.add_argument("--copy-fetch", required=True)
+        ap.add_argument("--copy-write", required=True)
+        a = ap.parse_args(argv[1:])
+        summarize(a.out_prefix, a.workload, a.fetch, a.write, a.copy_fetch, a.copy_write)
+    else:
+        raise SystemExit(__doc__)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])
