@@ -535,12 +535,12 @@ __global__ __launch_bounds__(kTPB) void k_fir_active_grp(FirArgs a, int n_job, i
     // rows present: quiet, quiet SED, then spot (if any spot cell) and facula (if any)
     const int has_sp = ufl & 1, has_fa = (ufl >> 1) & 1;
     const int nrow = 2 + has_sp + has_fa;
-    const double *src[4] = {a.q + (long)k * a.n, a.qs + (long)k * a.n,
-                            has_sp ? a.sp + (long)k * a.n : a.fa + (long)k * a.n, a.fa + (long)k * a.n};
-    for (int e = threadIdx.x; e < nrow * row_len; e += blockDim.x) {
-      int r = e / row_len, x = e - r * row_len;
-      long i = src0 + x;
-      rows[e] = (i >= 0 && i < a.n) ? src[r][i] : 0.0;
+    for (int r = 0; r < nrow; r++) {
+      const double *src = (r == 0 ? a.q : r == 1 ? a.qs : (r == 2 && has_sp) ? a.sp : a.fa) + (long)k * a.n;
+      for (int x = threadIdx.x; x < row_len; x += blockDim.x) {
+        long i = src0 + x;
+        rows[r * row_len + x] = (i >= 0 && i < a.n) ? src[i] : 0.0;
+      }
     }
     __syncthreads();
     int flg[kFirG];
