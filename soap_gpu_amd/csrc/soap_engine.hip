@@ -86,6 +86,9 @@ struct soap_ctx {
   RegionDev *d_geo = nullptr;
   int32_t *d_geo_off = nullptr, *d_box = nullptr;
   int max_col_blocks = 0;
+  // linear grids: staged source range per tile, bin-sorted disk cells, active cell lists
+  int lin_off_lo = 0, lin_span = 0;
+  DevBuf qcells, qbin_off, acells, abin_off, alist_off;
   // solar mode: Lagrange nodes over the facula temperature range
   int solar_k = 0, t_norm_fac = 0;
   double solar_t[kMaxSolarNodes] = {}, solar_w[kMaxSolarNodes] = {};
@@ -333,7 +336,8 @@ void soap_ctx_destroy(soap_ctx *c) {
                   c->d_line_w,  c->d_w_sorted, c->d_vgrid,   c->d_appod,   c->d_ccf,      c->acc.p,    c->taps.p,   c->krange.p,
                   c->kflags.p,  c->bins.p,    c->nbins.p,   c->part.p,     c->mm.p,     c->ctap.p,   c->qsel.p,
                   c->variant.p, c->m_start.p, c->m_len.p,   c->m_val.p,    c->m_alive.p, c->sorted.p, c->rank.p,
-                  c->f_par.p,   c->f_model.p, c->f_depth.p, c->f_bis.p,    c->f_iter.p, c->ccf_part.p};
+                  c->f_par.p,   c->f_model.p, c->f_depth.p, c->f_bis.p,    c->f_iter.p, c->ccf_part.p,
+                  c->qcells.p,  c->qbin_off.p, c->acells.p, c->abin_off.p, c->alist_off.p};
   for (void *p : ptrs) dfree(p);
   hipStreamDestroy(c->stream);
   delete c;
@@ -370,16 +374,20 @@ int32_t soap_num_stages(void) { return ST_COUNT; }
 int soap_set_wave(soap_ctx *c, int kind, const double *wave, int64_t n, double step) {
   if (!c || !wave || n < 8) return fail(c, SOAP_ERR_ARG, "set_wave: need a wavelength array of >= 8 pixels");
   if (kind != SOAP_GRID_LOG && kind != SOAP_GRID_LINEAR) return fail(c, SOAP_ERR_ARG, "set_wave: bad grid kind");
-  if (kind == SOAP_GRID_LINEAR)
-    return fail(c, SOAP_ERR_LIMIT, "set_wave: linear grids are handled by the gather path, not built in this library");
+  if (kind == SOAP_GRID_LINEAR && !(step > 0))
+    return fail(c, SOAP_ERR_ARG, "set_wave: a linear grid needs the step of the reference's index formula");
   HIPCHK(c, hipSetDevice(c->device));
   std::vector<double> w(wave, wave + n);
   double h = log(w[n - 1] / w[0]) / (double)(n - 1);
   for (int64_t j = 0; j < n; j++) {
-    double e = w[0] * exp((double)j * h);
-    if (!(fabs(w[j] - e) <= 1e-12 * fabs(w[j])))
-      return fail(c, SOAP_ERR_GRID, "set_wave: pixel %lld deviates from a log-uniform grid (%.17g vs %.17g)",
-                  (long long)j, w[j], e);
+    if (kind == SOAP_GRID_LOG) {
+      double e = w[0] * exp((double)j * h);
+      if (!(fabs(w[j] - e) <= 1e-12 * fabs(w[j])))
+        return fail(c, SOAP_ERR_GRID, "set_wave: pixel %lld deviates from a log-uniform grid (%.17g vs %.17g)",
+                    (long long)j, w[j], e);
+    } else if (j > 0 && !(w[j] > w[j - 1])) {
+      return fail(c, SOAP_ERR_GRID, "set_wave: wavelengths must increase (pixel %lld)", (long long)j);
+    }
   }
   if (c->n != n) {
     // every per-pixel buffer depends on n
@@ -478,6 +486,57 @@ static int set_tap_range(soap_ctx *c, double vmax_kms) {
   return SOAP_OK;
 }
 
+// Linear grids: the staged source-index range of a pixel tile for shifts up to
+// vmax (i(j) - j = (-fbeta lambda_0/step - fbeta j)/(1 + fbeta) at the extreme
+// fbeta and j, plus a margin), and the bin-sorted lists of the disk cells.
+static int linear_prepare(soap_ctx *c, double vmax) {
+  double lo = 0, hi = 0;
+  for (double v : {-vmax, vmax}) {
+    double beta = v * 1000. / kC;
+    double fb = -1 * (1 - sqrt((1 + beta) / (1 - beta)));
+    double fl = fb * c->wave[0] / c->step;
+    for (double j : {0.0, (double)(c->n - 1)}) {
+      double off = (-fl - fb * j) / (1 + fb);
+      lo = std::min(lo, off);
+      hi = std::max(hi, off);
+    }
+  }
+  c->lin_off_lo = (int)floor(lo) - 3;
+  c->lin_span = 256 + ((int)ceil(hi) + 4 - c->lin_off_lo);
+  if (c->lin_span > 1600)
+    return fail(c, SOAP_ERR_LIMIT, "linear grid: shifts of up to %.1f pixels exceed the tile window", hi - lo);
+  int rc;
+  size_t g2 = (size_t)c->G * c->G;
+  if ((rc = ensure(c, c->qbin_off, (size_t)(c->n_mu + 1) * 4)) || (rc = ensure(c, c->qcells, g2 * 4))) return rc;
+  std::vector<int32_t> cnt(c->n_mu), off(c->n_mu + 1, 0);
+  HIPCHK(c, launch_disk_cells_by_bin((int)g2, c->n_mu, c->d_limb, c->d_delta, c->d_region, 1, (int32_t *)c->qbin_off.p,
+                                     nullptr, nullptr, c->stream));
+  HIPCHK(c, hipMemcpyAsync(cnt.data(), c->qbin_off.p, c->n_mu * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int k = 0; k < c->n_mu; k++) off[k + 1] = off[k] + cnt[k];
+  HIPCHK(c, hipMemcpyAsync(c->qbin_off.p, off.data(), (c->n_mu + 1) * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch_disk_cells_by_bin((int)g2, c->n_mu, c->d_limb, c->d_delta, c->d_region, 0, nullptr,
+                                     (const int32_t *)c->qbin_off.p, (int32_t *)c->qcells.p, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SOAP_OK;
+}
+
+static LinArgs lin_args(soap_ctx *c) {
+  LinArgs a{};
+  a.n = c->n;
+  a.n_mu = c->n_mu;
+  a.grid = c->G;
+  a.lambda0 = c->wave[0];
+  a.step = c->step;
+  a.wave = c->d_wave;
+  a.off_lo = c->lin_off_lo;
+  a.span = c->lin_span;
+  a.delta = c->d_delta;
+  a.limb = c->d_limb;
+  a.lbm = c->d_lbm;
+  return a;
+}
+
 int soap_disk_init(soap_ctx *c, const soap_star *st) {
   if (!c || !st) return fail(c, SOAP_ERR_ARG, "disk_init: null argument");
   if (c->n == 0 || c->n_mu == 0) return fail(c, SOAP_ERR_STATE, "disk_init: set wave and mu bounds first");
@@ -514,12 +573,13 @@ int soap_disk_init(soap_ctx *c, const soap_star *st) {
   a.m = c->d_m;
   a.t = c->d_t;
   // |delta| <= (|omega0| + |omega1|) pi/(180*86400) R |sin i|, SOAP_initialize.cu:348-351
-  rc = set_tap_range(c, (fabs(a.omega0) + fabs(a.omega1)) * kPi / (180 * 24 * 3600) * fabs(a.radius) * fabs(a.sin_incl));
-  if (rc) return rc;
+  double vmax = (fabs(a.omega0) + fabs(a.omega1)) * kPi / (180 * 24 * 3600) * fabs(a.radius) * fabs(a.sin_incl);
+  if (c->kind == SOAP_GRID_LOG && (rc = set_tap_range(c, vmax))) return rc;
   {
     StageTimer tm(c, ST_DISK);
     HIPCHK(c, launch_disk_init(a, c->stream));
   }
+  if (c->kind == SOAP_GRID_LINEAR && (rc = linear_prepare(c, vmax * (1 + 1e-12)))) return rc;
   c->have_maps = true;
   c->have_quiet = c->have_raw = c->have_lower = c->have_ccf = c->have_fit = false;
   return SOAP_OK;
@@ -555,11 +615,11 @@ int soap_set_maps(soap_ctx *c, const soap_star *st, const double *delta, const d
   double vmax = 0;
   for (size_t i = 0; i < g2; i++)
     if (!(limb[i] == -999.0 && delta[i] == -999.0)) vmax = std::max(vmax, fabs(delta[i]));
-  rc = set_tap_range(c, vmax);
-  if (rc) return rc;
+  if (c->kind == SOAP_GRID_LOG && (rc = set_tap_range(c, vmax))) return rc;
   for (size_t i = 0; i < g2; i++) {
     if (region[i] < 0 || region[i] >= c->n_mu) return fail(c, SOAP_ERR_ARG, "set_maps: region %d out of range", region[i]);
     if (limb[i] == -999.0 && delta[i] == -999.0) continue;
+    if (c->kind != SOAP_GRID_LOG) continue;
     double beta = delta[i] * 1000. / kC;
     double fbeta = -1 * (1 - sqrt((1 + beta) / (1 - beta)));
     double sh = log1p(fbeta) / c->h;
@@ -575,6 +635,7 @@ int soap_set_maps(soap_ctx *c, const soap_star *st, const double *delta, const d
   HIPCHK(c, hipMemcpyAsync(c->d_m, m.data(), g2 * 4, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_t, t.data(), g2 * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->kind == SOAP_GRID_LINEAR && (rc = linear_prepare(c, vmax))) return rc;
   c->have_maps = true;
   c->have_quiet = c->have_raw = c->have_lower = c->have_ccf = c->have_fit = false;
   return SOAP_OK;
@@ -615,8 +676,19 @@ int soap_quiet_star(soap_ctx *c) {
   if ((rc = check_cube(c, SOAP_CUBE_QUIET, "quiet_star")) || (rc = check_cube(c, SOAP_CUBE_QUIET_SED, "quiet_star")))
     return rc;
   HIPCHK(c, hipSetDevice(c->device));
-  if ((rc = ensure_taps(c, 1, 1, 1))) return rc;
   if ((rc = dalloc(c, &c->d_quiet, (size_t)2 * c->n))) return rc;
+  if (c->kind == SOAP_GRID_LINEAR) {
+    {
+      StageTimer tm(c, ST_FIR_QUIET);
+      HIPCHK(c, launch_gather_quiet_linear(lin_args(c), c->d_cube[SOAP_CUBE_QUIET], c->d_cube[SOAP_CUBE_QUIET_SED],
+                                           (const int32_t *)c->qcells.p, (const int32_t *)c->qbin_off.p, c->d_quiet,
+                                           c->stream));
+    }
+    c->have_quiet = true;
+    c->have_lower = c->have_ccf = c->have_fit = false;
+    return SOAP_OK;
+  }
+  if ((rc = ensure_taps(c, 1, 1, 1))) return rc;
   TapAccArgs a{};
   a.grid = c->G;
   a.n_mu = c->n_mu;
@@ -815,6 +887,50 @@ int soap_set_activity(soap_ctx *c, const soap_activity *act, int32_t n_phase, co
   return SOAP_OK;
 }
 
+// Linear grids: per-phase active cell lists (by bin), then the gather kernel.
+static int integrate_linear(soap_ctx *c) {
+  int rc;
+  std::vector<int64_t> loff(c->P + 1, 0);
+  for (int p = 0; p < c->P; p++) {
+    const int32_t *b = &c->box[(size_t)p * 4];
+    loff[p + 1] = loff[p] + (int64_t)(b[1] - b[0]) * (b[3] - b[2]);
+  }
+  if ((rc = ensure(c, c->acells, (size_t)std::max<int64_t>(1, loff[c->P]) * 4)) ||
+      (rc = ensure(c, c->abin_off, (size_t)c->P * (c->n_mu + 1) * 4)) ||
+      (rc = ensure(c, c->alist_off, (size_t)(c->P + 1) * 8)))
+    return rc;
+  HIPCHK(c, hipMemcpyAsync(c->alist_off.p, loff.data(), (c->P + 1) * 8, hipMemcpyHostToDevice, c->stream));
+  {
+    StageTimer tm(c, ST_TAPS_ACTIVE);
+    HIPCHK(c, launch_active_cells(c->G, c->n_mu, c->P, c->d_region, c->d_geo, c->d_geo_off, c->d_box,
+                                  (const int64_t *)c->alist_off.p, (int32_t *)c->acells.p, (int32_t *)c->abin_off.p,
+                                  c->stream));
+  }
+  FirArgs f{};
+  f.q = c->d_cube[SOAP_CUBE_QUIET];
+  f.qs = c->d_cube[SOAP_CUBE_QUIET_SED];
+  f.sp = c->d_cube[SOAP_CUBE_SPOT];
+  f.sps = c->d_cube[SOAP_CUBE_SPOT_SED];
+  f.fa = c->d_cube[SOAP_CUBE_FAC];
+  f.fas = c->d_cube[SOAP_CUBE_FAC_SED];
+  f.n = c->n;
+  f.n_mu = c->n_mu;
+  f.out = c->d_raw;
+  LinSolar sol{};
+  sol.on = c->act.solar_mode != 0;
+  sol.t_sum = c->act.t_star + c->act.t_diff_fac;
+  sol.t_norm = (int)(sol.t_sum + 0.9 - 216.8);
+  {
+    StageTimer tm(c, ST_FIR_ACTIVE);
+    HIPCHK(c, launch_gather_active_linear(lin_args(c), f, (const int32_t *)c->acells.p, (const int64_t *)c->alist_off.p,
+                                          (const int32_t *)c->abin_off.p, c->P, sol, c->stream));
+  }
+  c->have_raw = true;
+  c->have_lower = c->have_ccf = c->have_fit = false;
+  c->alg_bytes = -1;
+  return SOAP_OK;
+}
+
 int soap_integrate(soap_ctx *c) {
   if (!c) return SOAP_ERR_ARG;
   if (!c->have_maps || c->phases.size() != (size_t)c->P) return fail(c, SOAP_ERR_STATE, "integrate: set activity first");
@@ -832,6 +948,7 @@ int soap_integrate(soap_ctx *c) {
     return SOAP_OK;
   }
   HIPCHK(c, hipSetDevice(c->device));
+  if (c->kind == SOAP_GRID_LINEAR) return integrate_linear(c);
   // sets: H1s, H1f, H2f ((1+LB) limb) -> + H1a; solar: H1s, H1f, L_0..L_{K-1} -> + H1a
   const bool solar = c->act.solar_mode != 0;
   const int nsets = solar ? 2 + c->solar_k : 3;
@@ -912,6 +1029,7 @@ int soap_integrate(soap_ctx *c) {
 int soap_get_integrate_stats(soap_ctx *c, double *bytes_min, double *bytes_phase_sum, int64_t *rows_sum) {
   if (!c) return SOAP_ERR_ARG;
   if (!c->have_raw || c->P == 0) return fail(c, SOAP_ERR_STATE, "integrate_stats: run integrate first");
+  if (c->kind != SOAP_GRID_LOG) return fail(c, SOAP_ERR_STATE, "integrate_stats: the traffic model covers log grids");
   HIPCHK(c, hipSetDevice(c->device));
   std::vector<int32_t> fl((size_t)c->P * c->n_mu);
   HIPCHK(c, hipMemcpyAsync(fl.data(), c->kflags.p, fl.size() * 4, hipMemcpyDeviceToHost, c->stream));
@@ -972,6 +1090,7 @@ int soap_get_stage_work(soap_ctx *c, int32_t stage, double *bytes, double *flops
     }
     case ST_CONV: {
       if (!c->have_lower) return fail(c, SOAP_ERR_STATE, "stage_work: run resolution first");
+      if (c->kind != SOAP_GRID_LOG) return SOAP_OK;  // per-pixel tap widths: not modelled
       double taps = 0;  // nonzero taps of the window
       std::vector<double> ct(c->res_ntp);
       HIPCHK(c, hipMemcpyAsync(ct.data(), c->ctap.p, ct.size() * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1089,7 +1208,6 @@ static int conv_window(const std::vector<double> &w, double sigma_ms) {
 }
 
 int soap_resolution(soap_ctx *c, double sigma_ms, double quiet_weight) {
-  (void)quiet_weight;  // weight of the quiet-star form on linear grids only
   if (!c) return SOAP_ERR_ARG;
   if (!c->have_quiet) return fail(c, SOAP_ERR_STATE, "resolution: quiet star missing");
   if (c->P > 0 && !c->have_raw) return fail(c, SOAP_ERR_STATE, "resolution: raw spectra missing");
@@ -1099,6 +1217,42 @@ int soap_resolution(soap_ctx *c, double sigma_ms, double quiet_weight) {
   if (wid < 1 || 2 * (int64_t)wid + 2 >= c->n)
     return fail(c, SOAP_ERR_ARG, "resolution: window %d vs %lld pixels", wid, (long long)c->n);
   int rc;
+  if (c->kind == SOAP_GRID_LINEAR) {
+    // conv_inst with per-pixel widths, SOAP_resolution.cu:76-95 (quiet form: SOAP_initialize.cu:78-96)
+    ConvLinArgs cl{};
+    cl.n = c->n;
+    cl.wid = wid;
+    cl.n_tiles = (int)((c->n + conv_linear_tile() - 1) / conv_linear_tile());
+    cl.n_active = 3 * c->P;
+    cl.quiet_sel[0] = 1;  // frame slots [flux, bconv, tot] = [SED, normalised, SED]
+    cl.quiet_sel[1] = 0;
+    cl.quiet_sel[2] = 1;
+    cl.c1 = 1 - sqrt((1 + sigma_ms / kC) / (1 - sigma_ms / kC));
+    cl.quiet_weight = quiet_weight;
+    cl.wave = c->d_wave;
+    int n_spec = 3 * (c->P + 1);
+    if ((rc = dalloc(c, &c->d_lower, (size_t)n_spec * c->n)) ||
+        (rc = ensure(c, c->part, (size_t)n_spec * cl.n_tiles * 2 * 8)) || (rc = ensure(c, c->mm, (size_t)n_spec * 2 * 8)))
+      return rc;
+    double *part = (double *)c->part.p;
+    {
+      StageTimer tm(c, ST_CONV);
+      hipError_t e = launch_conv_linear(cl, n_spec, c->d_quiet, c->d_raw, c->d_lower, part, c->stream);
+      if (e == hipErrorInvalidValue) return fail(c, SOAP_ERR_LIMIT, "resolution: window %d too wide", wid);
+      HIPCHK(c, e);
+    }
+    {
+      StageTimer tm(c, ST_MINMAX);
+      HIPCHK(c, launch_minmax(n_spec, cl.n_tiles, part, (double *)c->mm.p, c->stream));
+    }
+    {
+      StageTimer tm(c, ST_NORM);
+      HIPCHK(c, launch_norm(c->n, n_spec, (double *)c->mm.p, c->d_lower, c->stream));
+    }
+    c->have_lower = true;
+    c->have_ccf = c->have_fit = false;
+    return SOAP_OK;
+  }
   if (c->res_sigma != sigma_ms || c->res_wid != wid) {
     // log-grid taps c_d = exp(d h) expm1(h)/(fs sqrt(2 pi)) exp(-(expm1(d h)/fs)^2/2)
     // (generalisation of conv_inst's 0.005 A pixel), used where c_d >= 2^-60 c_0
@@ -1202,7 +1356,9 @@ int soap_set_mask(soap_ctx *c, const double *line_l, const double *line_w, int32
   double lmax = 0;
   for (int i = 0; i < n_lines; i++) lmax = std::max(lmax, fabs(line_l[i]));
   double beta = 820. / kC;
-  double lim = 2 * lmax * (sqrt((1 + beta) / (1 - beta)) - 1) + 4 * lmax * c->expm1_h + 1e-9;
+  // width of a mask pixel: the log grid's, or the 0.005 A of the linear index formula
+  double pix = c->kind == SOAP_GRID_LOG ? lmax * c->expm1_h : 0.005;
+  double lim = 2 * lmax * (sqrt((1 + beta) / (1 - beta)) - 1) + 4 * pix + 1e-9;
   int scan = 1;
   for (int i = 0, j = 0; i < n_lines; i++) {
     while (line_l[srt[i]] - line_l[srt[j]] > lim) j++;
@@ -1210,8 +1366,12 @@ int soap_set_mask(soap_ctx *c, const double *line_l, const double *line_w, int32
   }
   c->mask_scan = scan;
   // a mask hole must span at most 22 pixels (24-bit alive masks per line)
-  double hole_px = (sqrt((1 + beta) / (1 - beta)) - 1) * 1.001 / c->h + 3;
+  double hole_px = c->kind == SOAP_GRID_LOG ? (sqrt((1 + beta) / (1 - beta)) - 1) * 1.001 / c->h + 3
+                                             : lmax * 1.001 * (sqrt((1 + beta) / (1 - beta)) - 1) / 0.005 + 3;
   if (hole_px > 22) return fail(c, SOAP_ERR_LIMIT, "set_mask: mask holes span %.1f pixels (> 22)", hole_px);
+  // the reference slices the extended-grid mask at a fixed 20000 pixels (CCF_func.py:8-9)
+  if (c->kind == SOAP_GRID_LINEAR && (long)(100.0 / c->step) != 20000)
+    return fail(c, SOAP_ERR_LIMIT, "set_mask: the linear-grid CCF needs int(100/step) == 20000 (step 0.005)");
   // appodisation window, run_all_SOAP.py:306-311
   std::vector<double> ap;
   {
@@ -1247,10 +1407,14 @@ int soap_set_mask(soap_ctx *c, const double *line_l, const double *line_w, int32
 static int build_mask(soap_ctx *c) {
   StageTimer tm(c, ST_MASK);
   int64_t n_t = c->n - 2 * (int64_t)c->trim;
-  (void)n_t;
-  HIPCHK(c, launch_mask_build(c->d_line_l, c->d_line_w, (const int32_t *)c->sorted.p, c->n_lines, c->d_vgrid, c->n_v,
-                              c->wave[c->trim], c->h, (int32_t *)c->m_start.p, (int32_t *)c->m_len.p,
-                              (double2 *)c->m_val.p, c->stream));
+  if (c->kind == SOAP_GRID_LOG)
+    HIPCHK(c, launch_mask_build(c->d_line_l, c->d_line_w, (const int32_t *)c->sorted.p, c->n_lines, c->d_vgrid, c->n_v,
+                                c->wave[c->trim], c->h, (int32_t *)c->m_start.p, (int32_t *)c->m_len.p,
+                                (double2 *)c->m_val.p, c->stream));
+  else
+    HIPCHK(c, launch_mask_build_linear(c->d_line_l, c->d_line_w, (const int32_t *)c->sorted.p, c->n_lines, c->d_vgrid,
+                                       c->n_v, c->d_wave + c->trim, n_t, c->step, (int32_t *)c->m_start.p,
+                                       (int32_t *)c->m_len.p, (double2 *)c->m_val.p, c->stream));
   HIPCHK(c, launch_mask_overwrite((const int32_t *)c->sorted.p, c->n_lines, c->n_v, c->mask_scan,
                                   (const int32_t *)c->m_start.p, (const int32_t *)c->m_len.p, (uint32_t *)c->m_alive.p,
                                   c->stream));

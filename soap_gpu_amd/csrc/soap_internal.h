@@ -86,6 +86,58 @@ struct ConvArgs {
   int32_t n_tiles;       // ceil(n / conv_tile())
 };
 
+// ------------------------------------------------------------ device helpers
+// Cell classification of one phase: loop over the phase's visible regions in
+// order, later ones overwriting earlier ones (SOAP_integration.cu:724-785,
+// spot_scan :346-387).  Returns -1 (quiet), 0 (spot) or 1 (facula).
+__device__ __forceinline__ int classify_cell(int iy, int iz, double delta_grid, const RegionDev *geo, int g0,
+                                             int g1, double *r_cos_out) {
+#pragma clang fp contract(off)
+  double y = -1. + iy * delta_grid;
+  double z = -1. + iz * delta_grid;
+  if (!(z * z + y * y < 1.)) return -1;
+  double xa0 = sqrt(1. - (y * y + z * z));
+  int res = -1;
+  for (int r = g0; r < g1; r++) {
+    const RegionDev &g = geo[r];
+    if (iy < g.y0 || iy >= g.y1 || iz < g.z0 || iz >= g.z1) continue;
+    double xi0 = g.r0 * xa0 + g.r1 * y + g.r2 * z;
+    if (xi0 * xi0 >= g.oms2) res = g.type;
+  }
+  *r_cos_out = xa0;
+  return res;
+}
+
+// Facula temperature of a cell at r_cos (spot_scan, SOAP_integration.cu:376).
+__device__ __forceinline__ int plage_temp(int t_sum, double rc) {
+#pragma clang fp contract(off)
+  return (int)(t_sum + 0.9 - 407.7 * rc + 190.9 * (rc * rc));
+}
+
+
+// Linear-grid gather path (soap_linear.hip)
+struct LinArgs {
+  int64_t n;
+  int32_t n_mu, grid;
+  double lambda0, step;       // step of the reference's index formula (argv step_size)
+  const double *wave;         // [n]
+  int32_t off_lo, span;       // source indices staged per tile: [j0 + off_lo, j0 + off_lo + span)
+  const double *delta, *limb, *lbm;
+};
+
+struct LinSolar {
+  int32_t on, t_sum, t_norm;  // Planck-ratio facula intensity (SOAP_integration_solar.cu:433)
+};
+
+struct ConvLinArgs {
+  int64_t n;
+  int32_t wid, n_tiles;
+  int32_t n_active;           // 3 P active spectra first, then the quiet frame slots
+  int32_t quiet_sel[3];       // quiet spectrum of each frame slot
+  double c1;                  // 1 - sqrt((1+beta)/(1-beta)), beta = sigma/c
+  double quiet_weight;        // w of the quiet form (active form: 0.005)
+  const double *wave;
+};
 }  // namespace soap
 
 // launchers (soap_kernels.hip)
@@ -135,4 +187,22 @@ hipError_t launch_ccf_reduce(const double *part, int nc, int n_spec, int n_v, co
                              hipStream_t s);
 hipError_t launch_fit(const double *v, int n_v, const double *y, int n_ccf, double *par, double *model,
                       double *depth, double *bis, int32_t *iters, hipStream_t s);
+// linear grids
+hipError_t launch_disk_cells_by_bin(int ncell, int n_mu, const double *limb, const double *delta,
+                                    const int32_t *region, int count_only, int32_t *counts, const int32_t *offsets,
+                                    int32_t *cells, hipStream_t s);
+hipError_t launch_active_cells(int grid, int n_mu, int n_phase, const int32_t *region, const RegionDev *geo,
+                               const int32_t *geo_off, const int32_t *box, const int64_t *list_off, int32_t *cells,
+                               int32_t *bin_off, hipStream_t s);
+hipError_t launch_gather_quiet_linear(const LinArgs &a, const double *q, const double *qs, const int32_t *cells,
+                                      const int32_t *bin_off, double *out, hipStream_t s);
+hipError_t launch_gather_active_linear(const LinArgs &a, const FirArgs &f, const int32_t *cells,
+                                       const int64_t *list_off, const int32_t *bin_off, int n_phase,
+                                       const LinSolar &sol, hipStream_t s);
+hipError_t launch_conv_linear(const ConvLinArgs &c, int n_spec, const double *quiet, const double *raw, double *o_out,
+                              double *part, hipStream_t s);
+int conv_linear_tile();
+hipError_t launch_mask_build_linear(const double *line_l, const double *line_w, const int32_t *sorted, int n_lines,
+                                    const double *vgrid, int n_v, const double *wave_t, int64_t n_t, double step,
+                                    int32_t *m_start, int32_t *m_len, double2 *m_wv, hipStream_t s);
 }  // namespace soap

@@ -216,33 +216,6 @@ hipError_t launch_taps_quiet(const TapAccArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// Cell classification of one phase: loop over the phase's visible regions in
-// order, later ones overwriting earlier ones (SOAP_integration.cu:724-785,
-// spot_scan :346-387).  Returns -1 (quiet), 0 (spot) or 1 (facula).
-__device__ __forceinline__ int classify_cell(int iy, int iz, double delta_grid, const RegionDev *geo, int g0,
-                                             int g1, double *r_cos_out) {
-#pragma clang fp contract(off)
-  double y = -1. + iy * delta_grid;
-  double z = -1. + iz * delta_grid;
-  if (!(z * z + y * y < 1.)) return -1;
-  double xa0 = sqrt(1. - (y * y + z * z));
-  int res = -1;
-  for (int r = g0; r < g1; r++) {
-    const RegionDev &g = geo[r];
-    if (iy < g.y0 || iy >= g.y1 || iz < g.z0 || iz >= g.z1) continue;
-    double xi0 = g.r0 * xa0 + g.r1 * y + g.r2 * z;
-    if (xi0 * xi0 >= g.oms2) res = g.type;
-  }
-  *r_cos_out = xa0;
-  return res;
-}
-
-// Facula temperature of a cell at r_cos (spot_scan, SOAP_integration.cu:376).
-__device__ __forceinline__ int plage_temp(int t_sum, double rc) {
-#pragma clang fp contract(off)
-  return (int)(t_sum + 0.9 - 407.7 * rc + 190.9 * (rc * rc));
-}
-
 // Active taps of every phase (job = phase): spot cells add limb to set 0,
 // facula cells add limb to set 1 and (1+LB) limb to set 2 (the intensity factor
 // of fast_intensity_fac, SOAP_integration.cu:317-323,424-440).

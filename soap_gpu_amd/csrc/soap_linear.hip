@@ -1,0 +1,514 @@
+// gfx950 kernels of the linear-wavelength-grid path (the reference's native
+// 0.005 A seed format).  On a linear grid a Doppler shift is not a constant
+// pixel offset, so the per-cell shift + linear interpolation
+// (fast_shift / fast_interp_new, SOAP_integration.cu:232-281) is evaluated in
+// gather form for every (cell, pixel): pixel j takes the source index
+// i(j) = max{i : ceil(i (1 + fbeta) + fbeta lambda_0 / step) <= j} and the
+// reference's interpolation expression at i.  Cells are processed per mu bin
+// from bin-sorted cell lists, with the bin's seed-row segments and the
+// wavelengths of the tile staged in LDS.  The convolution (conv_inst,
+// SOAP_resolution.cu:76-95) has per-pixel Gaussian widths; the mask CCF uses
+// the reference's literal index arithmetic (CCF_func.py:45-91).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "soap_internal.h"
+
+namespace soap {
+
+constexpr int kLTPB = 256;
+
+// ------------------------------------------------------------ cell lists
+// Stable compaction of the block's predicate over 256 candidates: returns this
+// thread's slot among the block's selected candidates, *total the count.
+__device__ __forceinline__ int block_compact(bool pred, int *s_wave, int *total) {
+  unsigned long long m = __ballot(pred);
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int before = __popcll(m & ((1ull << lane) - 1ull));
+  if (lane == 0) s_wave[wv] = __popcll(m);
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+    if (w < wv) off += s_wave[w];
+    tot += s_wave[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + before;
+}
+
+// Disk cells of mu bin k = blockIdx.x in map order.  count_only: counts[k] only;
+// else writes the indices at cells + offsets[k].
+__global__ __launch_bounds__(kLTPB) void k_disk_cells_by_bin(int ncell, const double *limb, const double *delta,
+                                                             const int32_t *region, int count_only,
+                                                             int32_t *counts, const int32_t *offsets,
+                                                             int32_t *cells) {
+  __shared__ int s_wave[kLTPB / 64];
+  int k = blockIdx.x;
+  int run = 0;
+  for (int c0 = 0; c0 < ncell; c0 += kLTPB) {
+    int idx = c0 + threadIdx.x;
+    bool on = idx < ncell && !(limb[idx] == -999.0 && delta[idx] == -999.0) && region[idx] == k;
+    int tot;
+    int slot = block_compact(on, s_wave, &tot);
+    if (on && !count_only) cells[offsets[k] + run + slot] = idx;
+    run += tot;
+  }
+  if (count_only && threadIdx.x == 0) counts[k] = run;
+}
+
+hipError_t launch_disk_cells_by_bin(int ncell, int n_mu, const double *limb, const double *delta,
+                                    const int32_t *region, int count_only, int32_t *counts, const int32_t *offsets,
+                                    int32_t *cells, hipStream_t s) {
+  k_disk_cells_by_bin<<<n_mu, kLTPB, 0, s>>>(ncell, limb, delta, region, count_only, counts, offsets, cells);
+  return hipGetLastError();
+}
+
+// Active cells of phase p = blockIdx.x, by mu bin then map order, each entry
+// idx | type << 30 (type 0 spot, 1 facula); bin boundaries in bin_off[p][n_mu+1].
+// The phase's list starts at cells + list_off[p] (capacity = its scan box).
+__global__ __launch_bounds__(kLTPB) void k_active_cells(int grid, int n_mu, const int32_t *region,
+                                                        const RegionDev *geo, const int32_t *geo_off,
+                                                        const int32_t *box, const int64_t *list_off, int32_t *cells,
+                                                        int32_t *bin_off) {
+  __shared__ int s_wave[kLTPB / 64];
+  int p = blockIdx.x;
+  const int32_t *bx = box + p * 4;
+  int y0 = bx[0], y1 = bx[1], z0 = bx[2], z1 = bx[3];
+  int nz = z1 - z0;
+  long nbox = (long)(y1 - y0) * nz;
+  int g0 = geo_off[p], g1 = geo_off[p + 1];
+  int32_t *out = cells + list_off[p];
+  double dg = 2. / grid;
+  int run = 0;
+  for (int k = 0; k < n_mu; k++) {
+    if (threadIdx.x == 0) bin_off[(long)p * (n_mu + 1) + k] = run;
+    for (long c0 = 0; c0 < nbox; c0 += kLTPB) {
+      long c = c0 + threadIdx.x;
+      int ty = -1, idx = 0;
+      if (c < nbox) {
+        int iy = y0 + (int)(c / nz), iz = z0 + (int)(c % nz);
+        idx = iy * grid + iz;
+        if (region[idx] == k) {
+          double rc;
+          ty = classify_cell(iy, iz, dg, geo, g0, g1, &rc);
+        }
+      }
+      int tot;
+      int slot = block_compact(ty >= 0, s_wave, &tot);
+      if (ty >= 0) out[run + slot] = idx | (ty << 30);
+      run += tot;
+    }
+  }
+  if (threadIdx.x == 0) bin_off[(long)p * (n_mu + 1) + n_mu] = run;
+}
+
+hipError_t launch_active_cells(int grid, int n_mu, int n_phase, const int32_t *region, const RegionDev *geo,
+                               const int32_t *geo_off, const int32_t *box, const int64_t *list_off, int32_t *cells,
+                               int32_t *bin_off, hipStream_t s) {
+  if (n_phase == 0) return hipSuccess;
+  k_active_cells<<<n_phase, kLTPB, 0, s>>>(grid, n_mu, region, geo, geo_off, box, list_off, cells, bin_off);
+  return hipGetLastError();
+}
+
+// mu of a cell from its grid position (as classify_cell computes it)
+__device__ __forceinline__ double cell_rcos(int idx, int grid) {
+#pragma clang fp contract(off)
+  int iy = idx / grid, iz = idx % grid;
+  double dg = 2. / grid;
+  double y = -1. + iy * dg, z = -1. + iz * dg;
+  return sqrt(1. - (y * y + z * z));
+}
+
+// ------------------------------------------------------------ shift on a linear grid
+struct LinShift {
+  bool ident;            // delta == 0: the row itself (SOAP_integration.cu:262-265)
+  double opf, fl, c1;    // 1 + fbeta, fbeta lambda_0 / step, 1 - sqrt((1+beta)/(1-beta))
+};
+
+__device__ __forceinline__ LinShift lin_shift(double delta_kms, double lambda0, double step) {
+#pragma clang fp contract(off)
+  LinShift s;
+  s.ident = delta_kms == 0.0;
+  double beta = delta_kms * 1000. / kC;
+  double sq = sqrt((1 + beta) / (1 - beta));
+  double fbeta = -1 * (1 - sq);
+  s.c1 = 1 - sq;
+  s.opf = 1 + fbeta;
+  s.fl = fbeta * lambda0 / step;
+  return s;
+}
+
+// lambda + Delta_lambda(delta 1000, lambda) (SOAP_integration.cu:240)
+__device__ __forceinline__ double lin_lshift(double lam, const LinShift &s) {
+#pragma clang fp contract(off)
+  return lam + -1 * lam * s.c1;
+}
+
+// max{i : ceil(i (1+fbeta) + fbeta lambda_0/step) <= j}
+__device__ __forceinline__ long lin_gather_index(long j, const LinShift &s) {
+#pragma clang fp contract(off)
+  long i = (long)floor((j - s.fl) / s.opf);
+  for (int it = 0; it < 8 && (long)ceil(i * s.opf + s.fl) > j; it++) i--;
+  for (int it = 0; it < 8 && (long)ceil((i + 1) * s.opf + s.fl) <= j; it++) i++;
+  return i;
+}
+
+// Staged tile: source indices [base, base + span) of up to 4 rows and the
+// wavelengths (linear continuation outside the grid), with global fallbacks.
+struct LinTile {
+  const double *rows;  // [nrow][span] in LDS
+  const double *wl;    // [span] in LDS
+  long base;
+  int span;
+  long n;
+  const double *wave;
+  __device__ double wave_at(long i) const {
+    long li = i - base;
+    if (li >= 0 && li < span) return wl[li];
+    if (i < 0) return wave[0] + (double)i * (wave[1] - wave[0]);
+    if (i >= n) return wave[n - 1] + (double)(i - (n - 1)) * (wave[n - 1] - wave[n - 2]);
+    return wave[i];
+  }
+  __device__ double row_at(int r, const double *grow, long i) const {
+    long li = i - base;
+    if (li >= 0 && li < span) return rows[(long)r * span + li];
+    return (i >= 0 && i < n) ? grow[i] : 0.0;
+  }
+};
+
+__device__ void lin_stage(double *dst, const double *grow, long base, int span, long n) {
+  for (int e = threadIdx.x; e < span; e += blockDim.x) {
+    long i = base + e;
+    dst[e] = (i >= 0 && i < n) ? grow[i] : 0.0;
+  }
+}
+
+__device__ void lin_stage_wave(double *dst, const double *wave, long base, int span, long n) {
+  for (int e = threadIdx.x; e < span; e += blockDim.x) {
+    long i = base + e;
+    double v;
+    if (i < 0)
+      v = wave[0] + (double)i * (wave[1] - wave[0]);
+    else if (i >= n)
+      v = wave[n - 1] + (double)(i - (n - 1)) * (wave[n - 1] - wave[n - 2]);
+    else
+      v = wave[i];
+    dst[e] = v;
+  }
+}
+
+// Quiet star on a linear grid: out[0][j] = sum_cells limb Shift(Sq_k)(j), out[1] for
+// the SED cube (itot's cell loop, SOAP_initialize.cu:339-395), per tile of kLTPB
+// pixels, bins outer, cells of the bin inner.
+__global__ __launch_bounds__(kLTPB) void k_gather_quiet_linear(LinArgs a, const double *q, const double *qs,
+                                                               const int32_t *cells, const int32_t *bin_off,
+                                                               double *out) {
+  extern __shared__ double lds[];  // rows [2][span], wave [span]
+  long j0 = (long)blockIdx.x * kLTPB;
+  long j = j0 + threadIdx.x;
+  long base = j0 + a.off_lo;
+  int span = a.span;
+  double *rw = lds, *wl = lds + 2 * span;
+  lin_stage_wave(wl, a.wave, base, span, a.n);
+  LinTile T{rw, wl, base, span, a.n, a.wave};
+  double lam_j = a.wave[j < a.n ? j : a.n - 1];
+  double acc_q = 0.0, acc_s = 0.0;
+  for (int k = 0; k < a.n_mu; k++) {
+    int c0 = bin_off[k], c1 = bin_off[k + 1];
+    if (c0 == c1) continue;  // uniform
+    const double *gq = q + (long)k * a.n, *gs = qs + (long)k * a.n;
+    __syncthreads();
+    lin_stage(rw, gq, base, span, a.n);
+    lin_stage(rw + span, gs, base, span, a.n);
+    __syncthreads();
+    for (int c = c0; c < c1; c++) {
+      int idx = cells[c];
+      double L = a.limb[idx];
+      LinShift s = lin_shift(a.delta[idx], a.lambda0, a.step);
+      if (s.ident) {
+        acc_q += T.row_at(0, gq, j) * L;
+        acc_s += T.row_at(1, gs, j) * L;
+        continue;
+      }
+      long i = lin_gather_index(j, s);
+      double ls0 = lin_lshift(T.wave_at(i), s), ls1 = lin_lshift(T.wave_at(i + 1), s);
+      double t = (lam_j - ls0) / (ls1 - ls0);
+      double a0 = T.row_at(0, gq, i), a1 = T.row_at(0, gq, i + 1);
+      double b0 = T.row_at(1, gs, i), b1 = T.row_at(1, gs, i + 1);
+      acc_q += (a0 + (a1 - a0) * t) * L;
+      acc_s += (b0 + (b1 - b0) * t) * L;
+    }
+  }
+  if (j < a.n) {
+    out[j] = acc_q;
+    out[a.n + j] = acc_s;
+  }
+}
+
+hipError_t launch_gather_quiet_linear(const LinArgs &a, const double *q, const double *qs, const int32_t *cells,
+                                      const int32_t *bin_off, double *out, hipStream_t s) {
+  size_t lds = (size_t)3 * a.span * sizeof(double);
+  if (lds > 65536 || a.span < kLTPB) return hipErrorInvalidValue;
+  unsigned nb = (unsigned)((a.n + kLTPB - 1) / kLTPB);
+  k_gather_quiet_linear<<<nb, kLTPB, lds, s>>>(a, q, qs, cells, bin_off, out);
+  return hipGetLastError();
+}
+
+// Active deltas of phase p on a linear grid (spot_integrate + fast_flux / bconv /
+// tot, SOAP_integration.cu:285-307,392-447; solar variant :433 of
+// SOAP_integration_solar.cu): per active cell, Q = Shift(Sq_k), QS = Shift(SEDq_k),
+// P = Shift(Sspot_k or Sfac_k), all x limb; intensity I = C_spot (spot),
+// (1+LB) C_fac or B(T_cell)/B(T_norm) C_fac (facula); flux += QS - I Q,
+// bconv += Q - P, tot += QS - I P.  Blocks are numbered so the phases of a tile
+// run back to back on one XCD.
+__global__ __launch_bounds__(kLTPB) void k_gather_active_linear(LinArgs a, FirArgs f, const int32_t *cells,
+                                                                const int64_t *list_off, const int32_t *bin_off,
+                                                                int n_phase, int n_tiles, LinSolar sol) {
+  extern __shared__ double lds[];  // rows [4][span]: q, qs, spot, facula; wave [span]
+  long b = blockIdx.x;
+  long slot = b / 8;
+  long tile = (slot / n_phase) * 8 + b % 8;
+  int p = (int)(slot % n_phase);
+  if (tile >= n_tiles) return;  // uniform over the block
+  long j0 = tile * kLTPB;
+  long j = j0 + threadIdx.x;
+  long jc = j < a.n ? j : a.n - 1;
+  long base = j0 + a.off_lo;
+  int span = a.span;
+  double *rw = lds, *wl = lds + 4 * span;
+  lin_stage_wave(wl, a.wave, base, span, a.n);
+  LinTile T{rw, wl, base, span, a.n, a.wave};
+  double lam_j = a.wave[jc];
+  double cs = f.sps ? f.sps[jc] : 0.0, cf = f.fas ? f.fas[jc] : 0.0;
+  double hc = kPlanckH * kC;
+  double en = sol.on ? expm1(hc / (lam_j * kBoltzK * (double)sol.t_norm * 1e-10)) : 0.0;
+  double flux = 0.0, bconv = 0.0, tot = 0.0;
+  const int32_t *pc = cells + list_off[p];
+  const int32_t *bo = bin_off + (long)p * (a.n_mu + 1);
+  for (int k = 0; k < a.n_mu; k++) {
+    int c0 = bo[k], c1 = bo[k + 1];
+    if (c0 == c1) continue;  // uniform
+    const long ko = (long)k * a.n;
+    const double *g[4] = {f.q + ko, f.qs + ko, f.sp ? f.sp + ko : f.q + ko, f.fa ? f.fa + ko : f.q + ko};
+    __syncthreads();
+    for (int r = 0; r < 4; r++) lin_stage(rw + (long)r * span, g[r], base, span, a.n);
+    __syncthreads();
+    for (int c = c0; c < c1; c++) {
+      int e = pc[c];
+      int idx = e & 0x3fffffff, ty = e >> 30;
+      double L = a.limb[idx];
+      int r_act = ty == 0 ? 2 : 3;
+      LinShift s = lin_shift(a.delta[idx], a.lambda0, a.step);
+      double vq, vs, va;
+      if (s.ident) {
+        vq = T.row_at(0, g[0], j) * L;
+        vs = T.row_at(1, g[1], j) * L;
+        va = T.row_at(r_act, g[r_act], j) * L;
+      } else {
+        long i = lin_gather_index(j, s);
+        double ls0 = lin_lshift(T.wave_at(i), s), ls1 = lin_lshift(T.wave_at(i + 1), s);
+        double t = (lam_j - ls0) / (ls1 - ls0);
+        double q0 = T.row_at(0, g[0], i), q1 = T.row_at(0, g[0], i + 1);
+        double s0 = T.row_at(1, g[1], i), s1 = T.row_at(1, g[1], i + 1);
+        double a0 = T.row_at(r_act, g[r_act], i), a1 = T.row_at(r_act, g[r_act], i + 1);
+        vq = (q0 + (q1 - q0) * t) * L;
+        vs = (s0 + (s1 - s0) * t) * L;
+        va = (a0 + (a1 - a0) * t) * L;
+      }
+      double I;
+      if (ty == 0) {
+        I = cs;  // B(T_spot)/B(int T_spot) = 1
+      } else if (sol.on) {
+        double T_cell = (double)plage_temp(sol.t_sum, cell_rcos(idx, a.grid));
+        I = en / expm1(hc / (lam_j * kBoltzK * T_cell * 1e-10)) * cf;
+      } else {
+        I = (1 + a.lbm[idx]) * cf;
+      }
+      flux = flux + vs - I * vq;
+      bconv = bconv + vq - va;
+      tot = tot + vs - va * I;
+    }
+  }
+  if (j < a.n) {
+    double *o = f.out + (long)p * 3 * a.n;
+    o[j] = flux;
+    o[a.n + j] = bconv;
+    o[2 * a.n + j] = tot;
+  }
+}
+
+hipError_t launch_gather_active_linear(const LinArgs &a, const FirArgs &f, const int32_t *cells,
+                                       const int64_t *list_off, const int32_t *bin_off, int n_phase,
+                                       const LinSolar &sol, hipStream_t s) {
+  if (n_phase == 0) return hipSuccess;
+  size_t lds = (size_t)5 * a.span * sizeof(double);
+  if (lds > 65536 || a.span < kLTPB) return hipErrorInvalidValue;
+  int n_tiles = (int)((a.n + kLTPB - 1) / kLTPB);
+  long tiles8 = ((long)n_tiles + 7) / 8 * 8;
+  long blocks = tiles8 * n_phase;
+  if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
+  k_gather_active_linear<<<(unsigned)blocks, kLTPB, lds, s>>>(a, f, cells, list_off, bin_off, n_phase, n_tiles,
+                                                              sol);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ resolution
+// conv_inst on a linear grid (SOAP_initialize.cu:78-96, SOAP_resolution.cu:76-95):
+// o_i = sum_{d=-wid}^{wid-2} w (1 - A_{i+d}) / (sig_i sqrt(2 pi)) exp(-d^2 0.005^2 / (2 sig_i^2)),
+// sig_i = Delta_lambda(sigma, lambda_i), for wid < i < n-1-wid (0 elsewhere).
+// The Gaussian factors follow g_{d+1} = g_d u_d, u_{d+1} = u_d v from g_0 = 1,
+// u_0 = exp(-a), v = exp(-2a), a = 0.005^2 / (2 sig_i^2) (relative error of
+// g_d below |d| ulp).  Block = kLTPB pixels of one spectrum; spectrum sp =
+// blockIdx.y: A = quiet[sel] - raw (active) or the quiet spectrum itself.
+__global__ __launch_bounds__(kLTPB) void k_conv_linear(ConvLinArgs c, const double *quiet, const double *raw,
+                                                       double *o_out, double *part) {
+  extern __shared__ double xs[];  // 1 - A over [i0 - wid, i0 + kLTPB + wid)
+  long sp = blockIdx.y;
+  long i0 = (long)blockIdx.x * kLTPB;
+  int span = kLTPB + 2 * c.wid;
+  const double *Q, *D = nullptr;
+  double w;
+  if (sp < c.n_active) {
+    int t = (int)(sp % 3);
+    Q = quiet + (long)(t == 1 ? 0 : 1) * c.n;  // bconv uses the normalised quiet star
+    D = raw + sp * c.n;
+    w = 0.005;
+  } else {
+    Q = quiet + (long)c.quiet_sel[sp - c.n_active] * c.n;
+    w = c.quiet_weight;
+  }
+  double amin = HUGE_VAL, omax = -60000.0;
+  for (int e = threadIdx.x; e < span; e += blockDim.x) {
+    long m = i0 - c.wid + e;
+    double v = 1.0;
+    if (m >= 0 && m < c.n) {
+      double A = D ? Q[m] - D[m] : Q[m];
+      v = 1.0 - A;
+      if (m >= i0 && m < i0 + kLTPB) amin = A < amin ? A : amin;
+    }
+    xs[e] = v;
+  }
+  __syncthreads();
+  long i = i0 + threadIdx.x;
+  double o = 0.0;
+  if (i < c.n && i > c.wid && i + c.wid < c.n - 1) {
+    double sig = -1 * c.wave[i] * c.c1;  // Delta_lambda(sigma, lambda_i)
+    double aa = 0.005 * 0.005 / (2 * sig * sig);
+    double v = exp(-2 * aa);
+    const double *x = xs + threadIdx.x + c.wid;  // x[d] = 1 - A_{i+d}
+    double acc = x[0];
+    double g = 1.0, u = exp(-aa);
+    for (int d = 1; d <= c.wid; d++) {
+      g *= u;
+      u *= v;
+      double pair = x[-d];
+      if (d <= c.wid - 2) pair += x[d];
+      acc = fma(g, pair, acc);
+    }
+    o = w * acc / (sig * sqrt(2. * kPi));
+  }
+  if (i < c.n) {
+    o_out[sp * c.n + i] = o;
+    if (o != 0.0 && omax < o) omax = o;
+  }
+  // block min / max partials
+  __shared__ double s_min[kLTPB / 64], s_max[kLTPB / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    double x1 = __shfl_xor(amin, off), x2 = __shfl_xor(omax, off);
+    amin = x1 < amin ? x1 : amin;
+    omax = x2 > omax ? x2 : omax;
+  }
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) {
+    s_min[wv] = amin;
+    s_max[wv] = omax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double a0 = s_min[0], b0 = s_max[0];
+    for (int k = 1; k < kLTPB / 64; k++) {
+      a0 = s_min[k] < a0 ? s_min[k] : a0;
+      b0 = s_max[k] > b0 ? s_max[k] : b0;
+    }
+    part[(sp * c.n_tiles + blockIdx.x) * 2] = a0;
+    part[(sp * c.n_tiles + blockIdx.x) * 2 + 1] = b0;
+  }
+}
+
+hipError_t launch_conv_linear(const ConvLinArgs &c, int n_spec, const double *quiet, const double *raw, double *o_out,
+                              double *part, hipStream_t s) {
+  if (n_spec == 0) return hipSuccess;
+  size_t lds = (size_t)(kLTPB + 2 * c.wid) * sizeof(double);
+  if (lds > 65536) return hipErrorInvalidValue;
+  dim3 grid((unsigned)c.n_tiles, (unsigned)n_spec);
+  k_conv_linear<<<grid, kLTPB, lds, s>>>(c, quiet, raw, o_out, part);
+  return hipGetLastError();
+}
+
+int conv_linear_tile() { return kLTPB; }
+
+// ------------------------------------------------------------ CCF mask
+// calc_mask on a linear grid with the reference's literal index arithmetic
+// (CCF_func.py:45-91): pixel positions on the extended grid of run_all_SOAP.py:
+// 292-295 (no_points = 20000 on each side, step 0.005 in the index formula),
+// converted to trimmed-spectrum positions (the slice mask[20000:]); lines with
+// aa < 1 or bb + 1 >= n_ext are skipped (len 0).
+__device__ __forceinline__ double ext_wave(const double *wt, long n_t, double step, long np, long k) {
+  double begin = wt[0] - step * np;
+  if (k < np) {
+    if (k == np - 1) return wt[0];
+    double st = (wt[0] - begin) / (double)(np - 1);
+    return (double)k * st + begin;
+  }
+  if (k < np + n_t) return wt[k - np];
+  long kk = k - np - n_t;
+  double stop = wt[n_t - 1] + step * np;
+  if (kk == np - 1) return stop;
+  double st = (stop - wt[n_t - 1]) / (double)(np - 1);
+  return (double)kk * st + wt[n_t - 1];
+}
+
+__global__ void k_mask_build_linear(const double *line_l, const double *line_w, const int32_t *sorted, int n_lines,
+                                    const double *vgrid, int n_v, const double *wave_t, long n_t, double step,
+                                    int32_t *m_start, int32_t *m_len, double2 *m_wv) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n_v * n_lines) return;
+  int q = (int)(e / n_v), iv = (int)(e % n_v);
+  int li = sorted[q];
+  const long np = (long)(100.0 / step);
+  const long n_ext = n_t + 2 * np;
+  double begin_wave = wave_t[0] - step * np;
+  double ls = line_l[li] * (1 + vgrid[iv] / kC);
+  double beta = 820. / kC;
+  double hw = ls * (sqrt((1 + beta) / (1 - beta)) - 1);
+  double bgn = ls - hw / 2., end = ls + hw / 2.;
+  long aa = (long)ceil((bgn - begin_wave) / 0.005);
+  long bb = (long)(ceil((end - begin_wave) / 0.005) - 1);
+  double w = line_w[li];
+  if (aa < 1 || bb + 1 >= n_ext || bb - aa + 2 > 24 || bb - aa + 2 < 1) {
+    m_start[e] = 0;
+    m_len[e] = 0;
+    m_wv[e] = make_double2(0.0, 0.0);
+    return;
+  }
+  double fb = fabs(ext_wave(wave_t, n_t, step, np, aa) - bgn) /
+              (ext_wave(wave_t, n_t, step, np, aa) - ext_wave(wave_t, n_t, step, np, aa - 1));
+  double fa = fabs(ext_wave(wave_t, n_t, step, np, bb) - end) /
+              (ext_wave(wave_t, n_t, step, np, bb + 1) - ext_wave(wave_t, n_t, step, np, bb));
+  m_start[e] = (int32_t)(aa - 1 - 20000);
+  m_len[e] = (int32_t)(bb - aa + 2);
+  m_wv[e] = make_double2(w * fb, w * fa);
+}
+
+hipError_t launch_mask_build_linear(const double *line_l, const double *line_w, const int32_t *sorted, int n_lines,
+                                    const double *vgrid, int n_v, const double *wave_t, int64_t n_t, double step,
+                                    int32_t *m_start, int32_t *m_len, double2 *m_wv, hipStream_t s) {
+  long tot = (long)n_v * n_lines;
+  if (tot == 0) return hipSuccess;
+  k_mask_build_linear<<<(unsigned)((tot + kLTPB - 1) / kLTPB), kLTPB, 0, s>>>(
+      line_l, line_w, sorted, n_lines, vgrid, n_v, wave_t, n_t, step, m_start, m_len, m_wv);
+  return hipGetLastError();
+}
+
+}  // namespace soap

@@ -362,7 +362,10 @@ class Engine:
 
 def setup_workload(eng: Engine, w, with_mask: bool = True, trim: int = 210):
     """Upload a synth.Workload: grid, mu bounds, cubes, activity, mask."""
-    eng.set_wave(w.wave, GRID_LOG)
+    if w.grid_kind == GRID_LINEAR:
+        eng.set_wave(w.wave, GRID_LINEAR, step=w.meta["step"])
+    else:
+        eng.set_wave(w.wave, GRID_LOG)
     eng.set_mu_bounds(w.mu_bounds)
     eng.set_cubes(w.cubes)
     eng.disk_init(w.star)

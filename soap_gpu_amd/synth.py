@@ -66,6 +66,16 @@ def log_grid(lmin: float, lmax: float, n: int | None = None, dv_kms: float | Non
     return lmin * np.exp(np.arange(n, dtype=np.float64) * h)
 
 
+def linear_grid(lmin: float, n: int, step: float = 0.005) -> np.ndarray:
+    """lambda_j = lambda_min + j step (the reference's native seed format, 0.005 A)."""
+    return lmin + step * np.arange(n, dtype=np.float64)
+
+
+def is_linear(wave: np.ndarray) -> bool:
+    d = np.diff(wave)
+    return bool(np.all(np.abs(d - d[0]) <= 1e-9 * np.abs(wave[1:])))
+
+
 def mu_bins(n_mu: int):
     bounds = 1.0 - np.arange(1, n_mu, dtype=np.float64) / n_mu  # descending, n_mu-1 values
     centres = 1.0 - (np.arange(n_mu, dtype=np.float64) + 0.5) / n_mu
@@ -92,12 +102,18 @@ def sed_continuum(wave: np.ndarray, t_star: float = 5778.0) -> np.ndarray:
 
 def line_row(wave: np.ndarray, lines: dict, depth_scale: float, shift_ms: float,
              nsig: float = 6.0) -> np.ndarray:
-    """prod_l (1 - min(d_l*scale, 0.98) exp(-(lambda - lambda_l')^2 / 2 sigma_l^2)) on a log grid."""
+    """prod_l (1 - min(d_l*scale, 0.98) exp(-(lambda - lambda_l')^2 / 2 sigma_l^2)) on a
+    log-uniform or a linear grid."""
     n = wave.shape[0]
-    h = np.log(wave[-1] / wave[0]) / (n - 1)
     lc = lines["center"] * (1.0 + shift_ms / C_MS)
-    sig_px = (lines["sigma_kms"] * 1000.0 / C_MS) / h
-    xc = np.log(lc / wave[0]) / h
+    if is_linear(wave):
+        step = (wave[-1] - wave[0]) / (n - 1)
+        sig_px = lc * (lines["sigma_kms"] * 1000.0 / C_MS) / step
+        xc = (lc - wave[0]) / step
+    else:
+        h = np.log(wave[-1] / wave[0]) / (n - 1)
+        sig_px = (lines["sigma_kms"] * 1000.0 / C_MS) / h
+        xc = np.log(lc / wave[0]) / h
     half = int(np.ceil(nsig * sig_px.max())) + 1
     offs = np.arange(-half, half + 1)
     idx = np.floor(xc).astype(np.int64)[:, None] + offs[None, :]
@@ -218,3 +234,24 @@ def small(seed: int = 7, grid: int = 48, n: int = 4096, n_mu: int = 6, n_phase: 
     return Workload("small", wave, 1, bounds, cubes, lines, lines["center"].copy(), lines["depth"].copy(),
                     _star(grid, vsini, incl=incl, w_ratio=w_ratio), _act(), phases,
                     [list(regions) for _ in phases], sigma_inst_ms(115000), vel_grid(20000.0, 500.0))
+
+
+def linear_small(seed: int = 11, grid: int = 48, n: int = 4000, n_mu: int = 6, n_phase: int = 4,
+                 n_lines: int = 50, vsini: float = 3.0, incl: float = 70.0, regions=None,
+                 lmin: float = 5000.0) -> Workload:
+    """Tiny workload on the reference's native linear 0.005 A grid."""
+    rng = np.random.default_rng(seed)
+    wave = linear_grid(lmin, n)
+    lmax = wave[-1]
+    # lines away from the edges: a mask hole shifted across the first trimmed pixel
+    # meets the duplicated join point of the extended grid (run_all_SOAP.py:292-295)
+    # and gets an infinite edge weight, as in the reference
+    lines = draw_lines(rng, n_lines, lmin + 5.0, lmax - 5.0)
+    bounds, _ = mu_bins(n_mu)
+    cubes = make_cubes(wave, lines, n_mu)
+    if regions is None:
+        regions = [(20.0, 10.0, 0.25, 1.0), (25.0, 15.0, 0.15, 0.0), (-10.0, 200.0, 0.2, 0.0)]
+    phases = np.arange(n_phase, dtype=np.float64) / n_phase
+    return Workload("linear_small", wave, 0, bounds, cubes, lines, lines["center"].copy(), lines["depth"].copy(),
+                    _star(grid, vsini, incl=incl), _act(), phases, [list(regions) for _ in phases],
+                    sigma_inst_ms(115000), vel_grid(20000.0, 500.0), meta={"step": 0.005})
