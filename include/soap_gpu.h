@@ -193,6 +193,35 @@ int32_t soap_num_stages(void);
  * phase (read once) + continua + 3 outputs per phase; bytes_phase_sum = the same
  * counted per phase, sum_psi 8 n (R_psi + continua + 3); rows_sum = sum_psi R_psi */
 int soap_get_integrate_stats(soap_ctx *ctx, double *bytes_min, double *bytes_phase_sum, int64_t *rows_sum);
+/* ------------------------------------------------------------ a11: differentiable path
+ * No counterpart in the reference (a differentiable forward model with its FP64
+ * adjoint, for fitting).  Model on the context's log-uniform grid, mu bounds
+ * and mask (soap_set_wave, soap_set_mu_bounds, soap_set_mask): disk grid G,
+ * rigid rotation delta = vsini y, quadratic limb darkening, per-phase filling
+ * factor maps f_p (M x M over [-1, 1]^2, bilinear) blending the quiet seed rows
+ * Q and the spot rows S scaled by the contrast:
+ *   I_p = sum_c L_c [(1 - f) Shift(Q_k) + contrast f Shift(S_k)],
+ *   F_p = instrumental convolution of I_p, CCF_p = raw mask CCF of F_p,
+ *   (RV_p, FWHM_p) = Gaussian fit of CCF_p polished to the least-squares optimum,
+ *   loss = w_spec sum (F_p - T_p)^2 over the trimmed pixels
+ *        + w_rv sum (RV_p - rv_target_p)^2 + w_fwhm sum (FWHM_p - fwhm_target_p)^2.
+ * Array arguments named d_* are device (HIP) pointers: seeds [n_mu][n], maps
+ * [P][M][M], target [P][n]; they must stay valid until soap_diff_backward. */
+typedef struct soap_diff_params {
+  double vsini_kms, u1, u2, contrast;
+} soap_diff_params;
+int soap_diff_setup(soap_ctx *ctx, int32_t grid, int32_t n_phase, int32_t map_res, double sigma_ms);
+/* rv_target, fwhm_target, loss, rv, fwhm: host arrays ([P], scalar, [P], [P]) */
+int soap_diff_forward(soap_ctx *ctx, const soap_diff_params *prm, const double *d_seed_q, const double *d_seed_s,
+                      const double *d_maps, const double *d_target, const double *rv_target,
+                      const double *fwhm_target, double w_spec, double w_rv, double w_fwhm, double *loss, double *rv,
+                      double *fwhm);
+/* gradients of the last forward's loss: grad4 (host) = d/d(vsini, u1, u2, contrast);
+ * d_grad_q, d_grad_s [n_mu][n] and d_grad_maps [P][M][M] on the device (overwritten) */
+int soap_diff_backward(soap_ctx *ctx, double *grad4, double *d_grad_q, double *d_grad_s, double *d_grad_maps);
+/* host copies of the last forward: what 0 = F [P][n], 1 = raw CCF [P][n_v], 2 = (c, k, RV, FWHM) [P][4] */
+int soap_diff_get(soap_ctx *ctx, int32_t what, double *out);
+
 /* Algorithmic work of one run of a stage (index as soap_stage_name) on the
  * current inputs: HBM bytes that must move (each input read once, each output
  * written once) and FP64 flops (2 per multiply-add the algorithm needs, padding

@@ -77,6 +77,13 @@ def lib():
         L.orc_appodisation.argtypes = [ctypes.c_int, _dp]
         L.orc_ccf_finish.argtypes = [ctypes.c_int, _dp, _dp]
         L.orc_fit_ccf.argtypes = [_dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]
+        L.orc_gauss_hess.argtypes = [_dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_gauss_polish.argtypes = [_dp, _dp, ctypes.c_int, _dp]
+        _d_ = ctypes.c_double
+        L.orc_diff_forward.argtypes = ([_dp, ctypes.c_long, ctypes.c_int, ctypes.c_int, _dp, _d_, _d_, _d_, _d_,
+                                        _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _d_, ctypes.c_int, _dp,
+                                        ctypes.c_int, _dp, _dp, ctypes.c_long, _dp, _dp, _dp, _d_, _d_, _d_,
+                                        _dp, _dp, _dp, _dp, ctypes.c_int])
         _lib = L
     return _lib
 
@@ -296,3 +303,54 @@ def fit_ccf(v, y):
     st = lib().orc_fit_ccf(vp, yp, n, par.ctypes.data_as(_dp), model.ctypes.data_as(_dp),
                            depth.ctypes.data_as(_dp), bis.ctypes.data_as(_dp), it.ctypes.data_as(_ip))
     return dict(par=par, model=model, depth=depth, bis=bis, iters=int(it[0]), status=int(st))
+
+
+# ------------------------------------------------------------------- a11 (differentiable model)
+def gauss_hess(x, y, theta):
+    """(gradient[4], Hessian[4,4], J[n,4], sum r^2) of 0.5 sum r^2 for the BIS_FIT2 Gaussian."""
+    xx, xp = _d(x)
+    yy, yp = _d(y)
+    th, tp = _d(theta)
+    g = np.zeros(4)
+    H = np.zeros((4, 4))
+    J = np.zeros((len(xx), 4))
+    r2 = np.zeros(1)
+    lib().orc_gauss_hess(xp, yp, len(xx), tp, g.ctypes.data_as(_dp), H.ctypes.data_as(_dp),
+                         J.ctypes.data_as(_dp), r2.ctypes.data_as(_dp))
+    return g, H, J, float(r2[0])
+
+
+def gauss_polish(x, y, theta):
+    xx, xp = _d(x)
+    yy, yp = _d(y)
+    th = np.array(theta, dtype=np.float64, copy=True)
+    lib().orc_gauss_polish(xp, yp, len(xx), th.ctypes.data_as(_dp))
+    return th
+
+
+def diff_forward(wave, grid, mu_bounds, vsini, u1, u2, kappa, cube_q, cube_s, maps, sigma_ms, trim, vgrid,
+                 line_l, line_w, target, rv_t, fw_t, w_spec, w_rv, w_fw, nthreads=None):
+    """Forward model + loss of the differentiable path; returns dict(loss, F[P,n], ccf[P,nv], theta[P,4])."""
+    w, wp = _d(wave)
+    n = len(w)
+    mb, mbp = _d(mu_bounds)
+    q, qp = _d(cube_q)
+    s, sp = _d(cube_s)
+    mp_, mpp = _d(maps)
+    P, M = mp_.shape[0], mp_.shape[1]
+    v, vp = _d(vgrid)
+    ll, llp = _d(line_l)
+    lw, lwp = _d(line_w)
+    t, tp = _d(target)
+    rt, rtp = _d(rv_t)
+    ft, ftp = _d(fw_t)
+    F = np.zeros((P, n))
+    cc = np.zeros((P, len(v)))
+    th = np.zeros((P, 4))
+    loss = np.zeros(1)
+    lib().orc_diff_forward(wp, n, int(grid), len(mb) + 1, mbp, float(vsini), float(u1), float(u2), float(kappa),
+                           qp, sp, P, M, mpp, float(sigma_ms), int(trim), vp, len(v), llp, lwp, len(ll), tp, rtp,
+                           ftp, float(w_spec), float(w_rv), float(w_fw), F.ctypes.data_as(_dp),
+                           cc.ctypes.data_as(_dp), th.ctypes.data_as(_dp), loss.ctypes.data_as(_dp),
+                           nthreads or nthreads_default())
+    return dict(loss=float(loss[0]), F=F, ccf=cc, theta=th)

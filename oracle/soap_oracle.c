@@ -945,3 +945,260 @@ int orc_fit_ccf(const double *v, const double *y, int n, double *par5, double *m
   free(J);
   return status;
 }
+
+/* ----------------------------------- a11: differentiable model (no reference counterpart) */
+
+/* Exact gradient and Hessian of 0.5 sum r^2 for the Gaussian model of BIS_FIT2.cpp:31-81,
+ * f = c + k exp(-(x-x0)^2 / (2 s^2)), s = fwhm / (2 sqrt(2 ln 2)), theta = (c, k, x0, fwhm),
+ * over absolute velocities x.  J (n x 4) optional. */
+void orc_gauss_hess(const double *x, const double *y, int n, const double *th, double *g4, double H[4][4],
+                    double *J, double *res_sum2) {
+  double c = th[0], k = th[1], x0 = th[2], fw = th[3];
+  double s = fw / (2 * sqrt(2 * log(2.0)));
+  double r2 = 0;
+  for (int a = 0; a < 4; a++) {
+    g4[a] = 0;
+    for (int b = 0; b < 4; b++) H[a][b] = 0;
+  }
+  for (int i = 0; i < n; i++) {
+    double u = (x[i] - x0) / s, e = exp(-u * u / 2);
+    double f = c + k * e, r = f - y[i];
+    double j4[4] = {1.0, e, k * e * u / s, k * e * u * u / fw};
+    double h[4][4] = {{0}};
+    h[1][2] = h[2][1] = e * u / s;
+    h[1][3] = h[3][1] = e * u * u / fw;
+    h[2][2] = k * e * (u * u - 1) / (s * s);
+    h[2][3] = h[3][2] = k * e * u * (u * u - 2) / (s * fw);
+    h[3][3] = k * e * u * u * (u * u - 3) / (fw * fw);
+    for (int a = 0; a < 4; a++) {
+      g4[a] += j4[a] * r;
+      for (int b = 0; b < 4; b++) H[a][b] += j4[a] * j4[b] + r * h[a][b];
+      if (J) J[i * 4 + a] = j4[a];
+    }
+    r2 += r * r;
+  }
+  if (res_sum2) *res_sum2 = r2;
+}
+
+/* 4x4 solve by Gaussian elimination with partial pivoting; returns 0 on success */
+int orc_solve4_dense(const double A_in[4][4], const double *b, double *x) {
+  double A[4][5];
+  for (int i = 0; i < 4; i++) {
+    for (int j = 0; j < 4; j++) A[i][j] = A_in[i][j];
+    A[i][4] = b[i];
+  }
+  for (int col = 0; col < 4; col++) {
+    int piv = col;
+    for (int r = col + 1; r < 4; r++)
+      if (fabs(A[r][col]) > fabs(A[piv][col])) piv = r;
+    if (A[piv][col] == 0.0) return -1;
+    if (piv != col)
+      for (int j = 0; j < 5; j++) {
+        double t = A[col][j];
+        A[col][j] = A[piv][j];
+        A[piv][j] = t;
+      }
+    for (int r = col + 1; r < 4; r++) {
+      double f = A[r][col] / A[col][col];
+      for (int j = col; j < 5; j++) A[r][j] -= f * A[col][j];
+    }
+  }
+  for (int i = 3; i >= 0; i--) {
+    double s = A[i][4];
+    for (int j = i + 1; j < 4; j++) s -= A[i][j] * x[j];
+    x[i] = s / A[i][i];
+  }
+  return 0;
+}
+
+/* Newton steps on the exact Hessian from theta (damped when a step does not
+ * reduce the residual), then two undamped Newton steps on the gradient, which
+ * fix the optimum to rounding (a residual test alone leaves it loose by
+ * ~sqrt(eps) in flat directions). */
+static void orc_gauss_newton2(const double *x, const double *y, int n, double *th) {
+  for (int it = 0; it < 2; it++) {
+    double g[4], H[4][4], r2, mg[4], dx[4];
+    orc_gauss_hess(x, y, n, th, g, H, NULL, &r2);
+    for (int a = 0; a < 4; a++) mg[a] = -g[a];
+    if (orc_solve4_dense(H, mg, dx) != 0) return;
+    for (int a = 0; a < 4; a++) th[a] += dx[a];
+  }
+}
+
+int orc_gauss_polish(const double *x, const double *y, int n, double *th) {
+  double g[4], H[4][4], r2;
+  orc_gauss_hess(x, y, n, th, g, H, NULL, &r2);
+  for (int it = 0; it < 60; it++) {
+    double mu = 0.0, dx[4], tn[4], r2n = r2;
+    int ok = 0;
+    for (int tries = 0; tries < 30; tries++) {
+      double Hd[4][4], mg[4];
+      for (int a = 0; a < 4; a++) {
+        mg[a] = -g[a];
+        for (int b = 0; b < 4; b++) Hd[a][b] = H[a][b] + (a == b ? mu * (fabs(H[a][a]) + 1e-300) : 0.0);
+      }
+      if (orc_solve4_dense(Hd, mg, dx) == 0) {
+        for (int a = 0; a < 4; a++) tn[a] = th[a] + dx[a];
+        double gn[4], Hn[4][4];
+        orc_gauss_hess(x, y, n, tn, gn, Hn, NULL, &r2n);
+        if (r2n <= r2 * (1 + 1e-15)) {
+          ok = 1;
+          break;
+        }
+      }
+      mu = mu == 0.0 ? 1e-6 : mu * 10;
+    }
+    if (!ok) {
+      orc_gauss_newton2(x, y, n, th);
+      return it;
+    }
+    int small = 1;
+    for (int a = 0; a < 4; a++)
+      if (fabs(dx[a]) > 1e-12 * fabs(th[a]) + 1e-300) small = 0;
+    for (int a = 0; a < 4; a++) th[a] = tn[a];
+    orc_gauss_hess(x, y, n, th, g, H, NULL, &r2);
+    if (small) {
+      orc_gauss_newton2(x, y, n, th);
+      return it + 1;
+    }
+  }
+  orc_gauss_newton2(x, y, n, th);
+  return 60;
+}
+
+/* Forward model of the differentiable path (the adjoint's reference):
+ *   cells of the G x G disk grid on the disk (y^2 + z^2 <= 1), y = -1 + 2 iy/G,
+ *   z = -1 + 2 iz/G, mu = sqrt(1 - y^2 - z^2), mu bin as orc_disk_init,
+ *   limb L = 1 - u1 (1-mu) - u2 (1-mu)^2, rigid rotation delta = vsini y [km/s];
+ *   filling factor phi_pc = bilinear sample of map p (M x M over [-1, 1]^2);
+ *   I_p = sum_c [(1 - phi) Shift(L Q_k) + kappa phi Shift(L S_k)] with the log-grid
+ *   shift of orc_shift_value (SOAP_integration.cu:232-281);
+ *   F_p = conv(I_p) with the log-grid taps of orc_lower_resolution (no spec_norm),
+ *   zero outside wid < i < n-1-wid;
+ *   CCF_p = raw mask CCF of F_p[trim, n-trim) (orc_ccf, CCF_func.py:5-91);
+ *   theta_p = (c, k, x0, fwhm) of the Gaussian fit (orc_fit_ccf) polished to the
+ *   optimum (orc_gauss_polish); RV_p = x0, FWHM_p = fwhm;
+ *   loss = w_spec sum_p sum_{trim <= j < n-trim} (F_p - T_p)^2
+ *        + w_rv sum_p (RV_p - rv_t_p)^2 + w_fw sum_p (FWHM_p - fw_t_p)^2.
+ * Outputs F [P][n], ccf [P][n_v], theta [P][4] (each optional) and loss. */
+int orc_diff_forward(const double *wave, long n, int grid, int n_mu, const double *mu_bounds, double vsini,
+                     double u1, double u2, double kappa, const double *cube_q, const double *cube_s, int n_phase,
+                     int map_res, const double *maps, double sigma_ms, int trim, const double *vgrid, int n_v,
+                     const double *line_l, const double *line_w, long n_lines, const double *target,
+                     const double *rv_t, const double *fw_t, double w_spec, double w_rv, double w_fw, double *F_out,
+                     double *ccf_out, double *theta_out, double *loss_out, int nthreads) {
+  const double pi = ORC_PI;
+  long ncell = (long)grid * grid;
+  int M = map_res;
+  /* cells */
+  long non = 0;
+  for (long idx = 0; idx < ncell; idx++) {
+    int iy = (int)(idx / grid), iz = (int)(idx % grid);
+    double y = -1. + iy * (2. / grid), z = -1. + iz * (2. / grid);
+    if (y * y + z * z <= 1.) non++;
+  }
+  long *cidx = (long *)malloc(sizeof(long) * (non > 0 ? non : 1));
+  int *ck = (int *)malloc(sizeof(int) * (non > 0 ? non : 1));
+  double *cL = (double *)malloc(sizeof(double) * (non > 0 ? non : 1));
+  double *cy = (double *)malloc(sizeof(double) * (non > 0 ? non : 1));
+  double *cz = (double *)malloc(sizeof(double) * (non > 0 ? non : 1));
+  orc_shift *csh = (orc_shift *)malloc(sizeof(orc_shift) * (non > 0 ? non : 1));
+  non = 0;
+  for (long idx = 0; idx < ncell; idx++) {
+    int iy = (int)(idx / grid), iz = (int)(idx % grid);
+    double y = -1. + iy * (2. / grid), z = -1. + iz * (2. / grid);
+    if (!(y * y + z * z <= 1.)) continue;
+    double mu = sqrt(1. - (y * y + z * z));
+    int k = n_mu - 1;
+    for (int b = 0; b < n_mu - 1; b++)
+      if (mu > mu_bounds[b]) {
+        k = b;
+        break;
+      }
+    cidx[non] = idx;
+    ck[non] = k;
+    cL[non] = 1. - u1 * (1 - mu) - u2 * (1 - mu) * (1 - mu);
+    cy[non] = y;
+    cz[non] = z;
+    csh[non] = orc_shift_setup(ORC_GRID_LOG, wave, n, 0.0, vsini * y);
+    non++;
+  }
+  int wid = orc_conv_window(wave, sigma_ms);
+  double h = orc_grid_h(wave, n);
+  double fs = orc_delta_lambda(sigma_ms, 1.0);
+  double *ctap = (double *)malloc(sizeof(double) * (2 * (size_t)wid + 1));
+  for (int d = -wid; d <= wid; d++) {
+    double u = expm1(d * h) / fs;
+    ctap[d + wid] = exp(d * h) * expm1(h) / (fs * sqrt(2. * pi)) * exp(-u * u / 2);
+  }
+  long n_t = n - 2 * (long)trim;
+  double loss = 0.0;
+  double *I = (double *)malloc(sizeof(double) * n);
+  double *F = (double *)malloc(sizeof(double) * n);
+  double *cc = (double *)malloc(sizeof(double) * n_v);
+  double *phi = (double *)malloc(sizeof(double) * (non > 0 ? non : 1));
+  for (int p = 0; p < n_phase; p++) {
+    const double *mp = maps + (size_t)p * M * M;
+    for (long c = 0; c < non; c++) {
+      double gy = (cy[c] + 1) * 0.5 * (M - 1), gz = (cz[c] + 1) * 0.5 * (M - 1);
+      int i0 = (int)floor(gy), j0 = (int)floor(gz);
+      if (i0 > M - 2) i0 = M - 2;
+      if (j0 > M - 2) j0 = M - 2;
+      if (i0 < 0) i0 = 0;
+      if (j0 < 0) j0 = 0;
+      double fy = gy - i0, fz = gz - j0;
+      phi[c] = (1 - fy) * (1 - fz) * mp[i0 * M + j0] + fy * (1 - fz) * mp[(i0 + 1) * M + j0] +
+               (1 - fy) * fz * mp[i0 * M + j0 + 1] + fy * fz * mp[(i0 + 1) * M + j0 + 1];
+    }
+    int nchunk = nthreads > 0 ? nthreads * 4 : 32;
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
+    for (int ch = 0; ch < nchunk; ch++) {
+      long j0, j1;
+      orc_chunk(n, nchunk, ch, &j0, &j1);
+      for (long j = j0; j < j1; j++) I[j] = 0.0;
+      for (long c = 0; c < non; c++) {
+        const double *rq = cube_q + (long)ck[c] * n, *rs = cube_s + (long)ck[c] * n;
+        double a = 1 - phi[c], b = kappa * phi[c];
+        for (long j = j0; j < j1; j++)
+          I[j] += a * orc_shift_value(&csh[c], wave, n, rq, j, cL[c]) +
+                  b * orc_shift_value(&csh[c], wave, n, rs, j, cL[c]);
+      }
+    }
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
+    for (long i = 0; i < n; i++) {
+      double acc = 0.0;
+      if ((i > wid) && (i + wid < n - 1))
+        for (long m = i - wid; m < i + wid - 1; m++) acc += ctap[m - i + wid] * I[m];
+      F[i] = acc;
+    }
+    if (F_out) memcpy(F_out + (size_t)p * n, F, sizeof(double) * n);
+    for (long j = trim; j < n - trim; j++) {
+      double r = F[j] - target[(size_t)p * n + j];
+      loss += w_spec * r * r;
+    }
+    orc_ccf(ORC_GRID_LOG, wave + trim, n_t, h, vgrid, n_v, line_l, line_w, n_lines, F + trim, cc, nthreads);
+    if (ccf_out) memcpy(ccf_out + (size_t)p * n_v, cc, sizeof(double) * n_v);
+    double par[5];
+    orc_fit_ccf(vgrid, cc, n_v, par, NULL, NULL, NULL, NULL);
+    int inear = 0;
+    for (int i = 1; i < n_v; i++)
+      if (fabs(vgrid[i] - par[3]) < fabs(vgrid[inear] - par[3])) inear = i;
+    double th[4] = {par[0], cc[inear] < par[0] ? -par[1] : par[1], par[3], par[4]};
+    orc_gauss_polish(vgrid, cc, n_v, th);
+    if (theta_out) memcpy(theta_out + (size_t)p * 4, th, sizeof(th));
+    loss += w_rv * (th[2] - rv_t[p]) * (th[2] - rv_t[p]) + w_fw * (th[3] - fw_t[p]) * (th[3] - fw_t[p]);
+  }
+  *loss_out = loss;
+  free(cidx);
+  free(ck);
+  free(cL);
+  free(cy);
+  free(cz);
+  free(csh);
+  free(ctap);
+  free(I);
+  free(F);
+  free(cc);
+  free(phi);
+  return 0;
+}

@@ -91,6 +91,19 @@ void orc_ccf_finish(int n_v, const double *appod, double *ccf);
 int orc_fit_ccf(const double *v, const double *y, int n, double *par5, double *model, double *depth91,
                 double *bis91, int *n_iter);
 
+/* a11: differentiable model (no reference counterpart): Gaussian-fit Hessian,
+ * Newton polish, forward model and loss (definitions in soap_oracle.c) */
+void orc_gauss_hess(const double *x, const double *y, int n, const double *th, double *g4, double H[4][4],
+                    double *J, double *res_sum2);
+int orc_solve4_dense(const double A_in[4][4], const double *b, double *x);
+int orc_gauss_polish(const double *x, const double *y, int n, double *th);
+int orc_diff_forward(const double *wave, long n, int grid, int n_mu, const double *mu_bounds, double vsini,
+                     double u1, double u2, double kappa, const double *cube_q, const double *cube_s, int n_phase,
+                     int map_res, const double *maps, double sigma_ms, int trim, const double *vgrid, int n_v,
+                     const double *line_l, const double *line_w, long n_lines, const double *target,
+                     const double *rv_t, const double *fw_t, double w_spec, double w_rv, double w_fw, double *F_out,
+                     double *ccf_out, double *theta_out, double *loss_out, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

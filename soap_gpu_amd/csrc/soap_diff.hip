@@ -1,0 +1,606 @@
+// gfx950 kernels of the differentiable path (no counterpart in the reference).
+//
+// Model (log-uniform grid): disk cell c with bin k, limb L = 1 - u1(1-mu) -
+// u2(1-mu)^2 and rigid-rotation velocity delta = vsini y has the integer shift m
+// and weight t of the forward engine; phi_pc is the bilinear sample of phase p's
+// M x M filling-factor map over [-1, 1]^2; the phase spectrum is
+//   I_p = sum_k sum_d HQ_pk[d] Q_k[j - d] + HS_pk[d] S_k[j - d],
+//   HQ_pk = sum_c L (1 - phi) {(1-t) at d = m, t at d = m-1},  HS_pk likewise with kappa phi,
+// F_p = conv(I_p) with the resolution taps, CCF_p = raw mask CCF of F_p, and the
+// Gaussian fit of CCF_p polished to the least-squares optimum gives RV_p, FWHM_p.
+// The adjoint runs the same structures backwards: the implicit derivative of
+// the fit (H^-1 J^T), the transposed mask and convolution, lag correlations of
+// the adjoint spectrum with the seed rows (from which every per-cell derivative
+// follows), and the transposed FIR for the seed gradients.  Shifted spectra are
+// never stored: the backward recomputes from taps and correlations.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "soap_internal.h"
+
+namespace soap {
+
+constexpr int kDT = 256;
+
+// ------------------------------------------------------------ per cell
+__global__ void k_diff_cells(DiffArgs a) {
+#pragma clang fp contract(off)
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ncell = (long)a.grid * a.grid;
+  if (idx >= ncell) return;
+  int iy = (int)(idx / a.grid), iz = (int)(idx % a.grid);
+  double y = -1. + iy * (2. / a.grid), z = -1. + iz * (2. / a.grid);
+  if (!(y * y + z * z <= 1.)) {
+    a.ck[idx] = -1;
+    return;
+  }
+  double mu = sqrt(1. - (y * y + z * z));
+  int k = a.n_mu - 1;
+  for (int b = 0; b < a.n_mu - 1; b++)
+    if (mu > a.mu_bounds[b]) {
+      k = b;
+      break;
+    }
+  double L = 1. - a.u1 * (1 - mu) - a.u2 * (1 - mu) * (1 - mu);
+  double delta = a.vsini * y;
+  double beta = delta * 1000. / kC;
+  double g = sqrt((1 + beta) / (1 - beta));
+  double fbeta = -1 * (1 - g);
+  double sh = log1p(fbeta) / a.h;
+  double mc = ceil(sh);
+  double t = expm1((mc - sh) * a.h) / a.expm1_h;
+  a.ck[idx] = k;
+  a.cm[idx] = (int32_t)mc;
+  a.ct[idx] = t;
+  a.cL[idx] = L;
+  a.cmu[idx] = mu;
+  // dt/d(delta) = -exp((m-s)h)/expm1(h) / (g^2 (1-beta)^2) * 1000/c (m piecewise constant)
+  a.cdt[idx] = -exp((mc - sh) * a.h) / a.expm1_h / (g * g * (1 - beta) * (1 - beta)) * (1000. / kC) * y;
+}
+
+// bilinear sample of map p at the cell; i0, j0, fy, fz of the sample
+__device__ __forceinline__ double map_sample(const double *mp, int M, double y, double z, int &i0, int &j0,
+                                             double &fy, double &fz) {
+#pragma clang fp contract(off)
+  double gy = (y + 1) * 0.5 * (M - 1), gz = (z + 1) * 0.5 * (M - 1);
+  i0 = (int)floor(gy);
+  j0 = (int)floor(gz);
+  i0 = i0 > M - 2 ? M - 2 : (i0 < 0 ? 0 : i0);
+  j0 = j0 > M - 2 ? M - 2 : (j0 < 0 ? 0 : j0);
+  fy = gy - i0;
+  fz = gz - j0;
+  return (1 - fy) * (1 - fz) * mp[i0 * M + j0] + fy * (1 - fz) * mp[(i0 + 1) * M + j0] +
+         (1 - fy) * fz * mp[i0 * M + j0 + 1] + fy * fz * mp[(i0 + 1) * M + j0 + 1];
+}
+
+__device__ __forceinline__ void cell_yz(long idx, int grid, double &y, double &z) {
+#pragma clang fp contract(off)
+  int iy = (int)(idx / grid), iz = (int)(idx % grid);
+  y = -1. + iy * (2. / grid);
+  z = -1. + iz * (2. / grid);
+}
+
+// taps of every phase (blockIdx.y): fixed-point sums in LDS when they fit, then global
+__global__ __launch_bounds__(kDT) void k_diff_taps(DiffArgs a, int cells_per_block, int use_lds) {
+  extern __shared__ unsigned long long lacc[];  // [n_mu][nd][2 sets][2]
+  int p = blockIdx.y;
+  long ncell = (long)a.grid * a.grid;
+  long c0 = (long)blockIdx.x * cells_per_block, c1 = min(ncell, c0 + cells_per_block);
+  long nacc = (long)a.n_mu * a.nd * 4;
+  unsigned long long *gacc = a.acc + (long)p * nacc;
+  if (use_lds) {
+    for (long e = threadIdx.x; e < nacc; e += blockDim.x) lacc[e] = 0ull;
+    __syncthreads();
+  }
+  unsigned long long *dst = use_lds ? lacc : gacc;
+  const double *mp = a.maps + (long)p * a.map_res * a.map_res;
+  for (long idx = c0 + threadIdx.x; idx < c1; idx += blockDim.x) {
+    int k = a.ck[idx];
+    if (k < 0) continue;
+    double y, z, fy, fz;
+    int i0, j0;
+    cell_yz(idx, a.grid, y, z);
+    double phi = map_sample(mp, a.map_res, y, z, i0, j0, fy, fz);
+    double L = a.cL[idx], t = a.ct[idx];
+    int d0 = a.cm[idx] - a.dmin;  // tap index of offset m; m-1 at d0-1
+    if (d0 < 1 || d0 >= a.nd) continue;  // the host bounds every shift
+    double wq = L * (1 - phi), ws = a.kappa * L * phi;
+    unsigned long long *row = dst + (long)k * a.nd * 4;
+    fx_add(row + (long)d0 * 4 + 0, wq * (1.0 - t));
+    fx_add(row + (long)(d0 - 1) * 4 + 0, wq * t);
+    fx_add(row + (long)d0 * 4 + 2, ws * (1.0 - t));
+    fx_add(row + (long)(d0 - 1) * 4 + 2, ws * t);
+  }
+  if (use_lds) {
+    __syncthreads();
+    for (long e = threadIdx.x; e < nacc; e += blockDim.x) {
+      unsigned long long v = lacc[e];
+      if (v) atomicAdd(gacc + e, v);
+    }
+  }
+}
+
+// fixed point -> taps [p][k][d][2] and the nonzero range per (p, k)
+__global__ void k_diff_taps_finalize(DiffArgs a) {
+  int p = blockIdx.x;
+  for (int k = threadIdx.x; k < a.n_mu; k += blockDim.x) {
+    int lo = a.nd, hi = -1;
+    for (int d = 0; d < a.nd; d++) {
+      const unsigned long long *e = a.acc + (((long)p * a.n_mu + k) * a.nd + d) * 4;
+      double vq = fx_value(e[0], e[1]), vs = fx_value(e[2], e[3]);
+      double *o = a.taps + (((long)p * a.n_mu + k) * a.nd + d) * 2;
+      o[0] = vq;
+      o[1] = vs;
+      if (vq != 0.0 || vs != 0.0) {
+        lo = d < lo ? d : lo;
+        hi = d;
+      }
+    }
+    a.krange[((long)p * a.n_mu + k) * 2] = lo;
+    a.krange[((long)p * a.n_mu + k) * 2 + 1] = hi;
+  }
+}
+
+// I_p[j] = sum_k sum_d HQ[d] Q_k[j - dmin - d] + HS[d] S_k[j - dmin - d]; block = (tile, p)
+__global__ __launch_bounds__(kDT) void k_diff_fir(DiffArgs a) {
+  extern __shared__ double rows[];  // [2][row_len]
+  int p = blockIdx.y;
+  long j0 = (long)blockIdx.x * kDT;
+  int nd = a.nd, row_len = kDT + nd - 1;
+  long src0 = j0 - a.dmin - nd + 1;
+  double acc = 0.0;
+  for (int k = 0; k < a.n_mu; k++) {
+    int lo = a.krange[((long)p * a.n_mu + k) * 2], hi = a.krange[((long)p * a.n_mu + k) * 2 + 1];
+    if (lo > hi) continue;  // uniform
+    __syncthreads();
+    for (int e = threadIdx.x; e < row_len; e += blockDim.x) {
+      long i = src0 + e;
+      bool in = i >= 0 && i < a.n;
+      rows[e] = in ? a.q[(long)k * a.n + i] : 0.0;
+      rows[row_len + e] = in ? a.s[(long)k * a.n + i] : 0.0;
+    }
+    __syncthreads();
+    const double *T = a.taps + ((long)p * a.n_mu + k) * nd * 2;
+    for (int di = lo; di <= hi; di++) {
+      int e = threadIdx.x + nd - 1 - di;
+      acc = fma(T[di * 2], rows[e], acc);
+      acc = fma(T[di * 2 + 1], rows[row_len + e], acc);
+    }
+  }
+  long j = j0 + threadIdx.x;
+  if (j < a.n) a.I[(long)p * a.n + j] = acc;
+}
+
+// F_p = conv(I_p) (zero outside wid < i < n-1-wid); residual partials of the
+// trimmed range per block: part[p][block]
+__global__ __launch_bounds__(kDT) void k_diff_conv(DiffArgs a, ConvArgs c, const double *target, int trim,
+                                                   double *part) {
+  extern __shared__ double xs[];  // I over [i0 + dlo, i0 + 4*kDT + ntp - 1), taps after
+  int p = blockIdx.y;
+  long i0 = (long)blockIdx.x * (4 * kDT);
+  int span = 4 * kDT + c.ntp - 1;
+  double *ct = xs + span;
+  const double *X = a.I + (long)p * a.n;
+  for (int e = threadIdx.x; e < c.ntp; e += blockDim.x) ct[e] = c.ctap[e];
+  for (int e = threadIdx.x; e < span; e += blockDim.x) {
+    long m = i0 + c.dlo + e;
+    xs[e] = (m >= 0 && m < a.n) ? X[m] : 0.0;
+  }
+  __syncthreads();
+  double acc[4] = {0, 0, 0, 0};
+  for (int k = 0; k < c.ntp; k++) {
+    double cv = ct[k];
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[b] = fma(cv, xs[threadIdx.x + b * kDT + k], acc[b]);
+  }
+  double r2 = 0.0;
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    long i = i0 + threadIdx.x + b * kDT;
+    if (i >= a.n) continue;
+    double o = ((i > c.wid) && (i + c.wid < a.n - 1)) ? acc[b] : 0.0;
+    a.F[(long)p * a.n + i] = o;
+    if (i >= trim && i < a.n - trim) {
+      double r = o - target[(long)p * a.n + i];
+      r2 += r * r;
+    }
+  }
+  __shared__ double s_r[kDT / 64];
+  for (int off = 32; off > 0; off >>= 1) r2 += __shfl_xor(r2, off);
+  if ((threadIdx.x & 63) == 0) s_r[threadIdx.x >> 6] = r2;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0;
+    for (int w = 0; w < kDT / 64; w++) s += s_r[w];
+    part[(long)p * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// ------------------------------------------------------------ Gaussian fit: polish and adjoint
+// gradient (4) and Hessian (10, upper) of 0.5 sum r^2 (BIS_FIT2 model, exact second
+// derivatives), summed over the wave's lanes; every lane receives the totals
+__device__ void gauss_gh(const double *x, const double *y, int n, const double th[4], double g[4], double H[4][4],
+                         double *r2out) {
+  double s = th[3] / (2 * sqrt(2 * log(2.0)));
+  double acc[15];
+#pragma unroll
+  for (int i = 0; i < 15; i++) acc[i] = 0.0;
+  for (int i = threadIdx.x & 63; i < n; i += 64) {
+    double u = (x[i] - th[2]) / s, e = exp(-u * u / 2);
+    double r = th[0] + th[1] * e - y[i];
+    double j4[4] = {1.0, e, th[1] * e * u / s, th[1] * e * u * u / th[3]};
+    double h12 = e * u / s, h13 = e * u * u / th[3];
+    double h22 = th[1] * e * (u * u - 1) / (s * s), h23 = th[1] * e * u * (u * u - 2) / (s * th[3]);
+    double h33 = th[1] * e * u * u * (u * u - 3) / (th[3] * th[3]);
+    int q = 0;
+    for (int a = 0; a < 4; a++) acc[q++] += j4[a] * r;
+    double hh[4][4] = {{0, 0, 0, 0}, {0, 0, h12, h13}, {0, h12, h22, h23}, {0, h13, h23, h33}};
+    for (int a = 0; a < 4; a++)
+      for (int b = a; b < 4; b++) acc[q++] += j4[a] * j4[b] + r * hh[a][b];
+    acc[14] += r * r;
+  }
+#pragma unroll
+  for (int i = 0; i < 15; i++)
+    for (int off = 32; off > 0; off >>= 1) acc[i] += __shfl_xor(acc[i], off);
+  int q = 0;
+  for (int a = 0; a < 4; a++) g[a] = acc[q++];
+  for (int a = 0; a < 4; a++)
+    for (int b = a; b < 4; b++) H[a][b] = H[b][a] = acc[q++];
+  *r2out = acc[14];
+}
+
+__device__ int solve4(const double A_in[4][4], const double *b, double *x) {
+  double A[4][5];
+  for (int i = 0; i < 4; i++) {
+    for (int j = 0; j < 4; j++) A[i][j] = A_in[i][j];
+    A[i][4] = b[i];
+  }
+  for (int col = 0; col < 4; col++) {
+    int piv = col;
+    for (int r = col + 1; r < 4; r++)
+      if (fabs(A[r][col]) > fabs(A[piv][col])) piv = r;
+    if (A[piv][col] == 0.0) return -1;
+    if (piv != col)
+      for (int j = 0; j < 5; j++) {
+        double t = A[col][j];
+        A[col][j] = A[piv][j];
+        A[piv][j] = t;
+      }
+    for (int r = col + 1; r < 4; r++) {
+      double f = A[r][col] / A[col][col];
+      for (int j = col; j < 5; j++) A[r][j] -= f * A[col][j];
+    }
+  }
+  for (int i = 3; i >= 0; i--) {
+    double s = A[i][4];
+    for (int j = i + 1; j < 4; j++) s -= A[i][j] * x[j];
+    x[i] = s / A[i][i];
+  }
+  return 0;
+}
+
+// one wave per CCF: theta from the fit parameters (signed amplitude), then
+// damped Newton steps on the exact Hessian to the least-squares optimum
+__global__ void k_fit_polish(const double *v, int n_v, const double *ccf, const double *par, double *theta) {
+  int sp = blockIdx.x;
+  const double *y = ccf + (long)sp * n_v;
+  const double *pp = par + (long)sp * 5;
+  int inear = 0;
+  for (int i = 1; i < n_v; i++)
+    if (fabs(v[i] - pp[3]) < fabs(v[inear] - pp[3])) inear = i;
+  double th[4] = {pp[0], y[inear] < pp[0] ? -pp[1] : pp[1], pp[3], pp[4]};
+  double g[4], H[4][4], r2;
+  gauss_gh(v, y, n_v, th, g, H, &r2);
+  for (int it = 0; it < 60; it++) {
+    double mu = 0.0, dx[4] = {0, 0, 0, 0}, tn[4] = {th[0], th[1], th[2], th[3]}, r2n = r2;
+    bool ok = false;
+    for (int tries = 0; tries < 30; tries++) {
+      double Hd[4][4], mg[4];
+      for (int a = 0; a < 4; a++) {
+        mg[a] = -g[a];
+        for (int b = 0; b < 4; b++) Hd[a][b] = H[a][b] + (a == b ? mu * (fabs(H[a][a]) + 1e-300) : 0.0);
+      }
+      if (solve4(Hd, mg, dx) == 0) {
+        for (int a = 0; a < 4; a++) tn[a] = th[a] + dx[a];
+        double gn[4], Hn[4][4];
+        gauss_gh(v, y, n_v, tn, gn, Hn, &r2n);
+        if (r2n <= r2 * (1 + 1e-15)) {
+          ok = true;
+          break;
+        }
+      }
+      mu = mu == 0.0 ? 1e-6 : mu * 10;
+    }
+    if (!ok) break;
+    bool small = true;
+    for (int a = 0; a < 4; a++)
+      if (fabs(dx[a]) > 1e-12 * fabs(th[a]) + 1e-300) small = false;
+    for (int a = 0; a < 4; a++) th[a] = tn[a];
+    gauss_gh(v, y, n_v, th, g, H, &r2);
+    if (small) break;
+  }
+  // two undamped Newton steps on the gradient fix the optimum to rounding
+  for (int it = 0; it < 2; it++) {
+    double mg[4], dx[4];
+    gauss_gh(v, y, n_v, th, g, H, &r2);
+    for (int a = 0; a < 4; a++) mg[a] = -g[a];
+    if (solve4(H, mg, dx) != 0) break;
+    for (int a = 0; a < 4; a++) th[a] += dx[a];
+  }
+  if (threadIdx.x == 0)
+    for (int a = 0; a < 4; a++) theta[(long)sp * 4 + a] = th[a];
+}
+
+// dl/dy_v = J_v . z with H z = dl/dtheta (implicit derivative of the optimum)
+__global__ void k_fit_adjoint(const double *v, int n_v, const double *ccf, const double *theta, const double *dth,
+                              double *gy) {
+  int sp = blockIdx.x;
+  const double *y = ccf + (long)sp * n_v;
+  double th[4], g[4], H[4][4], r2, z[4] = {0, 0, 0, 0};
+  for (int a = 0; a < 4; a++) th[a] = theta[(long)sp * 4 + a];
+  gauss_gh(v, y, n_v, th, g, H, &r2);
+  double b[4];
+  for (int a = 0; a < 4; a++) b[a] = dth[(long)sp * 4 + a];
+  solve4(H, b, z);
+  double s = th[3] / (2 * sqrt(2 * log(2.0)));
+  for (int i = threadIdx.x; i < n_v; i += blockDim.x) {
+    double u = (v[i] - th[2]) / s, e = exp(-u * u / 2);
+    gy[(long)sp * n_v + i] = z[0] + z[1] * e + z[2] * th[1] * e * u / s + z[3] * th[1] * e * u * u / th[3];
+  }
+}
+
+// ------------------------------------------------------------ backward spectra
+// G_F = 2 w_spec (F - T) on [trim, n - trim), 0 elsewhere
+__global__ void k_diff_gradF(DiffArgs a, const double *target, int trim, double w_spec, double *GF) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)a.n_phase * a.n) return;
+  long i = e % a.n;
+  GF[e] = (i >= trim && i < a.n - trim) ? 2 * w_spec * (a.F[e] - target[e]) : 0.0;
+}
+
+// transposed mask: G_F[p][trim + pixel] += gy[p][v] * weight over the live pixels of
+// every (line, velocity) entry (the pixel weights of k_ccf)
+__global__ void k_mask_transpose(int n_lines, int n_v, int64_t n_t, int trim, int64_t n, int n_phase,
+                                 const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
+                                 const double *w_sorted, const double *gy, double *GF) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n_lines * n_v) return;
+  int q = (int)(e / n_v), iv = (int)(e % n_v);
+  int st = m_start[e];
+  uint32_t pk = m_pk[e];
+  double2 wv = m_wv[e];
+  double w = w_sorted[q];
+  int ln = (int)(pk & 0xffu);
+  uint32_t al = pk >> 8;
+  for (int b = 0; b < ln; b++) {
+    long pp = st + b;
+    if (!((al >> b) & 1u) || pp < 0 || pp >= n_t) continue;
+    double val = (b == ln - 1) ? wv.y : ((b == 0) ? wv.x : w);
+    for (int p = 0; p < n_phase; p++) atomicAdd(GF + (long)p * n + trim + pp, gy[(long)p * n_v + iv] * val);
+  }
+}
+
+// G_I[m] = sum_k c_k G_F[m - dlo - k] over the valid outputs (G_F is zero elsewhere)
+__global__ __launch_bounds__(kDT) void k_diff_convT(DiffArgs a, ConvArgs c, const double *GF, double *GI) {
+  extern __shared__ double xs[];  // G_F over [m0 - dlo - ntp + 1, m0 + 4 kDT - dlo)
+  int p = blockIdx.y;
+  long m0 = (long)blockIdx.x * (4 * kDT);
+  int span = 4 * kDT + c.ntp - 1;
+  double *ct = xs + span;
+  const double *X = GF + (long)p * a.n;
+  long base = m0 - c.dlo - c.ntp + 1;
+  for (int e = threadIdx.x; e < c.ntp; e += blockDim.x) ct[e] = c.ctap[e];
+  for (int e = threadIdx.x; e < span; e += blockDim.x) {
+    long i = base + e;
+    xs[e] = (i >= 0 && i < a.n && i > c.wid && i + c.wid < a.n - 1) ? X[i] : 0.0;
+  }
+  __syncthreads();
+  double acc[4] = {0, 0, 0, 0};
+  for (int k = 0; k < c.ntp; k++) {
+    double cv = ct[k];
+    // output m = m0 + tid + b kDT reads G_F[m - dlo - k] at xs[tid + b kDT + ntp - 1 - k]
+#pragma unroll
+    for (int b = 0; b < 4; b++) acc[b] = fma(cv, xs[threadIdx.x + b * kDT + c.ntp - 1 - k], acc[b]);
+  }
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    long m = m0 + threadIdx.x + b * kDT;
+    if (m < a.n) GI[(long)p * a.n + m] = acc[b];
+  }
+}
+
+// lag correlations C[p][k][d][set] += sum_{j in chunk} G_I[p][j] X_k[j - dmin - d];
+// block = (chunk of kCorrT pixels, bin k, phase p); thread = (d, set) pair
+constexpr int kCorrT = 1024;
+__global__ __launch_bounds__(kDT) void k_diff_corr(DiffArgs a, const double *GI, double *C) {
+  extern __shared__ double lds[];  // G [kCorrT], Q and S rows [kCorrT + nd - 1] each
+  int k = blockIdx.y, p = blockIdx.z;
+  long j0 = (long)blockIdx.x * kCorrT;
+  int nd = a.nd, rl = kCorrT + nd - 1;
+  double *G = lds, *rq = lds + kCorrT, *rs = rq + rl;
+  long src0 = j0 - a.dmin - nd + 1;
+  for (int e = threadIdx.x; e < kCorrT; e += blockDim.x) {
+    long j = j0 + e;
+    G[e] = j < a.n ? GI[(long)p * a.n + j] : 0.0;
+  }
+  for (int e = threadIdx.x; e < rl; e += blockDim.x) {
+    long i = src0 + e;
+    bool in = i >= 0 && i < a.n;
+    rq[e] = in ? a.q[(long)k * a.n + i] : 0.0;
+    rs[e] = in ? a.s[(long)k * a.n + i] : 0.0;
+  }
+  __syncthreads();
+  for (int pr = threadIdx.x; pr < 2 * nd; pr += blockDim.x) {
+    int d = pr >> 1, set = pr & 1;
+    const double *r = (set ? rs : rq) + nd - 1 - d;  // X[j0 + e - dmin - d] = r[e]
+    double acc = 0.0;
+    for (int e = 0; e < kCorrT; e++) acc = fma(G[e], r[e], acc);
+    atomicAdd(C + (((long)p * a.n_mu + k) * nd + d) * 2 + set, acc);
+  }
+}
+
+// per-cell derivatives from the correlations; phases summed per cell; map
+// pixel gradients scattered; scalar partials per block: [gvsini, gu1, gu2, gkappa]
+__global__ __launch_bounds__(kDT) void k_diff_cellgrad(DiffArgs a, const double *C, double *gmaps, double *part) {
+  long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long ncell = (long)a.grid * a.grid;
+  double sv = 0, su1 = 0, su2 = 0, sk = 0;
+  if (idx < ncell && a.ck[idx] >= 0) {
+    int k = a.ck[idx];
+    int d0 = a.cm[idx] - a.dmin;
+    if (d0 >= 1 && d0 < a.nd) {
+      double y, z;
+      cell_yz(idx, a.grid, y, z);
+      double L = a.cL[idx], t = a.ct[idx], mu = a.cmu[idx];
+      double gL = 0, gt = 0;
+      int M = a.map_res;
+      for (int p = 0; p < a.n_phase; p++) {
+        const double *Cp = C + (((long)p * a.n_mu + k) * a.nd) * 2;
+        double cq0 = Cp[d0 * 2], cq1 = Cp[(d0 - 1) * 2], cs0 = Cp[d0 * 2 + 1], cs1 = Cp[(d0 - 1) * 2 + 1];
+        double aq = (1 - t) * cq0 + t * cq1, as = (1 - t) * cs0 + t * cs1;
+        double bq = cq1 - cq0, bs = cs1 - cs0;
+        int i0, j0;
+        double fy, fz;
+        double phi = map_sample(a.maps + (long)p * M * M, M, y, z, i0, j0, fy, fz);
+        double gphi = L * (a.kappa * as - aq);
+        double *gm = gmaps + (long)p * M * M;
+        atomicAdd(gm + i0 * M + j0, gphi * (1 - fy) * (1 - fz));
+        atomicAdd(gm + (i0 + 1) * M + j0, gphi * fy * (1 - fz));
+        atomicAdd(gm + i0 * M + j0 + 1, gphi * (1 - fy) * fz);
+        atomicAdd(gm + (i0 + 1) * M + j0 + 1, gphi * fy * fz);
+        gL += (1 - phi) * aq + a.kappa * phi * as;
+        gt += L * ((1 - phi) * bq + a.kappa * phi * bs);
+        sk += L * phi * as;
+      }
+      su1 = -gL * (1 - mu);
+      su2 = -gL * (1 - mu) * (1 - mu);
+      sv = gt * a.cdt[idx];  // dt/d(delta) * d(delta)/d(vsini)
+    }
+  }
+  __shared__ double s_p[4][kDT / 64];
+  double v4[4] = {sv, su1, su2, sk};
+  for (int q = 0; q < 4; q++) {
+    double x = v4[q];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if ((threadIdx.x & 63) == 0) s_p[q][threadIdx.x >> 6] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double s = 0;
+    for (int w = 0; w < kDT / 64; w++) s += s_p[threadIdx.x][w];
+    part[(long)blockIdx.x * 4 + threadIdx.x] = s;
+  }
+}
+
+// seed gradients gX_k[i] = sum_p sum_d HX_pk[d] G_I,p[i + dmin + d]; block = (tile, bin)
+__global__ __launch_bounds__(kDT) void k_diff_seedgrad(DiffArgs a, const double *GI, double *gq, double *gs) {
+  extern __shared__ double gl[];  // G_I,p over [i0 + dmin, i0 + kDT + dmin + nd - 1)
+  int k = blockIdx.y;
+  long i0 = (long)blockIdx.x * kDT;
+  int nd = a.nd, len = kDT + nd - 1;
+  double aq = 0.0, as = 0.0;
+  for (int p = 0; p < a.n_phase; p++) {
+    int lo = a.krange[((long)p * a.n_mu + k) * 2], hi = a.krange[((long)p * a.n_mu + k) * 2 + 1];
+    if (lo > hi) continue;  // uniform
+    __syncthreads();
+    for (int e = threadIdx.x; e < len; e += blockDim.x) {
+      long j = i0 + a.dmin + e;
+      gl[e] = (j >= 0 && j < a.n) ? GI[(long)p * a.n + j] : 0.0;
+    }
+    __syncthreads();
+    const double *T = a.taps + ((long)p * a.n_mu + k) * nd * 2;
+    for (int d = lo; d <= hi; d++) {
+      double g = gl[threadIdx.x + d];
+      aq = fma(T[d * 2], g, aq);
+      as = fma(T[d * 2 + 1], g, as);
+    }
+  }
+  long i = i0 + threadIdx.x;
+  if (i < a.n) {
+    gq[(long)k * a.n + i] = aq;
+    gs[(long)k * a.n + i] = as;
+  }
+}
+
+// ------------------------------------------------------------ launchers
+hipError_t launch_diff_forward_spectra(const DiffArgs &a, const ConvArgs &c, const double *target, int trim,
+                                       double *res_part, int *res_blocks, hipStream_t s) {
+  long ncell = (long)a.grid * a.grid;
+  k_diff_cells<<<(unsigned)((ncell + kDT - 1) / kDT), kDT, 0, s>>>(a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  long nacc = (long)a.n_mu * a.nd * 4;
+  e = launch_fill_u64(a.acc, (size_t)a.n_phase * nacc, s);
+  if (e != hipSuccess) return e;
+  int use_lds = nacc * 8 <= 49152;
+  int cpb = 4096;
+  dim3 gt((unsigned)((ncell + cpb - 1) / cpb), a.n_phase);
+  k_diff_taps<<<gt, kDT, use_lds ? nacc * 8 : 0, s>>>(a, cpb, use_lds);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  k_diff_taps_finalize<<<a.n_phase, 64, 0, s>>>(a);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  size_t lds_fir = (size_t)2 * (kDT + a.nd - 1) * sizeof(double);
+  if (lds_fir > 65536) return hipErrorInvalidValue;
+  dim3 gf((unsigned)((a.n + kDT - 1) / kDT), a.n_phase);
+  k_diff_fir<<<gf, kDT, lds_fir, s>>>(a);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  size_t lds_conv = (size_t)(4 * kDT + 2 * c.ntp) * sizeof(double);
+  if (lds_conv > 65536) return hipErrorInvalidValue;
+  int nb = (int)((a.n + 4 * kDT - 1) / (4 * kDT));
+  *res_blocks = nb;
+  dim3 gc((unsigned)nb, a.n_phase);
+  k_diff_conv<<<gc, kDT, lds_conv, s>>>(a, c, target, trim, res_part);
+  return hipGetLastError();
+}
+
+int diff_conv_blocks(int64_t n) { return (int)((n + 4 * kDT - 1) / (4 * kDT)); }
+int diff_cell_blocks(int64_t ncell) { return (int)((ncell + kDT - 1) / kDT); }
+
+hipError_t launch_fit_polish(const double *v, int n_v, const double *ccf, const double *par, int n_ccf, double *theta,
+                             hipStream_t s) {
+  if (n_ccf == 0) return hipSuccess;
+  k_fit_polish<<<n_ccf, 64, 0, s>>>(v, n_v, ccf, par, theta);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const double *target, int trim, double w_spec,
+                                const double *v, int n_v, const double *ccf, const double *theta, const double *dth,
+                                double *gy, int n_lines, int64_t n_t, const int32_t *m_start, const uint32_t *m_pk,
+                                const double2 *m_wv, const double *w_sorted, double *GF, double *GI, double *C,
+                                double *gmaps, double *cell_part, double *gq, double *gs, hipStream_t s) {
+  hipError_t e;
+  k_fit_adjoint<<<a.n_phase, 64, 0, s>>>(v, n_v, ccf, theta, dth, gy);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  long tot = (long)a.n_phase * a.n;
+  k_diff_gradF<<<(unsigned)((tot + kDT - 1) / kDT), kDT, 0, s>>>(a, target, trim, w_spec, GF);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  long ne = (long)n_lines * n_v;
+  if (ne > 0) {
+    k_mask_transpose<<<(unsigned)((ne + kDT - 1) / kDT), kDT, 0, s>>>(n_lines, n_v, n_t, trim, a.n, a.n_phase,
+                                                                     m_start, m_pk, m_wv, w_sorted, gy, GF);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  size_t lds_conv = (size_t)(4 * kDT + 2 * c.ntp) * sizeof(double);
+  if (lds_conv > 65536) return hipErrorInvalidValue;
+  dim3 gc((unsigned)((a.n + 4 * kDT - 1) / (4 * kDT)), a.n_phase);
+  k_diff_convT<<<gc, kDT, lds_conv, s>>>(a, c, GF, GI);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  e = launch_fill_f64(C, (size_t)a.n_phase * a.n_mu * a.nd * 2, 0.0, s);
+  if (e != hipSuccess) return e;
+  size_t lds_corr = (size_t)(kCorrT + 2 * (kCorrT + a.nd - 1)) * sizeof(double);
+  if (lds_corr > 65536) return hipErrorInvalidValue;
+  dim3 gr((unsigned)((a.n + kCorrT - 1) / kCorrT), a.n_mu, a.n_phase);
+  k_diff_corr<<<gr, kDT, lds_corr, s>>>(a, GI, C);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  e = launch_fill_f64(gmaps, (size_t)a.n_phase * a.map_res * a.map_res, 0.0, s);
+  if (e != hipSuccess) return e;
+  long ncell = (long)a.grid * a.grid;
+  k_diff_cellgrad<<<(unsigned)((ncell + kDT - 1) / kDT), kDT, 0, s>>>(a, C, gmaps, cell_part);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  size_t lds_seed = (size_t)(kDT + a.nd - 1) * sizeof(double);
+  dim3 gsd((unsigned)((a.n + kDT - 1) / kDT), a.n_mu);
+  k_diff_seedgrad<<<gsd, kDT, lds_seed, s>>>(a, GI, gq, gs);
+  return hipGetLastError();
+}
+
+}  // namespace soap

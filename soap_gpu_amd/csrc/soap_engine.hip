@@ -86,6 +86,17 @@ struct soap_ctx {
   RegionDev *d_geo = nullptr;
   int32_t *d_geo_off = nullptr, *d_box = nullptr;
   int max_col_blocks = 0;
+  // differentiable path
+  struct {
+    bool ready = false, have_fwd = false;
+    int G = 0, P = 0, M = 0, dmin = 0, nd = 0, res_blocks = 0;
+    double sigma = 0, w_spec = 0, w_rv = 0, w_fw = 0;
+    soap_diff_params prm{};
+    const double *q = nullptr, *s = nullptr, *maps = nullptr, *target = nullptr;
+    std::vector<double> rv_t, fw_t, theta;
+    DevBuf ck, cm, ct, cL, cmu, cdt, acc, taps, krange, I, F, res_part, ccf, par, model, depth, bis, iters, th,
+        dth, gy, GF, GI, C, cell_part;
+  } diff;
   // linear grids: staged source range per tile, bin-sorted disk cells, active cell lists
   int lin_off_lo = 0, lin_span = 0;
   DevBuf qcells, qbin_off, acells, abin_off, alist_off;
@@ -337,7 +348,12 @@ void soap_ctx_destroy(soap_ctx *c) {
                   c->kflags.p,  c->bins.p,    c->nbins.p,   c->part.p,     c->mm.p,     c->ctap.p,   c->qsel.p,
                   c->variant.p, c->m_start.p, c->m_len.p,   c->m_val.p,    c->m_alive.p, c->sorted.p, c->rank.p,
                   c->f_par.p,   c->f_model.p, c->f_depth.p, c->f_bis.p,    c->f_iter.p, c->ccf_part.p,
-                  c->qcells.p,  c->qbin_off.p, c->acells.p, c->abin_off.p, c->alist_off.p};
+                  c->qcells.p,  c->qbin_off.p, c->acells.p, c->abin_off.p, c->alist_off.p,
+                  c->diff.ck.p, c->diff.cm.p,  c->diff.ct.p, c->diff.cL.p,  c->diff.cmu.p,  c->diff.cdt.p,
+                  c->diff.acc.p, c->diff.taps.p, c->diff.krange.p, c->diff.I.p, c->diff.F.p, c->diff.res_part.p,
+                  c->diff.ccf.p, c->diff.par.p, c->diff.model.p, c->diff.depth.p, c->diff.bis.p, c->diff.iters.p,
+                  c->diff.th.p, c->diff.dth.p, c->diff.gy.p, c->diff.GF.p, c->diff.GI.p, c->diff.C.p,
+                  c->diff.cell_part.p};
   for (void *p : ptrs) dfree(p);
   hipStreamDestroy(c->stream);
   delete c;
@@ -1207,6 +1223,37 @@ static int conv_window(const std::vector<double> &w, double sigma_ms) {
   return (int)rndup((50. * dl) / (w[1] - w[0]), 0);
 }
 
+// Log-grid resolution taps c_d = exp(d h) expm1(h)/(fs sqrt(2 pi)) exp(-(expm1(d h)/fs)^2/2)
+// (the generalisation of conv_inst's 0.005 A pixel), kept where c_d >= 2^-60 c_0,
+// d in [-wid, wid-2] (SOAP_resolution.cu:76-95), uploaded once per (sigma, wid).
+static int log_conv_taps(soap_ctx *c, double sigma_ms, int wid) {
+  int rc;
+  if (c->res_sigma != sigma_ms || c->res_wid != wid) {
+    double fs = -1 * 1.0 * (1 - sqrt((1 + sigma_ms / kC) / (1 - sigma_ms / kC)));
+    auto tap = [&](int d) {
+      double u = expm1(d * c->h) / fs;
+      return exp(d * c->h) * c->expm1_h / (fs * sqrt(2. * kPi)) * exp(-u * u / 2);
+    };
+    double c0 = tap(0);
+    int dt = 0;
+    for (int d = 1; d <= wid; d++)
+      if (tap(d) >= c0 * 8.673617379884035e-19 || tap(-d) >= c0 * 8.673617379884035e-19) dt = d;
+    int dlo = -std::min(wid, dt), dhi = std::min(wid - 2, dt);
+    int ch = conv_chunk();
+    int ntp = ((dhi - dlo + 1) + ch - 1) / ch * ch;
+    std::vector<double> ct(ntp, 0.0);
+    for (int d = dlo; d <= dhi; d++) ct[d - dlo] = tap(d);
+    if ((rc = ensure(c, c->ctap, ct.size() * 8))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->ctap.p, ct.data(), ct.size() * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->res_sigma = sigma_ms;
+    c->res_wid = wid;
+    c->res_dlo = dlo;
+    c->res_ntp = ntp;
+  }
+  return SOAP_OK;
+}
+
 int soap_resolution(soap_ctx *c, double sigma_ms, double quiet_weight) {
   if (!c) return SOAP_ERR_ARG;
   if (!c->have_quiet) return fail(c, SOAP_ERR_STATE, "resolution: quiet star missing");
@@ -1253,31 +1300,7 @@ int soap_resolution(soap_ctx *c, double sigma_ms, double quiet_weight) {
     c->have_ccf = c->have_fit = false;
     return SOAP_OK;
   }
-  if (c->res_sigma != sigma_ms || c->res_wid != wid) {
-    // log-grid taps c_d = exp(d h) expm1(h)/(fs sqrt(2 pi)) exp(-(expm1(d h)/fs)^2/2)
-    // (generalisation of conv_inst's 0.005 A pixel), used where c_d >= 2^-60 c_0
-    double fs = -1 * 1.0 * (1 - sqrt((1 + sigma_ms / kC) / (1 - sigma_ms / kC)));
-    auto tap = [&](int d) {
-      double u = expm1(d * c->h) / fs;
-      return exp(d * c->h) * c->expm1_h / (fs * sqrt(2. * kPi)) * exp(-u * u / 2);
-    };
-    double c0 = tap(0);
-    int dt = 0;
-    for (int d = 1; d <= wid; d++)
-      if (tap(d) >= c0 * 8.673617379884035e-19 || tap(-d) >= c0 * 8.673617379884035e-19) dt = d;
-    int dlo = -std::min(wid, dt), dhi = std::min(wid - 2, dt);
-    int ch = conv_chunk();
-    int ntp = ((dhi - dlo + 1) + ch - 1) / ch * ch;
-    std::vector<double> ct(ntp, 0.0);
-    for (int d = dlo; d <= dhi; d++) ct[d - dlo] = tap(d);
-    if ((rc = ensure(c, c->ctap, ct.size() * 8))) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->ctap.p, ct.data(), ct.size() * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->res_sigma = sigma_ms;
-    c->res_wid = wid;
-    c->res_dlo = dlo;
-    c->res_ntp = ntp;
-  }
+  if ((rc = log_conv_taps(c, sigma_ms, wid))) return rc;
   ConvArgs ca{};
   ca.n = c->n;
   ca.wid = wid;
@@ -1518,6 +1541,211 @@ int soap_fit_ccf_host(soap_ctx *c, const double *v, int32_t n_v, const double *y
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   for (double *p : {dv, dy, dpar, dmod, ddep, dbis}) dfree_c(c, p);
   if (e != hipSuccess) return fail(c, SOAP_ERR_HIP, "fit_ccf_host: %s", hipGetErrorString(e));
+  return SOAP_OK;
+}
+
+// ------------------------------------------------------------------ a11: differentiable path
+int soap_diff_setup(soap_ctx *c, int32_t grid, int32_t n_phase, int32_t map_res, double sigma_ms) {
+  if (!c || grid < 2 || grid > 8192 || n_phase < 1 || map_res < 2 || map_res > 8192 || !(sigma_ms > 0))
+    return fail(c, SOAP_ERR_ARG, "diff_setup: bad arguments");
+  if (c->kind != SOAP_GRID_LOG) return fail(c, SOAP_ERR_STATE, "diff_setup: the differentiable path needs a log grid");
+  if (c->n == 0 || c->n_mu == 0) return fail(c, SOAP_ERR_STATE, "diff_setup: set wave and mu bounds first");
+  if (!c->have_mask) return fail(c, SOAP_ERR_STATE, "diff_setup: set the mask first");
+  HIPCHK(c, hipSetDevice(c->device));
+  auto &d = c->diff;
+  size_t g2 = (size_t)grid * grid;
+  int rc;
+  if ((rc = ensure(c, d.ck, g2 * 4)) || (rc = ensure(c, d.cm, g2 * 4)) || (rc = ensure(c, d.ct, g2 * 8)) ||
+      (rc = ensure(c, d.cL, g2 * 8)) || (rc = ensure(c, d.cmu, g2 * 8)) || (rc = ensure(c, d.cdt, g2 * 8)) ||
+      (rc = ensure(c, d.I, (size_t)n_phase * c->n * 8)) || (rc = ensure(c, d.F, (size_t)n_phase * c->n * 8)) ||
+      (rc = ensure(c, d.GF, (size_t)n_phase * c->n * 8)) || (rc = ensure(c, d.GI, (size_t)n_phase * c->n * 8)) ||
+      (rc = ensure(c, d.res_part, (size_t)n_phase * diff_conv_blocks(c->n) * 8)) ||
+      (rc = ensure(c, d.ccf, (size_t)n_phase * c->n_v * 8)) || (rc = ensure(c, d.par, (size_t)n_phase * 5 * 8)) ||
+      (rc = ensure(c, d.model, (size_t)n_phase * c->n_v * 8)) || (rc = ensure(c, d.depth, (size_t)n_phase * 91 * 8)) ||
+      (rc = ensure(c, d.bis, (size_t)n_phase * 91 * 8)) || (rc = ensure(c, d.iters, (size_t)n_phase * 4)) ||
+      (rc = ensure(c, d.th, (size_t)n_phase * 4 * 8)) || (rc = ensure(c, d.dth, (size_t)n_phase * 4 * 8)) ||
+      (rc = ensure(c, d.gy, (size_t)n_phase * c->n_v * 8)) ||
+      (rc = ensure(c, d.cell_part, (size_t)diff_cell_blocks((int64_t)g2) * 4 * 8)) ||
+      (rc = ensure(c, c->ccf_part, (size_t)ccf_chunks(c->n_lines) * n_phase * c->n_v * 8)))
+    return rc;
+  int wid = conv_window(c->wave, sigma_ms);
+  if (wid < 1 || 2 * (int64_t)wid + 2 >= c->n) return fail(c, SOAP_ERR_ARG, "diff_setup: window %d vs %lld pixels", wid, (long long)c->n);
+  if ((rc = log_conv_taps(c, sigma_ms, wid))) return rc;
+  d.G = grid;
+  d.P = n_phase;
+  d.M = map_res;
+  d.sigma = sigma_ms;
+  d.ready = true;
+  d.have_fwd = false;
+  return SOAP_OK;
+}
+
+static DiffArgs diff_args(soap_ctx *c) {
+  auto &d = c->diff;
+  DiffArgs a{};
+  a.n = c->n;
+  a.grid = d.G;
+  a.n_mu = c->n_mu;
+  a.n_phase = d.P;
+  a.map_res = d.M;
+  a.dmin = d.dmin;
+  a.nd = d.nd;
+  a.h = c->h;
+  a.expm1_h = c->expm1_h;
+  a.vsini = d.prm.vsini_kms;
+  a.u1 = d.prm.u1;
+  a.u2 = d.prm.u2;
+  a.kappa = d.prm.contrast;
+  a.mu_bounds = c->d_mub;
+  a.ck = (int32_t *)d.ck.p;
+  a.cm = (int32_t *)d.cm.p;
+  a.ct = (double *)d.ct.p;
+  a.cL = (double *)d.cL.p;
+  a.cmu = (double *)d.cmu.p;
+  a.cdt = (double *)d.cdt.p;
+  a.maps = d.maps;
+  a.acc = (unsigned long long *)d.acc.p;
+  a.taps = (double *)d.taps.p;
+  a.krange = (int32_t *)d.krange.p;
+  a.q = d.q;
+  a.s = d.s;
+  a.I = (double *)d.I.p;
+  a.F = (double *)d.F.p;
+  return a;
+}
+
+static ConvArgs diff_conv(soap_ctx *c) {
+  ConvArgs ca{};
+  ca.n = c->n;
+  ca.wid = c->res_wid;
+  ca.dlo = c->res_dlo;
+  ca.ntp = c->res_ntp;
+  ca.ctap = (const double *)c->ctap.p;
+  return ca;
+}
+
+int soap_diff_forward(soap_ctx *c, const soap_diff_params *prm, const double *d_q, const double *d_s,
+                      const double *d_maps, const double *d_target, const double *rv_t, const double *fw_t,
+                      double w_spec, double w_rv, double w_fw, double *loss, double *rv, double *fw) {
+  if (!c || !prm || !d_q || !d_s || !d_maps || !d_target || !rv_t || !fw_t || !loss)
+    return fail(c, SOAP_ERR_ARG, "diff_forward: null argument");
+  auto &d = c->diff;
+  if (!d.ready) return fail(c, SOAP_ERR_STATE, "diff_forward: run diff_setup first");
+  if (!(prm->vsini_kms > 0 && prm->vsini_kms < 2000))
+    return fail(c, SOAP_ERR_ARG, "diff_forward: vsini %g km/s out of range", prm->vsini_kms);
+  HIPCHK(c, hipSetDevice(c->device));
+  // taps span the shifts of delta in [-vsini, vsini] (m monotone in delta), plus margins
+  auto mshift = [&](double v) {
+    double beta = v * 1000. / kC;
+    double fb = -1 * (1 - sqrt((1 + beta) / (1 - beta)));
+    return (int)ceil(log1p(fb) / c->h);
+  };
+  int mlo = mshift(-prm->vsini_kms), mhi = mshift(prm->vsini_kms);
+  d.dmin = mlo - 2;
+  d.nd = mhi - d.dmin + 3;
+  if (d.nd > 2048) return fail(c, SOAP_ERR_LIMIT, "diff_forward: shift range of %d pixels", d.nd);
+  int rc;
+  if ((rc = ensure(c, d.acc, (size_t)d.P * c->n_mu * d.nd * 4 * 8)) ||
+      (rc = ensure(c, d.taps, (size_t)d.P * c->n_mu * d.nd * 2 * 8)) ||
+      (rc = ensure(c, d.krange, (size_t)d.P * c->n_mu * 2 * 4)) ||
+      (rc = ensure(c, d.C, (size_t)d.P * c->n_mu * d.nd * 2 * 8)))
+    return rc;
+  if (!c->mask_built && (rc = build_mask(c))) return rc;
+  d.prm = *prm;
+  d.q = d_q;
+  d.s = d_s;
+  d.maps = d_maps;
+  d.target = d_target;
+  d.w_spec = w_spec;
+  d.w_rv = w_rv;
+  d.w_fw = w_fw;
+  d.rv_t.assign(rv_t, rv_t + d.P);
+  d.fw_t.assign(fw_t, fw_t + d.P);
+  DiffArgs a = diff_args(c);
+  ConvArgs ca = diff_conv(c);
+  {
+    StageTimer tm(c, ST_FIR_ACTIVE);
+    hipError_t e = launch_diff_forward_spectra(a, ca, d_target, c->trim, (double *)d.res_part.p, &d.res_blocks,
+                                               c->stream);
+    if (e == hipErrorInvalidValue) return fail(c, SOAP_ERR_LIMIT, "diff_forward: window or shift range too wide");
+    HIPCHK(c, e);
+  }
+  {
+    StageTimer tm(c, ST_CCF);
+    HIPCHK(c, launch_ccf((const double *)d.F.p, c->n, c->trim, d.P, c->n_lines, c->n_v, c->n - 2 * (int64_t)c->trim,
+                         (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p,
+                         c->d_w_sorted, (double *)c->ccf_part.p, c->stream));
+    HIPCHK(c, launch_ccf_sum((const double *)c->ccf_part.p, ccf_chunks(c->n_lines), d.P, c->n_v, (double *)d.ccf.p,
+                             c->stream));
+  }
+  {
+    StageTimer tm(c, ST_FIT);
+    HIPCHK(c, launch_fit(c->d_vgrid, c->n_v, (const double *)d.ccf.p, d.P, (double *)d.par.p, (double *)d.model.p,
+                         (double *)d.depth.p, (double *)d.bis.p, (int32_t *)d.iters.p, c->stream));
+    HIPCHK(c, launch_fit_polish(c->d_vgrid, c->n_v, (const double *)d.ccf.p, (const double *)d.par.p, d.P,
+                                (double *)d.th.p, c->stream));
+  }
+  std::vector<double> rp((size_t)d.P * d.res_blocks);
+  d.theta.resize((size_t)d.P * 4);
+  HIPCHK(c, hipMemcpyAsync(rp.data(), d.res_part.p, rp.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d.theta.data(), d.th.p, d.theta.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double l = 0;
+  for (int p = 0; p < d.P; p++) {
+    double r2 = 0;
+    for (int b = 0; b < d.res_blocks; b++) r2 += rp[(size_t)p * d.res_blocks + b];
+    double drv = d.theta[p * 4 + 2] - d.rv_t[p], dfw = d.theta[p * 4 + 3] - d.fw_t[p];
+    l += w_spec * r2 + w_rv * drv * drv + w_fw * dfw * dfw;
+    if (rv) rv[p] = d.theta[p * 4 + 2];
+    if (fw) fw[p] = d.theta[p * 4 + 3];
+  }
+  *loss = l;
+  d.have_fwd = true;
+  return SOAP_OK;
+}
+
+int soap_diff_backward(soap_ctx *c, double *grad4, double *d_gq, double *d_gs, double *d_gmaps) {
+  if (!c || !grad4 || !d_gq || !d_gs || !d_gmaps) return fail(c, SOAP_ERR_ARG, "diff_backward: null argument");
+  auto &d = c->diff;
+  if (!d.have_fwd) return fail(c, SOAP_ERR_STATE, "diff_backward: run diff_forward first");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<double> dth((size_t)d.P * 4, 0.0);
+  for (int p = 0; p < d.P; p++) {
+    dth[p * 4 + 2] = 2 * d.w_rv * (d.theta[p * 4 + 2] - d.rv_t[p]);
+    dth[p * 4 + 3] = 2 * d.w_fw * (d.theta[p * 4 + 3] - d.fw_t[p]);
+  }
+  HIPCHK(c, hipMemcpyAsync(d.dth.p, dth.data(), dth.size() * 8, hipMemcpyHostToDevice, c->stream));
+  DiffArgs a = diff_args(c);
+  ConvArgs ca = diff_conv(c);
+  {
+    StageTimer tm(c, ST_CONV);
+    hipError_t e = launch_diff_backward(
+        a, ca, d.target, c->trim, d.w_spec, c->d_vgrid, c->n_v, (const double *)d.ccf.p, (const double *)d.th.p,
+        (const double *)d.dth.p, (double *)d.gy.p, c->n_lines, c->n - 2 * (int64_t)c->trim,
+        (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p, c->d_w_sorted,
+        (double *)d.GF.p, (double *)d.GI.p, (double *)d.C.p, d_gmaps, (double *)d.cell_part.p, d_gq, d_gs, c->stream);
+    if (e == hipErrorInvalidValue) return fail(c, SOAP_ERR_LIMIT, "diff_backward: window or shift range too wide");
+    HIPCHK(c, e);
+  }
+  int nb = diff_cell_blocks((int64_t)d.G * d.G);
+  std::vector<double> cp((size_t)nb * 4);
+  HIPCHK(c, hipMemcpyAsync(cp.data(), d.cell_part.p, cp.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int q = 0; q < 4; q++) grad4[q] = 0;
+  for (int b = 0; b < nb; b++)
+    for (int q = 0; q < 4; q++) grad4[q] += cp[(size_t)b * 4 + q];
+  return SOAP_OK;
+}
+
+int soap_diff_get(soap_ctx *c, int32_t what, double *out) {
+  if (!c || !out || what < 0 || what > 2) return fail(c, SOAP_ERR_ARG, "diff_get: bad arguments");
+  auto &d = c->diff;
+  if (!d.have_fwd) return fail(c, SOAP_ERR_STATE, "diff_get: run diff_forward first");
+  HIPCHK(c, hipSetDevice(c->device));
+  const void *src = what == 0 ? d.F.p : what == 1 ? d.ccf.p : d.th.p;
+  size_t bytes = (size_t)d.P * (what == 0 ? c->n : what == 1 ? c->n_v : 4) * 8;
+  HIPCHK(c, hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   return SOAP_OK;
 }
 

@@ -87,6 +87,31 @@ struct ConvArgs {
 };
 
 // ------------------------------------------------------------ device helpers
+// ------------------------------------------------------------ fixed point taps
+// A contribution x is split into hi = floor(x 2^32) and lo = (x 2^32 - hi) 2^40
+// (exact: power-of-two scalings and the subtraction of an integer part do not
+// round); sums of 64-bit integers are associative, so atomics give the exact,
+// order-independent sum of all contributions truncated at 2^-72.
+__device__ __forceinline__ void fx_split(double x, unsigned long long &hi, unsigned long long &lo) {
+  double y = x * 4294967296.0;
+  double fl = floor(y);
+  hi = (unsigned long long)(long long)fl;
+  lo = (unsigned long long)((y - fl) * 1099511627776.0);
+}
+
+__device__ __forceinline__ void fx_add(unsigned long long *p, double x) {
+  unsigned long long hi, lo;
+  fx_split(x, hi, lo);
+  atomicAdd(p, hi);
+  atomicAdd(p + 1, lo);
+}
+
+__device__ __forceinline__ double fx_value(unsigned long long hi, unsigned long long lo) {
+  long long h = (long long)hi + (long long)(lo >> 40);
+  unsigned long long r = lo & ((1ull << 40) - 1);
+  return (double)h * 2.3283064365386963e-10 + (double)r * 2.117582368135751e-22;  // 2^-32, 2^-72
+}
+
 // Cell classification of one phase: loop over the phase's visible regions in
 // order, later ones overwriting earlier ones (SOAP_integration.cu:724-785,
 // spot_scan :346-387).  Returns -1 (quiet), 0 (spot) or 1 (facula).
@@ -127,6 +152,22 @@ struct LinArgs {
 
 struct LinSolar {
   int32_t on, t_sum, t_norm;  // Planck-ratio facula intensity (SOAP_integration_solar.cu:433)
+};
+
+// Differentiable path (soap_diff.hip)
+struct DiffArgs {
+  int64_t n;
+  int32_t grid, n_mu, n_phase, map_res, dmin, nd;
+  double h, expm1_h, vsini, u1, u2, kappa;
+  const double *mu_bounds;       // n_mu - 1
+  int32_t *ck, *cm;              // per cell [G*G]: bin (-1 off disk), shift m
+  double *ct, *cL, *cmu, *cdt;   // weight t, limb, mu, dt/d(vsini)
+  const double *maps;            // [P][M][M]
+  unsigned long long *acc;       // [P][n_mu][nd][2 sets][hi, lo]
+  double *taps;                  // [P][n_mu][nd][2]
+  int32_t *krange;               // [P][n_mu][2]
+  const double *q, *s;           // seed rows [n_mu][n]
+  double *I, *F;                 // [P][n]
 };
 
 struct ConvLinArgs {
@@ -205,4 +246,17 @@ int conv_linear_tile();
 hipError_t launch_mask_build_linear(const double *line_l, const double *line_w, const int32_t *sorted, int n_lines,
                                     const double *vgrid, int n_v, const double *wave_t, int64_t n_t, double step,
                                     int32_t *m_start, int32_t *m_len, double2 *m_wv, hipStream_t s);
+// differentiable path
+hipError_t launch_diff_forward_spectra(const DiffArgs &a, const ConvArgs &c, const double *target, int trim,
+                                       double *res_part, int *res_blocks, hipStream_t s);
+int diff_conv_blocks(int64_t n);
+int diff_cell_blocks(int64_t ncell);
+hipError_t launch_fit_polish(const double *v, int n_v, const double *ccf, const double *par, int n_ccf, double *theta,
+                             hipStream_t s);
+hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const double *target, int trim, double w_spec,
+                                const double *v, int n_v, const double *ccf, const double *theta, const double *dth,
+                                double *gy, int n_lines, int64_t n_t, const int32_t *m_start, const uint32_t *m_pk,
+                                const double2 *m_wv, const double *w_sorted, double *GF, double *GI, double *C,
+                                double *gmaps, double *cell_part, double *gq, double *gs, hipStream_t s);
+hipError_t launch_ccf_sum(const double *part, int nc, int n_spec, int n_v, double *ccf, hipStream_t s);
 }  // namespace soap

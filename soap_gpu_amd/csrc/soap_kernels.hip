@@ -109,31 +109,6 @@ hipError_t launch_disk_init(const DiskArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
-// ------------------------------------------------------------ fixed point taps
-// A contribution x is split into hi = floor(x 2^32) and lo = (x 2^32 - hi) 2^40
-// (exact: power-of-two scalings and the subtraction of an integer part do not
-// round); sums of 64-bit integers are associative, so atomics give the exact,
-// order-independent sum of all contributions truncated at 2^-72.
-__device__ __forceinline__ void fx_split(double x, unsigned long long &hi, unsigned long long &lo) {
-  double y = x * 4294967296.0;
-  double fl = floor(y);
-  hi = (unsigned long long)(long long)fl;
-  lo = (unsigned long long)((y - fl) * 1099511627776.0);
-}
-
-__device__ __forceinline__ void fx_add(unsigned long long *p, double x) {
-  unsigned long long hi, lo;
-  fx_split(x, hi, lo);
-  atomicAdd(p, hi);
-  atomicAdd(p + 1, lo);
-}
-
-__device__ __forceinline__ double fx_value(unsigned long long hi, unsigned long long lo) {
-  long long h = (long long)hi + (long long)(lo >> 40);
-  unsigned long long r = lo & ((1ull << 40) - 1);
-  return (double)h * 2.3283064365386963e-10 + (double)r * 2.117582368135751e-22;  // 2^-32, 2^-72
-}
-
 // Shared part of the tap kernels: accumulate into an LDS window of WD tap offsets
 // [base, base+WD) per (set, bin); offsets outside it go straight to global.
 struct TapWindow {
@@ -1188,6 +1163,23 @@ hipError_t launch_ccf_reduce(const double *part, int nc, int n_spec, int n_v, co
   while (rt < n_v && rt < 1024) rt <<= 1;
   rt = rt < 64 ? 64 : rt;
   k_ccf_reduce<<<n_spec, rt, 0, s>>>(part, nc, n_spec, n_v, appod, ccf);
+  return hipGetLastError();
+}
+
+// raw CCF (no normalisation): ccf[sp][v] = sum over line chunks, in chunk order
+__global__ void k_ccf_sum(const double *part, int n_chunks, int n_spec, int n_v, double *ccf) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (long)n_spec * n_v) return;
+  long sp = e / n_v, iv = e % n_v;
+  double a = 0.0;
+  for (int c = 0; c < n_chunks; c++) a += part[((long)c * n_spec + sp) * n_v + iv];
+  ccf[e] = a;
+}
+
+hipError_t launch_ccf_sum(const double *part, int nc, int n_spec, int n_v, double *ccf, hipStream_t s) {
+  long tot = (long)n_spec * n_v;
+  if (tot == 0) return hipSuccess;
+  k_ccf_sum<<<(unsigned)((tot + kTPB - 1) / kTPB), kTPB, 0, s>>>(part, nc, n_spec, n_v, ccf);
   return hipGetLastError();
 }
 

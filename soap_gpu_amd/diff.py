@@ -1,0 +1,112 @@
+"""Differentiable SOAP model: FP64 forward and adjoint on the GPU (soap_diff_* of
+the C ABI), exposed as a torch.autograd.Function.
+
+The model (no counterpart in the reference; built from its shift, convolution,
+mask CCF and Gaussian fit): disk cells with rigid rotation delta = vsini y and
+quadratic limb darkening; per-phase filling-factor maps f_p (M x M over
+[-1, 1]^2, bilinear) blending the quiet seed rows Q and the spot rows S scaled by
+the contrast; F_p = instrumental convolution of the phase spectrum; RV and FWHM
+from the Gaussian fit of the raw mask CCF of F_p, polished to the least-squares
+optimum.  Loss = w_spec sum (F - T)^2 (trimmed pixels) + w_rv sum (RV - RV_t)^2
++ w_fwhm sum (FWHM - FWHM_t)^2.  Gradients: vsini, u1, u2, contrast, the seed
+rows Q and S, and every map pixel.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from .engine import GRID_LOG, DiffParams, Engine, SoapError, _d
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def _dev(t: torch.Tensor) -> ctypes.c_void_p:
+    if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+        raise SoapError("expected a contiguous float64 tensor on the GPU")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class DiffModel:
+    """One differentiable problem on an Engine: grid, mu bins, mask and sizes fixed."""
+
+    def __init__(self, eng: Engine, wave, mu_bounds, line_l, line_w, vgrid, grid: int, n_phase: int,
+                 map_res: int, sigma_ms: float, trim: int = 210):
+        self.eng = eng
+        eng.set_wave(wave, GRID_LOG)
+        eng.set_mu_bounds(mu_bounds)
+        eng.set_mask(line_l, line_w, vgrid, trim)
+        eng._chk(eng.L.soap_diff_setup(eng.ctx, int(grid), int(n_phase), int(map_res), float(sigma_ms)),
+                 "diff_setup")
+        self.n = len(wave)
+        self.n_mu = len(mu_bounds) + 1
+        self.P = int(n_phase)
+        self.M = int(map_res)
+        self.n_v = len(vgrid)
+        self._fwd_id = 0
+        self._keep = None
+
+    def forward(self, params, q, s, maps, target, rv_t, fw_t, w_spec=1.0, w_rv=0.0, w_fw=0.0):
+        """params = (vsini [km/s], u1, u2, contrast); q, s [n_mu, n], maps [P, M, M], target [P, n]
+        float64 GPU tensors.  Returns (loss, rv[P], fwhm[P])."""
+        vs, u1, u2, kap = (float(x) for x in params)
+        torch.cuda.synchronize()
+        prm = DiffParams(vs, u1, u2, kap)
+        rt, rtp = _d(np.asarray(rv_t, dtype=np.float64).reshape(self.P))
+        ft, ftp = _d(np.asarray(fw_t, dtype=np.float64).reshape(self.P))
+        loss = ctypes.c_double()
+        rv, rvp = _d(np.zeros(self.P))
+        fw, fwp = _d(np.zeros(self.P))
+        self._keep = (q, s, maps, target)  # the engine reads them again in backward
+        self.eng._chk(self.eng.L.soap_diff_forward(self.eng.ctx, ctypes.byref(prm), _dev(q), _dev(s), _dev(maps),
+                                                   _dev(target), rtp, ftp, float(w_spec), float(w_rv), float(w_fw),
+                                                   ctypes.byref(loss), rvp, fwp), "diff_forward")
+        self._fwd_id += 1
+        return loss.value, rv, fw
+
+    def backward(self):
+        """Gradients of the last forward's loss: (g4 = d/d(vsini, u1, u2, contrast), gQ, gS, gmaps)."""
+        q = self._keep[0]
+        gq = torch.empty((self.n_mu, self.n), dtype=torch.float64, device=q.device)
+        gs = torch.empty_like(gq)
+        gm = torch.empty((self.P, self.M, self.M), dtype=torch.float64, device=q.device)
+        g4, g4p = _d(np.zeros(4))
+        torch.cuda.synchronize()
+        self.eng._chk(self.eng.L.soap_diff_backward(self.eng.ctx, g4p, _dev(gq), _dev(gs), _dev(gm)),
+                      "diff_backward")
+        return g4, gq, gs, gm
+
+    def get(self, what: str):
+        """Host copies of the last forward: 'F' [P, n], 'ccf' [P, n_v], 'theta' [P, 4] = (c, k, RV, FWHM)."""
+        shape = {"F": (self.P, self.n), "ccf": (self.P, self.n_v), "theta": (self.P, 4)}[what]
+        out = np.empty(shape)
+        self.eng._chk(self.eng.L.soap_diff_get(self.eng.ctx, ("F", "ccf", "theta").index(what),
+                                               out.ctypes.data_as(_dp)), "diff_get")
+        return out
+
+
+class SoapLoss(torch.autograd.Function):
+    """loss = SoapLoss.apply(model, params[4], q, s, maps, target, rv_t, fw_t, (w_spec, w_rv, w_fw)).
+    params is a float64 tensor (vsini, u1, u2, contrast); gradients flow to params,
+    q, s and maps.  One backward per forward (the engine keeps the last forward)."""
+
+    @staticmethod
+    def forward(ctx, model, params, q, s, maps, target, rv_t, fw_t, weights):
+        loss, rv, fw = model.forward(params.detach().cpu().numpy(), q.detach(), s.detach(), maps.detach(),
+                                     target.detach(), rv_t, fw_t, *weights)
+        ctx.model = model
+        ctx.fwd_id = model._fwd_id
+        ctx.params_device = params.device
+        ctx.rv, ctx.fw = rv, fw
+        return torch.tensor(loss, dtype=torch.float64, device=params.device)
+
+    @staticmethod
+    def backward(ctx, gout):
+        if ctx.model._fwd_id != ctx.fwd_id:
+            raise SoapError("SoapLoss: the model ran another forward since this graph was built")
+        g4, gq, gs, gm = ctx.model.backward()
+        gp = torch.as_tensor(g4, dtype=torch.float64, device=ctx.params_device) * gout.to(ctx.params_device)
+        go = gout.to(gq.device)
+        return None, gp, gq * go, gs * go, gm * go, None, None, None, None

@@ -53,6 +53,7 @@ EXPORTS = (
     "soap_set_timing", "soap_get_stage_ms", "soap_stage_name", "soap_num_stages",
     "soap_get_integrate_stats",
     "soap_get_stage_work",
+    "soap_diff_setup", "soap_diff_forward", "soap_diff_backward", "soap_diff_get",
 )
 
 _lib = None
@@ -60,6 +61,11 @@ _lib = None
 
 class SoapError(RuntimeError):
     pass
+
+
+class DiffParams(ctypes.Structure):
+    _fields_ = [("vsini_kms", ctypes.c_double), ("u1", ctypes.c_double), ("u2", ctypes.c_double),
+                ("contrast", ctypes.c_double)]
 
 
 def load(path: str = LIB_PATH):
@@ -109,6 +115,11 @@ def load(path: str = LIB_PATH):
         "soap_num_stages": (ctypes.c_int32, []),
         "soap_get_integrate_stats": (ctypes.c_int, [vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64)]),
         "soap_get_stage_work": (ctypes.c_int, [vp, ctypes.c_int32, _dp, _dp]),
+        "soap_diff_setup": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]),
+        "soap_diff_forward": (ctypes.c_int, [vp, ctypes.POINTER(DiffParams), vp, vp, vp, vp, _dp, _dp,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]),
+        "soap_diff_backward": (ctypes.c_int, [vp, _dp, vp, vp, vp]),
+        "soap_diff_get": (ctypes.c_int, [vp, ctypes.c_int32, _dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

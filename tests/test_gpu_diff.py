@@ -1,0 +1,124 @@
+"""The differentiable path on the GPU (soap_diff_*, soap_gpu_amd.diff) against the
+FP64 oracle's model (orc_diff_forward): forward parity, and every gradient
+(vsini, u1, u2, contrast, quiet / spot seed rows, filling-factor map pixels)
+against central finite differences of the oracle's loss with h = 1e-6 relative."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from soap_gpu_amd import synth
+from soap_gpu_amd.diff import DiffModel, SoapLoss
+from tests.pipeline_ref import normwise
+
+pytestmark = pytest.mark.gpu
+
+TRIM = 210
+P, M = 3, 10
+
+
+@pytest.fixture(scope="module")
+def problem(engine):
+    w = synth.small(grid=24, n=2600, n_mu=4)
+    rng = np.random.default_rng(21)
+    maps = np.clip(rng.uniform(-0.3, 0.7, (P, M, M)), 0.0, 1.0)
+    wave_t = w.wave[TRIM:w.n - TRIM]
+    sl, sw = O.select_lines(wave_t, w.mask_l, w.mask_w)
+    q, s = w.cubes["q"], w.cubes["sp"]
+    params = np.array([3.0, 0.29, 0.34, 0.3])
+    z = np.zeros((P, w.n))
+    # targets: the model at other parameters, plus offsets on RV and FWHM
+    tgt = O.diff_forward(w.wave, 24, w.mu_bounds, 3.2, 0.31, 0.3, 0.35, q, s, maps * 0.8, w.sigma_inst, TRIM,
+                         w.vgrid, sl, sw, z, np.zeros(P), np.zeros(P), 1.0, 0.0, 0.0)
+    T = tgt["F"] + 1e-4 * rng.standard_normal((P, w.n)) * np.abs(tgt["F"]).max()
+    rv_t = tgt["theta"][:, 2] + rng.uniform(-20, 20, P)
+    fw_t = tgt["theta"][:, 3] + rng.uniform(-50, 50, P)
+    wts = (1.0 / np.sum(T[:, TRIM:w.n - TRIM] ** 2), 1e-3, 1e-5)
+
+    def oracle_loss(prm, qq=q, ss=s, mm=maps):
+        return O.diff_forward(w.wave, 24, w.mu_bounds, *prm, qq, ss, mm, w.sigma_inst, TRIM, w.vgrid, sl, sw, T,
+                              rv_t, fw_t, *wts)
+
+    model = DiffModel(engine, w.wave, w.mu_bounds, sl, sw, w.vgrid, 24, P, M, w.sigma_inst, TRIM)
+    dev = torch.device("cuda")
+    tq = torch.tensor(q, dtype=torch.float64, device=dev)
+    ts = torch.tensor(s, dtype=torch.float64, device=dev)
+    tm = torch.tensor(maps, dtype=torch.float64, device=dev)
+    tT = torch.tensor(T, dtype=torch.float64, device=dev)
+    return dict(w=w, maps=maps, q=q, s=s, params=params, T=T, rv_t=rv_t, fw_t=fw_t, wts=wts, oracle=oracle_loss,
+                model=model, tq=tq, ts=ts, tm=tm, tT=tT)
+
+
+def _gpu(pb, params=None):
+    m = pb["model"]
+    return m.forward(pb["params"] if params is None else params, pb["tq"], pb["ts"], pb["tm"], pb["tT"],
+                     pb["rv_t"], pb["fw_t"], *pb["wts"])
+
+
+def test_diff_forward_parity(problem):
+    pb = problem
+    loss, rv, fw = _gpu(pb)
+    ref = pb["oracle"](pb["params"])
+    assert loss == pytest.approx(ref["loss"], rel=1e-10)
+    F = pb["model"].get("F")
+    for p in range(P):
+        assert normwise(F[p], ref["F"][p]) <= 1e-12
+    np.testing.assert_allclose(rv, ref["theta"][:, 2], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(fw, ref["theta"][:, 3], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(pb["model"].get("ccf"), ref["ccf"], rtol=1e-11)
+
+
+def _fd(f, h):
+    return (f(h) - f(-h)) / (2 * h)
+
+
+def test_diff_gradients_scalars_vs_oracle_fd(problem):
+    pb = problem
+    _gpu(pb)
+    g4, gq, gs, gm = pb["model"].backward()
+    p0 = pb["params"]
+    for i, name in enumerate(("vsini", "u1", "u2", "contrast")):
+        h = 1e-6 * abs(p0[i])
+        fd = _fd(lambda e: pb["oracle"](p0 + e * np.eye(4)[i])["loss"], h)
+        assert g4[i] == pytest.approx(fd, rel=2e-5, abs=1e-9 * np.abs(g4).max()), name
+
+
+@pytest.mark.parametrize("which", ["q", "s", "maps"])
+def test_diff_gradients_arrays_vs_oracle_fd(problem, which):
+    """Directional derivatives along two random directions of the seed rows and of
+    the maps (interior pixels, where the trimmed loss sees them)."""
+    pb = problem
+    _gpu(pb)
+    g4, gq, gs, gm = pb["model"].backward()
+    g = {"q": gq, "s": gs, "maps": gm}[which].cpu().numpy()
+    base = {"q": pb["q"], "s": pb["s"], "maps": pb["maps"]}[which]
+    rng = np.random.default_rng(5)
+    for _ in range(2):
+        v = rng.standard_normal(base.shape)
+        h = 1e-6 * np.linalg.norm(base) / np.linalg.norm(v)
+
+        def f(e):
+            arr = base + e * v
+            kw = {"q": dict(qq=arr), "s": dict(ss=arr), "maps": dict(mm=arr)}[which]
+            return pb["oracle"](pb["params"], **kw)["loss"]
+
+        fd = _fd(f, h)
+        assert np.sum(g * v) == pytest.approx(fd, rel=2e-5, abs=1e-12), which
+
+
+def test_soap_loss_autograd(problem):
+    pb = problem
+    params = torch.tensor(pb["params"], dtype=torch.float64, requires_grad=True)
+    q = pb["tq"].clone().requires_grad_(True)
+    s = pb["ts"].clone().requires_grad_(True)
+    mp = pb["tm"].clone().requires_grad_(True)
+    loss = SoapLoss.apply(pb["model"], params, q, s, mp, pb["tT"], pb["rv_t"], pb["fw_t"], pb["wts"])
+    (2.0 * loss).backward()
+    ref = pb["oracle"](pb["params"])
+    assert loss.item() == pytest.approx(ref["loss"], rel=1e-10)
+    _gpu(pb)
+    g4, gq, gs, gm = pb["model"].backward()
+    # the adjoint sums with floating-point atomics: runs agree to rounding
+    np.testing.assert_allclose(params.grad.numpy(), 2 * g4, rtol=1e-9)
+    for a, b in ((q.grad, gq), (s.grad, gs), (mp.grad, gm)):
+        torch.testing.assert_close(a, 2 * b, rtol=1e-9, atol=1e-12 * float(b.abs().max()))

@@ -1,0 +1,123 @@
+"""Pins of the oracle's differentiable model (oracle/soap_oracle.c, a11 section;
+no counterpart in the reference): the Gaussian-fit Hessian against finite
+differences, the Newton-polished fit against scipy's least squares, and the
+forward model against the quiet-star path of the reference's pipeline."""
+import numpy as np
+import pytest
+from scipy.optimize import least_squares
+
+import oracle as O
+from soap_gpu_amd import synth
+
+TRIM = 210
+
+
+def _gauss(x, th):
+    s = th[3] / (2 * np.sqrt(2 * np.log(2)))
+    return th[0] + th[1] * np.exp(-((x - th[2]) ** 2) / (2 * s * s))
+
+
+def _ccf_like(seed=0, n=81):
+    rng = np.random.default_rng(seed)
+    x = np.linspace(-20000, 20000, n)
+    y = _gauss(x, [1.0, -0.4, 350.0, 7000.0]) + 1e-3 * rng.standard_normal(n)
+    return x, y
+
+
+def test_gauss_hessian_finite_differences():
+    """orc_gauss_hess: gradient and exact Hessian of 0.5 sum r^2 against central
+    differences of the objective and of the gradient (catches a wrong sign or a
+    missing second-derivative term)."""
+    x, y = _ccf_like()
+    th = np.array([0.99, -0.38, 310.0, 7200.0])
+    g, H, J, r2 = O.gauss_hess(x, y, th)
+    obj = lambda t: 0.5 * np.sum((_gauss(x, t) - y) ** 2)
+    steps = np.array([1e-6, 1e-6, 1e-3, 1e-2])
+    gn = np.array([(obj(th + steps[a] * np.eye(4)[a]) - obj(th - steps[a] * np.eye(4)[a])) / (2 * steps[a])
+                   for a in range(4)])
+    np.testing.assert_allclose(g, gn, rtol=1e-6, atol=1e-12)
+    Hn = np.array([(O.gauss_hess(x, y, th + steps[a] * np.eye(4)[a])[0] -
+                    O.gauss_hess(x, y, th - steps[a] * np.eye(4)[a])[0]) / (2 * steps[a]) for a in range(4)])
+    np.testing.assert_allclose(H, Hn.T, rtol=1e-5, atol=1e-9 * np.abs(H).max())
+    np.testing.assert_allclose(J[:, 1], np.exp(-((x - th[2]) ** 2) / (2 * (th[3] / 2.3548200450309493) ** 2)),
+                               rtol=1e-12)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_gauss_polish_reaches_least_squares_optimum(seed):
+    """orc_gauss_polish from the BIS_FIT2-tolerance fit lands on scipy's least
+    squares optimum, with a vanishing gradient."""
+    x, y = _ccf_like(seed)
+    r = O.fit_ccf(x, y)
+    th0 = np.array([r["par"][0], -r["par"][1], r["par"][3], r["par"][4]])
+    th = O.gauss_polish(x, y, th0)
+    g, H, _, _ = O.gauss_hess(x, y, th)
+    assert np.all(np.abs(g) <= 1e-10 * np.sqrt(np.diag(H)) * np.abs(th).max())
+    ref = least_squares(lambda t: _gauss(x, t) - y, th0, xtol=1e-15, ftol=1e-15, gtol=1e-15, method="lm").x
+    np.testing.assert_allclose(th, ref, rtol=1e-9)
+
+
+def _problem(grid=20, n=2400, n_mu=4, P=2, M=10, seed=5):
+    w = synth.small(grid=grid, n=n, n_mu=n_mu)
+    rng = np.random.default_rng(seed)
+    maps = np.clip(rng.uniform(-0.2, 0.5, (P, M, M)), 0, 1)
+    wave_t = w.wave[TRIM:w.n - TRIM]
+    sl, sw = O.select_lines(wave_t, w.mask_l, w.mask_w)
+    return w, maps, sl, sw
+
+
+def test_diff_forward_without_activity_is_the_quiet_star():
+    """With empty maps, F is the instrumental convolution of the reference's quiet
+    star (itot, SOAP_initialize.cu:339-395) for rigid rotation vsini y and quadratic
+    limb darkening; the convolution taps are the log-grid taps of the resolution
+    stage."""
+    w, maps, sl, sw = _problem()
+    maps[:] = 0.0
+    vsini, u1, u2 = 3.0, 0.29, 0.34
+    r = O.diff_forward(w.wave, 20, w.mu_bounds, vsini, u1, u2, 0.3, w.cubes["q"], w.cubes["sp"], maps,
+                       w.sigma_inst, TRIM, w.vgrid, sl, sw, np.zeros((2, w.n)), np.zeros(2), np.zeros(2),
+                       1.0, 0.0, 0.0)
+    prot = synth.prot_for_vsini(vsini)
+    dm = O.disk_init(20, prot, 0.0, 90.0, synth.R_SUN_KM, u1, u2, [0, 0, 0, 0], w.mu_bounds)
+    q = O.quiet_star(O.GRID_LOG, w.wave, dm, w.cubes["q"], 20)
+    h = O.grid_h(w.wave)
+    wid = O.conv_window(w.wave, w.sigma_inst)
+    fs = O.delta_lambda(w.sigma_inst, 1.0)
+    d = np.arange(-wid, wid - 1)
+    u = np.expm1(d * h) / fs
+    taps = np.exp(d * h) * np.expm1(h) / (fs * np.sqrt(2 * np.pi)) * np.exp(-u * u / 2)
+    ref = np.zeros(w.n)
+    for i in range(wid + 1, w.n - wid - 1):
+        ref[i] = np.dot(taps, q[i - wid:i + wid - 1])
+    for p in range(2):
+        sl_ = slice(TRIM, w.n - TRIM)
+        assert np.max(np.abs(r["F"][p][sl_] - ref[sl_])) <= 1e-11 * np.max(np.abs(ref))
+
+
+def test_diff_forward_spot_seed_equal_to_quiet_ignores_maps():
+    """kappa = 1 and S = Q: (1 - phi) Q + kappa phi S = Q for any map."""
+    w, maps, sl, sw = _problem()
+    args = (w.wave, 20, w.mu_bounds, 2.5, 0.3, 0.2, 1.0, w.cubes["q"], w.cubes["q"])
+    tail = (w.sigma_inst, TRIM, w.vgrid, sl, sw, np.zeros((2, w.n)), np.zeros(2), np.zeros(2), 1.0, 1.0, 1.0)
+    a = O.diff_forward(*args, maps, *tail)
+    b = O.diff_forward(*args, np.zeros_like(maps), *tail)
+    np.testing.assert_allclose(a["F"], b["F"], rtol=0, atol=1e-11 * np.abs(b["F"]).max())
+    assert abs(a["loss"] - b["loss"]) <= 1e-11 * b["loss"]
+
+
+def test_diff_forward_rv_follows_a_dark_spot():
+    """A dark region on the approaching (y < 0) half removes blueshifted light, so
+    the RV of the CCF fit increases; on the receding half it decreases."""
+    w, maps, sl, sw = _problem(grid=32)
+    M = maps.shape[1]
+    maps[:] = 0.0
+    maps[0, : M // 3, :] = 1.0  # y < 0 rows of phase 0 covered
+    maps[1, -M // 3:, :] = 1.0  # y > 0 rows of phase 1 covered
+    r = O.diff_forward(w.wave, 32, w.mu_bounds, 5.0, 0.29, 0.34, 0.2, w.cubes["q"], w.cubes["q"], maps,
+                       w.sigma_inst, TRIM, w.vgrid, sl, sw, np.zeros((2, w.n)), np.zeros(2), np.zeros(2),
+                       1.0, 0.0, 0.0)
+    q0 = O.diff_forward(w.wave, 32, w.mu_bounds, 5.0, 0.29, 0.34, 0.2, w.cubes["q"], w.cubes["q"],
+                        np.zeros_like(maps), w.sigma_inst, TRIM, w.vgrid, sl, sw, np.zeros((2, w.n)), np.zeros(2),
+                        np.zeros(2), 1.0, 0.0, 0.0)
+    rv0 = q0["theta"][0, 2]
+    assert r["theta"][0, 2] > rv0 + 10 and r["theta"][1, 2] < rv0 - 10
