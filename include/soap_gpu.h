@@ -229,6 +229,42 @@ int soap_diff_get(soap_ctx *ctx, int32_t what, double *out);
  * No counterpart in the reference (measurement support). */
 int soap_get_stage_work(soap_ctx *ctx, int32_t stage, double *bytes, double *flops);
 
+
+/* ------------------------------------------------------------ a11: batched instances
+ * No counterpart in the reference (which runs one star per process, stage by
+ * stage).  A batch runs the per-phase step (integrate + resolution + CCF + fit)
+ * of many uneven instances -- each a prepared soap_ctx on the same device with
+ * its own grid size, wavelength grid, phases and regions -- with one persistent
+ * launch per stage covering every instance, optionally replayed as a HIP graph.
+ * Instances: log-uniform grid, non-solar activity with >= 1 phase, mask set,
+ * disk_init and quiet_star done, n_mu <= 64.  soap_batch_create runs each
+ * instance once through the plain path (allocating its buffers); results of a
+ * step are read through each instance's soap_get_* calls after soap_batch_sync.
+ * The contexts must outlive the batch and must not be changed while it exists.
+ * On failure soap_batch_create still returns the batch in *out (if allocated)
+ * so that soap_batch_last_error can be read; destroy it either way. */
+typedef struct soap_batch soap_batch;
+int soap_batch_create(soap_ctx *const *ctxs, int32_t n_inst, const double *sigma_ms, const double *quiet_weight,
+                      soap_batch **out);
+void soap_batch_destroy(soap_batch *b);
+const char *soap_batch_last_error(const soap_batch *b);
+/* enqueue one step on the batch's stream; use_graph != 0 replays a captured HIP
+ * graph of the step (captured on first use after any shape change) */
+int soap_batch_step(soap_batch *b, int32_t use_graph);
+int soap_batch_sync(soap_batch *b);
+/* one eager step with HIP events around each stage launch (synchronous) */
+int soap_batch_stage_ms(soap_batch *b, double *ms, int32_t n_stage);
+const char *soap_batch_stage_name(int32_t stage);
+int32_t soap_batch_num_stages(void);
+/* launch shapes: disk cells per taps block, phases per FIR block, and per stage
+ * the persistent block count (0 = one block per work item) and work items */
+int soap_batch_get_config(soap_batch *b, int32_t *taps_cells, int32_t *fir_g, int64_t *grid, int64_t *work,
+                          int32_t n_stage);
+int soap_batch_set_config(soap_batch *b, int32_t taps_cells, int32_t fir_g, const int64_t *grid, int32_t n_stage);
+/* autotuner: coordinate search over the shapes above, scored by stage time
+ * (best of reps eager steps per candidate) */
+int soap_batch_tune(soap_batch *b, int32_t reps);
+
 #ifdef __cplusplus
 }
 #endif

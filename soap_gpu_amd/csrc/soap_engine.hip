@@ -1762,4 +1762,456 @@ int soap_run_series(soap_ctx *c, const soap_star *star, double sigma_ms, double 
   return SOAP_OK;
 }
 
+// ------------------------------------------------------------------ a11: batched instances
+}  // extern "C"
+
+namespace {
+const char *kBatchStageNames[BS_COUNT] = {"fill",  "taps",   "finalize", "fir", "conv_quiet", "conv",
+                                          "minmax", "norm", "ccf",      "ccf_reduce", "fit"};
+
+struct BatchStageBuf {
+  void *jobs = nullptr;       // device job array [n_inst]
+  int64_t *prefix = nullptr;  // device [n_inst + 1]
+  size_t job_bytes = 0;
+  int64_t total = 0;
+  int threads = 64;
+  size_t lds = 0;
+  int64_t grid = 0;  // persistent block count; 0 = one block per work item
+};
+}  // namespace
+
+struct soap_batch {
+  int device = 0;
+  int n_cu = 256;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<soap_ctx *> inst;
+  BatchStageBuf st[BS_COUNT];
+  int taps_cells = 4096;  // disk cells per taps block (cells-per-block shape)
+  int fir_g = 4;          // phases per FIR block
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t ev[BS_COUNT + 1] = {};
+  double stage_ms[BS_COUNT] = {};
+  std::vector<std::vector<uintptr_t>> sig;  // per instance: buffers and sizes the job tables point at
+};
+
+namespace {
+int bfail(soap_batch *b, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf;
+  return code;
+}
+#define BHIP(b, x)                                                                                     \
+  do {                                                                                                 \
+    hipError_t e_ = (x);                                                                               \
+    if (e_ != hipSuccess) return bfail(b, SOAP_ERR_HIP, "%s: %s", #x, hipGetErrorString(e_));          \
+  } while (0)
+
+void batch_drop_graph(soap_batch *b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+template <class Job>
+int batch_upload(soap_batch *b, int stage, const std::vector<Job> &jobs, const std::vector<int64_t> &work, int threads,
+                 size_t lds) {
+  BatchStageBuf &sb = b->st[stage];
+  size_t nb = jobs.size() * sizeof(Job);
+  if (!sb.jobs || sb.job_bytes < nb) {
+    if (sb.jobs) hipFree(sb.jobs);
+    sb.jobs = nullptr;
+    BHIP(b, hipMalloc(&sb.jobs, nb));
+    sb.job_bytes = nb;
+  }
+  std::vector<int64_t> pre(jobs.size() + 1, 0);
+  for (size_t i = 0; i < jobs.size(); i++) pre[i + 1] = pre[i] + work[i];
+  if (!sb.prefix) BHIP(b, hipMalloc(&sb.prefix, pre.size() * sizeof(int64_t)));
+  BHIP(b, hipMemcpy(sb.jobs, jobs.data(), nb, hipMemcpyHostToDevice));
+  BHIP(b, hipMemcpy(sb.prefix, pre.data(), pre.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  sb.total = pre.back();
+  sb.threads = threads;
+  sb.lds = lds;
+  return SOAP_OK;
+}
+
+// Everything of an instance that the job tables capture: a context changed
+// after the batch was built (reallocated buffers, other sizes or taps) must not
+// be stepped through stale pointers.
+std::vector<uintptr_t> batch_signature(const soap_ctx *c) {
+  auto u = [](const void *p) { return (uintptr_t)p; };
+  return {u(c->d_raw),    u(c->d_lower), u(c->d_quiet),   u(c->acc.p),    u(c->taps.p),   u(c->krange.p),
+          u(c->kflags.p), u(c->bins.p),  u(c->nbins.p),   u(c->part.p),   u(c->mm.p),     u(c->ctap.p),
+          u(c->d_ccf),    u(c->ccf_part.p), u(c->f_par.p), u(c->f_model.p), u(c->f_depth.p), u(c->f_bis.p),
+          u(c->f_iter.p), u(c->m_start.p), u(c->m_alive.p), u(c->m_val.p), u(c->d_geo),   u(c->d_box),
+          u(c->d_region), u(c->d_m),     u(c->d_t),       (uintptr_t)c->P, (uintptr_t)c->n, (uintptr_t)c->G,
+          (uintptr_t)c->nd, (uintptr_t)c->res_ntp, (uintptr_t)c->res_wid, (uintptr_t)c->n_lines,
+          (uintptr_t)c->n_v, (uintptr_t)c->geo.size(), (uintptr_t)c->act.solar_mode};
+}
+
+int batch_check(soap_batch *b) {
+  for (size_t i = 0; i < b->inst.size(); i++)
+    if (batch_signature(b->inst[i]) != b->sig[i])
+      return bfail(b, SOAP_ERR_STATE, "batch: instance %zu was changed after the batch was created", i);
+  return SOAP_OK;
+}
+
+// Job tables of every stage from the prepared contexts and the current shapes.
+int batch_build(soap_batch *b) {
+  size_t ni = b->inst.size();
+  std::vector<FillJob> fill(ni);
+  std::vector<TapsJob> taps(ni);
+  std::vector<FinalizeJob> fin(ni);
+  std::vector<FirJob> fir(ni);
+  std::vector<ConvQJob> cq(ni);
+  std::vector<ConvAJob> ca(ni);
+  std::vector<MinmaxJob> mm(ni);
+  std::vector<NormJob> nm(ni);
+  std::vector<CcfJob> cc(ni);
+  std::vector<CcfRedJob> cr(ni);
+  std::vector<FitJob> ft(ni);
+  std::vector<int64_t> w[BS_COUNT];
+  for (auto &v : w) v.assign(ni, 0);
+  size_t lds_taps = 0, lds_fir = 0, lds_conv = 0;
+  int ccf_threads = 64, red_threads = 64;
+  for (size_t i = 0; i < ni; i++) {
+    soap_ctx *c = b->inst[i];
+    const int P = c->P, nsets = 3, n_spec = 3 * (P + 1);
+    // fill + taps + finalize (soap_integrate, non-solar log grid)
+    TapAccArgs a{};
+    a.grid = c->G;
+    a.n_mu = c->n_mu;
+    a.dmin = c->dmin;
+    a.nd = c->nd;
+    a.nsets = nsets;
+    a.wd = tap_window(c, nsets);
+    a.cols_per_block = std::max(1, b->taps_cells / std::max(1, c->G));
+    a.region = c->d_region;
+    a.limb = c->d_limb;
+    a.delta = c->d_delta;
+    a.lbm = c->d_lbm;
+    a.m = c->d_m;
+    a.t = c->d_t;
+    a.acc = (unsigned long long *)c->acc.p;
+    a.geo = c->d_geo;
+    a.geo_off = c->d_geo_off;
+    a.box = c->d_box;
+    int mcb = 0;
+    for (int p = 0; p < P; p++) mcb = std::max(mcb, (c->box[p * 4 + 1] - c->box[p * 4] + a.cols_per_block - 1) / a.cols_per_block);
+    fill[i] = FillJob{a.acc, (int64_t)P * nsets * c->n_mu * c->nd * 2};
+    w[BS_FILL][i] = (fill[i].count + batch_fill_chunk() - 1) / batch_fill_chunk();
+    taps[i] = TapsJob{a, std::max(1, mcb)};
+    w[BS_TAPS][i] = (int64_t)mcb * P;
+    lds_taps = std::max(lds_taps, taps_lds_bytes(a));
+    fin[i] = FinalizeJob{a.acc, nsets, nsets + 1, c->n_mu, c->nd, (double *)c->taps.p, (int32_t *)c->krange.p,
+                         (int32_t *)c->kflags.p, (int32_t *)c->bins.p, (int32_t *)c->nbins.p};
+    w[BS_FINALIZE][i] = P;
+    FirArgs f{};
+    f.q = c->d_cube[SOAP_CUBE_QUIET];
+    f.qs = c->d_cube[SOAP_CUBE_QUIET_SED];
+    f.sp = c->d_cube[SOAP_CUBE_SPOT];
+    f.sps = c->d_cube[SOAP_CUBE_SPOT_SED];
+    f.fa = c->d_cube[SOAP_CUBE_FAC];
+    f.fas = c->d_cube[SOAP_CUBE_FAC_SED];
+    f.n = c->n;
+    f.n_mu = c->n_mu;
+    f.dmin = c->dmin;
+    f.nd = c->nd;
+    f.taps = (const double *)c->taps.p;
+    f.krange = (const int32_t *)c->krange.p;
+    f.kflags = (const int32_t *)c->kflags.p;
+    f.bins = (const int32_t *)c->bins.p;
+    f.nbins = (const int32_t *)c->nbins.p;
+    f.out = c->d_raw;
+    int fir_tiles = (int)((c->n + fir_tile() - 1) / fir_tile());
+    int groups = (P + b->fir_g - 1) / b->fir_g;
+    fir[i] = FirJob{f, P, fir_tiles, groups};
+    w[BS_FIR][i] = (int64_t)(fir_tiles + 7) / 8 * 8 * groups;
+    lds_fir = std::max(lds_fir, fir_lds(c->nd));
+    // resolution (log grid)
+    ConvArgs cv{};
+    cv.n = c->n;
+    cv.wid = c->res_wid;
+    cv.dlo = c->res_dlo;
+    cv.ntp = c->res_ntp;
+    cv.ctap = (const double *)c->ctap.p;
+    cv.n_tiles = (int)((c->n + conv_tile() - 1) / conv_tile());
+    double *part = (double *)c->part.p;
+    cq[i] = ConvQJob{cv, c->d_quiet, c->d_lower + (size_t)3 * P * c->n, part + (size_t)3 * P * cv.n_tiles * 2};
+    w[BS_CONV_QUIET][i] = 2L * cv.n_tiles;
+    ca[i] = ConvAJob{cv, c->d_quiet, c->d_raw, c->d_lower, part, P};
+    w[BS_CONV][i] = (int64_t)(cv.n_tiles + 7) / 8 * 8 * P;
+    lds_conv = std::max(lds_conv, conv_lds_bytes(cv.ntp));
+    mm[i] = MinmaxJob{part, cv.n_tiles, (double *)c->mm.p};
+    w[BS_MINMAX][i] = n_spec;
+    int64_t nbx = (c->n + norm_px_per_block() - 1) / norm_px_per_block();
+    nm[i] = NormJob{c->n, nbx, (const double *)c->mm.p, c->d_lower};
+    w[BS_NORM][i] = nbx * n_spec;
+    // CCF + fit
+    int nc = ccf_chunks(c->n_lines);
+    cc[i] = CcfJob{c->d_lower, c->n, c->trim, c->n - 2 * (int64_t)c->trim, n_spec, c->n_lines, c->n_v, nc,
+                   (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p,
+                   c->d_w_sorted, (double *)c->ccf_part.p};
+    w[BS_CCF][i] = (int64_t)nc * ((n_spec + ccf_spec_chunk() - 1) / ccf_spec_chunk());
+    ccf_threads = std::max(ccf_threads, (c->n_v + 63) / 64 * 64);
+    int rt = 64;
+    while (rt < c->n_v) rt <<= 1;
+    red_threads = std::max(red_threads, rt);
+    cr[i] = CcfRedJob{(const double *)c->ccf_part.p, nc, n_spec, c->n_v, c->d_appod, c->d_ccf};
+    w[BS_CCF_REDUCE][i] = n_spec;
+    ft[i] = FitJob{c->d_vgrid, c->n_v, c->d_ccf, (double *)c->f_par.p, (double *)c->f_model.p,
+                   (double *)c->f_depth.p, (double *)c->f_bis.p, (int32_t *)c->f_iter.p};
+    w[BS_FIT][i] = n_spec;
+  }
+  if (lds_fir > 60000 || lds_conv > 65536 || lds_taps > 65536 || ccf_threads > 1024 || red_threads > 1024)
+    return bfail(b, SOAP_ERR_LIMIT, "batch: an instance exceeds a per-block limit (lds fir %zu conv %zu taps %zu)",
+                 lds_fir, lds_conv, lds_taps);
+  int rc;
+  if ((rc = batch_upload(b, BS_FILL, fill, w[BS_FILL], 256, 0)) ||
+      (rc = batch_upload(b, BS_TAPS, taps, w[BS_TAPS], 256, lds_taps)) ||
+      (rc = batch_upload(b, BS_FINALIZE, fin, w[BS_FINALIZE], 64, 0)) ||
+      (rc = batch_upload(b, BS_FIR, fir, w[BS_FIR], fir_tile(), lds_fir)) ||
+      (rc = batch_upload(b, BS_CONV_QUIET, cq, w[BS_CONV_QUIET], 256, lds_conv)) ||
+      (rc = batch_upload(b, BS_CONV, ca, w[BS_CONV], 256, lds_conv)) ||
+      (rc = batch_upload(b, BS_MINMAX, mm, w[BS_MINMAX], 256, 0)) ||
+      (rc = batch_upload(b, BS_NORM, nm, w[BS_NORM], 256, 0)) ||
+      (rc = batch_upload(b, BS_CCF, cc, w[BS_CCF], ccf_threads, 0)) ||
+      (rc = batch_upload(b, BS_CCF_REDUCE, cr, w[BS_CCF_REDUCE], red_threads, 0)) ||
+      (rc = batch_upload(b, BS_FIT, ft, w[BS_FIT], 64, 0)))
+    return rc;
+  b->sig.resize(ni);
+  for (size_t i = 0; i < ni; i++) b->sig[i] = batch_signature(b->inst[i]);
+  batch_drop_graph(b);
+  return SOAP_OK;
+}
+
+// The step's launches in stage order; with events, each stage is bracketed.
+int batch_enqueue(soap_batch *b, bool events) {
+  int rc = batch_check(b);
+  if (rc) return rc;
+  for (int s = 0; s < BS_COUNT; s++) {
+    BatchStageBuf &sb = b->st[s];
+    if (events) BHIP(b, hipEventRecord(b->ev[s], b->stream));
+    hipError_t e = launch_batch(s, sb.jobs, sb.prefix, (int)b->inst.size(), sb.total, sb.grid, sb.threads, sb.lds,
+                                b->fir_g, b->stream);
+    if (e != hipSuccess) return bfail(b, SOAP_ERR_HIP, "batch stage %s: %s", kBatchStageNames[s], hipGetErrorString(e));
+  }
+  if (events) BHIP(b, hipEventRecord(b->ev[BS_COUNT], b->stream));
+  return SOAP_OK;
+}
+
+int batch_timed(soap_batch *b, double *ms) {
+  int rc = batch_enqueue(b, true);
+  if (rc) return rc;
+  BHIP(b, hipEventSynchronize(b->ev[BS_COUNT]));
+  for (int s = 0; s < BS_COUNT; s++) {
+    float t = 0;
+    BHIP(b, hipEventElapsedTime(&t, b->ev[s], b->ev[s + 1]));
+    ms[s] = t;
+  }
+  return SOAP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int soap_batch_create(soap_ctx *const *ctxs, int32_t n_inst, const double *sigma_ms, const double *quiet_weight,
+                      soap_batch **out) {
+  if (!out) return SOAP_ERR_ARG;
+  *out = nullptr;
+  if (!ctxs || n_inst < 1 || !sigma_ms || !quiet_weight) return SOAP_ERR_ARG;
+  auto *b = new soap_batch();
+  b->device = ctxs[0] ? ctxs[0]->device : 0;
+  int rc = SOAP_OK;
+  auto bad = [&](int code, const char *msg, int i) {
+    rc = bfail(b, code, "batch_create: instance %d: %s", i, msg);
+  };
+  for (int i = 0; i < n_inst && !rc; i++) {
+    soap_ctx *c = ctxs[i];
+    if (!c) bad(SOAP_ERR_ARG, "null context", i);
+    else if (c->device != b->device) bad(SOAP_ERR_ARG, "contexts must share one device", i);
+    else if (c->kind != SOAP_GRID_LOG) bad(SOAP_ERR_GRID, "batched instances need log-uniform grids", i);
+    else if (c->act.solar_mode) bad(SOAP_ERR_LIMIT, "solar mode is not batched", i);
+    else if (!c->have_quiet || !c->have_maps) bad(SOAP_ERR_STATE, "run disk_init and quiet_star first", i);
+    else if (!c->have_mask) bad(SOAP_ERR_STATE, "set the mask first", i);
+    else if (c->P < 1) bad(SOAP_ERR_STATE, "set an activity with at least one phase", i);
+    else if (c->n_mu > 64) bad(SOAP_ERR_LIMIT, "n_mu > 64", i);
+    if (rc) break;
+    for (int j = 0; j < i; j++)
+      if (ctxs[j] == c) bad(SOAP_ERR_ARG, "context listed twice", i);
+    if (rc) break;
+    // one plain run allocates the instance's buffers, builds its mask tables and
+    // resolution taps, and leaves results the batched step must reproduce
+    int r;
+    if ((r = soap_integrate(c)) || (r = soap_resolution(c, sigma_ms[i], quiet_weight[i])) || (r = soap_ccf(c)) ||
+        (r = soap_fit(c)) || (r = soap_sync(c)))
+      rc = bfail(b, r, "batch_create: instance %d: %s", i, c->err.c_str());
+    b->inst.push_back(c);
+  }
+  if (!rc) {
+    hipError_t e = hipSetDevice(b->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    for (int s = 0; s <= BS_COUNT && e == hipSuccess; s++) e = hipEventCreate(&b->ev[s]);
+    int ncu = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device);
+    if (e != hipSuccess) rc = bfail(b, SOAP_ERR_HIP, "batch_create: %s", hipGetErrorString(e));
+    b->n_cu = ncu > 0 ? ncu : 256;
+  }
+  if (!rc) rc = batch_build(b);
+  if (rc) {
+    // the error string lives in the batch; keep it for soap_batch_last_error
+    *out = b;
+    return rc;
+  }
+  *out = b;
+  return SOAP_OK;
+}
+
+void soap_batch_destroy(soap_batch *b) {
+  if (!b) return;
+  hipSetDevice(b->device);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  batch_drop_graph(b);
+  for (auto &sb : b->st) {
+    if (sb.jobs) hipFree(sb.jobs);
+    if (sb.prefix) hipFree(sb.prefix);
+  }
+  for (auto &e : b->ev)
+    if (e) hipEventDestroy(e);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char *soap_batch_last_error(const soap_batch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int soap_batch_step(soap_batch *b, int32_t use_graph) {
+  if (!b || b->inst.empty()) return SOAP_ERR_ARG;
+  BHIP(b, hipSetDevice(b->device));
+  if (!use_graph) return batch_enqueue(b, false);
+  int rc0 = batch_check(b);
+  if (rc0) return rc0;
+  if (!b->exec) {
+    BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_enqueue(b, false);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    BHIP(b, e);
+    b->graph = g;
+    BHIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  BHIP(b, hipGraphLaunch(b->exec, b->stream));
+  return SOAP_OK;
+}
+
+int soap_batch_sync(soap_batch *b) {
+  if (!b) return SOAP_ERR_ARG;
+  BHIP(b, hipSetDevice(b->device));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  return SOAP_OK;
+}
+
+int soap_batch_stage_ms(soap_batch *b, double *ms, int32_t n_stage) {
+  if (!b || !ms || n_stage < BS_COUNT) return SOAP_ERR_ARG;
+  BHIP(b, hipSetDevice(b->device));
+  return batch_timed(b, ms);
+}
+
+const char *soap_batch_stage_name(int32_t s) { return (s >= 0 && s < BS_COUNT) ? kBatchStageNames[s] : ""; }
+int32_t soap_batch_num_stages(void) { return BS_COUNT; }
+
+int soap_batch_get_config(soap_batch *b, int32_t *taps_cells, int32_t *fir_g, int64_t *grid, int64_t *work,
+                          int32_t n_stage) {
+  if (!b || n_stage < BS_COUNT) return SOAP_ERR_ARG;
+  if (taps_cells) *taps_cells = b->taps_cells;
+  if (fir_g) *fir_g = b->fir_g;
+  for (int s = 0; s < BS_COUNT; s++) {
+    if (grid) grid[s] = b->st[s].grid;
+    if (work) work[s] = b->st[s].total;
+  }
+  return SOAP_OK;
+}
+
+int soap_batch_set_config(soap_batch *b, int32_t taps_cells, int32_t fir_g, const int64_t *grid, int32_t n_stage) {
+  if (!b || n_stage < BS_COUNT || !grid) return SOAP_ERR_ARG;
+  if (taps_cells < 64 || taps_cells > (1 << 20)) return bfail(b, SOAP_ERR_ARG, "set_config: taps_cells out of range");
+  if (fir_g != 2 && fir_g != 4 && fir_g != 6) return bfail(b, SOAP_ERR_ARG, "set_config: fir_g must be 2, 4 or 6");
+  for (int s = 0; s < BS_COUNT; s++)
+    if (grid[s] < 0 || grid[s] % 8) return bfail(b, SOAP_ERR_ARG, "set_config: grid sizes are multiples of 8 (0 = all)");
+  BHIP(b, hipSetDevice(b->device));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  b->taps_cells = taps_cells;
+  b->fir_g = fir_g;
+  for (int s = 0; s < BS_COUNT; s++) b->st[s].grid = grid[s];
+  return batch_build(b);
+}
+
+// Coordinate search over the launch shapes: the taps block size in disk cells,
+// the FIR phase group, then each stage's persistent grid (0 = one block per
+// work item, else k * CUs).  Each candidate runs whole steps (so every stage
+// sees its real inputs) and is scored by its stage's fastest of `reps` runs.
+int soap_batch_tune(soap_batch *b, int32_t reps) {
+  if (!b || reps < 1) return SOAP_ERR_ARG;
+  BHIP(b, hipSetDevice(b->device));
+  BHIP(b, hipStreamSynchronize(b->stream));
+  double ms[BS_COUNT];
+  auto score = [&](int s0, int s1, double *best) -> int {
+    *best = HUGE_VAL;
+    for (int r = 0; r < reps; r++) {
+      int rc = batch_timed(b, ms);
+      if (rc) return rc;
+      double t = 0;
+      for (int s = s0; s <= s1; s++) t += ms[s];
+      *best = std::min(*best, t);
+    }
+    return SOAP_OK;
+  };
+  int rc;
+  double t, best;
+  // taps block shape (taps stage only; fill and finalize do not depend on it)
+  const int cells_c[] = {1024, 2048, 4096, 8192, 16384};
+  int best_cells = b->taps_cells;
+  best = HUGE_VAL;
+  for (int cells : cells_c) {
+    b->taps_cells = cells;
+    if ((rc = batch_build(b)) || (rc = score(BS_TAPS, BS_TAPS, &t))) return rc;
+    if (t < best) best = t, best_cells = cells;
+  }
+  b->taps_cells = best_cells;
+  const int g_c[] = {2, 4, 6};
+  int best_g = b->fir_g;
+  best = HUGE_VAL;
+  for (int g : g_c) {
+    b->fir_g = g;
+    if ((rc = batch_build(b)) || (rc = score(BS_FIR, BS_FIR, &t))) return rc;
+    if (t < best) best = t, best_g = g;
+  }
+  b->fir_g = best_g;
+  if ((rc = batch_build(b))) return rc;
+  const int64_t mult[] = {0, 1, 2, 4, 8, 16};
+  for (int s = 0; s < BS_COUNT; s++) {
+    int64_t best_grid = 0;
+    best = HUGE_VAL;
+    for (int64_t m : mult) {
+      int64_t g = m * b->n_cu / 8 * 8;
+      if (m > 0 && g >= b->st[s].total) continue;  // no smaller than one block per item
+      b->st[s].grid = g;
+      if ((rc = score(s, s, &t))) return rc;
+      if (t < best * 0.98 || (m == 0 && t < best)) best = t, best_grid = g;
+    }
+    b->st[s].grid = best_grid;
+  }
+  batch_drop_graph(b);
+  return SOAP_OK;
+}
+
 }  // extern "C"

@@ -179,6 +179,47 @@ struct ConvLinArgs {
   double quiet_weight;        // w of the quiet form (active form: 0.005)
   const double *wave;
 };
+
+// Batched instances: one persistent launch per stage covers every instance.
+// Work item w of the flattened index space belongs to the instance i with
+// prefix[i] <= w < prefix[i + 1]; local = w - prefix[i] is the block index the
+// single-instance kernel would have had.  Counts of the XCD-ordered stages
+// (FIR, active convolution) are multiples of 8, so local % 8 == w % 8.
+enum BatchStage {
+  BS_FILL = 0, BS_TAPS, BS_FINALIZE, BS_FIR, BS_CONV_QUIET, BS_CONV, BS_MINMAX, BS_NORM, BS_CCF, BS_CCF_REDUCE,
+  BS_FIT, BS_COUNT
+};
+struct FillJob { unsigned long long *p; int64_t count; };           // work: ceil(count / batch_fill_chunk())
+struct TapsJob { TapAccArgs a; int32_t mcb; };                        // work: mcb * P, local = p * mcb + col block
+struct FinalizeJob {                                                  // work: P
+  const unsigned long long *acc;
+  int32_t nsets_in, nsets_out, n_mu, nd;
+  double *taps;
+  int32_t *krange, *kflags, *bins, *nbins;
+};
+struct FirJob { FirArgs a; int32_t n_job, n_tiles, n_groups; };      // work: tiles8 * n_groups
+struct ConvQJob { ConvArgs c; const double *quiet; double *frame, *part; };  // work: 2 n_tiles
+struct ConvAJob { ConvArgs c; const double *quiet, *raw; double *out, *part; int32_t n_phase; };  // tiles8 * P
+struct MinmaxJob { const double *part; int32_t n_tiles; double *mm; };    // work: n_spec
+struct NormJob { int64_t n, bx; const double *mm; double *o; };          // work: bx * n_spec
+struct CcfJob {                                                          // work: nc * ceil(n_spec / ccf_spec_chunk())
+  const double *spec;
+  int64_t stride, trim, n_t;
+  int32_t n_spec, n_lines, n_v, nc;
+  const int32_t *m_start;
+  const uint32_t *m_pk;
+  const double2 *m_wv;
+  const double *w_sorted;
+  double *part;
+};
+struct CcfRedJob { const double *part; int32_t nc, n_spec, n_v; const double *appod; double *ccf; };  // n_spec
+struct FitJob {                                                          // work: n_ccf
+  const double *v;
+  int32_t n_v;
+  const double *y;
+  double *par, *model, *depth, *bis;
+  int32_t *iters;
+};
 }  // namespace soap
 
 // launchers (soap_kernels.hip)
@@ -259,4 +300,16 @@ hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const doub
                                 const double2 *m_wv, const double *w_sorted, double *GF, double *GI, double *C,
                                 double *gmaps, double *cell_part, double *gq, double *gs, hipStream_t s);
 hipError_t launch_ccf_sum(const double *part, int nc, int n_spec, int n_v, double *ccf, hipStream_t s);
+// batched instances: stage st over jobs[n_inst] (device), prefix[n_inst + 1]
+// (device, prefix[n_inst] = total); grid = persistent block count (0: one block
+// per work item), threads / lds as the stage needs; fir_g = phases per FIR block
+hipError_t launch_batch(int st, const void *jobs, const int64_t *prefix, int n_inst, int64_t total, int64_t grid,
+                        int threads, size_t lds, int fir_g, hipStream_t s);
+int batch_fill_chunk();
+int norm_px_per_block();
+int ccf_spec_chunk();
+int fir_tile();
+size_t fir_lds(int nd);
+size_t conv_lds_bytes(int ntp);
+size_t taps_lds_bytes(const TapAccArgs &a);
 }  // namespace soap

@@ -200,14 +200,14 @@ hipError_t launch_taps_quiet(const TapAccArgs &a, hipStream_t s) {
 // Chebyshev nodes T_q (the host checks the interpolation error), so facula cells
 // add limb to set 1 and limb * l_q(T_cell) to set 2+q; the FIR epilogue then
 // weights set 2+q by B(lambda, T_q)/B(lambda, T_norm).
-__global__ void k_taps_active(TapAccArgs a) {
+__device__ __forceinline__ void taps_active_body(const TapAccArgs &a, long bx_, long by_) {
   extern __shared__ unsigned long long lds_u64[];
   __shared__ int scratch[16];
-  int job = blockIdx.y;
+  int job = by_;
   int G = a.grid;
   const int32_t *bx = a.box + job * 4;
   int cy0 = bx[0], cy1 = bx[1], cz0 = bx[2], cz1 = bx[3];
-  int iy0 = cy0 + blockIdx.x * a.cols_per_block;
+  int iy0 = cy0 + bx_ * a.cols_per_block;
   if (iy0 >= cy1) return;  // uniform over the block
   int iy1 = min(cy1, iy0 + a.cols_per_block);
   int nz = cz1 - cz0;
@@ -258,6 +258,10 @@ __global__ void k_taps_active(TapAccArgs a) {
   flush_window(w, a.nsets);
 }
 
+__global__ void k_taps_active(TapAccArgs a) {
+  taps_active_body(a, blockIdx.x, blockIdx.y);
+}
+
 hipError_t launch_taps_active(const TapAccArgs &a, int n_job, int max_col_blocks, hipStream_t s) {
   if (n_job == 0 || max_col_blocks == 0) return hipSuccess;
   size_t lds = (size_t)a.nsets * a.n_mu * a.wd * 2 * sizeof(unsigned long long);
@@ -270,9 +274,8 @@ hipError_t launch_taps_active(const TapAccArgs &a, int n_job, int max_col_blocks
 // Active taps get one more output set than they accumulate: the last one is
 // set 0 + set 1 (all active cells, weight limb), used on the SED quiet rows.
 // Flags: bit 0 the spot set is nonzero, bit 1 a facula set is nonzero.
-__global__ void k_taps_finalize(const unsigned long long *acc, int nsets_in, int nsets_out, int n_mu, int nd,
-                                double *taps, int32_t *krange, int32_t *kflags, int32_t *bins, int32_t *nbins) {
-  int job = blockIdx.x;
+__device__ __forceinline__ void taps_finalize_body(const unsigned long long *acc, int nsets_in, int nsets_out, int n_mu, int nd, double *taps, int32_t *krange, int32_t *kflags, int32_t *bins, int32_t *nbins, long bx_, long by_) {
+  int job = bx_;
   __shared__ int s_lo[256], s_hi[256], s_fl[256];
   for (int k = threadIdx.x; k < n_mu; k += blockDim.x) {
     int lo = nd, hi = -1, fl = 0;
@@ -310,6 +313,11 @@ __global__ void k_taps_finalize(const unsigned long long *acc, int nsets_in, int
       if (s_lo[k] <= s_hi[k]) bins[(long)job * n_mu + c++] = k;
     nbins[job] = c;
   }
+}
+
+__global__ void k_taps_finalize(const unsigned long long *acc, int nsets_in, int nsets_out, int n_mu, int nd,
+                                double *taps, int32_t *krange, int32_t *kflags, int32_t *bins, int32_t *nbins) {
+  taps_finalize_body(acc, nsets_in, nsets_out, n_mu, nd, taps, krange, kflags, bins, nbins, blockIdx.x, blockIdx.y);
 }
 
 hipError_t launch_taps_finalize(const unsigned long long *acc, int n_job, int nsets_in, int nsets_out, int n_mu,
@@ -420,24 +428,25 @@ constexpr int kFirTile = kTPB;
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-__global__ __launch_bounds__(kTPB) void k_fir_active_grp(FirArgs a, int n_job, int n_tiles, int n_groups) {
+template <int G>
+__device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job, int n_tiles, int n_groups, long bx_) {
   extern __shared__ double rows[];  // [4][row_len]: quiet, quiet SED, spot, facula rows of one bin
-  __shared__ int s_fl[kFirG][64];
+  __shared__ int s_fl[G][64];
   __shared__ int s_bins[64], s_ufl[64], s_ulo[64], s_uhi[64];
   __shared__ int s_nb;
-  long b = blockIdx.x;
+  long b = bx_;
   long xcd = b % 8, slot = b / 8;
   long tile = (slot / n_groups) * 8 + xcd;
   int grp = (int)(slot % n_groups);
   if (tile >= n_tiles) return;  // uniform over the block
-  int p0 = grp * kFirG;
-  int ng = min(kFirG, n_job - p0);
+  int p0 = grp * G;
+  int ng = min(G, n_job - p0);
   int nd = a.nd, n_mu = a.n_mu;
   int row_len = kFirTile + nd - 1;
   long j0 = tile * kFirTile;
   long src0 = j0 - a.dmin - nd + 1;
   // group metadata: per-phase flags, union flags and union tap range per bin
-  for (int e = threadIdx.x; e < kFirG * 64; e += blockDim.x) {
+  for (int e = threadIdx.x; e < G * 64; e += blockDim.x) {
     int g = e / 64, k = e % 64;
     int fl = 0;
     if (g < ng && k < n_mu) {
@@ -470,9 +479,9 @@ __global__ __launch_bounds__(kTPB) void k_fir_active_grp(FirArgs a, int n_job, i
     s_nb = c;
   }
   __syncthreads();
-  double F[kFirG][7];
+  double F[G][7];
 #pragma unroll
-  for (int g = 0; g < kFirG; g++)
+  for (int g = 0; g < G; g++)
 #pragma unroll
     for (int f = 0; f < 7; f++) F[g][f] = 0.0;
   const int nb = uni(s_nb);
@@ -491,9 +500,9 @@ __global__ __launch_bounds__(kTPB) void k_fir_active_grp(FirArgs a, int n_job, i
       }
     }
     __syncthreads();
-    int flg[kFirG];
+    int flg[G];
 #pragma unroll
-    for (int g = 0; g < kFirG; g++) flg[g] = uni(s_fl[g][k]);
+    for (int g = 0; g < G; g++) flg[g] = uni(s_fl[g][k]);
     // taps of phase g at offset di: a.taps[(((p0+g)*n_mu + k)*nd + di)*4 + set], uniform addresses
     const double *Tk = a.taps + ((long)p0 * n_mu + k) * nd * 4;
     const long gstride = (long)n_mu * nd * 4;
@@ -504,7 +513,7 @@ __global__ __launch_bounds__(kTPB) void k_fir_active_grp(FirArgs a, int n_job, i
       double vp = (ufl & 1) ? rp[e] : 0.0;
       double vf = (ufl & 2) ? rf[e] : 0.0;
 #pragma unroll
-      for (int g = 0; g < kFirG; g++) {
+      for (int g = 0; g < G; g++) {
         const int fl = flg[g];
         if (!fl) continue;
         const double *T = Tk + g * gstride + di * 4;
@@ -527,7 +536,7 @@ __global__ __launch_bounds__(kTPB) void k_fir_active_grp(FirArgs a, int n_job, i
     double cs = a.sps ? a.sps[j] : 0.0;
     double cf = a.fas ? a.fas[j] : 0.0;
 #pragma unroll
-    for (int g = 0; g < kFirG; g++) {
+    for (int g = 0; g < G; g++) {
       if (g >= ng) break;
       double *o = a.out + (long)(p0 + g) * 3 * a.n;
       // F: 0 = H1a*SEDq, 1 = H1s*Sq, 2 = H2f*Sq, 3 = H1f*Sq, 4 = H1s*Ssp, 5 = H1f*Sfac, 6 = H2f*Sfac
@@ -536,6 +545,10 @@ __global__ __launch_bounds__(kTPB) void k_fir_active_grp(FirArgs a, int n_job, i
       o[2 * a.n + j] = F[g][0] - cs * F[g][4] - cf * F[g][6];
     }
   }
+}
+
+__global__ __launch_bounds__(kTPB) void k_fir_active_grp(FirArgs a, int n_job, int n_tiles, int n_groups) {
+  fir_active_grp_body<kFirG>(a, n_job, n_tiles, n_groups, blockIdx.x);
 }
 
 hipError_t launch_fir_active_grp(const FirArgs &a, int n_job, hipStream_t s) {
@@ -815,10 +828,10 @@ __device__ __forceinline__ void block_minmax_store(double amin, double omax, dou
 // written to the frame slots of the lower-spectrum layout [flux, bconv, tot] =
 // [SED, normalised, SED] (run_all_SOAP.py:359-362), with the partial min(Q) and
 // max(o != 0) (seed -60000) of each tile per slot.
-__global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *quiet, double *frame, double *part) {
+__device__ __forceinline__ void conv_quiet_body(const ConvArgs &c, const double *quiet, double *frame, double *part, long bx_, long by_) {
   extern __shared__ double xs[];
-  int q = blockIdx.y;
-  long i0 = (long)blockIdx.x * kConvTile;
+  int q = by_;
+  long i0 = (long)bx_ * kConvTile;
   int span = kConvTile + c.ntp - 1;
   const double *Q = quiet + (long)q * c.n;
   double *ct = xs + span + (span & 1);  // taps, 16-byte aligned
@@ -844,11 +857,15 @@ __global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *q
     amin = Ai < amin ? Ai : amin;
     if (o != 0.0 && omax < o) omax = o;
   }
-  block_minmax_store(amin, omax, part + ((long)s0 * c.n_tiles + blockIdx.x) * 2);
+  block_minmax_store(amin, omax, part + ((long)s0 * c.n_tiles + bx_) * 2);
   if (q == 1 && threadIdx.x == 0) {
-    part[(2L * c.n_tiles + blockIdx.x) * 2] = part[((long)s0 * c.n_tiles + blockIdx.x) * 2];
-    part[(2L * c.n_tiles + blockIdx.x) * 2 + 1] = part[((long)s0 * c.n_tiles + blockIdx.x) * 2 + 1];
+    part[(2L * c.n_tiles + bx_) * 2] = part[((long)s0 * c.n_tiles + bx_) * 2];
+    part[(2L * c.n_tiles + bx_) * 2 + 1] = part[((long)s0 * c.n_tiles + bx_) * 2 + 1];
   }
+}
+
+__global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *quiet, double *frame, double *part) {
+  conv_quiet_body(c, quiet, frame, part, blockIdx.x, blockIdx.y);
 }
 
 // Active spectra of phase p = blockIdx.y, types 0..2 (flux, bconv, tot):
@@ -861,10 +878,9 @@ __global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *q
 // sel = 0 for bconv, 1 otherwise (SOAP_resolution.cu:414-416).  Blocks are
 // numbered so that all phases of one lambda tile run back to back on one XCD
 // (block b -> XCD b % 8): the tile's quiet rows stay in that XCD's L2.
-__global__ __launch_bounds__(kTPB) void k_conv_active(ConvArgs c, const double *quiet, const double *raw,
-                                                      double *o_out, double *part, int n_phase) {
+__device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double *quiet, const double *raw, double *o_out, double *part, int n_phase, long bx_, long by_) {
   extern __shared__ double xs[];
-  long b = blockIdx.x;
+  long b = bx_;
   long slot = b / 8;
   int tile = (int)((slot / n_phase) * 8 + b % 8);
   int p = (int)(slot % n_phase);
@@ -908,6 +924,11 @@ __global__ __launch_bounds__(kTPB) void k_conv_active(ConvArgs c, const double *
   }
 }
 
+__global__ __launch_bounds__(kTPB) void k_conv_active(ConvArgs c, const double *quiet, const double *raw,
+                                                      double *o_out, double *part, int n_phase) {
+  conv_active_body(c, quiet, raw, o_out, part, n_phase, blockIdx.x, blockIdx.y);
+}
+
 static size_t conv_lds(const ConvArgs &c) { return (size_t)(kConvTile + 2 * c.ntp + 1) * sizeof(double); }
 
 hipError_t launch_conv_quiet(const ConvArgs &c, const double *quiet, double *frame, double *part, hipStream_t s) {
@@ -935,9 +956,9 @@ int conv_tile() { return kConvTile; }
 int conv_chunk() { return kConvCH; }
 
 // min / max of each spectrum over its tile partials (exact: order independent)
-__global__ __launch_bounds__(kTPB) void k_minmax_reduce(const double *part, int n_tiles, double *mm) {
+__device__ __forceinline__ void minmax_body(const double *part, int n_tiles, double *mm, long bx_, long by_) {
   __shared__ double s_min[kTPB], s_max[kTPB];
-  int sp = blockIdx.x;
+  int sp = bx_;
   double a = HUGE_VAL, b = -HUGE_VAL;
   for (int t = threadIdx.x; t < n_tiles; t += blockDim.x) {
     double pa = part[((long)sp * n_tiles + t) * 2], pb = part[((long)sp * n_tiles + t) * 2 + 1];
@@ -960,15 +981,19 @@ __global__ __launch_bounds__(kTPB) void k_minmax_reduce(const double *part, int 
   }
 }
 
+__global__ __launch_bounds__(kTPB) void k_minmax_reduce(const double *part, int n_tiles, double *mm) {
+  minmax_body(part, n_tiles, mm, blockIdx.x, blockIdx.y);
+}
+
 // spec_norm, SOAP_resolution.cu:225-232: out = 1 - o (1 - min) / max, in place.
 // Spectrum sp = blockIdx.y at o + sp*n; a block covers kNormPer*kTPB pixels with
 // coalesced loads issued before the stores.
 constexpr int kNormPer = 8;
-__global__ __launch_bounds__(kTPB) void k_norm(long n, const double *mm, double *o) {
-  long sp = blockIdx.y;
+__device__ __forceinline__ void norm_body(long n, const double *mm, double *o, long bx_, long by_) {
+  long sp = by_;
   double mn = mm[sp * 2], mx = mm[sp * 2 + 1];
   double *x = o + sp * n;
-  long base = (long)blockIdx.x * (kTPB * kNormPer) + threadIdx.x;
+  long base = (long)bx_ * (kTPB * kNormPer) + threadIdx.x;
   double v[kNormPer];
 #pragma unroll
   for (int k = 0; k < kNormPer; k++) {
@@ -980,6 +1005,10 @@ __global__ __launch_bounds__(kTPB) void k_norm(long n, const double *mm, double 
     long i = base + (long)k * kTPB;
     if (i < n) x[i] = 1.0 - v[k] * (1 - mn) / mx;
   }
+}
+
+__global__ __launch_bounds__(kTPB) void k_norm(long n, const double *mm, double *o) {
+  norm_body(n, mm, o, blockIdx.x, blockIdx.y);
 }
 
 hipError_t launch_minmax(int n_spec, int n_tiles, const double *part, double *mm, hipStream_t s) {
@@ -1081,11 +1110,9 @@ hipError_t launch_mask_overwrite(const int32_t *sorted, int n_lines, int n_v, in
 constexpr int kCcfLines = 32;
 constexpr int kCcfSpec = 8;
 
-__global__ void k_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
-                      int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
-                      const double *w_sorted, double *part) {
-  int chunk = blockIdx.x;
-  int s0 = blockIdx.y * kCcfSpec;
+__device__ __forceinline__ void ccf_body(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v, int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv, const double *w_sorted, double *part, long bx_, long by_) {
+  int chunk = bx_;
+  int s0 = by_ * kCcfSpec;
   int ns = min(kCcfSpec, n_spec - s0);
   int q0 = chunk * kCcfLines, q1 = min(n_lines, q0 + kCcfLines);
   const double *S[kCcfSpec];
@@ -1115,9 +1142,15 @@ __global__ void k_ccf(const double *spec, int64_t spec_stride, int64_t trim, int
   }
 }
 
-__global__ void k_ccf_reduce(const double *part, int n_chunks, int n_spec, int n_v, const double *appod, double *ccf) {
+__global__ void k_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
+                      int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
+                      const double *w_sorted, double *part) {
+  ccf_body(spec, spec_stride, trim, n_spec, n_lines, n_v, n_t, m_start, m_pk, m_wv, w_sorted, part, blockIdx.x, blockIdx.y);
+}
+
+__device__ __forceinline__ void ccf_reduce_body(const double *part, int n_chunks, int n_spec, int n_v, const double *appod, double *ccf, long bx_, long by_) {
   __shared__ double s_red[1024];
-  int sp = blockIdx.x;
+  int sp = bx_;
   double v = -HUGE_VAL;
   for (int iv = threadIdx.x; iv < n_v; iv += blockDim.x) {
     double a = 0.0;
@@ -1138,6 +1171,10 @@ __global__ void k_ccf_reduce(const double *part, int n_chunks, int n_spec, int n
     double cc = ccf[(long)sp * n_v + iv] / mx;
     ccf[(long)sp * n_v + iv] = 1 - ((-cc + 1) * appod[iv]);
   }
+}
+
+__global__ void k_ccf_reduce(const double *part, int n_chunks, int n_spec, int n_v, const double *appod, double *ccf) {
+  ccf_reduce_body(part, n_chunks, n_spec, n_v, appod, ccf, blockIdx.x, blockIdx.y);
 }
 
 int ccf_chunks(int n_lines) { return (n_lines + kCcfLines - 1) / kCcfLines; }
@@ -1270,12 +1307,12 @@ __device__ int solve4(const double *A, const double *D, double mu, const double 
 // BIS_FIT2 (BIS_FIT2.cpp:97-320) for one CCF per 64-lane block: the same
 // Levenberg-Marquardt restatement of GSL's trust-region solver as the oracle
 // (More scaling, Nielsen damping, xtol = gtol = 1e-8), then the bisector.
-__global__ __launch_bounds__(64) void k_fit(const double *v, int n, const double *Y, double *par, double *model,
-                                            double *depth_out, double *bis_out, int32_t *iters) {
+__device__ __forceinline__ void fit_body(const double *v, int n, const double *Y, double *par, double *model,
+                                         double *depth_out, double *bis_out, int32_t *iters, long bx_) {
   __shared__ double s_px[kMaxV], s_y[kMaxV], s_nc[kMaxV], s_p0[kMaxV], s_p1[kMaxV], s_p2[kMaxV];
   __shared__ double s_bis[91];
   __shared__ double s_init[4];
-  int cidx = blockIdx.x;
+  int cidx = bx_;
   const double *y = Y + (long)cidx * n;
   for (int i = threadIdx.x; i < n; i += 64) {
     s_px[i] = v[i] - v[0];
@@ -1437,11 +1474,164 @@ __global__ __launch_bounds__(64) void k_fit(const double *v, int n, const double
   }
 }
 
+__global__ __launch_bounds__(64) void k_fit(const double *v, int n, const double *Y, double *par, double *model,
+                                            double *depth_out, double *bis_out, int32_t *iters) {
+  fit_body(v, n, Y, par, model, depth_out, bis_out, iters, blockIdx.x);
+}
+
 hipError_t launch_fit(const double *v, int n_v, const double *y, int n_ccf, double *par, double *model, double *depth,
                       double *bis, int32_t *iters, hipStream_t s) {
   if (n_ccf == 0) return hipSuccess;
   if (n_v > kMaxV || n_v < 3) return hipErrorInvalidValue;
   k_fit<<<n_ccf, 64, 0, s>>>(v, n_v, y, par, model, depth, bis, iters);
+  return hipGetLastError();
+}
+
+
+// ------------------------------------------------------------------ a11: batched instances
+// One launch per stage for all instances of a batch.  Each block walks the
+// flattened work space with a grid stride; the owning instance of item w is
+// found by binary search over the prefix of per-instance counts (uniform over
+// the block, so its loads are scalar).  The body is the single-instance
+// kernel's, given the item's local block index.  A persistent grid whose size
+// is a multiple of 8 keeps item w on XCD w % 8, as in the single-instance
+// launch of the XCD-ordered stages.
+constexpr int kBatchFillChunk = kTPB * 16;
+
+__device__ __forceinline__ int batch_find(const int64_t *__restrict__ prefix, int n_inst, int64_t w) {
+  int lo = 0, hi = n_inst;  // prefix[lo] <= w < prefix[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= w) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <class Job, class F>
+__device__ __forceinline__ void batch_loop(const Job *__restrict__ jobs, const int64_t *__restrict__ prefix, int n_inst,
+                                           int64_t total, F body) {
+  for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
+    int i = batch_find(prefix, n_inst, w);
+    body(jobs[i], (long)(w - prefix[i]));
+    __syncthreads();  // the next item reuses the block's LDS
+  }
+}
+
+#define SOAP_BATCH_ARGS(JOB) const JOB *__restrict__ jobs, const int64_t *__restrict__ prefix, int n_inst, int64_t total
+
+__global__ __launch_bounds__(kTPB) void kb_fill(SOAP_BATCH_ARGS(FillJob)) {
+  batch_loop(jobs, prefix, n_inst, total, [&](const FillJob &j, long local) {
+    int64_t e0 = (int64_t)local * kBatchFillChunk;
+    int64_t e1 = min(j.count, e0 + kBatchFillChunk);
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) j.p[e] = 0ull;
+  });
+}
+
+__global__ __launch_bounds__(kTPB) void kb_taps(SOAP_BATCH_ARGS(TapsJob)) {
+  batch_loop(jobs, prefix, n_inst, total,
+             [&](const TapsJob &j, long local) { taps_active_body(j.a, local % j.mcb, local / j.mcb); });
+}
+
+__global__ __launch_bounds__(64) void kb_finalize(SOAP_BATCH_ARGS(FinalizeJob)) {
+  batch_loop(jobs, prefix, n_inst, total, [&](const FinalizeJob &j, long local) {
+    taps_finalize_body(j.acc, j.nsets_in, j.nsets_out, j.n_mu, j.nd, j.taps, j.krange, j.kflags, j.bins, j.nbins, local,
+                       0);
+  });
+}
+
+template <int G>
+__global__ __launch_bounds__(kTPB) void kb_fir(SOAP_BATCH_ARGS(FirJob)) {
+  batch_loop(jobs, prefix, n_inst, total, [&](const FirJob &j, long local) {
+    fir_active_grp_body<G>(j.a, j.n_job, j.n_tiles, j.n_groups, local);
+  });
+}
+
+__global__ __launch_bounds__(kTPB) void kb_conv_quiet(SOAP_BATCH_ARGS(ConvQJob)) {
+  batch_loop(jobs, prefix, n_inst, total, [&](const ConvQJob &j, long local) {
+    conv_quiet_body(j.c, j.quiet, j.frame, j.part, local % j.c.n_tiles, local / j.c.n_tiles);
+  });
+}
+
+__global__ __launch_bounds__(kTPB) void kb_conv(SOAP_BATCH_ARGS(ConvAJob)) {
+  batch_loop(jobs, prefix, n_inst, total, [&](const ConvAJob &j, long local) {
+    conv_active_body(j.c, j.quiet, j.raw, j.out, j.part, j.n_phase, local, 0);
+  });
+}
+
+__global__ __launch_bounds__(kTPB) void kb_minmax(SOAP_BATCH_ARGS(MinmaxJob)) {
+  batch_loop(jobs, prefix, n_inst, total,
+             [&](const MinmaxJob &j, long local) { minmax_body(j.part, j.n_tiles, j.mm, local, 0); });
+}
+
+__global__ __launch_bounds__(kTPB) void kb_norm(SOAP_BATCH_ARGS(NormJob)) {
+  batch_loop(jobs, prefix, n_inst, total,
+             [&](const NormJob &j, long local) { norm_body(j.n, j.mm, j.o, local % j.bx, local / j.bx); });
+}
+
+__global__ __launch_bounds__(1024) void kb_ccf(SOAP_BATCH_ARGS(CcfJob)) {
+  batch_loop(jobs, prefix, n_inst, total, [&](const CcfJob &j, long local) {
+    ccf_body(j.spec, j.stride, j.trim, j.n_spec, j.n_lines, j.n_v, j.n_t, j.m_start, j.m_pk, j.m_wv, j.w_sorted, j.part,
+             local % j.nc, local / j.nc);
+  });
+}
+
+__global__ __launch_bounds__(1024) void kb_ccf_reduce(SOAP_BATCH_ARGS(CcfRedJob)) {
+  batch_loop(jobs, prefix, n_inst, total, [&](const CcfRedJob &j, long local) {
+    ccf_reduce_body(j.part, j.nc, j.n_spec, j.n_v, j.appod, j.ccf, local, 0);
+  });
+}
+
+__global__ __launch_bounds__(64) void kb_fit(SOAP_BATCH_ARGS(FitJob)) {
+  batch_loop(jobs, prefix, n_inst, total, [&](const FitJob &j, long local) {
+    fit_body(j.v, j.n_v, j.y, j.par, j.model, j.depth, j.bis, j.iters, local);
+  });
+}
+
+int batch_fill_chunk() { return kBatchFillChunk; }
+int norm_px_per_block() { return kTPB * kNormPer; }
+int ccf_spec_chunk() { return kCcfSpec; }
+int fir_tile() { return kFirTile; }
+size_t fir_lds(int nd) { return (size_t)4 * (kFirTile + nd - 1) * sizeof(double); }
+size_t conv_lds_bytes(int ntp) { return (size_t)(kConvTile + 2 * ntp + 1) * sizeof(double); }
+size_t taps_lds_bytes(const TapAccArgs &a) { return (size_t)a.nsets * a.n_mu * a.wd * 2 * sizeof(unsigned long long); }
+
+hipError_t launch_batch(int st, const void *jobs, const int64_t *prefix, int n_inst, int64_t total, int64_t grid,
+                        int threads, size_t lds, int fir_g, hipStream_t s) {
+  if (total == 0) return hipSuccess;
+  if (n_inst < 1 || total < 0 || grid < 0 || lds > 65536) return hipErrorInvalidValue;
+  int64_t g = grid == 0 ? total : std::min<int64_t>(grid, total);
+  if (g > 0x7fffffffL) return hipErrorInvalidValue;
+  unsigned nb = (unsigned)g;
+  switch (st) {
+#define SOAP_BATCH_CASE(ID, K, JOB, TPB)                                                                 \
+  case ID:                                                                                               \
+    if (threads > TPB || threads < 64 || threads % 64) return hipErrorInvalidValue;                     \
+    K<<<nb, threads, lds, s>>>((const JOB *)jobs, prefix, n_inst, total);                               \
+    break;
+    SOAP_BATCH_CASE(BS_FILL, kb_fill, FillJob, kTPB)
+    SOAP_BATCH_CASE(BS_TAPS, kb_taps, TapsJob, kTPB)
+    SOAP_BATCH_CASE(BS_FINALIZE, kb_finalize, FinalizeJob, 64)
+    SOAP_BATCH_CASE(BS_CONV_QUIET, kb_conv_quiet, ConvQJob, kTPB)
+    SOAP_BATCH_CASE(BS_CONV, kb_conv, ConvAJob, kTPB)
+    SOAP_BATCH_CASE(BS_MINMAX, kb_minmax, MinmaxJob, kTPB)
+    SOAP_BATCH_CASE(BS_NORM, kb_norm, NormJob, kTPB)
+    SOAP_BATCH_CASE(BS_CCF, kb_ccf, CcfJob, 1024)
+    SOAP_BATCH_CASE(BS_CCF_REDUCE, kb_ccf_reduce, CcfRedJob, 1024)
+    SOAP_BATCH_CASE(BS_FIT, kb_fit, FitJob, 64)
+#undef SOAP_BATCH_CASE
+    case BS_FIR:
+      if (threads != kTPB) return hipErrorInvalidValue;
+      switch (fir_g) {
+        case 2: kb_fir<2><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total); break;
+        case 4: kb_fir<4><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total); break;
+        case 6: kb_fir<6><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total); break;
+        default: return hipErrorInvalidValue;
+      }
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -54,6 +54,9 @@ EXPORTS = (
     "soap_get_integrate_stats",
     "soap_get_stage_work",
     "soap_diff_setup", "soap_diff_forward", "soap_diff_backward", "soap_diff_get",
+    "soap_batch_create", "soap_batch_destroy", "soap_batch_last_error", "soap_batch_step", "soap_batch_sync",
+    "soap_batch_stage_ms", "soap_batch_stage_name", "soap_batch_num_stages", "soap_batch_get_config",
+    "soap_batch_set_config", "soap_batch_tune",
 )
 
 _lib = None
@@ -120,6 +123,19 @@ def load(path: str = LIB_PATH):
                                              ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]),
         "soap_diff_backward": (ctypes.c_int, [vp, _dp, vp, vp, vp]),
         "soap_diff_get": (ctypes.c_int, [vp, ctypes.c_int32, _dp]),
+        "soap_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int32, _dp, _dp, ctypes.POINTER(vp)]),
+        "soap_batch_destroy": (None, [vp]),
+        "soap_batch_last_error": (ctypes.c_char_p, [vp]),
+        "soap_batch_step": (ctypes.c_int, [vp, ctypes.c_int32]),
+        "soap_batch_sync": (ctypes.c_int, [vp]),
+        "soap_batch_stage_ms": (ctypes.c_int, [vp, _dp, ctypes.c_int32]),
+        "soap_batch_stage_name": (ctypes.c_char_p, [ctypes.c_int32]),
+        "soap_batch_num_stages": (ctypes.c_int32, []),
+        "soap_batch_get_config": (ctypes.c_int, [vp, _ip, _ip, ctypes.POINTER(ctypes.c_int64),
+                                                 ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
+        "soap_batch_set_config": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32,
+                                                 ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
+        "soap_batch_tune": (ctypes.c_int, [vp, ctypes.c_int32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -385,3 +401,70 @@ def setup_workload(eng: Engine, w, with_mask: bool = True, trim: int = 210):
         wave_t = w.wave[trim:len(w.wave) - trim]
         sel = (w.mask_l > wave_t.min()) & (w.mask_l < wave_t.max()) & (w.mask_w > 0.1)
         eng.set_mask(w.mask_l[sel], w.mask_w[sel], w.vgrid, trim)
+
+
+class Batch:
+    """Many prepared Engines (log grids, activity and mask set, disk_init and
+    quiet_star done) stepped together: one persistent launch per stage over all
+    instances (soap_batch_* of the C ABI), optionally as a HIP graph."""
+
+    def __init__(self, engines, sigma_ms, quiet_weight=0.005):
+        self.L = load()
+        self.engines = list(engines)
+        n = len(self.engines)
+        ctxs = (ctypes.c_void_p * n)(*[e.ctx for e in self.engines])
+        sig, sp = _d(np.broadcast_to(np.asarray(sigma_ms, dtype=np.float64), (n,)))
+        qw, qp = _d(np.broadcast_to(np.asarray(quiet_weight, dtype=np.float64), (n,)))
+        self.b = ctypes.c_void_p()
+        rc = self.L.soap_batch_create(ctxs, n, sp, qp, ctypes.byref(self.b))
+        if rc != 0:
+            msg = self.L.soap_batch_last_error(self.b) if self.b else b"allocation failed"
+            self.close()
+            raise SoapError(f"soap_batch_create failed ({rc}): {msg.decode() if msg else ''}")
+        self.n_stage = self.L.soap_batch_num_stages()
+        self.stage_names = [self.L.soap_batch_stage_name(i).decode() for i in range(self.n_stage)]
+
+    def close(self):
+        if self.b:
+            self.L.soap_batch_destroy(self.b)
+            self.b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.soap_batch_last_error(self.b)
+            raise SoapError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def step(self, graph: bool = False):
+        self._chk(self.L.soap_batch_step(self.b, int(bool(graph))), "batch_step")
+
+    def sync(self):
+        self._chk(self.L.soap_batch_sync(self.b), "batch_sync")
+
+    def stage_ms(self) -> dict:
+        ms = np.zeros(self.n_stage)
+        self._chk(self.L.soap_batch_stage_ms(self.b, ms.ctypes.data_as(_dp), self.n_stage), "batch_stage_ms")
+        return dict(zip(self.stage_names, ms.tolist()))
+
+    def tune(self, reps: int = 2):
+        self._chk(self.L.soap_batch_tune(self.b, int(reps)), "batch_tune")
+
+    def config(self) -> dict:
+        cells = ctypes.c_int32()
+        g = ctypes.c_int32()
+        grid = (ctypes.c_int64 * self.n_stage)()
+        work = (ctypes.c_int64 * self.n_stage)()
+        self._chk(self.L.soap_batch_get_config(self.b, ctypes.byref(cells), ctypes.byref(g), grid, work,
+                                               self.n_stage), "batch_get_config")
+        return {"taps_cells": cells.value, "fir_g": g.value,
+                "grid": dict(zip(self.stage_names, list(grid))), "work": dict(zip(self.stage_names, list(work)))}
+
+    def set_config(self, taps_cells: int, fir_g: int, grid: dict):
+        arr = (ctypes.c_int64 * self.n_stage)(*[int(grid.get(n, 0)) for n in self.stage_names])
+        self._chk(self.L.soap_batch_set_config(self.b, int(taps_cells), int(fir_g), arr, self.n_stage),
+                  "batch_set_config")

@@ -255,3 +255,77 @@ def linear_small(seed: int = 11, grid: int = 48, n: int = 4000, n_mu: int = 6, n
     return Workload("linear_small", wave, 0, bounds, cubes, lines, lines["center"].copy(), lines["depth"].copy(),
                     _star(grid, vsini, incl=incl), _act(), phases, [list(regions) for _ in phases],
                     sigma_inst_ms(115000), vel_grid(20000.0, 500.0), meta={"step": 0.005})
+
+
+def random_regions(rng: np.random.Generator, n_reg: int):
+    """n_reg regions, faculae (r 0.1-0.2) first, then spots (r 0.05-0.12) each placed
+    on a facula so that the two overlap (the later region wins a shared cell,
+    SOAP_integration.cu:364-381)."""
+    n_fac = (n_reg + 1) // 2
+    regs = []
+    for _ in range(n_fac):
+        regs.append((rng.uniform(-60.0, 60.0), rng.uniform(0.0, 360.0), rng.uniform(0.1, 0.2), 1.0))
+    for _ in range(n_reg - n_fac):
+        lat_f, lon_f, rf, _t = regs[int(rng.integers(0, n_fac))]
+        off = np.rad2deg(0.5 * rf)
+        regs.append((lat_f + rng.uniform(-off, off), lon_f + rng.uniform(-off, off), rng.uniform(0.05, 0.12), 0.0))
+    return regs
+
+
+def batch_instance(rng: np.random.Generator, n: int, grid: int, n_phase: int, n_reg: int, vsini: float,
+                   incl: float, name: str = "instance", n_mu: int = 10) -> Workload:
+    """One instance of a batch, built like the small plain workload: log-uniform grid
+    from 5000 A at the small workload's spacing (20k points per 100 A, so the
+    lambda range grows with n), 2 lines per A with the usual depth / width
+    draws, n_mu mu bins, R = 115000, a mask of all lines and a velocity window of
+    20 km/s + 2 vsini at 0.5 km/s."""
+    h = np.log(5100.0 / 5000.0) / 19999
+    lmax = 5000.0 * np.exp(h * (n - 1))
+    wave = 5000.0 * np.exp(np.arange(n, dtype=np.float64) * h)
+    lines = draw_lines(rng, max(4, int(round(2.0 * (lmax - 5000.0)))), 5000.0, lmax)
+    bounds, _ = mu_bins(n_mu)
+    cubes = make_cubes(wave, lines, n_mu)
+    regs = random_regions(rng, n_reg)
+    phases = np.arange(n_phase, dtype=np.float64) / n_phase
+    window = 20000.0 + 2000.0 * np.ceil(vsini)
+    return Workload(name, wave, 1, bounds, cubes, lines, lines["center"].copy(), lines["depth"].copy(),
+                    _star(grid, vsini, incl=incl), _act(), phases, [list(regs) for _ in phases],
+                    sigma_inst_ms(115000), vel_grid(window, 500.0),
+                    meta={"n_reg": n_reg, "vsini": vsini, "incl": incl})
+
+
+def config2_params(seed: int = 1236, n_inst: int = 256):
+    """Per-instance draws of the batched forward workload: (child seed, n, grid,
+    phases, regions, vsini, inclination) with grid N in [200, 1000], 5k-200k lambda
+    points, 10-100 phases, 1-6 overlapping regions, vsini U(1, 20) km/s,
+    inclination U(10, 90) deg."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_inst):
+        out.append((int(rng.integers(0, 2 ** 62)), int(rng.integers(5000, 200001)), int(rng.integers(200, 1001)),
+                    int(rng.integers(10, 101)), int(rng.integers(1, 7)), float(rng.uniform(1.0, 20.0)),
+                    float(rng.uniform(10.0, 90.0))))
+    return out
+
+
+def config2_instance(params, i: int = 0) -> Workload:
+    cs, n, grid, n_phase, n_reg, vsini, incl = params
+    return batch_instance(np.random.default_rng(cs), n, grid, n_phase, n_reg, vsini, incl, name=f"config2[{i}]")
+
+
+def config2(seed: int = 1236, n_inst: int = 256):
+    """Batched forward: n_inst uneven instances (config2_params).  A generator:
+    instances are built one at a time so that the host never holds all seed
+    cubes."""
+    for i, prm in enumerate(config2_params(seed, n_inst)):
+        yield config2_instance(prm, i)
+
+
+def small_batch(seed: int = 17, n_inst: int = 5):
+    """Uneven tiny instances for batch parity tests (grid 24-72, 2600-6000 points,
+    2-7 phases, 1-5 regions, vsini 1-20 km/s, inclination 10-90 deg)."""
+    rng = np.random.default_rng(seed)
+    return [batch_instance(rng, int(rng.integers(2600, 6001)), int(rng.integers(24, 73)), int(rng.integers(2, 8)),
+                           int(rng.integers(1, 6)), float(rng.uniform(1.0, 20.0)), float(rng.uniform(10.0, 90.0)),
+                           name=f"small_batch[{i}]", n_mu=6)
+            for i in range(n_inst)]
