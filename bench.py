@@ -10,6 +10,12 @@ star (normalised + SED), active-region deltas of all 100 phases, instrumental
 convolution + normalisation of 3x100 + 3 spectra, mask CCF of 303 spectra and
 the Gaussian fit + bisector of 303 CCFs.  value = 100 x steps / wall time.
 
+Further legs in the same line: "adjoint" (the differentiable model on a grid
+600 / 100k-pixel / 50-phase / 512x512-map shape: forward and forward + adjoint
+cost) and "batch" (256 uneven instances stepped by one persistent launch per
+stage, eager and as a HIP graph, at autotuned launch shapes, against the plain
+per-instance path).
+
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -96,6 +102,209 @@ def cpu_baseline(w, frac_active: float, frac_quiet: float, nthreads: int):
                 seconds=measured)
 
 
+BATCH_STAGE_WORK = {"taps": "taps_active", "fir": "fir_active", "conv_quiet": "conv_quiet", "conv": "conv",
+                    "minmax": "minmax", "norm": "norm", "ccf": "ccf", "ccf_reduce": "ccf_reduce", "fit": "fit"}
+
+
+def batch_leg(n_inst: int, steps: int, warmup: int, workers: int = 8):
+    """Batched forward of n_inst uneven instances (grid 200-1000, 5k-200k lambda
+    points, 10-100 phases, 1-6 overlapping regions, vsini U(1, 20) km/s,
+    inclination U(10, 90) deg): the per-instance plain path (each instance's
+    stage launches on its own stream), the batched step (one persistent launch
+    per stage for all instances) eager, and its HIP graph, at the autotuned
+    shapes.  value = sum of phases / step time."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from soap_gpu_amd import synth
+    from soap_gpu_amd.engine import Batch, Engine, setup_workload
+
+    t0 = time.time()
+    prm = synth.config2_params(n_inst=n_inst)
+    engines, sig, info = [], [], []
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        for c0 in range(0, n_inst, 2 * workers):
+            ws = list(ex.map(lambda ip: synth.config2_instance(ip[1], ip[0]),
+                             list(enumerate(prm))[c0:c0 + 2 * workers]))
+            for w in ws:
+                e = Engine(0)
+                setup_workload(e, w)
+                e.quiet_star()
+                engines.append(e)
+                sig.append(w.sigma_inst)
+                info.append((w.n, len(w.phases), w.star["grid"], len(w.mask_l), len(w.vgrid)))
+            del ws
+            if len(engines) % 64 == 0:
+                log(f"[bench:batch] {len(engines)}/{n_inst} instances on device ({time.time() - t0:.0f} s)")
+    n_arr = np.array([i[0] for i in info])
+    p_arr = np.array([i[1] for i in info])
+    g_arr = np.array([i[2] for i in info])
+    log(f"[bench:batch] {n_inst} instances set up in {time.time() - t0:.1f} s "
+        f"(sum n*P = {np.sum(n_arr * p_arr):.3g})")
+    t0 = time.time()
+    b = Batch(engines, sig)
+    log(f"[bench:batch] batch created (one plain run per instance) in {time.time() - t0:.1f} s")
+    phases = int(p_arr.sum())
+
+    def run_plain():
+        for e, s_ in zip(engines, sig):
+            e.integrate()
+            e.resolution(s_)
+            e.ccf()
+            e.fit()
+
+    def sync_all():
+        for e in engines:
+            e.sync()
+
+    def timed(fn, sync, k):
+        sync()
+        t = time.perf_counter()
+        for _ in range(k):
+            fn()
+        sync()
+        return (time.perf_counter() - t) / k * 1e3
+
+    run_plain()
+    sync_all()
+    ms_plain = timed(run_plain, sync_all, steps)
+    ref_par = [e.get_fit()["par"].copy() for e in engines]
+    log(f"[bench:batch] plain per-instance path: {ms_plain:.2f} ms/step")
+    plain_stage = {}
+    for e in engines:
+        e.set_timing(True)
+    run_plain()
+    sync_all()
+    for e in engines:
+        for k, v in e.stage_ms().items():
+            plain_stage[k] = plain_stage.get(k, 0.0) + v
+        e.set_timing(False)
+    plain_stage = {k: round(v, 4) for k, v in plain_stage.items() if v > 0}
+    log("[bench:batch] plain per-instance stage sums (ms): " + ", ".join(f"{k}={v:.2f}" for k, v in plain_stage.items()))
+    for _ in range(warmup):
+        b.step(graph=False)
+    ms_eager_default = timed(lambda: b.step(graph=False), b.sync, steps)
+    log(f"[bench:batch] batched eager, default shapes: {ms_eager_default:.2f} ms/step")
+    t0 = time.time()
+    b.tune(reps=2)
+    t_tune = time.time() - t0
+    cfg = b.config()
+    log(f"[bench:batch] autotuned in {t_tune:.1f} s: taps_cells={cfg['taps_cells']} fir_g={cfg['fir_g']} "
+        f"grid={cfg['grid']}")
+    ms_eager = timed(lambda: b.step(graph=False), b.sync, steps)
+    b.step(graph=True)
+    b.sync()
+    ms_graph = timed(lambda: b.step(graph=True), b.sync, steps)
+    log(f"[bench:batch] tuned eager {ms_eager:.2f} ms/step, graph {ms_graph:.2f} ms/step")
+    equal = all(np.array_equal(e.get_fit()["par"], r) for e, r in zip(engines, ref_par))
+    stage = {k: round(v, 4) for k, v in b.stage_ms().items()}
+    work = {}
+    for bs, es in BATCH_STAGE_WORK.items():
+        by = fl = 0.0
+        for e in engines:
+            x, y = e.stage_work(es)
+            by += x
+            fl += y
+        work[bs] = (by, fl)
+    rows = sum(e.integrate_stats()[2] for e in engines)
+    series_bytes = float(np.sum(8.0 * n_arr * 10 * p_arr) + sum(8.0 * e.n * e.integrate_stats()[2] for e in engines))
+    kern = {}
+    for bs, (by, fl) in work.items():
+        ms = stage[bs]
+        if ms <= 0:
+            continue
+        kern[bs] = dict(ms=ms, bytes=by, flops=fl, GBs=round(by / (ms * 1e-3) / 1e9, 1),
+                        TFs=round(fl / (ms * 1e-3) / 1e12, 2))
+    ms_best = min(ms_eager, ms_graph)
+    out = dict(
+        instances=n_inst, phases=phases, lambda_points=[int(n_arr.min()), int(n_arr.max())],
+        grid=[int(g_arr.min()), int(g_arr.max())], phase_range=[int(p_arr.min()), int(p_arr.max())],
+        value=round(phases / (ms_best * 1e-3), 1), unit="phase-spectra/s",
+        ms_per_step={"plain_per_instance": round(ms_plain, 3), "batched_eager_default": round(ms_eager_default, 3),
+                     "batched_eager_tuned": round(ms_eager, 3), "batched_graph_tuned": round(ms_graph, 3)},
+        speedup_vs_plain=round(ms_plain / ms_best, 2),
+        plain_stage_ms_sum=plain_stage,
+        results_equal_plain=bool(equal),
+        tuned=cfg, tune_seconds=round(t_tune, 1), stage_ms=stage, kernels=kern,
+        series_hbm={"bytes_per_step": series_bytes, "rows_sum": int(rows),
+                    "GBs": round(series_bytes / (ms_best * 1e-3) / 1e9, 1),
+                    "hbm_frac": round(series_bytes / (ms_best * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)},
+        data="synthetic, seeded (uneven instances, config2_params seed 1236)")
+    b.close()
+    for e in engines:
+        e.close()
+    return out
+
+
+def adjoint_leg(steps: int):
+    """Differentiable model on the adjoint shape (grid 600, 100k-pt lambda
+    5000-5500 A, 50 phases, 8000-line mask, 512 x 512 smooth-blob filling-factor
+    map per phase): time of the forward (loss) and of forward + adjoint (loss and
+    every gradient), and a directional check of the vsini and contrast gradients
+    against central differences (h = 1e-6 relative) of the same forward."""
+    import torch
+
+    from soap_gpu_amd import synth
+    from soap_gpu_amd.diff import DiffModel
+    from soap_gpu_amd.engine import Engine, select_mask_lines
+
+    t0 = time.time()
+    w, maps = synth.config3()
+    P, M = maps.shape[0], maps.shape[1]
+    trim = 210
+    sl, sw = select_mask_lines(w.wave, w.mask_l, w.mask_w, trim)
+    eng = Engine(0)
+    model = DiffModel(eng, w.wave, w.mu_bounds, sl, sw, w.vgrid, w.star["grid"], P, M, w.sigma_inst, trim)
+    dev = torch.device("cuda")
+    tq = torch.tensor(w.cubes["q"], dtype=torch.float64, device=dev)
+    ts = torch.tensor(w.cubes["sp"], dtype=torch.float64, device=dev)
+    tm = torch.tensor(maps, dtype=torch.float64, device=dev)
+    z = torch.zeros((P, w.n), dtype=torch.float64, device=dev)
+    log(f"[bench:adjoint] set up in {time.time() - t0:.1f} s: n={w.n} grid={w.star['grid']} P={P} map={M}x{M} "
+        f"mask={len(sl)} lines")
+    # target: the model at perturbed parameters and maps, plus seeded noise
+    model.forward((2.3, 0.31, 0.30, 0.35), tq, ts, tm * 0.8, z, np.zeros(P), np.zeros(P), 1.0, 0.0, 0.0)
+    F = model.get("F")
+    th = model.get("theta")
+    rng = np.random.default_rng(1237)
+    T = torch.tensor(F + 1e-4 * np.abs(F).max() * rng.standard_normal(F.shape), dtype=torch.float64, device=dev)
+    rv_t = th[:, 2] + rng.uniform(-5.0, 5.0, P)
+    wts = (1.0 / float(torch.sum(T[:, trim:w.n - trim] ** 2)), 1e-3, 0.0)
+    prm = np.array([2.0, 0.29, 0.34, 0.3])
+
+    def fwd(p=prm):
+        return model.forward(p, tq, ts, tm, T, rv_t, np.zeros(P), *wts)[0]
+
+    fwd()
+    model.backward()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for _ in range(steps):
+        fwd()
+    torch.cuda.synchronize()
+    ms_f = (time.perf_counter() - t) / steps * 1e3
+    t = time.perf_counter()
+    for _ in range(steps):
+        fwd()
+        g4, gq, gs, gm = model.backward()
+    torch.cuda.synchronize()
+    ms_fb = (time.perf_counter() - t) / steps * 1e3
+    log(f"[bench:adjoint] forward {ms_f:.2f} ms, forward + adjoint {ms_fb:.2f} ms")
+    fd = {}
+    for i, name in ((0, "vsini"), (3, "contrast")):
+        h = 1e-6 * prm[i]
+        e = np.eye(4)[i] * h
+        d = (fwd(prm + e) - fwd(prm - e)) / (2 * h)
+        fd[name] = {"adjoint": float(g4[i]), "central_fd": float(d), "rel_err": float(abs(g4[i] - d) / abs(d))}
+    eng.close()
+    return dict(config={"workload": f"differentiable model: grid {w.star['grid']}, {w.n}-pt log lambda 5000-5500 A, "
+                                    f"{P} phases, {len(sl)}-line mask, {M}x{M} filling-factor maps, "
+                                    f"{w.n_mu} mu bins"},
+                ms_forward=round(ms_f, 3), ms_forward_adjoint=round(ms_fb, 3),
+                adjoint_over_forward=round(ms_fb / ms_f, 3),
+                gradients=["vsini", "u1", "u2", "contrast", "quiet seed rows", "spot seed rows", "map pixels"],
+                fd_check_vs_gpu_forward=fd)
+
+
 def load_traffic(kernel: str, workload: str):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel calibration)."""
@@ -120,14 +329,29 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frac", type=float, default=0.25, help="pixel fraction of the active-phase CPU sample")
     ap.add_argument("--cpu-frac-quiet", type=float, default=0.08, help="pixel fraction of the quiet-star CPU sample")
+    ap.add_argument("--batch-instances", type=int, default=256, help="instances of the batched leg (0 skips it)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost leg")
+    ap.add_argument("--legs", default="series,adjoint,batch", help="comma list of legs to run (series always prints)")
     args = ap.parse_args()
+    legs = set(args.legs.split(","))
     if args.gpus != 1:
         print("bench.py measures one GPU: the series shards by phase across GPUs by design, but this build "
               "runs and reports a single MI355X only; use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    import torch  # noqa: F401  (HIP runtime of torch first: the adjoint leg moves tensors)
+
     from soap_gpu_amd import synth
     from soap_gpu_amd.engine import Engine, setup_workload
+
+    if "series" not in legs:
+        out = {"legs": sorted(legs)}
+        if "adjoint" in legs:
+            out["adjoint"] = adjoint_leg(max(3, min(args.steps, 10)))
+        if "batch" in legs:
+            out["batch"] = batch_leg(args.batch_instances, max(2, min(args.steps, 5)), 1)
+        print(json.dumps(out), flush=True)
+        return
 
     t0 = time.time()
     w = synth.config1(n_phase=args.phases, grid=args.grid)
@@ -202,7 +426,7 @@ def main():
         cpu.pop("seconds")
 
     out = {
-        "metric": "phase-spectra/s (1000^2 grid, 733k-pt lambda, FP64)",
+        "metric": "phase-spectra/s (1000^2 grid, 733k-pt lambda, FP64), % HBM peak; adjoint/fwd cost",
         "value": round(value, 3),
         "unit": "phase-spectra/s",
         "n_gpus": 1,
@@ -224,8 +448,22 @@ def main():
     }
     if cpu:
         out["vs_cpu"] = round(value / cpu["value"], 1)
-    print(json.dumps(out), flush=True)
     eng.close()
+    del w
+    # further legs: the adjoint cost and the batched uneven instances
+    if not args.no_adjoint and "adjoint" in legs:
+        try:
+            out["adjoint"] = adjoint_leg(max(3, min(args.steps, 10)))
+        except Exception as ex:  # reported, not hidden: the main line still prints
+            out["adjoint"] = {"error": f"{type(ex).__name__}: {ex}"}
+            log(f"[bench:adjoint] failed: {ex}")
+    if args.batch_instances > 0 and "batch" in legs:
+        try:
+            out["batch"] = batch_leg(args.batch_instances, max(2, min(args.steps, 5)), 1)
+        except Exception as ex:
+            out["batch"] = {"error": f"{type(ex).__name__}: {ex}"}
+            log(f"[bench:batch] failed: {ex}")
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -193,6 +193,12 @@ int32_t soap_num_stages(void);
  * phase (read once) + continua + 3 outputs per phase; bytes_phase_sum = the same
  * counted per phase, sum_psi 8 n (R_psi + continua + 3); rows_sum = sum_psi R_psi */
 int soap_get_integrate_stats(soap_ctx *ctx, double *bytes_min, double *bytes_phase_sum, int64_t *rows_sum);
+/* Launch shape of the active-phase FIR of soap_integrate (log grids, non-solar):
+ * (4, 0) = four phases per block, one pixel per thread; (G, B) with B > 0 =
+ * G phases per block, B pixels per thread (register-blocked), one of (1, 3),
+ * (1, 5), (2, 3), (2, 1), (4, 1); (-1, -1) = automatic.  Results agree to
+ * rounding across shapes.  No counterpart in the reference (tuning support). */
+int soap_set_fir_shape(soap_ctx *ctx, int32_t phases_per_block, int32_t pixels_per_thread);
 /* ------------------------------------------------------------ a11: differentiable path
  * No counterpart in the reference (a differentiable forward model with its FP64
  * adjoint, for fitting).  Model on the context's log-uniform grid, mu bounds
@@ -256,11 +262,14 @@ int soap_batch_sync(soap_batch *b);
 int soap_batch_stage_ms(soap_batch *b, double *ms, int32_t n_stage);
 const char *soap_batch_stage_name(int32_t stage);
 int32_t soap_batch_num_stages(void);
-/* launch shapes: disk cells per taps block, phases per FIR block, and per stage
- * the persistent block count (0 = one block per work item) and work items */
-int soap_batch_get_config(soap_batch *b, int32_t *taps_cells, int32_t *fir_g, int64_t *grid, int64_t *work,
+/* launch shapes: disk cells per taps block; FIR phases per block (fir_g) and
+ * pixels per thread (fir_b, 0 = one-pixel form; (g, b) one of (2,0) (4,0) (6,0)
+ * (1,3) (1,5) (2,3)); per stage the persistent block count (0 = one block per
+ * work item, else a multiple of 8) and the work items */
+int soap_batch_get_config(soap_batch *b, int32_t *taps_cells, int32_t *fir_g, int32_t *fir_b, int64_t *grid,
+                          int64_t *work, int32_t n_stage);
+int soap_batch_set_config(soap_batch *b, int32_t taps_cells, int32_t fir_g, int32_t fir_b, const int64_t *grid,
                           int32_t n_stage);
-int soap_batch_set_config(soap_batch *b, int32_t taps_cells, int32_t fir_g, const int64_t *grid, int32_t n_stage);
 /* autotuner: coordinate search over the shapes above, scored by stage time
  * (best of reps eager steps per candidate) */
 int soap_batch_tune(soap_batch *b, int32_t reps);

@@ -86,6 +86,7 @@ struct soap_ctx {
   RegionDev *d_geo = nullptr;
   int32_t *d_geo_off = nullptr, *d_box = nullptr;
   int max_col_blocks = 0;
+  int fir_g = 0, fir_b = -1;  // FIR launch shape (phases per block, pixels per thread); -1 = automatic
   // differentiable path
   struct {
     bool ready = false, have_fwd = false;
@@ -1031,7 +1032,9 @@ int soap_integrate(soap_ctx *c) {
         return fail(c, SOAP_ERR_LIMIT, "integrate: solar mode needs n_mu <= 64 and a shift range of < 7000 pixels");
       HIPCHK(c, e);
     } else {
-      hipError_t e = launch_fir_active_grp(f, c->P, c->stream);
+      int fg = c->fir_g, fb = c->fir_b;
+      if (fb < 0) fg = 4, fb = 0;
+      hipError_t e = fb == 0 ? launch_fir_active_grp(f, c->P, c->stream) : launch_fir2(f, c->P, fg, fb, c->stream);
       if (e == hipErrorInvalidValue) e = launch_fir_active(f, c->P, c->stream);  // very wide shift ranges
       HIPCHK(c, e);
     }
@@ -1040,6 +1043,18 @@ int soap_integrate(soap_ctx *c) {
   c->have_lower = c->have_ccf = c->have_fit = false;
   c->alg_bytes = -1;  // computed lazily
   return SOAP_OK;
+}
+
+int soap_set_fir_shape(soap_ctx *c, int32_t phases_per_block, int32_t pixels_per_thread) {
+  if (!c) return SOAP_ERR_ARG;
+  const int ok[][2] = {{-1, -1}, {4, 0}, {1, 3}, {1, 5}, {2, 3}, {2, 1}, {4, 1}};
+  for (auto &s : ok)
+    if (s[0] == phases_per_block && s[1] == pixels_per_thread) {
+      c->fir_g = phases_per_block;
+      c->fir_b = pixels_per_thread;
+      return SOAP_OK;
+    }
+  return fail(c, SOAP_ERR_ARG, "set_fir_shape: unsupported shape (%d, %d)", phases_per_block, pixels_per_thread);
 }
 
 int soap_get_integrate_stats(soap_ctx *c, double *bytes_min, double *bytes_phase_sum, int64_t *rows_sum) {
@@ -1789,6 +1804,7 @@ struct soap_batch {
   BatchStageBuf st[BS_COUNT];
   int taps_cells = 4096;  // disk cells per taps block (cells-per-block shape)
   int fir_g = 4;          // phases per FIR block
+  int fir_b = 0;          // pixels per FIR thread (0: one-pixel form)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev[BS_COUNT + 1] = {};
@@ -1929,11 +1945,11 @@ int batch_build(soap_batch *b) {
     f.bins = (const int32_t *)c->bins.p;
     f.nbins = (const int32_t *)c->nbins.p;
     f.out = c->d_raw;
-    int fir_tiles = (int)((c->n + fir_tile() - 1) / fir_tile());
+    int fir_tiles = b->fir_b > 0 ? fir2_tiles(c->n, b->fir_b) : (int)((c->n + fir_tile() - 1) / fir_tile());
     int groups = (P + b->fir_g - 1) / b->fir_g;
     fir[i] = FirJob{f, P, fir_tiles, groups};
     w[BS_FIR][i] = (int64_t)(fir_tiles + 7) / 8 * 8 * groups;
-    lds_fir = std::max(lds_fir, fir_lds(c->nd));
+    lds_fir = std::max(lds_fir, b->fir_b > 0 ? fir2_lds(c->nd, b->fir_g, b->fir_b) : fir_lds(c->nd));
     // resolution (log grid)
     ConvArgs cv{};
     cv.n = c->n;
@@ -1969,7 +1985,10 @@ int batch_build(soap_batch *b) {
                    (double *)c->f_depth.p, (double *)c->f_bis.p, (int32_t *)c->f_iter.p};
     w[BS_FIT][i] = n_spec;
   }
-  if (lds_fir > 60000 || lds_conv > 65536 || lds_taps > 65536 || ccf_threads > 1024 || red_threads > 1024)
+  for (auto &v : w)
+    for (int64_t x : v)
+      if (x > 0x7fffffffL) return bfail(b, SOAP_ERR_LIMIT, "batch: an instance has more than 2^31 blocks in a stage");
+  if (lds_fir > 65536 || lds_conv > 65536 || lds_taps > 65536 || ccf_threads > 1024 || red_threads > 1024)
     return bfail(b, SOAP_ERR_LIMIT, "batch: an instance exceeds a per-block limit (lds fir %zu conv %zu taps %zu)",
                  lds_fir, lds_conv, lds_taps);
   int rc;
@@ -1999,7 +2018,7 @@ int batch_enqueue(soap_batch *b, bool events) {
     BatchStageBuf &sb = b->st[s];
     if (events) BHIP(b, hipEventRecord(b->ev[s], b->stream));
     hipError_t e = launch_batch(s, sb.jobs, sb.prefix, (int)b->inst.size(), sb.total, sb.grid, sb.threads, sb.lds,
-                                b->fir_g, b->stream);
+                                b->fir_g, b->fir_b, b->stream);
     if (e != hipSuccess) return bfail(b, SOAP_ERR_HIP, "batch stage %s: %s", kBatchStageNames[s], hipGetErrorString(e));
   }
   if (events) BHIP(b, hipEventRecord(b->ev[BS_COUNT], b->stream));
@@ -2129,11 +2148,12 @@ int soap_batch_stage_ms(soap_batch *b, double *ms, int32_t n_stage) {
 const char *soap_batch_stage_name(int32_t s) { return (s >= 0 && s < BS_COUNT) ? kBatchStageNames[s] : ""; }
 int32_t soap_batch_num_stages(void) { return BS_COUNT; }
 
-int soap_batch_get_config(soap_batch *b, int32_t *taps_cells, int32_t *fir_g, int64_t *grid, int64_t *work,
-                          int32_t n_stage) {
+int soap_batch_get_config(soap_batch *b, int32_t *taps_cells, int32_t *fir_g, int32_t *fir_b, int64_t *grid,
+                          int64_t *work, int32_t n_stage) {
   if (!b || n_stage < BS_COUNT) return SOAP_ERR_ARG;
   if (taps_cells) *taps_cells = b->taps_cells;
   if (fir_g) *fir_g = b->fir_g;
+  if (fir_b) *fir_b = b->fir_b;
   for (int s = 0; s < BS_COUNT; s++) {
     if (grid) grid[s] = b->st[s].grid;
     if (work) work[s] = b->st[s].total;
@@ -2141,16 +2161,27 @@ int soap_batch_get_config(soap_batch *b, int32_t *taps_cells, int32_t *fir_g, in
   return SOAP_OK;
 }
 
-int soap_batch_set_config(soap_batch *b, int32_t taps_cells, int32_t fir_g, const int64_t *grid, int32_t n_stage) {
+static bool batch_fir_shape_ok(int g, int bb) {
+  const int ok[][2] = {{2, 0}, {4, 0}, {6, 0}, {1, 3}, {1, 5}, {2, 3}};
+  for (auto &s : ok)
+    if (s[0] == g && s[1] == bb) return true;
+  return false;
+}
+
+int soap_batch_set_config(soap_batch *b, int32_t taps_cells, int32_t fir_g, int32_t fir_b, const int64_t *grid,
+                          int32_t n_stage) {
   if (!b || n_stage < BS_COUNT || !grid) return SOAP_ERR_ARG;
   if (taps_cells < 64 || taps_cells > (1 << 20)) return bfail(b, SOAP_ERR_ARG, "set_config: taps_cells out of range");
-  if (fir_g != 2 && fir_g != 4 && fir_g != 6) return bfail(b, SOAP_ERR_ARG, "set_config: fir_g must be 2, 4 or 6");
+  if (!batch_fir_shape_ok(fir_g, fir_b))
+    return bfail(b, SOAP_ERR_ARG, "set_config: FIR shape (%d, %d) not one of (2,0) (4,0) (6,0) (1,3) (1,5) (2,3)",
+                 fir_g, fir_b);
   for (int s = 0; s < BS_COUNT; s++)
     if (grid[s] < 0 || grid[s] % 8) return bfail(b, SOAP_ERR_ARG, "set_config: grid sizes are multiples of 8 (0 = all)");
   BHIP(b, hipSetDevice(b->device));
   BHIP(b, hipStreamSynchronize(b->stream));
   b->taps_cells = taps_cells;
   b->fir_g = fir_g;
+  b->fir_b = fir_b;
   for (int s = 0; s < BS_COUNT; s++) b->st[s].grid = grid[s];
   return batch_build(b);
 }
@@ -2187,15 +2218,18 @@ int soap_batch_tune(soap_batch *b, int32_t reps) {
     if (t < best) best = t, best_cells = cells;
   }
   b->taps_cells = best_cells;
-  const int g_c[] = {2, 4, 6};
-  int best_g = b->fir_g;
+  // FIR shape: phases per block and pixels per thread (the lambda tile)
+  const int g_c[][2] = {{2, 0}, {4, 0}, {6, 0}, {1, 3}, {1, 5}, {2, 3}};
+  int best_g = b->fir_g, best_b = b->fir_b;
   best = HUGE_VAL;
-  for (int g : g_c) {
-    b->fir_g = g;
+  for (auto &gb : g_c) {
+    b->fir_g = gb[0];
+    b->fir_b = gb[1];
     if ((rc = batch_build(b)) || (rc = score(BS_FIR, BS_FIR, &t))) return rc;
-    if (t < best) best = t, best_g = g;
+    if (t < best) best = t, best_g = gb[0], best_b = gb[1];
   }
   b->fir_g = best_g;
+  b->fir_b = best_b;
   if ((rc = batch_build(b))) return rc;
   const int64_t mult[] = {0, 1, 2, 4, 8, 16};
   for (int s = 0; s < BS_COUNT; s++) {

@@ -237,6 +237,11 @@ hipError_t launch_fir_active(const FirArgs &a, int n_job, hipStream_t s);
 hipError_t launch_fir_quiet_lds(const FirArgs &a, hipStream_t s);
 // phase-group form (bin outer, phases inner, LDS-staged rows and taps)
 hipError_t launch_fir_active_grp(const FirArgs &a, int n_job, hipStream_t s);
+// register-blocked form: G phases per block, B pixels per thread (G, B in
+// {(1,3), (1,5), (2,3), (2,1), (4,1)})
+hipError_t launch_fir2(const FirArgs &a, int n_job, int G, int B, hipStream_t s);
+size_t fir2_lds(int nd, int G, int B);
+int fir2_tiles(int64_t n, int B);
 // solar mode: taps [job][n_mu][nd][K+3] = H1s, H1f, L_0..L_{K-1}, H1a
 hipError_t launch_fir_solar(const FirArgs &a, const SolarFir &sf, int n_job, hipStream_t s);
 hipError_t launch_phase_maps(int grid, int t_star, int t_diff_spot, int t_diff_fac, const RegionDev *geo,
@@ -302,9 +307,10 @@ hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const doub
 hipError_t launch_ccf_sum(const double *part, int nc, int n_spec, int n_v, double *ccf, hipStream_t s);
 // batched instances: stage st over jobs[n_inst] (device), prefix[n_inst + 1]
 // (device, prefix[n_inst] = total); grid = persistent block count (0: one block
-// per work item), threads / lds as the stage needs; fir_g = phases per FIR block
+// per work item), threads / lds as the stage needs; FIR shape: fir_g phases per
+// block, fir_b pixels per thread (0: the one-pixel-per-thread form)
 hipError_t launch_batch(int st, const void *jobs, const int64_t *prefix, int n_inst, int64_t total, int64_t grid,
-                        int threads, size_t lds, int fir_g, hipStream_t s);
+                        int threads, size_t lds, int fir_g, int fir_b, hipStream_t s);
 int batch_fill_chunk();
 int norm_px_per_block();
 int ccf_spec_chunk();

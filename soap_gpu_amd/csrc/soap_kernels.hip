@@ -200,7 +200,7 @@ hipError_t launch_taps_quiet(const TapAccArgs &a, hipStream_t s) {
 // Chebyshev nodes T_q (the host checks the interpolation error), so facula cells
 // add limb to set 1 and limb * l_q(T_cell) to set 2+q; the FIR epilogue then
 // weights set 2+q by B(lambda, T_q)/B(lambda, T_norm).
-__device__ __forceinline__ void taps_active_body(const TapAccArgs &a, long bx_, long by_) {
+__device__ __forceinline__ void taps_active_body(const TapAccArgs &a, int bx_, int by_) {
   extern __shared__ unsigned long long lds_u64[];
   __shared__ int scratch[16];
   int job = by_;
@@ -274,7 +274,7 @@ hipError_t launch_taps_active(const TapAccArgs &a, int n_job, int max_col_blocks
 // Active taps get one more output set than they accumulate: the last one is
 // set 0 + set 1 (all active cells, weight limb), used on the SED quiet rows.
 // Flags: bit 0 the spot set is nonzero, bit 1 a facula set is nonzero.
-__device__ __forceinline__ void taps_finalize_body(const unsigned long long *acc, int nsets_in, int nsets_out, int n_mu, int nd, double *taps, int32_t *krange, int32_t *kflags, int32_t *bins, int32_t *nbins, long bx_, long by_) {
+__device__ __forceinline__ void taps_finalize_body(const unsigned long long *acc, int nsets_in, int nsets_out, int n_mu, int nd, double *taps, int32_t *krange, int32_t *kflags, int32_t *bins, int32_t *nbins, int bx_, int by_) {
   int job = bx_;
   __shared__ int s_lo[256], s_hi[256], s_fl[256];
   for (int k = threadIdx.x; k < n_mu; k += blockDim.x) {
@@ -429,21 +429,21 @@ constexpr int kFirTile = kTPB;
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 template <int G>
-__device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job, int n_tiles, int n_groups, long bx_) {
+__device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job, int n_tiles, int n_groups, int bx_) {
   extern __shared__ double rows[];  // [4][row_len]: quiet, quiet SED, spot, facula rows of one bin
   __shared__ int s_fl[G][64];
   __shared__ int s_bins[64], s_ufl[64], s_ulo[64], s_uhi[64];
   __shared__ int s_nb;
-  long b = bx_;
-  long xcd = b % 8, slot = b / 8;
-  long tile = (slot / n_groups) * 8 + xcd;
-  int grp = (int)(slot % n_groups);
+  // 32-bit block arithmetic (block counts are < 2^31): keeps the division short
+  const int xcd = bx_ & 7, slot = bx_ >> 3;
+  const int tile = (slot / n_groups) * 8 + xcd;
+  const int grp = slot % n_groups;
   if (tile >= n_tiles) return;  // uniform over the block
   int p0 = grp * G;
   int ng = min(G, n_job - p0);
   int nd = a.nd, n_mu = a.n_mu;
   int row_len = kFirTile + nd - 1;
-  long j0 = tile * kFirTile;
+  long j0 = (long)tile * kFirTile;
   long src0 = j0 - a.dmin - nd + 1;
   // group metadata: per-phase flags, union flags and union tap range per bin
   for (int e = threadIdx.x; e < G * 64; e += blockDim.x) {
@@ -562,6 +562,223 @@ hipError_t launch_fir_active_grp(const FirArgs &a, int n_job, hipStream_t s) {
   if (lds > 60000) return hipErrorInvalidValue;
   k_fir_active_grp<<<(unsigned)blocks, kTPB, lds, s>>>(a, n_job, n_tiles, n_groups);
   return hipGetLastError();
+}
+
+// Register-blocked form of the active FIR.  Per block: a lambda tile of
+// kTPB * B pixels and a group of G phases; each thread owns B consecutive
+// output pixels of every phase of the group.  For each mu bin used by the
+// group, the group's taps (reversed, per set, zero outside each phase's range)
+// and then each needed seed row segment are staged in LDS; the row is read
+// into registers in windows of kF2CH + B - 1 values and every tap (an LDS
+// broadcast) feeds B FMAs per phase.  Six accumulators per (phase, pixel):
+//   0: F0 = H1a * Sq_SED      1: F1 = H1s * Sq      2: F2 = H2f * Sq
+//   3: F4 = H1s * S_spot      4: F6 = H2f * S_fac   5: Bc = H1f * (Sq - S_fac)
+// flux = F0 - cs F1 - cf F2, bconv = F1 + Bc - F4, tot = F0 - cs F4 - cf F6
+// (cs, cf the spot / facula SED continua; the products of the single-phase
+// form in the same combination).  B is odd: a thread stride of B doubles is
+// conflict-free for 64-bit LDS reads.  Blocks are XCD-ordered as in the other
+// forms (all phase groups of a tile on XCD b % 8).
+constexpr int kF2CH = 8;
+
+template <int G, int B, int R>
+__device__ __forceinline__ void fir2_row(const double *xs, const double *tp, int Lc, int lcs, double (&F)[G][6][B]) {
+  const double *base = xs + threadIdx.x * B;
+  for (int k0 = 0; k0 < Lc; k0 += kF2CH) {
+    double win[kF2CH + B - 1];
+#pragma unroll
+    for (int e = 0; e < kF2CH + B - 1; e++) win[e] = base[k0 + e];
+#pragma unroll
+    for (int k = 0; k < kF2CH; k++) {
+#pragma unroll
+      for (int g = 0; g < G; g++) {
+        const double *t = tp + (g * 4) * lcs + k0 + k;
+        if (R == 0) {  // quiet SED row: H1a
+          const double c = t[3 * lcs];
+#pragma unroll
+          for (int b = 0; b < B; b++) F[g][0][b] = fma(c, win[b + k], F[g][0][b]);
+        } else if (R == 1) {  // quiet row: H1s, H1f, H2f
+          const double c0 = t[0], c1 = t[lcs], c2 = t[2 * lcs];
+#pragma unroll
+          for (int b = 0; b < B; b++) {
+            F[g][1][b] = fma(c0, win[b + k], F[g][1][b]);
+            F[g][5][b] = fma(c1, win[b + k], F[g][5][b]);
+            F[g][2][b] = fma(c2, win[b + k], F[g][2][b]);
+          }
+        } else if (R == 2) {  // spot row: H1s
+          const double c0 = t[0];
+#pragma unroll
+          for (int b = 0; b < B; b++) F[g][3][b] = fma(c0, win[b + k], F[g][3][b]);
+        } else {  // facula row: H1f (subtracted), H2f
+          const double c1 = t[lcs], c2 = t[2 * lcs];
+#pragma unroll
+          for (int b = 0; b < B; b++) {
+            F[g][5][b] = fma(-c1, win[b + k], F[g][5][b]);
+            F[g][4][b] = fma(c2, win[b + k], F[g][4][b]);
+          }
+        }
+      }
+    }
+  }
+}
+
+template <int G, int B>
+__device__ __forceinline__ void fir2_body(const FirArgs &a, int n_job, int n_tiles, int n_groups, int bx_) {
+  constexpr int T = kTPB * B;
+  extern __shared__ double f2s[];
+  __shared__ int s_fl[G][64];
+  __shared__ int s_bins[64], s_ufl[64], s_ulo[64], s_uhi[64];
+  __shared__ int s_nb;
+  const int xcd = bx_ & 7, slot = bx_ >> 3;
+  const int tile = (slot / n_groups) * 8 + xcd;
+  const int grp = slot % n_groups;
+  if (tile >= n_tiles) return;  // uniform over the block
+  const int p0 = grp * G;
+  const int ng = min(G, n_job - p0);
+  const int nd = a.nd, n_mu = a.n_mu;
+  const int lcs = (nd + kF2CH - 1) / kF2CH * kF2CH + kF2CH;  // LDS stride of one tap set
+  double *tp = f2s;                    // [G][4][lcs]
+  double *xs = f2s + (long)G * 4 * lcs;  // [T + lcs]
+  const long j0 = (long)tile * T;
+  const long src0 = j0 - a.dmin - (nd - 1);
+  // group metadata: per-phase flags, union flags and union tap range per bin
+  for (int e = threadIdx.x; e < G * 64; e += blockDim.x) {
+    int g = e / 64, k = e % 64;
+    int fl = 0;
+    if (g < ng && k < n_mu) {
+      long jk = (long)(p0 + g) * n_mu + k;
+      fl = a.kflags[jk];
+      if (a.krange[jk * 2] > a.krange[jk * 2 + 1]) fl = 0;
+    }
+    s_fl[g][k] = fl;
+  }
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    int k = threadIdx.x;
+    int ufl = 0, ulo = nd, uhi = -1;
+    for (int g = 0; g < ng; g++)
+      if (s_fl[g][k]) {
+        long jk = (long)(p0 + g) * n_mu + k;
+        ufl |= s_fl[g][k] | 4;
+        ulo = min(ulo, nd - 1 - a.krange[jk * 2 + 1]);
+        uhi = max(uhi, nd - 1 - a.krange[jk * 2]);
+      }
+    s_ufl[k] = ufl;
+    s_ulo[k] = ulo;
+    s_uhi[k] = uhi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int c = 0;
+    for (int k = 0; k < n_mu; k++)
+      if (s_ufl[k]) s_bins[c++] = k;
+    s_nb = c;
+  }
+  __syncthreads();
+  double F[G][6][B];
+#pragma unroll
+  for (int g = 0; g < G; g++)
+#pragma unroll
+    for (int f = 0; f < 6; f++)
+#pragma unroll
+      for (int b = 0; b < B; b++) F[g][f][b] = 0.0;
+  const int nb = uni(s_nb);
+  for (int kk = 0; kk < nb; kk++) {
+    const int k = uni(s_bins[kk]);
+    const int ufl = uni(s_ufl[k]), ulo = uni(s_ulo[k]), uhi = uni(s_uhi[k]);
+    const int Lc = (uhi - ulo + kF2CH) / kF2CH * kF2CH;  // taps used, padded with zeros
+    __syncthreads();  // the previous bin's LDS reads are done
+    // taps of the group, reversed (u = nd - 1 - d), zero outside each phase's range
+    for (int e = threadIdx.x; e < G * 4 * Lc; e += blockDim.x) {
+      int g = e / (4 * Lc), r = e % (4 * Lc), st = r / Lc, u = ulo + r % Lc;
+      double v = 0.0;
+      if (g < ng && s_fl[g][k]) {
+        long jk = (long)(p0 + g) * n_mu + k;
+        int di = nd - 1 - u;
+        if (di >= a.krange[jk * 2] && di <= a.krange[jk * 2 + 1]) v = a.taps[(jk * nd + di) * 4 + st];
+      }
+      tp[(g * 4 + st) * lcs + (r % Lc)] = v;
+    }
+    const int span = T + Lc - 1;
+    const long rs0 = src0 + ulo;
+#pragma unroll 1
+    for (int r = 0; r < 4; r++) {
+      if ((r == 2 && !(ufl & 1)) || (r == 3 && !(ufl & 2))) continue;  // no spot / facula cells in this bin
+      const double *src = (r == 0 ? a.qs : r == 1 ? a.q : r == 2 ? a.sp : a.fa) + (long)k * a.n;
+      if (r > 0) __syncthreads();  // the previous row's reads are done
+      for (int x = threadIdx.x; x < span; x += blockDim.x) {
+        long i = rs0 + x;
+        xs[x] = (i >= 0 && i < a.n) ? src[i] : 0.0;
+      }
+      __syncthreads();
+      if (r == 0) fir2_row<G, B, 0>(xs, tp, Lc, lcs, F);
+      else if (r == 1) fir2_row<G, B, 1>(xs, tp, Lc, lcs, F);
+      else if (r == 2) fir2_row<G, B, 2>(xs, tp, Lc, lcs, F);
+      else fir2_row<G, B, 3>(xs, tp, Lc, lcs, F);
+    }
+  }
+  // epilogue: the three spectra of each phase through LDS for coalesced stores
+  double cs[B], cf[B];
+#pragma unroll
+  for (int b = 0; b < B; b++) {
+    long j = j0 + threadIdx.x * B + b;
+    cs[b] = (a.sps && j < a.n) ? a.sps[j] : 0.0;
+    cf[b] = (a.fas && j < a.n) ? a.fas[j] : 0.0;
+  }
+#pragma unroll
+  for (int g = 0; g < G; g++) {
+    if (g >= ng) break;
+    double *o = a.out + (long)(p0 + g) * 3 * a.n;
+#pragma unroll
+    for (int t = 0; t < 3; t++) {
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < B; b++) {
+        double v = t == 0 ? F[g][0][b] - cs[b] * F[g][1][b] - cf[b] * F[g][2][b]
+                 : t == 1 ? F[g][1][b] + F[g][5][b] - F[g][3][b]
+                          : F[g][0][b] - cs[b] * F[g][3][b] - cf[b] * F[g][4][b];
+        xs[threadIdx.x * B + b] = v;
+      }
+      __syncthreads();
+      for (int e = threadIdx.x; e < T; e += blockDim.x) {
+        long j = j0 + e;
+        if (j < a.n) o[(long)t * a.n + j] = xs[e];
+      }
+    }
+  }
+}
+
+template <int G, int B>
+__global__ __launch_bounds__(kTPB) void k_fir2(FirArgs a, int n_job, int n_tiles, int n_groups) {
+  fir2_body<G, B>(a, n_job, n_tiles, n_groups, blockIdx.x);
+}
+
+size_t fir2_lds(int nd, int G, int B) {
+  int lcs = (nd + kF2CH - 1) / kF2CH * kF2CH + kF2CH;
+  return ((size_t)G * 4 * lcs + (size_t)kTPB * B + lcs) * sizeof(double);
+}
+
+int fir2_tiles(int64_t n, int B) { return (int)((n + (int64_t)kTPB * B - 1) / ((int64_t)kTPB * B)); }
+
+hipError_t launch_fir2(const FirArgs &a, int n_job, int G, int B, hipStream_t s) {
+  if (n_job == 0) return hipSuccess;
+  if (a.n_mu > 64) return hipErrorInvalidValue;
+  int n_tiles = fir2_tiles(a.n, B);
+  int n_groups = (n_job + G - 1) / G;
+  long blocks = ((long)n_tiles + 7) / 8 * 8 * n_groups;
+  size_t lds = fir2_lds(a.nd, G, B);
+  if (lds > 65536 || blocks > 0x7fffffffL) return hipErrorInvalidValue;
+#define SOAP_FIR2(GG, BB)                                                                                  \
+  if (G == GG && B == BB) {                                                                                \
+    k_fir2<GG, BB><<<(unsigned)blocks, kTPB, lds, s>>>(a, n_job, n_tiles, n_groups);                       \
+    return hipGetLastError();                                                                              \
+  }
+  SOAP_FIR2(1, 3)
+  SOAP_FIR2(1, 5)
+  SOAP_FIR2(2, 3)
+  SOAP_FIR2(2, 1)
+  SOAP_FIR2(4, 1)
+#undef SOAP_FIR2
+  return hipErrorInvalidValue;
 }
 
 // Solar-mode active deltas, one phase per block (taps [job][n_mu][nd][K+3]:
@@ -828,7 +1045,7 @@ __device__ __forceinline__ void block_minmax_store(double amin, double omax, dou
 // written to the frame slots of the lower-spectrum layout [flux, bconv, tot] =
 // [SED, normalised, SED] (run_all_SOAP.py:359-362), with the partial min(Q) and
 // max(o != 0) (seed -60000) of each tile per slot.
-__device__ __forceinline__ void conv_quiet_body(const ConvArgs &c, const double *quiet, double *frame, double *part, long bx_, long by_) {
+__device__ __forceinline__ void conv_quiet_body(const ConvArgs &c, const double *quiet, double *frame, double *part, int bx_, int by_) {
   extern __shared__ double xs[];
   int q = by_;
   long i0 = (long)bx_ * kConvTile;
@@ -878,12 +1095,11 @@ __global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *q
 // sel = 0 for bconv, 1 otherwise (SOAP_resolution.cu:414-416).  Blocks are
 // numbered so that all phases of one lambda tile run back to back on one XCD
 // (block b -> XCD b % 8): the tile's quiet rows stay in that XCD's L2.
-__device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double *quiet, const double *raw, double *o_out, double *part, int n_phase, long bx_, long by_) {
+__device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double *quiet, const double *raw, double *o_out, double *part, int n_phase, int bx_, int by_) {
   extern __shared__ double xs[];
-  long b = bx_;
-  long slot = b / 8;
-  int tile = (int)((slot / n_phase) * 8 + b % 8);
-  int p = (int)(slot % n_phase);
+  const int slot = bx_ >> 3;
+  const int tile = (slot / n_phase) * 8 + (bx_ & 7);
+  const int p = slot % n_phase;
   if (tile >= c.n_tiles) return;  // uniform over the block
   long i0 = (long)tile * kConvTile;
   int span = kConvTile + c.ntp - 1;
@@ -956,7 +1172,7 @@ int conv_tile() { return kConvTile; }
 int conv_chunk() { return kConvCH; }
 
 // min / max of each spectrum over its tile partials (exact: order independent)
-__device__ __forceinline__ void minmax_body(const double *part, int n_tiles, double *mm, long bx_, long by_) {
+__device__ __forceinline__ void minmax_body(const double *part, int n_tiles, double *mm, int bx_, int by_) {
   __shared__ double s_min[kTPB], s_max[kTPB];
   int sp = bx_;
   double a = HUGE_VAL, b = -HUGE_VAL;
@@ -989,7 +1205,7 @@ __global__ __launch_bounds__(kTPB) void k_minmax_reduce(const double *part, int 
 // Spectrum sp = blockIdx.y at o + sp*n; a block covers kNormPer*kTPB pixels with
 // coalesced loads issued before the stores.
 constexpr int kNormPer = 8;
-__device__ __forceinline__ void norm_body(long n, const double *mm, double *o, long bx_, long by_) {
+__device__ __forceinline__ void norm_body(long n, const double *mm, double *o, int bx_, int by_) {
   long sp = by_;
   double mn = mm[sp * 2], mx = mm[sp * 2 + 1];
   double *x = o + sp * n;
@@ -1110,7 +1326,7 @@ hipError_t launch_mask_overwrite(const int32_t *sorted, int n_lines, int n_v, in
 constexpr int kCcfLines = 32;
 constexpr int kCcfSpec = 8;
 
-__device__ __forceinline__ void ccf_body(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v, int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv, const double *w_sorted, double *part, long bx_, long by_) {
+__device__ __forceinline__ void ccf_body(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v, int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv, const double *w_sorted, double *part, int bx_, int by_) {
   int chunk = bx_;
   int s0 = by_ * kCcfSpec;
   int ns = min(kCcfSpec, n_spec - s0);
@@ -1148,7 +1364,7 @@ __global__ void k_ccf(const double *spec, int64_t spec_stride, int64_t trim, int
   ccf_body(spec, spec_stride, trim, n_spec, n_lines, n_v, n_t, m_start, m_pk, m_wv, w_sorted, part, blockIdx.x, blockIdx.y);
 }
 
-__device__ __forceinline__ void ccf_reduce_body(const double *part, int n_chunks, int n_spec, int n_v, const double *appod, double *ccf, long bx_, long by_) {
+__device__ __forceinline__ void ccf_reduce_body(const double *part, int n_chunks, int n_spec, int n_v, const double *appod, double *ccf, int bx_, int by_) {
   __shared__ double s_red[1024];
   int sp = bx_;
   double v = -HUGE_VAL;
@@ -1308,7 +1524,7 @@ __device__ int solve4(const double *A, const double *D, double mu, const double 
 // Levenberg-Marquardt restatement of GSL's trust-region solver as the oracle
 // (More scaling, Nielsen damping, xtol = gtol = 1e-8), then the bisector.
 __device__ __forceinline__ void fit_body(const double *v, int n, const double *Y, double *par, double *model,
-                                         double *depth_out, double *bis_out, int32_t *iters, long bx_) {
+                                         double *depth_out, double *bis_out, int32_t *iters, int bx_) {
   __shared__ double s_px[kMaxV], s_y[kMaxV], s_nc[kMaxV], s_p0[kMaxV], s_p1[kMaxV], s_p2[kMaxV];
   __shared__ double s_bis[91];
   __shared__ double s_init[4];
@@ -1508,82 +1724,115 @@ __device__ __forceinline__ int batch_find(const int64_t *__restrict__ prefix, in
   return lo;
 }
 
-template <class Job, class F>
+template <bool PERSIST, class Job, class F>
 __device__ __forceinline__ void batch_loop(const Job *__restrict__ jobs, const int64_t *__restrict__ prefix, int n_inst,
                                            int64_t total, F body) {
+  if (!PERSIST) {  // one work item per block: no loop state live across items
+    int64_t w = blockIdx.x;
+    int i = batch_find(prefix, n_inst, w);
+    body(jobs[i], (int)(w - prefix[i]));  // per-instance counts are < 2^31
+    return;
+  }
   for (int64_t w = blockIdx.x; w < total; w += gridDim.x) {
     int i = batch_find(prefix, n_inst, w);
-    body(jobs[i], (long)(w - prefix[i]));
+    body(jobs[i], (int)(w - prefix[i]));
     __syncthreads();  // the next item reuses the block's LDS
   }
 }
 
 #define SOAP_BATCH_ARGS(JOB) const JOB *__restrict__ jobs, const int64_t *__restrict__ prefix, int n_inst, int64_t total
 
+template <bool P>
 __global__ __launch_bounds__(kTPB) void kb_fill(SOAP_BATCH_ARGS(FillJob)) {
-  batch_loop(jobs, prefix, n_inst, total, [&](const FillJob &j, long local) {
+  batch_loop<P>(jobs, prefix, n_inst, total, [&](const FillJob &j, int local) {
     int64_t e0 = (int64_t)local * kBatchFillChunk;
     int64_t e1 = min(j.count, e0 + kBatchFillChunk);
     for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) j.p[e] = 0ull;
   });
 }
 
+template <bool P>
 __global__ __launch_bounds__(kTPB) void kb_taps(SOAP_BATCH_ARGS(TapsJob)) {
-  batch_loop(jobs, prefix, n_inst, total,
-             [&](const TapsJob &j, long local) { taps_active_body(j.a, local % j.mcb, local / j.mcb); });
+  batch_loop<P>(jobs, prefix, n_inst, total,
+                [&](const TapsJob &j, int local) { taps_active_body(j.a, local % j.mcb, local / j.mcb); });
 }
 
+template <bool P>
 __global__ __launch_bounds__(64) void kb_finalize(SOAP_BATCH_ARGS(FinalizeJob)) {
-  batch_loop(jobs, prefix, n_inst, total, [&](const FinalizeJob &j, long local) {
+  batch_loop<P>(jobs, prefix, n_inst, total, [&](const FinalizeJob &j, int local) {
     taps_finalize_body(j.acc, j.nsets_in, j.nsets_out, j.n_mu, j.nd, j.taps, j.krange, j.kflags, j.bins, j.nbins, local,
                        0);
   });
 }
 
-template <int G>
-__global__ __launch_bounds__(kTPB) void kb_fir(SOAP_BATCH_ARGS(FirJob)) {
-  batch_loop(jobs, prefix, n_inst, total, [&](const FirJob &j, long local) {
+// occupancy floor of the FIR blocks: the single-instance kernel's register budget
+// (more waves per SIMD would spill the accumulators of the inner loop)
+template <bool P, int G>
+struct FirWaves {
+  static constexpr int v = G == 2 ? 8 : G == 4 ? (P ? 5 : 6) : 4;
+};
+
+template <bool P, int G>
+__global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(FirWaves<P, G>::v, 8))) void kb_fir(
+    SOAP_BATCH_ARGS(FirJob)) {
+  batch_loop<P>(jobs, prefix, n_inst, total, [&](const FirJob &j, int local) {
     fir_active_grp_body<G>(j.a, j.n_job, j.n_tiles, j.n_groups, local);
   });
 }
 
-__global__ __launch_bounds__(kTPB) void kb_conv_quiet(SOAP_BATCH_ARGS(ConvQJob)) {
-  batch_loop(jobs, prefix, n_inst, total, [&](const ConvQJob &j, long local) {
+template <bool P, int G, int B>
+__global__ __launch_bounds__(kTPB) void kb_fir2(SOAP_BATCH_ARGS(FirJob)) {
+  batch_loop<P>(jobs, prefix, n_inst, total, [&](const FirJob &j, int local) {
+    fir2_body<G, B>(j.a, j.n_job, j.n_tiles, j.n_groups, local);
+  });
+}
+
+template <bool P>
+__global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(6, 8))) void kb_conv_quiet(
+    SOAP_BATCH_ARGS(ConvQJob)) {
+  batch_loop<P>(jobs, prefix, n_inst, total, [&](const ConvQJob &j, int local) {
     conv_quiet_body(j.c, j.quiet, j.frame, j.part, local % j.c.n_tiles, local / j.c.n_tiles);
   });
 }
 
-__global__ __launch_bounds__(kTPB) void kb_conv(SOAP_BATCH_ARGS(ConvAJob)) {
-  batch_loop(jobs, prefix, n_inst, total, [&](const ConvAJob &j, long local) {
+template <bool P>
+__global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(6, 8))) void kb_conv(SOAP_BATCH_ARGS(ConvAJob)) {
+  batch_loop<P>(jobs, prefix, n_inst, total, [&](const ConvAJob &j, int local) {
     conv_active_body(j.c, j.quiet, j.raw, j.out, j.part, j.n_phase, local, 0);
   });
 }
 
+template <bool P>
 __global__ __launch_bounds__(kTPB) void kb_minmax(SOAP_BATCH_ARGS(MinmaxJob)) {
-  batch_loop(jobs, prefix, n_inst, total,
-             [&](const MinmaxJob &j, long local) { minmax_body(j.part, j.n_tiles, j.mm, local, 0); });
+  batch_loop<P>(jobs, prefix, n_inst, total,
+                [&](const MinmaxJob &j, int local) { minmax_body(j.part, j.n_tiles, j.mm, local, 0); });
 }
 
+template <bool P>
 __global__ __launch_bounds__(kTPB) void kb_norm(SOAP_BATCH_ARGS(NormJob)) {
-  batch_loop(jobs, prefix, n_inst, total,
-             [&](const NormJob &j, long local) { norm_body(j.n, j.mm, j.o, local % j.bx, local / j.bx); });
+  batch_loop<P>(jobs, prefix, n_inst, total, [&](const NormJob &j, int local) {
+    norm_body(j.n, j.mm, j.o, local % (int)j.bx, local / (int)j.bx);
+  });
 }
 
+template <bool P>
 __global__ __launch_bounds__(1024) void kb_ccf(SOAP_BATCH_ARGS(CcfJob)) {
-  batch_loop(jobs, prefix, n_inst, total, [&](const CcfJob &j, long local) {
+  batch_loop<P>(jobs, prefix, n_inst, total, [&](const CcfJob &j, int local) {
     ccf_body(j.spec, j.stride, j.trim, j.n_spec, j.n_lines, j.n_v, j.n_t, j.m_start, j.m_pk, j.m_wv, j.w_sorted, j.part,
              local % j.nc, local / j.nc);
   });
 }
 
+template <bool P>
 __global__ __launch_bounds__(1024) void kb_ccf_reduce(SOAP_BATCH_ARGS(CcfRedJob)) {
-  batch_loop(jobs, prefix, n_inst, total, [&](const CcfRedJob &j, long local) {
+  batch_loop<P>(jobs, prefix, n_inst, total, [&](const CcfRedJob &j, int local) {
     ccf_reduce_body(j.part, j.nc, j.n_spec, j.n_v, j.appod, j.ccf, local, 0);
   });
 }
 
+template <bool P>
 __global__ __launch_bounds__(64) void kb_fit(SOAP_BATCH_ARGS(FitJob)) {
-  batch_loop(jobs, prefix, n_inst, total, [&](const FitJob &j, long local) {
+  batch_loop<P>(jobs, prefix, n_inst, total, [&](const FitJob &j, int local) {
     fit_body(j.v, j.n_v, j.y, j.par, j.model, j.depth, j.bis, j.iters, local);
   });
 }
@@ -1597,17 +1846,19 @@ size_t conv_lds_bytes(int ntp) { return (size_t)(kConvTile + 2 * ntp + 1) * size
 size_t taps_lds_bytes(const TapAccArgs &a) { return (size_t)a.nsets * a.n_mu * a.wd * 2 * sizeof(unsigned long long); }
 
 hipError_t launch_batch(int st, const void *jobs, const int64_t *prefix, int n_inst, int64_t total, int64_t grid,
-                        int threads, size_t lds, int fir_g, hipStream_t s) {
+                        int threads, size_t lds, int fir_g, int fir_b, hipStream_t s) {
   if (total == 0) return hipSuccess;
   if (n_inst < 1 || total < 0 || grid < 0 || lds > 65536) return hipErrorInvalidValue;
   int64_t g = grid == 0 ? total : std::min<int64_t>(grid, total);
   if (g > 0x7fffffffL) return hipErrorInvalidValue;
   unsigned nb = (unsigned)g;
+  const bool pers = grid > 0 && grid < total;
   switch (st) {
 #define SOAP_BATCH_CASE(ID, K, JOB, TPB)                                                                 \
   case ID:                                                                                               \
     if (threads > TPB || threads < 64 || threads % 64) return hipErrorInvalidValue;                     \
-    K<<<nb, threads, lds, s>>>((const JOB *)jobs, prefix, n_inst, total);                               \
+    if (pers) K<true><<<nb, threads, lds, s>>>((const JOB *)jobs, prefix, n_inst, total);                \
+    else K<false><<<nb, threads, lds, s>>>((const JOB *)jobs, prefix, n_inst, total);                    \
     break;
     SOAP_BATCH_CASE(BS_FILL, kb_fill, FillJob, kTPB)
     SOAP_BATCH_CASE(BS_TAPS, kb_taps, TapsJob, kTPB)
@@ -1622,12 +1873,31 @@ hipError_t launch_batch(int st, const void *jobs, const int64_t *prefix, int n_i
 #undef SOAP_BATCH_CASE
     case BS_FIR:
       if (threads != kTPB) return hipErrorInvalidValue;
+#define SOAP_FIR_CASE(G)                                                                                 \
+  case G:                                                                                                \
+    if (pers) kb_fir<true, G><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total);     \
+    else kb_fir<false, G><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total);         \
+    break;
+#define SOAP_FIR2_CASE(GG, BB)                                                                            \
+  if (fir_g == GG && fir_b == BB) {                                                                      \
+    if (pers) kb_fir2<true, GG, BB><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total); \
+    else kb_fir2<false, GG, BB><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total);     \
+    break;                                                                                               \
+  }
+      if (fir_b > 0) {
+        SOAP_FIR2_CASE(1, 3)
+        SOAP_FIR2_CASE(1, 5)
+        SOAP_FIR2_CASE(2, 3)
+        return hipErrorInvalidValue;
+      }
       switch (fir_g) {
-        case 2: kb_fir<2><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total); break;
-        case 4: kb_fir<4><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total); break;
-        case 6: kb_fir<6><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total); break;
+        SOAP_FIR_CASE(2)
+        SOAP_FIR_CASE(4)
+        SOAP_FIR_CASE(6)
         default: return hipErrorInvalidValue;
       }
+#undef SOAP_FIR_CASE
+#undef SOAP_FIR2_CASE
       break;
     default:
       return hipErrorInvalidValue;

@@ -52,7 +52,7 @@ EXPORTS = (
     "soap_run_series",
     "soap_set_timing", "soap_get_stage_ms", "soap_stage_name", "soap_num_stages",
     "soap_get_integrate_stats",
-    "soap_get_stage_work",
+    "soap_get_stage_work", "soap_set_fir_shape",
     "soap_diff_setup", "soap_diff_forward", "soap_diff_backward", "soap_diff_get",
     "soap_batch_create", "soap_batch_destroy", "soap_batch_last_error", "soap_batch_step", "soap_batch_sync",
     "soap_batch_stage_ms", "soap_batch_stage_name", "soap_batch_num_stages", "soap_batch_get_config",
@@ -118,6 +118,7 @@ def load(path: str = LIB_PATH):
         "soap_num_stages": (ctypes.c_int32, []),
         "soap_get_integrate_stats": (ctypes.c_int, [vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64)]),
         "soap_get_stage_work": (ctypes.c_int, [vp, ctypes.c_int32, _dp, _dp]),
+        "soap_set_fir_shape": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32]),
         "soap_diff_setup": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]),
         "soap_diff_forward": (ctypes.c_int, [vp, ctypes.POINTER(DiffParams), vp, vp, vp, vp, _dp, _dp,
                                              ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]),
@@ -131,9 +132,9 @@ def load(path: str = LIB_PATH):
         "soap_batch_stage_ms": (ctypes.c_int, [vp, _dp, ctypes.c_int32]),
         "soap_batch_stage_name": (ctypes.c_char_p, [ctypes.c_int32]),
         "soap_batch_num_stages": (ctypes.c_int32, []),
-        "soap_batch_get_config": (ctypes.c_int, [vp, _ip, _ip, ctypes.POINTER(ctypes.c_int64),
+        "soap_batch_get_config": (ctypes.c_int, [vp, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_int64),
                                                  ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
-        "soap_batch_set_config": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32,
+        "soap_batch_set_config": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                  ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
         "soap_batch_tune": (ctypes.c_int, [vp, ctypes.c_int32]),
     }
@@ -377,6 +378,11 @@ class Engine:
                   "stage_work")
         return b.value, f.value
 
+    def set_fir_shape(self, phases_per_block: int, pixels_per_thread: int):
+        """Launch shape of the active FIR: (4, 0), (1, 3), (1, 5), (2, 3), (2, 1), (4, 1); (-1, -1) automatic."""
+        self._chk(self.L.soap_set_fir_shape(self.ctx, int(phases_per_block), int(pixels_per_thread)),
+                  "set_fir_shape")
+
     def integrate_stats(self):
         """(bytes_min, bytes_phase_sum, rows_sum) of the last integrate."""
         b = ctypes.c_double()
@@ -385,6 +391,16 @@ class Engine:
         self._chk(self.L.soap_get_integrate_stats(self.ctx, ctypes.byref(b), ctypes.byref(bp), ctypes.byref(r)),
                   "integrate_stats")
         return float(b.value), float(bp.value), int(r.value)
+
+
+def select_mask_lines(wave, line_l, line_w, trim: int = 210):
+    """Mask lines inside the trimmed wavelength range with weight > 0.1, in their
+    given order (run_all_SOAP.py:285-290)."""
+    wave_t = np.asarray(wave)[trim:len(wave) - trim]
+    line_l = np.asarray(line_l, dtype=np.float64)
+    line_w = np.asarray(line_w, dtype=np.float64)
+    sel = (line_l > wave_t.min()) & (line_l < wave_t.max()) & (line_w > 0.1)
+    return line_l[sel], line_w[sel]
 
 
 def setup_workload(eng: Engine, w, with_mask: bool = True, trim: int = 210):
@@ -398,9 +414,7 @@ def setup_workload(eng: Engine, w, with_mask: bool = True, trim: int = 210):
     eng.disk_init(w.star)
     eng.set_activity(w.act, w.phases, w.regions)
     if with_mask:
-        wave_t = w.wave[trim:len(w.wave) - trim]
-        sel = (w.mask_l > wave_t.min()) & (w.mask_l < wave_t.max()) & (w.mask_w > 0.1)
-        eng.set_mask(w.mask_l[sel], w.mask_w[sel], w.vgrid, trim)
+        eng.set_mask(*select_mask_lines(w.wave, w.mask_l, w.mask_w, trim), w.vgrid, trim)
 
 
 class Batch:
@@ -457,14 +471,15 @@ class Batch:
     def config(self) -> dict:
         cells = ctypes.c_int32()
         g = ctypes.c_int32()
+        bb = ctypes.c_int32()
         grid = (ctypes.c_int64 * self.n_stage)()
         work = (ctypes.c_int64 * self.n_stage)()
-        self._chk(self.L.soap_batch_get_config(self.b, ctypes.byref(cells), ctypes.byref(g), grid, work,
-                                               self.n_stage), "batch_get_config")
-        return {"taps_cells": cells.value, "fir_g": g.value,
+        self._chk(self.L.soap_batch_get_config(self.b, ctypes.byref(cells), ctypes.byref(g), ctypes.byref(bb), grid,
+                                               work, self.n_stage), "batch_get_config")
+        return {"taps_cells": cells.value, "fir_g": g.value, "fir_b": bb.value,
                 "grid": dict(zip(self.stage_names, list(grid))), "work": dict(zip(self.stage_names, list(work)))}
 
-    def set_config(self, taps_cells: int, fir_g: int, grid: dict):
+    def set_config(self, taps_cells: int, fir_g: int, fir_b: int, grid: dict):
         arr = (ctypes.c_int64 * self.n_stage)(*[int(grid.get(n, 0)) for n in self.stage_names])
-        self._chk(self.L.soap_batch_set_config(self.b, int(taps_cells), int(fir_g), arr, self.n_stage),
+        self._chk(self.L.soap_batch_set_config(self.b, int(taps_cells), int(fir_g), int(fir_b), arr, self.n_stage),
                   "batch_set_config")

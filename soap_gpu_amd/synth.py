@@ -329,3 +329,40 @@ def small_batch(seed: int = 17, n_inst: int = 5):
                            int(rng.integers(1, 6)), float(rng.uniform(1.0, 20.0)), float(rng.uniform(10.0, 90.0)),
                            name=f"small_batch[{i}]", n_mu=6)
             for i in range(n_inst)]
+
+
+def smooth_blob_maps(rng: np.random.Generator, n_phase: int, res: int, n_blobs: int = 6) -> np.ndarray:
+    """Filling-factor maps [n_phase][res][res] over [-1, 1]^2: a sum of Gaussian blobs
+    (centres in the disk, widths 0.05-0.25) clipped to [0, 1], drifting in y from
+    phase to phase like rotating regions."""
+    x = np.linspace(-1.0, 1.0, res)
+    Y, Z = np.meshgrid(x, x, indexing="ij")
+    c = rng.uniform(-0.7, 0.7, (n_blobs, 2))
+    wdt = rng.uniform(0.05, 0.25, n_blobs)
+    amp = rng.uniform(0.5, 1.5, n_blobs)
+    out = np.empty((n_phase, res, res))
+    for p in range(n_phase):
+        dy = 1.6 * p / max(1, n_phase) - 0.8
+        m = np.zeros((res, res))
+        for b in range(n_blobs):
+            yc = (c[b, 0] + dy + 1.0) % 2.0 - 1.0
+            m += amp[b] * np.exp(-((Y - yc) ** 2 + (Z - c[b, 1]) ** 2) / (2 * wdt[b] ** 2))
+        out[p] = np.clip(m, 0.0, 1.0)
+    return out
+
+
+def config3(seed: int = 1237, n_phase: int = 50, grid: int = 600, map_res: int = 512, n_mu: int = 10):
+    """Adjoint shape: grid 600, lambda 5000-5500 A log-uniform (~100k points), 50
+    phases, 8000 seeded lines (all in the mask), a 512 x 512 smooth-blob filling
+    factor map per phase.  Returns (workload, maps)."""
+    rng = np.random.default_rng(seed)
+    wave = log_grid(5000.0, 5500.0, n=100000)
+    lines = draw_lines(rng, 8000, 5000.0, 5500.0)
+    bounds, _ = mu_bins(n_mu)
+    cubes = make_cubes(wave, lines, n_mu)
+    maps = smooth_blob_maps(rng, n_phase, map_res)
+    phases = np.arange(n_phase, dtype=np.float64) / n_phase
+    w = Workload("config3", wave, 1, bounds, cubes, lines, lines["center"].copy(), lines["depth"].copy(),
+                 _star(grid, 2.0), _act(), phases, [[] for _ in phases], sigma_inst_ms(115000),
+                 vel_grid(20000.0, 250.0), meta={"map_res": map_res})
+    return w, maps

@@ -105,22 +105,38 @@ def test_batch_tuned_shapes(batch_case):
     ws, engines, b, refs = batch_case
     b.tune(reps=1)
     cfg = b.config()
-    assert cfg["fir_g"] in (2, 4, 6)
+    assert (cfg["fir_g"], cfg["fir_b"]) in ((2, 0), (4, 0), (6, 0), (1, 3), (1, 5), (2, 3))
     assert all(g % 8 == 0 and g >= 0 for g in cfg["grid"].values())
     for name in ("taps", "fir", "conv", "ccf", "fit"):
         assert cfg["work"][name] > 0, name
     rng = np.random.default_rng(5)
-    for cells, g, mult in ((1024, 2, 8), (16384, 6, 16), (cfg["taps_cells"], cfg["fir_g"], None)):
-        if mult is None:
+    shapes = [(1024, 2, 0, 8), (16384, 6, 0, 16), (4096, 4, 0, None), (2048, 1, 3, None), (4096, 1, 5, 4),
+              (8192, 2, 3, None), (cfg["taps_cells"], cfg["fir_g"], cfg["fir_b"], "tuned")]
+    for cells, g, bb, mult in shapes:
+        if mult == "tuned":
             grid = cfg["grid"]
+        elif mult is None:
+            grid = {n: 0 for n in cfg["grid"]}
         else:
             grid = {n: mult * 8 for n in cfg["grid"]}  # persistent: far fewer blocks than items
-        b.set_config(cells, g, grid)
+        b.set_config(cells, g, bb, grid)
         for w, e in zip(ws, engines):
             _clobber(e, w, rng)
         b.step(graph=True)
         b.sync()
-        _check_all(ws, engines, refs, f"shapes {cells}/{g}/{mult}")
+        got = [_outputs(e, w) for w, e in zip(ws, engines)]
+        if bb == 0:  # the plain path's FIR form: bit for bit
+            for i, r in enumerate(refs):
+                _same(got[i], r, f"shapes {cells}/{g}/{bb}/{mult}[{i}]")
+        else:  # register-blocked FIR: the same sums in another order
+            for i, (w, r) in enumerate(zip(ws, refs)):
+                for p in range(len(w.phases)):
+                    for t in range(3):
+                        ref = r["raw"][p, t]
+                        scale = max(np.max(np.abs(ref)), 1e-300)
+                        assert np.max(np.abs(got[i]["raw"][p, t] - ref)) <= 1e-13 * scale, (cells, g, bb, i, p, t)
+                np.testing.assert_allclose(got[i]["fit"]["par"][..., 3], r["fit"]["par"][..., 3], rtol=0,
+                                           atol=1e-4)
     ms = b.stage_ms()
     assert set(ms) == set(cfg["grid"]) and all(v >= 0 for v in ms.values())
 

@@ -95,6 +95,41 @@ def test_integrate_parity(case, request):
             assert normwise(g["raw"][p, t], ref) <= SPEC_TOL, (p, t)
 
 
+FIR_SHAPES = [(4, 0), (1, 3), (1, 5), (2, 3), (2, 1), (4, 1)]
+
+
+@pytest.mark.parametrize("shape", FIR_SHAPES)
+def test_integrate_fir_shapes(engine, shape):
+    """Every launch shape of the active FIR against the oracle: a 7-phase
+    workload (a partial last phase group) with a wide shift range (vsini 18
+    km/s) and a tile count that leaves a partial last tile."""
+    w = synth.small(grid=56, n=5003, n_phase=7, vsini=18.0, incl=80.0)
+    setup_workload(engine, w, with_mask=False)
+    engine.set_fir_shape(*shape)
+    try:
+        engine.integrate()
+        got = np.stack([np.stack(engine.get_raw(p)) for p in range(len(w.phases))])
+    finally:
+        engine.set_fir_shape(-1, -1)
+    maps = oracle_maps(w)
+    a = w.act
+    for p in range(len(w.phases)):
+        temp, bi = O.phase_maps(56, w.star["incl"], a["nrho"], a["t_star"], a["t_diff_spot"], a["t_diff_fac"],
+                                w.phases[p], w.regions[p])
+        ref = O.integrate_phase(O.GRID_LOG, w.wave, maps, temp, bi, w.cubes, 56, a["t_star"], a["t_diff_fac"])
+        for t in range(3):
+            if np.max(np.abs(ref[t])) == 0:
+                assert np.max(np.abs(got[p, t])) == 0, (shape, p, t)
+                continue
+            assert normwise(got[p, t], ref[t]) <= SPEC_TOL, (shape, p, t)
+
+
+def test_fir_shape_rejects_unknown(engine):
+    from soap_gpu_amd.engine import SoapError
+    with pytest.raises(SoapError, match="unsupported"):
+        engine.set_fir_shape(3, 3)
+
+
 @pytest.mark.parametrize("t_star", [5778, 3700, 9000])
 def test_integrate_solar_mode(engine, t_star):
     """Solar mode: facula intensity by the Planck ratio of the cell's plage
