@@ -99,11 +99,13 @@ __device__ __forceinline__ void fx_split(double x, unsigned long long &hi, unsig
   lo = (unsigned long long)((y - fl) * 1099511627776.0);
 }
 
-__device__ __forceinline__ void fx_add(unsigned long long *p, double x) {
+// P: a pointer to unsigned long long in any address space (LDS, global, generic)
+template <class P>
+__device__ __forceinline__ void fx_add(P p, double x) {
   unsigned long long hi, lo;
   fx_split(x, hi, lo);
-  atomicAdd(p, hi);
-  atomicAdd(p + 1, lo);
+  __hip_atomic_fetch_add(p, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_fetch_add(p + 1, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ double fx_value(unsigned long long hi, unsigned long long lo) {
@@ -115,8 +117,9 @@ __device__ __forceinline__ double fx_value(unsigned long long hi, unsigned long 
 // Cell classification of one phase: loop over the phase's visible regions in
 // order, later ones overwriting earlier ones (SOAP_integration.cu:724-785,
 // spot_scan :346-387).  Returns -1 (quiet), 0 (spot) or 1 (facula).
-__device__ __forceinline__ int classify_cell(int iy, int iz, double delta_grid, const RegionDev *geo, int g0,
-                                             int g1, double *r_cos_out) {
+template <class GeoPtr>
+__device__ __forceinline__ int classify_cell(int iy, int iz, double delta_grid, GeoPtr geo, int g0, int g1,
+                                             double *r_cos_out) {
 #pragma clang fp contract(off)
   double y = -1. + iy * delta_grid;
   double z = -1. + iz * delta_grid;
@@ -124,7 +127,7 @@ __device__ __forceinline__ int classify_cell(int iy, int iz, double delta_grid, 
   double xa0 = sqrt(1. - (y * y + z * z));
   int res = -1;
   for (int r = g0; r < g1; r++) {
-    const RegionDev &g = geo[r];
+    const auto &g = geo[r];
     if (iy < g.y0 || iy >= g.y1 || iz < g.z0 || iz >= g.z1) continue;
     double xi0 = g.r0 * xa0 + g.r1 * y + g.r2 * z;
     if (xi0 * xi0 >= g.oms2) res = g.type;

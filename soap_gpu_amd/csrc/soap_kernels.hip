@@ -21,6 +21,21 @@ namespace soap {
 
 constexpr int kTPB = 256;
 
+// Global-memory views of pointers.  A pointer loaded from memory (a field of a
+// batch job) is generic to the compiler, whose accesses then become flat
+// instructions and uniform loads cannot use the scalar unit; the kernel bodies
+// below access device buffers through these views so that the single-instance
+// kernels (pointers in kernel arguments) and the batched ones compile alike.
+#define GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const GLOBAL T *gl(const T *p) {
+  return (const GLOBAL T *)p;
+}
+template <class T>
+__device__ __forceinline__ GLOBAL T *gw(T *p) {
+  return (GLOBAL T *)p;
+}
+
 // ------------------------------------------------------------------ utilities
 __global__ void k_fill_u64(unsigned long long *p, size_t count) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -111,9 +126,10 @@ hipError_t launch_disk_init(const DiskArgs &a, hipStream_t s) {
 
 // Shared part of the tap kernels: accumulate into an LDS window of WD tap offsets
 // [base, base+WD) per (set, bin); offsets outside it go straight to global.
+#define LDSMEM __attribute__((address_space(3)))
 struct TapWindow {
-  unsigned long long *lds;  // [nsets][n_mu][wd][2]
-  unsigned long long *gacc; // [nsets][n_mu][nd][2] of this job
+  LDSMEM unsigned long long *lds;  // [nsets][n_mu][wd][2]
+  GLOBAL unsigned long long *gacc; // [nsets][n_mu][nd][2] of this job
   int base, wd, nd, dmin, n_mu;
   __device__ void add(int set, int k, int d, double x) const {
     if (x == 0.0) return;
@@ -149,8 +165,8 @@ __device__ void flush_window(const TapWindow &w, int nsets) {
     int wi = (int)(e % w.wd);
     long sk = e / w.wd;  // set * n_mu + k
     long g = (sk * w.nd + (w.base + wi - w.dmin)) * 2;
-    atomicAdd(w.gacc + g, hi);
-    atomicAdd(w.gacc + g + 1, lo);
+    __hip_atomic_fetch_add(w.gacc + g, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_add(w.gacc + g + 1, lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -169,7 +185,7 @@ __global__ void k_taps_quiet(TapAccArgs a) {
     if (!(a.limb[idx] == -999.0 && a.delta[idx] == -999.0)) mn = min(mn, a.m[idx] - 1);
   }
   int base = block_min_int(mn, scratch);
-  TapWindow w{lds_u64, a.acc, base, a.wd, a.nd, a.dmin, a.n_mu};
+  TapWindow w{(LDSMEM unsigned long long *)lds_u64, gw(a.acc), base, a.wd, a.nd, a.dmin, a.n_mu};
   long lds_n = (long)a.nsets * a.n_mu * a.wd * 2;
   for (long e = threadIdx.x; e < lds_n; e += blockDim.x) lds_u64[e] = 0ull;
   __syncthreads();
@@ -205,22 +221,26 @@ __device__ __forceinline__ void taps_active_body(const TapAccArgs &a, int bx_, i
   __shared__ int scratch[16];
   int job = by_;
   int G = a.grid;
-  const int32_t *bx = a.box + job * 4;
+  const GLOBAL int32_t *region = gl(a.region), *mcell = gl(a.m), *geo_off = gl(a.geo_off);
+  const GLOBAL double *limb = gl(a.limb), *delta = gl(a.delta), *tcell = gl(a.t), *lbm = gl(a.lbm);
+  const GLOBAL RegionDev *geo = gl(a.geo);
+  const GLOBAL int32_t *bx = gl(a.box) + job * 4;
   int cy0 = bx[0], cy1 = bx[1], cz0 = bx[2], cz1 = bx[3];
   int iy0 = cy0 + bx_ * a.cols_per_block;
   if (iy0 >= cy1) return;  // uniform over the block
   int iy1 = min(cy1, iy0 + a.cols_per_block);
   int nz = cz1 - cz0;
   long ncell = (long)(iy1 - iy0) * nz;
-  int g0 = a.geo_off[job], g1 = a.geo_off[job + 1];
+  int g0 = geo_off[job], g1 = geo_off[job + 1];
   double dg = 2. / G;
   int mn = 1 << 30;
   for (long c = threadIdx.x; c < ncell; c += blockDim.x) {
     long idx = (long)(iy0 + c / nz) * G + (cz0 + c % nz);
-    if (!(a.limb[idx] == -999.0 && a.delta[idx] == -999.0)) mn = min(mn, a.m[idx] - 1);
+    if (!(limb[idx] == -999.0 && delta[idx] == -999.0)) mn = min(mn, mcell[idx] - 1);
   }
   int base = block_min_int(mn, scratch);
-  TapWindow w{lds_u64, a.acc + (long)job * a.nsets * a.n_mu * a.nd * 2, base, a.wd, a.nd, a.dmin, a.n_mu};
+  TapWindow w{(LDSMEM unsigned long long *)lds_u64, gw(a.acc) + (long)job * a.nsets * a.n_mu * a.nd * 2, base, a.wd,
+              a.nd, a.dmin, a.n_mu};
   long lds_n = (long)a.nsets * a.n_mu * a.wd * 2;
   for (long e = threadIdx.x; e < lds_n; e += blockDim.x) lds_u64[e] = 0ull;
   __syncthreads();
@@ -228,15 +248,15 @@ __device__ __forceinline__ void taps_active_body(const TapAccArgs &a, int bx_, i
     int iy = iy0 + (int)(c / nz), iz = cz0 + (int)(c % nz);
     long idx = (long)iy * G + iz;
     double rc;
-    int ty = classify_cell(iy, iz, dg, a.geo, g0, g1, &rc);
+    int ty = classify_cell(iy, iz, dg, geo, g0, g1, &rc);
     if (ty < 0) continue;
-    double L = a.limb[idx], t = a.t[idx];
-    int k = a.region[idx], m = a.m[idx];
+    double L = limb[idx], t = tcell[idx];
+    int k = region[idx], m = mcell[idx];
     if (ty == 0) {
       w.add(0, k, m, L * (1.0 - t));
       w.add(0, k, m - 1, L * t);
     } else if (a.solar_nodes == 0) {
-      double L2 = (1 + a.lbm[idx]) * L;
+      double L2 = (1 + lbm[idx]) * L;
       w.add(1, k, m, L * (1.0 - t));
       w.add(1, k, m - 1, L * t);
       w.add(2, k, m, L2 * (1.0 - t));
@@ -274,7 +294,12 @@ hipError_t launch_taps_active(const TapAccArgs &a, int n_job, int max_col_blocks
 // Active taps get one more output set than they accumulate: the last one is
 // set 0 + set 1 (all active cells, weight limb), used on the SED quiet rows.
 // Flags: bit 0 the spot set is nonzero, bit 1 a facula set is nonzero.
-__device__ __forceinline__ void taps_finalize_body(const unsigned long long *acc, int nsets_in, int nsets_out, int n_mu, int nd, double *taps, int32_t *krange, int32_t *kflags, int32_t *bins, int32_t *nbins, int bx_, int by_) {
+__device__ __forceinline__ void taps_finalize_body(const unsigned long long *acc_, int nsets_in, int nsets_out, int n_mu,
+                                                   int nd, double *taps_, int32_t *krange_, int32_t *kflags_,
+                                                   int32_t *bins_, int32_t *nbins_, int bx_, int by_) {
+  const GLOBAL unsigned long long *acc = gl(acc_);
+  GLOBAL double *taps = gw(taps_);
+  GLOBAL int32_t *krange = gw(krange_), *kflags = gw(kflags_), *bins = gw(bins_), *nbins = gw(nbins_);
   int job = bx_;
   __shared__ int s_lo[256], s_hi[256], s_fl[256];
   for (int k = threadIdx.x; k < n_mu; k += blockDim.x) {
@@ -282,7 +307,7 @@ __device__ __forceinline__ void taps_finalize_body(const unsigned long long *acc
     for (int d = 0; d < nd; d++) {
       bool nz = false;
       double v01 = 0.0;
-      double *out = taps + (((long)job * n_mu + k) * nd + d) * nsets_out;
+      GLOBAL double *out = taps + (((long)job * n_mu + k) * nd + d) * nsets_out;
       for (int st = 0; st < nsets_in; st++) {
         long e = ((((long)job * nsets_in + st) * n_mu + k) * nd + d) * 2;
         double v = fx_value(acc[e], acc[e + 1]);
@@ -443,6 +468,7 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
   int ng = min(G, n_job - p0);
   int nd = a.nd, n_mu = a.n_mu;
   int row_len = kFirTile + nd - 1;
+  const GLOBAL int32_t *kflags = gl(a.kflags), *krange = gl(a.krange);
   long j0 = (long)tile * kFirTile;
   long src0 = j0 - a.dmin - nd + 1;
   // group metadata: per-phase flags, union flags and union tap range per bin
@@ -451,8 +477,8 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
     int fl = 0;
     if (g < ng && k < n_mu) {
       long jk = (long)(p0 + g) * n_mu + k;
-      fl = a.kflags[jk];
-      if (a.krange[jk * 2] > a.krange[jk * 2 + 1]) fl = 0;
+      fl = kflags[jk];
+      if (krange[jk * 2] > krange[jk * 2 + 1]) fl = 0;
     }
     s_fl[g][k] = fl;
   }
@@ -464,8 +490,8 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
       if (s_fl[g][k]) {
         long jk = (long)(p0 + g) * n_mu + k;
         ufl |= s_fl[g][k] | 4;
-        ulo = min(ulo, a.krange[jk * 2]);
-        uhi = max(uhi, a.krange[jk * 2 + 1]);
+        ulo = min(ulo, krange[jk * 2]);
+        uhi = max(uhi, krange[jk * 2 + 1]);
       }
     s_ufl[k] = ufl;
     s_ulo[k] = ulo;
@@ -493,7 +519,7 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
     const int has_sp = ufl & 1, has_fa = (ufl >> 1) & 1;
     const int nrow = 2 + has_sp + has_fa;
     for (int r = 0; r < nrow; r++) {
-      const double *src = (r == 0 ? a.q : r == 1 ? a.qs : (r == 2 && has_sp) ? a.sp : a.fa) + (long)k * a.n;
+      const GLOBAL double *src = gl(r == 0 ? a.q : r == 1 ? a.qs : (r == 2 && has_sp) ? a.sp : a.fa) + (long)k * a.n;
       for (int x = threadIdx.x; x < row_len; x += blockDim.x) {
         long i = src0 + x;
         rows[r * row_len + x] = (i >= 0 && i < a.n) ? src[i] : 0.0;
@@ -504,7 +530,7 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
 #pragma unroll
     for (int g = 0; g < G; g++) flg[g] = uni(s_fl[g][k]);
     // taps of phase g at offset di: a.taps[(((p0+g)*n_mu + k)*nd + di)*4 + set], uniform addresses
-    const double *Tk = a.taps + ((long)p0 * n_mu + k) * nd * 4;
+    const GLOBAL double *Tk = gl(a.taps) + ((long)p0 * n_mu + k) * nd * 4;
     const long gstride = (long)n_mu * nd * 4;
     const double *rq = rows, *rs = rows + row_len, *rp = rows + 2 * row_len, *rf = rows + (2 + has_sp) * row_len;
     for (int di = ulo; di <= uhi; di++) {
@@ -516,7 +542,7 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
       for (int g = 0; g < G; g++) {
         const int fl = flg[g];
         if (!fl) continue;
-        const double *T = Tk + g * gstride + di * 4;
+        const GLOBAL double *T = Tk + g * gstride + di * 4;
         F[g][0] = fma(T[3], vs, F[g][0]);
         if (fl & 1) {
           F[g][1] = fma(T[0], vq, F[g][1]);
@@ -533,12 +559,12 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
   }
   long j = j0 + threadIdx.x;
   if (j < a.n) {
-    double cs = a.sps ? a.sps[j] : 0.0;
-    double cf = a.fas ? a.fas[j] : 0.0;
+    double cs = a.sps ? gl(a.sps)[j] : 0.0;
+    double cf = a.fas ? gl(a.fas)[j] : 0.0;
 #pragma unroll
     for (int g = 0; g < G; g++) {
       if (g >= ng) break;
-      double *o = a.out + (long)(p0 + g) * 3 * a.n;
+      GLOBAL double *o = gw(a.out) + (long)(p0 + g) * 3 * a.n;
       // F: 0 = H1a*SEDq, 1 = H1s*Sq, 2 = H2f*Sq, 3 = H1f*Sq, 4 = H1s*Ssp, 5 = H1f*Sfac, 6 = H2f*Sfac
       o[j] = F[g][0] - cs * F[g][1] - cf * F[g][2];
       o[a.n + j] = (F[g][1] + F[g][3]) - F[g][4] - F[g][5];
@@ -638,6 +664,8 @@ __device__ __forceinline__ void fir2_body(const FirArgs &a, int n_job, int n_til
   const int lcs = (nd + kF2CH - 1) / kF2CH * kF2CH + kF2CH;  // LDS stride of one tap set
   double *tp = f2s;                    // [G][4][lcs]
   double *xs = f2s + (long)G * 4 * lcs;  // [T + lcs]
+  const GLOBAL int32_t *kflags = gl(a.kflags), *krange = gl(a.krange);
+  const GLOBAL double *taps = gl(a.taps);
   const long j0 = (long)tile * T;
   const long src0 = j0 - a.dmin - (nd - 1);
   // group metadata: per-phase flags, union flags and union tap range per bin
@@ -646,8 +674,8 @@ __device__ __forceinline__ void fir2_body(const FirArgs &a, int n_job, int n_til
     int fl = 0;
     if (g < ng && k < n_mu) {
       long jk = (long)(p0 + g) * n_mu + k;
-      fl = a.kflags[jk];
-      if (a.krange[jk * 2] > a.krange[jk * 2 + 1]) fl = 0;
+      fl = kflags[jk];
+      if (krange[jk * 2] > krange[jk * 2 + 1]) fl = 0;
     }
     s_fl[g][k] = fl;
   }
@@ -659,8 +687,8 @@ __device__ __forceinline__ void fir2_body(const FirArgs &a, int n_job, int n_til
       if (s_fl[g][k]) {
         long jk = (long)(p0 + g) * n_mu + k;
         ufl |= s_fl[g][k] | 4;
-        ulo = min(ulo, nd - 1 - a.krange[jk * 2 + 1]);
-        uhi = max(uhi, nd - 1 - a.krange[jk * 2]);
+        ulo = min(ulo, nd - 1 - krange[jk * 2 + 1]);
+        uhi = max(uhi, nd - 1 - krange[jk * 2]);
       }
     s_ufl[k] = ufl;
     s_ulo[k] = ulo;
@@ -694,7 +722,7 @@ __device__ __forceinline__ void fir2_body(const FirArgs &a, int n_job, int n_til
       if (g < ng && s_fl[g][k]) {
         long jk = (long)(p0 + g) * n_mu + k;
         int di = nd - 1 - u;
-        if (di >= a.krange[jk * 2] && di <= a.krange[jk * 2 + 1]) v = a.taps[(jk * nd + di) * 4 + st];
+        if (di >= krange[jk * 2] && di <= krange[jk * 2 + 1]) v = taps[(jk * nd + di) * 4 + st];
       }
       tp[(g * 4 + st) * lcs + (r % Lc)] = v;
     }
@@ -703,7 +731,7 @@ __device__ __forceinline__ void fir2_body(const FirArgs &a, int n_job, int n_til
 #pragma unroll 1
     for (int r = 0; r < 4; r++) {
       if ((r == 2 && !(ufl & 1)) || (r == 3 && !(ufl & 2))) continue;  // no spot / facula cells in this bin
-      const double *src = (r == 0 ? a.qs : r == 1 ? a.q : r == 2 ? a.sp : a.fa) + (long)k * a.n;
+      const GLOBAL double *src = gl(r == 0 ? a.qs : r == 1 ? a.q : r == 2 ? a.sp : a.fa) + (long)k * a.n;
       if (r > 0) __syncthreads();  // the previous row's reads are done
       for (int x = threadIdx.x; x < span; x += blockDim.x) {
         long i = rs0 + x;
@@ -721,13 +749,13 @@ __device__ __forceinline__ void fir2_body(const FirArgs &a, int n_job, int n_til
 #pragma unroll
   for (int b = 0; b < B; b++) {
     long j = j0 + threadIdx.x * B + b;
-    cs[b] = (a.sps && j < a.n) ? a.sps[j] : 0.0;
-    cf[b] = (a.fas && j < a.n) ? a.fas[j] : 0.0;
+    cs[b] = (a.sps && j < a.n) ? gl(a.sps)[j] : 0.0;
+    cf[b] = (a.fas && j < a.n) ? gl(a.fas)[j] : 0.0;
   }
 #pragma unroll
   for (int g = 0; g < G; g++) {
     if (g >= ng) break;
-    double *o = a.out + (long)(p0 + g) * 3 * a.n;
+    GLOBAL double *o = gw(a.out) + (long)(p0 + g) * 3 * a.n;
 #pragma unroll
     for (int t = 0; t < 3; t++) {
       __syncthreads();
@@ -988,7 +1016,7 @@ constexpr int kConvMaxTaps = 4095;
 // stage x[i0 + dlo + e], e in [0, kConvTile + ntp - 1), into LDS; g(x) = 1 - x for
 // the quiet term, x otherwise; zero outside [0, n)
 template <bool ONE_MINUS>
-__device__ __forceinline__ void conv_stage(double *xs, const double *X, long i0, int dlo, int span, long n) {
+__device__ __forceinline__ void conv_stage(double *xs, const GLOBAL double *X, long i0, int dlo, int span, long n) {
   for (int e = threadIdx.x; e < span; e += blockDim.x) {
     long m = i0 + dlo + e;
     double v = (m >= 0 && m < n) ? X[m] : 0.0;
@@ -1015,7 +1043,7 @@ __device__ __forceinline__ void conv_accumulate(const double *xs, const double *
   }
 }
 
-__device__ __forceinline__ void block_minmax_store(double amin, double omax, double *part_slot) {
+__device__ __forceinline__ void block_minmax_store(double amin, double omax, GLOBAL double *part_slot) {
   __shared__ double s_min[kTPB / 64], s_max[kTPB / 64];
   for (int off = 32; off > 0; off >>= 1) {
     double a = __shfl_xor(amin, off), b = __shfl_xor(omax, off);
@@ -1045,14 +1073,16 @@ __device__ __forceinline__ void block_minmax_store(double amin, double omax, dou
 // written to the frame slots of the lower-spectrum layout [flux, bconv, tot] =
 // [SED, normalised, SED] (run_all_SOAP.py:359-362), with the partial min(Q) and
 // max(o != 0) (seed -60000) of each tile per slot.
-__device__ __forceinline__ void conv_quiet_body(const ConvArgs &c, const double *quiet, double *frame, double *part, int bx_, int by_) {
+__device__ __forceinline__ void conv_quiet_body(const ConvArgs &c, const double *quiet_, double *frame_, double *part_, int bx_, int by_) {
+  const GLOBAL double *quiet = gl(quiet_);
+  GLOBAL double *frame = gw(frame_), *part = gw(part_);
   extern __shared__ double xs[];
   int q = by_;
   long i0 = (long)bx_ * kConvTile;
   int span = kConvTile + c.ntp - 1;
-  const double *Q = quiet + (long)q * c.n;
+  const GLOBAL double *Q = quiet + (long)q * c.n;
   double *ct = xs + span + (span & 1);  // taps, 16-byte aligned
-  for (int e = threadIdx.x; e < c.ntp; e += blockDim.x) ct[e] = c.ctap[e];
+  for (int e = threadIdx.x; e < c.ntp; e += blockDim.x) ct[e] = gl(c.ctap)[e];
   conv_stage<true>(xs, Q, i0, c.dlo, span, c.n);
   __syncthreads();
   double acc[kConvB];
@@ -1095,7 +1125,9 @@ __global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *q
 // sel = 0 for bconv, 1 otherwise (SOAP_resolution.cu:414-416).  Blocks are
 // numbered so that all phases of one lambda tile run back to back on one XCD
 // (block b -> XCD b % 8): the tile's quiet rows stay in that XCD's L2.
-__device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double *quiet, const double *raw, double *o_out, double *part, int n_phase, int bx_, int by_) {
+__device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double *quiet_, const double *raw_, double *o_out_, double *part_, int n_phase, int bx_, int by_) {
+  const GLOBAL double *quiet = gl(quiet_), *raw = gl(raw_);
+  GLOBAL double *o_out = gw(o_out_), *part = gw(part_);
   extern __shared__ double xs[];
   const int slot = bx_ >> 3;
   const int tile = (slot / n_phase) * 8 + (bx_ & 7);
@@ -1104,13 +1136,13 @@ __device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double
   long i0 = (long)tile * kConvTile;
   int span = kConvTile + c.ntp - 1;
   double *ct = xs + span + (span & 1);  // taps, 16-byte aligned
-  for (int e = threadIdx.x; e < c.ntp; e += blockDim.x) ct[e] = c.ctap[e];
+  for (int e = threadIdx.x; e < c.ntp; e += blockDim.x) ct[e] = gl(c.ctap)[e];
 #pragma unroll 1
   for (int t = 0; t < 3; t++) {
     int sel = t == 1 ? 0 : 1;
     long sp = (long)p * 3 + t;
-    const double *D = raw + sp * c.n;
-    const double *Q = quiet + (long)sel * c.n;
+    const GLOBAL double *D = raw + sp * c.n;
+    const GLOBAL double *Q = quiet + (long)sel * c.n;
     double amin = HUGE_VAL, omax = -60000.0;
     for (int e = threadIdx.x; e < span; e += blockDim.x) {
       long m = i0 + c.dlo + e;
@@ -1172,7 +1204,9 @@ int conv_tile() { return kConvTile; }
 int conv_chunk() { return kConvCH; }
 
 // min / max of each spectrum over its tile partials (exact: order independent)
-__device__ __forceinline__ void minmax_body(const double *part, int n_tiles, double *mm, int bx_, int by_) {
+__device__ __forceinline__ void minmax_body(const double *part_, int n_tiles, double *mm_, int bx_, int by_) {
+  const GLOBAL double *part = gl(part_);
+  GLOBAL double *mm = gw(mm_);
   __shared__ double s_min[kTPB], s_max[kTPB];
   int sp = bx_;
   double a = HUGE_VAL, b = -HUGE_VAL;
@@ -1205,10 +1239,12 @@ __global__ __launch_bounds__(kTPB) void k_minmax_reduce(const double *part, int 
 // Spectrum sp = blockIdx.y at o + sp*n; a block covers kNormPer*kTPB pixels with
 // coalesced loads issued before the stores.
 constexpr int kNormPer = 8;
-__device__ __forceinline__ void norm_body(long n, const double *mm, double *o, int bx_, int by_) {
+__device__ __forceinline__ void norm_body(long n, const double *mm_, double *o_, int bx_, int by_) {
+  const GLOBAL double *mm = gl(mm_);
+  GLOBAL double *o = gw(o_);
   long sp = by_;
   double mn = mm[sp * 2], mx = mm[sp * 2 + 1];
-  double *x = o + sp * n;
+  GLOBAL double *x = o + sp * n;
   long base = (long)bx_ * (kTPB * kNormPer) + threadIdx.x;
   double v[kNormPer];
 #pragma unroll
@@ -1326,12 +1362,17 @@ hipError_t launch_mask_overwrite(const int32_t *sorted, int n_lines, int n_v, in
 constexpr int kCcfLines = 32;
 constexpr int kCcfSpec = 8;
 
-__device__ __forceinline__ void ccf_body(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v, int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv, const double *w_sorted, double *part, int bx_, int by_) {
+__device__ __forceinline__ void ccf_body(const double *spec_, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v, int64_t n_t, const int32_t *m_start_, const uint32_t *m_pk_, const double2 *m_wv_, const double *w_sorted_, double *part_, int bx_, int by_) {
+  const GLOBAL double *spec = gl(spec_), *w_sorted = gl(w_sorted_);
+  const GLOBAL int32_t *m_start = gl(m_start_);
+  const GLOBAL uint32_t *m_pk = gl(m_pk_);
+  const GLOBAL double *m_wv = gl((const double *)m_wv_);  // {x, y} pairs
+  GLOBAL double *part = gw(part_);
   int chunk = bx_;
   int s0 = by_ * kCcfSpec;
   int ns = min(kCcfSpec, n_spec - s0);
   int q0 = chunk * kCcfLines, q1 = min(n_lines, q0 + kCcfLines);
-  const double *S[kCcfSpec];
+  const GLOBAL double *S[kCcfSpec];
 #pragma unroll
   for (int k = 0; k < kCcfSpec; k++) S[k] = spec + (long)(s0 + (k < ns ? k : 0)) * spec_stride + trim;
   for (int iv = threadIdx.x; iv < n_v; iv += blockDim.x) {
@@ -1342,7 +1383,9 @@ __device__ __forceinline__ void ccf_body(const double *spec, int64_t spec_stride
       long e = (long)q * n_v + iv;
       int st = m_start[e];
       uint32_t pk = m_pk[e];
-      double2 wv = m_wv[e];
+      double2 wv;
+      wv.x = m_wv[2 * e];
+      wv.y = m_wv[2 * e + 1];
       double w = w_sorted[q];
       int ln = (int)(pk & 0xffu);
       uint32_t al = pk >> 8;
@@ -1364,7 +1407,9 @@ __global__ void k_ccf(const double *spec, int64_t spec_stride, int64_t trim, int
   ccf_body(spec, spec_stride, trim, n_spec, n_lines, n_v, n_t, m_start, m_pk, m_wv, w_sorted, part, blockIdx.x, blockIdx.y);
 }
 
-__device__ __forceinline__ void ccf_reduce_body(const double *part, int n_chunks, int n_spec, int n_v, const double *appod, double *ccf, int bx_, int by_) {
+__device__ __forceinline__ void ccf_reduce_body(const double *part_, int n_chunks, int n_spec, int n_v, const double *appod_, double *ccf_, int bx_, int by_) {
+  const GLOBAL double *part = gl(part_), *appod = gl(appod_);
+  GLOBAL double *ccf = gw(ccf_);
   __shared__ double s_red[1024];
   int sp = bx_;
   double v = -HUGE_VAL;
@@ -1523,13 +1568,16 @@ __device__ int solve4(const double *A, const double *D, double mu, const double 
 // BIS_FIT2 (BIS_FIT2.cpp:97-320) for one CCF per 64-lane block: the same
 // Levenberg-Marquardt restatement of GSL's trust-region solver as the oracle
 // (More scaling, Nielsen damping, xtol = gtol = 1e-8), then the bisector.
-__device__ __forceinline__ void fit_body(const double *v, int n, const double *Y, double *par, double *model,
-                                         double *depth_out, double *bis_out, int32_t *iters, int bx_) {
+__device__ __forceinline__ void fit_body(const double *v_, int n, const double *Y_, double *par_, double *model_,
+                                         double *depth_out_, double *bis_out_, int32_t *iters_, int bx_) {
+  const GLOBAL double *v = gl(v_), *Y = gl(Y_);
+  GLOBAL double *par = gw(par_), *model = gw(model_), *depth_out = gw(depth_out_), *bis_out = gw(bis_out_);
+  GLOBAL int32_t *iters = gw(iters_);
   __shared__ double s_px[kMaxV], s_y[kMaxV], s_nc[kMaxV], s_p0[kMaxV], s_p1[kMaxV], s_p2[kMaxV];
   __shared__ double s_bis[91];
   __shared__ double s_init[4];
   int cidx = bx_;
-  const double *y = Y + (long)cidx * n;
+  const GLOBAL double *y = Y + (long)cidx * n;
   for (int i = threadIdx.x; i < n; i += 64) {
     s_px[i] = v[i] - v[0];
     s_y[i] = y[i];
@@ -1680,7 +1728,7 @@ __device__ __forceinline__ void fit_body(const double *v, int n, const double *Y
     }
     top /= n1;
     bot /= n2;
-    double *pp = par + (long)cidx * 5;
+    GLOBAL double *pp = par + (long)cidx * 5;
     pp[0] = c;
     pp[1] = fabs(k);
     pp[2] = top - bot;
@@ -1747,7 +1795,8 @@ __global__ __launch_bounds__(kTPB) void kb_fill(SOAP_BATCH_ARGS(FillJob)) {
   batch_loop<P>(jobs, prefix, n_inst, total, [&](const FillJob &j, int local) {
     int64_t e0 = (int64_t)local * kBatchFillChunk;
     int64_t e1 = min(j.count, e0 + kBatchFillChunk);
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) j.p[e] = 0ull;
+    GLOBAL unsigned long long *p = gw(j.p);
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) p[e] = 0ull;
   });
 }
 
