@@ -358,26 +358,44 @@ __global__ void k_diff_gradF(DiffArgs a, const double *target, int trim, double 
   GF[e] = (i >= trim && i < a.n - trim) ? 2 * w_spec * (a.F[e] - target[e]) : 0.0;
 }
 
-// transposed mask: G_F[p][trim + pixel] += gy[p][v] * weight over the live pixels of
-// every (line, velocity) entry (the pixel weights of k_ccf)
-__global__ void k_mask_transpose(int n_lines, int n_v, int64_t n_t, int trim, int64_t n, int n_phase,
-                                 const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
-                                 const double *w_sorted, const double *gy, double *GF) {
-  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (long)n_lines * n_v) return;
-  int q = (int)(e / n_v), iv = (int)(e % n_v);
-  int st = m_start[e];
-  uint32_t pk = m_pk[e];
-  double2 wv = m_wv[e];
-  double w = w_sorted[q];
-  int ln = (int)(pk & 0xffu);
-  uint32_t al = pk >> 8;
-  for (int b = 0; b < ln; b++) {
-    long pp = st + b;
-    if (!((al >> b) & 1u) || pp < 0 || pp >= n_t) continue;
-    double val = (b == ln - 1) ? wv.y : ((b == 0) ? wv.x : w);
-    for (int p = 0; p < n_phase; p++) atomicAdd(GF + (long)p * n + trim + pp, gy[(long)p * n_v + iv] * val);
+// transposed mask as a gather: G_F[p][trim + t] += sum_e gy[p][col_e] val_e over the
+// entries e of row t of the pixel-major table (the (velocity, weight) pairs of
+// every live mask pixel of k_ccf, merged per pixel).  The table is sliced by 64
+// rows (one wave): entry k of row t sits at slice_ptr[t / 64] + 64 k + t % 64,
+// rows padded with zero weights to their slice's width, so every load is
+// coalesced.  Block = (kDT pixels, chunk of kMtP phases), the chunk's gy rows in
+// LDS, one owner per output: no atomics.
+constexpr int kMtP = 8;
+__global__ __launch_bounds__(kDT) void k_mask_transpose(int64_t n_t, int trim, int64_t n, int n_phase, int n_v,
+                                                        const int64_t *slice_ptr, const int32_t *slice_w,
+                                                        const int32_t *col, const double *val, const double *gy,
+                                                        double *GF) {
+  extern __shared__ double sg[];  // [kMtP][n_v]
+  const int p0 = blockIdx.y * kMtP;
+  const int np = min(kMtP, n_phase - p0);
+  for (int e = threadIdx.x; e < kMtP * n_v; e += blockDim.x) {
+    int c = e / n_v;
+    sg[e] = c < np ? gy[(long)(p0 + c) * n_v + e % n_v] : 0.0;
   }
+  __syncthreads();
+  const long t = (long)blockIdx.x * kDT + threadIdx.x;
+  const long sl = t >> 6;
+  if (sl * 64 >= n_t) return;  // whole wave past the end (n_t rounds up to slices)
+  const int W = slice_w[sl];
+  const int64_t base = slice_ptr[sl] + (t & 63);
+  double acc[kMtP];
+#pragma unroll
+  for (int c = 0; c < kMtP; c++) acc[c] = 0.0;
+  for (int k = 0; k < W; k++) {
+    const int v = col[base + 64L * k];
+    const double w = val[base + 64L * k];
+#pragma unroll
+    for (int c = 0; c < kMtP; c++) acc[c] = fma(sg[c * n_v + v], w, acc[c]);
+  }
+  if (t >= n_t) return;
+#pragma unroll
+  for (int c = 0; c < kMtP; c++)
+    if (c < np) GF[(long)(p0 + c) * n + trim + t] += acc[c];
 }
 
 // G_I[m] = sum_k c_k G_F[m - dlo - k] over the valid outputs (G_F is zero elsewhere)
@@ -410,38 +428,53 @@ __global__ __launch_bounds__(kDT) void k_diff_convT(DiffArgs a, ConvArgs c, cons
 }
 
 // lag correlations C[p][k][d][set] += sum_{j in chunk} G_I[p][j] X_k[j - dmin - d];
-// block = (chunk of kCorrT pixels, bin k, phase p); thread = (d, set) pair
-constexpr int kCorrT = 1024;
+// block = (chunk of kCorrT pixels, bin k, phase p).  Each thread keeps the G_I
+// values of its kCorrT / kDT pixels in registers; for every (d, set) pair the
+// block sums the products over its pixels (wave shuffles, then the 4 waves),
+// and one thread adds the chunk's sum to C.
+constexpr int kCorrT = 2048;
+constexpr int kCorrPer = kCorrT / kDT;
 __global__ __launch_bounds__(kDT) void k_diff_corr(DiffArgs a, const double *GI, double *C) {
-  extern __shared__ double lds[];  // G [kCorrT], Q and S rows [kCorrT + nd - 1] each
+  extern __shared__ double lds[];  // Q and S rows [kCorrT + nd - 1] each, then [4 waves][2 nd] partials
   int k = blockIdx.y, p = blockIdx.z;
   long j0 = (long)blockIdx.x * kCorrT;
   int nd = a.nd, rl = kCorrT + nd - 1;
-  double *G = lds, *rq = lds + kCorrT, *rs = rq + rl;
+  double *rq = lds, *rs = rq + rl, *wp = rs + rl;
   long src0 = j0 - a.dmin - nd + 1;
-  for (int e = threadIdx.x; e < kCorrT; e += blockDim.x) {
-    long j = j0 + e;
-    G[e] = j < a.n ? GI[(long)p * a.n + j] : 0.0;
-  }
   for (int e = threadIdx.x; e < rl; e += blockDim.x) {
     long i = src0 + e;
     bool in = i >= 0 && i < a.n;
     rq[e] = in ? a.q[(long)k * a.n + i] : 0.0;
     rs[e] = in ? a.s[(long)k * a.n + i] : 0.0;
   }
+  double g[kCorrPer];
+#pragma unroll
+  for (int u = 0; u < kCorrPer; u++) {
+    long j = j0 + threadIdx.x + u * kDT;
+    g[u] = j < a.n ? GI[(long)p * a.n + j] : 0.0;
+  }
   __syncthreads();
-  for (int pr = threadIdx.x; pr < 2 * nd; pr += blockDim.x) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int pr = 0; pr < 2 * nd; pr++) {
     int d = pr >> 1, set = pr & 1;
     const double *r = (set ? rs : rq) + nd - 1 - d;  // X[j0 + e - dmin - d] = r[e]
-    double acc = 0.0;
-    for (int e = 0; e < kCorrT; e++) acc = fma(G[e], r[e], acc);
-    atomicAdd(C + (((long)p * a.n_mu + k) * nd + d) * 2 + set, acc);
+    double x = 0.0;
+#pragma unroll
+    for (int u = 0; u < kCorrPer; u++) x = fma(g[u], r[threadIdx.x + u * kDT], x);
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if (lane == 0) wp[wv * 2 * nd + pr] = x;
+  }
+  __syncthreads();
+  for (int pr = threadIdx.x; pr < 2 * nd; pr += blockDim.x) {
+    double x = 0.0;
+    for (int w = 0; w < kDT / 64; w++) x += wp[w * 2 * nd + pr];
+    atomicAdd(C + (((long)p * a.n_mu + k) * nd + (pr >> 1)) * 2 + (pr & 1), x);
   }
 }
 
-// per-cell derivatives from the correlations; phases summed per cell; map
-// pixel gradients scattered; scalar partials per block: [gvsini, gu1, gu2, gkappa]
-__global__ __launch_bounds__(kDT) void k_diff_cellgrad(DiffArgs a, const double *C, double *gmaps, double *part) {
+// per-cell derivatives from the correlations, phases summed per cell; scalar
+// partials per block: [gvsini, gu1, gu2, gkappa]
+__global__ __launch_bounds__(kDT) void k_diff_cellgrad(DiffArgs a, const double *C, double *part) {
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long ncell = (long)a.grid * a.grid;
   double sv = 0, su1 = 0, su2 = 0, sk = 0;
@@ -462,12 +495,6 @@ __global__ __launch_bounds__(kDT) void k_diff_cellgrad(DiffArgs a, const double 
         int i0, j0;
         double fy, fz;
         double phi = map_sample(a.maps + (long)p * M * M, M, y, z, i0, j0, fy, fz);
-        double gphi = L * (a.kappa * as - aq);
-        double *gm = gmaps + (long)p * M * M;
-        atomicAdd(gm + i0 * M + j0, gphi * (1 - fy) * (1 - fz));
-        atomicAdd(gm + (i0 + 1) * M + j0, gphi * fy * (1 - fz));
-        atomicAdd(gm + i0 * M + j0 + 1, gphi * (1 - fy) * fz);
-        atomicAdd(gm + (i0 + 1) * M + j0 + 1, gphi * fy * fz);
         gL += (1 - phi) * aq + a.kappa * phi * as;
         gt += L * ((1 - phi) * bq + a.kappa * phi * bs);
         sk += L * phi * as;
@@ -490,6 +517,55 @@ __global__ __launch_bounds__(kDT) void k_diff_cellgrad(DiffArgs a, const double 
     for (int w = 0; w < kDT / 64; w++) s += s_p[threadIdx.x][w];
     part[(long)blockIdx.x * 4 + threadIdx.x] = s;
   }
+}
+
+// map-pixel gradients as a gather (no atomics): node (i, j) of phase p sums, over
+// the cells whose bilinear stencil in map_sample holds it, weight x L (kappa a_S -
+// a_Q), with a_Q, a_S the cell's interpolated correlations.  A cell at map
+// coordinate gy = iy (M - 1) / G touches nodes floor(gy) and floor(gy) + 1, so
+// node i looks at the rows with floor(gy) in {i - 1, i} (a superset is scanned
+// and filtered with the same arithmetic as map_sample).
+__global__ __launch_bounds__(kDT) void k_diff_mapgrad(DiffArgs a, const double *C, double *gmaps) {
+#pragma clang fp contract(off)
+  const int M = a.map_res, G = a.grid;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;  // map column
+  const int i = blockIdx.y;                              // map row
+  const int p = blockIdx.z;
+  if (j >= M) return;
+  const double sc = (double)G / (M - 1), dg = 2. / G;
+  const int y_lo = max(0, (int)floor((i - 1) * sc) - 1), y_hi = min(G - 1, (int)ceil((i + 1) * sc) + 1);
+  const int z_lo = max(0, (int)floor((j - 1) * sc) - 1), z_hi = min(G - 1, (int)ceil((j + 1) * sc) + 1);
+  double acc = 0.0;
+  for (int iy = y_lo; iy <= y_hi; iy++) {
+    // the row half of map_sample: gy, its clamped floor and fraction
+    double y = -1. + iy * dg;
+    double gy = (y + 1) * 0.5 * (M - 1);
+    int i0 = (int)floor(gy);
+    i0 = i0 > M - 2 ? M - 2 : (i0 < 0 ? 0 : i0);
+    if (i0 != i && i0 + 1 != i) continue;
+    double fy = gy - i0;
+    double wy = i0 == i ? 1 - fy : fy;
+    for (int iz = z_lo; iz <= z_hi; iz++) {
+      double z = -1. + iz * dg;
+      double gz = (z + 1) * 0.5 * (M - 1);
+      int j0 = (int)floor(gz);
+      j0 = j0 > M - 2 ? M - 2 : (j0 < 0 ? 0 : j0);
+      if (j0 != j && j0 + 1 != j) continue;
+      long idx = (long)iy * G + iz;
+      int k = a.ck[idx];
+      if (k < 0) continue;
+      int d0 = a.cm[idx] - a.dmin;
+      if (d0 < 1 || d0 >= a.nd) continue;
+      double fz = gz - j0;
+      double wz = j0 == j ? 1 - fz : fz;
+      const double *Cp = C + (((long)p * a.n_mu + k) * a.nd) * 2;
+      double t = a.ct[idx];
+      double aq = (1 - t) * Cp[d0 * 2] + t * Cp[(d0 - 1) * 2];
+      double as = (1 - t) * Cp[d0 * 2 + 1] + t * Cp[(d0 - 1) * 2 + 1];
+      acc += wy * wz * (a.cL[idx] * (a.kappa * as - aq));
+    }
+  }
+  gmaps[((long)p * M + i) * M + j] = acc;
 }
 
 // seed gradients gX_k[i] = sum_p sum_d HX_pk[d] G_I,p[i + dmin + d]; block = (tile, bin)
@@ -565,8 +641,8 @@ hipError_t launch_fit_polish(const double *v, int n_v, const double *ccf, const 
 
 hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const double *target, int trim, double w_spec,
                                 const double *v, int n_v, const double *ccf, const double *theta, const double *dth,
-                                double *gy, int n_lines, int64_t n_t, const int32_t *m_start, const uint32_t *m_pk,
-                                const double2 *m_wv, const double *w_sorted, double *GF, double *GI, double *C,
+                                double *gy, int64_t n_t, const int64_t *mt_ptr, const int32_t *mt_w,
+                                const int32_t *mt_col, const double *mt_val, double *GF, double *GI, double *C,
                                 double *gmaps, double *cell_part, double *gq, double *gs, hipStream_t s) {
   hipError_t e;
   k_fit_adjoint<<<a.n_phase, 64, 0, s>>>(v, n_v, ccf, theta, dth, gy);
@@ -574,10 +650,11 @@ hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const doub
   long tot = (long)a.n_phase * a.n;
   k_diff_gradF<<<(unsigned)((tot + kDT - 1) / kDT), kDT, 0, s>>>(a, target, trim, w_spec, GF);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  long ne = (long)n_lines * n_v;
-  if (ne > 0) {
-    k_mask_transpose<<<(unsigned)((ne + kDT - 1) / kDT), kDT, 0, s>>>(n_lines, n_v, n_t, trim, a.n, a.n_phase,
-                                                                     m_start, m_pk, m_wv, w_sorted, gy, GF);
+  if (n_t > 0) {
+    size_t lds_mt = (size_t)kMtP * n_v * sizeof(double);
+    if (lds_mt > 65536) return hipErrorInvalidValue;
+    dim3 gm((unsigned)((n_t + kDT - 1) / kDT), (a.n_phase + kMtP - 1) / kMtP);
+    k_mask_transpose<<<gm, kDT, lds_mt, s>>>(n_t, trim, a.n, a.n_phase, n_v, mt_ptr, mt_w, mt_col, mt_val, gy, GF);
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   size_t lds_conv = (size_t)(4 * kDT + 2 * c.ntp) * sizeof(double);
@@ -587,15 +664,16 @@ hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const doub
   if ((e = hipGetLastError()) != hipSuccess) return e;
   e = launch_fill_f64(C, (size_t)a.n_phase * a.n_mu * a.nd * 2, 0.0, s);
   if (e != hipSuccess) return e;
-  size_t lds_corr = (size_t)(kCorrT + 2 * (kCorrT + a.nd - 1)) * sizeof(double);
+  size_t lds_corr = (size_t)(2 * (kCorrT + a.nd - 1) + (kDT / 64) * 2 * a.nd) * sizeof(double);
   if (lds_corr > 65536) return hipErrorInvalidValue;
   dim3 gr((unsigned)((a.n + kCorrT - 1) / kCorrT), a.n_mu, a.n_phase);
   k_diff_corr<<<gr, kDT, lds_corr, s>>>(a, GI, C);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  e = launch_fill_f64(gmaps, (size_t)a.n_phase * a.map_res * a.map_res, 0.0, s);
-  if (e != hipSuccess) return e;
   long ncell = (long)a.grid * a.grid;
-  k_diff_cellgrad<<<(unsigned)((ncell + kDT - 1) / kDT), kDT, 0, s>>>(a, C, gmaps, cell_part);
+  k_diff_cellgrad<<<(unsigned)((ncell + kDT - 1) / kDT), kDT, 0, s>>>(a, C, cell_part);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  dim3 gmg((unsigned)((a.map_res + kDT - 1) / kDT), a.map_res, a.n_phase);
+  k_diff_mapgrad<<<gmg, kDT, 0, s>>>(a, C, gmaps);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   size_t lds_seed = (size_t)(kDT + a.nd - 1) * sizeof(double);
   dim3 gsd((unsigned)((a.n + kDT - 1) / kDT), a.n_mu);

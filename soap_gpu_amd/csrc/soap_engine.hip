@@ -97,6 +97,8 @@ struct soap_ctx {
     std::vector<double> rv_t, fw_t, theta;
     DevBuf ck, cm, ct, cL, cmu, cdt, acc, taps, krange, I, F, res_part, ccf, par, model, depth, bis, iters, th,
         dth, gy, GF, GI, C, cell_part;
+    DevBuf mt_ptr, mt_w, mt_col, mt_val;  // the mask in sliced pixel-major form (adjoint of the CCF)
+    bool mt_ready = false;
   } diff;
   // linear grids: staged source range per tile, bin-sorted disk cells, active cell lists
   int lin_off_lo = 0, lin_span = 0;
@@ -1457,8 +1459,102 @@ static int build_mask(soap_ctx *c) {
                                   (const int32_t *)c->m_start.p, (const int32_t *)c->m_len.p, (uint32_t *)c->m_alive.p,
                                   c->stream));
   c->mask_built = true;
+  c->diff.mt_ready = false;
   return SOAP_OK;
 }
+
+// Pixel-major form of the built mask for the CCF adjoint: for every trimmed
+// pixel t, the (velocity, weight) pairs of the live mask pixels that k_ccf
+// multiplies with spectrum pixel t, pairs with the same velocity merged in line
+// order; stored in slices of 64 rows (one wave), column-major inside a slice.
+// Built on the host once per mask.
+static int diff_mask_transpose(soap_ctx *c) {
+  auto &d = c->diff;
+  if (d.mt_ready) return SOAP_OK;
+  const int nl = c->n_lines, nv = c->n_v;
+  const int64_t n_t = c->n - 2 * (int64_t)c->trim;
+  const size_t ne = (size_t)nl * nv;
+  std::vector<int32_t> st(ne);
+  std::vector<uint32_t> pk(ne);
+  std::vector<double> wv(2 * ne), ws(nl);
+  HIPCHK(c, hipMemcpyAsync(st.data(), c->m_start.p, ne * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(pk.data(), c->m_alive.p, ne * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(wv.data(), c->m_val.p, ne * 16, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ws.data(), c->d_w_sorted, (size_t)nl * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // the live (pixel, velocity, weight) triples, in line order, as k_ccf visits them
+  auto visit = [&](auto &&f) {
+    for (int q = 0; q < nl; q++)
+      for (int iv = 0; iv < nv; iv++) {
+        size_t e = (size_t)q * nv + iv;
+        int ln = (int)(pk[e] & 0xffu);
+        uint32_t al = pk[e] >> 8;
+        for (int b = 0; b < ln; b++) {
+          int64_t pp = (int64_t)st[e] + b;
+          if (!((al >> b) & 1u) || pp < 0 || pp >= n_t) continue;
+          f(pp, iv, b == ln - 1 ? wv[2 * e + 1] : (b == 0 ? wv[2 * e] : ws[q]));
+        }
+      }
+  };
+  std::vector<int64_t> cnt(n_t + 1, 0);
+  visit([&](int64_t pp, int, double) { cnt[pp + 1]++; });
+  for (int64_t t = 0; t < n_t; t++) cnt[t + 1] += cnt[t];
+  if (cnt[n_t] > 0x7fffffffL) return fail(c, SOAP_ERR_LIMIT, "diff: mask too large for the adjoint table");
+  std::vector<int32_t> col((size_t)cnt[n_t]);
+  std::vector<double> val((size_t)cnt[n_t]);
+  std::vector<int64_t> fillp(cnt.begin(), cnt.end() - 1);
+  visit([&](int64_t pp, int iv, double w) {
+    col[fillp[pp]] = iv;
+    val[fillp[pp]++] = w;
+  });
+  // merge equal velocities within each row (stable: line order kept per velocity)
+  std::vector<std::vector<std::pair<int32_t, double>>> rows(n_t);
+  std::vector<std::pair<int32_t, int64_t>> idx;
+  for (int64_t t = 0; t < n_t; t++) {
+    idx.clear();
+    for (int64_t e = cnt[t]; e < cnt[t + 1]; e++) idx.push_back({col[e], e});
+    std::stable_sort(idx.begin(), idx.end(), [](const auto &x, const auto &y) { return x.first < y.first; });
+    auto &r = rows[t];
+    for (size_t k = 0; k < idx.size(); k++) {
+      if (k > 0 && idx[k].first == idx[k - 1].first)
+        r.back().second += val[idx[k].second];
+      else
+        r.push_back({idx[k].first, val[idx[k].second]});
+    }
+  }
+  // slices of 64 rows, column-major within a slice, zero-padded to the slice's widest row
+  const int64_t ns = (n_t + 63) / 64;
+  std::vector<int64_t> sp(ns + 1, 0);
+  std::vector<int32_t> sw(ns, 0);
+  for (int64_t sl = 0; sl < ns; sl++) {
+    int w = 0;
+    for (int64_t t = sl * 64; t < std::min(n_t, sl * 64 + 64); t++) w = std::max(w, (int)rows[t].size());
+    sw[sl] = w;
+    sp[sl + 1] = sp[sl] + 64L * w;
+  }
+  std::vector<int32_t> mcol((size_t)std::max<int64_t>(1, sp[ns]), 0);
+  std::vector<double> mval((size_t)std::max<int64_t>(1, sp[ns]), 0.0);
+  for (int64_t t = 0; t < n_t; t++) {
+    int64_t sl = t / 64;
+    for (size_t k = 0; k < rows[t].size(); k++) {
+      int64_t e = sp[sl] + 64L * (int64_t)k + (t & 63);
+      mcol[e] = rows[t][k].first;
+      mval[e] = rows[t][k].second;
+    }
+  }
+  int rc;
+  if ((rc = ensure(c, d.mt_ptr, sp.size() * 8)) || (rc = ensure(c, d.mt_w, std::max<size_t>(1, sw.size()) * 4)) ||
+      (rc = ensure(c, d.mt_col, mcol.size() * 4)) || (rc = ensure(c, d.mt_val, mval.size() * 8)))
+    return rc;
+  HIPCHK(c, hipMemcpyAsync(d.mt_ptr.p, sp.data(), sp.size() * 8, hipMemcpyHostToDevice, c->stream));
+  if (!sw.empty()) HIPCHK(c, hipMemcpyAsync(d.mt_w.p, sw.data(), sw.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d.mt_col.p, mcol.data(), mcol.size() * 4, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d.mt_val.p, mval.data(), mval.size() * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  d.mt_ready = true;
+  return SOAP_OK;
+}
+
 
 int soap_ccf(soap_ctx *c) {
   if (!c) return SOAP_ERR_ARG;
@@ -1730,15 +1826,17 @@ int soap_diff_backward(soap_ctx *c, double *grad4, double *d_gq, double *d_gs, d
     dth[p * 4 + 3] = 2 * d.w_fw * (d.theta[p * 4 + 3] - d.fw_t[p]);
   }
   HIPCHK(c, hipMemcpyAsync(d.dth.p, dth.data(), dth.size() * 8, hipMemcpyHostToDevice, c->stream));
+  int rc0 = diff_mask_transpose(c);
+  if (rc0) return rc0;
   DiffArgs a = diff_args(c);
   ConvArgs ca = diff_conv(c);
   {
     StageTimer tm(c, ST_CONV);
     hipError_t e = launch_diff_backward(
         a, ca, d.target, c->trim, d.w_spec, c->d_vgrid, c->n_v, (const double *)d.ccf.p, (const double *)d.th.p,
-        (const double *)d.dth.p, (double *)d.gy.p, c->n_lines, c->n - 2 * (int64_t)c->trim,
-        (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p, c->d_w_sorted,
-        (double *)d.GF.p, (double *)d.GI.p, (double *)d.C.p, d_gmaps, (double *)d.cell_part.p, d_gq, d_gs, c->stream);
+        (const double *)d.dth.p, (double *)d.gy.p, c->n - 2 * (int64_t)c->trim, (const int64_t *)d.mt_ptr.p,
+        (const int32_t *)d.mt_w.p, (const int32_t *)d.mt_col.p, (const double *)d.mt_val.p, (double *)d.GF.p,
+        (double *)d.GI.p, (double *)d.C.p, d_gmaps, (double *)d.cell_part.p, d_gq, d_gs, c->stream);
     if (e == hipErrorInvalidValue) return fail(c, SOAP_ERR_LIMIT, "diff_backward: window or shift range too wide");
     HIPCHK(c, e);
   }

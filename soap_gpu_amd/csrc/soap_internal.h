@@ -302,10 +302,14 @@ int diff_conv_blocks(int64_t n);
 int diff_cell_blocks(int64_t ncell);
 hipError_t launch_fit_polish(const double *v, int n_v, const double *ccf, const double *par, int n_ccf, double *theta,
                              hipStream_t s);
+// mt_*: the mask in pixel-major form, sliced by 64 rows (row t = trimmed pixel
+// t; columns = velocities; values = the k_ccf pixel weights, merged per (pixel,
+// velocity)): entry k of row t at mt_ptr[t / 64] + 64 k + t % 64, mt_w[t / 64]
+// entries per row of the slice (zero-padded)
 hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const double *target, int trim, double w_spec,
                                 const double *v, int n_v, const double *ccf, const double *theta, const double *dth,
-                                double *gy, int n_lines, int64_t n_t, const int32_t *m_start, const uint32_t *m_pk,
-                                const double2 *m_wv, const double *w_sorted, double *GF, double *GI, double *C,
+                                double *gy, int64_t n_t, const int64_t *mt_ptr, const int32_t *mt_w,
+                                const int32_t *mt_col, const double *mt_val, double *GF, double *GI, double *C,
                                 double *gmaps, double *cell_part, double *gq, double *gs, hipStream_t s);
 hipError_t launch_ccf_sum(const double *part, int nc, int n_spec, int n_v, double *ccf, hipStream_t s);
 // batched instances: stage st over jobs[n_inst] (device), prefix[n_inst + 1]
