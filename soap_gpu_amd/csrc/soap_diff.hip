@@ -427,50 +427,85 @@ __global__ __launch_bounds__(kDT) void k_diff_convT(DiffArgs a, ConvArgs c, cons
   }
 }
 
-// lag correlations C[p][k][d][set] += sum_{j in chunk} G_I[p][j] X_k[j - dmin - d];
-// block = (chunk of kCorrT pixels, bin k, phase p).  Each thread keeps the G_I
-// values of its kCorrT / kDT pixels in registers; for every (d, set) pair the
-// block sums the products over its pixels (wave shuffles, then the 4 waves),
-// and one thread adds the chunk's sum to C.
-constexpr int kCorrT = 2048;
-constexpr int kCorrPer = kCorrT / kDT;
-__global__ __launch_bounds__(kDT) void k_diff_corr(DiffArgs a, const double *GI, double *C) {
-  extern __shared__ double lds[];  // Q and S rows [kCorrT + nd - 1] each, then [4 waves][2 nd] partials
-  int k = blockIdx.y, p = blockIdx.z;
-  long j0 = (long)blockIdx.x * kCorrT;
-  int nd = a.nd, rl = kCorrT + nd - 1;
-  double *rq = lds, *rs = rq + rl, *wp = rs + rl;
-  long src0 = j0 - a.dmin - nd + 1;
+// lag correlations C[p][k][d][set] = sum_j G_I[p][j] X_k[j - dmin - d], in two
+// passes.  Pass 1: block = (chunk of kCorrT pixels, bin k, group of kCorrPG
+// phases); the bin's two seed-row segments are staged once for the group.  Each
+// thread owns kCorrPer consecutive pixels (G_I of the phase in registers) and,
+// per group of kCorrL lags, reads one register window of the staged row and
+// forms kCorrL partial sums; the block adds them up through LDS ([lag][thread]
+// then 32-lane shuffles) into the chunk's partial.  Pass 2 sums the chunk
+// partials in chunk order (deterministic, no atomics).
+constexpr int kCorrPer = 9;  // odd: a thread stride of kCorrPer doubles is free of LDS bank conflicts
+constexpr int kCorrT = kDT * kCorrPer;
+constexpr int kCorrL = 8;
+constexpr int kCorrPG = 10;
+__global__ __launch_bounds__(kDT) void k_diff_corr(DiffArgs a, const double *GI, double *Cpart) {
+  extern __shared__ double lds[];  // Q and S rows [kCorrT + nd - 1 + kCorrL] each, then [kCorrL][kDT] partials
+  const int k = blockIdx.y, chunk = blockIdx.x, nchunk = gridDim.x;
+  const long j0 = (long)chunk * kCorrT;
+  const int nd = a.nd, rl = kCorrT + nd - 1 + kCorrL;
+  double *rq = lds, *rs = rq + rl, *red = rs + rl;
+  const long src0 = j0 - a.dmin - nd + 1;
   for (int e = threadIdx.x; e < rl; e += blockDim.x) {
     long i = src0 + e;
-    bool in = i >= 0 && i < a.n;
+    bool in = i >= 0 && i < a.n && e < kCorrT + nd - 1;
     rq[e] = in ? a.q[(long)k * a.n + i] : 0.0;
     rs[e] = in ? a.s[(long)k * a.n + i] : 0.0;
   }
-  double g[kCorrPer];
-#pragma unroll
-  for (int u = 0; u < kCorrPer; u++) {
-    long j = j0 + threadIdx.x + u * kDT;
-    g[u] = j < a.n ? GI[(long)p * a.n + j] : 0.0;
-  }
   __syncthreads();
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int pr = 0; pr < 2 * nd; pr++) {
-    int d = pr >> 1, set = pr & 1;
-    const double *r = (set ? rs : rq) + nd - 1 - d;  // X[j0 + e - dmin - d] = r[e]
-    double x = 0.0;
+  const int p1 = min(a.n_phase, (int)(blockIdx.z + 1) * kCorrPG);
+  for (int p = blockIdx.z * kCorrPG; p < p1; p++) {
+    double g[kCorrPer];
 #pragma unroll
-    for (int u = 0; u < kCorrPer; u++) x = fma(g[u], r[threadIdx.x + u * kDT], x);
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-    if (lane == 0) wp[wv * 2 * nd + pr] = x;
-  }
-  __syncthreads();
-  for (int pr = threadIdx.x; pr < 2 * nd; pr += blockDim.x) {
-    double x = 0.0;
-    for (int w = 0; w < kDT / 64; w++) x += wp[w * 2 * nd + pr];
-    atomicAdd(C + (((long)p * a.n_mu + k) * nd + (pr >> 1)) * 2 + (pr & 1), x);
+    for (int u = 0; u < kCorrPer; u++) {
+      long j = j0 + threadIdx.x * kCorrPer + u;
+      g[u] = j < a.n ? GI[(long)p * a.n + j] : 0.0;
+    }
+    double *out = Cpart + (((long)p * a.n_mu + k) * nchunk + chunk) * nd * 2;
+    for (int set = 0; set < 2; set++) {
+      const double *row = (set ? rs : rq) + threadIdx.x * kCorrPer;
+      for (int d0 = 0; d0 < nd; d0 += kCorrL) {
+        // lag d = d0 + l reads X[j - dmin - d] = row[u + nd - 1 - d]; window base nd - d0 - kCorrL
+        const int wb = nd - d0 - kCorrL;
+        double w[kCorrPer + kCorrL - 1];
+#pragma unroll
+        for (int e = 0; e < kCorrPer + kCorrL - 1; e++) {
+          int x = wb + e;
+          w[e] = x >= 0 ? row[x] : 0.0;  // x < 0 only for lags d >= nd (discarded)
+        }
+        double acc[kCorrL];
+#pragma unroll
+        for (int l = 0; l < kCorrL; l++) {
+          acc[l] = 0.0;
+#pragma unroll
+          for (int u = 0; u < kCorrPer; u++) acc[l] = fma(g[u], w[u + kCorrL - 1 - l], acc[l]);
+        }
+        __syncthreads();  // red is free
+#pragma unroll
+        for (int l = 0; l < kCorrL; l++) red[l * kDT + threadIdx.x] = acc[l];
+        __syncthreads();
+        // kDT threads = kCorrL lags x 32 lanes: each lane sums kDT / 32 partials
+        const int l = threadIdx.x >> 5, part = threadIdx.x & 31;
+        double x = 0.0;
+#pragma unroll
+        for (int r = 0; r < kDT / 32; r++) x += red[l * kDT + r * 32 + part];
+        for (int off = 16; off > 0; off >>= 1) x += __shfl_xor(x, off);
+        if (part == 0 && d0 + l < nd) out[(d0 + l) * 2 + set] = x;
+      }
+    }
   }
 }
+
+__global__ void k_diff_corr_reduce(int n_pk, int nd, int nchunk, const double *Cpart, double *C) {
+  long e = (long)blockIdx.x * blockDim.x + threadIdx.x;  // (p, k) x (d, set)
+  if (e >= (long)n_pk * nd * 2) return;
+  long pk = e / (nd * 2), r = e % (nd * 2);
+  double x = 0.0;
+  for (int c = 0; c < nchunk; c++) x += Cpart[(pk * nchunk + c) * nd * 2 + r];
+  C[e] = x;
+}
+
+int diff_corr_chunks(int64_t n) { return (int)((n + kCorrT - 1) / kCorrT); }
 
 // per-cell derivatives from the correlations, phases summed per cell; scalar
 // partials per block: [gvsini, gu1, gu2, gkappa]
@@ -642,8 +677,8 @@ hipError_t launch_fit_polish(const double *v, int n_v, const double *ccf, const 
 hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const double *target, int trim, double w_spec,
                                 const double *v, int n_v, const double *ccf, const double *theta, const double *dth,
                                 double *gy, int64_t n_t, const int64_t *mt_ptr, const int32_t *mt_w,
-                                const int32_t *mt_col, const double *mt_val, double *GF, double *GI, double *C,
-                                double *gmaps, double *cell_part, double *gq, double *gs, hipStream_t s) {
+                                const int32_t *mt_col, const double *mt_val, double *GF, double *GI, double *Cpart,
+                                double *C, double *gmaps, double *cell_part, double *gq, double *gs, hipStream_t s) {
   hipError_t e;
   k_fit_adjoint<<<a.n_phase, 64, 0, s>>>(v, n_v, ccf, theta, dth, gy);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -662,12 +697,14 @@ hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const doub
   dim3 gc((unsigned)((a.n + 4 * kDT - 1) / (4 * kDT)), a.n_phase);
   k_diff_convT<<<gc, kDT, lds_conv, s>>>(a, c, GF, GI);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  e = launch_fill_f64(C, (size_t)a.n_phase * a.n_mu * a.nd * 2, 0.0, s);
-  if (e != hipSuccess) return e;
-  size_t lds_corr = (size_t)(2 * (kCorrT + a.nd - 1) + (kDT / 64) * 2 * a.nd) * sizeof(double);
+  size_t lds_corr = (size_t)(2 * (kCorrT + a.nd - 1 + kCorrL) + kCorrL * kDT) * sizeof(double);
   if (lds_corr > 65536) return hipErrorInvalidValue;
-  dim3 gr((unsigned)((a.n + kCorrT - 1) / kCorrT), a.n_mu, a.n_phase);
-  k_diff_corr<<<gr, kDT, lds_corr, s>>>(a, GI, C);
+  const int nchunk = diff_corr_chunks(a.n);
+  dim3 gr((unsigned)nchunk, a.n_mu, (a.n_phase + kCorrPG - 1) / kCorrPG);
+  k_diff_corr<<<gr, kDT, lds_corr, s>>>(a, GI, Cpart);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  long nc = (long)a.n_phase * a.n_mu * a.nd * 2;
+  k_diff_corr_reduce<<<(unsigned)((nc + kDT - 1) / kDT), kDT, 0, s>>>(a.n_phase * a.n_mu, a.nd, nchunk, Cpart, C);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   long ncell = (long)a.grid * a.grid;
   k_diff_cellgrad<<<(unsigned)((ncell + kDT - 1) / kDT), kDT, 0, s>>>(a, C, cell_part);

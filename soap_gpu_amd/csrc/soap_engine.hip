@@ -96,7 +96,7 @@ struct soap_ctx {
     const double *q = nullptr, *s = nullptr, *maps = nullptr, *target = nullptr;
     std::vector<double> rv_t, fw_t, theta;
     DevBuf ck, cm, ct, cL, cmu, cdt, acc, taps, krange, I, F, res_part, ccf, par, model, depth, bis, iters, th,
-        dth, gy, GF, GI, C, cell_part;
+        dth, gy, GF, GI, C, Cpart, cell_part;
     DevBuf mt_ptr, mt_w, mt_col, mt_val;  // the mask in sliced pixel-major form (adjoint of the CCF)
     bool mt_ready = false;
   } diff;
@@ -1759,7 +1759,8 @@ int soap_diff_forward(soap_ctx *c, const soap_diff_params *prm, const double *d_
   if ((rc = ensure(c, d.acc, (size_t)d.P * c->n_mu * d.nd * 4 * 8)) ||
       (rc = ensure(c, d.taps, (size_t)d.P * c->n_mu * d.nd * 2 * 8)) ||
       (rc = ensure(c, d.krange, (size_t)d.P * c->n_mu * 2 * 4)) ||
-      (rc = ensure(c, d.C, (size_t)d.P * c->n_mu * d.nd * 2 * 8)))
+      (rc = ensure(c, d.C, (size_t)d.P * c->n_mu * d.nd * 2 * 8)) ||
+      (rc = ensure(c, d.Cpart, (size_t)d.P * c->n_mu * diff_corr_chunks(c->n) * d.nd * 2 * 8)))
     return rc;
   if (!c->mask_built && (rc = build_mask(c))) return rc;
   d.prm = *prm;
@@ -1836,7 +1837,8 @@ int soap_diff_backward(soap_ctx *c, double *grad4, double *d_gq, double *d_gs, d
         a, ca, d.target, c->trim, d.w_spec, c->d_vgrid, c->n_v, (const double *)d.ccf.p, (const double *)d.th.p,
         (const double *)d.dth.p, (double *)d.gy.p, c->n - 2 * (int64_t)c->trim, (const int64_t *)d.mt_ptr.p,
         (const int32_t *)d.mt_w.p, (const int32_t *)d.mt_col.p, (const double *)d.mt_val.p, (double *)d.GF.p,
-        (double *)d.GI.p, (double *)d.C.p, d_gmaps, (double *)d.cell_part.p, d_gq, d_gs, c->stream);
+        (double *)d.GI.p, (double *)d.Cpart.p, (double *)d.C.p, d_gmaps, (double *)d.cell_part.p, d_gq, d_gs,
+        c->stream);
     if (e == hipErrorInvalidValue) return fail(c, SOAP_ERR_LIMIT, "diff_backward: window or shift range too wide");
     HIPCHK(c, e);
   }

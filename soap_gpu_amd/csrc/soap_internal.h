@@ -309,8 +309,10 @@ hipError_t launch_fit_polish(const double *v, int n_v, const double *ccf, const 
 hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const double *target, int trim, double w_spec,
                                 const double *v, int n_v, const double *ccf, const double *theta, const double *dth,
                                 double *gy, int64_t n_t, const int64_t *mt_ptr, const int32_t *mt_w,
-                                const int32_t *mt_col, const double *mt_val, double *GF, double *GI, double *C,
-                                double *gmaps, double *cell_part, double *gq, double *gs, hipStream_t s);
+                                const int32_t *mt_col, const double *mt_val, double *GF, double *GI, double *Cpart,
+                                double *C, double *gmaps, double *cell_part, double *gq, double *gs, hipStream_t s);
+// chunks of the lag-correlation pass (Cpart holds [P][n_mu][chunks][nd][2])
+int diff_corr_chunks(int64_t n);
 hipError_t launch_ccf_sum(const double *part, int nc, int n_spec, int n_v, double *ccf, hipStream_t s);
 // batched instances: stage st over jobs[n_inst] (device), prefix[n_inst + 1]
 // (device, prefix[n_inst] = total); grid = persistent block count (0: one block
