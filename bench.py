@@ -14,7 +14,8 @@ Further legs in the same line: "adjoint" (the differentiable model on a grid
 600 / 100k-pixel / 50-phase / 512x512-map shape: forward and forward + adjoint
 cost) and "batch" (256 uneven instances stepped by one persistent launch per
 stage, eager and as a HIP graph, at autotuned launch shapes, against the plain
-per-instance path).
+per-instance path) and "fit" (L-BFGS over 128 differentiable instances with
+256x256 filling-factor maps: time per forward + adjoint evaluation).
 
 Prints one JSON line.
 """
@@ -305,6 +306,47 @@ def adjoint_leg(steps: int):
                 fd_check_vs_gpu_forward=fd)
 
 
+def fit_leg(n_inst: int, iters: int, workers: int = 8):
+    """Fitting loop on the fitting workload: n_inst uneven differentiable instances
+    (grid 300-500, 30k-80k lambda points, 20-60 phases, 256x256 filling-factor maps
+    of 1-4 blobs), targets = RV and FWHM series of the model at the true
+    (vsini, contrast, maps); L-BFGS from perturbed vsini and contrast and smoothed
+    maps.  Reports the time per objective evaluation (forward + adjoint of every
+    instance) and the loss reduction."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from soap_gpu_amd import synth
+    from soap_gpu_amd.fitting import Instance, fit
+
+    t0 = time.time()
+    prm = synth.config4_params(n_inst=n_inst)
+    with ThreadPoolExecutor(max_workers=workers) as ex:
+        made = list(ex.map(lambda ip: synth.config4_instance(ip[1], 256, ip[0]), list(enumerate(prm))))
+    insts, truth = [], []
+    for (w, maps, tr), p_ in zip(made, prm):
+        P = len(w.phases)
+        start = np.clip(0.5 * maps + 0.25 * maps.mean(), 0.0, 1.0)
+        inst = Instance(w, start, P, 256, vsini0=tr["vsini"] * 1.2, contrast0=0.5 * (tr["contrast"] + 0.3),
+                        weights=(0.0, 1e-2, 1e-4))
+        rv, fw = inst.observe((tr["vsini"], 0.29, 0.34, tr["contrast"]), maps)
+        inst.rv_t, inst.fw_t = rv, fw
+        insts.append(inst)
+        truth.append(tr)
+    del made
+    log(f"[bench:fit] {n_inst} instances set up in {time.time() - t0:.1f} s")
+    r = fit(insts, iters=iters, log=log)
+    vs_err = [abs(float(i.theta[0].detach()) - t["vsini"]) / t["vsini"] for i, t in zip(insts, truth)]
+    n_par = sum(2 + i.z.numel() for i in insts)
+    out = dict(instances=n_inst, iterations=r["iters"], evaluations=r["evals"], seconds=round(r["seconds"], 2),
+               ms_per_evaluation=round(r["ms_per_eval"], 2), loss0=r["loss0"], loss=r["loss"], best=r["best"],
+               parameters=int(n_par), vsini_rel_err_median=float(np.median(vs_err)),
+               note="per-instance launches (one context and stream each) with host round trips per forward and "
+                    "adjoint; L-BFGS over the summed loss (torch.optim.LBFGS, strong Wolfe)")
+    for i in insts:
+        i.close()
+    return out
+
+
 def load_traffic(kernel: str, workload: str):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel calibration)."""
@@ -331,7 +373,9 @@ def main():
     ap.add_argument("--cpu-frac-quiet", type=float, default=0.08, help="pixel fraction of the quiet-star CPU sample")
     ap.add_argument("--batch-instances", type=int, default=256, help="instances of the batched leg (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost leg")
-    ap.add_argument("--legs", default="series,adjoint,batch", help="comma list of legs to run (series always prints)")
+    ap.add_argument("--legs", default="series,adjoint,batch,fit", help="comma list of legs to run")
+    ap.add_argument("--fit-instances", type=int, default=128)
+    ap.add_argument("--fit-iters", type=int, default=50)
     args = ap.parse_args()
     legs = set(args.legs.split(","))
     if args.gpus != 1:
@@ -350,6 +394,8 @@ def main():
             out["adjoint"] = adjoint_leg(max(3, min(args.steps, 10)))
         if "batch" in legs:
             out["batch"] = batch_leg(args.batch_instances, max(2, min(args.steps, 5)), 1)
+        if "fit" in legs:
+            out["fit"] = fit_leg(args.fit_instances, args.fit_iters)
         print(json.dumps(out), flush=True)
         return
 
@@ -463,6 +509,12 @@ def main():
         except Exception as ex:
             out["batch"] = {"error": f"{type(ex).__name__}: {ex}"}
             log(f"[bench:batch] failed: {ex}")
+    if "fit" in legs and args.fit_instances > 0:
+        try:
+            out["fit"] = fit_leg(args.fit_instances, args.fit_iters)
+        except Exception as ex:
+            out["fit"] = {"error": f"{type(ex).__name__}: {ex}"}
+            log(f"[bench:fit] failed: {ex}")
     print(json.dumps(out), flush=True)
 
 

@@ -1,0 +1,102 @@
+"""Fitting loop over many differentiable instances: L-BFGS on the contrast, vsini
+and the filling-factor maps of each instance, against target RV and FWHM series
+(and optionally target spectra), with the FP64 adjoint of soap_diff_* for the
+gradients.
+
+No counterpart in the reference.  Each instance is a DiffModel on its own
+Engine (its own context and stream); the objective is the sum of the
+instances' losses, so the gradient of each instance's parameters is that
+instance's own gradient.  The maps are kept in [0, 1] through a logistic
+parameterisation, maps = 1 / (1 + exp(-z)).  The forward of every instance is
+recomputed at each evaluation; the shifted spectra are never stored between
+forward and adjoint beyond one instance's buffers (recompute-in-backward).
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+import torch
+
+from .diff import DiffModel, SoapLoss
+from .engine import Engine, select_mask_lines
+
+
+class Instance:
+    """One fitting problem: a DiffModel with its seeds, targets and starting point."""
+
+    def __init__(self, w, maps0, n_phase: int, map_res: int, target_spec=None, rv_t=None, fw_t=None,
+                 weights=(0.0, 1e-3, 1e-5), u=(0.29, 0.34), vsini0: float = 2.0, contrast0: float = 0.3,
+                 trim: int = 210, device: int = 0):
+        self.eng = Engine(device)
+        sl, sw = select_mask_lines(w.wave, w.mask_l, w.mask_w, trim)
+        self.model = DiffModel(self.eng, w.wave, w.mu_bounds, sl, sw, w.vgrid, w.star["grid"], n_phase, map_res,
+                               w.sigma_inst, trim)
+        dev = torch.device("cuda", device)
+        self.q = torch.tensor(w.cubes["q"], dtype=torch.float64, device=dev)
+        self.s = torch.tensor(w.cubes["sp"], dtype=torch.float64, device=dev)
+        self.P, self.M, self.n = n_phase, map_res, w.n
+        self.target = (torch.zeros((n_phase, w.n), dtype=torch.float64, device=dev) if target_spec is None
+                       else torch.as_tensor(target_spec, dtype=torch.float64, device=dev))
+        self.rv_t = np.zeros(n_phase) if rv_t is None else np.asarray(rv_t, dtype=np.float64)
+        self.fw_t = np.zeros(n_phase) if fw_t is None else np.asarray(fw_t, dtype=np.float64)
+        self.weights = tuple(float(x) for x in weights)
+        self.u = tuple(float(x) for x in u)
+        m = np.clip(np.asarray(maps0, dtype=np.float64), 1e-6, 1 - 1e-6)
+        # fitted: (vsini, contrast) and the map logits
+        self.theta = torch.tensor([vsini0, contrast0], dtype=torch.float64, device=dev, requires_grad=True)
+        self._u = torch.tensor(self.u, dtype=torch.float64, device=dev)
+        self.z = torch.tensor(np.log(m / (1 - m)), dtype=torch.float64, device=dev, requires_grad=True)
+
+    def maps(self):
+        return torch.sigmoid(self.z)
+
+    def params(self):
+        return torch.stack([self.theta[0], self._u[0], self._u[1], self.theta[1]])
+
+    def loss(self):
+        return SoapLoss.apply(self.model, self.params(), self.q, self.s, self.maps(), self.target, self.rv_t,
+                              self.fw_t, self.weights)
+
+    def observe(self, params, maps):
+        """RV and FWHM series of the model at given parameters (targets of a synthetic problem)."""
+        _, rv, fw = self.model.forward(params, self.q, self.s, torch.as_tensor(maps, dtype=torch.float64,
+                                                                              device=self.q.device),
+                                       self.target, self.rv_t, self.fw_t, 0.0, 0.0, 0.0)
+        return rv, fw
+
+    def close(self):
+        self.eng.close()
+
+
+def fit(instances, iters: int = 50, history: int = 10, lr: float = 1.0, log=None):
+    """L-BFGS (strong-Wolfe line search) on the summed loss of all instances.
+    Returns dict(loss0, loss, iters, evals, seconds, ms_per_eval)."""
+    params = [t for inst in instances for t in (inst.theta, inst.z)]
+    opt = torch.optim.LBFGS(params, lr=lr, max_iter=iters, history_size=history, line_search_fn="strong_wolfe",
+                            tolerance_grad=0.0, tolerance_change=0.0)
+    evals = [0]
+    hist = []
+
+    def closure():
+        opt.zero_grad()
+        total = torch.zeros((), dtype=torch.float64)
+        for inst in instances:
+            l_i = inst.loss()
+            l_i.backward()
+            total = total + l_i.detach().cpu()
+        evals[0] += 1
+        hist.append(float(total))
+        if log and evals[0] % 10 == 0:
+            log(f"[fit] eval {evals[0]}: loss {float(total):.6e}")
+        return total
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss0 = float(closure())
+    opt.step(closure)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = opt.state[opt._params[0]]
+    return dict(loss0=loss0, loss=hist[-1], best=min(hist), iters=int(st.get("n_iter", 0)), evals=evals[0],
+                seconds=dt, ms_per_eval=dt / evals[0] * 1e3, history=hist)

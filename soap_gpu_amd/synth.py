@@ -366,3 +366,24 @@ def config3(seed: int = 1237, n_phase: int = 50, grid: int = 600, map_res: int =
                  _star(grid, 2.0), _act(), phases, [[] for _ in phases], sigma_inst_ms(115000),
                  vel_grid(20000.0, 250.0), meta={"map_res": map_res})
     return w, maps
+
+
+def config4_params(seed: int = 1238, n_inst: int = 128):
+    """Per-instance draws of the fitting workload: (child seed, n, grid, phases,
+    regions) with grid 300-500, 30k-80k lambda points, 20-60 phases, 1-4 regions
+    (the blobs of the filling-factor maps), vsini U(1, 10) km/s."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.integers(0, 2 ** 62)), int(rng.integers(30000, 80001)), int(rng.integers(300, 501)),
+             int(rng.integers(20, 61)), int(rng.integers(1, 5)), float(rng.uniform(1.0, 10.0)))
+            for _ in range(n_inst)]
+
+
+def config4_instance(params, map_res: int = 256, i: int = 0):
+    """(workload, true maps, truth) of one fitting instance: a log-grid instance as
+    in the batched workload, and true filling-factor maps of n_reg smooth blobs;
+    truth = dict(vsini, contrast)."""
+    cs, n, grid, n_phase, n_reg, vsini = params
+    rng = np.random.default_rng(cs)
+    w = batch_instance(rng, n, grid, n_phase, n_reg, vsini, 90.0, name=f"config4[{i}]")
+    maps = smooth_blob_maps(rng, n_phase, map_res, n_blobs=n_reg)
+    return w, maps, dict(vsini=vsini, contrast=float(rng.uniform(0.2, 0.5)))
