@@ -8,7 +8,11 @@
         averages the per-dispatch counters of each kernel from the CSV output of
         `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE --output-format csv`,
         scales each by (known bytes / counter reading) of the copy kernel and
-        writes OUT_PREFIX.json (read by bench.py) and OUT_PREFIX.md.
+        writes OUT_PREFIX.json (read by bench.py) and OUT_PREFIX.md;
+    python -m soap_gpu_amd.profiling stats OUT.md TITLE COMMAND DB [--top N]
+        kernel statistics (calls, average / min / max microseconds, share of the
+        total) from the SQLite output of `rocprofv3 --kernel-trace --stats`,
+        written as a markdown table headed by the title and the command.
 """
 from __future__ import annotations
 
@@ -99,6 +103,24 @@ def summarize(out_prefix, workload, fetch, write, copy_fetch, copy_write):
     print("\n".join(lines))
 
 
+def stats(out_md, title, command, db, top=25):
+    import sqlite3
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, count(*), avg(duration), min(duration), max(duration), sum(duration) "
+                       "from kernels group by name order by sum(duration) desc").fetchall()
+    total = sum(r[5] for r in rows) or 1.0
+    lines = [f"# rocprofv3 kernel statistics: {title}", "", "Command (repository root on the MI355X box):", "",
+             f"    {command}", "",
+             "| kernel | calls | average us | min us | max us | total % |", "|---|---:|---:|---:|---:|---:|"]
+    for name, n, avg, mn, mx, tot in rows[:top]:
+        short = name.split("(")[0]
+        lines.append(f"| `{short}` | {n} | {avg / 1e3:.1f} | {mn / 1e3:.1f} | {mx / 1e3:.1f} | "
+                     f"{100.0 * tot / total:.2f} |")
+    with open(out_md, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def main(argv):
     if not argv:
         raise SystemExit(__doc__)
@@ -115,6 +137,16 @@ def main(argv):
         ap.add_argument("--copy-write", required=True)
         a = ap.parse_args(argv[1:])
         summarize(a.out_prefix, a.workload, a.fetch, a.write, a.copy_fetch, a.copy_write)
+    elif argv[0] == "stats":
+        import argparse
+        ap = argparse.ArgumentParser()
+        ap.add_argument("out_md")
+        ap.add_argument("title")
+        ap.add_argument("command")
+        ap.add_argument("db")
+        ap.add_argument("--top", type=int, default=25)
+        a = ap.parse_args(argv[1:])
+        stats(a.out_md, a.title, a.command, a.db, a.top)
     else:
         raise SystemExit(__doc__)
 
