@@ -9,7 +9,7 @@
         `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE --output-format csv`,
         scales each by (known bytes / counter reading) of the copy kernel and
         writes OUT_PREFIX.json (read by bench.py) and OUT_PREFIX.md;
-    python -m soap_gpu_amd.profiling stats OUT.md TITLE COMMAND DB [--top N]
+    python -m soap_gpu_amd.profiling stats OUT.md TITLE COMMAND DB [--top N] [--match REGEX]
         kernel statistics (calls, average / min / max microseconds, share of the
         total) from the SQLite output of `rocprofv3 --kernel-trace --stats`,
         written as a markdown table headed by the title and the command.
@@ -103,11 +103,14 @@ def summarize(out_prefix, workload, fetch, write, copy_fetch, copy_write):
     print("\n".join(lines))
 
 
-def stats(out_md, title, command, db, top=25):
+def stats(out_md, title, command, db, top=25, match=None):
+    import re
     import sqlite3
     con = sqlite3.connect(db)
     rows = con.execute("select name, count(*), avg(duration), min(duration), max(duration), sum(duration) "
                        "from kernels group by name order by sum(duration) desc").fetchall()
+    if match:
+        rows = [r for r in rows if re.search(match, r[0])]
     total = sum(r[5] for r in rows) or 1.0
     lines = [f"# rocprofv3 kernel statistics: {title}", "", "Command (repository root on the MI355X box):", "",
              f"    {command}", "",
@@ -145,8 +148,9 @@ def main(argv):
         ap.add_argument("command")
         ap.add_argument("db")
         ap.add_argument("--top", type=int, default=25)
+        ap.add_argument("--match", default=None, help="regular expression on kernel names")
         a = ap.parse_args(argv[1:])
-        stats(a.out_md, a.title, a.command, a.db, a.top)
+        stats(a.out_md, a.title, a.command, a.db, a.top, a.match)
     else:
         raise SystemExit(__doc__)
 
