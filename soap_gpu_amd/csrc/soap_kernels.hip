@@ -442,12 +442,17 @@ hipError_t launch_fir_active(const FirArgs &a, int n_job, hipStream_t s) {
 // kFirG phases; every pixel's 7 partial sums of the kFirG phases live in
 // registers.  For each mu bin touched by any phase of the group, the block
 // stages that bin's seed-row segments (quiet, quiet SED, and spot / facula rows
-// when present) and the group's taps in LDS; each staged row value then feeds
-// the FMAs of all phases of the group.  Blocks are numbered so the groups of a
-// tile run back to back on one XCD (b % 8), keeping the tile's rows in its L2.
+// when present) in LDS; each staged row value then feeds the FMAs of all phases
+// of the group, with the taps read by uniform (scalar) loads.  The inner loop
+// is branch-free: all seven products of every phase slot, over the union of the
+// group's tap ranges (taps are zero where a phase has no cells), which costs
+// some zero FMAs but no per-phase flag branches.  Blocks are numbered so the
+// groups of a tile run back to back on one XCD (b % 8), keeping the tile's rows
+// in its L2.
 #ifndef SOAP_FIR_G
 #define SOAP_FIR_G 4
 #endif
+
 constexpr int kFirG = SOAP_FIR_G;  // phases per block
 constexpr int kFirTile = kTPB;
 
@@ -526,9 +531,6 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
       }
     }
     __syncthreads();
-    int flg[G];
-#pragma unroll
-    for (int g = 0; g < G; g++) flg[g] = uni(s_fl[g][k]);
     // taps of phase g at offset di: a.taps[(((p0+g)*n_mu + k)*nd + di)*4 + set], uniform addresses
     const GLOBAL double *Tk = gl(a.taps) + ((long)p0 * n_mu + k) * nd * 4;
     const long gstride = (long)n_mu * nd * 4;
@@ -538,22 +540,18 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
       double vq = rq[e], vs = rs[e];
       double vp = (ufl & 1) ? rp[e] : 0.0;
       double vf = (ufl & 2) ? rf[e] : 0.0;
+      // every product of every phase slot (taps are zero where a phase has no cells)
 #pragma unroll
       for (int g = 0; g < G; g++) {
-        const int fl = flg[g];
-        if (!fl) continue;
-        const GLOBAL double *T = Tk + g * gstride + di * 4;
-        F[g][0] = fma(T[3], vs, F[g][0]);
-        if (fl & 1) {
-          F[g][1] = fma(T[0], vq, F[g][1]);
-          F[g][4] = fma(T[0], vp, F[g][4]);
-        }
-        if (fl & 2) {
-          F[g][2] = fma(T[2], vq, F[g][2]);
-          F[g][3] = fma(T[1], vq, F[g][3]);
-          F[g][5] = fma(T[1], vf, F[g][5]);
-          F[g][6] = fma(T[2], vf, F[g][6]);
-        }
+        const GLOBAL double *T = Tk + min(g, ng - 1) * gstride + di * 4;
+        const double t0 = T[0], t1 = T[1], t2 = T[2], t3 = T[3];
+        F[g][0] = fma(t3, vs, F[g][0]);
+        F[g][1] = fma(t0, vq, F[g][1]);
+        F[g][4] = fma(t0, vp, F[g][4]);
+        F[g][2] = fma(t2, vq, F[g][2]);
+        F[g][3] = fma(t1, vq, F[g][3]);
+        F[g][5] = fma(t1, vf, F[g][5]);
+        F[g][6] = fma(t2, vf, F[g][6]);
       }
     }
   }
@@ -1236,27 +1234,48 @@ __global__ __launch_bounds__(kTPB) void k_minmax_reduce(const double *part, int 
 }
 
 // spec_norm, SOAP_resolution.cu:225-232: out = 1 - o (1 - min) / max, in place.
-// Spectrum sp = blockIdx.y at o + sp*n; a block covers kNormPer*kTPB pixels with
-// coalesced loads issued before the stores.
+// Spectrum sp = blockIdx.y at o + sp*n; a block covers kNormPer*kTPB pixels as
+// 16-byte (two-pixel) loads, all issued before the stores.  Spectra start at
+// multiples of n pixels, so an odd n leaves odd-aligned spectra: those take the
+// one-pixel path.
 constexpr int kNormPer = 8;
+typedef double dvec2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void norm_body(long n, const double *mm_, double *o_, int bx_, int by_) {
   const GLOBAL double *mm = gl(mm_);
   GLOBAL double *o = gw(o_);
-  long sp = by_;
-  double mn = mm[sp * 2], mx = mm[sp * 2 + 1];
+  const long sp = by_;
+  const double mn = mm[sp * 2], mx = mm[sp * 2 + 1];
   GLOBAL double *x = o + sp * n;
-  long base = (long)bx_ * (kTPB * kNormPer) + threadIdx.x;
-  double v[kNormPer];
+  const long blk = (long)bx_ * (kTPB * kNormPer);
+  const long end = min(n, blk + (long)kTPB * kNormPer);
+  // pairs start at the first 16-byte boundary of the block; a block that starts
+  // off the boundary (odd n, odd spectrum) does its first pixel alone
+  const int s0 = ((uintptr_t)(x + blk) & 15) ? 1 : 0;
+  constexpr int V = kNormPer / 2;
+  dvec2 v[V];
 #pragma unroll
-  for (int k = 0; k < kNormPer; k++) {
-    long i = base + (long)k * kTPB;
-    v[k] = i < n ? x[i] : 0.0;
+  for (int k = 0; k < V; k++) {
+    long i = blk + s0 + 2L * (threadIdx.x + (long)k * kTPB);
+    if (i + 1 < end) {
+      v[k] = *(const GLOBAL dvec2 *)(x + i);
+    } else {
+      v[k].x = i < end ? x[i] : 0.0;
+      v[k].y = 0.0;
+    }
   }
+  double head = (s0 && threadIdx.x == 0 && blk < n) ? x[blk] : 0.0;
 #pragma unroll
-  for (int k = 0; k < kNormPer; k++) {
-    long i = base + (long)k * kTPB;
-    if (i < n) x[i] = 1.0 - v[k] * (1 - mn) / mx;
+  for (int k = 0; k < V; k++) {
+    long i = blk + s0 + 2L * (threadIdx.x + (long)k * kTPB);
+    dvec2 r;
+    r.x = 1.0 - v[k].x * (1 - mn) / mx;
+    r.y = 1.0 - v[k].y * (1 - mn) / mx;
+    if (i + 1 < end)
+      *(GLOBAL dvec2 *)(x + i) = r;
+    else if (i < end)
+      x[i] = r.x;
   }
+  if (s0 && threadIdx.x == 0 && blk < n) x[blk] = 1.0 - head * (1 - mn) / mx;
 }
 
 __global__ __launch_bounds__(kTPB) void k_norm(long n, const double *mm, double *o) {
