@@ -274,6 +274,31 @@ int soap_batch_set_config(soap_batch *b, int32_t taps_cells, int32_t fir_g, int3
  * (best of reps eager steps per candidate) */
 int soap_batch_tune(soap_batch *b, int32_t reps);
 
+/* ------------------------------------------------------------ a11: differentiable instances in one graph
+ * No counterpart in the reference.  Forward + adjoint of many instances (each a
+ * soap_ctx after soap_diff_setup) with every input, parameter and result in
+ * device memory, so that one evaluation is one HIP graph launch:
+ *   d_params [4] = (vsini, u1, u2, contrast), d_q / d_s [n_mu][n], d_maps [P][M][M],
+ *   d_target [P][n], d_rv_t / d_fw_t [P]  ->  d_loss [1], d_grad4 [4],
+ *   d_grad_q / d_grad_s [n_mu][n], d_grad_maps [P][M][M].
+ * The tap range is fixed at bind time by vsini_max; an evaluation with vsini
+ * outside (0, vsini_max] returns loss = NaN.  The graph is captured on the first
+ * step after a bind and replayed afterwards (values at the bound addresses may
+ * change between steps, the addresses may not).  A per-call soap_diff_forward
+ * on a bound context invalidates its binding (the step refuses it). */
+typedef struct soap_diffbatch soap_diffbatch;
+int soap_diff_batch_create(soap_ctx *const *ctxs, int32_t n_inst, soap_diffbatch **out);
+void soap_diff_batch_destroy(soap_diffbatch *b);
+const char *soap_diff_batch_last_error(const soap_diffbatch *b);
+int soap_diff_batch_bind(soap_diffbatch *b, int32_t inst, double vsini_max, const double *d_params,
+                         const double *d_seed_q, const double *d_seed_s, const double *d_maps, const double *d_target,
+                         const double *d_rv_target, const double *d_fwhm_target, double w_spec, double w_rv,
+                         double w_fwhm, double *d_loss, double *d_grad4, double *d_grad_q, double *d_grad_s,
+                         double *d_grad_maps);
+/* enqueue forward + adjoint of every instance; use_graph != 0 replays the captured graph */
+int soap_diff_batch_step(soap_diffbatch *b, int32_t use_graph);
+int soap_diff_batch_sync(soap_diffbatch *b);
+
 #ifdef __cplusplus
 }
 #endif

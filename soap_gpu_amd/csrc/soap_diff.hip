@@ -41,8 +41,9 @@ __global__ void k_diff_cells(DiffArgs a) {
       k = b;
       break;
     }
-  double L = 1. - a.u1 * (1 - mu) - a.u2 * (1 - mu) * (1 - mu);
-  double delta = a.vsini * y;
+  const double u1 = a.prm ? a.prm[1] : a.u1, u2 = a.prm ? a.prm[2] : a.u2, vsini = a.prm ? a.prm[0] : a.vsini;
+  double L = 1. - u1 * (1 - mu) - u2 * (1 - mu) * (1 - mu);
+  double delta = vsini * y;
   double beta = delta * 1000. / kC;
   double g = sqrt((1 + beta) / (1 - beta));
   double fbeta = -1 * (1 - g);
@@ -82,6 +83,7 @@ __device__ __forceinline__ void cell_yz(long idx, int grid, double &y, double &z
 
 // taps of every phase (blockIdx.y): fixed-point sums in LDS when they fit, then global
 __global__ __launch_bounds__(kDT) void k_diff_taps(DiffArgs a, int cells_per_block, int use_lds) {
+  const double kappa = a.prm ? a.prm[3] : a.kappa;
   extern __shared__ unsigned long long lacc[];  // [n_mu][nd][2 sets][2]
   int p = blockIdx.y;
   long ncell = (long)a.grid * a.grid;
@@ -104,7 +106,7 @@ __global__ __launch_bounds__(kDT) void k_diff_taps(DiffArgs a, int cells_per_blo
     double L = a.cL[idx], t = a.ct[idx];
     int d0 = a.cm[idx] - a.dmin;  // tap index of offset m; m-1 at d0-1
     if (d0 < 1 || d0 >= a.nd) continue;  // the host bounds every shift
-    double wq = L * (1 - phi), ws = a.kappa * L * phi;
+    double wq = L * (1 - phi), ws = kappa * L * phi;
     unsigned long long *row = dst + (long)k * a.nd * 4;
     fx_add(row + (long)d0 * 4 + 0, wq * (1.0 - t));
     fx_add(row + (long)(d0 - 1) * 4 + 0, wq * t);
@@ -510,6 +512,7 @@ int diff_corr_chunks(int64_t n) { return (int)((n + kCorrT - 1) / kCorrT); }
 // per-cell derivatives from the correlations, phases summed per cell; scalar
 // partials per block: [gvsini, gu1, gu2, gkappa]
 __global__ __launch_bounds__(kDT) void k_diff_cellgrad(DiffArgs a, const double *C, double *part) {
+  const double kappa = a.prm ? a.prm[3] : a.kappa;
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long ncell = (long)a.grid * a.grid;
   double sv = 0, su1 = 0, su2 = 0, sk = 0;
@@ -530,8 +533,8 @@ __global__ __launch_bounds__(kDT) void k_diff_cellgrad(DiffArgs a, const double 
         int i0, j0;
         double fy, fz;
         double phi = map_sample(a.maps + (long)p * M * M, M, y, z, i0, j0, fy, fz);
-        gL += (1 - phi) * aq + a.kappa * phi * as;
-        gt += L * ((1 - phi) * bq + a.kappa * phi * bs);
+        gL += (1 - phi) * aq + kappa * phi * as;
+        gt += L * ((1 - phi) * bq + kappa * phi * bs);
         sk += L * phi * as;
       }
       su1 = -gL * (1 - mu);
@@ -562,6 +565,7 @@ __global__ __launch_bounds__(kDT) void k_diff_cellgrad(DiffArgs a, const double 
 // and filtered with the same arithmetic as map_sample).
 __global__ __launch_bounds__(kDT) void k_diff_mapgrad(DiffArgs a, const double *C, double *gmaps) {
 #pragma clang fp contract(off)
+  const double kappa = a.prm ? a.prm[3] : a.kappa;
   const int M = a.map_res, G = a.grid;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;  // map column
   const int i = blockIdx.y;                              // map row
@@ -597,7 +601,7 @@ __global__ __launch_bounds__(kDT) void k_diff_mapgrad(DiffArgs a, const double *
       double t = a.ct[idx];
       double aq = (1 - t) * Cp[d0 * 2] + t * Cp[(d0 - 1) * 2];
       double as = (1 - t) * Cp[d0 * 2 + 1] + t * Cp[(d0 - 1) * 2 + 1];
-      acc += wy * wz * (a.cL[idx] * (a.kappa * as - aq));
+      acc += wy * wz * (a.cL[idx] * (kappa * as - aq));
     }
   }
   gmaps[((long)p * M + i) * M + j] = acc;
@@ -665,6 +669,54 @@ hipError_t launch_diff_forward_spectra(const DiffArgs &a, const ConvArgs &c, con
 }
 
 int diff_conv_blocks(int64_t n) { return (int)((n + 4 * kDT - 1) / (4 * kDT)); }
+
+// loss = sum_p [w_spec r2_p + w_rv (RV_p - rv_t)^2 + w_fw (FWHM_p - fw_t)^2], r2_p the
+// residual partials of phase p summed in order; the same order as the host sum
+__global__ __launch_bounds__(kDT) void k_diff_loss(int n_phase, int res_blocks, const double *res_part,
+                                                   const double *theta, const double *rv_t, const double *fw_t,
+                                                   double w_spec, double w_rv, double w_fw, const double *prm,
+                                                   double vsini_max, double *loss, double *dth) {
+  extern __shared__ double term[];  // [n_phase]
+  for (int p = threadIdx.x; p < n_phase; p += blockDim.x) {
+    double r2 = 0;
+    for (int b = 0; b < res_blocks; b++) r2 += res_part[(long)p * res_blocks + b];
+    double drv = theta[p * 4 + 2] - rv_t[p], dfw = theta[p * 4 + 3] - fw_t[p];
+    term[p] = w_spec * r2 + w_rv * drv * drv + w_fw * dfw * dfw;
+    dth[p * 4 + 0] = 0.0;
+    dth[p * 4 + 1] = 0.0;
+    dth[p * 4 + 2] = 2 * w_rv * drv;
+    dth[p * 4 + 3] = 2 * w_fw * dfw;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double l = 0;
+    for (int p = 0; p < n_phase; p++) l += term[p];
+    double vs = prm ? prm[0] : 0.0;
+    if (prm && !(vs > 0 && vs <= vsini_max)) l = __longlong_as_double(0x7ff8000000000000LL);  // outside the tap range
+    loss[0] = l;
+  }
+}
+
+__global__ void k_diff_grad4(int n_blocks, const double *cell_part, double *g4) {
+  if (threadIdx.x >= 4) return;
+  double s = 0;
+  for (int b = 0; b < n_blocks; b++) s += cell_part[(long)b * 4 + threadIdx.x];
+  g4[threadIdx.x] = s;
+}
+
+hipError_t launch_diff_loss(int n_phase, int res_blocks, const double *res_part, const double *theta,
+                            const double *rv_t, const double *fw_t, double w_spec, double w_rv, double w_fw,
+                            const double *prm, double vsini_max, double *loss, double *dth, hipStream_t s) {
+  if (n_phase < 1 || (size_t)n_phase * 8 > 65536) return hipErrorInvalidValue;
+  k_diff_loss<<<1, kDT, (size_t)n_phase * 8, s>>>(n_phase, res_blocks, res_part, theta, rv_t, fw_t, w_spec, w_rv,
+                                                  w_fw, prm, vsini_max, loss, dth);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_grad4(int n_blocks, const double *cell_part, double *g4, hipStream_t s) {
+  k_diff_grad4<<<1, 64, 0, s>>>(n_blocks, cell_part, g4);
+  return hipGetLastError();
+}
 int diff_cell_blocks(int64_t ncell) { return (int)((ncell + kDT - 1) / kDT); }
 
 hipError_t launch_fit_polish(const double *v, int n_v, const double *ccf, const double *par, int n_ccf, double *theta,

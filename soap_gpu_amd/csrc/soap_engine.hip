@@ -2349,3 +2349,241 @@ int soap_batch_tune(soap_batch *b, int32_t reps) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------ a11: differentiable instances in one graph
+// Forward + adjoint of many differentiable instances (each a soap_ctx after
+// soap_diff_setup) enqueued on one stream with every input, parameter and result
+// in device memory, so that the whole evaluation can be captured once as a HIP
+// graph and replayed with new parameter values: the tap range of each instance
+// is fixed by a bound on vsini, the loss, the fit-parameter derivatives and the
+// scalar gradients are reduced on the device.
+struct soap_diffbatch {
+  struct Item {
+    soap_ctx *c = nullptr;
+    bool bound = false;
+    double vsini_max = 0;
+    const double *prm = nullptr, *q = nullptr, *s = nullptr, *maps = nullptr, *target = nullptr, *rv_t = nullptr,
+                 *fw_t = nullptr;
+    double w_spec = 0, w_rv = 0, w_fw = 0;
+    double *loss = nullptr, *g4 = nullptr, *gq = nullptr, *gs = nullptr, *gmaps = nullptr;
+    int dmin = 0, nd = 0;
+    std::vector<uintptr_t> sig;  // buffers and sizes the enqueued launches use
+  };
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<Item> items;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::string err;
+};
+
+namespace {
+std::vector<uintptr_t> dbatch_signature(const soap_ctx *c) {
+  auto &d = c->diff;
+  auto u = [](const void *p) { return (uintptr_t)p; };
+  return {u(d.acc.p),  u(d.taps.p),  u(d.krange.p), u(d.C.p),     u(d.Cpart.p),  u(d.res_part.p), u(d.ccf.p),
+          u(d.par.p),  u(d.model.p), u(d.depth.p),  u(d.bis.p),   u(d.iters.p),  u(d.th.p),       u(d.dth.p),
+          u(d.gy.p),   u(d.GF.p),    u(d.GI.p),     u(d.cell_part.p), u(d.I.p),  u(d.F.p),        u(d.ck.p),
+          u(d.cm.p),   u(d.ct.p),    u(d.cL.p),     u(d.cmu.p),   u(d.cdt.p),    u(d.mt_ptr.p),   u(d.mt_w.p),
+          u(d.mt_col.p), u(d.mt_val.p), u(c->ccf_part.p), u(c->m_start.p), u(c->m_alive.p), u(c->m_val.p),
+          u(c->d_w_sorted), u(c->ctap.p), u(d.maps), u(d.q), u(d.s), u(d.target), (uintptr_t)d.nd,
+          (uintptr_t)d.dmin, (uintptr_t)d.P, (uintptr_t)d.G, (uintptr_t)d.M, (uintptr_t)c->n, (uintptr_t)c->res_ntp,
+          (uintptr_t)c->n_lines, (uintptr_t)c->n_v};
+}
+
+void dbatch_drop_graph(soap_diffbatch *b) {
+  if (b->exec) hipGraphExecDestroy(b->exec);
+  if (b->graph) hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+int dfail(soap_diffbatch *b, int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (b) b->err = buf;
+  return code;
+}
+
+int dbatch_enqueue(soap_diffbatch *b) {
+  for (size_t i = 0; i < b->items.size(); i++) {
+    auto &it = b->items[i];
+    if (!it.bound) return dfail(b, SOAP_ERR_STATE, "diff batch: instance %zu not bound", i);
+    soap_ctx *c = it.c;
+    auto &d = c->diff;
+    DiffArgs a = diff_args(c);
+    a.prm = it.prm;
+    ConvArgs ca = diff_conv(c);
+    int rb = 0;
+    hipError_t e = launch_diff_forward_spectra(a, ca, it.target, c->trim, (double *)d.res_part.p, &rb, b->stream);
+    const int64_t n_t = c->n - 2 * (int64_t)c->trim;
+    if (e == hipSuccess)
+      e = launch_ccf((const double *)d.F.p, c->n, c->trim, d.P, c->n_lines, c->n_v, n_t, (const int32_t *)c->m_start.p,
+                     (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p, c->d_w_sorted,
+                     (double *)c->ccf_part.p, b->stream);
+    if (e == hipSuccess)
+      e = launch_ccf_sum((const double *)c->ccf_part.p, ccf_chunks(c->n_lines), d.P, c->n_v, (double *)d.ccf.p,
+                         b->stream);
+    if (e == hipSuccess)
+      e = launch_fit(c->d_vgrid, c->n_v, (const double *)d.ccf.p, d.P, (double *)d.par.p, (double *)d.model.p,
+                     (double *)d.depth.p, (double *)d.bis.p, (int32_t *)d.iters.p, b->stream);
+    if (e == hipSuccess)
+      e = launch_fit_polish(c->d_vgrid, c->n_v, (const double *)d.ccf.p, (const double *)d.par.p, d.P,
+                            (double *)d.th.p, b->stream);
+    if (e == hipSuccess)
+      e = launch_diff_loss(d.P, rb, (const double *)d.res_part.p, (const double *)d.th.p, it.rv_t, it.fw_t, it.w_spec,
+                           it.w_rv, it.w_fw, it.prm, it.vsini_max, it.loss, (double *)d.dth.p, b->stream);
+    if (e == hipSuccess)
+      e = launch_diff_backward(a, ca, it.target, c->trim, it.w_spec, c->d_vgrid, c->n_v, (const double *)d.ccf.p,
+                               (const double *)d.th.p, (const double *)d.dth.p, (double *)d.gy.p, n_t,
+                               (const int64_t *)d.mt_ptr.p, (const int32_t *)d.mt_w.p, (const int32_t *)d.mt_col.p,
+                               (const double *)d.mt_val.p, (double *)d.GF.p, (double *)d.GI.p, (double *)d.Cpart.p,
+                               (double *)d.C.p, it.gmaps, (double *)d.cell_part.p, it.gq, it.gs, b->stream);
+    if (e == hipSuccess) e = launch_diff_grad4(diff_cell_blocks((int64_t)d.G * d.G), (const double *)d.cell_part.p,
+                                               it.g4, b->stream);
+    if (e != hipSuccess)
+      return dfail(b, e == hipErrorInvalidValue ? SOAP_ERR_LIMIT : SOAP_ERR_HIP, "diff batch: instance %zu: %s", i,
+                   hipGetErrorString(e));
+  }
+  return SOAP_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int soap_diff_batch_create(soap_ctx *const *ctxs, int32_t n_inst, soap_diffbatch **out) {
+  if (!out) return SOAP_ERR_ARG;
+  *out = nullptr;
+  if (!ctxs || n_inst < 1) return SOAP_ERR_ARG;
+  auto *b = new soap_diffbatch();
+  *out = b;
+  b->device = ctxs[0] ? ctxs[0]->device : 0;
+  for (int i = 0; i < n_inst; i++) {
+    soap_ctx *c = ctxs[i];
+    if (!c) return dfail(b, SOAP_ERR_ARG, "diff_batch_create: instance %d: null context", i);
+    if (c->device != b->device) return dfail(b, SOAP_ERR_ARG, "diff_batch_create: contexts must share one device");
+    if (!c->diff.ready) return dfail(b, SOAP_ERR_STATE, "diff_batch_create: instance %d: run diff_setup first", i);
+    for (int j = 0; j < i; j++)
+      if (ctxs[j] == c) return dfail(b, SOAP_ERR_ARG, "diff_batch_create: instance %d listed twice", i);
+    soap_diffbatch::Item it;
+    it.c = c;
+    b->items.push_back(it);
+  }
+  hipError_t e = hipSetDevice(b->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff_batch_create: %s", hipGetErrorString(e));
+  return SOAP_OK;
+}
+
+void soap_diff_batch_destroy(soap_diffbatch *b) {
+  if (!b) return;
+  hipSetDevice(b->device);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  dbatch_drop_graph(b);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+}
+
+const char *soap_diff_batch_last_error(const soap_diffbatch *b) { return b ? b->err.c_str() : "null batch"; }
+
+int soap_diff_batch_bind(soap_diffbatch *b, int32_t i, double vsini_max, const double *d_params, const double *d_q,
+                         const double *d_s, const double *d_maps, const double *d_target, const double *d_rv_t,
+                         const double *d_fw_t, double w_spec, double w_rv, double w_fw, double *d_loss,
+                         double *d_grad4, double *d_gq, double *d_gs, double *d_gmaps) {
+  if (!b || i < 0 || i >= (int32_t)b->items.size()) return SOAP_ERR_ARG;
+  if (!d_params || !d_q || !d_s || !d_maps || !d_target || !d_rv_t || !d_fw_t || !d_loss || !d_grad4 || !d_gq ||
+      !d_gs || !d_gmaps)
+    return dfail(b, SOAP_ERR_ARG, "diff_batch_bind: null pointer");
+  if (!(vsini_max > 0 && vsini_max < 2000)) return dfail(b, SOAP_ERR_ARG, "diff_batch_bind: vsini_max out of range");
+  auto &it = b->items[i];
+  soap_ctx *c = it.c;
+  auto &d = c->diff;
+  if (hipSetDevice(c->device) != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff_batch_bind: hipSetDevice");
+  // fixed tap range: the shifts of delta in [-vsini_max, vsini_max] plus margins
+  auto mshift = [&](double v) {
+    double beta = v * 1000. / kC;
+    double fb = -1 * (1 - sqrt((1 + beta) / (1 - beta)));
+    return (int)ceil(log1p(fb) / c->h);
+  };
+  int dmin = mshift(-vsini_max) - 2;
+  int nd = mshift(vsini_max) - dmin + 3;
+  if (nd > 2048) return dfail(b, SOAP_ERR_LIMIT, "diff_batch_bind: shift range of %d pixels", nd);
+  int rc;
+  if ((rc = ensure(c, d.acc, (size_t)d.P * c->n_mu * nd * 4 * 8)) ||
+      (rc = ensure(c, d.taps, (size_t)d.P * c->n_mu * nd * 2 * 8)) ||
+      (rc = ensure(c, d.krange, (size_t)d.P * c->n_mu * 2 * 4)) ||
+      (rc = ensure(c, d.C, (size_t)d.P * c->n_mu * nd * 2 * 8)) ||
+      (rc = ensure(c, d.Cpart, (size_t)d.P * c->n_mu * diff_corr_chunks(c->n) * nd * 2 * 8)) ||
+      (!c->mask_built && (rc = build_mask(c))) || (rc = diff_mask_transpose(c)))
+    return dfail(b, rc, "diff_batch_bind: instance %d: %s", i, c->err.c_str());
+  d.dmin = dmin;
+  d.nd = nd;
+  d.q = d_q;
+  d.s = d_s;
+  d.maps = d_maps;
+  d.target = d_target;
+  d.have_fwd = false;  // the per-call forward state no longer matches the buffers
+  it.vsini_max = vsini_max;
+  it.dmin = dmin;
+  it.nd = nd;
+  it.prm = d_params;
+  it.q = d_q;
+  it.s = d_s;
+  it.maps = d_maps;
+  it.target = d_target;
+  it.rv_t = d_rv_t;
+  it.fw_t = d_fw_t;
+  it.w_spec = w_spec;
+  it.w_rv = w_rv;
+  it.w_fw = w_fw;
+  it.loss = d_loss;
+  it.g4 = d_grad4;
+  it.gq = d_gq;
+  it.gs = d_gs;
+  it.gmaps = d_gmaps;
+  it.bound = true;
+  it.sig = dbatch_signature(c);
+  dbatch_drop_graph(b);
+  return SOAP_OK;
+}
+
+int soap_diff_batch_step(soap_diffbatch *b, int32_t use_graph) {
+  if (!b) return SOAP_ERR_ARG;
+  if (hipSetDevice(b->device) != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff_batch_step: hipSetDevice");
+  for (auto &it : b->items) {
+    // the buffers were sized for the bound tap range; a per-call forward in between changes it
+    if (it.bound && dbatch_signature(it.c) != it.sig)
+      return dfail(b, SOAP_ERR_STATE, "diff_batch_step: an instance was changed after binding");
+  }
+  if (!use_graph) return dbatch_enqueue(b);
+  if (!b->exec) {
+    hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff_batch_step: begin capture: %s", hipGetErrorString(e));
+    int rc = dbatch_enqueue(b);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+      if (g) hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff_batch_step: end capture: %s", hipGetErrorString(e));
+    b->graph = g;
+    e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff_batch_step: instantiate: %s", hipGetErrorString(e));
+  }
+  hipError_t e = hipGraphLaunch(b->exec, b->stream);
+  if (e != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff_batch_step: launch: %s", hipGetErrorString(e));
+  return SOAP_OK;
+}
+
+int soap_diff_batch_sync(soap_diffbatch *b) {
+  if (!b) return SOAP_ERR_ARG;
+  if (hipSetDevice(b->device) != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess)
+    return dfail(b, SOAP_ERR_HIP, "diff_batch_sync failed");
+  return SOAP_OK;
+}
+
+}  // extern "C"

@@ -162,6 +162,7 @@ struct DiffArgs {
   int64_t n;
   int32_t grid, n_mu, n_phase, map_res, dmin, nd;
   double h, expm1_h, vsini, u1, u2, kappa;
+  const double *prm;             // device (vsini, u1, u2, kappa) if not null (graph replays), else the fields above
   const double *mu_bounds;       // n_mu - 1
   int32_t *ck, *cm;              // per cell [G*G]: bin (-1 off disk), shift m
   double *ct, *cL, *cmu, *cdt;   // weight t, limb, mu, dt/d(vsini)
@@ -314,6 +315,15 @@ hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const doub
 // chunks of the lag-correlation pass (Cpart holds [P][n_mu][chunks][nd][2])
 int diff_corr_chunks(int64_t n);
 hipError_t launch_ccf_sum(const double *part, int nc, int n_spec, int n_v, double *ccf, hipStream_t s);
+// device-side loss of the differentiable model: per phase the residual partials
+// (res_blocks per phase, in order) and the RV / FWHM terms; writes loss[0] and
+// the fit-parameter derivatives dth [P][4]; loss = NaN when prm's vsini is
+// outside (0, vsini_max] (the fixed tap range of a graph)
+hipError_t launch_diff_loss(int n_phase, int res_blocks, const double *res_part, const double *theta,
+                            const double *rv_t, const double *fw_t, double w_spec, double w_rv, double w_fw,
+                            const double *prm, double vsini_max, double *loss, double *dth, hipStream_t s);
+// g4[q] = sum over blocks (in order) of cell_part[b][q], q < 4
+hipError_t launch_diff_grad4(int n_blocks, const double *cell_part, double *g4, hipStream_t s);
 // batched instances: stage st over jobs[n_inst] (device), prefix[n_inst + 1]
 // (device, prefix[n_inst] = total); grid = persistent block count (0: one block
 // per work item), threads / lds as the stage needs; FIR shape: fir_g phases per

@@ -110,3 +110,54 @@ class SoapLoss(torch.autograd.Function):
         gp = torch.as_tensor(g4, dtype=torch.float64, device=ctx.params_device) * gout.to(ctx.params_device)
         go = gout.to(gq.device)
         return None, gp, gq * go, gs * go, gm * go, None, None, None, None
+
+
+class DiffBatch:
+    """Forward + adjoint of many DiffModels as one HIP graph (soap_diff_batch_* of
+    the C ABI).  Every input, parameter and result is a device tensor bound once;
+    a step replays the graph on the values those tensors hold."""
+
+    def __init__(self, models):
+        from .engine import load
+        self.L = load()
+        self.models = list(models)
+        n = len(self.models)
+        ctxs = (ctypes.c_void_p * n)(*[m.eng.ctx for m in self.models])
+        self.b = ctypes.c_void_p()
+        rc = self.L.soap_diff_batch_create(ctxs, n, ctypes.byref(self.b))
+        if rc != 0:
+            msg = self.L.soap_diff_batch_last_error(self.b) if self.b else b""
+            self.close()
+            raise SoapError(f"soap_diff_batch_create failed ({rc}): {msg.decode() if msg else ''}")
+        self._keep = [None] * n
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            msg = self.L.soap_diff_batch_last_error(self.b)
+            raise SoapError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+    def bind(self, i, vsini_max, params, q, s, maps, target, rv_t, fw_t, weights, loss, g4, gq, gs, gm):
+        """All tensors float64, contiguous, on the GPU: params [4], q / s [n_mu][n], maps [P][M][M],
+        target [P][n], rv_t / fw_t [P]; outputs loss [1], g4 [4], gq / gs like q, gm like maps."""
+        ts = (params, q, s, maps, target, rv_t, fw_t, loss, g4, gq, gs, gm)
+        w_spec, w_rv, w_fw = (float(x) for x in weights)
+        self._chk(self.L.soap_diff_batch_bind(self.b, int(i), float(vsini_max), *[_dev(t) for t in ts[:7]], w_spec,
+                                              w_rv, w_fw, *[_dev(t) for t in ts[7:]]), "diff_batch_bind")
+        self._keep[i] = ts
+
+    def step(self, graph: bool = True):
+        self._chk(self.L.soap_diff_batch_step(self.b, int(bool(graph))), "diff_batch_step")
+
+    def sync(self):
+        self._chk(self.L.soap_diff_batch_sync(self.b), "diff_batch_sync")
+
+    def close(self):
+        if self.b:
+            self.L.soap_diff_batch_destroy(self.b)
+            self.b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -57,6 +57,8 @@ EXPORTS = (
     "soap_batch_create", "soap_batch_destroy", "soap_batch_last_error", "soap_batch_step", "soap_batch_sync",
     "soap_batch_stage_ms", "soap_batch_stage_name", "soap_batch_num_stages", "soap_batch_get_config",
     "soap_batch_set_config", "soap_batch_tune",
+    "soap_diff_batch_create", "soap_diff_batch_destroy", "soap_diff_batch_last_error", "soap_diff_batch_bind",
+    "soap_diff_batch_step", "soap_diff_batch_sync",
 )
 
 _lib = None
@@ -137,6 +139,14 @@ def load(path: str = LIB_PATH):
         "soap_batch_set_config": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                  ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
         "soap_batch_tune": (ctypes.c_int, [vp, ctypes.c_int32]),
+        "soap_diff_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp)]),
+        "soap_diff_batch_destroy": (None, [vp]),
+        "soap_diff_batch_last_error": (ctypes.c_char_p, [vp]),
+        "soap_diff_batch_bind": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_double, vp, vp, vp, vp, vp, vp, vp,
+                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, vp, vp, vp, vp,
+                                                vp]),
+        "soap_diff_batch_step": (ctypes.c_int, [vp, ctypes.c_int32]),
+        "soap_diff_batch_sync": (ctypes.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

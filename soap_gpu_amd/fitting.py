@@ -18,7 +18,7 @@ import time
 import numpy as np
 import torch
 
-from .diff import DiffModel, SoapLoss
+from .diff import DiffBatch, DiffModel, SoapLoss
 from .engine import Engine, select_mask_lines
 
 
@@ -100,3 +100,69 @@ def fit(instances, iters: int = 50, history: int = 10, lr: float = 1.0, log=None
     st = opt.state[opt._params[0]]
     return dict(loss0=loss0, loss=hist[-1], best=min(hist), iters=int(st.get("n_iter", 0)), evals=evals[0],
                 seconds=dt, ms_per_eval=dt / evals[0] * 1e3, history=hist)
+
+
+def fit_graph(instances, iters: int = 50, history: int = 10, lr: float = 1.0, vsini_max=None, log=None):
+    """As fit(), with each objective evaluation (forward + adjoint of every
+    instance) one replay of a captured HIP graph (DiffBatch): the parameters and
+    maps are written into bound device buffers, the losses and gradients are read
+    from bound device buffers.  vsini_max bounds the fitted vsini of each instance
+    (default 2 x its start + 5 km/s); the loss is NaN beyond it."""
+    batch = DiffBatch([inst.model for inst in instances])
+    bufs = []
+    for i, inst in enumerate(instances):
+        dev = inst.q.device
+        vmax = float(vsini_max[i]) if vsini_max is not None else 2.0 * float(inst.theta[0].detach()) + 5.0
+        b = dict(prm=torch.empty(4, dtype=torch.float64, device=dev),
+                 maps=torch.empty_like(inst.z, memory_format=torch.contiguous_format).detach(),
+                 rv_t=torch.as_tensor(inst.rv_t, dtype=torch.float64, device=dev).contiguous(),
+                 fw_t=torch.as_tensor(inst.fw_t, dtype=torch.float64, device=dev).contiguous(),
+                 loss=torch.empty(1, dtype=torch.float64, device=dev),
+                 g4=torch.empty(4, dtype=torch.float64, device=dev),
+                 gq=torch.empty_like(inst.q), gs=torch.empty_like(inst.s),
+                 gm=torch.empty_like(inst.z).detach())
+        batch.bind(i, vmax, b["prm"], inst.q, inst.s, b["maps"], inst.target, b["rv_t"], b["fw_t"], inst.weights,
+                   b["loss"], b["g4"], b["gq"], b["gs"], b["gm"])
+        bufs.append(b)
+    params = [t for inst in instances for t in (inst.theta, inst.z)]
+    opt = torch.optim.LBFGS(params, lr=lr, max_iter=iters, history_size=history, line_search_fn="strong_wolfe",
+                            tolerance_grad=0.0, tolerance_change=0.0)
+    evals = [0]
+    hist = []
+
+    def evaluate():
+        with torch.no_grad():
+            for inst, b in zip(instances, bufs):
+                b["prm"].copy_(inst.params())
+                b["maps"].copy_(inst.maps())
+        torch.cuda.synchronize()
+        batch.step(graph=True)
+        batch.sync()
+        with torch.no_grad():
+            total = torch.zeros((), dtype=torch.float64, device=bufs[0]["loss"].device)
+            for inst, b in zip(instances, bufs):
+                total += b["loss"][0]
+                m = torch.sigmoid(inst.z)
+                inst.theta.grad = torch.stack([b["g4"][0], b["g4"][3]])
+                inst.z.grad = b["gm"] * m * (1 - m)
+        return total
+
+    def closure():
+        opt.zero_grad()
+        total = evaluate()
+        evals[0] += 1
+        hist.append(float(total))
+        if log and evals[0] % 10 == 0:
+            log(f"[fit] eval {evals[0]}: loss {hist[-1]:.6e}")
+        return total
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss0 = float(closure())
+    opt.step(closure)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = opt.state[opt._params[0]]
+    batch.close()
+    return dict(loss0=loss0, loss=hist[-1], best=min(h for h in hist if h == h), iters=int(st.get("n_iter", 0)),
+                evals=evals[0], seconds=dt, ms_per_eval=dt / evals[0] * 1e3, history=hist)

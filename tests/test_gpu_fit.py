@@ -40,3 +40,89 @@ def test_lbfgs_fit_reduces_loss_and_recovers_vsini():
     finally:
         for inst in insts:
             inst.close()
+
+
+def test_graph_evaluation_matches_per_call():
+    """One graph replay (forward + adjoint of two uneven instances, parameters,
+    maps and results in device buffers) against the per-call path at two
+    parameter sets, replaying the same graph: loss, the four scalar gradients and
+    the seed / map gradients agree to rounding."""
+    import torch
+
+    from soap_gpu_amd.diff import DiffBatch
+
+    rng = np.random.default_rng(12)
+    specs = ((3100, 36, 3, 2, 3.5), (4300, 48, 4, 1, 6.0))
+    made = []
+    for n, grid, P, nreg, vs in specs:
+        w = synth.batch_instance(rng, n, grid, P, nreg, vs, 90.0, n_mu=5)
+        maps = synth.smooth_blob_maps(rng, P, 12, n_blobs=nreg)
+        made.append((w, maps, vs, P))
+    weights = (1e-3, 1e-2, 1e-4)
+    graph_insts = [Instance(w, maps, P, 12, weights=weights) for w, maps, vs, P in made]
+    call_insts = [Instance(w, maps, P, 12, weights=weights) for w, maps, vs, P in made]
+    try:
+        dev = torch.device("cuda")
+        bufs = []
+        batch = DiffBatch([g.model for g in graph_insts])
+        for i, (g, (w, maps, vs, P)) in enumerate(zip(graph_insts, made)):
+            rv_t = rng.uniform(-50, 50, P)
+            fw_t = rng.uniform(7000, 9000, P)
+            g.rv_t, g.fw_t = rv_t, fw_t
+            call_insts[i].rv_t, call_insts[i].fw_t = rv_t, fw_t
+            tgt = torch.tensor(rng.uniform(0.9, 1.0, (P, w.n)), dtype=torch.float64, device=dev)
+            g.target = tgt
+            call_insts[i].target = tgt.clone()
+            b = dict(prm=torch.empty(4, dtype=torch.float64, device=dev),
+                     maps=torch.empty((P, 12, 12), dtype=torch.float64, device=dev),
+                     rv=torch.tensor(rv_t, dtype=torch.float64, device=dev),
+                     fw=torch.tensor(fw_t, dtype=torch.float64, device=dev),
+                     loss=torch.empty(1, dtype=torch.float64, device=dev),
+                     g4=torch.empty(4, dtype=torch.float64, device=dev),
+                     gq=torch.empty_like(g.q), gs=torch.empty_like(g.s),
+                     gm=torch.empty((P, 12, 12), dtype=torch.float64, device=dev))
+            batch.bind(i, 15.0, b["prm"], g.q, g.s, b["maps"], g.target, b["rv"], b["fw"], weights, b["loss"],
+                       b["g4"], b["gq"], b["gs"], b["gm"])
+            bufs.append(b)
+        for rep, scale in enumerate((1.0, 1.3)):
+            for (w, maps, vs, P), b in zip(made, bufs):
+                b["prm"].copy_(torch.tensor([vs * scale, 0.29, 0.34, 0.3 + 0.1 * rep], dtype=torch.float64,
+                                            device=dev))
+                b["maps"].copy_(torch.tensor(np.clip(maps * scale, 0, 1), device=dev))
+            torch.cuda.synchronize()
+            batch.step(graph=True)
+            batch.sync()
+            for (w, maps, vs, P), b, c in zip(made, bufs, call_insts):
+                prm = (vs * scale, 0.29, 0.34, 0.3 + 0.1 * rep)
+                mp = torch.tensor(np.clip(maps * scale, 0, 1), device=dev)
+                loss, _, _ = c.model.forward(prm, c.q, c.s, mp, c.target, c.rv_t, c.fw_t, *weights)
+                g4, gq, gs, gm = c.model.backward()
+                assert float(b["loss"][0]) == pytest.approx(loss, rel=1e-11), rep
+                np.testing.assert_allclose(b["g4"].cpu().numpy(), g4, rtol=1e-9, atol=1e-12 * np.abs(g4).max())
+                for got, ref in ((b["gq"], gq), (b["gs"], gs), (b["gm"], gm)):
+                    torch.testing.assert_close(got, ref, rtol=1e-9, atol=1e-12 * float(ref.abs().max()))
+        # a bound vsini beyond the graph's tap range gives NaN, not a silently clipped model
+        bufs[0]["prm"][0] = 20.0
+        torch.cuda.synchronize()
+        batch.step(graph=True)
+        batch.sync()
+        assert np.isnan(float(bufs[0]["loss"][0]))
+        batch.close()
+    finally:
+        for x in graph_insts + call_insts:
+            x.close()
+
+
+def test_graph_fit_reduces_loss():
+    from soap_gpu_amd.fitting import fit_graph
+
+    pairs = _instances()
+    insts = [p[0] for p in pairs]
+    try:
+        r = fit_graph(insts, iters=25)
+        assert r["best"] <= 1e-3 * r["loss0"], (r["loss0"], r["best"])
+        for inst, vs in pairs:
+            assert abs(float(inst.theta[0].detach()) - vs) < 0.05 * vs
+    finally:
+        for inst in insts:
+            inst.close()
