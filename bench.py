@@ -320,8 +320,10 @@ def fit_leg(n_inst: int, iters: int, workers: int = 8):
     instance) and the loss reduction."""
     from concurrent.futures import ThreadPoolExecutor
 
+    import torch
+
     from soap_gpu_amd import synth
-    from soap_gpu_amd.fitting import Instance, fit
+    from soap_gpu_amd.fitting import Instance, fit_graph
 
     t0 = time.time()
     prm = synth.config4_params(n_inst=n_inst)
@@ -339,14 +341,28 @@ def fit_leg(n_inst: int, iters: int, workers: int = 8):
         truth.append(tr)
     del made
     log(f"[bench:fit] {n_inst} instances set up in {time.time() - t0:.1f} s")
-    r = fit(insts, iters=iters, log=log)
+    # one objective evaluation through the per-call path (each instance's forward and
+    # adjoint launched separately, host round trips in between), for comparison
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for inst in insts:
+        inst.loss().backward()
+    torch.cuda.synchronize()
+    ms_per_call = (time.perf_counter() - t1) * 1e3
+    for inst in insts:
+        inst.theta.grad = None
+        inst.z.grad = None
+    r = fit_graph(insts, iters=iters, log=log)
     vs_err = [abs(float(i.theta[0].detach()) - t["vsini"]) / t["vsini"] for i, t in zip(insts, truth)]
     n_par = sum(2 + i.z.numel() for i in insts)
     out = dict(instances=n_inst, iterations=r["iters"], evaluations=r["evals"], seconds=round(r["seconds"], 2),
-               ms_per_evaluation=round(r["ms_per_eval"], 2), loss0=r["loss0"], loss=r["loss"], best=r["best"],
+               ms_per_evaluation=round(r["ms_per_eval"], 2), ms_per_evaluation_per_call_path=round(ms_per_call, 2),
+               ms_per_graph_replay=round(r["graph_ms"], 2),
+               loss0=r["loss0"], loss=r["loss"], best=r["best"],
                parameters=int(n_par), vsini_rel_err_median=float(np.median(vs_err)),
-               note="per-instance launches (one context and stream each) with host round trips per forward and "
-                    "adjoint; L-BFGS over the summed loss (torch.optim.LBFGS, strong Wolfe)")
+               note="each evaluation = forward + adjoint of every instance replayed as one captured HIP graph "
+                    "(parameters, maps, losses and gradients in bound device buffers); L-BFGS over the summed "
+                    "loss (torch.optim.LBFGS, strong Wolfe)")
     for i in insts:
         i.close()
     return out

@@ -2357,6 +2357,8 @@ int soap_batch_tune(soap_batch *b, int32_t reps) {
 // graph and replayed with new parameter values: the tap range of each instance
 // is fixed by a bound on vsini, the loss, the fit-parameter derivatives and the
 // scalar gradients are reduced on the device.
+constexpr int kDiffLanes = 4;
+
 struct soap_diffbatch {
   struct Item {
     soap_ctx *c = nullptr;
@@ -2371,6 +2373,11 @@ struct soap_diffbatch {
   };
   int device = 0;
   hipStream_t stream = nullptr;
+  // instances run on kDiffLanes streams forked from `stream` and joined back, so
+  // that small instances overlap; the captured graph has one branch per lane
+  hipStream_t lane[kDiffLanes] = {};
+  hipEvent_t fork = nullptr, join[kDiffLanes] = {};
+  std::vector<int> lane_of;  // lane of each instance, balanced by estimated work
   std::vector<Item> items;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -2408,47 +2415,91 @@ int dfail(soap_diffbatch *b, int code, const char *fmt, ...) {
   return code;
 }
 
+// Lanes by estimated work (spectra pixels and disk cells per phase), largest
+// instance first onto the least-loaded lane.
+void dbatch_assign_lanes(soap_diffbatch *b) {
+  const size_t n = b->items.size();
+  std::vector<double> work(n);
+  std::vector<size_t> order(n);
+  for (size_t i = 0; i < n; i++) {
+    const soap_ctx *c = b->items[i].c;
+    work[i] = (double)c->diff.P * ((double)c->n + (double)c->diff.G * c->diff.G);
+    order[i] = i;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return work[x] > work[y]; });
+  double load[kDiffLanes] = {};
+  b->lane_of.assign(n, 0);
+  for (size_t i : order) {
+    int best = 0;
+    for (int k = 1; k < kDiffLanes; k++)
+      if (load[k] < load[best]) best = k;
+    b->lane_of[i] = best;
+    load[best] += work[i];
+  }
+}
+
+hipError_t dbatch_enqueue_one(soap_diffbatch::Item &it, hipStream_t st);
+
 int dbatch_enqueue(soap_diffbatch *b) {
-  for (size_t i = 0; i < b->items.size(); i++) {
-    auto &it = b->items[i];
-    if (!it.bound) return dfail(b, SOAP_ERR_STATE, "diff batch: instance %zu not bound", i);
+  for (size_t i = 0; i < b->items.size(); i++)
+    if (!b->items[i].bound) return dfail(b, SOAP_ERR_STATE, "diff batch: instance %zu not bound", i);
+  hipError_t e = hipEventRecord(b->fork, b->stream);
+  for (int k = 0; k < kDiffLanes && e == hipSuccess; k++) e = hipStreamWaitEvent(b->lane[k], b->fork, 0);
+  if (e != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff batch: fork: %s", hipGetErrorString(e));
+  int status = SOAP_OK;
+  for (size_t i = 0; i < b->items.size() && status == SOAP_OK; i++) {
+    e = dbatch_enqueue_one(b->items[i], b->lane[b->lane_of[i]]);
+    if (e != hipSuccess)
+      status = dfail(b, e == hipErrorInvalidValue ? SOAP_ERR_LIMIT : SOAP_ERR_HIP, "diff batch: instance %zu: %s", i,
+                     hipGetErrorString(e));
+  }
+  // join every lane back, also after a failed launch, so that a capture can end
+  for (int k = 0; k < kDiffLanes; k++) {
+    hipError_t ej = hipEventRecord(b->join[k], b->lane[k]);
+    if (ej == hipSuccess) ej = hipStreamWaitEvent(b->stream, b->join[k], 0);
+    if (ej != hipSuccess && status == SOAP_OK)
+      status = dfail(b, SOAP_ERR_HIP, "diff batch: join: %s", hipGetErrorString(ej));
+  }
+  return status;
+}
+
+// Forward, loss and adjoint of one bound instance on stream st.
+hipError_t dbatch_enqueue_one(soap_diffbatch::Item &it, hipStream_t st) {
+  {
     soap_ctx *c = it.c;
     auto &d = c->diff;
     DiffArgs a = diff_args(c);
     a.prm = it.prm;
     ConvArgs ca = diff_conv(c);
     int rb = 0;
-    hipError_t e = launch_diff_forward_spectra(a, ca, it.target, c->trim, (double *)d.res_part.p, &rb, b->stream);
+    hipError_t e = launch_diff_forward_spectra(a, ca, it.target, c->trim, (double *)d.res_part.p, &rb, st);
     const int64_t n_t = c->n - 2 * (int64_t)c->trim;
     if (e == hipSuccess)
       e = launch_ccf((const double *)d.F.p, c->n, c->trim, d.P, c->n_lines, c->n_v, n_t, (const int32_t *)c->m_start.p,
                      (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p, c->d_w_sorted,
-                     (double *)c->ccf_part.p, b->stream);
+                     (double *)c->ccf_part.p, st);
     if (e == hipSuccess)
       e = launch_ccf_sum((const double *)c->ccf_part.p, ccf_chunks(c->n_lines), d.P, c->n_v, (double *)d.ccf.p,
-                         b->stream);
+                         st);
     if (e == hipSuccess)
       e = launch_fit(c->d_vgrid, c->n_v, (const double *)d.ccf.p, d.P, (double *)d.par.p, (double *)d.model.p,
-                     (double *)d.depth.p, (double *)d.bis.p, (int32_t *)d.iters.p, b->stream);
+                     (double *)d.depth.p, (double *)d.bis.p, (int32_t *)d.iters.p, st);
     if (e == hipSuccess)
       e = launch_fit_polish(c->d_vgrid, c->n_v, (const double *)d.ccf.p, (const double *)d.par.p, d.P,
-                            (double *)d.th.p, b->stream);
+                            (double *)d.th.p, st);
     if (e == hipSuccess)
       e = launch_diff_loss(d.P, rb, (const double *)d.res_part.p, (const double *)d.th.p, it.rv_t, it.fw_t, it.w_spec,
-                           it.w_rv, it.w_fw, it.prm, it.vsini_max, it.loss, (double *)d.dth.p, b->stream);
+                           it.w_rv, it.w_fw, it.prm, it.vsini_max, it.loss, (double *)d.dth.p, st);
     if (e == hipSuccess)
       e = launch_diff_backward(a, ca, it.target, c->trim, it.w_spec, c->d_vgrid, c->n_v, (const double *)d.ccf.p,
                                (const double *)d.th.p, (const double *)d.dth.p, (double *)d.gy.p, n_t,
                                (const int64_t *)d.mt_ptr.p, (const int32_t *)d.mt_w.p, (const int32_t *)d.mt_col.p,
                                (const double *)d.mt_val.p, (double *)d.GF.p, (double *)d.GI.p, (double *)d.Cpart.p,
-                               (double *)d.C.p, it.gmaps, (double *)d.cell_part.p, it.gq, it.gs, b->stream);
+                               (double *)d.C.p, it.gmaps, (double *)d.cell_part.p, it.gq, it.gs, st);
     if (e == hipSuccess) e = launch_diff_grad4(diff_cell_blocks((int64_t)d.G * d.G), (const double *)d.cell_part.p,
-                                               it.g4, b->stream);
-    if (e != hipSuccess)
-      return dfail(b, e == hipErrorInvalidValue ? SOAP_ERR_LIMIT : SOAP_ERR_HIP, "diff batch: instance %zu: %s", i,
-                   hipGetErrorString(e));
+                                               it.g4, st);
+    return e;
   }
-  return SOAP_OK;
 }
 }  // namespace
 
@@ -2474,7 +2525,13 @@ int soap_diff_batch_create(soap_ctx *const *ctxs, int32_t n_inst, soap_diffbatch
   }
   hipError_t e = hipSetDevice(b->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  for (int k = 0; k < kDiffLanes && e == hipSuccess; k++) {
+    e = hipStreamCreateWithFlags(&b->lane[k], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming);
+  }
   if (e != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff_batch_create: %s", hipGetErrorString(e));
+  dbatch_assign_lanes(b);
   return SOAP_OK;
 }
 
@@ -2482,7 +2539,14 @@ void soap_diff_batch_destroy(soap_diffbatch *b) {
   if (!b) return;
   hipSetDevice(b->device);
   if (b->stream) hipStreamSynchronize(b->stream);
+  for (int k = 0; k < kDiffLanes; k++)
+    if (b->lane[k]) hipStreamSynchronize(b->lane[k]);
   dbatch_drop_graph(b);
+  for (int k = 0; k < kDiffLanes; k++) {
+    if (b->join[k]) hipEventDestroy(b->join[k]);
+    if (b->lane[k]) hipStreamDestroy(b->lane[k]);
+  }
+  if (b->fork) hipEventDestroy(b->fork);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
 }
