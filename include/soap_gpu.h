@@ -209,23 +209,27 @@ int soap_set_fir_shape(soap_ctx *ctx, int32_t phases_per_block, int32_t pixels_p
  *   I_p = sum_c L_c [(1 - f) Shift(Q_k) + contrast f Shift(S_k)],
  *   F_p = instrumental convolution of I_p, CCF_p = raw mask CCF of F_p,
  *   (RV_p, FWHM_p) = Gaussian fit of CCF_p polished to the least-squares optimum,
+ *   BIS_p = bisector span of CCF_p at that Gaussian (BIS_FIT2.cpp:214-268),
  *   loss = w_spec sum (F_p - T_p)^2 over the trimmed pixels
- *        + w_rv sum (RV_p - rv_target_p)^2 + w_fwhm sum (FWHM_p - fwhm_target_p)^2.
+ *        + w_rv sum (RV_p - rv_target_p)^2 + w_fwhm sum (FWHM_p - fwhm_target_p)^2
+ *        + w_bis sum (BIS_p - bis_target_p)^2.
  * Array arguments named d_* are device (HIP) pointers: seeds [n_mu][n], maps
  * [P][M][M], target [P][n]; they must stay valid until soap_diff_backward. */
 typedef struct soap_diff_params {
   double vsini_kms, u1, u2, contrast;
 } soap_diff_params;
 int soap_diff_setup(soap_ctx *ctx, int32_t grid, int32_t n_phase, int32_t map_res, double sigma_ms);
-/* rv_target, fwhm_target, loss, rv, fwhm: host arrays ([P], scalar, [P], [P]) */
+/* rv_target, fwhm_target, bis_target [P], loss (scalar), rv, fwhm, bis [P]: host
+ * arrays (rv, fwhm, bis may be NULL) */
 int soap_diff_forward(soap_ctx *ctx, const soap_diff_params *prm, const double *d_seed_q, const double *d_seed_s,
                       const double *d_maps, const double *d_target, const double *rv_target,
-                      const double *fwhm_target, double w_spec, double w_rv, double w_fwhm, double *loss, double *rv,
-                      double *fwhm);
+                      const double *fwhm_target, const double *bis_target, double w_spec, double w_rv, double w_fwhm,
+                      double w_bis, double *loss, double *rv, double *fwhm, double *bis);
 /* gradients of the last forward's loss: grad4 (host) = d/d(vsini, u1, u2, contrast);
  * d_grad_q, d_grad_s [n_mu][n] and d_grad_maps [P][M][M] on the device (overwritten) */
 int soap_diff_backward(soap_ctx *ctx, double *grad4, double *d_grad_q, double *d_grad_s, double *d_grad_maps);
-/* host copies of the last forward: what 0 = F [P][n], 1 = raw CCF [P][n_v], 2 = (c, k, RV, FWHM) [P][4] */
+/* host copies of the last forward: what 0 = F [P][n], 1 = raw CCF [P][n_v], 2 = (c, k, RV, FWHM) [P][4],
+ * 3 = BIS span [P] */
 int soap_diff_get(soap_ctx *ctx, int32_t what, double *out);
 
 /* Algorithmic work of one run of a stage (index as soap_stage_name) on the
@@ -279,7 +283,7 @@ int soap_batch_tune(soap_batch *b, int32_t reps);
  * soap_ctx after soap_diff_setup) with every input, parameter and result in
  * device memory, so that one evaluation is one HIP graph launch:
  *   d_params [4] = (vsini, u1, u2, contrast), d_q / d_s [n_mu][n], d_maps [P][M][M],
- *   d_target [P][n], d_rv_t / d_fw_t [P]  ->  d_loss [1], d_grad4 [4],
+ *   d_target [P][n], d_rv_t / d_fw_t / d_bis_t [P]  ->  d_loss [1], d_grad4 [4],
  *   d_grad_q / d_grad_s [n_mu][n], d_grad_maps [P][M][M].
  * The tap range is fixed at bind time by vsini_max; an evaluation with vsini
  * outside (0, vsini_max] returns loss = NaN.  The graph is captured on the first
@@ -292,11 +296,13 @@ void soap_diff_batch_destroy(soap_diffbatch *b);
 const char *soap_diff_batch_last_error(const soap_diffbatch *b);
 int soap_diff_batch_bind(soap_diffbatch *b, int32_t inst, double vsini_max, const double *d_params,
                          const double *d_seed_q, const double *d_seed_s, const double *d_maps, const double *d_target,
-                         const double *d_rv_target, const double *d_fwhm_target, double w_spec, double w_rv,
-                         double w_fwhm, double *d_loss, double *d_grad4, double *d_grad_q, double *d_grad_s,
-                         double *d_grad_maps);
-/* enqueue forward + adjoint of every instance; use_graph != 0 replays the captured graph */
-int soap_diff_batch_step(soap_diffbatch *b, int32_t use_graph);
+                         const double *d_rv_target, const double *d_fwhm_target, const double *d_bis_target,
+                         double w_spec, double w_rv, double w_fwhm, double w_bis, double *d_loss, double *d_grad4,
+                         double *d_grad_q, double *d_grad_s, double *d_grad_maps);
+/* enqueue forward + loss of every instance, and the adjoint when with_adjoint != 0
+ * (without it d_grad* are left as they are); use_graph != 0 replays a captured
+ * graph (one per with_adjoint value) */
+int soap_diff_batch_step(soap_diffbatch *b, int32_t use_graph, int32_t with_adjoint);
 int soap_diff_batch_sync(soap_diffbatch *b);
 
 #ifdef __cplusplus

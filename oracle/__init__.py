@@ -80,10 +80,12 @@ def lib():
         L.orc_gauss_hess.argtypes = [_dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_gauss_polish.argtypes = [_dp, _dp, ctypes.c_int, _dp]
         _d_ = ctypes.c_double
+        L.orc_bisector.argtypes = [_dp, _dp, ctypes.c_int, _d_, _d_, _d_, _d_, _dp, _dp]
+        L.orc_bisector.restype = _d_
         L.orc_diff_forward.argtypes = ([_dp, ctypes.c_long, ctypes.c_int, ctypes.c_int, _dp, _d_, _d_, _d_, _d_,
                                         _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _d_, ctypes.c_int, _dp,
-                                        ctypes.c_int, _dp, _dp, ctypes.c_long, _dp, _dp, _dp, _d_, _d_, _d_,
-                                        _dp, _dp, _dp, _dp, ctypes.c_int])
+                                        ctypes.c_int, _dp, _dp, ctypes.c_long, _dp, _dp, _dp, _dp, _d_, _d_, _d_,
+                                        _d_, _dp, _dp, _dp, _dp, _dp, ctypes.c_int])
         _lib = L
     return _lib
 
@@ -328,9 +330,21 @@ def gauss_polish(x, y, theta):
     return th
 
 
+def bisector(v, y, theta):
+    """(span, depth[91], bis[91]) of a CCF for the Gaussian theta = (c, k signed, v0, fwhm)."""
+    vv, vp = _d(v)
+    yy, yp = _d(y)
+    depth = np.zeros(91)
+    bis = np.zeros(91)
+    c, k, v0, fw = (float(x) for x in theta)
+    span = lib().orc_bisector(vp, yp, len(vv), c, k, v0, fw, depth.ctypes.data_as(_dp), bis.ctypes.data_as(_dp))
+    return float(span), depth, bis
+
+
 def diff_forward(wave, grid, mu_bounds, vsini, u1, u2, kappa, cube_q, cube_s, maps, sigma_ms, trim, vgrid,
-                 line_l, line_w, target, rv_t, fw_t, w_spec, w_rv, w_fw, nthreads=None):
-    """Forward model + loss of the differentiable path; returns dict(loss, F[P,n], ccf[P,nv], theta[P,4])."""
+                 line_l, line_w, target, rv_t, fw_t, w_spec, w_rv, w_fw, bis_t=None, w_bis=0.0, nthreads=None):
+    """Forward model + loss of the differentiable path; returns dict(loss, F[P,n], ccf[P,nv], theta[P,4],
+    span[P])."""
     w, wp = _d(wave)
     n = len(w)
     mb, mbp = _d(mu_bounds)
@@ -344,13 +358,15 @@ def diff_forward(wave, grid, mu_bounds, vsini, u1, u2, kappa, cube_q, cube_s, ma
     t, tp = _d(target)
     rt, rtp = _d(rv_t)
     ft, ftp = _d(fw_t)
+    bt, btp = _d(np.zeros(P) if bis_t is None else bis_t)
     F = np.zeros((P, n))
     cc = np.zeros((P, len(v)))
     th = np.zeros((P, 4))
+    span = np.zeros(P)
     loss = np.zeros(1)
     lib().orc_diff_forward(wp, n, int(grid), len(mb) + 1, mbp, float(vsini), float(u1), float(u2), float(kappa),
                            qp, sp, P, M, mpp, float(sigma_ms), int(trim), vp, len(v), llp, lwp, len(ll), tp, rtp,
-                           ftp, float(w_spec), float(w_rv), float(w_fw), F.ctypes.data_as(_dp),
-                           cc.ctypes.data_as(_dp), th.ctypes.data_as(_dp), loss.ctypes.data_as(_dp),
-                           nthreads or nthreads_default())
-    return dict(loss=float(loss[0]), F=F, ccf=cc, theta=th)
+                           ftp, btp, float(w_spec), float(w_rv), float(w_fw), float(w_bis), F.ctypes.data_as(_dp),
+                           cc.ctypes.data_as(_dp), th.ctypes.data_as(_dp), span.ctypes.data_as(_dp),
+                           loss.ctypes.data_as(_dp), nthreads or nthreads_default())
+    return dict(loss=float(loss[0]), F=F, ccf=cc, theta=th, span=span)

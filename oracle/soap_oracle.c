@@ -761,6 +761,71 @@ static int orc_solve4(const double A[4][4], const double *D, double mu, const do
   return 0;
 }
 
+/* Bisector and span of a CCF y over velocities v for the Gaussian (c, k, v0, fwhm)
+ * with signed amplitude k and absolute centre v0, BIS_FIT2.cpp:214-268: per-interval
+ * quadratic inverse p0 + p1 d + p2 d^2 of the normalised CCF norm = -c / k (1 - y / c),
+ * p2 from the Gaussian's inverse curvature at the interval centre; for each depth d = (i + 5) / 100, i = 0..90, the intervals
+ * reached by walking out from argmax(norm) while norm > d; bis = the mean of the
+ * two sides; span = mean(bis, d in [0.1, 0.4]) - mean(bis, d in [0.6, 0.9]).
+ * depth91 = 1 - |k| d and bis91 optional.  Returns the span. */
+double orc_bisector(const double *v, const double *y, int n, double c, double k, double v0, double fwhm,
+                    double *depth91, double *bis91) {
+  int nstep = 100, margin = 5, len_depth = nstep - 2 * margin + 1;
+  double sigma = fwhm / 2. / sqrt(2. * log(2.));
+  double *norm_ccf = (double *)malloc(sizeof(double) * n);
+  double *p0 = (double *)calloc((size_t)n, sizeof(double));
+  double *p1 = (double *)calloc((size_t)n, sizeof(double));
+  double *p2 = (double *)calloc((size_t)n, sizeof(double));
+  double depth[91], bis[91];
+  for (int i = 0; i < n; i++) norm_ccf[i] = -c / k * (1. - y[i] / c);
+  for (int i = 0; i < len_depth; i++) depth[i] = (double)(i + margin) / nstep;
+  for (int i = 0; i < n - 1; i++) {
+    double vr = (v[i] + v[i + 1]) / 2.;
+    double s2 = sigma * sigma;
+    double dd = (vr - v0) * (vr - v0);
+    double dccf = -(vr - v0) / s2 * exp(-dd / 2. / s2);
+    double d2ccf = (dd / s2 - 1) / s2 * exp(-dd / 2. / s2);
+    double d2rv = -d2ccf / (dccf * dccf * dccf);
+    p2[i] = d2rv / 2.;
+    p1[i] = (v[i + 1] - v[i] - p2[i] * (norm_ccf[i + 1] * norm_ccf[i + 1] - norm_ccf[i] * norm_ccf[i])) /
+            (norm_ccf[i + 1] - norm_ccf[i]);
+    p0[i] = v[i] - p1[i] * norm_ccf[i] - p2[i] * (norm_ccf[i] * norm_ccf[i]);
+  }
+  int ind_max = 0;
+  for (int i = 0; i < n; i++)
+    if (norm_ccf[i] > norm_ccf[ind_max]) ind_max = i;
+  for (int i = 0; i < len_depth; i++) {
+    int i_b = ind_max, i_r = ind_max;
+    while ((norm_ccf[i_b] > depth[i]) && (i_b > 1)) i_b--;
+    while ((norm_ccf[i_r + 1] > depth[i]) && (i_r < (n - 2))) i_r++;
+    bis[i] = (p0[i_b] + p0[i_r]) + (p1[i_b] + p1[i_r]) * depth[i] + (p2[i_b] + p2[i_r]) * (depth[i] * depth[i]);
+    bis[i] /= 2.;
+  }
+  int n1 = 0, n2 = 0;
+  double rv_top = 0., rv_bottom = 0.;
+  for (int i = 0; i < len_depth; i++) {
+    if ((depth[i] >= 0.1) && (depth[i] <= 0.4)) {
+      n1++;
+      rv_top += bis[i];
+    }
+    if ((depth[i] >= 0.6) && (depth[i] <= 0.9)) {
+      n2++;
+      rv_bottom += bis[i];
+    }
+  }
+  rv_top /= n1;
+  rv_bottom /= n2;
+  if (depth91)
+    for (int i = 0; i < len_depth; i++) depth91[i] = 1.0 - fabs(k) * depth[i];
+  if (bis91)
+    for (int i = 0; i < len_depth; i++) bis91[i] = bis[i];
+  free(norm_ccf);
+  free(p0);
+  free(p1);
+  free(p2);
+  return rv_top - rv_bottom;
+}
+
 /* BIS_FIT2 main, BIS_FIT2.cpp:97-320.  The GSL trust-region solver
  * (gsl_multifit_nlinear_trust, default parameters) is restated as a
  * Levenberg-Marquardt iteration with More scaling, Nielsen damping updates and
@@ -878,67 +943,12 @@ int orc_fit_ccf(const double *v, const double *y, int n, double *par5, double *m
   fwhm = x[3];
   if (model) orc_gauss(px, y, n, x, NULL, model, NULL);
 
-  /* bisector, BIS_FIT2.cpp:214-268 */
-  int nstep = 100, margin = 5, len_depth = nstep - 2 * margin + 1;
-  double sigma = fwhm / 2. / sqrt(2. * log(2.));
-  double v0 = x0 + v[0];
-  double *norm_ccf = (double *)malloc(sizeof(double) * n);
-  double *p0 = (double *)calloc((size_t)n, sizeof(double));
-  double *p1 = (double *)calloc((size_t)n, sizeof(double));
-  double *p2 = (double *)calloc((size_t)n, sizeof(double));
-  double depth[91], bis[91];
-  for (int i = 0; i < n; i++) norm_ccf[i] = -c / k * (1. - y[i] / c);
-  for (int i = 0; i < len_depth; i++) depth[i] = (double)(i + margin) / nstep;
-  for (int i = 0; i < n - 1; i++) {
-    double vr = (v[i] + v[i + 1]) / 2.;
-    double s2 = sigma * sigma;
-    double dd = (vr - v0) * (vr - v0);
-    double dccf = -(vr - v0) / s2 * exp(-dd / 2. / s2);
-    double d2ccf = (dd / s2 - 1) / s2 * exp(-dd / 2. / s2);
-    double d2rv = -d2ccf / (dccf * dccf * dccf);
-    p2[i] = d2rv / 2.;
-    p1[i] = (v[i + 1] - v[i] - p2[i] * (norm_ccf[i + 1] * norm_ccf[i + 1] - norm_ccf[i] * norm_ccf[i])) /
-            (norm_ccf[i + 1] - norm_ccf[i]);
-    p0[i] = v[i] - p1[i] * norm_ccf[i] - p2[i] * (norm_ccf[i] * norm_ccf[i]);
-  }
-  int ind_max = 0;
-  for (int i = 0; i < n; i++)
-    if (norm_ccf[i] > norm_ccf[ind_max]) ind_max = i;
-  for (int i = 0; i < len_depth; i++) {
-    int i_b = ind_max, i_r = ind_max;
-    while ((norm_ccf[i_b] > depth[i]) && (i_b > 1)) i_b--;
-    while ((norm_ccf[i_r + 1] > depth[i]) && (i_r < (n - 2))) i_r++;
-    bis[i] = (p0[i_b] + p0[i_r]) + (p1[i_b] + p1[i_r]) * depth[i] + (p2[i_b] + p2[i_r]) * (depth[i] * depth[i]);
-    bis[i] /= 2.;
-  }
-  int n1 = 0, n2 = 0;
-  double rv_top = 0., rv_bottom = 0.;
-  for (int i = 0; i < len_depth; i++) {
-    if ((depth[i] >= 0.1) && (depth[i] <= 0.4)) {
-      n1++;
-      rv_top += bis[i];
-    }
-    if ((depth[i] >= 0.6) && (depth[i] <= 0.9)) {
-      n2++;
-      rv_bottom += bis[i];
-    }
-  }
-  rv_top /= n1;
-  rv_bottom /= n2;
-  double span = rv_top - rv_bottom;
-  if (depth91)
-    for (int i = 0; i < len_depth; i++) depth91[i] = 1.0 - fabs(k) * depth[i];
-  if (bis91)
-    for (int i = 0; i < len_depth; i++) bis91[i] = bis[i];
+  double span = orc_bisector(v, y, n, c, k, x0 + v[0], fwhm, depth91, bis91);
   par5[0] = c;
   par5[1] = fabs(k);
   par5[2] = span;
-  par5[3] = v0;
+  par5[3] = x0 + v[0];
   par5[4] = fwhm;
-  free(norm_ccf);
-  free(p0);
-  free(p1);
-  free(p2);
   free(px);
   free(f);
   free(ft);
@@ -1077,16 +1087,19 @@ int orc_gauss_polish(const double *x, const double *y, int n, double *th) {
  *   zero outside wid < i < n-1-wid;
  *   CCF_p = raw mask CCF of F_p[trim, n-trim) (orc_ccf, CCF_func.py:5-91);
  *   theta_p = (c, k, x0, fwhm) of the Gaussian fit (orc_fit_ccf) polished to the
- *   optimum (orc_gauss_polish); RV_p = x0, FWHM_p = fwhm;
+ *   optimum (orc_gauss_polish); RV_p = x0, FWHM_p = fwhm; BIS_p = the bisector span of
+ *   the CCF at that optimum (orc_bisector);
  *   loss = w_spec sum_p sum_{trim <= j < n-trim} (F_p - T_p)^2
- *        + w_rv sum_p (RV_p - rv_t_p)^2 + w_fw sum_p (FWHM_p - fw_t_p)^2.
- * Outputs F [P][n], ccf [P][n_v], theta [P][4] (each optional) and loss. */
+ *        + w_rv sum_p (RV_p - rv_t_p)^2 + w_fw sum_p (FWHM_p - fw_t_p)^2
+ *        + w_bis sum_p (BIS_p - bis_t_p)^2   (the last term only when w_bis != 0).
+ * Outputs F [P][n], ccf [P][n_v], theta [P][4], span [P] (each optional) and loss. */
 int orc_diff_forward(const double *wave, long n, int grid, int n_mu, const double *mu_bounds, double vsini,
                      double u1, double u2, double kappa, const double *cube_q, const double *cube_s, int n_phase,
                      int map_res, const double *maps, double sigma_ms, int trim, const double *vgrid, int n_v,
                      const double *line_l, const double *line_w, long n_lines, const double *target,
-                     const double *rv_t, const double *fw_t, double w_spec, double w_rv, double w_fw, double *F_out,
-                     double *ccf_out, double *theta_out, double *loss_out, int nthreads) {
+                     const double *rv_t, const double *fw_t, const double *bis_t, double w_spec, double w_rv,
+                     double w_fw, double w_bis, double *F_out, double *ccf_out, double *theta_out, double *span_out,
+                     double *loss_out, int nthreads) {
   const double pi = ORC_PI;
   long ncell = (long)grid * grid;
   int M = map_res;
@@ -1187,6 +1200,9 @@ int orc_diff_forward(const double *wave, long n, int grid, int n_mu, const doubl
     orc_gauss_polish(vgrid, cc, n_v, th);
     if (theta_out) memcpy(theta_out + (size_t)p * 4, th, sizeof(th));
     loss += w_rv * (th[2] - rv_t[p]) * (th[2] - rv_t[p]) + w_fw * (th[3] - fw_t[p]) * (th[3] - fw_t[p]);
+    double span = orc_bisector(vgrid, cc, n_v, th[0], th[1], th[2], th[3], NULL, NULL);
+    if (span_out) span_out[p] = span;
+    if (w_bis != 0.0) loss += w_bis * (span - bis_t[p]) * (span - bis_t[p]);
   }
   *loss_out = loss;
   free(cidx);

@@ -333,9 +333,10 @@ __global__ void k_fit_polish(const double *v, int n_v, const double *ccf, const 
     for (int a = 0; a < 4; a++) theta[(long)sp * 4 + a] = th[a];
 }
 
-// dl/dy_v = J_v . z with H z = dl/dtheta (implicit derivative of the optimum)
+// dl/dy_v = J_v . z with H z = dl/dtheta (implicit derivative of the optimum), plus
+// the bisector span's direct term coef_p dspan_p/dy_v when bisd holds one
 __global__ void k_fit_adjoint(const double *v, int n_v, const double *ccf, const double *theta, const double *dth,
-                              double *gy) {
+                              const double *bisd, double *gy) {
   int sp = blockIdx.x;
   const double *y = ccf + (long)sp * n_v;
   double th[4], g[4], H[4][4], r2, z[4] = {0, 0, 0, 0};
@@ -345,10 +346,151 @@ __global__ void k_fit_adjoint(const double *v, int n_v, const double *ccf, const
   for (int a = 0; a < 4; a++) b[a] = dth[(long)sp * 4 + a];
   solve4(H, b, z);
   double s = th[3] / (2 * sqrt(2 * log(2.0)));
+  const double *bp = bisd ? bisd + (long)sp * (n_v + kBisHead) : nullptr;
+  const double coef = bp ? bp[1] : 0.0;
   for (int i = threadIdx.x; i < n_v; i += blockDim.x) {
     double u = (v[i] - th[2]) / s, e = exp(-u * u / 2);
-    gy[(long)sp * n_v + i] = z[0] + z[1] * e + z[2] * th[1] * e * u / s + z[3] * th[1] * e * u * u / th[3];
+    double g = z[0] + z[1] * e + z[2] * th[1] * e * u / s + z[3] * th[1] * e * u * u / th[3];
+    if (coef != 0.0) g += coef * bp[kBisHead + i];
+    gy[(long)sp * n_v + i] = g;
   }
+}
+
+// ------------------------------------------------------------ bisector span
+// p2 of the interval centred at vr (BIS_FIT2.cpp:233-236: half the inverse
+// curvature of the Gaussian at vr) and its derivatives in the centre v0 and in
+// s2 = sigma^2.  With u = vr - v0, E = exp(-u^2 / 2 s2): dccf = -u E / s2,
+// d2ccf = (u^2 / s2 - 1) E / s2, p2 = -d2ccf / (2 dccf^3); d dccf / du = d2ccf,
+// d d2ccf / du = E u (3 - u^2 / s2) / s2^2, d dccf / ds2 = u E (1 - u^2 / 2 s2) / s2^2,
+// d d2ccf / ds2 = E [(s2 - 2 u^2) / s2^3 + (u^2 - s2) u^2 / (2 s2^4)].
+__device__ void bis_p2(double vr, double v0, double s2, double *p2, double *dp2_dv0, double *dp2_ds2) {
+  double dd = (vr - v0) * (vr - v0);
+  double dccf = -(vr - v0) / s2 * exp(-dd / 2. / s2);
+  double d2ccf = (dd / s2 - 1) / s2 * exp(-dd / 2. / s2);
+  double d2rv = -d2ccf / (dccf * dccf * dccf);
+  *p2 = d2rv / 2.;
+  double u = vr - v0, E = exp(-dd / 2. / s2);
+  double d2ccf_u = E * u * (3 - dd / s2) / (s2 * s2);
+  double dccf_s = u * E / (s2 * s2) * (1 - dd / (2 * s2));
+  double d2ccf_s = E * ((s2 - 2 * dd) / (s2 * s2 * s2) + (dd - s2) * dd / (2 * s2 * s2 * s2 * s2));
+  double i3 = 1.0 / (dccf * dccf * dccf);
+  double dp2_u = -d2ccf_u * i3 / 2 + 1.5 * d2ccf * d2ccf * i3 / dccf;
+  *dp2_dv0 = -dp2_u;
+  *dp2_ds2 = -d2ccf_s * i3 / 2 + 1.5 * d2ccf * dccf_s * i3 / dccf;
+}
+
+// One 64-lane block per CCF: the span of BIS_FIT2.cpp:214-268 at the polished
+// Gaussian theta = (c, k, v0, fwhm) (the forward of the plain fit's bisector,
+// with the same per-interval quadratic inverse and walk), and its partial
+// derivatives at fixed theta: dspan/dy_v and dspan/dtheta.  Per interval s with
+// a = norm_s, b = norm_{s+1}, the quadratic inverse at depth d is
+// v_s + dv (d - a) / (b - a) + p2 (d - a) (d - b), so
+//   dterm/da = (d - b) (dv / (b - a)^2 - p2), dterm/db = -(d - a) (dv / (b - a)^2 + p2),
+//   dterm/dp2 = (d - a) (d - b);  norm_j = (y_j - c) / k.
+// Output per CCF (stride n_v + kBisHead): [0] span, [1] loss coefficient (written
+// by k_diff_loss), [2..5] dspan/dtheta, [kBisHead + v] dspan/dy_v.
+constexpr int kBisLevels = 91;
+__global__ __launch_bounds__(64) void k_diff_bis(const double *v, int n_v, const double *ccf, const double *theta,
+                                                 double *bisd) {
+  __shared__ double s_nc[kMaxVDiff], s_p0[kMaxVDiff], s_p1[kMaxVDiff], s_p2[kMaxVDiff];
+  __shared__ double s_bis[kBisLevels], s_dA[kBisLevels][2], s_dB[kBisLevels][2], s_th[kBisLevels][4];
+  __shared__ int s_side[kBisLevels][2];
+  __shared__ int s_imax;
+  const int sp = blockIdx.x;
+  const double *y = ccf + (long)sp * n_v;
+  double *out = bisd + (long)sp * (n_v + kBisHead);
+  const double c = theta[sp * 4 + 0], k = theta[sp * 4 + 1], v0 = theta[sp * 4 + 2], fwhm = theta[sp * 4 + 3];
+  const double sigma = fwhm / 2. / sqrt(2. * log(2.));
+  const double s2 = sigma * sigma;
+  for (int i = threadIdx.x; i < n_v; i += 64) s_nc[i] = -c / k * (1. - y[i] / c);
+  __syncthreads();
+  for (int i = threadIdx.x; i < n_v - 1; i += 64) {
+    double p2, d0, d1;
+    bis_p2((v[i] + v[i + 1]) / 2., v0, s2, &p2, &d0, &d1);
+    double p1 = (v[i + 1] - v[i] - p2 * (s_nc[i + 1] * s_nc[i + 1] - s_nc[i] * s_nc[i])) / (s_nc[i + 1] - s_nc[i]);
+    s_p2[i] = p2;
+    s_p1[i] = p1;
+    s_p0[i] = v[i] - p1 * s_nc[i] - p2 * (s_nc[i] * s_nc[i]);
+  }
+  if (threadIdx.x == 0) {
+    int im = 0;
+    for (int i = 0; i < n_v; i++)
+      if (s_nc[i] > s_nc[im]) im = i;
+    s_imax = im;
+  }
+  __syncthreads();
+  const int ind_max = s_imax;
+  for (int i = threadIdx.x; i < kBisLevels; i += 64) {
+    double dep = (double)(i + 5) / 100;
+    int i_b = ind_max, i_r = ind_max;
+    while ((s_nc[i_b] > dep) && (i_b > 1)) i_b--;
+    while ((s_nc[i_r + 1] > dep) && (i_r < (n_v - 2))) i_r++;
+    double b = (s_p0[i_b] + s_p0[i_r]) + (s_p1[i_b] + s_p1[i_r]) * dep + (s_p2[i_b] + s_p2[i_r]) * (dep * dep);
+    s_bis[i] = b / 2.;
+    // weight of this level in the span, halved for the mean of the two sides
+    double w = ((dep >= 0.1) && (dep <= 0.4)) ? 0.5 / 31 : ((dep >= 0.6) && (dep <= 0.9)) ? -0.5 / 31 : 0.0;
+    double th4[4] = {0, 0, 0, 0};
+    for (int side = 0; side < 2; side++) {
+      int sx = side ? i_r : i_b;
+      s_side[i][side] = sx;
+      if (w == 0.0 || sx + 1 >= n_v) {
+        s_dA[i][side] = 0.0;
+        s_dB[i][side] = 0.0;
+        continue;
+      }
+      double a = s_nc[sx], bb = s_nc[sx + 1], dv = v[sx + 1] - v[sx];
+      double p2, dp2_v0, dp2_s2;
+      bis_p2((v[sx] + v[sx + 1]) / 2., v0, s2, &p2, &dp2_v0, &dp2_s2);
+      double q = dv / ((bb - a) * (bb - a));
+      double dA = w * (dep - bb) * (q - p2), dB = -w * (dep - a) * (q + p2), dP = w * (dep - a) * (dep - bb);
+      s_dA[i][side] = dA / k;
+      s_dB[i][side] = dB / k;
+      th4[0] += -(dA + dB) / k;
+      th4[1] += -(dA * a + dB * bb) / k;
+      th4[2] += dP * dp2_v0;
+      th4[3] += dP * dp2_s2 * 2 * s2 / fwhm;
+    }
+    for (int q4 = 0; q4 < 4; q4++) s_th[i][q4] = th4[q4];
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < n_v; j += 64) {
+    double g = 0.0;
+    for (int i = 0; i < kBisLevels; i++)
+      for (int side = 0; side < 2; side++) {
+        int sx = s_side[i][side];
+        if (sx == j) g += s_dA[i][side];
+        if (sx + 1 == j) g += s_dB[i][side];
+      }
+    out[kBisHead + j] = g;
+  }
+  if (threadIdx.x == 0) {
+    int n1 = 0, n2 = 0;
+    double top = 0., bot = 0.;
+    double th4[4] = {0, 0, 0, 0};
+    for (int i = 0; i < kBisLevels; i++) {
+      double dep = (double)(i + 5) / 100;
+      if ((dep >= 0.1) && (dep <= 0.4)) {
+        n1++;
+        top += s_bis[i];
+      }
+      if ((dep >= 0.6) && (dep <= 0.9)) {
+        n2++;
+        bot += s_bis[i];
+      }
+      for (int q4 = 0; q4 < 4; q4++) th4[q4] += s_th[i][q4];
+    }
+    out[0] = top / n1 - bot / n2;
+    out[1] = 0.0;
+    for (int q4 = 0; q4 < 4; q4++) out[2 + q4] = th4[q4];
+  }
+}
+
+hipError_t launch_diff_bis(const double *v, int n_v, const double *ccf, const double *theta, int n_ccf, double *bisd,
+                           hipStream_t s) {
+  if (n_ccf == 0) return hipSuccess;
+  if (n_v < 3 || n_v > kMaxVDiff) return hipErrorInvalidValue;
+  k_diff_bis<<<n_ccf, 64, 0, s>>>(v, n_v, ccf, theta, bisd);
+  return hipGetLastError();
 }
 
 // ------------------------------------------------------------ backward spectra
@@ -674,18 +816,26 @@ int diff_conv_blocks(int64_t n) { return (int)((n + 4 * kDT - 1) / (4 * kDT)); }
 // residual partials of phase p summed in order; the same order as the host sum
 __global__ __launch_bounds__(kDT) void k_diff_loss(int n_phase, int res_blocks, const double *res_part,
                                                    const double *theta, const double *rv_t, const double *fw_t,
-                                                   double w_spec, double w_rv, double w_fw, const double *prm,
+                                                   const double *bis_t, double w_spec, double w_rv, double w_fw,
+                                                   double w_bis, double *bisd, int n_v, const double *prm,
                                                    double vsini_max, double *loss, double *dth) {
   extern __shared__ double term[];  // [n_phase]
   for (int p = threadIdx.x; p < n_phase; p += blockDim.x) {
     double r2 = 0;
     for (int b = 0; b < res_blocks; b++) r2 += res_part[(long)p * res_blocks + b];
     double drv = theta[p * 4 + 2] - rv_t[p], dfw = theta[p * 4 + 3] - fw_t[p];
-    term[p] = w_spec * r2 + w_rv * drv * drv + w_fw * dfw * dfw;
-    dth[p * 4 + 0] = 0.0;
-    dth[p * 4 + 1] = 0.0;
-    dth[p * 4 + 2] = 2 * w_rv * drv;
-    dth[p * 4 + 3] = 2 * w_fw * dfw;
+    double t = w_spec * r2 + w_rv * drv * drv + w_fw * dfw * dfw;
+    double d4[4] = {0.0, 0.0, 2 * w_rv * drv, 2 * w_fw * dfw};
+    if (w_bis != 0.0) {  // span term; its coefficient feeds k_fit_adjoint's direct term
+      double *bp = bisd + (long)p * (n_v + kBisHead);
+      double db = bp[0] - bis_t[p];
+      t += w_bis * db * db;
+      double coef = 2 * w_bis * db;
+      for (int q = 0; q < 4; q++) d4[q] += coef * bp[2 + q];
+      bp[1] = coef;
+    }
+    term[p] = t;
+    for (int q = 0; q < 4; q++) dth[p * 4 + q] = d4[q];
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -705,11 +855,13 @@ __global__ void k_diff_grad4(int n_blocks, const double *cell_part, double *g4) 
 }
 
 hipError_t launch_diff_loss(int n_phase, int res_blocks, const double *res_part, const double *theta,
-                            const double *rv_t, const double *fw_t, double w_spec, double w_rv, double w_fw,
-                            const double *prm, double vsini_max, double *loss, double *dth, hipStream_t s) {
+                            const double *rv_t, const double *fw_t, const double *bis_t, double w_spec, double w_rv,
+                            double w_fw, double w_bis, double *bisd, int n_v, const double *prm, double vsini_max,
+                            double *loss, double *dth, hipStream_t s) {
   if (n_phase < 1 || (size_t)n_phase * 8 > 65536) return hipErrorInvalidValue;
-  k_diff_loss<<<1, kDT, (size_t)n_phase * 8, s>>>(n_phase, res_blocks, res_part, theta, rv_t, fw_t, w_spec, w_rv,
-                                                  w_fw, prm, vsini_max, loss, dth);
+  if (w_bis != 0.0 && (!bisd || !bis_t)) return hipErrorInvalidValue;
+  k_diff_loss<<<1, kDT, (size_t)n_phase * 8, s>>>(n_phase, res_blocks, res_part, theta, rv_t, fw_t, bis_t, w_spec,
+                                                  w_rv, w_fw, w_bis, bisd, n_v, prm, vsini_max, loss, dth);
   return hipGetLastError();
 }
 
@@ -730,9 +882,10 @@ hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const doub
                                 const double *v, int n_v, const double *ccf, const double *theta, const double *dth,
                                 double *gy, int64_t n_t, const int64_t *mt_ptr, const int32_t *mt_w,
                                 const int32_t *mt_col, const double *mt_val, double *GF, double *GI, double *Cpart,
-                                double *C, double *gmaps, double *cell_part, double *gq, double *gs, hipStream_t s) {
+                                double *C, double *gmaps, double *cell_part, double *gq, double *gs,
+                                const double *bisd, hipStream_t s) {
   hipError_t e;
-  k_fit_adjoint<<<a.n_phase, 64, 0, s>>>(v, n_v, ccf, theta, dth, gy);
+  k_fit_adjoint<<<a.n_phase, 64, 0, s>>>(v, n_v, ccf, theta, dth, bisd, gy);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   long tot = (long)a.n_phase * a.n;
   k_diff_gradF<<<(unsigned)((tot + kDT - 1) / kDT), kDT, 0, s>>>(a, target, trim, w_spec, GF);

@@ -91,12 +91,13 @@ struct soap_ctx {
   struct {
     bool ready = false, have_fwd = false;
     int G = 0, P = 0, M = 0, dmin = 0, nd = 0, res_blocks = 0;
-    double sigma = 0, w_spec = 0, w_rv = 0, w_fw = 0;
+    double sigma = 0, w_spec = 0;
     soap_diff_params prm{};
     const double *q = nullptr, *s = nullptr, *maps = nullptr, *target = nullptr;
-    std::vector<double> rv_t, fw_t, theta;
     DevBuf ck, cm, ct, cL, cmu, cdt, acc, taps, krange, I, F, res_part, ccf, par, model, depth, bis, iters, th,
         dth, gy, GF, GI, C, Cpart, cell_part;
+    DevBuf bisd;  // [P][n_v + kBisHead]: span, loss coefficient, dspan/dtheta, dspan/dy
+    DevBuf tg;    // per-call targets [3][P] (RV, FWHM, BIS span) and the loss [1]
     DevBuf mt_ptr, mt_w, mt_col, mt_val;  // the mask in sliced pixel-major form (adjoint of the CCF)
     bool mt_ready = false;
   } diff;
@@ -1662,6 +1663,7 @@ int soap_diff_setup(soap_ctx *c, int32_t grid, int32_t n_phase, int32_t map_res,
   if (c->kind != SOAP_GRID_LOG) return fail(c, SOAP_ERR_STATE, "diff_setup: the differentiable path needs a log grid");
   if (c->n == 0 || c->n_mu == 0) return fail(c, SOAP_ERR_STATE, "diff_setup: set wave and mu bounds first");
   if (!c->have_mask) return fail(c, SOAP_ERR_STATE, "diff_setup: set the mask first");
+  if (c->n_v < 3 || c->n_v > kMaxVDiff) return fail(c, SOAP_ERR_LIMIT, "diff_setup: %d CCF velocities", c->n_v);
   HIPCHK(c, hipSetDevice(c->device));
   auto &d = c->diff;
   size_t g2 = (size_t)grid * grid;
@@ -1676,6 +1678,8 @@ int soap_diff_setup(soap_ctx *c, int32_t grid, int32_t n_phase, int32_t map_res,
       (rc = ensure(c, d.bis, (size_t)n_phase * 91 * 8)) || (rc = ensure(c, d.iters, (size_t)n_phase * 4)) ||
       (rc = ensure(c, d.th, (size_t)n_phase * 4 * 8)) || (rc = ensure(c, d.dth, (size_t)n_phase * 4 * 8)) ||
       (rc = ensure(c, d.gy, (size_t)n_phase * c->n_v * 8)) ||
+      (rc = ensure(c, d.bisd, (size_t)n_phase * (c->n_v + kBisHead) * 8)) ||
+      (rc = ensure(c, d.tg, ((size_t)n_phase * 3 + 1) * 8)) ||
       (rc = ensure(c, d.cell_part, (size_t)diff_cell_blocks((int64_t)g2) * 4 * 8)) ||
       (rc = ensure(c, c->ccf_part, (size_t)ccf_chunks(c->n_lines) * n_phase * c->n_v * 8)))
     return rc;
@@ -1737,8 +1741,9 @@ static ConvArgs diff_conv(soap_ctx *c) {
 
 int soap_diff_forward(soap_ctx *c, const soap_diff_params *prm, const double *d_q, const double *d_s,
                       const double *d_maps, const double *d_target, const double *rv_t, const double *fw_t,
-                      double w_spec, double w_rv, double w_fw, double *loss, double *rv, double *fw) {
-  if (!c || !prm || !d_q || !d_s || !d_maps || !d_target || !rv_t || !fw_t || !loss)
+                      const double *bis_t, double w_spec, double w_rv, double w_fw, double w_bis, double *loss,
+                      double *rv, double *fw, double *bis) {
+  if (!c || !prm || !d_q || !d_s || !d_maps || !d_target || !rv_t || !fw_t || !bis_t || !loss)
     return fail(c, SOAP_ERR_ARG, "diff_forward: null argument");
   auto &d = c->diff;
   if (!d.ready) return fail(c, SOAP_ERR_STATE, "diff_forward: run diff_setup first");
@@ -1769,10 +1774,10 @@ int soap_diff_forward(soap_ctx *c, const soap_diff_params *prm, const double *d_
   d.maps = d_maps;
   d.target = d_target;
   d.w_spec = w_spec;
-  d.w_rv = w_rv;
-  d.w_fw = w_fw;
-  d.rv_t.assign(rv_t, rv_t + d.P);
-  d.fw_t.assign(fw_t, fw_t + d.P);
+  double *tg = (double *)d.tg.p;  // [rv_t | fw_t | bis_t | loss]
+  HIPCHK(c, hipMemcpyAsync(tg, rv_t, (size_t)d.P * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(tg + d.P, fw_t, (size_t)d.P * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(tg + 2 * d.P, bis_t, (size_t)d.P * 8, hipMemcpyHostToDevice, c->stream));
   DiffArgs a = diff_args(c);
   ConvArgs ca = diff_conv(c);
   {
@@ -1796,22 +1801,23 @@ int soap_diff_forward(soap_ctx *c, const soap_diff_params *prm, const double *d_
                          (double *)d.depth.p, (double *)d.bis.p, (int32_t *)d.iters.p, c->stream));
     HIPCHK(c, launch_fit_polish(c->d_vgrid, c->n_v, (const double *)d.ccf.p, (const double *)d.par.p, d.P,
                                 (double *)d.th.p, c->stream));
+    HIPCHK(c, launch_diff_bis(c->d_vgrid, c->n_v, (const double *)d.ccf.p, (const double *)d.th.p, d.P,
+                              (double *)d.bisd.p, c->stream));
+    // the loss and the fit-parameter derivatives for the backward stay on the device
+    HIPCHK(c, launch_diff_loss(d.P, d.res_blocks, (const double *)d.res_part.p, (const double *)d.th.p, tg, tg + d.P,
+                               tg + 2 * d.P, w_spec, w_rv, w_fw, w_bis, (double *)d.bisd.p, c->n_v, nullptr, 0.0,
+                               tg + 3 * d.P, (double *)d.dth.p, c->stream));
   }
-  std::vector<double> rp((size_t)d.P * d.res_blocks);
-  d.theta.resize((size_t)d.P * 4);
-  HIPCHK(c, hipMemcpyAsync(rp.data(), d.res_part.p, rp.size() * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(d.theta.data(), d.th.p, d.theta.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> th((size_t)d.P * 4), bd((size_t)d.P * (c->n_v + kBisHead));
+  HIPCHK(c, hipMemcpyAsync(loss, tg + 3 * d.P, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(th.data(), d.th.p, th.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  if (bis) HIPCHK(c, hipMemcpyAsync(bd.data(), d.bisd.p, bd.size() * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  double l = 0;
   for (int p = 0; p < d.P; p++) {
-    double r2 = 0;
-    for (int b = 0; b < d.res_blocks; b++) r2 += rp[(size_t)p * d.res_blocks + b];
-    double drv = d.theta[p * 4 + 2] - d.rv_t[p], dfw = d.theta[p * 4 + 3] - d.fw_t[p];
-    l += w_spec * r2 + w_rv * drv * drv + w_fw * dfw * dfw;
-    if (rv) rv[p] = d.theta[p * 4 + 2];
-    if (fw) fw[p] = d.theta[p * 4 + 3];
+    if (rv) rv[p] = th[p * 4 + 2];
+    if (fw) fw[p] = th[p * 4 + 3];
+    if (bis) bis[p] = bd[(size_t)p * (c->n_v + kBisHead)];
   }
-  *loss = l;
   d.have_fwd = true;
   return SOAP_OK;
 }
@@ -1821,12 +1827,6 @@ int soap_diff_backward(soap_ctx *c, double *grad4, double *d_gq, double *d_gs, d
   auto &d = c->diff;
   if (!d.have_fwd) return fail(c, SOAP_ERR_STATE, "diff_backward: run diff_forward first");
   HIPCHK(c, hipSetDevice(c->device));
-  std::vector<double> dth((size_t)d.P * 4, 0.0);
-  for (int p = 0; p < d.P; p++) {
-    dth[p * 4 + 2] = 2 * d.w_rv * (d.theta[p * 4 + 2] - d.rv_t[p]);
-    dth[p * 4 + 3] = 2 * d.w_fw * (d.theta[p * 4 + 3] - d.fw_t[p]);
-  }
-  HIPCHK(c, hipMemcpyAsync(d.dth.p, dth.data(), dth.size() * 8, hipMemcpyHostToDevice, c->stream));
   int rc0 = diff_mask_transpose(c);
   if (rc0) return rc0;
   DiffArgs a = diff_args(c);
@@ -1838,7 +1838,7 @@ int soap_diff_backward(soap_ctx *c, double *grad4, double *d_gq, double *d_gs, d
         (const double *)d.dth.p, (double *)d.gy.p, c->n - 2 * (int64_t)c->trim, (const int64_t *)d.mt_ptr.p,
         (const int32_t *)d.mt_w.p, (const int32_t *)d.mt_col.p, (const double *)d.mt_val.p, (double *)d.GF.p,
         (double *)d.GI.p, (double *)d.Cpart.p, (double *)d.C.p, d_gmaps, (double *)d.cell_part.p, d_gq, d_gs,
-        c->stream);
+        (const double *)d.bisd.p, c->stream);
     if (e == hipErrorInvalidValue) return fail(c, SOAP_ERR_LIMIT, "diff_backward: window or shift range too wide");
     HIPCHK(c, e);
   }
@@ -1853,10 +1853,16 @@ int soap_diff_backward(soap_ctx *c, double *grad4, double *d_gq, double *d_gs, d
 }
 
 int soap_diff_get(soap_ctx *c, int32_t what, double *out) {
-  if (!c || !out || what < 0 || what > 2) return fail(c, SOAP_ERR_ARG, "diff_get: bad arguments");
+  if (!c || !out || what < 0 || what > 3) return fail(c, SOAP_ERR_ARG, "diff_get: bad arguments");
   auto &d = c->diff;
   if (!d.have_fwd) return fail(c, SOAP_ERR_STATE, "diff_get: run diff_forward first");
   HIPCHK(c, hipSetDevice(c->device));
+  if (what == 3) {  // the span of each phase: the head of its bisd record
+    HIPCHK(c, hipMemcpy2DAsync(out, 8, d.bisd.p, (size_t)(c->n_v + kBisHead) * 8, 8, d.P, hipMemcpyDeviceToHost,
+                               c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return SOAP_OK;
+  }
   const void *src = what == 0 ? d.F.p : what == 1 ? d.ccf.p : d.th.p;
   size_t bytes = (size_t)d.P * (what == 0 ? c->n : what == 1 ? c->n_v : 4) * 8;
   HIPCHK(c, hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, c->stream));
@@ -2365,8 +2371,8 @@ struct soap_diffbatch {
     bool bound = false;
     double vsini_max = 0;
     const double *prm = nullptr, *q = nullptr, *s = nullptr, *maps = nullptr, *target = nullptr, *rv_t = nullptr,
-                 *fw_t = nullptr;
-    double w_spec = 0, w_rv = 0, w_fw = 0;
+                 *fw_t = nullptr, *bis_t = nullptr;
+    double w_spec = 0, w_rv = 0, w_fw = 0, w_bis = 0;
     double *loss = nullptr, *g4 = nullptr, *gq = nullptr, *gs = nullptr, *gmaps = nullptr;
     int dmin = 0, nd = 0;
     std::vector<uintptr_t> sig;  // buffers and sizes the enqueued launches use
@@ -2379,8 +2385,8 @@ struct soap_diffbatch {
   hipEvent_t fork = nullptr, join[kDiffLanes] = {};
   std::vector<int> lane_of;  // lane of each instance, balanced by estimated work
   std::vector<Item> items;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph[2] = {};  // [with_adjoint]
+  hipGraphExec_t exec[2] = {};
   std::string err;
 };
 
@@ -2393,16 +2399,18 @@ std::vector<uintptr_t> dbatch_signature(const soap_ctx *c) {
           u(d.gy.p),   u(d.GF.p),    u(d.GI.p),     u(d.cell_part.p), u(d.I.p),  u(d.F.p),        u(d.ck.p),
           u(d.cm.p),   u(d.ct.p),    u(d.cL.p),     u(d.cmu.p),   u(d.cdt.p),    u(d.mt_ptr.p),   u(d.mt_w.p),
           u(d.mt_col.p), u(d.mt_val.p), u(c->ccf_part.p), u(c->m_start.p), u(c->m_alive.p), u(c->m_val.p),
-          u(c->d_w_sorted), u(c->ctap.p), u(d.maps), u(d.q), u(d.s), u(d.target), (uintptr_t)d.nd,
+          u(c->d_w_sorted), u(c->ctap.p), u(d.bisd.p), u(d.maps), u(d.q), u(d.s), u(d.target), (uintptr_t)d.nd,
           (uintptr_t)d.dmin, (uintptr_t)d.P, (uintptr_t)d.G, (uintptr_t)d.M, (uintptr_t)c->n, (uintptr_t)c->res_ntp,
           (uintptr_t)c->n_lines, (uintptr_t)c->n_v};
 }
 
 void dbatch_drop_graph(soap_diffbatch *b) {
-  if (b->exec) hipGraphExecDestroy(b->exec);
-  if (b->graph) hipGraphDestroy(b->graph);
-  b->exec = nullptr;
-  b->graph = nullptr;
+  for (int k = 0; k < 2; k++) {
+    if (b->exec[k]) hipGraphExecDestroy(b->exec[k]);
+    if (b->graph[k]) hipGraphDestroy(b->graph[k]);
+    b->exec[k] = nullptr;
+    b->graph[k] = nullptr;
+  }
 }
 
 int dfail(soap_diffbatch *b, int code, const char *fmt, ...) {
@@ -2438,9 +2446,9 @@ void dbatch_assign_lanes(soap_diffbatch *b) {
   }
 }
 
-hipError_t dbatch_enqueue_one(soap_diffbatch::Item &it, hipStream_t st);
+hipError_t dbatch_enqueue_one(soap_diffbatch::Item &it, hipStream_t st, bool adjoint);
 
-int dbatch_enqueue(soap_diffbatch *b) {
+int dbatch_enqueue(soap_diffbatch *b, bool adjoint) {
   for (size_t i = 0; i < b->items.size(); i++)
     if (!b->items[i].bound) return dfail(b, SOAP_ERR_STATE, "diff batch: instance %zu not bound", i);
   hipError_t e = hipEventRecord(b->fork, b->stream);
@@ -2448,7 +2456,7 @@ int dbatch_enqueue(soap_diffbatch *b) {
   if (e != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff batch: fork: %s", hipGetErrorString(e));
   int status = SOAP_OK;
   for (size_t i = 0; i < b->items.size() && status == SOAP_OK; i++) {
-    e = dbatch_enqueue_one(b->items[i], b->lane[b->lane_of[i]]);
+    e = dbatch_enqueue_one(b->items[i], b->lane[b->lane_of[i]], adjoint);
     if (e != hipSuccess)
       status = dfail(b, e == hipErrorInvalidValue ? SOAP_ERR_LIMIT : SOAP_ERR_HIP, "diff batch: instance %zu: %s", i,
                      hipGetErrorString(e));
@@ -2463,8 +2471,8 @@ int dbatch_enqueue(soap_diffbatch *b) {
   return status;
 }
 
-// Forward, loss and adjoint of one bound instance on stream st.
-hipError_t dbatch_enqueue_one(soap_diffbatch::Item &it, hipStream_t st) {
+// Forward, loss and (if adjoint) the adjoint of one bound instance on stream st.
+hipError_t dbatch_enqueue_one(soap_diffbatch::Item &it, hipStream_t st, bool adjoint) {
   {
     soap_ctx *c = it.c;
     auto &d = c->diff;
@@ -2488,14 +2496,21 @@ hipError_t dbatch_enqueue_one(soap_diffbatch::Item &it, hipStream_t st) {
       e = launch_fit_polish(c->d_vgrid, c->n_v, (const double *)d.ccf.p, (const double *)d.par.p, d.P,
                             (double *)d.th.p, st);
     if (e == hipSuccess)
-      e = launch_diff_loss(d.P, rb, (const double *)d.res_part.p, (const double *)d.th.p, it.rv_t, it.fw_t, it.w_spec,
-                           it.w_rv, it.w_fw, it.prm, it.vsini_max, it.loss, (double *)d.dth.p, st);
+      e = it.w_bis != 0.0 ? launch_diff_bis(c->d_vgrid, c->n_v, (const double *)d.ccf.p, (const double *)d.th.p, d.P,
+                                            (double *)d.bisd.p, st)
+                          : hipSuccess;
+    double *bisd = it.w_bis != 0.0 ? (double *)d.bisd.p : nullptr;  // no span term: no span kernel
+    if (e == hipSuccess)
+      e = launch_diff_loss(d.P, rb, (const double *)d.res_part.p, (const double *)d.th.p, it.rv_t, it.fw_t, it.bis_t,
+                           it.w_spec, it.w_rv, it.w_fw, it.w_bis, bisd, c->n_v, it.prm, it.vsini_max, it.loss,
+                           (double *)d.dth.p, st);
+    if (!adjoint) return e;
     if (e == hipSuccess)
       e = launch_diff_backward(a, ca, it.target, c->trim, it.w_spec, c->d_vgrid, c->n_v, (const double *)d.ccf.p,
                                (const double *)d.th.p, (const double *)d.dth.p, (double *)d.gy.p, n_t,
                                (const int64_t *)d.mt_ptr.p, (const int32_t *)d.mt_w.p, (const int32_t *)d.mt_col.p,
                                (const double *)d.mt_val.p, (double *)d.GF.p, (double *)d.GI.p, (double *)d.Cpart.p,
-                               (double *)d.C.p, it.gmaps, (double *)d.cell_part.p, it.gq, it.gs, st);
+                               (double *)d.C.p, it.gmaps, (double *)d.cell_part.p, it.gq, it.gs, bisd, st);
     if (e == hipSuccess) e = launch_diff_grad4(diff_cell_blocks((int64_t)d.G * d.G), (const double *)d.cell_part.p,
                                                it.g4, st);
     return e;
@@ -2555,10 +2570,12 @@ const char *soap_diff_batch_last_error(const soap_diffbatch *b) { return b ? b->
 
 int soap_diff_batch_bind(soap_diffbatch *b, int32_t i, double vsini_max, const double *d_params, const double *d_q,
                          const double *d_s, const double *d_maps, const double *d_target, const double *d_rv_t,
-                         const double *d_fw_t, double w_spec, double w_rv, double w_fw, double *d_loss,
-                         double *d_grad4, double *d_gq, double *d_gs, double *d_gmaps) {
+                         const double *d_fw_t, const double *d_bis_t, double w_spec, double w_rv, double w_fw,
+                         double w_bis, double *d_loss, double *d_grad4, double *d_gq, double *d_gs,
+                         double *d_gmaps) {
   if (!b || i < 0 || i >= (int32_t)b->items.size()) return SOAP_ERR_ARG;
-  if (!d_params || !d_q || !d_s || !d_maps || !d_target || !d_rv_t || !d_fw_t || !d_loss || !d_grad4 || !d_gq ||
+  if (!d_params || !d_q || !d_s || !d_maps || !d_target || !d_rv_t || !d_fw_t || !d_bis_t || !d_loss || !d_grad4 ||
+      !d_gq ||
       !d_gs || !d_gmaps)
     return dfail(b, SOAP_ERR_ARG, "diff_batch_bind: null pointer");
   if (!(vsini_max > 0 && vsini_max < 2000)) return dfail(b, SOAP_ERR_ARG, "diff_batch_bind: vsini_max out of range");
@@ -2600,6 +2617,8 @@ int soap_diff_batch_bind(soap_diffbatch *b, int32_t i, double vsini_max, const d
   it.target = d_target;
   it.rv_t = d_rv_t;
   it.fw_t = d_fw_t;
+  it.bis_t = d_bis_t;
+  it.w_bis = w_bis;
   it.w_spec = w_spec;
   it.w_rv = w_rv;
   it.w_fw = w_fw;
@@ -2614,7 +2633,7 @@ int soap_diff_batch_bind(soap_diffbatch *b, int32_t i, double vsini_max, const d
   return SOAP_OK;
 }
 
-int soap_diff_batch_step(soap_diffbatch *b, int32_t use_graph) {
+int soap_diff_batch_step(soap_diffbatch *b, int32_t use_graph, int32_t with_adjoint) {
   if (!b) return SOAP_ERR_ARG;
   if (hipSetDevice(b->device) != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff_batch_step: hipSetDevice");
   for (auto &it : b->items) {
@@ -2622,11 +2641,12 @@ int soap_diff_batch_step(soap_diffbatch *b, int32_t use_graph) {
     if (it.bound && dbatch_signature(it.c) != it.sig)
       return dfail(b, SOAP_ERR_STATE, "diff_batch_step: an instance was changed after binding");
   }
-  if (!use_graph) return dbatch_enqueue(b);
-  if (!b->exec) {
+  const bool adj = with_adjoint != 0;
+  if (!use_graph) return dbatch_enqueue(b, adj);
+  if (!b->exec[adj]) {
     hipError_t e = hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff_batch_step: begin capture: %s", hipGetErrorString(e));
-    int rc = dbatch_enqueue(b);
+    int rc = dbatch_enqueue(b, adj);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(b->stream, &g);
     if (rc) {
@@ -2634,11 +2654,11 @@ int soap_diff_batch_step(soap_diffbatch *b, int32_t use_graph) {
       return rc;
     }
     if (e != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff_batch_step: end capture: %s", hipGetErrorString(e));
-    b->graph = g;
-    e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+    b->graph[adj] = g;
+    e = hipGraphInstantiate(&b->exec[adj], b->graph[adj], nullptr, nullptr, 0);
     if (e != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff_batch_step: instantiate: %s", hipGetErrorString(e));
   }
-  hipError_t e = hipGraphLaunch(b->exec, b->stream);
+  hipError_t e = hipGraphLaunch(b->exec[adj], b->stream);
   if (e != hipSuccess) return dfail(b, SOAP_ERR_HIP, "diff_batch_step: launch: %s", hipGetErrorString(e));
   return SOAP_OK;
 }

@@ -311,17 +311,26 @@ hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const doub
                                 const double *v, int n_v, const double *ccf, const double *theta, const double *dth,
                                 double *gy, int64_t n_t, const int64_t *mt_ptr, const int32_t *mt_w,
                                 const int32_t *mt_col, const double *mt_val, double *GF, double *GI, double *Cpart,
-                                double *C, double *gmaps, double *cell_part, double *gq, double *gs, hipStream_t s);
+                                double *C, double *gmaps, double *cell_part, double *gq, double *gs,
+                                const double *bisd, hipStream_t s);
+// bisector span of each CCF at the polished Gaussian and its partials (k_diff_bis):
+// bisd [n_ccf][n_v + kBisHead] = (span, loss coefficient, dspan/dtheta[4], dspan/dy[n_v])
+constexpr int kBisHead = 6;
+constexpr int kMaxVDiff = 1024;
+hipError_t launch_diff_bis(const double *v, int n_v, const double *ccf, const double *theta, int n_ccf, double *bisd,
+                           hipStream_t s);
 // chunks of the lag-correlation pass (Cpart holds [P][n_mu][chunks][nd][2])
 int diff_corr_chunks(int64_t n);
 hipError_t launch_ccf_sum(const double *part, int nc, int n_spec, int n_v, double *ccf, hipStream_t s);
 // device-side loss of the differentiable model: per phase the residual partials
-// (res_blocks per phase, in order) and the RV / FWHM terms; writes loss[0] and
-// the fit-parameter derivatives dth [P][4]; loss = NaN when prm's vsini is
-// outside (0, vsini_max] (the fixed tap range of a graph)
+// (res_blocks per phase, in order), the RV / FWHM terms and, when w_bis != 0, the
+// bisector-span term from bisd; writes loss[0], the fit-parameter derivatives
+// dth [P][4] and the span coefficients of bisd; loss = NaN when prm (optional) has
+// vsini outside (0, vsini_max] (the fixed tap range of a graph)
 hipError_t launch_diff_loss(int n_phase, int res_blocks, const double *res_part, const double *theta,
-                            const double *rv_t, const double *fw_t, double w_spec, double w_rv, double w_fw,
-                            const double *prm, double vsini_max, double *loss, double *dth, hipStream_t s);
+                            const double *rv_t, const double *fw_t, const double *bis_t, double w_spec, double w_rv,
+                            double w_fw, double w_bis, double *bisd, int n_v, const double *prm, double vsini_max,
+                            double *loss, double *dth, hipStream_t s);
 // g4[q] = sum over blocks (in order) of cell_part[b][q], q < 4
 hipError_t launch_diff_grad4(int n_blocks, const double *cell_part, double *g4, hipStream_t s);
 // batched instances: stage st over jobs[n_inst] (device), prefix[n_inst + 1]

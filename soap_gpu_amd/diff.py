@@ -7,9 +7,10 @@ quadratic limb darkening; per-phase filling-factor maps f_p (M x M over
 [-1, 1]^2, bilinear) blending the quiet seed rows Q and the spot rows S scaled by
 the contrast; F_p = instrumental convolution of the phase spectrum; RV and FWHM
 from the Gaussian fit of the raw mask CCF of F_p, polished to the least-squares
-optimum.  Loss = w_spec sum (F - T)^2 (trimmed pixels) + w_rv sum (RV - RV_t)^2
-+ w_fwhm sum (FWHM - FWHM_t)^2.  Gradients: vsini, u1, u2, contrast, the seed
-rows Q and S, and every map pixel.
+optimum; BIS = the bisector span of that CCF at the polished Gaussian
+(BIS_FIT2.cpp:214-268).  Loss = w_spec sum (F - T)^2 (trimmed pixels)
++ w_rv sum (RV - RV_t)^2 + w_fwhm sum (FWHM - FWHM_t)^2 + w_bis sum (BIS - BIS_t)^2.
+Gradients: vsini, u1, u2, contrast, the seed rows Q and S, and every map pixel.
 """
 from __future__ import annotations
 
@@ -48,23 +49,27 @@ class DiffModel:
         self._fwd_id = 0
         self._keep = None
 
-    def forward(self, params, q, s, maps, target, rv_t, fw_t, w_spec=1.0, w_rv=0.0, w_fw=0.0):
+    def forward(self, params, q, s, maps, target, rv_t, fw_t, w_spec=1.0, w_rv=0.0, w_fw=0.0, bis_t=None,
+                w_bis=0.0):
         """params = (vsini [km/s], u1, u2, contrast); q, s [n_mu, n], maps [P, M, M], target [P, n]
-        float64 GPU tensors.  Returns (loss, rv[P], fwhm[P])."""
+        float64 GPU tensors.  Returns (loss, rv[P], fwhm[P], bis[P])."""
         vs, u1, u2, kap = (float(x) for x in params)
         torch.cuda.synchronize()
         prm = DiffParams(vs, u1, u2, kap)
         rt, rtp = _d(np.asarray(rv_t, dtype=np.float64).reshape(self.P))
         ft, ftp = _d(np.asarray(fw_t, dtype=np.float64).reshape(self.P))
+        bt, btp = _d(np.zeros(self.P) if bis_t is None else np.asarray(bis_t, dtype=np.float64).reshape(self.P))
         loss = ctypes.c_double()
         rv, rvp = _d(np.zeros(self.P))
         fw, fwp = _d(np.zeros(self.P))
+        bs, bsp = _d(np.zeros(self.P))
         self._keep = (q, s, maps, target)  # the engine reads them again in backward
         self.eng._chk(self.eng.L.soap_diff_forward(self.eng.ctx, ctypes.byref(prm), _dev(q), _dev(s), _dev(maps),
-                                                   _dev(target), rtp, ftp, float(w_spec), float(w_rv), float(w_fw),
-                                                   ctypes.byref(loss), rvp, fwp), "diff_forward")
+                                                   _dev(target), rtp, ftp, btp, float(w_spec), float(w_rv),
+                                                   float(w_fw), float(w_bis), ctypes.byref(loss), rvp, fwp, bsp),
+                      "diff_forward")
         self._fwd_id += 1
-        return loss.value, rv, fw
+        return loss.value, rv, fw, bs
 
     def backward(self):
         """Gradients of the last forward's loss: (g4 = d/d(vsini, u1, u2, contrast), gQ, gS, gmaps)."""
@@ -79,27 +84,30 @@ class DiffModel:
         return g4, gq, gs, gm
 
     def get(self, what: str):
-        """Host copies of the last forward: 'F' [P, n], 'ccf' [P, n_v], 'theta' [P, 4] = (c, k, RV, FWHM)."""
-        shape = {"F": (self.P, self.n), "ccf": (self.P, self.n_v), "theta": (self.P, 4)}[what]
+        """Host copies of the last forward: 'F' [P, n], 'ccf' [P, n_v], 'theta' [P, 4] = (c, k, RV, FWHM),
+        'bis' [P] (bisector spans)."""
+        shape = {"F": (self.P, self.n), "ccf": (self.P, self.n_v), "theta": (self.P, 4), "bis": (self.P,)}[what]
         out = np.empty(shape)
-        self.eng._chk(self.eng.L.soap_diff_get(self.eng.ctx, ("F", "ccf", "theta").index(what),
+        self.eng._chk(self.eng.L.soap_diff_get(self.eng.ctx, ("F", "ccf", "theta", "bis").index(what),
                                                out.ctypes.data_as(_dp)), "diff_get")
         return out
 
 
 class SoapLoss(torch.autograd.Function):
-    """loss = SoapLoss.apply(model, params[4], q, s, maps, target, rv_t, fw_t, (w_spec, w_rv, w_fw)).
-    params is a float64 tensor (vsini, u1, u2, contrast); gradients flow to params,
-    q, s and maps.  One backward per forward (the engine keeps the last forward)."""
+    """loss = SoapLoss.apply(model, params[4], q, s, maps, target, rv_t, fw_t, weights, bis_t=None) with
+    weights = (w_spec, w_rv, w_fw) or (w_spec, w_rv, w_fw, w_bis).  params is a float64
+    tensor (vsini, u1, u2, contrast); gradients flow to params, q, s and maps.  One
+    backward per forward (the engine keeps the last forward)."""
 
     @staticmethod
-    def forward(ctx, model, params, q, s, maps, target, rv_t, fw_t, weights):
-        loss, rv, fw = model.forward(params.detach().cpu().numpy(), q.detach(), s.detach(), maps.detach(),
-                                     target.detach(), rv_t, fw_t, *weights)
+    def forward(ctx, model, params, q, s, maps, target, rv_t, fw_t, weights, bis_t=None):
+        w = tuple(weights) + (0.0,) * (4 - len(weights))
+        loss, rv, fw, bs = model.forward(params.detach().cpu().numpy(), q.detach(), s.detach(), maps.detach(),
+                                         target.detach(), rv_t, fw_t, w[0], w[1], w[2], bis_t, w[3])
         ctx.model = model
         ctx.fwd_id = model._fwd_id
         ctx.params_device = params.device
-        ctx.rv, ctx.fw = rv, fw
+        ctx.rv, ctx.fw, ctx.bis = rv, fw, bs
         return torch.tensor(loss, dtype=torch.float64, device=params.device)
 
     @staticmethod
@@ -109,7 +117,7 @@ class SoapLoss(torch.autograd.Function):
         g4, gq, gs, gm = ctx.model.backward()
         gp = torch.as_tensor(g4, dtype=torch.float64, device=ctx.params_device) * gout.to(ctx.params_device)
         go = gout.to(gq.device)
-        return None, gp, gq * go, gs * go, gm * go, None, None, None, None
+        return None, gp, gq * go, gs * go, gm * go, None, None, None, None, None
 
 
 class DiffBatch:
@@ -136,17 +144,19 @@ class DiffBatch:
             msg = self.L.soap_diff_batch_last_error(self.b)
             raise SoapError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
 
-    def bind(self, i, vsini_max, params, q, s, maps, target, rv_t, fw_t, weights, loss, g4, gq, gs, gm):
+    def bind(self, i, vsini_max, params, q, s, maps, target, rv_t, fw_t, bis_t, weights, loss, g4, gq, gs, gm):
         """All tensors float64, contiguous, on the GPU: params [4], q / s [n_mu][n], maps [P][M][M],
-        target [P][n], rv_t / fw_t [P]; outputs loss [1], g4 [4], gq / gs like q, gm like maps."""
-        ts = (params, q, s, maps, target, rv_t, fw_t, loss, g4, gq, gs, gm)
-        w_spec, w_rv, w_fw = (float(x) for x in weights)
-        self._chk(self.L.soap_diff_batch_bind(self.b, int(i), float(vsini_max), *[_dev(t) for t in ts[:7]], w_spec,
-                                              w_rv, w_fw, *[_dev(t) for t in ts[7:]]), "diff_batch_bind")
+        target [P][n], rv_t / fw_t / bis_t [P]; weights (w_spec, w_rv, w_fw, w_bis); outputs
+        loss [1], g4 [4], gq / gs like q, gm like maps."""
+        ts = (params, q, s, maps, target, rv_t, fw_t, bis_t, loss, g4, gq, gs, gm)
+        w_spec, w_rv, w_fw, w_bis = (float(x) for x in tuple(weights) + (0.0,) * (4 - len(weights)))
+        self._chk(self.L.soap_diff_batch_bind(self.b, int(i), float(vsini_max), *[_dev(t) for t in ts[:8]], w_spec,
+                                              w_rv, w_fw, w_bis, *[_dev(t) for t in ts[8:]]), "diff_batch_bind")
         self._keep[i] = ts
 
-    def step(self, graph: bool = True):
-        self._chk(self.L.soap_diff_batch_step(self.b, int(bool(graph))), "diff_batch_step")
+    def step(self, graph: bool = True, adjoint: bool = True):
+        """Forward + loss (+ adjoint when adjoint) of every bound instance."""
+        self._chk(self.L.soap_diff_batch_step(self.b, int(bool(graph)), int(bool(adjoint))), "diff_batch_step")
 
     def sync(self):
         self._chk(self.L.soap_diff_batch_sync(self.b), "diff_batch_sync")

@@ -122,8 +122,9 @@ def load(path: str = LIB_PATH):
         "soap_get_stage_work": (ctypes.c_int, [vp, ctypes.c_int32, _dp, _dp]),
         "soap_set_fir_shape": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32]),
         "soap_diff_setup": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]),
-        "soap_diff_forward": (ctypes.c_int, [vp, ctypes.POINTER(DiffParams), vp, vp, vp, vp, _dp, _dp,
-                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]),
+        "soap_diff_forward": (ctypes.c_int, [vp, ctypes.POINTER(DiffParams), vp, vp, vp, vp, _dp, _dp, _dp,
+                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             _dp, _dp, _dp, _dp]),
         "soap_diff_backward": (ctypes.c_int, [vp, _dp, vp, vp, vp]),
         "soap_diff_get": (ctypes.c_int, [vp, ctypes.c_int32, _dp]),
         "soap_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int32, _dp, _dp, ctypes.POINTER(vp)]),
@@ -143,9 +144,9 @@ def load(path: str = LIB_PATH):
         "soap_diff_batch_destroy": (None, [vp]),
         "soap_diff_batch_last_error": (ctypes.c_char_p, [vp]),
         "soap_diff_batch_bind": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_double, vp, vp, vp, vp, vp, vp, vp,
-                                                ctypes.c_double, ctypes.c_double, ctypes.c_double, vp, vp, vp, vp,
-                                                vp]),
-        "soap_diff_batch_step": (ctypes.c_int, [vp, ctypes.c_int32]),
+                                                vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_double, vp, vp, vp, vp, vp]),
+        "soap_diff_batch_step": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32]),
         "soap_diff_batch_sync": (ctypes.c_int, [vp]),
     }
     for name, (res, args) in sig.items():

@@ -1,7 +1,7 @@
 """Fitting loop over many differentiable instances: L-BFGS on the contrast, vsini
-and the filling-factor maps of each instance, against target RV and FWHM series
-(and optionally target spectra), with the FP64 adjoint of soap_diff_* for the
-gradients.
+and the filling-factor maps of each instance, against target RV, FWHM and BIS
+(bisector span) series (and optionally target spectra), with the FP64 adjoint of
+soap_diff_* for the gradients.
 
 No counterpart in the reference.  Each instance is a DiffModel on its own
 Engine (its own context and stream); the objective is the sum of the
@@ -25,9 +25,10 @@ from .engine import Engine, select_mask_lines
 class Instance:
     """One fitting problem: a DiffModel with its seeds, targets and starting point."""
 
-    def __init__(self, w, maps0, n_phase: int, map_res: int, target_spec=None, rv_t=None, fw_t=None,
-                 weights=(0.0, 1e-3, 1e-5), u=(0.29, 0.34), vsini0: float = 2.0, contrast0: float = 0.3,
+    def __init__(self, w, maps0, n_phase: int, map_res: int, target_spec=None, rv_t=None, fw_t=None, bis_t=None,
+                 weights=(0.0, 1e-3, 1e-5, 1e-3), u=(0.29, 0.34), vsini0: float = 2.0, contrast0: float = 0.3,
                  trim: int = 210, device: int = 0):
+        """weights = (w_spec, w_rv, w_fwhm[, w_bis])."""
         self.eng = Engine(device)
         sl, sw = select_mask_lines(w.wave, w.mask_l, w.mask_w, trim)
         self.model = DiffModel(self.eng, w.wave, w.mu_bounds, sl, sw, w.vgrid, w.star["grid"], n_phase, map_res,
@@ -40,7 +41,8 @@ class Instance:
                        else torch.as_tensor(target_spec, dtype=torch.float64, device=dev))
         self.rv_t = np.zeros(n_phase) if rv_t is None else np.asarray(rv_t, dtype=np.float64)
         self.fw_t = np.zeros(n_phase) if fw_t is None else np.asarray(fw_t, dtype=np.float64)
-        self.weights = tuple(float(x) for x in weights)
+        self.bis_t = np.zeros(n_phase) if bis_t is None else np.asarray(bis_t, dtype=np.float64)
+        self.weights = tuple(float(x) for x in weights) + (0.0,) * (4 - len(weights))
         self.u = tuple(float(x) for x in u)
         m = np.clip(np.asarray(maps0, dtype=np.float64), 1e-6, 1 - 1e-6)
         # fitted: (vsini, contrast) and the map logits
@@ -56,14 +58,14 @@ class Instance:
 
     def loss(self):
         return SoapLoss.apply(self.model, self.params(), self.q, self.s, self.maps(), self.target, self.rv_t,
-                              self.fw_t, self.weights)
+                              self.fw_t, self.weights, self.bis_t)
 
     def observe(self, params, maps):
-        """RV and FWHM series of the model at given parameters (targets of a synthetic problem)."""
-        _, rv, fw = self.model.forward(params, self.q, self.s, torch.as_tensor(maps, dtype=torch.float64,
-                                                                              device=self.q.device),
-                                       self.target, self.rv_t, self.fw_t, 0.0, 0.0, 0.0)
-        return rv, fw
+        """RV, FWHM and BIS series of the model at given parameters (targets of a synthetic problem)."""
+        _, rv, fw, bis = self.model.forward(params, self.q, self.s,
+                                            torch.as_tensor(maps, dtype=torch.float64, device=self.q.device),
+                                            self.target, self.rv_t, self.fw_t, 0.0, 0.0, 0.0)
+        return rv, fw, bis
 
     def close(self):
         self.eng.close()
@@ -117,12 +119,13 @@ def fit_graph(instances, iters: int = 50, history: int = 10, lr: float = 1.0, vs
                  maps=torch.empty_like(inst.z, memory_format=torch.contiguous_format).detach(),
                  rv_t=torch.as_tensor(inst.rv_t, dtype=torch.float64, device=dev).contiguous(),
                  fw_t=torch.as_tensor(inst.fw_t, dtype=torch.float64, device=dev).contiguous(),
+                 bis_t=torch.as_tensor(inst.bis_t, dtype=torch.float64, device=dev).contiguous(),
                  loss=torch.empty(1, dtype=torch.float64, device=dev),
                  g4=torch.empty(4, dtype=torch.float64, device=dev),
                  gq=torch.empty_like(inst.q), gs=torch.empty_like(inst.s),
                  gm=torch.empty_like(inst.z).detach())
-        batch.bind(i, vmax, b["prm"], inst.q, inst.s, b["maps"], inst.target, b["rv_t"], b["fw_t"], inst.weights,
-                   b["loss"], b["g4"], b["gq"], b["gs"], b["gm"])
+        batch.bind(i, vmax, b["prm"], inst.q, inst.s, b["maps"], inst.target, b["rv_t"], b["fw_t"], b["bis_t"],
+                   inst.weights, b["loss"], b["g4"], b["gq"], b["gs"], b["gm"])
         bufs.append(b)
     params = [t for inst in instances for t in (inst.theta, inst.z)]
     opt = torch.optim.LBFGS(params, lr=lr, max_iter=iters, history_size=history, line_search_fn="strong_wolfe",
@@ -163,12 +166,18 @@ def fit_graph(instances, iters: int = 50, history: int = 10, lr: float = 1.0, vs
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     st = opt.state[opt._params[0]]
-    # the device part of one evaluation alone: graph replays at the final parameters
-    t1 = time.perf_counter()
-    for _ in range(3):
-        batch.step(graph=True)
-    batch.sync()
-    graph_ms = (time.perf_counter() - t1) / 3 * 1e3
+    # the device part of one evaluation alone: graph replays at the final parameters,
+    # forward + adjoint and the forward (with the loss) alone
+    times = {}
+    for adj in (True, False):
+        batch.step(graph=True, adjoint=adj)
+        batch.sync()
+        t1 = time.perf_counter()
+        for _ in range(3):
+            batch.step(graph=True, adjoint=adj)
+        batch.sync()
+        times[adj] = (time.perf_counter() - t1) / 3 * 1e3
     batch.close()
     return dict(loss0=loss0, loss=hist[-1], best=min(h for h in hist if h == h), iters=int(st.get("n_iter", 0)),
-                evals=evals[0], seconds=dt, ms_per_eval=dt / evals[0] * 1e3, graph_ms=graph_ms, history=hist)
+                evals=evals[0], seconds=dt, ms_per_eval=dt / evals[0] * 1e3, graph_ms=times[True],
+                graph_fwd_ms=times[False], history=hist)

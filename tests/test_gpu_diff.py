@@ -1,7 +1,9 @@
 """The differentiable path on the GPU (soap_diff_*, soap_gpu_amd.diff) against the
-FP64 oracle's model (orc_diff_forward): forward parity, and every gradient
-(vsini, u1, u2, contrast, quiet / spot seed rows, filling-factor map pixels)
-against central finite differences of the oracle's loss with h = 1e-6 relative."""
+FP64 oracle's model (orc_diff_forward): forward parity (spectra, CCF, RV, FWHM and
+the bisector span), and every gradient (vsini, u1, u2, contrast, quiet / spot seed
+rows, filling-factor map pixels) against central finite differences of the
+oracle's loss with h = 1e-6 relative, for the full loss and for the bisector-span
+term alone."""
 import numpy as np
 import pytest
 import torch
@@ -33,11 +35,12 @@ def problem(engine):
     T = tgt["F"] + 1e-4 * rng.standard_normal((P, w.n)) * np.abs(tgt["F"]).max()
     rv_t = tgt["theta"][:, 2] + rng.uniform(-20, 20, P)
     fw_t = tgt["theta"][:, 3] + rng.uniform(-50, 50, P)
-    wts = (1.0 / np.sum(T[:, TRIM:w.n - TRIM] ** 2), 1e-3, 1e-5)
+    bis_t = tgt["span"] + rng.uniform(-20, 20, P)
+    wts = (1.0 / np.sum(T[:, TRIM:w.n - TRIM] ** 2), 1e-3, 1e-5, 1e-3)
 
-    def oracle_loss(prm, qq=q, ss=s, mm=maps):
+    def oracle_loss(prm, qq=q, ss=s, mm=maps, weights=wts):
         return O.diff_forward(w.wave, 24, w.mu_bounds, *prm, qq, ss, mm, w.sigma_inst, TRIM, w.vgrid, sl, sw, T,
-                              rv_t, fw_t, *wts)
+                              rv_t, fw_t, *weights[:3], bis_t=bis_t, w_bis=weights[3])
 
     model = DiffModel(engine, w.wave, w.mu_bounds, sl, sw, w.vgrid, 24, P, M, w.sigma_inst, TRIM)
     dev = torch.device("cuda")
@@ -45,19 +48,30 @@ def problem(engine):
     ts = torch.tensor(s, dtype=torch.float64, device=dev)
     tm = torch.tensor(maps, dtype=torch.float64, device=dev)
     tT = torch.tensor(T, dtype=torch.float64, device=dev)
-    return dict(w=w, maps=maps, q=q, s=s, params=params, T=T, rv_t=rv_t, fw_t=fw_t, wts=wts, oracle=oracle_loss,
-                model=model, tq=tq, ts=ts, tm=tm, tT=tT)
+    return dict(w=w, maps=maps, q=q, s=s, params=params, T=T, rv_t=rv_t, fw_t=fw_t, bis_t=bis_t, wts=wts,
+                oracle=oracle_loss, model=model, tq=tq, ts=ts, tm=tm, tT=tT)
 
 
-def _gpu(pb, params=None):
+# the full loss, and the bisector-span term alone (its adjoint is otherwise a small
+# part of the total)
+WEIGHTS = {"all": None, "bis": (0.0, 0.0, 0.0, 1.0)}
+# relative finite-difference steps: 1e-6 for the full loss; 1e-5 for the span term
+# alone, whose value moves with the rounding of the polished fit optimum (about
+# 1e-11 relative in the centre), enough to shift a 1e-6-step central difference
+# of this term by ~3e-5 relative (steps from 1e-4 to 1e-5 agree to 5e-6)
+HREL = {"all": 1e-6, "bis": 1e-5}
+
+
+def _gpu(pb, params=None, weights=None):
     m = pb["model"]
+    w = pb["wts"] if weights is None else weights
     return m.forward(pb["params"] if params is None else params, pb["tq"], pb["ts"], pb["tm"], pb["tT"],
-                     pb["rv_t"], pb["fw_t"], *pb["wts"])
+                     pb["rv_t"], pb["fw_t"], *w[:3], bis_t=pb["bis_t"], w_bis=w[3])
 
 
 def test_diff_forward_parity(problem):
     pb = problem
-    loss, rv, fw = _gpu(pb)
+    loss, rv, fw, bis = _gpu(pb)
     ref = pb["oracle"](pb["params"])
     assert loss == pytest.approx(ref["loss"], rel=1e-10)
     F = pb["model"].get("F")
@@ -65,6 +79,8 @@ def test_diff_forward_parity(problem):
         assert normwise(F[p], ref["F"][p]) <= 1e-12
     np.testing.assert_allclose(rv, ref["theta"][:, 2], rtol=0, atol=1e-6)
     np.testing.assert_allclose(fw, ref["theta"][:, 3], rtol=0, atol=1e-6)
+    np.testing.assert_allclose(bis, ref["span"], rtol=0, atol=1e-6)
+    np.testing.assert_array_equal(pb["model"].get("bis"), bis)
     np.testing.assert_allclose(pb["model"].get("ccf"), ref["ccf"], rtol=1e-11)
 
 
@@ -72,35 +88,39 @@ def _fd(f, h):
     return (f(h) - f(-h)) / (2 * h)
 
 
-def test_diff_gradients_scalars_vs_oracle_fd(problem):
+@pytest.mark.parametrize("terms", sorted(WEIGHTS))
+def test_diff_gradients_scalars_vs_oracle_fd(problem, terms):
     pb = problem
-    _gpu(pb)
+    wts = WEIGHTS[terms] or pb["wts"]
+    _gpu(pb, weights=wts)
     g4, gq, gs, gm = pb["model"].backward()
     p0 = pb["params"]
     for i, name in enumerate(("vsini", "u1", "u2", "contrast")):
-        h = 1e-6 * abs(p0[i])
-        fd = _fd(lambda e: pb["oracle"](p0 + e * np.eye(4)[i])["loss"], h)
+        h = HREL[terms] * abs(p0[i])
+        fd = _fd(lambda e: pb["oracle"](p0 + e * np.eye(4)[i], weights=wts)["loss"], h)
         assert g4[i] == pytest.approx(fd, rel=2e-5, abs=1e-9 * np.abs(g4).max()), name
 
 
+@pytest.mark.parametrize("terms", sorted(WEIGHTS))
 @pytest.mark.parametrize("which", ["q", "s", "maps"])
-def test_diff_gradients_arrays_vs_oracle_fd(problem, which):
+def test_diff_gradients_arrays_vs_oracle_fd(problem, which, terms):
     """Directional derivatives along two random directions of the seed rows and of
     the maps (interior pixels, where the trimmed loss sees them)."""
     pb = problem
-    _gpu(pb)
+    wts = WEIGHTS[terms] or pb["wts"]
+    _gpu(pb, weights=wts)
     g4, gq, gs, gm = pb["model"].backward()
     g = {"q": gq, "s": gs, "maps": gm}[which].cpu().numpy()
     base = {"q": pb["q"], "s": pb["s"], "maps": pb["maps"]}[which]
     rng = np.random.default_rng(5)
     for _ in range(2):
         v = rng.standard_normal(base.shape)
-        h = 1e-6 * np.linalg.norm(base) / np.linalg.norm(v)
+        h = HREL[terms] * np.linalg.norm(base) / np.linalg.norm(v)
 
         def f(e):
             arr = base + e * v
             kw = {"q": dict(qq=arr), "s": dict(ss=arr), "maps": dict(mm=arr)}[which]
-            return pb["oracle"](pb["params"], **kw)["loss"]
+            return pb["oracle"](pb["params"], weights=wts, **kw)["loss"]
 
         fd = _fd(f, h)
         assert np.sum(g * v) == pytest.approx(fd, rel=2e-5, abs=1e-12), which
@@ -112,7 +132,7 @@ def test_soap_loss_autograd(problem):
     q = pb["tq"].clone().requires_grad_(True)
     s = pb["ts"].clone().requires_grad_(True)
     mp = pb["tm"].clone().requires_grad_(True)
-    loss = SoapLoss.apply(pb["model"], params, q, s, mp, pb["tT"], pb["rv_t"], pb["fw_t"], pb["wts"])
+    loss = SoapLoss.apply(pb["model"], params, q, s, mp, pb["tT"], pb["rv_t"], pb["fw_t"], pb["wts"], pb["bis_t"])
     (2.0 * loss).backward()
     ref = pb["oracle"](pb["params"])
     assert loss.item() == pytest.approx(ref["loss"], rel=1e-10)

@@ -1,6 +1,7 @@
 """The L-BFGS fitting loop over differentiable instances (soap_gpu_amd.fitting):
-on uneven tiny instances whose targets come from the model at known parameters,
-the summed loss falls by orders of magnitude and vsini approaches its true value.
+on uneven tiny instances whose RV, FWHM and BIS targets come from the model at
+known parameters, the summed loss falls by orders of magnitude and vsini
+approaches its true value.
 The gradients it uses are checked against finite differences of the oracle in
 test_gpu_diff."""
 import numpy as np
@@ -19,9 +20,9 @@ def _instances():
         w = synth.batch_instance(rng, n, grid, P, nreg, vs, 90.0, n_mu=6)
         maps = synth.smooth_blob_maps(rng, P, 16, n_blobs=nreg)
         kappa = 0.35
-        inst = Instance(w, maps, P, 16, vsini0=vs * 1.15, contrast0=kappa, weights=(0.0, 1e-2, 1e-4))
-        rv, fw = inst.observe((vs, 0.29, 0.34, kappa), maps)
-        inst.rv_t, inst.fw_t = rv, fw
+        inst = Instance(w, maps, P, 16, vsini0=vs * 1.15, contrast0=kappa, weights=(0.0, 1e-2, 1e-4, 1e-3))
+        rv, fw, bis = inst.observe((vs, 0.29, 0.34, kappa), maps)
+        inst.rv_t, inst.fw_t, inst.bis_t = rv, fw, bis
         out.append((inst, vs))
     return out
 
@@ -58,7 +59,7 @@ def test_graph_evaluation_matches_per_call():
         w = synth.batch_instance(rng, n, grid, P, nreg, vs, 90.0, n_mu=5)
         maps = synth.smooth_blob_maps(rng, P, 12, n_blobs=nreg)
         made.append((w, maps, vs, P))
-    weights = (1e-3, 1e-2, 1e-4)
+    weights = (1e-3, 1e-2, 1e-4, 1e-2)
     graph_insts = [Instance(w, maps, P, 12, weights=weights) for w, maps, vs, P in made]
     call_insts = [Instance(w, maps, P, 12, weights=weights) for w, maps, vs, P in made]
     try:
@@ -68,8 +69,9 @@ def test_graph_evaluation_matches_per_call():
         for i, (g, (w, maps, vs, P)) in enumerate(zip(graph_insts, made)):
             rv_t = rng.uniform(-50, 50, P)
             fw_t = rng.uniform(7000, 9000, P)
-            g.rv_t, g.fw_t = rv_t, fw_t
-            call_insts[i].rv_t, call_insts[i].fw_t = rv_t, fw_t
+            bis_t = rng.uniform(-30, 30, P)
+            g.rv_t, g.fw_t, g.bis_t = rv_t, fw_t, bis_t
+            call_insts[i].rv_t, call_insts[i].fw_t, call_insts[i].bis_t = rv_t, fw_t, bis_t
             tgt = torch.tensor(rng.uniform(0.9, 1.0, (P, w.n)), dtype=torch.float64, device=dev)
             g.target = tgt
             call_insts[i].target = tgt.clone()
@@ -77,12 +79,13 @@ def test_graph_evaluation_matches_per_call():
                      maps=torch.empty((P, 12, 12), dtype=torch.float64, device=dev),
                      rv=torch.tensor(rv_t, dtype=torch.float64, device=dev),
                      fw=torch.tensor(fw_t, dtype=torch.float64, device=dev),
+                     bis=torch.tensor(bis_t, dtype=torch.float64, device=dev),
                      loss=torch.empty(1, dtype=torch.float64, device=dev),
                      g4=torch.empty(4, dtype=torch.float64, device=dev),
                      gq=torch.empty_like(g.q), gs=torch.empty_like(g.s),
                      gm=torch.empty((P, 12, 12), dtype=torch.float64, device=dev))
-            batch.bind(i, 15.0, b["prm"], g.q, g.s, b["maps"], g.target, b["rv"], b["fw"], weights, b["loss"],
-                       b["g4"], b["gq"], b["gs"], b["gm"])
+            batch.bind(i, 15.0, b["prm"], g.q, g.s, b["maps"], g.target, b["rv"], b["fw"], b["bis"], weights,
+                       b["loss"], b["g4"], b["gq"], b["gs"], b["gm"])
             bufs.append(b)
         for rep, scale in enumerate((1.0, 1.3)):
             for (w, maps, vs, P), b in zip(made, bufs):
@@ -95,12 +98,30 @@ def test_graph_evaluation_matches_per_call():
             for (w, maps, vs, P), b, c in zip(made, bufs, call_insts):
                 prm = (vs * scale, 0.29, 0.34, 0.3 + 0.1 * rep)
                 mp = torch.tensor(np.clip(maps * scale, 0, 1), device=dev)
-                loss, _, _ = c.model.forward(prm, c.q, c.s, mp, c.target, c.rv_t, c.fw_t, *weights)
+                loss, _, _, _ = c.model.forward(prm, c.q, c.s, mp, c.target, c.rv_t, c.fw_t, *weights[:3],
+                                                bis_t=c.bis_t, w_bis=weights[3])
                 g4, gq, gs, gm = c.model.backward()
-                assert float(b["loss"][0]) == pytest.approx(loss, rel=1e-11), rep
+                # the same kernels in the same order on the same inputs
+                assert float(b["loss"][0]) == loss, rep
                 np.testing.assert_allclose(b["g4"].cpu().numpy(), g4, rtol=1e-9, atol=1e-12 * np.abs(g4).max())
                 for got, ref in ((b["gq"], gq), (b["gs"], gs), (b["gm"], gm)):
                     torch.testing.assert_close(got, ref, rtol=1e-9, atol=1e-12 * float(ref.abs().max()))
+        # the forward-only graph gives the same losses and leaves the gradients alone
+        losses = [float(b["loss"][0]) for b in bufs]
+        g4s = [b["g4"].clone() for b in bufs]
+        for b in bufs:
+            b["loss"].fill_(0.0)
+            b["g4"].fill_(7.0)
+        torch.cuda.synchronize()
+        batch.step(graph=True, adjoint=False)
+        batch.sync()
+        batch.step(graph=False, adjoint=False)
+        batch.sync()
+        for b, l0 in zip(bufs, losses):
+            assert float(b["loss"][0]) == l0
+            assert bool(torch.all(b["g4"] == 7.0))
+        for b, g in zip(bufs, g4s):
+            b["g4"].copy_(g)
         # a bound vsini beyond the graph's tap range gives NaN, not a silently clipped model
         bufs[0]["prm"][0] = 20.0
         torch.cuda.synchronize()

@@ -1,7 +1,9 @@
 """Pins of the oracle's differentiable model (oracle/soap_oracle.c, a11 section;
 no counterpart in the reference): the Gaussian-fit Hessian against finite
 differences, the Newton-polished fit against scipy's least squares, and the
-forward model against the quiet-star path of the reference's pipeline."""
+forward model against the quiet-star path of the reference's pipeline; the
+bisector (shared by the fit and the model's BIS term) against the closed forms of
+an exact Gaussian."""
 import numpy as np
 import pytest
 from scipy.optimize import least_squares
@@ -55,6 +57,42 @@ def test_gauss_polish_reaches_least_squares_optimum(seed):
     assert np.all(np.abs(g) <= 1e-10 * np.sqrt(np.diag(H)) * np.abs(th).max())
     ref = least_squares(lambda t: _gauss(x, t) - y, th0, xtol=1e-15, ftol=1e-15, gtol=1e-15, method="lm").x
     np.testing.assert_allclose(th, ref, rtol=1e-9)
+
+
+@pytest.mark.parametrize("v0,sig", [(137.0, 4000.0), (-2310.0, 3000.0), (71.0, 6000.0)])
+def test_bisector_of_a_gaussian(v0, sig):
+    """orc_bisector (BIS_FIT2.cpp:214-268) on an exact Gaussian CCF, centre off the
+    velocity grid: the bisector is the centre at every depth to the accuracy of the
+    per-interval quadratic inverse (< 3 m/s on a 500 m/s grid; the linear inverse,
+    p2 = 0, is off by 6-8 m/s and a sign error in p2 by 12-16 m/s), the span
+    vanishes, and translating the grid and the centre translates the bisector."""
+    v = np.arange(-20000.0, 20001.0, 500.0)
+    fw = sig * 2 * np.sqrt(2 * np.log(2))
+    th = (1.0, -0.6, v0, fw)
+    y = _gauss(v, th)
+    span, depth, bis = O.bisector(v, y, th)
+    assert np.abs(bis - v0).max() < 3.0
+    assert abs(span) < 0.1
+    np.testing.assert_allclose(depth, 1 - 0.6 * (np.arange(91) + 5) / 100, rtol=0, atol=1e-15)
+    sh = 1234.5
+    span2, _, bis2 = O.bisector(v + sh, y, (1.0, -0.6, v0 + sh, fw))
+    np.testing.assert_allclose(bis2 - sh, bis, rtol=0, atol=1e-8)
+    assert span2 == pytest.approx(span, abs=1e-8)
+    # an affine change of the CCF (continuum and amplitude) leaves the bisector unchanged
+    span3, _, bis3 = O.bisector(v, 3.0 + 2.5 * (y - 1.0), (3.0, -1.5, v0, fw))
+    np.testing.assert_allclose(bis3, bis, rtol=0, atol=1e-8)
+
+
+def test_bisector_matches_fit_span():
+    """The span BIS_FIT2 reports (orc_fit_ccf par[2]) is orc_bisector at the fitted
+    Gaussian (c, signed k, RV, FWHM)."""
+    x, y = _ccf_like(3)
+    r = O.fit_ccf(x, y)
+    c, k_abs, span, rv, fw = r["par"]
+    k = -k_abs if y.min() < c else k_abs
+    s2, d2, b2 = O.bisector(x, y, (c, k, rv, fw))
+    assert s2 == span
+    np.testing.assert_array_equal(b2, r["bis"])
 
 
 def _problem(grid=20, n=2400, n_mu=4, P=2, M=10, seed=5):
