@@ -81,7 +81,12 @@ __device__ __forceinline__ void cell_yz(long idx, int grid, double &y, double &z
   z = -1. + iz * (2. / grid);
 }
 
-// taps of every phase (blockIdx.y): fixed-point sums in LDS when they fit, then global
+// taps of every phase (blockIdx.y): fixed-point sums in LDS when they fit, then global.
+// All cells of one disk row share the shift (m, t), and neighbouring cells of a row
+// share the mu bin, so each wave first sums its lanes' weights over runs of equal
+// (row, bin) (a segmented shuffle scan, lanes = consecutive cells); the last lane of
+// each run adds the run's four tap terms.  This removes the many-way atomic
+// contention of one atomic per cell on the same few tap addresses.
 __global__ __launch_bounds__(kDT) void k_diff_taps(DiffArgs a, int cells_per_block, int use_lds) {
   const double kappa = a.prm ? a.prm[3] : a.kappa;
   extern __shared__ unsigned long long lacc[];  // [n_mu][nd][2 sets][2]
@@ -96,22 +101,46 @@ __global__ __launch_bounds__(kDT) void k_diff_taps(DiffArgs a, int cells_per_blo
   }
   unsigned long long *dst = use_lds ? lacc : gacc;
   const double *mp = a.maps + (long)p * a.map_res * a.map_res;
-  for (long idx = c0 + threadIdx.x; idx < c1; idx += blockDim.x) {
-    int k = a.ck[idx];
-    if (k < 0) continue;
-    double y, z, fy, fz;
-    int i0, j0;
-    cell_yz(idx, a.grid, y, z);
-    double phi = map_sample(mp, a.map_res, y, z, i0, j0, fy, fz);
-    double L = a.cL[idx], t = a.ct[idx];
-    int d0 = a.cm[idx] - a.dmin;  // tap index of offset m; m-1 at d0-1
-    if (d0 < 1 || d0 >= a.nd) continue;  // the host bounds every shift
-    double wq = L * (1 - phi), ws = kappa * L * phi;
-    unsigned long long *row = dst + (long)k * a.nd * 4;
-    fx_add(row + (long)d0 * 4 + 0, wq * (1.0 - t));
-    fx_add(row + (long)(d0 - 1) * 4 + 0, wq * t);
-    fx_add(row + (long)d0 * 4 + 2, ws * (1.0 - t));
-    fx_add(row + (long)(d0 - 1) * 4 + 2, ws * t);
+  const int lane = threadIdx.x & 63;
+  // every lane of a wave runs every iteration (the shuffles need the whole wave)
+  for (long base = c0; base < c1; base += blockDim.x) {
+    long idx = base + threadIdx.x;
+    int key = -1, k = -1, d0 = 0;
+    double wq = 0.0, ws = 0.0, t = 0.0;
+    if (idx < c1) k = a.ck[idx];
+    if (k >= 0) {
+      d0 = a.cm[idx] - a.dmin;  // tap index of offset m; m-1 at d0-1
+      if (d0 >= 1 && d0 < a.nd) {  // the host bounds every shift
+        double y, z, fy, fz;
+        int i0, j0;
+        cell_yz(idx, a.grid, y, z);
+        double phi = map_sample(mp, a.map_res, y, z, i0, j0, fy, fz);
+        double L = a.cL[idx];
+        t = a.ct[idx];
+        wq = L * (1 - phi);
+        ws = kappa * L * phi;
+        key = (int)(idx / a.grid) * 1024 + k;  // (row, bin): rows share (m, t); k < 1024
+      }
+    }
+    const int kprev = __shfl_up(key, 1), knext = __shfl_down(key, 1);
+    bool f = lane == 0 || kprev != key;  // run head
+    const bool tail = lane == 63 || knext != key;
+    for (int off = 1; off < 64; off <<= 1) {  // inclusive segmented scan of (wq, ws)
+      double xq = __shfl_up(wq, off), xs = __shfl_up(ws, off);
+      int fo = __shfl_up((int)f, off);
+      if (lane >= off && !f) {
+        wq += xq;
+        ws += xs;
+        f = fo != 0;
+      }
+    }
+    if (tail && key >= 0) {
+      unsigned long long *row = dst + (long)k * a.nd * 4;
+      fx_add(row + (long)d0 * 4 + 0, wq * (1.0 - t));
+      fx_add(row + (long)(d0 - 1) * 4 + 0, wq * t);
+      fx_add(row + (long)d0 * 4 + 2, ws * (1.0 - t));
+      fx_add(row + (long)(d0 - 1) * 4 + 2, ws * t);
+    }
   }
   if (use_lds) {
     __syncthreads();

@@ -1664,6 +1664,7 @@ int soap_diff_setup(soap_ctx *c, int32_t grid, int32_t n_phase, int32_t map_res,
   if (c->n == 0 || c->n_mu == 0) return fail(c, SOAP_ERR_STATE, "diff_setup: set wave and mu bounds first");
   if (!c->have_mask) return fail(c, SOAP_ERR_STATE, "diff_setup: set the mask first");
   if (c->n_v < 3 || c->n_v > kMaxVDiff) return fail(c, SOAP_ERR_LIMIT, "diff_setup: %d CCF velocities", c->n_v);
+  if (c->n_mu > 1024) return fail(c, SOAP_ERR_LIMIT, "diff_setup: %d mu bins (at most 1024)", c->n_mu);
   HIPCHK(c, hipSetDevice(c->device));
   auto &d = c->diff;
   size_t g2 = (size_t)grid * grid;
