@@ -111,6 +111,7 @@ struct soap_ctx {
   bool have_raw = false;
   // tap scratch
   DevBuf acc, taps, krange, kflags, bins, nbins;
+  DevBuf taps_g;  // phase-interleaved active taps [n_mu][nd][taps_g_pad(P)][4]
 
   // resolution
   double *d_lower = nullptr;  // [(P+1)][3][n]
@@ -679,6 +680,8 @@ static int ensure_taps(soap_ctx *c, int n_job, int nsets_in, int nsets_out) {
       (rc = ensure(c, c->kflags, (size_t)n_job * c->n_mu * 4)) || (rc = ensure(c, c->bins, (size_t)n_job * c->n_mu * 4)) ||
       (rc = ensure(c, c->nbins, (size_t)n_job * 4)))
     return rc;
+  // the phase-interleaved copy is read by the grouped FIR (four sets per entry)
+  if (nsets_out == 4 && (rc = ensure(c, c->taps_g, (size_t)taps_g_pad(n_job) * nsets_out * per * 8))) return rc;
   return SOAP_OK;
 }
 
@@ -729,7 +732,8 @@ int soap_quiet_star(soap_ctx *c) {
     HIPCHK(c, launch_fill_u64(a.acc, (size_t)c->n_mu * c->nd * 2, c->stream));
     HIPCHK(c, launch_taps_quiet(a, c->stream));
     HIPCHK(c, launch_taps_finalize(a.acc, 1, 1, 1, c->n_mu, c->nd, (double *)c->taps.p, (int32_t *)c->krange.p,
-                                   (int32_t *)c->kflags.p, (int32_t *)c->bins.p, (int32_t *)c->nbins.p, c->stream));
+                                   (int32_t *)c->kflags.p, (int32_t *)c->bins.p, (int32_t *)c->nbins.p, nullptr, 0,
+                                   c->stream));
   }
   FirArgs f{};
   f.q = c->d_cube[SOAP_CUBE_QUIET];
@@ -1003,7 +1007,8 @@ int soap_integrate(soap_ctx *c) {
     HIPCHK(c, launch_fill_u64(a.acc, (size_t)c->P * nsets * c->n_mu * c->nd * 2, c->stream));
     HIPCHK(c, launch_taps_active(a, c->P, c->max_col_blocks, c->stream));
     HIPCHK(c, launch_taps_finalize(a.acc, c->P, nsets, nsets + 1, c->n_mu, c->nd, (double *)c->taps.p, (int32_t *)c->krange.p,
-                                   (int32_t *)c->kflags.p, (int32_t *)c->bins.p, (int32_t *)c->nbins.p, c->stream));
+                                   (int32_t *)c->kflags.p, (int32_t *)c->bins.p, (int32_t *)c->nbins.p,
+                                   solar ? nullptr : (double *)c->taps_g.p, taps_g_pad(c->P), c->stream));
   }
   FirArgs f{};
   f.q = c->d_cube[SOAP_CUBE_QUIET];
@@ -1017,6 +1022,8 @@ int soap_integrate(soap_ctx *c) {
   f.dmin = c->dmin;
   f.nd = c->nd;
   f.taps = (const double *)c->taps.p;
+  f.taps_g = (const double *)c->taps_g.p;
+  f.p_pad = taps_g_pad(c->P);
   f.krange = (const int32_t *)c->krange.p;
   f.kflags = (const int32_t *)c->kflags.p;
   f.bins = (const int32_t *)c->bins.p;
@@ -1969,7 +1976,7 @@ int batch_upload(soap_batch *b, int stage, const std::vector<Job> &jobs, const s
 // be stepped through stale pointers.
 std::vector<uintptr_t> batch_signature(const soap_ctx *c) {
   auto u = [](const void *p) { return (uintptr_t)p; };
-  return {u(c->d_raw),    u(c->d_lower), u(c->d_quiet),   u(c->acc.p),    u(c->taps.p),   u(c->krange.p),
+  return {u(c->d_raw),    u(c->d_lower), u(c->d_quiet),   u(c->acc.p),    u(c->taps.p),   u(c->taps_g.p), u(c->krange.p),
           u(c->kflags.p), u(c->bins.p),  u(c->nbins.p),   u(c->part.p),   u(c->mm.p),     u(c->ctap.p),
           u(c->d_ccf),    u(c->ccf_part.p), u(c->f_par.p), u(c->f_model.p), u(c->f_depth.p), u(c->f_bis.p),
           u(c->f_iter.p), u(c->m_start.p), u(c->m_alive.p), u(c->m_val.p), u(c->d_geo),   u(c->d_box),
@@ -2033,8 +2040,9 @@ int batch_build(soap_batch *b) {
     w[BS_TAPS][i] = (int64_t)mcb * P;
     lds_taps = std::max(lds_taps, taps_lds_bytes(a));
     fin[i] = FinalizeJob{a.acc, nsets, nsets + 1, c->n_mu, c->nd, (double *)c->taps.p, (int32_t *)c->krange.p,
-                         (int32_t *)c->kflags.p, (int32_t *)c->bins.p, (int32_t *)c->nbins.p};
-    w[BS_FINALIZE][i] = P;
+                         (int32_t *)c->kflags.p, (int32_t *)c->bins.p, (int32_t *)c->nbins.p,
+                         (double *)c->taps_g.p, P, taps_g_pad(P)};
+    w[BS_FINALIZE][i] = taps_g_pad(P);
     FirArgs f{};
     f.q = c->d_cube[SOAP_CUBE_QUIET];
     f.qs = c->d_cube[SOAP_CUBE_QUIET_SED];
@@ -2047,6 +2055,8 @@ int batch_build(soap_batch *b) {
     f.dmin = c->dmin;
     f.nd = c->nd;
     f.taps = (const double *)c->taps.p;
+    f.taps_g = (const double *)c->taps_g.p;
+    f.p_pad = taps_g_pad(P);
     f.krange = (const int32_t *)c->krange.p;
     f.kflags = (const int32_t *)c->kflags.p;
     f.bins = (const int32_t *)c->bins.p;

@@ -61,6 +61,8 @@ struct FirArgs {
   int64_t n;
   int32_t n_mu, dmin, nd;
   const double *taps;    // quiet: [n_mu][nd]; active: [job][n_mu][nd][4] (sets contiguous)
+  const double *taps_g;  // active, phase-interleaved: [n_mu][nd][p_pad][4], jobs >= n_job zero
+  int32_t p_pad;         // job stride of taps_g (>= n_job + 8)
   const int32_t *krange; // [job][n_mu][2] inclusive d-index range, lo > hi: empty
   const int32_t *kflags; // [job][n_mu]: bit0 spot sets nonzero, bit1 facula sets nonzero
   const int32_t *bins;   // [job][n_mu]
@@ -195,11 +197,13 @@ enum BatchStage {
 };
 struct FillJob { unsigned long long *p; int64_t count; };           // work: ceil(count / batch_fill_chunk())
 struct TapsJob { TapAccArgs a; int32_t mcb; };                        // work: mcb * P, local = p * mcb + col block
-struct FinalizeJob {                                                  // work: P
+struct FinalizeJob {                                                  // work: p_pad
   const unsigned long long *acc;
   int32_t nsets_in, nsets_out, n_mu, nd;
   double *taps;
   int32_t *krange, *kflags, *bins, *nbins;
+  double *taps_g;
+  int32_t n_job, p_pad;
 };
 struct FirJob { FirArgs a; int32_t n_job, n_tiles, n_groups; };      // work: tiles8 * n_groups
 struct ConvQJob { ConvArgs c; const double *quiet; double *frame, *part; };  // work: 2 n_tiles
@@ -233,9 +237,13 @@ hipError_t launch_fill_f64(double *p, size_t count, double v, hipStream_t s);
 hipError_t launch_disk_init(const DiskArgs &a, hipStream_t s);
 hipError_t launch_taps_quiet(const TapAccArgs &a, hipStream_t s);
 hipError_t launch_taps_active(const TapAccArgs &a, int n_job, int max_col_blocks, hipStream_t s);
+// taps_g (optional): the phase-interleaved copy [n_mu][nd][p_pad][nsets_out] of the
+// FIR, zero for jobs n_job..p_pad-1
 hipError_t launch_taps_finalize(const unsigned long long *acc, int n_job, int nsets_in, int nsets_out, int n_mu,
                                 int nd, double *taps, int32_t *krange, int32_t *kflags, int32_t *bins,
-                                int32_t *nbins, hipStream_t s);
+                                int32_t *nbins, double *taps_g, int p_pad, hipStream_t s);
+// job stride of the phase-interleaved tap table for n_job jobs
+inline int taps_g_pad(int n_job) { return (n_job + 7) / 8 * 8 + 8; }
 hipError_t launch_fir_quiet(const FirArgs &a, hipStream_t s);
 hipError_t launch_fir_active(const FirArgs &a, int n_job, hipStream_t s);
 hipError_t launch_fir_quiet_lds(const FirArgs &a, hipStream_t s);

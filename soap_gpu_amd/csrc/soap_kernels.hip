@@ -296,11 +296,18 @@ hipError_t launch_taps_active(const TapAccArgs &a, int n_job, int max_col_blocks
 // Flags: bit 0 the spot set is nonzero, bit 1 a facula set is nonzero.
 __device__ __forceinline__ void taps_finalize_body(const unsigned long long *acc_, int nsets_in, int nsets_out, int n_mu,
                                                    int nd, double *taps_, int32_t *krange_, int32_t *kflags_,
-                                                   int32_t *bins_, int32_t *nbins_, int bx_, int by_) {
+                                                   int32_t *bins_, int32_t *nbins_, double *taps_g_, int n_job,
+                                                   int p_pad, int bx_, int by_) {
   const GLOBAL unsigned long long *acc = gl(acc_);
   GLOBAL double *taps = gw(taps_);
   GLOBAL int32_t *krange = gw(krange_), *kflags = gw(kflags_), *bins = gw(bins_), *nbins = gw(nbins_);
   int job = bx_;
+  if (job >= n_job) {  // padding jobs of the interleaved table: zero taps
+    if (taps_g_)
+      for (long e = threadIdx.x; e < (long)n_mu * nd; e += blockDim.x)
+        for (int st = 0; st < nsets_out; st++) gw(taps_g_)[(e * p_pad + job) * nsets_out + st] = 0.0;
+    return;  // uniform over the block
+  }
   __shared__ int s_lo[256], s_hi[256], s_fl[256];
   for (int k = threadIdx.x; k < n_mu; k += blockDim.x) {
     int lo = nd, hi = -1, fl = 0;
@@ -308,17 +315,22 @@ __device__ __forceinline__ void taps_finalize_body(const unsigned long long *acc
       bool nz = false;
       double v01 = 0.0;
       GLOBAL double *out = taps + (((long)job * n_mu + k) * nd + d) * nsets_out;
+      GLOBAL double *outg = taps_g_ ? gw(taps_g_) + (((long)k * nd + d) * p_pad + job) * nsets_out : nullptr;
       for (int st = 0; st < nsets_in; st++) {
         long e = ((((long)job * nsets_in + st) * n_mu + k) * nd + d) * 2;
         double v = fx_value(acc[e], acc[e + 1]);
         out[st] = v;
+        if (outg) outg[st] = v;
         if (st < 2) v01 += v;
         if (v != 0.0) {
           nz = true;
           fl |= (st == 0 || nsets_in == 1) ? 1 : 2;
         }
       }
-      if (nsets_out > nsets_in) out[nsets_in] = v01;
+      if (nsets_out > nsets_in) {
+        out[nsets_in] = v01;
+        if (outg) outg[nsets_in] = v01;
+      }
       if (nz) {
         lo = d < lo ? d : lo;
         hi = d;
@@ -341,15 +353,18 @@ __device__ __forceinline__ void taps_finalize_body(const unsigned long long *acc
 }
 
 __global__ void k_taps_finalize(const unsigned long long *acc, int nsets_in, int nsets_out, int n_mu, int nd,
-                                double *taps, int32_t *krange, int32_t *kflags, int32_t *bins, int32_t *nbins) {
-  taps_finalize_body(acc, nsets_in, nsets_out, n_mu, nd, taps, krange, kflags, bins, nbins, blockIdx.x, blockIdx.y);
+                                double *taps, int32_t *krange, int32_t *kflags, int32_t *bins, int32_t *nbins,
+                                double *taps_g, int n_job, int p_pad) {
+  taps_finalize_body(acc, nsets_in, nsets_out, n_mu, nd, taps, krange, kflags, bins, nbins, taps_g, n_job, p_pad,
+                     blockIdx.x, blockIdx.y);
 }
 
 hipError_t launch_taps_finalize(const unsigned long long *acc, int n_job, int nsets_in, int nsets_out, int n_mu,
                                 int nd, double *taps, int32_t *krange, int32_t *kflags, int32_t *bins,
-                                int32_t *nbins, hipStream_t s) {
-  if (n_mu > 256) return hipErrorInvalidValue;
-  k_taps_finalize<<<n_job, 64, 0, s>>>(acc, nsets_in, nsets_out, n_mu, nd, taps, krange, kflags, bins, nbins);
+                                int32_t *nbins, double *taps_g, int p_pad, hipStream_t s) {
+  if (n_mu > 256 || (taps_g && p_pad < n_job + 8)) return hipErrorInvalidValue;
+  k_taps_finalize<<<taps_g ? p_pad : n_job, 64, 0, s>>>(acc, nsets_in, nsets_out, n_mu, nd, taps, krange, kflags,
+                                                        bins, nbins, taps_g, n_job, p_pad);
   return hipGetLastError();
 }
 
@@ -443,8 +458,10 @@ hipError_t launch_fir_active(const FirArgs &a, int n_job, hipStream_t s) {
 // registers.  For each mu bin touched by any phase of the group, the block
 // stages that bin's seed-row segments (quiet, quiet SED, and spot / facula rows
 // when present) in LDS; each staged row value then feeds the FMAs of all phases
-// of the group, with the taps read by uniform (scalar) loads.  The inner loop
-// is branch-free: all seven products of every phase slot, over the union of the
+// of the group, with the taps read by uniform (scalar) loads from the
+// phase-interleaved copy of the tap table (the group's taps at one offset are
+// contiguous: two wide scalar loads from one pointer).  The inner loop is
+// branch-free: all seven products of every phase slot, over the union of the
 // group's tap ranges (taps are zero where a phase has no cells), which costs
 // some zero FMAs but no per-phase flag branches.  Blocks are numbered so the
 // groups of a tile run back to back on one XCD (b % 8), keeping the tile's rows
@@ -458,7 +475,7 @@ constexpr int kFirTile = kTPB;
 
 __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
-template <int G>
+template <int G, bool PRE>
 __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job, int n_tiles, int n_groups, int bx_) {
   extern __shared__ double rows[];  // [4][row_len]: quiet, quiet SED, spot, facula rows of one bin
   __shared__ int s_fl[G][64];
@@ -516,6 +533,31 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
 #pragma unroll
     for (int f = 0; f < 7; f++) F[g][f] = 0.0;
   const int nb = uni(s_nb);
+  // PRE: the next bin's rows are loaded into registers (the first kPre x kTPB values
+  // of each row) while the current bin computes, hiding their latency when the tap
+  // ranges are short; costs registers (occupancy) when they are long
+  constexpr int kPre = 2;
+  double pre[4][kPre];
+  auto row_src = [&](int r, int k, int has_sp) {
+    return gl(r == 0 ? a.q : r == 1 ? a.qs : (r == 2 && has_sp) ? a.sp : a.fa) + (long)k * a.n;
+  };
+  auto prefetch = [&](int kk) {
+    const int k = uni(s_bins[kk]);
+    const int ufl = uni(s_ufl[k]);
+    const int has_sp = ufl & 1, nrow = 2 + has_sp + ((ufl >> 1) & 1);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      if (r >= nrow) break;
+      const GLOBAL double *src = row_src(r, k, has_sp);
+#pragma unroll
+      for (int m = 0; m < kPre; m++) {
+        int x = threadIdx.x + m * kTPB;
+        long i = src0 + x;
+        pre[r][m] = (x < row_len && i >= 0 && i < a.n) ? src[i] : 0.0;
+      }
+    }
+  };
+  if (PRE && nb > 0) prefetch(0);
   for (int kk = 0; kk < nb; kk++) {
     const int k = uni(s_bins[kk]);
     const int ufl = uni(s_ufl[k]), ulo = uni(s_ulo[k]), uhi = uni(s_uhi[k]);
@@ -523,27 +565,49 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
     // rows present: quiet, quiet SED, then spot (if any spot cell) and facula (if any)
     const int has_sp = ufl & 1, has_fa = (ufl >> 1) & 1;
     const int nrow = 2 + has_sp + has_fa;
-    for (int r = 0; r < nrow; r++) {
-      const GLOBAL double *src = gl(r == 0 ? a.q : r == 1 ? a.qs : (r == 2 && has_sp) ? a.sp : a.fa) + (long)k * a.n;
-      for (int x = threadIdx.x; x < row_len; x += blockDim.x) {
-        long i = src0 + x;
-        rows[r * row_len + x] = (i >= 0 && i < a.n) ? src[i] : 0.0;
+    if constexpr (PRE) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        if (r >= nrow) break;
+#pragma unroll
+        for (int m = 0; m < kPre; m++) {
+          int x = threadIdx.x + m * kTPB;
+          if (x < row_len) rows[r * row_len + x] = pre[r][m];
+        }
+        const GLOBAL double *src = row_src(r, k, has_sp);
+        for (int x = threadIdx.x + kPre * kTPB; x < row_len; x += blockDim.x) {
+          long i = src0 + x;
+          rows[r * row_len + x] = (i >= 0 && i < a.n) ? src[i] : 0.0;
+        }
       }
+      __syncthreads();
+      if (kk + 1 < nb) prefetch(kk + 1);
+    } else {
+      for (int r = 0; r < nrow; r++) {
+        const GLOBAL double *src = row_src(r, k, has_sp);
+        for (int x = threadIdx.x; x < row_len; x += blockDim.x) {
+          long i = src0 + x;
+          rows[r * row_len + x] = (i >= 0 && i < a.n) ? src[i] : 0.0;
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
-    // taps of phase g at offset di: a.taps[(((p0+g)*n_mu + k)*nd + di)*4 + set], uniform addresses
-    const GLOBAL double *Tk = gl(a.taps) + ((long)p0 * n_mu + k) * nd * 4;
-    const long gstride = (long)n_mu * nd * 4;
+    // taps of the group's phases at offset di: G x 4 consecutive doubles of the
+    // phase-interleaved table (zero for phases past the last), so one uniform base
+    // pointer per offset serves every phase slot
+    const GLOBAL double *Tg = gl(a.taps_g) + (((long)k * nd + ulo) * a.p_pad + p0) * 4;
+    const long dstride = (long)a.p_pad * 4;
     const double *rq = rows, *rs = rows + row_len, *rp = rows + 2 * row_len, *rf = rows + (2 + has_sp) * row_len;
-    for (int di = ulo; di <= uhi; di++) {
-      int e = threadIdx.x + nd - 1 - di;
+    const int e0 = threadIdx.x + nd - 1 - ulo, n_off = uhi - ulo + 1;
+    for (int i = 0; i < n_off; i++, Tg += dstride) {
+      const int e = e0 - i;
       double vq = rq[e], vs = rs[e];
       double vp = (ufl & 1) ? rp[e] : 0.0;
       double vf = (ufl & 2) ? rf[e] : 0.0;
       // every product of every phase slot (taps are zero where a phase has no cells)
 #pragma unroll
       for (int g = 0; g < G; g++) {
-        const GLOBAL double *T = Tk + min(g, ng - 1) * gstride + di * 4;
+        const GLOBAL double *T = Tg + g * 4;
         const double t0 = T[0], t1 = T[1], t2 = T[2], t3 = T[3];
         F[g][0] = fma(t3, vs, F[g][0]);
         F[g][1] = fma(t0, vq, F[g][1]);
@@ -571,9 +635,13 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
   }
 }
 
+template <bool PRE>
 __global__ __launch_bounds__(kTPB) void k_fir_active_grp(FirArgs a, int n_job, int n_tiles, int n_groups) {
-  fir_active_grp_body<kFirG>(a, n_job, n_tiles, n_groups, blockIdx.x);
+  fir_active_grp_body<kFirG, PRE>(a, n_job, n_tiles, n_groups, blockIdx.x);
 }
+
+// shift ranges up to this many offsets take the row-prefetching form
+constexpr int kFirPrefetchMaxNd = 64;
 
 hipError_t launch_fir_active_grp(const FirArgs &a, int n_job, hipStream_t s) {
   if (n_job == 0) return hipSuccess;
@@ -584,7 +652,10 @@ hipError_t launch_fir_active_grp(const FirArgs &a, int n_job, hipStream_t s) {
   long blocks = tiles8 * n_groups;
   size_t lds = (size_t)4 * (kFirTile + a.nd - 1) * sizeof(double);
   if (lds > 60000) return hipErrorInvalidValue;
-  k_fir_active_grp<<<(unsigned)blocks, kTPB, lds, s>>>(a, n_job, n_tiles, n_groups);
+  if (a.nd <= kFirPrefetchMaxNd)
+    k_fir_active_grp<true><<<(unsigned)blocks, kTPB, lds, s>>>(a, n_job, n_tiles, n_groups);
+  else
+    k_fir_active_grp<false><<<(unsigned)blocks, kTPB, lds, s>>>(a, n_job, n_tiles, n_groups);
   return hipGetLastError();
 }
 
@@ -1828,8 +1899,8 @@ __global__ __launch_bounds__(kTPB) void kb_taps(SOAP_BATCH_ARGS(TapsJob)) {
 template <bool P>
 __global__ __launch_bounds__(64) void kb_finalize(SOAP_BATCH_ARGS(FinalizeJob)) {
   batch_loop<P>(jobs, prefix, n_inst, total, [&](const FinalizeJob &j, int local) {
-    taps_finalize_body(j.acc, j.nsets_in, j.nsets_out, j.n_mu, j.nd, j.taps, j.krange, j.kflags, j.bins, j.nbins, local,
-                       0);
+    taps_finalize_body(j.acc, j.nsets_in, j.nsets_out, j.n_mu, j.nd, j.taps, j.krange, j.kflags, j.bins, j.nbins,
+                       j.taps_g, j.n_job, j.p_pad, local, 0);
   });
 }
 
@@ -1844,7 +1915,7 @@ template <bool P, int G>
 __global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(FirWaves<P, G>::v, 8))) void kb_fir(
     SOAP_BATCH_ARGS(FirJob)) {
   batch_loop<P>(jobs, prefix, n_inst, total, [&](const FirJob &j, int local) {
-    fir_active_grp_body<G>(j.a, j.n_job, j.n_tiles, j.n_groups, local);
+    fir_active_grp_body<G, false>(j.a, j.n_job, j.n_tiles, j.n_groups, local);
   });
 }
 
