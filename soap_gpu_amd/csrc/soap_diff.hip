@@ -612,6 +612,22 @@ constexpr int kCorrPer = 9;  // odd: a thread stride of kCorrPer doubles is free
 constexpr int kCorrT = kDT * kCorrPer;
 constexpr int kCorrL = 8;
 constexpr int kCorrPG = 10;
+// Lags the cells can use: a cell's shift index d0 (k_diff_cells: m = ceil(log1p(fb)/h),
+// fb monotone in delta = vsini y, |y| <= 1) lies in [m(-vsini), m(vsini)] - dmin, and the
+// per-cell derivatives read lags d0 and d0 - 1; two lags of margin on each side.
+// Lags outside need no correlation (the tap range fixed by a bound on vsini can be
+// much wider than the current vsini needs).
+__device__ __forceinline__ void diff_lag_range(const DiffArgs &a, int &lo, int &hi) {
+  const double vs = a.prm ? a.prm[0] : a.vsini;
+  auto m = [&](double v) {
+    double beta = v * 1000. / kC;
+    double fb = -1 * (1 - sqrt((1 + beta) / (1 - beta)));
+    return (int)ceil(log1p(fb) / a.h);
+  };
+  lo = max(0, m(-vs) - a.dmin - 3);
+  hi = min(a.nd - 1, m(vs) - a.dmin + 2);
+}
+
 __global__ __launch_bounds__(kDT) void k_diff_corr(DiffArgs a, const double *GI, double *Cpart) {
   extern __shared__ double lds[];  // Q and S rows [kCorrT + nd - 1 + kCorrL] each, then [kCorrL][kDT] partials
   const int k = blockIdx.y, chunk = blockIdx.x, nchunk = gridDim.x;
@@ -635,9 +651,11 @@ __global__ __launch_bounds__(kDT) void k_diff_corr(DiffArgs a, const double *GI,
       g[u] = j < a.n ? GI[(long)p * a.n + j] : 0.0;
     }
     double *out = Cpart + (((long)p * a.n_mu + k) * nchunk + chunk) * nd * 2;
+    int dlo, dhi;
+    diff_lag_range(a, dlo, dhi);  // uniform
     for (int set = 0; set < 2; set++) {
       const double *row = (set ? rs : rq) + threadIdx.x * kCorrPer;
-      for (int d0 = 0; d0 < nd; d0 += kCorrL) {
+      for (int d0 = dlo; d0 <= dhi; d0 += kCorrL) {
         // lag d = d0 + l reads X[j - dmin - d] = row[u + nd - 1 - d]; window base nd - d0 - kCorrL
         const int wb = nd - d0 - kCorrL;
         double w[kCorrPer + kCorrL - 1];
@@ -669,12 +687,16 @@ __global__ __launch_bounds__(kDT) void k_diff_corr(DiffArgs a, const double *GI,
   }
 }
 
-__global__ void k_diff_corr_reduce(int n_pk, int nd, int nchunk, const double *Cpart, double *C) {
+__global__ void k_diff_corr_reduce(DiffArgs a, int n_pk, int nd, int nchunk, const double *Cpart, double *C) {
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;  // (p, k) x (d, set)
   if (e >= (long)n_pk * nd * 2) return;
   long pk = e / (nd * 2), r = e % (nd * 2);
+  int dlo, dhi;
+  diff_lag_range(a, dlo, dhi);
+  const int d = (int)(r / 2);
   double x = 0.0;
-  for (int c = 0; c < nchunk; c++) x += Cpart[(pk * nchunk + c) * nd * 2 + r];
+  if (d >= dlo && d <= dhi)  // k_diff_corr computes only these lags
+    for (int c = 0; c < nchunk; c++) x += Cpart[(pk * nchunk + c) * nd * 2 + r];
   C[e] = x;
 }
 
@@ -938,7 +960,7 @@ hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const doub
   k_diff_corr<<<gr, kDT, lds_corr, s>>>(a, GI, Cpart);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   long nc = (long)a.n_phase * a.n_mu * a.nd * 2;
-  k_diff_corr_reduce<<<(unsigned)((nc + kDT - 1) / kDT), kDT, 0, s>>>(a.n_phase * a.n_mu, a.nd, nchunk, Cpart, C);
+  k_diff_corr_reduce<<<(unsigned)((nc + kDT - 1) / kDT), kDT, 0, s>>>(a, a.n_phase * a.n_mu, a.nd, nchunk, Cpart, C);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   long ncell = (long)a.grid * a.grid;
   k_diff_cellgrad<<<(unsigned)((ncell + kDT - 1) / kDT), kDT, 0, s>>>(a, C, cell_part);
