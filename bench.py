@@ -190,6 +190,7 @@ def batch_leg(n_inst: int, steps: int, warmup: int, workers: int = 8):
     t_tune = time.time() - t0
     cfg = b.config()
     log(f"[bench:batch] autotuned in {t_tune:.1f} s: taps_cells={cfg['taps_cells']} fir_g={cfg['fir_g']} "
+        f"fir_b={cfg['fir_b']} fir_pre={cfg['fir_pre']} "
         f"grid={cfg['grid']}")
     ms_eager = timed(lambda: b.step(graph=False), b.sync, steps)
     b.step(graph=True)

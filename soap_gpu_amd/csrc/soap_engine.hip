@@ -1919,6 +1919,7 @@ struct soap_batch {
   int taps_cells = 4096;  // disk cells per taps block (cells-per-block shape)
   int fir_g = 4;          // phases per FIR block
   int fir_b = 0;          // pixels per FIR thread (0: one-pixel form)
+  int fir_pre = 0;        // one-pixel form: prefetch the next bin's seed rows into registers
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   hipEvent_t ev[BS_COUNT + 1] = {};
@@ -2079,7 +2080,7 @@ int batch_build(soap_batch *b) {
     cq[i] = ConvQJob{cv, c->d_quiet, c->d_lower + (size_t)3 * P * c->n, part + (size_t)3 * P * cv.n_tiles * 2};
     w[BS_CONV_QUIET][i] = 2L * cv.n_tiles;
     ca[i] = ConvAJob{cv, c->d_quiet, c->d_raw, c->d_lower, part, P};
-    w[BS_CONV][i] = (int64_t)(cv.n_tiles + 7) / 8 * 8 * P;
+    w[BS_CONV][i] = (int64_t)(cv.n_tiles + 7) / 8 * 8 * P * 3;  // one block per (tile, phase, type)
     lds_conv = std::max(lds_conv, conv_lds_bytes(cv.ntp));
     mm[i] = MinmaxJob{part, cv.n_tiles, (double *)c->mm.p};
     w[BS_MINMAX][i] = n_spec;
@@ -2135,7 +2136,7 @@ int batch_enqueue(soap_batch *b, bool events) {
     BatchStageBuf &sb = b->st[s];
     if (events) BHIP(b, hipEventRecord(b->ev[s], b->stream));
     hipError_t e = launch_batch(s, sb.jobs, sb.prefix, (int)b->inst.size(), sb.total, sb.grid, sb.threads, sb.lds,
-                                b->fir_g, b->fir_b, b->stream);
+                                b->fir_g, b->fir_b, b->fir_pre, b->stream);
     if (e != hipSuccess) return bfail(b, SOAP_ERR_HIP, "batch stage %s: %s", kBatchStageNames[s], hipGetErrorString(e));
   }
   if (events) BHIP(b, hipEventRecord(b->ev[BS_COUNT], b->stream));
@@ -2265,12 +2266,13 @@ int soap_batch_stage_ms(soap_batch *b, double *ms, int32_t n_stage) {
 const char *soap_batch_stage_name(int32_t s) { return (s >= 0 && s < BS_COUNT) ? kBatchStageNames[s] : ""; }
 int32_t soap_batch_num_stages(void) { return BS_COUNT; }
 
-int soap_batch_get_config(soap_batch *b, int32_t *taps_cells, int32_t *fir_g, int32_t *fir_b, int64_t *grid,
-                          int64_t *work, int32_t n_stage) {
+int soap_batch_get_config(soap_batch *b, int32_t *taps_cells, int32_t *fir_g, int32_t *fir_b, int32_t *fir_pre,
+                          int64_t *grid, int64_t *work, int32_t n_stage) {
   if (!b || n_stage < BS_COUNT) return SOAP_ERR_ARG;
   if (taps_cells) *taps_cells = b->taps_cells;
   if (fir_g) *fir_g = b->fir_g;
   if (fir_b) *fir_b = b->fir_b;
+  if (fir_pre) *fir_pre = b->fir_pre;
   for (int s = 0; s < BS_COUNT; s++) {
     if (grid) grid[s] = b->st[s].grid;
     if (work) work[s] = b->st[s].total;
@@ -2285,13 +2287,15 @@ static bool batch_fir_shape_ok(int g, int bb) {
   return false;
 }
 
-int soap_batch_set_config(soap_batch *b, int32_t taps_cells, int32_t fir_g, int32_t fir_b, const int64_t *grid,
-                          int32_t n_stage) {
+int soap_batch_set_config(soap_batch *b, int32_t taps_cells, int32_t fir_g, int32_t fir_b, int32_t fir_pre,
+                          const int64_t *grid, int32_t n_stage) {
   if (!b || n_stage < BS_COUNT || !grid) return SOAP_ERR_ARG;
   if (taps_cells < 64 || taps_cells > (1 << 20)) return bfail(b, SOAP_ERR_ARG, "set_config: taps_cells out of range");
   if (!batch_fir_shape_ok(fir_g, fir_b))
     return bfail(b, SOAP_ERR_ARG, "set_config: FIR shape (%d, %d) not one of (2,0) (4,0) (6,0) (1,3) (1,5) (2,3)",
                  fir_g, fir_b);
+  if (fir_pre != 0 && (fir_pre != 1 || fir_b != 0))
+    return bfail(b, SOAP_ERR_ARG, "set_config: fir_pre is 0 or 1, and 1 only with the one-pixel form (fir_b 0)");
   for (int s = 0; s < BS_COUNT; s++)
     if (grid[s] < 0 || grid[s] % 8) return bfail(b, SOAP_ERR_ARG, "set_config: grid sizes are multiples of 8 (0 = all)");
   BHIP(b, hipSetDevice(b->device));
@@ -2299,6 +2303,7 @@ int soap_batch_set_config(soap_batch *b, int32_t taps_cells, int32_t fir_g, int3
   b->taps_cells = taps_cells;
   b->fir_g = fir_g;
   b->fir_b = fir_b;
+  b->fir_pre = fir_pre;
   for (int s = 0; s < BS_COUNT; s++) b->st[s].grid = grid[s];
   return batch_build(b);
 }
@@ -2335,18 +2340,21 @@ int soap_batch_tune(soap_batch *b, int32_t reps) {
     if (t < best) best = t, best_cells = cells;
   }
   b->taps_cells = best_cells;
-  // FIR shape: phases per block and pixels per thread (the lambda tile)
-  const int g_c[][2] = {{2, 0}, {4, 0}, {6, 0}, {1, 3}, {1, 5}, {2, 3}};
-  int best_g = b->fir_g, best_b = b->fir_b;
+  // FIR shape: phases per block, pixels per thread (the lambda tile) and, for the
+  // one-pixel form, the seed-row prefetch
+  const int g_c[][3] = {{2, 0, 0}, {4, 0, 0}, {6, 0, 0}, {2, 0, 1}, {4, 0, 1}, {6, 0, 1}, {1, 3, 0}, {1, 5, 0}, {2, 3, 0}};
+  int best_g = b->fir_g, best_b = b->fir_b, best_pre = b->fir_pre;
   best = HUGE_VAL;
   for (auto &gb : g_c) {
     b->fir_g = gb[0];
     b->fir_b = gb[1];
+    b->fir_pre = gb[2];
     if ((rc = batch_build(b)) || (rc = score(BS_FIR, BS_FIR, &t))) return rc;
-    if (t < best) best = t, best_g = gb[0], best_b = gb[1];
+    if (t < best) best = t, best_g = gb[0], best_b = gb[1], best_pre = gb[2];
   }
   b->fir_g = best_g;
   b->fir_b = best_b;
+  b->fir_pre = best_pre;
   if ((rc = batch_build(b))) return rc;
   const int64_t mult[] = {0, 1, 2, 4, 8, 16};
   for (int s = 0; s < BS_COUNT; s++) {

@@ -344,9 +344,10 @@ hipError_t launch_diff_grad4(int n_blocks, const double *cell_part, double *g4, 
 // batched instances: stage st over jobs[n_inst] (device), prefix[n_inst + 1]
 // (device, prefix[n_inst] = total); grid = persistent block count (0: one block
 // per work item), threads / lds as the stage needs; FIR shape: fir_g phases per
-// block, fir_b pixels per thread (0: the one-pixel-per-thread form)
+// block, fir_b pixels per thread (0: the one-pixel-per-thread form), fir_pre
+// (one-pixel form only): prefetch the next bin's seed rows into registers
 hipError_t launch_batch(int st, const void *jobs, const int64_t *prefix, int n_inst, int64_t total, int64_t grid,
-                        int threads, size_t lds, int fir_g, int fir_b, hipStream_t s);
+                        int threads, size_t lds, int fir_g, int fir_b, int fir_pre, hipStream_t s);
 int batch_fill_chunk();
 int norm_px_per_block();
 int ccf_spec_chunk();

@@ -1078,18 +1078,34 @@ hipError_t launch_phase_maps(int grid, int t_star, int t_diff_spot, int t_diff_f
 #define SOAP_CONV_B 11
 #endif
 constexpr int kConvB = SOAP_CONV_B;  // outputs per thread (odd: conflict-free LDS stride)
-constexpr int kConvCH = 9;
+#ifndef SOAP_CONV_CH
+#define SOAP_CONV_CH 9
+#endif
+constexpr int kConvCH = SOAP_CONV_CH;
 constexpr int kConvTile = kTPB * kConvB;
+constexpr int kStageU = 8;  // global loads per thread issued together when staging
 constexpr int kConvMaxTaps = 4095;
 
 // stage x[i0 + dlo + e], e in [0, kConvTile + ntp - 1), into LDS; g(x) = 1 - x for
 // the quiet term, x otherwise; zero outside [0, n)
 template <bool ONE_MINUS>
 __device__ __forceinline__ void conv_stage(double *xs, const GLOBAL double *X, long i0, int dlo, int span, long n) {
-  for (int e = threadIdx.x; e < span; e += blockDim.x) {
-    long m = i0 + dlo + e;
-    double v = (m >= 0 && m < n) ? X[m] : 0.0;
-    xs[e] = ONE_MINUS ? 1.0 - v : v;
+  for (int e0 = 0; e0 < span; e0 += kStageU * kTPB) {
+    double xv[kStageU];
+#pragma unroll
+    for (int u = 0; u < kStageU; u++) {  // all loads in flight before the first use
+      long m = i0 + dlo + e0 + threadIdx.x + u * kTPB;
+      xv[u] = X[m < 0 ? 0 : (m >= n ? n - 1 : m)];
+    }
+    __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
+#pragma unroll
+    for (int u = 0; u < kStageU; u++) {  // selects, not branches: the loads stay in flight together
+      int e = e0 + threadIdx.x + u * kTPB;
+      long m = i0 + dlo + e;
+      const double x = xv[u];
+      double v = (m >= 0 && m < n) ? x : 0.0;
+      if (e < span) xs[e] = ONE_MINUS ? 1.0 - v : v;
+    }
   }
 }
 
@@ -1191,37 +1207,51 @@ __global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *q
 // lower_resolution_gpu (SOAP_resolution.cu:235-310): A = Q[sel] - D is staged as
 // 1 - A, o = sum_d c_d (1 - A_{i+d}) for wid < i < n-1-wid (0 elsewhere), with
 // the partial min(A) over the tile's pixels and max(o != 0) (seed -60000).
-// sel = 0 for bconv, 1 otherwise (SOAP_resolution.cu:414-416).  Blocks are
-// numbered so that all phases of one lambda tile run back to back on one XCD
-// (block b -> XCD b % 8): the tile's quiet rows stay in that XCD's L2.
+// sel = 0 for bconv, 1 otherwise (SOAP_resolution.cu:414-416).  One block per
+// (lambda tile, phase, type); blocks are numbered so that all spectra of one
+// lambda tile run back to back on one XCD (block b -> XCD b % 8): the tile's
+// quiet rows stay in that XCD's L2.  Staging issues kStageU loads of Q and D per
+// thread before their first use.
 __device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double *quiet_, const double *raw_, double *o_out_, double *part_, int n_phase, int bx_, int by_) {
   const GLOBAL double *quiet = gl(quiet_), *raw = gl(raw_);
   GLOBAL double *o_out = gw(o_out_), *part = gw(part_);
   extern __shared__ double xs[];
   const int slot = bx_ >> 3;
-  const int tile = (slot / n_phase) * 8 + (bx_ & 7);
-  const int p = slot % n_phase;
+  const int tile = (slot / (3 * n_phase)) * 8 + (bx_ & 7);
+  const int p = (slot % (3 * n_phase)) / 3, t = slot % 3;
   if (tile >= c.n_tiles) return;  // uniform over the block
   long i0 = (long)tile * kConvTile;
   int span = kConvTile + c.ntp - 1;
   double *ct = xs + span + (span & 1);  // taps, 16-byte aligned
   for (int e = threadIdx.x; e < c.ntp; e += blockDim.x) ct[e] = gl(c.ctap)[e];
-#pragma unroll 1
-  for (int t = 0; t < 3; t++) {
+  {
     int sel = t == 1 ? 0 : 1;
     long sp = (long)p * 3 + t;
     const GLOBAL double *D = raw + sp * c.n;
     const GLOBAL double *Q = quiet + (long)sel * c.n;
     double amin = HUGE_VAL, omax = -60000.0;
-    for (int e = threadIdx.x; e < span; e += blockDim.x) {
-      long m = i0 + c.dlo + e;
-      double v = 1.0;
-      if (m >= 0 && m < c.n) {
-        double A = Q[m] - D[m];
-        v = 1.0 - A;
-        if (m >= i0 && m < i0 + kConvTile) amin = A < amin ? A : amin;
+    // kStageU loads of Q and of D in flight per thread before the first use
+    // (clamped addresses, so every load is unconditional)
+    for (int e0 = 0; e0 < span; e0 += kStageU * kTPB) {
+      double qv[kStageU], dv[kStageU];
+#pragma unroll
+      for (int u = 0; u < kStageU; u++) {
+        long m = i0 + c.dlo + e0 + threadIdx.x + u * kTPB;
+        long mc = m < 0 ? 0 : (m >= c.n ? c.n - 1 : m);
+        qv[u] = Q[mc];
+        dv[u] = D[mc];
       }
-      xs[e] = v;
+      __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
+#pragma unroll
+      for (int u = 0; u < kStageU; u++) {  // selects, not branches: the loads stay in flight together
+        int e = e0 + threadIdx.x + u * kTPB;
+        long m = i0 + c.dlo + e;
+        const bool in = m >= 0 && m < c.n;
+        const double A = qv[u] - dv[u];
+        const double v = in ? 1.0 - A : 1.0;
+        amin = (in && m >= i0 && m < i0 + kConvTile && A < amin) ? A : amin;
+        if (e < span) xs[e] = v;
+      }
     }
     __syncthreads();
     double acc[kConvB];
@@ -1263,7 +1293,7 @@ hipError_t launch_conv_active(const ConvArgs &c, const double *quiet, const doub
   if (c.ntp % kConvCH || c.ntp > kConvMaxTaps) return hipErrorInvalidValue;
   size_t lds = conv_lds(c);
   long tiles8 = ((long)c.n_tiles + 7) / 8 * 8;
-  long blocks = tiles8 * n_phase;
+  long blocks = tiles8 * n_phase * 3;
   if (blocks > 0x7fffffffL || lds > 65536) return hipErrorInvalidValue;
   k_conv_active<<<(unsigned)blocks, kTPB, lds, s>>>(c, quiet, raw, o_out, part, n_phase);
   return hipGetLastError();
@@ -1906,16 +1936,17 @@ __global__ __launch_bounds__(64) void kb_finalize(SOAP_BATCH_ARGS(FinalizeJob)) 
 
 // occupancy floor of the FIR blocks: the single-instance kernel's register budget
 // (more waves per SIMD would spill the accumulators of the inner loop)
-template <bool P, int G>
+// (the row-prefetching form holds 16 more registers per thread)
+template <bool P, int G, bool PRE>
 struct FirWaves {
-  static constexpr int v = G == 2 ? 8 : G == 4 ? (P ? 5 : 6) : 4;
+  static constexpr int v = PRE ? (G == 2 ? 6 : 4) : G == 2 ? 8 : G == 4 ? (P ? 5 : 6) : 4;
 };
 
-template <bool P, int G>
-__global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(FirWaves<P, G>::v, 8))) void kb_fir(
+template <bool P, int G, bool PRE>
+__global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(FirWaves<P, G, PRE>::v, 8))) void kb_fir(
     SOAP_BATCH_ARGS(FirJob)) {
   batch_loop<P>(jobs, prefix, n_inst, total, [&](const FirJob &j, int local) {
-    fir_active_grp_body<G, false>(j.a, j.n_job, j.n_tiles, j.n_groups, local);
+    fir_active_grp_body<G, PRE>(j.a, j.n_job, j.n_tiles, j.n_groups, local);
   });
 }
 
@@ -1985,7 +2016,7 @@ size_t conv_lds_bytes(int ntp) { return (size_t)(kConvTile + 2 * ntp + 1) * size
 size_t taps_lds_bytes(const TapAccArgs &a) { return (size_t)a.nsets * a.n_mu * a.wd * 2 * sizeof(unsigned long long); }
 
 hipError_t launch_batch(int st, const void *jobs, const int64_t *prefix, int n_inst, int64_t total, int64_t grid,
-                        int threads, size_t lds, int fir_g, int fir_b, hipStream_t s) {
+                        int threads, size_t lds, int fir_g, int fir_b, int fir_pre, hipStream_t s) {
   if (total == 0) return hipSuccess;
   if (n_inst < 1 || total < 0 || grid < 0 || lds > 65536) return hipErrorInvalidValue;
   int64_t g = grid == 0 ? total : std::min<int64_t>(grid, total);
@@ -2012,10 +2043,15 @@ hipError_t launch_batch(int st, const void *jobs, const int64_t *prefix, int n_i
 #undef SOAP_BATCH_CASE
     case BS_FIR:
       if (threads != kTPB) return hipErrorInvalidValue;
-#define SOAP_FIR_CASE(G)                                                                                 \
-  case G:                                                                                                \
-    if (pers) kb_fir<true, G><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total);     \
-    else kb_fir<false, G><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total);         \
+#define SOAP_FIR_CASE(G)                                                                                   \
+  case G:                                                                                                  \
+    if (fir_pre) {                                                                                         \
+      if (pers) kb_fir<true, G, true><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total); \
+      else kb_fir<false, G, true><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total);     \
+    } else {                                                                                               \
+      if (pers) kb_fir<true, G, false><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total); \
+      else kb_fir<false, G, false><<<nb, threads, lds, s>>>((const FirJob *)jobs, prefix, n_inst, total);     \
+    }                                                                                                      \
     break;
 #define SOAP_FIR2_CASE(GG, BB)                                                                            \
   if (fir_g == GG && fir_b == BB) {                                                                      \

@@ -135,9 +135,9 @@ def load(path: str = LIB_PATH):
         "soap_batch_stage_ms": (ctypes.c_int, [vp, _dp, ctypes.c_int32]),
         "soap_batch_stage_name": (ctypes.c_char_p, [ctypes.c_int32]),
         "soap_batch_num_stages": (ctypes.c_int32, []),
-        "soap_batch_get_config": (ctypes.c_int, [vp, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_int64),
+        "soap_batch_get_config": (ctypes.c_int, [vp, _ip, _ip, _ip, _ip, ctypes.POINTER(ctypes.c_int64),
                                                  ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
-        "soap_batch_set_config": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+        "soap_batch_set_config": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                                  ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]),
         "soap_batch_tune": (ctypes.c_int, [vp, ctypes.c_int32]),
         "soap_diff_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp)]),
@@ -483,14 +483,15 @@ class Batch:
         cells = ctypes.c_int32()
         g = ctypes.c_int32()
         bb = ctypes.c_int32()
+        pre = ctypes.c_int32()
         grid = (ctypes.c_int64 * self.n_stage)()
         work = (ctypes.c_int64 * self.n_stage)()
-        self._chk(self.L.soap_batch_get_config(self.b, ctypes.byref(cells), ctypes.byref(g), ctypes.byref(bb), grid,
-                                               work, self.n_stage), "batch_get_config")
-        return {"taps_cells": cells.value, "fir_g": g.value, "fir_b": bb.value,
+        self._chk(self.L.soap_batch_get_config(self.b, ctypes.byref(cells), ctypes.byref(g), ctypes.byref(bb),
+                                               ctypes.byref(pre), grid, work, self.n_stage), "batch_get_config")
+        return {"taps_cells": cells.value, "fir_g": g.value, "fir_b": bb.value, "fir_pre": pre.value,
                 "grid": dict(zip(self.stage_names, list(grid))), "work": dict(zip(self.stage_names, list(work)))}
 
-    def set_config(self, taps_cells: int, fir_g: int, fir_b: int, grid: dict):
+    def set_config(self, taps_cells: int, fir_g: int, fir_b: int, grid: dict, fir_pre: int = 0):
         arr = (ctypes.c_int64 * self.n_stage)(*[int(grid.get(n, 0)) for n in self.stage_names])
-        self._chk(self.L.soap_batch_set_config(self.b, int(taps_cells), int(fir_g), int(fir_b), arr, self.n_stage),
-                  "batch_set_config")
+        self._chk(self.L.soap_batch_set_config(self.b, int(taps_cells), int(fir_g), int(fir_b), int(fir_pre), arr,
+                                               self.n_stage), "batch_set_config")

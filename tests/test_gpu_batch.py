@@ -106,20 +106,22 @@ def test_batch_tuned_shapes(batch_case):
     b.tune(reps=1)
     cfg = b.config()
     assert (cfg["fir_g"], cfg["fir_b"]) in ((2, 0), (4, 0), (6, 0), (1, 3), (1, 5), (2, 3))
+    assert cfg["fir_pre"] in (0, 1) and (cfg["fir_pre"] == 0 or cfg["fir_b"] == 0)
     assert all(g % 8 == 0 and g >= 0 for g in cfg["grid"].values())
     for name in ("taps", "fir", "conv", "ccf", "fit"):
         assert cfg["work"][name] > 0, name
     rng = np.random.default_rng(5)
-    shapes = [(1024, 2, 0, 8), (16384, 6, 0, 16), (4096, 4, 0, None), (2048, 1, 3, None), (4096, 1, 5, 4),
-              (8192, 2, 3, None), (cfg["taps_cells"], cfg["fir_g"], cfg["fir_b"], "tuned")]
-    for cells, g, bb, mult in shapes:
+    shapes = [(1024, 2, 0, 0, 8), (16384, 6, 0, 0, 16), (4096, 4, 0, 0, None), (2048, 1, 3, 0, None),
+              (4096, 1, 5, 0, 4), (8192, 2, 3, 0, None), (4096, 4, 0, 1, None), (2048, 2, 0, 1, 8),
+              (4096, 6, 0, 1, 16), (cfg["taps_cells"], cfg["fir_g"], cfg["fir_b"], cfg["fir_pre"], "tuned")]
+    for cells, g, bb, pre, mult in shapes:
         if mult == "tuned":
             grid = cfg["grid"]
         elif mult is None:
             grid = {n: 0 for n in cfg["grid"]}
         else:
             grid = {n: mult * 8 for n in cfg["grid"]}  # persistent: far fewer blocks than items
-        b.set_config(cells, g, bb, grid)
+        b.set_config(cells, g, bb, grid, fir_pre=pre)
         for w, e in zip(ws, engines):
             _clobber(e, w, rng)
         b.step(graph=True)
@@ -127,7 +129,7 @@ def test_batch_tuned_shapes(batch_case):
         got = [_outputs(e, w) for w, e in zip(ws, engines)]
         if bb == 0:  # the plain path's FIR form: bit for bit
             for i, r in enumerate(refs):
-                _same(got[i], r, f"shapes {cells}/{g}/{bb}/{mult}[{i}]")
+                _same(got[i], r, f"shapes {cells}/{g}/{bb}/{pre}/{mult}[{i}]")
         else:  # register-blocked FIR: the same sums in another order
             for i, (w, r) in enumerate(zip(ws, refs)):
                 for p in range(len(w.phases)):
