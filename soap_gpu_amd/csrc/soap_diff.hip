@@ -898,11 +898,24 @@ __global__ __launch_bounds__(kDT) void k_diff_loss(int n_phase, int res_blocks, 
   }
 }
 
-__global__ void k_diff_grad4(int n_blocks, const double *cell_part, double *g4) {
-  if (threadIdx.x >= 4) return;
-  double s = 0;
-  for (int b = 0; b < n_blocks; b++) s += cell_part[(long)b * 4 + threadIdx.x];
-  g4[threadIdx.x] = s;
+// g4[q] = sum over blocks of cell_part[b][q]: thread t sums blocks t, t + kDT, ...
+// (independent loads), then a fixed-order tree over the threads
+__global__ __launch_bounds__(kDT) void k_diff_grad4(int n_blocks, const double *cell_part, double *g4) {
+  __shared__ double red[4][kDT];
+  double s4[4] = {0, 0, 0, 0};
+  for (int b = threadIdx.x; b < n_blocks; b += kDT)
+#pragma unroll
+    for (int q = 0; q < 4; q++) s4[q] += cell_part[(long)b * 4 + q];
+#pragma unroll
+  for (int q = 0; q < 4; q++) red[q][threadIdx.x] = s4[q];
+  __syncthreads();
+  for (int off = kDT / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off)
+#pragma unroll
+      for (int q = 0; q < 4; q++) red[q][threadIdx.x] += red[q][threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x < 4) g4[threadIdx.x] = red[threadIdx.x][0];
 }
 
 hipError_t launch_diff_loss(int n_phase, int res_blocks, const double *res_part, const double *theta,
@@ -917,7 +930,7 @@ hipError_t launch_diff_loss(int n_phase, int res_blocks, const double *res_part,
 }
 
 hipError_t launch_diff_grad4(int n_blocks, const double *cell_part, double *g4, hipStream_t s) {
-  k_diff_grad4<<<1, 64, 0, s>>>(n_blocks, cell_part, g4);
+  k_diff_grad4<<<1, kDT, 0, s>>>(n_blocks, cell_part, g4);
   return hipGetLastError();
 }
 int diff_cell_blocks(int64_t ncell) { return (int)((ncell + kDT - 1) / kDT); }

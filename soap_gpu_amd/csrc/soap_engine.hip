@@ -97,7 +97,7 @@ struct soap_ctx {
     DevBuf ck, cm, ct, cL, cmu, cdt, acc, taps, krange, I, F, res_part, ccf, par, model, depth, bis, iters, th,
         dth, gy, GF, GI, C, Cpart, cell_part;
     DevBuf bisd;  // [P][n_v + kBisHead]: span, loss coefficient, dspan/dtheta, dspan/dy
-    DevBuf tg;    // per-call targets [3][P] (RV, FWHM, BIS span) and the loss [1]
+    DevBuf tg;    // per-call targets [3][P] (RV, FWHM, BIS span), the loss [1], the scalar gradients [4]
     DevBuf mt_ptr, mt_w, mt_col, mt_val;  // the mask in sliced pixel-major form (adjoint of the CCF)
     bool mt_ready = false;
   } diff;
@@ -1687,7 +1687,7 @@ int soap_diff_setup(soap_ctx *c, int32_t grid, int32_t n_phase, int32_t map_res,
       (rc = ensure(c, d.th, (size_t)n_phase * 4 * 8)) || (rc = ensure(c, d.dth, (size_t)n_phase * 4 * 8)) ||
       (rc = ensure(c, d.gy, (size_t)n_phase * c->n_v * 8)) ||
       (rc = ensure(c, d.bisd, (size_t)n_phase * (c->n_v + kBisHead) * 8)) ||
-      (rc = ensure(c, d.tg, ((size_t)n_phase * 3 + 1) * 8)) ||
+      (rc = ensure(c, d.tg, ((size_t)n_phase * 3 + 5) * 8)) ||
       (rc = ensure(c, d.cell_part, (size_t)diff_cell_blocks((int64_t)g2) * 4 * 8)) ||
       (rc = ensure(c, c->ccf_part, (size_t)ccf_chunks(c->n_lines) * n_phase * c->n_v * 8)))
     return rc;
@@ -1850,13 +1850,12 @@ int soap_diff_backward(soap_ctx *c, double *grad4, double *d_gq, double *d_gs, d
     if (e == hipErrorInvalidValue) return fail(c, SOAP_ERR_LIMIT, "diff_backward: window or shift range too wide");
     HIPCHK(c, e);
   }
-  int nb = diff_cell_blocks((int64_t)d.G * d.G);
-  std::vector<double> cp((size_t)nb * 4);
-  HIPCHK(c, hipMemcpyAsync(cp.data(), d.cell_part.p, cp.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  // the scalar gradients: the same device reduction as the graph path; the sums
+  // land in the tail of the loss slot buffer
+  double *g4 = (double *)d.tg.p + 3 * d.P + 1;
+  HIPCHK(c, launch_diff_grad4(diff_cell_blocks((int64_t)d.G * d.G), (const double *)d.cell_part.p, g4, c->stream));
+  HIPCHK(c, hipMemcpyAsync(grad4, g4, 4 * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (int q = 0; q < 4; q++) grad4[q] = 0;
-  for (int b = 0; b < nb; b++)
-    for (int q = 0; q < 4; q++) grad4[q] += cp[(size_t)b * 4 + q];
   return SOAP_OK;
 }
 
