@@ -112,6 +112,11 @@ struct soap_ctx {
   // tap scratch
   DevBuf acc, taps, krange, kflags, bins, nbins;
   DevBuf taps_g;  // phase-interleaved active taps [n_mu][nd][taps_g_pad(P)][4]
+  // quiet-star tap scratch of its own, so that the quiet star can run beside the
+  // active phases (soap_run_series puts it on a second stream)
+  DevBuf qacc, qtaps, qkrange, qkflags, qbins, qnbins;
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   // resolution
   double *d_lower = nullptr;  // [(P+1)][3][n]
@@ -342,24 +347,15 @@ void soap_ctx_destroy(soap_ctx *c) {
   if (!c) return;
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
+  if (c->aux) hipStreamSynchronize(c->aux);
   collect_timing(c);
-  dfree(c->d_wave);
-  dfree(c->d_mub);
-  for (int i = 0; i < 6; i++)
-    if (c->cube_owned[i]) dfree(c->d_cube[i]);
-  void *ptrs[] = {c->d_mu,      c->d_delta,   c->d_limb,    c->d_lbm,      c->d_t,      c->d_region, c->d_m,
-                  c->d_quiet,   c->d_geo,     c->d_geo_off, c->d_box,      c->d_raw,    c->d_lower,  c->d_line_l,
-                  c->d_line_w,  c->d_w_sorted, c->d_vgrid,   c->d_appod,   c->d_ccf,      c->acc.p,    c->taps.p,   c->krange.p,
-                  c->kflags.p,  c->bins.p,    c->nbins.p,   c->part.p,     c->mm.p,     c->ctap.p,   c->qsel.p,
-                  c->variant.p, c->m_start.p, c->m_len.p,   c->m_val.p,    c->m_alive.p, c->sorted.p, c->rank.p,
-                  c->f_par.p,   c->f_model.p, c->f_depth.p, c->f_bis.p,    c->f_iter.p, c->ccf_part.p,
-                  c->qcells.p,  c->qbin_off.p, c->acells.p, c->abin_off.p, c->alist_off.p,
-                  c->diff.ck.p, c->diff.cm.p,  c->diff.ct.p, c->diff.cL.p,  c->diff.cmu.p,  c->diff.cdt.p,
-                  c->diff.acc.p, c->diff.taps.p, c->diff.krange.p, c->diff.I.p, c->diff.F.p, c->diff.res_part.p,
-                  c->diff.ccf.p, c->diff.par.p, c->diff.model.p, c->diff.depth.p, c->diff.bis.p, c->diff.iters.p,
-                  c->diff.th.p, c->diff.dth.p, c->diff.gy.p, c->diff.GF.p, c->diff.GI.p, c->diff.C.p,
-                  c->diff.cell_part.p};
-  for (void *p : ptrs) dfree(p);
+  // every device buffer of the context came from dalloc, which records it in caps
+  // (caller-owned cube pointers are not recorded and are not freed)
+  for (auto &kv : c->caps) hipFree(kv.first);
+  c->caps.clear();
+  if (c->aux) hipStreamDestroy(c->aux);
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
   hipStreamDestroy(c->stream);
   delete c;
 }
@@ -711,7 +707,13 @@ int soap_quiet_star(soap_ctx *c) {
     c->have_lower = c->have_ccf = c->have_fit = false;
     return SOAP_OK;
   }
-  if ((rc = ensure_taps(c, 1, 1, 1))) return rc;
+  {
+    size_t per = (size_t)c->n_mu * c->nd;
+    if ((rc = ensure(c, c->qacc, per * 2 * 8)) || (rc = ensure(c, c->qtaps, per * 8)) ||
+        (rc = ensure(c, c->qkrange, (size_t)c->n_mu * 2 * 4)) || (rc = ensure(c, c->qkflags, (size_t)c->n_mu * 4)) ||
+        (rc = ensure(c, c->qbins, (size_t)c->n_mu * 4)) || (rc = ensure(c, c->qnbins, 4)))
+      return rc;
+  }
   TapAccArgs a{};
   a.grid = c->G;
   a.n_mu = c->n_mu;
@@ -726,14 +728,14 @@ int soap_quiet_star(soap_ctx *c) {
   a.lbm = c->d_lbm;
   a.m = c->d_m;
   a.t = c->d_t;
-  a.acc = (unsigned long long *)c->acc.p;
+  a.acc = (unsigned long long *)c->qacc.p;
   {
     StageTimer tm(c, ST_TAPS_QUIET);
     HIPCHK(c, launch_fill_u64(a.acc, (size_t)c->n_mu * c->nd * 2, c->stream));
     HIPCHK(c, launch_taps_quiet(a, c->stream));
-    HIPCHK(c, launch_taps_finalize(a.acc, 1, 1, 1, c->n_mu, c->nd, (double *)c->taps.p, (int32_t *)c->krange.p,
-                                   (int32_t *)c->kflags.p, (int32_t *)c->bins.p, (int32_t *)c->nbins.p, nullptr, 0,
-                                   c->stream));
+    HIPCHK(c, launch_taps_finalize(a.acc, 1, 1, 1, c->n_mu, c->nd, (double *)c->qtaps.p, (int32_t *)c->qkrange.p,
+                                   (int32_t *)c->qkflags.p, (int32_t *)c->qbins.p, (int32_t *)c->qnbins.p, nullptr,
+                                   0, c->stream));
   }
   FirArgs f{};
   f.q = c->d_cube[SOAP_CUBE_QUIET];
@@ -742,11 +744,11 @@ int soap_quiet_star(soap_ctx *c) {
   f.n_mu = c->n_mu;
   f.dmin = c->dmin;
   f.nd = c->nd;
-  f.taps = (const double *)c->taps.p;
-  f.krange = (const int32_t *)c->krange.p;
-  f.kflags = (const int32_t *)c->kflags.p;
-  f.bins = (const int32_t *)c->bins.p;
-  f.nbins = (const int32_t *)c->nbins.p;
+  f.taps = (const double *)c->qtaps.p;
+  f.krange = (const int32_t *)c->qkrange.p;
+  f.kflags = (const int32_t *)c->qkflags.p;
+  f.bins = (const int32_t *)c->qbins.p;
+  f.nbins = (const int32_t *)c->qnbins.p;
   f.out = c->d_quiet;
   {
     StageTimer tm(c, ST_FIR_QUIET);
@@ -1880,8 +1882,36 @@ int soap_diff_get(soap_ctx *c, int32_t what, double *out) {
 int soap_run_series(soap_ctx *c, const soap_star *star, double sigma_ms, double quiet_weight) {
   int rc;
   if ((rc = soap_disk_init(c, star))) return rc;
-  if ((rc = soap_quiet_star(c))) return rc;
-  if ((rc = soap_integrate(c))) return rc;
+  // the quiet star (its own tap scratch) runs on a second stream beside the active
+  // phases; the resolution waits for it.  With stage timing on, stages run in order.
+  const bool fork = !c->timing && c->kind == SOAP_GRID_LOG;
+  if (fork) {
+    HIPCHK(c, hipSetDevice(c->device));
+    if (!c->aux) {
+      HIPCHK(c, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+      HIPCHK(c, hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming));
+      HIPCHK(c, hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming));
+    }
+    HIPCHK(c, hipEventRecord(c->ev_fork, c->stream));
+    HIPCHK(c, hipStreamWaitEvent(c->aux, c->ev_fork, 0));
+    hipStream_t main_stream = c->stream;
+    c->stream = c->aux;
+    rc = soap_quiet_star(c);
+    c->stream = main_stream;
+    hipError_t e = hipEventRecord(c->ev_join, c->aux);  // recorded even after a failure, so the wait below ends
+    if (rc) {
+      hipStreamWaitEvent(c->stream, c->ev_join, 0);
+      return rc;
+    }
+    HIPCHK(c, e);
+    rc = soap_integrate(c);
+    hipError_t ew = hipStreamWaitEvent(c->stream, c->ev_join, 0);
+    if (rc) return rc;
+    HIPCHK(c, ew);
+  } else {
+    if ((rc = soap_quiet_star(c))) return rc;
+    if ((rc = soap_integrate(c))) return rc;
+  }
   if ((rc = soap_resolution(c, sigma_ms, quiet_weight))) return rc;
   if (c->have_mask) {
     if ((rc = soap_ccf(c))) return rc;
