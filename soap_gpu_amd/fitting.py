@@ -106,49 +106,57 @@ def fit(instances, iters: int = 50, history: int = 10, lr: float = 1.0, log=None
 
 def fit_graph(instances, iters: int = 50, history: int = 10, lr: float = 1.0, vsini_max=None, log=None):
     """As fit(), with each objective evaluation (forward + adjoint of every
-    instance) one replay of a captured HIP graph (DiffBatch): the parameters and
-    maps are written into bound device buffers, the losses and gradients are read
-    from bound device buffers.  vsini_max bounds the fitted vsini of each instance
-    (default 2 x its start + 5 km/s); the loss is NaN beyond it."""
+    instance) one replay of a captured HIP graph (DiffBatch).  The fitted values of
+    all instances live in two flat tensors (vsini and contrast per instance, and
+    the concatenated map logits) whose slices are bound to the graph, so an
+    evaluation is a handful of tensor operations plus one graph launch; the results
+    are written back to each instance's theta and z.  vsini_max bounds the fitted
+    vsini of each instance (default 2 x its start + 5 km/s); the loss is NaN beyond
+    it."""
+    dev = instances[0].q.device
+    n = len(instances)
+    sizes = [inst.z.numel() for inst in instances]
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    theta = torch.stack([inst.theta.detach() for inst in instances]).clone().requires_grad_(True)  # (n, 2)
+    z = torch.cat([inst.z.detach().reshape(-1) for inst in instances]).clone().requires_grad_(True)
+    u = torch.tensor([inst.u for inst in instances], dtype=torch.float64, device=dev)  # (n, 2)
+    prm = torch.empty((n, 4), dtype=torch.float64, device=dev)
+    maps = torch.empty_like(z).detach()
+    loss = torch.empty(n, dtype=torch.float64, device=dev)
+    g4 = torch.empty((n, 4), dtype=torch.float64, device=dev)
+    gm = torch.empty_like(maps)
     batch = DiffBatch([inst.model for inst in instances])
-    bufs = []
+    keep = []
     for i, inst in enumerate(instances):
-        dev = inst.q.device
         vmax = float(vsini_max[i]) if vsini_max is not None else 2.0 * float(inst.theta[0].detach()) + 5.0
-        b = dict(prm=torch.empty(4, dtype=torch.float64, device=dev),
-                 maps=torch.empty_like(inst.z, memory_format=torch.contiguous_format).detach(),
-                 rv_t=torch.as_tensor(inst.rv_t, dtype=torch.float64, device=dev).contiguous(),
+        shape = inst.z.shape
+        t = dict(rv_t=torch.as_tensor(inst.rv_t, dtype=torch.float64, device=dev).contiguous(),
                  fw_t=torch.as_tensor(inst.fw_t, dtype=torch.float64, device=dev).contiguous(),
                  bis_t=torch.as_tensor(inst.bis_t, dtype=torch.float64, device=dev).contiguous(),
-                 loss=torch.empty(1, dtype=torch.float64, device=dev),
-                 g4=torch.empty(4, dtype=torch.float64, device=dev),
-                 gq=torch.empty_like(inst.q), gs=torch.empty_like(inst.s),
-                 gm=torch.empty_like(inst.z).detach())
-        batch.bind(i, vmax, b["prm"], inst.q, inst.s, b["maps"], inst.target, b["rv_t"], b["fw_t"], b["bis_t"],
-                   inst.weights, b["loss"], b["g4"], b["gq"], b["gs"], b["gm"])
-        bufs.append(b)
-    params = [t for inst in instances for t in (inst.theta, inst.z)]
-    opt = torch.optim.LBFGS(params, lr=lr, max_iter=iters, history_size=history, line_search_fn="strong_wolfe",
+                 gq=torch.empty_like(inst.q), gs=torch.empty_like(inst.s))
+        sl = slice(int(offs[i]), int(offs[i + 1]))
+        batch.bind(i, vmax, prm[i], inst.q, inst.s, maps[sl].view(shape), inst.target, t["rv_t"], t["fw_t"],
+                   t["bis_t"], inst.weights, loss[i:i + 1], g4[i], t["gq"], t["gs"], gm[sl].view(shape))
+        keep.append(t)
+    opt = torch.optim.LBFGS([theta, z], lr=lr, max_iter=iters, history_size=history, line_search_fn="strong_wolfe",
                             tolerance_grad=0.0, tolerance_change=0.0)
     evals = [0]
     hist = []
 
     def evaluate():
         with torch.no_grad():
-            for inst, b in zip(instances, bufs):
-                b["prm"].copy_(inst.params())
-                b["maps"].copy_(inst.maps())
+            prm[:, 0] = theta[:, 0]
+            prm[:, 1:3] = u
+            prm[:, 3] = theta[:, 1]
+            m = torch.sigmoid(z)
+            maps.copy_(m)
         torch.cuda.synchronize()
         batch.step(graph=True)
         batch.sync()
         with torch.no_grad():
-            total = torch.zeros((), dtype=torch.float64, device=bufs[0]["loss"].device)
-            for inst, b in zip(instances, bufs):
-                total += b["loss"][0]
-                m = torch.sigmoid(inst.z)
-                inst.theta.grad = torch.stack([b["g4"][0], b["g4"][3]])
-                inst.z.grad = b["gm"] * m * (1 - m)
-        return total
+            theta.grad = g4[:, [0, 3]].clone()
+            z.grad = gm * m * (1 - m)
+            return loss.sum()
 
     def closure():
         opt.zero_grad()
@@ -178,6 +186,10 @@ def fit_graph(instances, iters: int = 50, history: int = 10, lr: float = 1.0, vs
         batch.sync()
         times[adj] = (time.perf_counter() - t1) / 3 * 1e3
     batch.close()
+    with torch.no_grad():
+        for i, inst in enumerate(instances):
+            inst.theta.copy_(theta[i])
+            inst.z.copy_(z[int(offs[i]):int(offs[i + 1])].view(inst.z.shape))
     return dict(loss0=loss0, loss=hist[-1], best=min(h for h in hist if h == h), iters=int(st.get("n_iter", 0)),
                 evals=evals[0], seconds=dt, ms_per_eval=dt / evals[0] * 1e3, graph_ms=times[True],
                 graph_fwd_ms=times[False], history=hist)
