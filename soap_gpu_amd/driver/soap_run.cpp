@@ -3,7 +3,14 @@
 // HIP device through libsoapgpu; the stage files and the .npz products keep
 // the reference's names and keys.
 //
-//   soap_run [config_hpc.cfg] [--no-maps] [--no-stage-files]
+//   soap_run [config_hpc.cfg] [--no-maps] [--no-stage-files] [--size-evolution[=RATE]]
+//
+// --size-evolution: the spot-size evolution of SOAP_code/run_all_SOAP_evo.py
+// (:214-220, spot_evo): with t = (phase - first phase) * prot days, each generated
+// region's size at a phase is sqrt(size^2 - RATE t) (RATE defaults to -0.002, the
+// value hard-coded there: the area grows by 0.002 per day).  The reference applies
+// it to a single region; here it applies to every region of the configuration.
+// Like the reference, it acts on generated maps only (not keyword 'real').
 //
 // Differences from the Python driver, by design: the output directory is not
 // deleted first (files are overwritten), no PNG plot is drawn, and the stage
@@ -11,6 +18,8 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
 #include <chrono>
 #include <iostream>
 
@@ -49,14 +58,22 @@ static std::string mask_template(int teff) {
 
 int main(int argc, char **argv) {
   std::string cfg_name = "config_hpc.cfg";
-  bool write_maps = true, stage_files = true;
+  bool write_maps = true, stage_files = true, evolve = false;
+  double evo_rate = -0.002;
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
     if (a == "--no-maps")
       write_maps = false;
     else if (a == "--no-stage-files")
       stage_files = false;
-    else
+    else if (a == "--size-evolution")
+      evolve = true;
+    else if (a.rfind("--size-evolution=", 0) == 0) {
+      evolve = true;
+      char *end = nullptr;
+      evo_rate = std::strtod(a.c_str() + 17, &end);
+      if (!end || *end) die("--size-evolution=RATE: RATE is not a number");
+    } else
       cfg_name = a;
   }
   auto t_start = std::chrono::system_clock::now();
@@ -134,13 +151,22 @@ int main(int argc, char **argv) {
     for (double p : phases) prow.push_back({p});
     savetxt(ploc, prow);
     for (size_t i = 0; i < phases.size(); i++) {
+      std::vector<soap_region> at = list;
+      if (evolve) {  // run_all_SOAP_evo.py:214-220: size = sqrt(spot_evo(size^2, rate, t))
+        double t = (phases[i] - phases[0]) * prot;
+        for (auto &r : at) {
+          double area = r.size * r.size - evo_rate * t;
+          if (!(area >= 0)) die("size evolution: negative area at phase " + std::to_string(phases[i]));
+          r.size = std::sqrt(area);
+        }
+      }
       std::vector<std::vector<double>> rows;
-      for (auto &r : list) rows.push_back({r.lat_deg, r.lon_deg, r.size, r.type});
+      for (auto &r : at) rows.push_back({r.lat_deg, r.lon_deg, r.size, r.type});
       savetxt(maps_dir + "maps" + std::to_string(i) + "_for_all.txt", rows);
-      info.push_back({(double)i, (double)list.size()});
+      info.push_back({(double)i, (double)at.size()});
       ids.push_back((int32_t)i);
-      counts.push_back((int32_t)list.size());
-      regs.insert(regs.end(), list.begin(), list.end());
+      counts.push_back((int32_t)at.size());
+      regs.insert(regs.end(), at.begin(), at.end());
     }
     savetxt(maps_dir + "final_info.txt", info);
   }

@@ -295,3 +295,32 @@ def test_driver_real_keyword_ragged_regions(tmp_path, tables):
         d = np.load(os.path.join(op, f"test_RV_{name}_GPU.npz"))
         np.testing.assert_allclose(d["RV"], ref["rv"][:, t], rtol=0, atol=1e-2)
         assert d["RV"][1] == pytest.approx(0.0, abs=1e-6)  # no active region: the quiet frame
+
+
+def test_driver_size_evolution(tmp_path, tables):
+    """--size-evolution: the spot-size evolution of run_all_SOAP_evo.py:214-220, size
+    at phase p = sqrt(size^2 + 0.002 (p - p0) prot) for the default rate, written to
+    the per-phase map files and used for the spectra (checked against the oracle
+    run with the evolved regions)."""
+    root = str(tmp_path)
+    w = make_workload(tables, grid=96)
+    write_inputs(root, w, tables)
+    cfg = write_config(root, w)
+    run([tool("soap_run"), cfg, "--no-maps", "--size-evolution"], cwd=root)
+    op = out_prefix(root)
+    prot = float(w.star["prot"])
+    evolved = []
+    for i, ph in enumerate(w.phases):
+        t = (ph - w.phases[0]) * prot
+        regs = [(lat, lon, float(np.sqrt(size * size + 0.002 * t)), ty) for lat, lon, size, ty in REGIONS]
+        evolved.append(regs)
+        got = np.loadtxt(os.path.join(op, "maps", f"maps{i}_for_all.txt")).reshape(-1, 4)
+        np.testing.assert_allclose(got, np.array(regs), rtol=1e-15, atol=0)
+    assert evolved[-1][0][2] > REGIONS[0][2] + 1e-3  # the sizes do change over the run
+    w.regions = evolved
+    ref = PR.run_oracle(w)
+    tn = ("flux", "bconv", "tot")
+    for i in range(len(w.phases)):
+        for t in range(3):
+            raw = load_bin(os.path.join(op, "raw", f"test_Spec_{tn[t]}_{i}_.bin"))
+            assert PR.normwise(raw, ref["raw"][i, t]) <= 1e-12, (i, t)
