@@ -178,6 +178,25 @@ int soap_get_fit(soap_ctx *ctx, double *par, double *model, double *depth, doubl
 int soap_fit_ccf_host(soap_ctx *ctx, const double *v, int32_t n_v, const double *y, int32_t n_ccf,
                       double *par, double *model, double *depth, double *bis);
 
+/* ------------------------------------------------------------ seed preprocessing: bisector injection */
+/* The convective-blueshift injection of CB_injection_faculae.py:191-244 on one
+ * spectrum (host arrays wave[n], flux[n] -> out[n]): each pixel moves by the
+ * velocity v(flux) (Delta_wavelength, :119-123), the moved samples are sorted by
+ * wavelength and resampled linearly (scipy interp1d) onto wave, zero outside the
+ * open range of the moved wavelengths.
+ *   mode 0, BIS_inject_spectral_quiet (:227-244): v = poly(up)(flux) * scale;
+ *   mode 1, BIS_inject_spectral_active (:191-224): v = (poly(up)(flux) + velo_shift) * scale
+ *     if 0.5 < flux < dep_cut, (poly(low)(flux) + velo_shift) * scale if flux <= 0.5,
+ *     (vel_cut + velo_shift) * scale if flux > dep_cut, unmoved otherwise.
+ * poly(c) = numpy.poly1d(c): coefficients highest degree first, 1..16 of them. */
+typedef struct soap_bis_inject_params {
+  int32_t mode, n_up, n_low;
+  double up[16], low[16];
+  double dep_cut, vel_cut, velo_shift, scale;
+} soap_bis_inject_params;
+int soap_bis_inject(soap_ctx *ctx, const soap_bis_inject_params *p, const double *wave, const double *flux, int64_t n,
+                    double *out);
+
 /* ------------------------------------------------------------ whole series */
 /* disk init + quiet star + integrate + resolution + CCF + fit, all on device */
 int soap_run_series(soap_ctx *ctx, const soap_star *star, double sigma_ms, double quiet_weight);

@@ -370,3 +370,48 @@ def diff_forward(wave, grid, mu_bounds, vsini, u1, u2, kappa, cube_q, cube_s, ma
                            cc.ctypes.data_as(_dp), th.ctypes.data_as(_dp), span.ctypes.data_as(_dp),
                            loss.ctypes.data_as(_dp), nthreads or nthreads_default())
     return dict(loss=float(loss[0]), F=F, ccf=cc, theta=th, span=span)
+
+
+# ------------------------------------------------------------------- seed preprocessing
+def _poly1d(c, x):
+    """numpy.poly1d(c)(x): y = y * x + c_i over the coefficients, highest degree first."""
+    y = np.zeros_like(x)
+    for ci in np.atleast_1d(c):
+        y = y * x + ci
+    return y
+
+
+def _delta_wavelength(v, w):
+    """Relativistic Doppler offset of CB_injection_faculae.py:119-123."""
+    beta = v / 299792458.
+    return w * (np.sqrt((1 + beta) / (1 - beta)) - 1)
+
+
+def bis_inject(wave, flux, up, scale, low=None, dep_cut=0.0, vel_cut=0.0, velo_shift=0.0):
+    """Restatement of BIS_inject_spectral_quiet (low is None, CB_injection_faculae.py:227-244)
+    and BIS_inject_spectral_active (:191-224), vectorised: each pixel moves by its own
+    velocity, then scipy's linear interp1d on the moved samples is its documented
+    algorithm for float64 data (stable argsort by x, then numpy.interp), evaluated at
+    the original wavelengths in the open range of the moved ones, zero elsewhere."""
+    w = np.asarray(wave, dtype=np.float64)
+    f = np.asarray(flux, dtype=np.float64)
+    if low is None:
+        rv = _poly1d(up, f) * scale
+        moved = w + _delta_wavelength(rv, w)
+    else:
+        a = (f < dep_cut) & (f > 0.5)  # the if / elif order of the reference
+        b = (f <= 0.5) & ~a
+        c = (f > dep_cut) & ~(a | b)
+        rv = np.zeros_like(f)
+        rv[a] = (_poly1d(up, f[a]) + velo_shift) * scale
+        rv[b] = (_poly1d(low, f[b]) + velo_shift) * scale
+        rv[c] = (vel_cut + velo_shift) * scale
+        moved = w.copy()
+        m = a | b | c
+        moved[m] = w[m] + _delta_wavelength(rv[m], w[m])
+    order = np.argsort(moved, kind="mergesort")
+    xs, ys = moved[order], f[order]
+    out = np.zeros_like(w)
+    inside = (w < moved.max()) & (w > moved.min())
+    out[inside] = np.interp(w[inside], xs, ys)
+    return out

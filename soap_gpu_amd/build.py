@@ -21,7 +21,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-result", "-Wno-unused-variable",
 ]
-SOURCES = [os.path.join(HERE, "csrc", f) for f in ("soap_kernels.hip", "soap_linear.hip", "soap_diff.hip", "soap_engine.hip")]
+SOURCES = [os.path.join(HERE, "csrc", f) for f in ("soap_kernels.hip", "soap_linear.hip", "soap_diff.hip", "soap_inject.hip",
+                                                          "soap_engine.hip")]
 
 
 def _run(cmd):

@@ -112,6 +112,7 @@ struct soap_ctx {
   // tap scratch
   DevBuf acc, taps, krange, kflags, bins, nbins;
   DevBuf taps_g;  // phase-interleaved active taps [n_mu][nd][taps_g_pad(P)][4]
+  DevBuf inj;      // bisector injection: wave, flux, moved, sorted x, sorted y, out [6][n], partials, sort scratch
   // quiet-star tap scratch of its own, so that the quiet star can run beside the
   // active phases (soap_run_series puts it on a second stream)
   DevBuf qacc, qtaps, qkrange, qkflags, qbins, qnbins;
@@ -1636,6 +1637,55 @@ int soap_get_fit(soap_ctx *c, double *par, double *model, double *depth, double 
   if (model) HIPCHK(c, hipMemcpyAsync(model, c->f_model.p, n_ccf * c->n_v * 8, hipMemcpyDeviceToHost, c->stream));
   if (depth) HIPCHK(c, hipMemcpyAsync(depth, c->f_depth.p, n_ccf * 91 * 8, hipMemcpyDeviceToHost, c->stream));
   if (bis) HIPCHK(c, hipMemcpyAsync(bis, c->f_bis.p, n_ccf * 91 * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SOAP_OK;
+}
+
+int soap_bis_inject(soap_ctx *c, const soap_bis_inject_params *p, const double *wave, const double *flux, int64_t n,
+                    double *out) {
+  if (!c || !p || !wave || !flux || !out) return fail(c, SOAP_ERR_ARG, "bis_inject: null argument");
+  if (n < 2 || n > (int64_t)1 << 30) return fail(c, SOAP_ERR_ARG, "bis_inject: n = %lld", (long long)n);
+  if ((p->mode != 0 && p->mode != 1) || p->n_up < 1 || p->n_up > kInjMaxCoef ||
+      (p->mode == 1 && (p->n_low < 1 || p->n_low > kInjMaxCoef)))
+    return fail(c, SOAP_ERR_ARG, "bis_inject: mode %d with %d / %d coefficients", p->mode, p->n_up, p->n_low);
+  HIPCHK(c, hipSetDevice(c->device));
+  InjectArgs a{};
+  a.mode = p->mode;
+  a.n_up = p->n_up;
+  a.n_low = p->mode == 1 ? p->n_low : 0;
+  for (int i = 0; i < kInjMaxCoef; i++) {
+    a.up[i] = p->up[i];
+    a.low[i] = p->low[i];
+  }
+  a.dep_cut = p->dep_cut;
+  a.vel_cut = p->vel_cut;
+  a.velo_shift = p->velo_shift;
+  a.scale = p->scale;
+  size_t sort_bytes = 0;
+  HIPCHK(c, inject_sort_bytes(n, &sort_bytes));
+  const size_t nb = (size_t)inject_blocks(n);
+  const size_t arrays = (size_t)6 * n * 8, parts = (nb * 3 + 4) * 8;
+  int rc;
+  if ((rc = ensure(c, c->inj, arrays + parts + sort_bytes + 256))) return rc;
+  double *d_w = (double *)c->inj.p, *d_f = d_w + n, *d_m = d_f + n, *d_xs = d_m + n, *d_ys = d_xs + n,
+         *d_out = d_ys + n, *d_part = d_out + n, *d_sum = d_part + nb * 3;
+  void *d_tmp = (void *)(((uintptr_t)(d_sum + 4) + 255) & ~(uintptr_t)255);
+  HIPCHK(c, hipMemcpyAsync(d_w, wave, n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_f, flux, n * 8, hipMemcpyHostToDevice, c->stream));
+  hipError_t e = launch_inject_move(a, d_w, d_f, n, d_m, d_part, d_sum, c->stream);
+  if (e == hipErrorInvalidValue) return fail(c, SOAP_ERR_ARG, "bis_inject: bad sizes");
+  HIPCHK(c, e);
+  double summary[3];
+  HIPCHK(c, hipMemcpyAsync(summary, d_sum, sizeof summary, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const double *xs = d_m, *ys = d_f;
+  if (summary[2] != 0.0) {  // some moved pixel passed its neighbour: sort (stable) by moved wavelength
+    HIPCHK(c, launch_inject_sort(d_tmp, sort_bytes, d_m, d_xs, d_f, d_ys, n, c->stream));
+    xs = d_xs;
+    ys = d_ys;
+  }
+  HIPCHK(c, launch_inject_interp(d_w, xs, ys, n, d_sum, d_out, c->stream));
+  HIPCHK(c, hipMemcpyAsync(out, d_out, n * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SOAP_OK;
 }

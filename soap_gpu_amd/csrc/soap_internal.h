@@ -355,4 +355,22 @@ int fir_tile();
 size_t fir_lds(int nd);
 size_t conv_lds_bytes(int ntp);
 size_t taps_lds_bytes(const TapAccArgs &a);
+// ------------------------------------------------------------ bisector injection (seed preprocessing)
+constexpr int kInjMaxCoef = 16;
+struct InjectArgs {
+  int32_t mode;  // 0: BIS_inject_spectral_quiet, 1: BIS_inject_spectral_active
+  int32_t n_up, n_low;
+  double up[kInjMaxCoef], low[kInjMaxCoef];
+  double dep_cut, vel_cut, velo_shift, scale;
+};
+int inject_blocks(int64_t n);
+// moved wavelengths, and summary[3] = (min, max, 1 if they need sorting); part: 3 per block
+hipError_t launch_inject_move(const InjectArgs &a, const double *wave, const double *flux, int64_t n, double *moved,
+                              double *part, double *summary, hipStream_t s);
+hipError_t inject_sort_bytes(int64_t n, size_t *bytes);
+hipError_t launch_inject_sort(void *tmp, size_t tmp_bytes, const double *moved, double *xs, const double *flux,
+                              double *ys, int64_t n, hipStream_t s);
+hipError_t launch_inject_interp(const double *wave, const double *xs, const double *ys, int64_t n,
+                                const double *summary, double *out, hipStream_t s);
+
 }  // namespace soap

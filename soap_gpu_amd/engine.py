@@ -48,7 +48,7 @@ EXPORTS = (
     "soap_set_activity", "soap_integrate", "soap_get_raw", "soap_set_raw", "soap_alloc_phases", "soap_phase_maps",
     "soap_resolution", "soap_get_lower", "soap_get_norm",
     "soap_set_mask", "soap_ccf", "soap_get_ccf",
-    "soap_fit", "soap_get_fit", "soap_fit_ccf_host",
+    "soap_fit", "soap_get_fit", "soap_fit_ccf_host", "soap_bis_inject",
     "soap_run_series",
     "soap_set_timing", "soap_get_stage_ms", "soap_stage_name", "soap_num_stages",
     "soap_get_integrate_stats",
@@ -71,6 +71,12 @@ class SoapError(RuntimeError):
 class DiffParams(ctypes.Structure):
     _fields_ = [("vsini_kms", ctypes.c_double), ("u1", ctypes.c_double), ("u2", ctypes.c_double),
                 ("contrast", ctypes.c_double)]
+
+
+class BisInjectParams(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("n_up", ctypes.c_int32), ("n_low", ctypes.c_int32),
+                ("up", ctypes.c_double * 16), ("low", ctypes.c_double * 16), ("dep_cut", ctypes.c_double),
+                ("vel_cut", ctypes.c_double), ("velo_shift", ctypes.c_double), ("scale", ctypes.c_double)]
 
 
 def load(path: str = LIB_PATH):
@@ -113,6 +119,7 @@ def load(path: str = LIB_PATH):
         "soap_fit": (ctypes.c_int, [vp]),
         "soap_get_fit": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp]),
         "soap_fit_ccf_host": (ctypes.c_int, [vp, _dp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, _dp, _dp, _dp]),
+        "soap_bis_inject": (ctypes.c_int, [vp, ctypes.POINTER(BisInjectParams), _dp, _dp, ctypes.c_int64, _dp]),
         "soap_run_series": (ctypes.c_int, [vp, ctypes.POINTER(Star), ctypes.c_double, ctypes.c_double]),
         "soap_set_timing": (ctypes.c_int, [vp, ctypes.c_int]),
         "soap_get_stage_ms": (ctypes.c_int, [vp, _dp, ctypes.c_int32]),
@@ -349,6 +356,32 @@ class Engine:
         sh = (self.n_phase + 1, 3)
         return dict(par=par.reshape(*sh, 5), model=mod.reshape(*sh, self.n_v), depth=dep.reshape(*sh, 91),
                     bis=bis.reshape(*sh, 91))
+
+    def bis_inject(self, wave, flux, up, scale, low=None, dep_cut=0.0, vel_cut=0.0, velo_shift=0.0):
+        """Bisector injection of CB_injection_faculae.py:191-244 on the GPU: low=None is
+        BIS_inject_spectral_quiet(wave, flux, up, scale), otherwise
+        BIS_inject_spectral_active(wave, flux, up, low, dep_cut, vel_cut, velo_shift, scale).
+        Coefficients in numpy.poly1d order (highest degree first), at most 16."""
+        ww, wp = _d(wave)
+        ff, fp = _d(flux)
+        if ww.shape != ff.shape:
+            raise SoapError("bis_inject: wave and flux differ in length")
+        up = np.atleast_1d(np.asarray(up, dtype=np.float64))
+        lo = np.atleast_1d(np.asarray([] if low is None else low, dtype=np.float64))
+        if not 1 <= up.size <= 16 or lo.size > 16 or (low is not None and lo.size < 1):
+            raise SoapError("bis_inject: 1..16 coefficients per polynomial")
+        prm = BisInjectParams()
+        prm.mode = 0 if low is None else 1
+        prm.n_up, prm.n_low = up.size, lo.size
+        for i, x in enumerate(up):
+            prm.up[i] = float(x)
+        for i, x in enumerate(lo):
+            prm.low[i] = float(x)
+        prm.dep_cut, prm.vel_cut, prm.velo_shift, prm.scale = (float(dep_cut), float(vel_cut), float(velo_shift),
+                                                             float(scale))
+        out, op = _out(ww.size)
+        self._chk(self.L.soap_bis_inject(self.ctx, ctypes.byref(prm), wp, fp, ww.size, op), "bis_inject")
+        return out
 
     def fit_ccf_host(self, v, y):
         vv, vp = _d(v)
