@@ -120,7 +120,8 @@ struct soap_ctx {
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   // resolution
-  double *d_lower = nullptr;  // [(P+1)][3][n]
+  double *d_lower = nullptr;  // [(P+1)][3][n], after spec_norm
+  double *d_lraw = nullptr;   // [(P+1)][3][n], the convolution output before spec_norm
   DevBuf part, mm, ctap, qsel, variant;
   int res_dlo = 0, res_ntp = 0, res_wid = -1;
   bool have_lower = false;
@@ -128,6 +129,9 @@ struct soap_ctx {
   int n_lines = 0, n_v = 0, trim = 0;
   double *d_line_l = nullptr, *d_line_w = nullptr, *d_vgrid = nullptr, *d_appod = nullptr, *d_w_sorted = nullptr;
   DevBuf m_start, m_len, m_val, m_alive, sorted, rank, ccf_part;
+  DevBuf wsum;  // [n_v] live mask weight per velocity (the CCF of a spectrum of ones)
+  DevBuf fp;    // [n_lines] int2 line footprints (first trimmed pixel, length)
+  int ccf_fmax = -1;  // longest footprint; -1: the CCF gathers from global memory
   int mask_scan = 0;
   bool have_mask = false, mask_built = false;
   double *d_ccf = nullptr;  // [(P+1)*3][n_v]
@@ -1282,7 +1286,10 @@ static int log_conv_taps(soap_ctx *c, double sigma_ms, int wid) {
   return SOAP_OK;
 }
 
-int soap_resolution(soap_ctx *c, double sigma_ms, double quiet_weight) {
+// The convolution writes d_lraw and the tile min / max; spec_norm (d_lraw ->
+// d_lower) runs here when norm is set, else in the same launch as the CCF (the
+// series, ccf_impl).
+static int resolution_impl(soap_ctx *c, double sigma_ms, double quiet_weight, bool norm) {
   if (!c) return SOAP_ERR_ARG;
   if (!c->have_quiet) return fail(c, SOAP_ERR_STATE, "resolution: quiet star missing");
   if (c->P > 0 && !c->have_raw) return fail(c, SOAP_ERR_STATE, "resolution: raw spectra missing");
@@ -1306,13 +1313,13 @@ int soap_resolution(soap_ctx *c, double sigma_ms, double quiet_weight) {
     cl.quiet_weight = quiet_weight;
     cl.wave = c->d_wave;
     int n_spec = 3 * (c->P + 1);
-    if ((rc = dalloc(c, &c->d_lower, (size_t)n_spec * c->n)) ||
+    if ((rc = dalloc(c, &c->d_lower, (size_t)n_spec * c->n)) || (rc = dalloc(c, &c->d_lraw, (size_t)n_spec * c->n)) ||
         (rc = ensure(c, c->part, (size_t)n_spec * cl.n_tiles * 2 * 8)) || (rc = ensure(c, c->mm, (size_t)n_spec * 2 * 8)))
       return rc;
     double *part = (double *)c->part.p;
     {
       StageTimer tm(c, ST_CONV);
-      hipError_t e = launch_conv_linear(cl, n_spec, c->d_quiet, c->d_raw, c->d_lower, part, c->stream);
+      hipError_t e = launch_conv_linear(cl, n_spec, c->d_quiet, c->d_raw, c->d_lraw, part, c->stream);
       if (e == hipErrorInvalidValue) return fail(c, SOAP_ERR_LIMIT, "resolution: window %d too wide", wid);
       HIPCHK(c, e);
     }
@@ -1320,11 +1327,11 @@ int soap_resolution(soap_ctx *c, double sigma_ms, double quiet_weight) {
       StageTimer tm(c, ST_MINMAX);
       HIPCHK(c, launch_minmax(n_spec, cl.n_tiles, part, (double *)c->mm.p, c->stream));
     }
-    {
+    if (norm) {
       StageTimer tm(c, ST_NORM);
-      HIPCHK(c, launch_norm(c->n, n_spec, (double *)c->mm.p, c->d_lower, c->stream));
+      HIPCHK(c, launch_norm(c->n, n_spec, (double *)c->mm.p, c->d_lraw, c->d_lower, c->stream));
     }
-    c->have_lower = true;
+    c->have_lower = norm;
     c->have_ccf = c->have_fit = false;
     return SOAP_OK;
   }
@@ -1337,30 +1344,34 @@ int soap_resolution(soap_ctx *c, double sigma_ms, double quiet_weight) {
   ca.ctap = (const double *)c->ctap.p;
   ca.n_tiles = (int)((c->n + conv_tile() - 1) / conv_tile());
   int n_spec = 3 * (c->P + 1);
-  if ((rc = dalloc(c, &c->d_lower, (size_t)n_spec * c->n)) ||
+  if ((rc = dalloc(c, &c->d_lower, (size_t)n_spec * c->n)) || (rc = dalloc(c, &c->d_lraw, (size_t)n_spec * c->n)) ||
       (rc = ensure(c, c->part, (size_t)n_spec * ca.n_tiles * 2 * 8)) || (rc = ensure(c, c->mm, (size_t)n_spec * 2 * 8)))
     return rc;
   double *part = (double *)c->part.p;
   {
     StageTimer tm(c, ST_CONV_QUIET);
-    HIPCHK(c, launch_conv_quiet(ca, c->d_quiet, c->d_lower + (size_t)3 * c->P * c->n,
+    HIPCHK(c, launch_conv_quiet(ca, c->d_quiet, c->d_lraw + (size_t)3 * c->P * c->n,
                                 part + (size_t)3 * c->P * ca.n_tiles * 2, c->stream));
   }
   {
     StageTimer tm(c, ST_CONV);
-    HIPCHK(c, launch_conv_active(ca, c->d_quiet, c->d_raw, c->P, c->d_lower, part, c->stream));
+    HIPCHK(c, launch_conv_active(ca, c->d_quiet, c->d_raw, c->P, c->d_lraw, part, c->stream));
   }
   {
     StageTimer tm(c, ST_MINMAX);
     HIPCHK(c, launch_minmax(n_spec, ca.n_tiles, part, (double *)c->mm.p, c->stream));
   }
-  {
+  if (norm) {
     StageTimer tm(c, ST_NORM);
-    HIPCHK(c, launch_norm(c->n, n_spec, (double *)c->mm.p, c->d_lower, c->stream));
+    HIPCHK(c, launch_norm(c->n, n_spec, (double *)c->mm.p, c->d_lraw, c->d_lower, c->stream));
   }
-  c->have_lower = true;
+  c->have_lower = norm;
   c->have_ccf = c->have_fit = false;
   return SOAP_OK;
+}
+
+int soap_resolution(soap_ctx *c, double sigma_ms, double quiet_weight) {
+  return resolution_impl(c, sigma_ms, quiet_weight, true);
 }
 
 int soap_get_lower(soap_ctx *c, int32_t phase, int type, double *out) {
@@ -1469,6 +1480,20 @@ static int build_mask(soap_ctx *c) {
   HIPCHK(c, launch_mask_overwrite((const int32_t *)c->sorted.p, c->n_lines, c->n_v, c->mask_scan,
                                   (const int32_t *)c->m_start.p, (const int32_t *)c->m_len.p, (uint32_t *)c->m_alive.p,
                                   c->stream));
+  int rc;
+  if ((rc = ensure(c, c->wsum, (size_t)c->n_v * 8))) return rc;
+  HIPCHK(c, launch_mask_wsum(c->n_lines, c->n_v, n_t, (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p,
+                             (const double2 *)c->m_val.p, c->d_w_sorted, (double *)c->wsum.p, c->stream));
+  // line footprints: the CCF stages them in LDS when the longest fits
+  if ((rc = ensure(c, c->fp, (size_t)c->n_lines * 8))) return rc;
+  HIPCHK(c, launch_mask_footprint(c->n_lines, c->n_v, n_t, (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p,
+                                  (int2 *)c->fp.p, c->stream));
+  std::vector<int32_t> fp((size_t)c->n_lines * 2);
+  HIPCHK(c, hipMemcpyAsync(fp.data(), c->fp.p, fp.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int fmax = 0;
+  for (int q = 0; q < c->n_lines; q++) fmax = std::max(fmax, fp[2 * q + 1]);
+  c->ccf_fmax = ccf_lds_bytes(fmax) <= 65536 ? fmax : -1;
   c->mask_built = true;
   c->diff.mt_ready = false;
   return SOAP_OK;
@@ -1567,10 +1592,14 @@ static int diff_mask_transpose(soap_ctx *c) {
 }
 
 
-int soap_ccf(soap_ctx *c) {
+// The CCF runs on the unnormalised spectra d_lraw (the reduction applies the
+// spec_norm map, see k_ccf_reduce).  With norm set, spec_norm itself (d_lraw ->
+// d_lower) runs in the same launch (the series: resolution_impl without norm
+// just before).
+static int ccf_impl(soap_ctx *c, bool norm) {
   if (!c) return SOAP_ERR_ARG;
   if (!c->have_mask) return fail(c, SOAP_ERR_STATE, "ccf: set the mask first");
-  if (!c->have_lower) return fail(c, SOAP_ERR_STATE, "ccf: run resolution first");
+  if (!norm && !c->have_lower) return fail(c, SOAP_ERR_STATE, "ccf: run resolution first");
   HIPCHK(c, hipSetDevice(c->device));
   int rc;
   if (!c->mask_built && (rc = build_mask(c))) return rc;
@@ -1578,21 +1607,49 @@ int soap_ccf(soap_ctx *c) {
   if ((rc = dalloc(c, &c->d_ccf, (size_t)n_spec * c->n_v)) ||
       (rc = ensure(c, c->ccf_part, (size_t)ccf_chunks(c->n_lines) * n_spec * c->n_v * 8)))
     return rc;
-  {
+  if (norm) {
+    NormCcfArgs a{};
+    a.n = c->n;
+    a.trim = c->trim;
+    a.n_t = c->n - 2 * (int64_t)c->trim;
+    a.n_spec = n_spec;
+    a.n_lines = c->n_lines;
+    a.n_v = c->n_v;
+    a.mm = (const double *)c->mm.p;
+    a.raw = c->d_lraw;
+    a.out = c->d_lower;
+    a.m_start = (const int32_t *)c->m_start.p;
+    a.m_pk = (const uint32_t *)c->m_alive.p;
+    a.m_wv = (const double2 *)c->m_val.p;
+    a.w_sorted = c->d_w_sorted;
+    a.fp = c->ccf_fmax >= 0 ? (const int2 *)c->fp.p : nullptr;
+    a.fmax = c->ccf_fmax;
+    a.part = (double *)c->ccf_part.p;
     StageTimer tm(c, ST_CCF);
-    HIPCHK(c, launch_ccf(c->d_lower, c->n, c->trim, n_spec, c->n_lines, c->n_v, c->n - 2 * (int64_t)c->trim,
+    HIPCHK(c, launch_norm_ccf(a, c->stream));
+    c->have_lower = true;
+  } else if (c->ccf_fmax >= 0) {
+    StageTimer tm(c, ST_CCF);
+    HIPCHK(c, launch_ccf_lds(c->d_lraw, c->n, c->trim, n_spec, c->n_lines, c->n_v, c->n - 2 * (int64_t)c->trim,
+                             (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p,
+                             c->d_w_sorted, (const int2 *)c->fp.p, c->ccf_fmax, (double *)c->ccf_part.p, c->stream));
+  } else {
+    StageTimer tm(c, ST_CCF);
+    HIPCHK(c, launch_ccf(c->d_lraw, c->n, c->trim, n_spec, c->n_lines, c->n_v, c->n - 2 * (int64_t)c->trim,
                          (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p,
                          c->d_w_sorted, (double *)c->ccf_part.p, c->stream));
   }
   {
     StageTimer tm(c, ST_CCF_REDUCE);
     HIPCHK(c, launch_ccf_reduce((const double *)c->ccf_part.p, ccf_chunks(c->n_lines), n_spec, c->n_v, c->d_appod,
-                                c->d_ccf, c->stream));
+                                (const double *)c->mm.p, (const double *)c->wsum.p, c->d_ccf, c->stream));
   }
   c->have_ccf = true;
   c->have_fit = false;
   return SOAP_OK;
 }
+
+int soap_ccf(soap_ctx *c) { return ccf_impl(c, false); }
 
 int soap_get_ccf(soap_ctx *c, double *out) {
   if (!c || !out) return SOAP_ERR_ARG;
@@ -1962,9 +2019,11 @@ int soap_run_series(soap_ctx *c, const soap_star *star, double sigma_ms, double 
     if ((rc = soap_quiet_star(c))) return rc;
     if ((rc = soap_integrate(c))) return rc;
   }
-  if ((rc = soap_resolution(c, sigma_ms, quiet_weight))) return rc;
+  // spec_norm shares its launch with the CCF unless stages are timed one by one
+  const bool fuse = !c->timing && c->have_mask;
+  if ((rc = resolution_impl(c, sigma_ms, quiet_weight, !fuse))) return rc;
   if (c->have_mask) {
-    if ((rc = soap_ccf(c))) return rc;
+    if ((rc = ccf_impl(c, fuse))) return rc;
     if ((rc = soap_fit(c))) return rc;
   }
   return SOAP_OK;
@@ -2056,7 +2115,7 @@ int batch_upload(soap_batch *b, int stage, const std::vector<Job> &jobs, const s
 // be stepped through stale pointers.
 std::vector<uintptr_t> batch_signature(const soap_ctx *c) {
   auto u = [](const void *p) { return (uintptr_t)p; };
-  return {u(c->d_raw),    u(c->d_lower), u(c->d_quiet),   u(c->acc.p),    u(c->taps.p),   u(c->taps_g.p), u(c->krange.p),
+  return {u(c->d_raw),    u(c->d_lower), u(c->d_lraw),   u(c->wsum.p),   u(c->d_quiet),   u(c->acc.p),    u(c->taps.p),   u(c->taps_g.p), u(c->krange.p),
           u(c->kflags.p), u(c->bins.p),  u(c->nbins.p),   u(c->part.p),   u(c->mm.p),     u(c->ctap.p),
           u(c->d_ccf),    u(c->ccf_part.p), u(c->f_par.p), u(c->f_model.p), u(c->f_depth.p), u(c->f_bis.p),
           u(c->f_iter.p), u(c->m_start.p), u(c->m_alive.p), u(c->m_val.p), u(c->d_geo),   u(c->d_box),
@@ -2156,19 +2215,19 @@ int batch_build(soap_batch *b) {
     cv.ctap = (const double *)c->ctap.p;
     cv.n_tiles = (int)((c->n + conv_tile() - 1) / conv_tile());
     double *part = (double *)c->part.p;
-    cq[i] = ConvQJob{cv, c->d_quiet, c->d_lower + (size_t)3 * P * c->n, part + (size_t)3 * P * cv.n_tiles * 2};
+    cq[i] = ConvQJob{cv, c->d_quiet, c->d_lraw + (size_t)3 * P * c->n, part + (size_t)3 * P * cv.n_tiles * 2};
     w[BS_CONV_QUIET][i] = 2L * cv.n_tiles;
-    ca[i] = ConvAJob{cv, c->d_quiet, c->d_raw, c->d_lower, part, P};
+    ca[i] = ConvAJob{cv, c->d_quiet, c->d_raw, c->d_lraw, part, P};
     w[BS_CONV][i] = (int64_t)(cv.n_tiles + 7) / 8 * 8 * P * 3;  // one block per (tile, phase, type)
     lds_conv = std::max(lds_conv, conv_lds_bytes(cv.ntp));
     mm[i] = MinmaxJob{part, cv.n_tiles, (double *)c->mm.p};
     w[BS_MINMAX][i] = n_spec;
     int64_t nbx = (c->n + norm_px_per_block() - 1) / norm_px_per_block();
-    nm[i] = NormJob{c->n, nbx, (const double *)c->mm.p, c->d_lower};
+    nm[i] = NormJob{c->n, nbx, (const double *)c->mm.p, c->d_lraw, c->d_lower};
     w[BS_NORM][i] = nbx * n_spec;
     // CCF + fit
     int nc = ccf_chunks(c->n_lines);
-    cc[i] = CcfJob{c->d_lower, c->n, c->trim, c->n - 2 * (int64_t)c->trim, n_spec, c->n_lines, c->n_v, nc,
+    cc[i] = CcfJob{c->d_lraw, c->n, c->trim, c->n - 2 * (int64_t)c->trim, n_spec, c->n_lines, c->n_v, nc,
                    (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p,
                    c->d_w_sorted, (double *)c->ccf_part.p};
     w[BS_CCF][i] = (int64_t)nc * ((n_spec + ccf_spec_chunk() - 1) / ccf_spec_chunk());
@@ -2176,7 +2235,8 @@ int batch_build(soap_batch *b) {
     int rt = 64;
     while (rt < c->n_v) rt <<= 1;
     red_threads = std::max(red_threads, rt);
-    cr[i] = CcfRedJob{(const double *)c->ccf_part.p, nc, n_spec, c->n_v, c->d_appod, c->d_ccf};
+    cr[i] = CcfRedJob{(const double *)c->ccf_part.p, nc,        n_spec, c->n_v, c->d_appod, (const double *)c->mm.p,
+                      (const double *)c->wsum.p,    c->d_ccf};
     w[BS_CCF_REDUCE][i] = n_spec;
     ft[i] = FitJob{c->d_vgrid, c->n_v, c->d_ccf, (double *)c->f_par.p, (double *)c->f_model.p,
                    (double *)c->f_depth.p, (double *)c->f_bis.p, (int32_t *)c->f_iter.p};

@@ -209,7 +209,7 @@ struct FirJob { FirArgs a; int32_t n_job, n_tiles, n_groups; };      // work: ti
 struct ConvQJob { ConvArgs c; const double *quiet; double *frame, *part; };  // work: 2 n_tiles
 struct ConvAJob { ConvArgs c; const double *quiet, *raw; double *out, *part; int32_t n_phase; };  // tiles8 * P
 struct MinmaxJob { const double *part; int32_t n_tiles; double *mm; };    // work: n_spec
-struct NormJob { int64_t n, bx; const double *mm; double *o; };          // work: bx * n_spec
+struct NormJob { int64_t n, bx; const double *mm, *in; double *o; };     // work: bx * n_spec
 struct CcfJob {                                                          // work: nc * ceil(n_spec / ccf_spec_chunk())
   const double *spec;
   int64_t stride, trim, n_t;
@@ -220,7 +220,12 @@ struct CcfJob {                                                          // work
   const double *w_sorted;
   double *part;
 };
-struct CcfRedJob { const double *part; int32_t nc, n_spec, n_v; const double *appod; double *ccf; };  // n_spec
+struct CcfRedJob {  // work: n_spec
+  const double *part;
+  int32_t nc, n_spec, n_v;
+  const double *appod, *mm, *wsum;
+  double *ccf;
+};
 struct FitJob {                                                          // work: n_ccf
   const double *v;
   int32_t n_v;
@@ -268,7 +273,34 @@ hipError_t launch_conv_active(const ConvArgs &c, const double *quiet, const doub
                               double *part, hipStream_t s);
 // spectra sp = 0..n_spec-1 stored contiguously at o + sp*n, partials part[sp][n_tiles][2]
 hipError_t launch_minmax(int n_spec, int n_tiles, const double *part, double *mm, hipStream_t s);
-hipError_t launch_norm(int64_t n, int n_spec, const double *mm, double *o, hipStream_t s);
+hipError_t launch_norm(int64_t n, int n_spec, const double *mm, const double *in, double *o, hipStream_t s);
+// spec_norm (raw -> out) and the CCF partials of raw in one launch; the launcher
+// fills nc, n_ccf, nbx, n_norm and ratio
+struct NormCcfArgs {
+  int64_t n, trim, n_t;
+  int32_t n_spec, n_lines, n_v;
+  const double *mm, *raw;
+  double *out;
+  const int32_t *m_start;
+  const uint32_t *m_pk;
+  const double2 *m_wv;
+  const double *w_sorted;
+  const int2 *fp;  // line footprints: the CCF stages them in LDS (null: the gather form)
+  int32_t fmax;    // longest footprint
+  double *part;
+  int32_t nc, n_ccf, nbx, n_norm, ratio;
+};
+hipError_t launch_norm_ccf(NormCcfArgs a, hipStream_t s);
+// the CCF with line footprints staged in LDS (fp from launch_mask_footprint;
+// needs ccf_lds_bytes(fmax) <= 64 KiB)
+hipError_t launch_mask_footprint(int n_lines, int n_v, int64_t n_t, const int32_t *m_start, const uint32_t *m_pk,
+                                 int2 *fp, hipStream_t s);
+size_t ccf_lds_bytes(int fmax);
+hipError_t launch_ccf_lds(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
+                          int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
+                          const double *w_sorted, const int2 *fp, int fmax, double *part, hipStream_t s);
+hipError_t launch_mask_wsum(int n_lines, int n_v, int64_t n_t, const int32_t *m_start, const uint32_t *m_pk,
+                            const double2 *m_wv, const double *w_sorted, double *wsum, hipStream_t s);
 // CCF
 // mask tables [q][v], q = position in wavelength order (sorted[q] = line index)
 hipError_t launch_mask_build(const double *line_l, const double *line_w, const int32_t *sorted, int n_lines,
@@ -282,8 +314,8 @@ int ccf_chunks(int n_lines);
 hipError_t launch_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
                       int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
                       const double *w_sorted, double *part, hipStream_t s);
-hipError_t launch_ccf_reduce(const double *part, int nc, int n_spec, int n_v, const double *appod, double *ccf,
-                             hipStream_t s);
+hipError_t launch_ccf_reduce(const double *part, int nc, int n_spec, int n_v, const double *appod, const double *mm,
+                             const double *wsum, double *ccf, hipStream_t s);
 hipError_t launch_fit(const double *v, int n_v, const double *y, int n_ccf, double *par, double *model,
                       double *depth, double *bis, int32_t *iters, hipStream_t s);
 // linear grids

@@ -1334,21 +1334,23 @@ __global__ __launch_bounds__(kTPB) void k_minmax_reduce(const double *part, int 
   minmax_body(part, n_tiles, mm, blockIdx.x, blockIdx.y);
 }
 
-// spec_norm, SOAP_resolution.cu:225-232: out = 1 - o (1 - min) / max, in place.
-// Spectrum sp = blockIdx.y at o + sp*n; a block covers kNormPer*kTPB pixels as
-// 16-byte (two-pixel) loads, all issued before the stores.  Spectra start at
-// multiples of n pixels, so an odd n leaves odd-aligned spectra: those take the
-// one-pixel path.
+// spec_norm, SOAP_resolution.cu:225-232: out = 1 - in (1 - min) / max, from the
+// unnormalised spectra into the output buffer.  Spectrum sp = by_ at in + sp*n and
+// out + sp*n; a block of nt threads covers kNormPer*nt pixels as 16-byte
+// (two-pixel) loads, all issued before the stores.  Spectra start at multiples of
+// n pixels, so an odd n leaves odd-aligned spectra: those take the one-pixel path
+// for their first pixel (in and out have the same 16-byte alignment).
 constexpr int kNormPer = 8;
 typedef double dvec2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ void norm_body(long n, const double *mm_, double *o_, int bx_, int by_) {
+__device__ __forceinline__ void norm_body(long n, const double *mm_, const double *in_, double *o_, int bx_, int by_,
+                                          int nt) {
   const GLOBAL double *mm = gl(mm_);
-  GLOBAL double *o = gw(o_);
   const long sp = by_;
   const double mn = mm[sp * 2], mx = mm[sp * 2 + 1];
-  GLOBAL double *x = o + sp * n;
-  const long blk = (long)bx_ * (kTPB * kNormPer);
-  const long end = min(n, blk + (long)kTPB * kNormPer);
+  const GLOBAL double *x = gl(in_) + sp * n;
+  GLOBAL double *y = gw(o_) + sp * n;
+  const long blk = (long)bx_ * (nt * kNormPer);
+  const long end = min(n, blk + (long)nt * kNormPer);
   // pairs start at the first 16-byte boundary of the block; a block that starts
   // off the boundary (odd n, odd spectrum) does its first pixel alone
   const int s0 = ((uintptr_t)(x + blk) & 15) ? 1 : 0;
@@ -1356,7 +1358,7 @@ __device__ __forceinline__ void norm_body(long n, const double *mm_, double *o_,
   dvec2 v[V];
 #pragma unroll
   for (int k = 0; k < V; k++) {
-    long i = blk + s0 + 2L * (threadIdx.x + (long)k * kTPB);
+    long i = blk + s0 + 2L * (threadIdx.x + (long)k * nt);
     if (i + 1 < end) {
       v[k] = *(const GLOBAL dvec2 *)(x + i);
     } else {
@@ -1367,20 +1369,20 @@ __device__ __forceinline__ void norm_body(long n, const double *mm_, double *o_,
   double head = (s0 && threadIdx.x == 0 && blk < n) ? x[blk] : 0.0;
 #pragma unroll
   for (int k = 0; k < V; k++) {
-    long i = blk + s0 + 2L * (threadIdx.x + (long)k * kTPB);
+    long i = blk + s0 + 2L * (threadIdx.x + (long)k * nt);
     dvec2 r;
     r.x = 1.0 - v[k].x * (1 - mn) / mx;
     r.y = 1.0 - v[k].y * (1 - mn) / mx;
     if (i + 1 < end)
-      *(GLOBAL dvec2 *)(x + i) = r;
+      *(GLOBAL dvec2 *)(y + i) = r;
     else if (i < end)
-      x[i] = r.x;
+      y[i] = r.x;
   }
-  if (s0 && threadIdx.x == 0 && blk < n) x[blk] = 1.0 - head * (1 - mn) / mx;
+  if (s0 && threadIdx.x == 0 && blk < n) y[blk] = 1.0 - head * (1 - mn) / mx;
 }
 
-__global__ __launch_bounds__(kTPB) void k_norm(long n, const double *mm, double *o) {
-  norm_body(n, mm, o, blockIdx.x, blockIdx.y);
+__global__ __launch_bounds__(kTPB) void k_norm(long n, const double *mm, const double *in, double *o) {
+  norm_body(n, mm, in, o, blockIdx.x, blockIdx.y, kTPB);
 }
 
 hipError_t launch_minmax(int n_spec, int n_tiles, const double *part, double *mm, hipStream_t s) {
@@ -1389,11 +1391,12 @@ hipError_t launch_minmax(int n_spec, int n_tiles, const double *part, double *mm
   return hipGetLastError();
 }
 
-hipError_t launch_norm(int64_t n, int n_spec, const double *mm, double *o, hipStream_t s) {
+hipError_t launch_norm(int64_t n, int n_spec, const double *mm, const double *in, double *o, hipStream_t s) {
   if (n_spec == 0) return hipSuccess;
+  if (((uintptr_t)in ^ (uintptr_t)o) & 15) return hipErrorInvalidValue;
   long bx = (n + (long)kTPB * kNormPer - 1) / ((long)kTPB * kNormPer);
   dim3 grid((unsigned)bx, n_spec);
-  k_norm<<<grid, kTPB, 0, s>>>(n, mm, o);
+  k_norm<<<grid, kTPB, 0, s>>>(n, mm, in, o);
   return hipGetLastError();
 }
 
@@ -1479,6 +1482,13 @@ hipError_t launch_mask_overwrite(const int32_t *sorted, int n_lines, int n_v, in
 // entry is read once per block).  Partials per line chunk are summed in order
 // by k_ccf_reduce, which also normalises by the maximum and appodises
 // (run_all_SOAP.py:368-370).
+//
+// The series takes the CCF of the spec_norm output y = 1 - x (1 - min) / max.
+// That map is affine per spectrum, so the CCF of y is W(v) - C_x(v) (1 - min) /
+// max with C_x the CCF of the unnormalised spectrum x and W(v) the sum of the
+// live mask weights at velocity v (the CCF of a spectrum of ones, k_mask_wsum).
+// k_ccf therefore runs on x, and spec_norm does not have to finish before it: the
+// two share one launch (k_norm_ccf).
 constexpr int kCcfLines = 32;
 constexpr int kCcfSpec = 8;
 
@@ -1521,22 +1531,149 @@ __device__ __forceinline__ void ccf_body(const double *spec_, int64_t spec_strid
   }
 }
 
+// Line footprints for the LDS form below: fp[q] = (lo, len), the trimmed pixels
+// [lo, lo + len) that the window of line q touches at any velocity, clipped to
+// [0, n_t) (len 0: none).
+__global__ __launch_bounds__(kTPB) void k_mask_footprint(int n_lines, int n_v, int64_t n_t, const int32_t *m_start,
+                                                         const uint32_t *m_pk, int2 *fp) {
+  const int q = blockIdx.x * kTPB + threadIdx.x;
+  if (q >= n_lines) return;
+  long lo = n_t, hi = -1;
+  for (int iv = 0; iv < n_v; iv++) {
+    const long e = (long)q * n_v + iv;
+    const int ln = (int)(m_pk[e] & 0xffu);
+    if (ln == 0) continue;
+    const long st = m_start[e];
+    const long a = st > 0 ? st : 0, b = st + ln - 1 < n_t - 1 ? st + ln - 1 : n_t - 1;
+    if (a > b) continue;
+    lo = a < lo ? a : lo;
+    hi = b > hi ? b : hi;
+  }
+  fp[q] = hi >= lo ? make_int2((int)lo, (int)(hi - lo + 1)) : make_int2(0, 0);
+}
+
+hipError_t launch_mask_footprint(int n_lines, int n_v, int64_t n_t, const int32_t *m_start, const uint32_t *m_pk,
+                                 int2 *fp, hipStream_t s) {
+  if (n_lines == 0) return hipSuccess;
+  k_mask_footprint<<<(n_lines + kTPB - 1) / kTPB, kTPB, 0, s>>>(n_lines, n_v, n_t, m_start, m_pk, fp);
+  return hipGetLastError();
+}
+
+// ccf_body with each line's footprint staged in LDS: per line, the block copies
+// the footprint pixels of its spectra with coalesced loads (8 in flight per
+// thread), then every thread (one velocity; blockDim.x >= n_v) reads its window
+// from LDS.  The sums are those of ccf_body, term for term in the same order.
+// lds: kCcfSpec * fmax doubles, fmax >= every footprint length.
+__device__ __forceinline__ void ccf_lds_body(const double *spec_, int64_t spec_stride, int64_t trim, int n_spec,
+                                             int n_lines, int n_v, int64_t n_t, const int32_t *m_start_,
+                                             const uint32_t *m_pk_, const double2 *m_wv_, const double *w_sorted_,
+                                             const int2 *fp_, int fmax, double *part_, int bx_, int by_,
+                                             double *lds) {
+  const GLOBAL double *spec = gl(spec_), *w_sorted = gl(w_sorted_);
+  const GLOBAL int32_t *m_start = gl(m_start_);
+  const GLOBAL uint32_t *m_pk = gl(m_pk_);
+  const GLOBAL double *m_wv = gl((const double *)m_wv_);
+  const GLOBAL int32_t *fp = gl((const int32_t *)fp_);  // {lo, len} pairs
+  GLOBAL double *part = gw(part_);
+  const int chunk = bx_;
+  const int s0 = by_ * kCcfSpec;
+  const int ns = min(kCcfSpec, n_spec - s0);
+  const int q0 = chunk * kCcfLines, q1 = min(n_lines, q0 + kCcfLines);
+  const GLOBAL double *S[kCcfSpec];
+#pragma unroll
+  for (int k = 0; k < kCcfSpec; k++) S[k] = spec + (long)(s0 + (k < ns ? k : 0)) * spec_stride + trim;
+  const int iv = threadIdx.x;
+  const bool act = iv < n_v;
+  double acc[kCcfSpec];
+#pragma unroll
+  for (int k = 0; k < kCcfSpec; k++) acc[k] = 0.0;
+  for (int q = q0; q < q1; q++) {
+    // this line's mask entry, loaded before the staging so that the two overlap
+    int st = 0;
+    uint32_t pk = 0;
+    double wx = 0.0, wy = 0.0;
+    if (act) {
+      const long e = (long)q * n_v + iv;
+      st = m_start[e];
+      pk = m_pk[e];
+      wx = m_wv[2 * e];
+      wy = m_wv[2 * e + 1];
+    }
+    const double w = w_sorted[q];
+    const int f_lo = fp[2 * q], f_len = fp[2 * q + 1];
+    __syncthreads();  // the previous line's readers are done with the buffer
+    for (int j0 = 0; j0 < f_len; j0 += blockDim.x) {
+      const int j = j0 + threadIdx.x;
+      double v[kCcfSpec];
+#pragma unroll
+      for (int k = 0; k < kCcfSpec; k++) v[k] = (j < f_len && k < ns) ? S[k][f_lo + j] : 0.0;
+      if (j < f_len) {
+#pragma unroll
+        for (int k = 0; k < kCcfSpec; k++) lds[k * fmax + j] = v[k];
+      }
+    }
+    __syncthreads();
+    if (act) {
+      const int ln = (int)(pk & 0xffu);
+      const uint32_t al = pk >> 8;
+      for (int b = 0; b < ln; b++) {
+        const long pp = st + b;
+        if (!((al >> b) & 1u) || pp < 0 || pp >= n_t) continue;
+        const double val = (b == ln - 1) ? wy : ((b == 0) ? wx : w);
+        const double *L = lds + (pp - f_lo);
+#pragma unroll
+        for (int k = 0; k < kCcfSpec; k++) acc[k] = fma(L[k * fmax], val, acc[k]);
+      }
+    }
+  }
+  if (act)
+    for (int k = 0; k < ns; k++) part[((long)chunk * n_spec + s0 + k) * n_v + iv] = acc[k];
+}
+
+__global__ void k_ccf_lds(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
+                          int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
+                          const double *w_sorted, const int2 *fp, int fmax, double *part) {
+  extern __shared__ double ccf_lds[];
+  ccf_lds_body(spec, spec_stride, trim, n_spec, n_lines, n_v, n_t, m_start, m_pk, m_wv, w_sorted, fp, fmax, part,
+               blockIdx.x, blockIdx.y, ccf_lds);
+}
+
+size_t ccf_lds_bytes(int fmax) { return (size_t)kCcfSpec * (fmax > 0 ? fmax : 1) * 8; }
+
+hipError_t launch_ccf_lds(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
+                          int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
+                          const double *w_sorted, const int2 *fp, int fmax, double *part, hipStream_t s) {
+  if (n_spec == 0) return hipSuccess;
+  if (n_v > 1024 || fmax < 0 || ccf_lds_bytes(fmax) > 65536) return hipErrorInvalidValue;
+  int nc = ccf_chunks(n_lines);
+  int threads = ((n_v + 63) / 64) * 64;
+  dim3 grid(nc, (n_spec + kCcfSpec - 1) / kCcfSpec);
+  k_ccf_lds<<<grid, threads, ccf_lds_bytes(fmax), s>>>(spec, spec_stride, trim, n_spec, n_lines, n_v, n_t, m_start,
+                                                        m_pk, m_wv, w_sorted, fp, fmax, part);
+  return hipGetLastError();
+}
+
 __global__ void k_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
                       int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
                       const double *w_sorted, double *part) {
   ccf_body(spec, spec_stride, trim, n_spec, n_lines, n_v, n_t, m_start, m_pk, m_wv, w_sorted, part, blockIdx.x, blockIdx.y);
 }
 
-__device__ __forceinline__ void ccf_reduce_body(const double *part_, int n_chunks, int n_spec, int n_v, const double *appod_, double *ccf_, int bx_, int by_) {
-  const GLOBAL double *part = gl(part_), *appod = gl(appod_);
+// ccf = W - (sum of the chunk partials of C_x) (1 - min) / max, then the
+// normalisation by the maximum and the appodisation
+__device__ __forceinline__ void ccf_reduce_body(const double *part_, int n_chunks, int n_spec, int n_v,
+                                                const double *appod_, const double *mm_, const double *wsum_,
+                                                double *ccf_, int bx_, int by_) {
+  const GLOBAL double *part = gl(part_), *appod = gl(appod_), *mm = gl(mm_), *wsum = gl(wsum_);
   GLOBAL double *ccf = gw(ccf_);
   __shared__ double s_red[1024];
   int sp = bx_;
+  const double s_min = mm[sp * 2], s_max = mm[sp * 2 + 1];
   double v = -HUGE_VAL;
   for (int iv = threadIdx.x; iv < n_v; iv += blockDim.x) {
     double a = 0.0;
     for (int c = 0; c < n_chunks; c++) a += part[((long)c * n_spec + sp) * n_v + iv];
-    ccf[(long)sp * n_v + iv] = a;
+    ccf[(long)sp * n_v + iv] = wsum[iv] - a * (1 - s_min) / s_max;
   }
   __syncthreads();
   // max over velocities (np.max), then normalise and appodise
@@ -1554,8 +1691,94 @@ __device__ __forceinline__ void ccf_reduce_body(const double *part_, int n_chunk
   }
 }
 
-__global__ void k_ccf_reduce(const double *part, int n_chunks, int n_spec, int n_v, const double *appod, double *ccf) {
-  ccf_reduce_body(part, n_chunks, n_spec, n_v, appod, ccf, blockIdx.x, blockIdx.y);
+__global__ void k_ccf_reduce(const double *part, int n_chunks, int n_spec, int n_v, const double *appod,
+                             const double *mm, const double *wsum, double *ccf) {
+  ccf_reduce_body(part, n_chunks, n_spec, n_v, appod, mm, wsum, ccf, blockIdx.x, blockIdx.y);
+}
+
+// W(v): per velocity, the live mask weights k_ccf multiplies in-range pixels
+// with, summed over the lines (one block per velocity, lines strided over the
+// threads, then a fixed-order tree)
+__global__ __launch_bounds__(kTPB) void k_mask_wsum(int n_lines, int n_v, int64_t n_t, const int32_t *m_start,
+                                                    const uint32_t *m_pk, const double2 *m_wv, const double *w_sorted,
+                                                    double *wsum) {
+  __shared__ double s_red[kTPB];
+  const int iv = blockIdx.x;
+  const double *wv = (const double *)m_wv;
+  double acc = 0.0;
+  for (int q = threadIdx.x; q < n_lines; q += kTPB) {
+    long e = (long)q * n_v + iv;
+    int st = m_start[e];
+    uint32_t pk = m_pk[e];
+    int ln = (int)(pk & 0xffu);
+    uint32_t al = pk >> 8;
+    for (int b = 0; b < ln; b++) {
+      long pp = st + b;
+      if (!((al >> b) & 1u) || pp < 0 || pp >= n_t) continue;
+      acc += (b == ln - 1) ? wv[2 * e + 1] : ((b == 0) ? wv[2 * e] : w_sorted[q]);
+    }
+  }
+  s_red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kTPB / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) s_red[threadIdx.x] += s_red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) wsum[iv] = s_red[0];
+}
+
+hipError_t launch_mask_wsum(int n_lines, int n_v, int64_t n_t, const int32_t *m_start, const uint32_t *m_pk,
+                            const double2 *m_wv, const double *w_sorted, double *wsum, hipStream_t s) {
+  if (n_v == 0) return hipSuccess;
+  k_mask_wsum<<<n_v, kTPB, 0, s>>>(n_lines, n_v, n_t, m_start, m_pk, m_wv, w_sorted, wsum);
+  return hipGetLastError();
+}
+
+// spec_norm and the CCF of the unnormalised spectra in one launch: the blocks come
+// in groups of one CCF block followed by `ratio` norm blocks (ratio = n_norm /
+// n_ccf), the remaining norm blocks after the groups, so that every CU holds both
+// kinds and the norm's streaming runs beside the CCF's gathers.  blockDim.x is the
+// CCF's thread count (velocities rounded up to a wave); a norm block covers
+// kNormPer * blockDim.x pixels of one spectrum.
+__global__ void k_norm_ccf(NormCcfArgs a) {
+  extern __shared__ double ccf_lds[];
+  const long b = blockIdx.x;
+  const long grp = (long)a.ratio + 1;
+  const long mixed = (long)a.n_ccf * grp;
+  long ci = -1, ni;
+  if (b < mixed) {
+    long g = b / grp, r = b - g * grp;
+    if (r == 0)
+      ci = g;
+    else
+      ni = g * a.ratio + r - 1;
+  } else {
+    ni = (long)a.n_ccf * a.ratio + (b - mixed);
+  }
+  if (ci >= 0 && a.fp)
+    ccf_lds_body(a.raw, a.n, a.trim, a.n_spec, a.n_lines, a.n_v, a.n_t, a.m_start, a.m_pk, a.m_wv, a.w_sorted, a.fp,
+                 a.fmax, a.part, (int)(ci % a.nc), (int)(ci / a.nc), ccf_lds);
+  else if (ci >= 0)
+    ccf_body(a.raw, a.n, a.trim, a.n_spec, a.n_lines, a.n_v, a.n_t, a.m_start, a.m_pk, a.m_wv, a.w_sorted, a.part,
+             (int)(ci % a.nc), (int)(ci / a.nc));
+  else if (ni < a.n_norm)
+    norm_body(a.n, a.mm, a.raw, a.out, (int)(ni % a.nbx), (int)(ni / a.nbx), blockDim.x);
+}
+
+hipError_t launch_norm_ccf(NormCcfArgs a, hipStream_t s) {
+  if (a.n_spec == 0) return hipSuccess;
+  if (a.n_v > 1024 || (((uintptr_t)a.raw ^ (uintptr_t)a.out) & 15)) return hipErrorInvalidValue;
+  if (a.fp && (a.fmax < 0 || ccf_lds_bytes(a.fmax) > 65536)) return hipErrorInvalidValue;
+  const int threads = ((a.n_v + 63) / 64) * 64;
+  a.nc = ccf_chunks(a.n_lines);
+  a.n_ccf = a.nc * ((a.n_spec + kCcfSpec - 1) / kCcfSpec);
+  a.nbx = (int)((a.n + (int64_t)threads * kNormPer - 1) / ((int64_t)threads * kNormPer));
+  a.n_norm = a.nbx * a.n_spec;
+  a.ratio = a.n_ccf > 0 ? a.n_norm / a.n_ccf : 0;
+  const long blocks = (long)a.n_ccf + a.n_norm;
+  if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
+  k_norm_ccf<<<(unsigned)blocks, threads, a.fp ? ccf_lds_bytes(a.fmax) : 0, s>>>(a);
+  return hipGetLastError();
 }
 
 int ccf_chunks(int n_lines) { return (n_lines + kCcfLines - 1) / kCcfLines; }
@@ -1573,14 +1796,14 @@ hipError_t launch_ccf(const double *spec, int64_t spec_stride, int64_t trim, int
   return hipGetLastError();
 }
 
-hipError_t launch_ccf_reduce(const double *part, int nc, int n_spec, int n_v, const double *appod, double *ccf,
-                             hipStream_t s) {
+hipError_t launch_ccf_reduce(const double *part, int nc, int n_spec, int n_v, const double *appod, const double *mm,
+                             const double *wsum, double *ccf, hipStream_t s) {
   if (n_spec == 0) return hipSuccess;
   if (n_v > 1024) return hipErrorInvalidValue;
   int rt = 1;
   while (rt < n_v && rt < 1024) rt <<= 1;
   rt = rt < 64 ? 64 : rt;
-  k_ccf_reduce<<<n_spec, rt, 0, s>>>(part, nc, n_spec, n_v, appod, ccf);
+  k_ccf_reduce<<<n_spec, rt, 0, s>>>(part, nc, n_spec, n_v, appod, mm, wsum, ccf);
   return hipGetLastError();
 }
 
@@ -1981,7 +2204,7 @@ __global__ __launch_bounds__(kTPB) void kb_minmax(SOAP_BATCH_ARGS(MinmaxJob)) {
 template <bool P>
 __global__ __launch_bounds__(kTPB) void kb_norm(SOAP_BATCH_ARGS(NormJob)) {
   batch_loop<P>(jobs, prefix, n_inst, total, [&](const NormJob &j, int local) {
-    norm_body(j.n, j.mm, j.o, local % (int)j.bx, local / (int)j.bx);
+    norm_body(j.n, j.mm, j.in, j.o, local % (int)j.bx, local / (int)j.bx, kTPB);
   });
 }
 
@@ -1996,7 +2219,7 @@ __global__ __launch_bounds__(1024) void kb_ccf(SOAP_BATCH_ARGS(CcfJob)) {
 template <bool P>
 __global__ __launch_bounds__(1024) void kb_ccf_reduce(SOAP_BATCH_ARGS(CcfRedJob)) {
   batch_loop<P>(jobs, prefix, n_inst, total, [&](const CcfRedJob &j, int local) {
-    ccf_reduce_body(j.part, j.nc, j.n_spec, j.n_v, j.appod, j.ccf, local, 0);
+    ccf_reduce_body(j.part, j.nc, j.n_spec, j.n_v, j.appod, j.mm, j.wsum, j.ccf, local, 0);
   });
 }
 
