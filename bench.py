@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 PEAK_FP64_TFS = 78.6       # MI355X FP64 (vector and matrix)
-KERNELS = {"fir_active": "k_fir_active_grp", "conv": "k_conv_active", "norm": "k_norm", "ccf": "k_ccf"}
+KERNELS = {"fir_active": "k_fir_active_grp", "conv": "k_conv_active", "norm": "k_norm", "ccf": "k_ccf_lds"}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_series_1000grid_733k.json")
 
 

@@ -533,7 +533,7 @@ __global__ void k_diff_gradF(DiffArgs a, const double *target, int trim, double 
 
 // transposed mask as a gather: G_F[p][trim + t] += sum_e gy[p][col_e] val_e over the
 // entries e of row t of the pixel-major table (the (velocity, weight) pairs of
-// every live mask pixel of k_ccf, merged per pixel).  The table is sliced by 64
+// every live mask pixel of the CCF, merged per pixel).  The table is sliced by 64
 // rows (one wave): entry k of row t sits at slice_ptr[t / 64] + 64 k + t % 64,
 // rows padded with zero weights to their slice's width, so every load is
 // coalesced.  Block = (kDT pixels, chunk of kMtP phases), the chunk's gy rows in

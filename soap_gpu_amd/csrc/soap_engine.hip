@@ -131,7 +131,7 @@ struct soap_ctx {
   DevBuf m_start, m_len, m_val, m_alive, sorted, rank, ccf_part;
   DevBuf wsum;  // [n_v] live mask weight per velocity (the CCF of a spectrum of ones)
   DevBuf fp;    // [n_lines] int2 line footprints (first trimmed pixel, length)
-  int ccf_fmax = -1;  // longest footprint; -1: the CCF gathers from global memory
+  int ccf_cap = 0;  // LDS segment length of the CCF (ccf_lds_cap of the longest footprint)
   int mask_scan = 0;
   bool have_mask = false, mask_built = false;
   double *d_ccf = nullptr;  // [(P+1)*3][n_v]
@@ -1484,7 +1484,7 @@ static int build_mask(soap_ctx *c) {
   if ((rc = ensure(c, c->wsum, (size_t)c->n_v * 8))) return rc;
   HIPCHK(c, launch_mask_wsum(c->n_lines, c->n_v, n_t, (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p,
                              (const double2 *)c->m_val.p, c->d_w_sorted, (double *)c->wsum.p, c->stream));
-  // line footprints: the CCF stages them in LDS when the longest fits
+  // line footprints: the CCF stages them in LDS
   if ((rc = ensure(c, c->fp, (size_t)c->n_lines * 8))) return rc;
   HIPCHK(c, launch_mask_footprint(c->n_lines, c->n_v, n_t, (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p,
                                   (int2 *)c->fp.p, c->stream));
@@ -1493,14 +1493,14 @@ static int build_mask(soap_ctx *c) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   int fmax = 0;
   for (int q = 0; q < c->n_lines; q++) fmax = std::max(fmax, fp[2 * q + 1]);
-  c->ccf_fmax = ccf_lds_bytes(fmax) <= 65536 ? fmax : -1;
+  c->ccf_cap = ccf_lds_cap(fmax);
   c->mask_built = true;
   c->diff.mt_ready = false;
   return SOAP_OK;
 }
 
 // Pixel-major form of the built mask for the CCF adjoint: for every trimmed
-// pixel t, the (velocity, weight) pairs of the live mask pixels that k_ccf
+// pixel t, the (velocity, weight) pairs of the live mask pixels that the CCF
 // multiplies with spectrum pixel t, pairs with the same velocity merged in line
 // order; stored in slices of 64 rows (one wave), column-major inside a slice.
 // Built on the host once per mask.
@@ -1518,7 +1518,7 @@ static int diff_mask_transpose(soap_ctx *c) {
   HIPCHK(c, hipMemcpyAsync(wv.data(), c->m_val.p, ne * 16, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(ws.data(), c->d_w_sorted, (size_t)nl * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  // the live (pixel, velocity, weight) triples, in line order, as k_ccf visits them
+  // the live (pixel, velocity, weight) triples, in line order, as the CCF visits them
   auto visit = [&](auto &&f) {
     for (int q = 0; q < nl; q++)
       for (int iv = 0; iv < nv; iv++) {
@@ -1592,6 +1592,14 @@ static int diff_mask_transpose(soap_ctx *c) {
 }
 
 
+// CCF chunk partials of n_spec spectra (stride n) with the built mask, line
+// footprints staged in LDS
+static hipError_t ctx_ccf(soap_ctx *c, const double *spec, int n_spec, double *part, hipStream_t st) {
+  return launch_ccf_lds(spec, c->n, c->trim, n_spec, c->n_lines, c->n_v, c->n - 2 * (int64_t)c->trim,
+                        (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p,
+                        c->d_w_sorted, (const int2 *)c->fp.p, c->ccf_cap, part, st);
+}
+
 // The CCF runs on the unnormalised spectra d_lraw (the reduction applies the
 // spec_norm map, see k_ccf_reduce).  With norm set, spec_norm itself (d_lraw ->
 // d_lower) runs in the same launch (the series: resolution_impl without norm
@@ -1622,22 +1630,15 @@ static int ccf_impl(soap_ctx *c, bool norm) {
     a.m_pk = (const uint32_t *)c->m_alive.p;
     a.m_wv = (const double2 *)c->m_val.p;
     a.w_sorted = c->d_w_sorted;
-    a.fp = c->ccf_fmax >= 0 ? (const int2 *)c->fp.p : nullptr;
-    a.fmax = c->ccf_fmax;
+    a.fp = (const int2 *)c->fp.p;
+    a.cap = c->ccf_cap;
     a.part = (double *)c->ccf_part.p;
     StageTimer tm(c, ST_CCF);
     HIPCHK(c, launch_norm_ccf(a, c->stream));
     c->have_lower = true;
-  } else if (c->ccf_fmax >= 0) {
-    StageTimer tm(c, ST_CCF);
-    HIPCHK(c, launch_ccf_lds(c->d_lraw, c->n, c->trim, n_spec, c->n_lines, c->n_v, c->n - 2 * (int64_t)c->trim,
-                             (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p,
-                             c->d_w_sorted, (const int2 *)c->fp.p, c->ccf_fmax, (double *)c->ccf_part.p, c->stream));
   } else {
     StageTimer tm(c, ST_CCF);
-    HIPCHK(c, launch_ccf(c->d_lraw, c->n, c->trim, n_spec, c->n_lines, c->n_v, c->n - 2 * (int64_t)c->trim,
-                         (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p,
-                         c->d_w_sorted, (double *)c->ccf_part.p, c->stream));
+    HIPCHK(c, ctx_ccf(c, c->d_lraw, n_spec, (double *)c->ccf_part.p, c->stream));
   }
   {
     StageTimer tm(c, ST_CCF_REDUCE);
@@ -1906,9 +1907,7 @@ int soap_diff_forward(soap_ctx *c, const soap_diff_params *prm, const double *d_
   }
   {
     StageTimer tm(c, ST_CCF);
-    HIPCHK(c, launch_ccf((const double *)d.F.p, c->n, c->trim, d.P, c->n_lines, c->n_v, c->n - 2 * (int64_t)c->trim,
-                         (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p,
-                         c->d_w_sorted, (double *)c->ccf_part.p, c->stream));
+    HIPCHK(c, ctx_ccf(c, (const double *)d.F.p, d.P, (double *)c->ccf_part.p, c->stream));
     HIPCHK(c, launch_ccf_sum((const double *)c->ccf_part.p, ccf_chunks(c->n_lines), d.P, c->n_v, (double *)d.ccf.p,
                              c->stream));
   }
@@ -2115,7 +2114,8 @@ int batch_upload(soap_batch *b, int stage, const std::vector<Job> &jobs, const s
 // be stepped through stale pointers.
 std::vector<uintptr_t> batch_signature(const soap_ctx *c) {
   auto u = [](const void *p) { return (uintptr_t)p; };
-  return {u(c->d_raw),    u(c->d_lower), u(c->d_lraw),   u(c->wsum.p),   u(c->d_quiet),   u(c->acc.p),    u(c->taps.p),   u(c->taps_g.p), u(c->krange.p),
+  return {u(c->d_raw),    u(c->d_lower), u(c->d_lraw),   u(c->wsum.p),   u(c->fp.p),     (uintptr_t)c->ccf_cap,
+          u(c->d_quiet),   u(c->acc.p),    u(c->taps.p),   u(c->taps_g.p), u(c->krange.p),
           u(c->kflags.p), u(c->bins.p),  u(c->nbins.p),   u(c->part.p),   u(c->mm.p),     u(c->ctap.p),
           u(c->d_ccf),    u(c->ccf_part.p), u(c->f_par.p), u(c->f_model.p), u(c->f_depth.p), u(c->f_bis.p),
           u(c->f_iter.p), u(c->m_start.p), u(c->m_alive.p), u(c->m_val.p), u(c->d_geo),   u(c->d_box),
@@ -2147,7 +2147,7 @@ int batch_build(soap_batch *b) {
   std::vector<FitJob> ft(ni);
   std::vector<int64_t> w[BS_COUNT];
   for (auto &v : w) v.assign(ni, 0);
-  size_t lds_taps = 0, lds_fir = 0, lds_conv = 0;
+  size_t lds_taps = 0, lds_fir = 0, lds_conv = 0, lds_ccf = 0;
   int ccf_threads = 64, red_threads = 64;
   for (size_t i = 0; i < ni; i++) {
     soap_ctx *c = b->inst[i];
@@ -2229,7 +2229,8 @@ int batch_build(soap_batch *b) {
     int nc = ccf_chunks(c->n_lines);
     cc[i] = CcfJob{c->d_lraw, c->n, c->trim, c->n - 2 * (int64_t)c->trim, n_spec, c->n_lines, c->n_v, nc,
                    (const int32_t *)c->m_start.p, (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p,
-                   c->d_w_sorted, (double *)c->ccf_part.p};
+                   c->d_w_sorted, (const int2 *)c->fp.p, c->ccf_cap, (double *)c->ccf_part.p};
+    lds_ccf = std::max(lds_ccf, ccf_lds_bytes(c->ccf_cap));
     w[BS_CCF][i] = (int64_t)nc * ((n_spec + ccf_spec_chunk() - 1) / ccf_spec_chunk());
     ccf_threads = std::max(ccf_threads, (c->n_v + 63) / 64 * 64);
     int rt = 64;
@@ -2257,7 +2258,7 @@ int batch_build(soap_batch *b) {
       (rc = batch_upload(b, BS_CONV, ca, w[BS_CONV], 256, lds_conv)) ||
       (rc = batch_upload(b, BS_MINMAX, mm, w[BS_MINMAX], 256, 0)) ||
       (rc = batch_upload(b, BS_NORM, nm, w[BS_NORM], 256, 0)) ||
-      (rc = batch_upload(b, BS_CCF, cc, w[BS_CCF], ccf_threads, 0)) ||
+      (rc = batch_upload(b, BS_CCF, cc, w[BS_CCF], ccf_threads, lds_ccf)) ||
       (rc = batch_upload(b, BS_CCF_REDUCE, cr, w[BS_CCF_REDUCE], red_threads, 0)) ||
       (rc = batch_upload(b, BS_FIT, ft, w[BS_FIT], 64, 0)))
     return rc;
@@ -2639,11 +2640,7 @@ hipError_t dbatch_enqueue_one(soap_diffbatch::Item &it, hipStream_t st, bool adj
     ConvArgs ca = diff_conv(c);
     int rb = 0;
     hipError_t e = launch_diff_forward_spectra(a, ca, it.target, c->trim, (double *)d.res_part.p, &rb, st);
-    const int64_t n_t = c->n - 2 * (int64_t)c->trim;
-    if (e == hipSuccess)
-      e = launch_ccf((const double *)d.F.p, c->n, c->trim, d.P, c->n_lines, c->n_v, n_t, (const int32_t *)c->m_start.p,
-                     (const uint32_t *)c->m_alive.p, (const double2 *)c->m_val.p, c->d_w_sorted,
-                     (double *)c->ccf_part.p, st);
+    if (e == hipSuccess) e = ctx_ccf(c, (const double *)d.F.p, d.P, (double *)c->ccf_part.p, st);
     if (e == hipSuccess)
       e = launch_ccf_sum((const double *)c->ccf_part.p, ccf_chunks(c->n_lines), d.P, c->n_v, (double *)d.ccf.p,
                          st);
@@ -2665,8 +2662,8 @@ hipError_t dbatch_enqueue_one(soap_diffbatch::Item &it, hipStream_t st, bool adj
     if (!adjoint) return e;
     if (e == hipSuccess)
       e = launch_diff_backward(a, ca, it.target, c->trim, it.w_spec, c->d_vgrid, c->n_v, (const double *)d.ccf.p,
-                               (const double *)d.th.p, (const double *)d.dth.p, (double *)d.gy.p, n_t,
-                               (const int64_t *)d.mt_ptr.p, (const int32_t *)d.mt_w.p, (const int32_t *)d.mt_col.p,
+                               (const double *)d.th.p, (const double *)d.dth.p, (double *)d.gy.p,
+                               c->n - 2 * (int64_t)c->trim, (const int64_t *)d.mt_ptr.p, (const int32_t *)d.mt_w.p, (const int32_t *)d.mt_col.p,
                                (const double *)d.mt_val.p, (double *)d.GF.p, (double *)d.GI.p, (double *)d.Cpart.p,
                                (double *)d.C.p, it.gmaps, (double *)d.cell_part.p, it.gq, it.gs, bisd, st);
     if (e == hipSuccess) e = launch_diff_grad4(diff_cell_blocks((int64_t)d.G * d.G), (const double *)d.cell_part.p,

@@ -218,6 +218,8 @@ struct CcfJob {                                                          // work
   const uint32_t *m_pk;
   const double2 *m_wv;
   const double *w_sorted;
+  const int2 *fp;  // line footprints, staged in LDS
+  int32_t cap;     // LDS segment length (ccf_lds_cap)
   double *part;
 };
 struct CcfRedJob {  // work: n_spec
@@ -285,20 +287,22 @@ struct NormCcfArgs {
   const uint32_t *m_pk;
   const double2 *m_wv;
   const double *w_sorted;
-  const int2 *fp;  // line footprints: the CCF stages them in LDS (null: the gather form)
-  int32_t fmax;    // longest footprint
+  const int2 *fp;  // line footprints, staged in LDS by the CCF
+  int32_t cap;     // LDS segment length (ccf_lds_cap)
   double *part;
   int32_t nc, n_ccf, nbx, n_norm, ratio;
 };
 hipError_t launch_norm_ccf(NormCcfArgs a, hipStream_t s);
-// the CCF with line footprints staged in LDS (fp from launch_mask_footprint;
-// needs ccf_lds_bytes(fmax) <= 64 KiB)
+// the CCF with line footprints staged in LDS (fp from launch_mask_footprint) in
+// segments of cap = ccf_lds_cap(longest footprint) pixels, ccf_lds_bytes(cap) of
+// LDS per block
 hipError_t launch_mask_footprint(int n_lines, int n_v, int64_t n_t, const int32_t *m_start, const uint32_t *m_pk,
                                  int2 *fp, hipStream_t s);
-size_t ccf_lds_bytes(int fmax);
+int ccf_lds_cap(int fmax);
+size_t ccf_lds_bytes(int cap);
 hipError_t launch_ccf_lds(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
                           int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
-                          const double *w_sorted, const int2 *fp, int fmax, double *part, hipStream_t s);
+                          const double *w_sorted, const int2 *fp, int cap, double *part, hipStream_t s);
 hipError_t launch_mask_wsum(int n_lines, int n_v, int64_t n_t, const int32_t *m_start, const uint32_t *m_pk,
                             const double2 *m_wv, const double *w_sorted, double *wsum, hipStream_t s);
 // CCF
@@ -311,9 +315,6 @@ hipError_t launch_mask_overwrite(const int32_t *sorted, int n_lines, int n_v, in
 int ccf_chunks(int n_lines);
 // CCF of n_spec spectra (spec + sp*spec_stride + trim), normalised and appodised;
 // part: ccf_chunks(n_lines) * n_spec * n_v doubles; w_sorted[q] = weight of line sorted[q]
-hipError_t launch_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
-                      int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
-                      const double *w_sorted, double *part, hipStream_t s);
 hipError_t launch_ccf_reduce(const double *part, int nc, int n_spec, int n_v, const double *appod, const double *mm,
                              const double *wsum, double *ccf, hipStream_t s);
 hipError_t launch_fit(const double *v, int n_v, const double *y, int n_ccf, double *par, double *model,
@@ -344,7 +345,7 @@ int diff_cell_blocks(int64_t ncell);
 hipError_t launch_fit_polish(const double *v, int n_v, const double *ccf, const double *par, int n_ccf, double *theta,
                              hipStream_t s);
 // mt_*: the mask in pixel-major form, sliced by 64 rows (row t = trimmed pixel
-// t; columns = velocities; values = the k_ccf pixel weights, merged per (pixel,
+// t; columns = velocities; values = the CCF pixel weights, merged per (pixel,
 // velocity)): entry k of row t at mt_ptr[t / 64] + 64 k + t % 64, mt_w[t / 64]
 // entries per row of the slice (zero-padded)
 hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const double *target, int trim, double w_spec,

@@ -10,7 +10,7 @@
 //   2. k_taps_*      : exact fixed-point accumulation of the taps   (a2/a5/a6)
 //   3. k_fir_*       : lambda-streaming FIR over the seed rows       (a2/a6)
 //   4. k_conv/k_norm : Gaussian instrumental convolution + spec_norm (a7)
-//   5. k_mask_*/k_ccf: sparse mask CCF                               (a8)
+//   5. k_mask_*/k_ccf_lds: sparse mask CCF                           (a8)
 //   6. k_fit         : Gaussian LM fit + bisector, one wave per CCF  (a9)
 #include <hip/hip_runtime.h>
 #include <math.h>
@@ -1479,7 +1479,9 @@ hipError_t launch_mask_overwrite(const int32_t *sorted, int n_lines, int n_v, in
 // calculate_CCF_2 (CCF_func.py:27-36) on the sparse mask.  Block = (chunk of
 // kCcfLines lines in wavelength order, chunk of kCcfSpec spectra); thread = one
 // velocity, summing its lines in order for all spectra of the chunk (each mask
-// entry is read once per block).  Partials per line chunk are summed in order
+// entry is read once per block).  Each line's pixel footprint is staged in LDS
+// first: consecutive velocities' windows overlap, so a pixel is loaded once per
+// line instead of once per window.  Partials per line chunk are summed in order
 // by k_ccf_reduce, which also normalises by the maximum and appodises
 // (run_all_SOAP.py:368-370).
 //
@@ -1487,49 +1489,11 @@ hipError_t launch_mask_overwrite(const int32_t *sorted, int n_lines, int n_v, in
 // That map is affine per spectrum, so the CCF of y is W(v) - C_x(v) (1 - min) /
 // max with C_x the CCF of the unnormalised spectrum x and W(v) the sum of the
 // live mask weights at velocity v (the CCF of a spectrum of ones, k_mask_wsum).
-// k_ccf therefore runs on x, and spec_norm does not have to finish before it: the
-// two share one launch (k_norm_ccf).
+// The CCF therefore runs on x, and spec_norm does not have to finish before it:
+// the two share one launch (k_norm_ccf).
 constexpr int kCcfLines = 32;
 constexpr int kCcfSpec = 8;
-
-__device__ __forceinline__ void ccf_body(const double *spec_, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v, int64_t n_t, const int32_t *m_start_, const uint32_t *m_pk_, const double2 *m_wv_, const double *w_sorted_, double *part_, int bx_, int by_) {
-  const GLOBAL double *spec = gl(spec_), *w_sorted = gl(w_sorted_);
-  const GLOBAL int32_t *m_start = gl(m_start_);
-  const GLOBAL uint32_t *m_pk = gl(m_pk_);
-  const GLOBAL double *m_wv = gl((const double *)m_wv_);  // {x, y} pairs
-  GLOBAL double *part = gw(part_);
-  int chunk = bx_;
-  int s0 = by_ * kCcfSpec;
-  int ns = min(kCcfSpec, n_spec - s0);
-  int q0 = chunk * kCcfLines, q1 = min(n_lines, q0 + kCcfLines);
-  const GLOBAL double *S[kCcfSpec];
-#pragma unroll
-  for (int k = 0; k < kCcfSpec; k++) S[k] = spec + (long)(s0 + (k < ns ? k : 0)) * spec_stride + trim;
-  for (int iv = threadIdx.x; iv < n_v; iv += blockDim.x) {
-    double acc[kCcfSpec];
-#pragma unroll
-    for (int k = 0; k < kCcfSpec; k++) acc[k] = 0.0;
-    for (int q = q0; q < q1; q++) {
-      long e = (long)q * n_v + iv;
-      int st = m_start[e];
-      uint32_t pk = m_pk[e];
-      double2 wv;
-      wv.x = m_wv[2 * e];
-      wv.y = m_wv[2 * e + 1];
-      double w = w_sorted[q];
-      int ln = (int)(pk & 0xffu);
-      uint32_t al = pk >> 8;
-      for (int b = 0; b < ln; b++) {
-        long pp = st + b;
-        if (!((al >> b) & 1u) || pp < 0 || pp >= n_t) continue;
-        double val = (b == ln - 1) ? wv.y : ((b == 0) ? wv.x : w);
-#pragma unroll
-        for (int k = 0; k < kCcfSpec; k++) acc[k] = fma(S[k][pp], val, acc[k]);
-      }
-    }
-    for (int k = 0; k < ns; k++) part[((long)chunk * n_spec + s0 + k) * n_v + iv] = acc[k];
-  }
-}
+constexpr int kCcfLdsCap = 512;  // LDS segment of the footprint form: 8 spectra x 512 pixels = 32 KiB
 
 // Line footprints for the LDS form below: fp[q] = (lo, len), the trimmed pixels
 // [lo, lo + len) that the window of line q touches at any velocity, clipped to
@@ -1559,15 +1523,16 @@ hipError_t launch_mask_footprint(int n_lines, int n_v, int64_t n_t, const int32_
   return hipGetLastError();
 }
 
-// ccf_body with each line's footprint staged in LDS: per line, the block copies
-// the footprint pixels of its spectra with coalesced loads (8 in flight per
-// thread), then every thread (one velocity; blockDim.x >= n_v) reads its window
-// from LDS.  The sums are those of ccf_body, term for term in the same order.
-// lds: kCcfSpec * fmax doubles, fmax >= every footprint length.
+// Per line, the block copies the footprint pixels of its spectra with coalesced
+// loads (8 in flight per thread), then every thread (one velocity; blockDim.x >=
+// n_v) adds its window's pixels from LDS, in pixel order.  A footprint longer
+// than cap pixels goes through in segments of cap (a window that straddles two
+// takes its low pixels from the first, so the order of the terms is kept).
+// lds: kCcfSpec * cap doubles.
 __device__ __forceinline__ void ccf_lds_body(const double *spec_, int64_t spec_stride, int64_t trim, int n_spec,
                                              int n_lines, int n_v, int64_t n_t, const int32_t *m_start_,
                                              const uint32_t *m_pk_, const double2 *m_wv_, const double *w_sorted_,
-                                             const int2 *fp_, int fmax, double *part_, int bx_, int by_,
+                                             const int2 *fp_, int cap, double *part_, int bx_, int by_,
                                              double *lds) {
   const GLOBAL double *spec = gl(spec_), *w_sorted = gl(w_sorted_);
   const GLOBAL int32_t *m_start = gl(m_start_);
@@ -1601,28 +1566,32 @@ __device__ __forceinline__ void ccf_lds_body(const double *spec_, int64_t spec_s
     }
     const double w = w_sorted[q];
     const int f_lo = fp[2 * q], f_len = fp[2 * q + 1];
-    __syncthreads();  // the previous line's readers are done with the buffer
-    for (int j0 = 0; j0 < f_len; j0 += blockDim.x) {
-      const int j = j0 + threadIdx.x;
-      double v[kCcfSpec];
+    for (int g0 = 0; g0 < f_len; g0 += cap) {
+      const int seg = min(cap, f_len - g0);
+      const long base = (long)f_lo + g0;
+      __syncthreads();  // the previous segment's readers are done with the buffer
+      for (int j0 = 0; j0 < seg; j0 += blockDim.x) {
+        const int j = j0 + threadIdx.x;
+        double v[kCcfSpec];
 #pragma unroll
-      for (int k = 0; k < kCcfSpec; k++) v[k] = (j < f_len && k < ns) ? S[k][f_lo + j] : 0.0;
-      if (j < f_len) {
+        for (int k = 0; k < kCcfSpec; k++) v[k] = (j < seg && k < ns) ? S[k][base + j] : 0.0;
+        if (j < seg) {
 #pragma unroll
-        for (int k = 0; k < kCcfSpec; k++) lds[k * fmax + j] = v[k];
+          for (int k = 0; k < kCcfSpec; k++) lds[k * cap + j] = v[k];
+        }
       }
-    }
-    __syncthreads();
-    if (act) {
-      const int ln = (int)(pk & 0xffu);
-      const uint32_t al = pk >> 8;
-      for (int b = 0; b < ln; b++) {
-        const long pp = st + b;
-        if (!((al >> b) & 1u) || pp < 0 || pp >= n_t) continue;
-        const double val = (b == ln - 1) ? wy : ((b == 0) ? wx : w);
-        const double *L = lds + (pp - f_lo);
+      __syncthreads();
+      if (act) {
+        const int ln = (int)(pk & 0xffu);
+        const uint32_t al = pk >> 8;
+        for (int b = 0; b < ln; b++) {
+          const long pp = st + b;
+          if (!((al >> b) & 1u) || pp < 0 || pp >= n_t || pp < base || pp >= base + seg) continue;
+          const double val = (b == ln - 1) ? wy : ((b == 0) ? wx : w);
+          const double *L = lds + (pp - base);
 #pragma unroll
-        for (int k = 0; k < kCcfSpec; k++) acc[k] = fma(L[k * fmax], val, acc[k]);
+          for (int k = 0; k < kCcfSpec; k++) acc[k] = fma(L[k * cap], val, acc[k]);
+        }
       }
     }
   }
@@ -1632,31 +1601,27 @@ __device__ __forceinline__ void ccf_lds_body(const double *spec_, int64_t spec_s
 
 __global__ void k_ccf_lds(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
                           int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
-                          const double *w_sorted, const int2 *fp, int fmax, double *part) {
+                          const double *w_sorted, const int2 *fp, int cap, double *part) {
   extern __shared__ double ccf_lds[];
-  ccf_lds_body(spec, spec_stride, trim, n_spec, n_lines, n_v, n_t, m_start, m_pk, m_wv, w_sorted, fp, fmax, part,
+  ccf_lds_body(spec, spec_stride, trim, n_spec, n_lines, n_v, n_t, m_start, m_pk, m_wv, w_sorted, fp, cap, part,
                blockIdx.x, blockIdx.y, ccf_lds);
 }
 
-size_t ccf_lds_bytes(int fmax) { return (size_t)kCcfSpec * (fmax > 0 ? fmax : 1) * 8; }
+// LDS segment length for a longest footprint of fmax pixels
+int ccf_lds_cap(int fmax) { return fmax < 1 ? 1 : (fmax < kCcfLdsCap ? fmax : kCcfLdsCap); }
+size_t ccf_lds_bytes(int cap) { return (size_t)kCcfSpec * (cap > 0 ? cap : 1) * 8; }
 
 hipError_t launch_ccf_lds(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
                           int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
-                          const double *w_sorted, const int2 *fp, int fmax, double *part, hipStream_t s) {
+                          const double *w_sorted, const int2 *fp, int cap, double *part, hipStream_t s) {
   if (n_spec == 0) return hipSuccess;
-  if (n_v > 1024 || fmax < 0 || ccf_lds_bytes(fmax) > 65536) return hipErrorInvalidValue;
+  if (n_v > 1024 || cap < 1 || cap > kCcfLdsCap) return hipErrorInvalidValue;
   int nc = ccf_chunks(n_lines);
   int threads = ((n_v + 63) / 64) * 64;
   dim3 grid(nc, (n_spec + kCcfSpec - 1) / kCcfSpec);
-  k_ccf_lds<<<grid, threads, ccf_lds_bytes(fmax), s>>>(spec, spec_stride, trim, n_spec, n_lines, n_v, n_t, m_start,
-                                                        m_pk, m_wv, w_sorted, fp, fmax, part);
+  k_ccf_lds<<<grid, threads, ccf_lds_bytes(cap), s>>>(spec, spec_stride, trim, n_spec, n_lines, n_v, n_t, m_start,
+                                                       m_pk, m_wv, w_sorted, fp, cap, part);
   return hipGetLastError();
-}
-
-__global__ void k_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
-                      int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
-                      const double *w_sorted, double *part) {
-  ccf_body(spec, spec_stride, trim, n_spec, n_lines, n_v, n_t, m_start, m_pk, m_wv, w_sorted, part, blockIdx.x, blockIdx.y);
 }
 
 // ccf = W - (sum of the chunk partials of C_x) (1 - min) / max, then the
@@ -1696,7 +1661,7 @@ __global__ void k_ccf_reduce(const double *part, int n_chunks, int n_spec, int n
   ccf_reduce_body(part, n_chunks, n_spec, n_v, appod, mm, wsum, ccf, blockIdx.x, blockIdx.y);
 }
 
-// W(v): per velocity, the live mask weights k_ccf multiplies in-range pixels
+// W(v): per velocity, the live mask weights the CCF multiplies in-range pixels
 // with, summed over the lines (one block per velocity, lines strided over the
 // threads, then a fixed-order tree)
 __global__ __launch_bounds__(kTPB) void k_mask_wsum(int n_lines, int n_v, int64_t n_t, const int32_t *m_start,
@@ -1755,12 +1720,9 @@ __global__ void k_norm_ccf(NormCcfArgs a) {
   } else {
     ni = (long)a.n_ccf * a.ratio + (b - mixed);
   }
-  if (ci >= 0 && a.fp)
+  if (ci >= 0)
     ccf_lds_body(a.raw, a.n, a.trim, a.n_spec, a.n_lines, a.n_v, a.n_t, a.m_start, a.m_pk, a.m_wv, a.w_sorted, a.fp,
-                 a.fmax, a.part, (int)(ci % a.nc), (int)(ci / a.nc), ccf_lds);
-  else if (ci >= 0)
-    ccf_body(a.raw, a.n, a.trim, a.n_spec, a.n_lines, a.n_v, a.n_t, a.m_start, a.m_pk, a.m_wv, a.w_sorted, a.part,
-             (int)(ci % a.nc), (int)(ci / a.nc));
+                 a.cap, a.part, (int)(ci % a.nc), (int)(ci / a.nc), ccf_lds);
   else if (ni < a.n_norm)
     norm_body(a.n, a.mm, a.raw, a.out, (int)(ni % a.nbx), (int)(ni / a.nbx), blockDim.x);
 }
@@ -1768,7 +1730,7 @@ __global__ void k_norm_ccf(NormCcfArgs a) {
 hipError_t launch_norm_ccf(NormCcfArgs a, hipStream_t s) {
   if (a.n_spec == 0) return hipSuccess;
   if (a.n_v > 1024 || (((uintptr_t)a.raw ^ (uintptr_t)a.out) & 15)) return hipErrorInvalidValue;
-  if (a.fp && (a.fmax < 0 || ccf_lds_bytes(a.fmax) > 65536)) return hipErrorInvalidValue;
+  if (a.cap < 1 || a.cap > kCcfLdsCap) return hipErrorInvalidValue;
   const int threads = ((a.n_v + 63) / 64) * 64;
   a.nc = ccf_chunks(a.n_lines);
   a.n_ccf = a.nc * ((a.n_spec + kCcfSpec - 1) / kCcfSpec);
@@ -1777,24 +1739,11 @@ hipError_t launch_norm_ccf(NormCcfArgs a, hipStream_t s) {
   a.ratio = a.n_ccf > 0 ? a.n_norm / a.n_ccf : 0;
   const long blocks = (long)a.n_ccf + a.n_norm;
   if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
-  k_norm_ccf<<<(unsigned)blocks, threads, a.fp ? ccf_lds_bytes(a.fmax) : 0, s>>>(a);
+  k_norm_ccf<<<(unsigned)blocks, threads, ccf_lds_bytes(a.cap), s>>>(a);
   return hipGetLastError();
 }
 
 int ccf_chunks(int n_lines) { return (n_lines + kCcfLines - 1) / kCcfLines; }
-
-hipError_t launch_ccf(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
-                      int64_t n_t, const int32_t *m_start, const uint32_t *m_pk, const double2 *m_wv,
-                      const double *w_sorted, double *part, hipStream_t s) {
-  if (n_spec == 0) return hipSuccess;
-  if (n_v > 1024) return hipErrorInvalidValue;
-  int nc = ccf_chunks(n_lines);
-  int threads = ((n_v + 63) / 64) * 64;
-  dim3 grid(nc, (n_spec + kCcfSpec - 1) / kCcfSpec);
-  k_ccf<<<grid, threads, 0, s>>>(spec, spec_stride, trim, n_spec, n_lines, n_v, n_t, m_start, m_pk, m_wv, w_sorted,
-                                 part);
-  return hipGetLastError();
-}
 
 hipError_t launch_ccf_reduce(const double *part, int nc, int n_spec, int n_v, const double *appod, const double *mm,
                              const double *wsum, double *ccf, hipStream_t s) {
@@ -2210,9 +2159,10 @@ __global__ __launch_bounds__(kTPB) void kb_norm(SOAP_BATCH_ARGS(NormJob)) {
 
 template <bool P>
 __global__ __launch_bounds__(1024) void kb_ccf(SOAP_BATCH_ARGS(CcfJob)) {
+  extern __shared__ double ccf_lds[];
   batch_loop<P>(jobs, prefix, n_inst, total, [&](const CcfJob &j, int local) {
-    ccf_body(j.spec, j.stride, j.trim, j.n_spec, j.n_lines, j.n_v, j.n_t, j.m_start, j.m_pk, j.m_wv, j.w_sorted, j.part,
-             local % j.nc, local / j.nc);
+    ccf_lds_body(j.spec, j.stride, j.trim, j.n_spec, j.n_lines, j.n_v, j.n_t, j.m_start, j.m_pk, j.m_wv, j.w_sorted,
+                 j.fp, j.cap, j.part, local % j.nc, local / j.nc, ccf_lds);
   });
 }
 
