@@ -107,7 +107,6 @@ def summarize(out_prefix, workload, fetch, write, copy_fetch, copy_write):
 
 
 def stats(out_md, title, command, db, top=25, match=None):
-    import re
     import sqlite3
     con = sqlite3.connect(db)
     rows = con.execute("select name, count(*), avg(duration), min(duration), max(duration), sum(duration) "
