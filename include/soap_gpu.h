@@ -40,7 +40,8 @@ enum soap_status {
   SOAP_ERR_HIP = -3,     /* HIP runtime error */
   SOAP_ERR_NOMEM = -4,   /* device allocation failed */
   SOAP_ERR_GRID = -5,    /* wavelength grid not of the declared kind */
-  SOAP_ERR_LIMIT = -6    /* size beyond a compiled limit */
+  SOAP_ERR_LIMIT = -6,   /* size beyond a compiled limit */
+  SOAP_ERR_IO = -7       /* file missing or unreadable */
 };
 
 enum soap_grid_kind { SOAP_GRID_LINEAR = 0, SOAP_GRID_LOG = 1 };
@@ -196,6 +197,37 @@ typedef struct soap_bis_inject_params {
 } soap_bis_inject_params;
 int soap_bis_inject(soap_ctx *ctx, const soap_bis_inject_params *p, const double *wave, const double *flux, int64_t n,
                     double *out);
+
+/* ------------------------------------------------------------ seed preprocessing: local PHOENIX spectra */
+/* phoenix_extraction.py, local files only (it never downloads).  Host-only helpers
+ * (no GPU needed):
+ * soap_phoenix_bracket: the two library temperatures around temp (:136-146, the
+ *   reference's temp_array including its entry 48 = 6800 K); SOAP_ERR_LIMIT past
+ *   either end of the grid (where the reference fails or wraps to the last entry). */
+int soap_phoenix_bracket(double temp, double *t_low, double *t_high);
+/* the library grid values nearest to logg, [Fe/H], [alpha/Fe] (:84-104,126-131) */
+int soap_phoenix_grid(double logg, double feh, double alpha, double *logg_val, double *feh_val, double *alpha_val);
+/* PHOENIX_name (:36-54): the library subdirectory ("Z-0.0") and file name
+ * ("/lte05700-4.50-0.0.PHOENIX-ACES-AGSS-COND-2011-HiRes.fits") */
+int soap_phoenix_name(int32_t temp, double logg, double feh, double alpha, char *subdir, int32_t subdir_cap,
+                      char *name, int32_t name_cap);
+/* the primary image of a FITS file (astropy's fits.getdata for the PHOENIX files):
+ * BITPIX -64 / -32 / 32 / 16, all NAXISn values multiplied, BSCALE / BZERO applied.
+ * With out NULL or cap < n, only n and bitpix are returned.  SOAP_ERR_IO if the
+ * file cannot be read, SOAP_ERR_ARG if it is not such a FITS file. */
+int soap_fits_read(const char *path, double *out, int64_t cap, int64_t *n, int32_t *bitpix);
+/* On the device (:170-185): for the vacuum wavelengths wave_vac[n] and the spectra
+ * of the bracketing temperatures, the pixels with wave_vac in [wmin - 500, wmax +
+ * 500] and air wavelength (vac2air, Birch and Downs 1994, :13-22) in [wmin, wmax],
+ * in order: wave_air and flux = low + (temp - t_low) / (t_high - t_low) (high -
+ * low), the difference taken in float32 when f32 is set (the library's float32
+ * spectra).  Writes *n_out <= n values to wave_air and flux. */
+typedef struct soap_phoenix_params {
+  double temp, t_low, t_high, wmin, wmax;
+  int32_t f32;
+} soap_phoenix_params;
+int soap_phoenix_interp(soap_ctx *ctx, const soap_phoenix_params *p, const double *wave_vac, const double *spec_low,
+                        const double *spec_high, int64_t n, double *wave_air, double *flux, int64_t *n_out);
 
 /* ------------------------------------------------------------ whole series */
 /* disk init + quiet star + integrate + resolution + CCF + fit, all on device */

@@ -415,3 +415,79 @@ def bis_inject(wave, flux, up, scale, low=None, dep_cut=0.0, vel_cut=0.0, velo_s
     inside = (w < moved.max()) & (w > moved.min())
     out[inside] = np.interp(w[inside], xs, ys)
     return out
+
+
+# ----------------------------------------------- seed preprocessing: local PHOENIX spectra
+def phoenix_vac2air(wavelength):
+    """vac2air of phoenix_extraction.py:13-22 (Birch and Downs 1994): lambda / n_air."""
+    s = 1.e4 / wavelength
+    n_air = 1 + 0.0000834254 + 0.02406147 / (130 - s**2) + 0.00015998 / (38.9 - s**2)
+    return wavelength / n_air
+
+
+def phoenix_air2vac(wavelength):
+    """air2vac of phoenix_extraction.py:25-33 (N. Piskunov's inverse): lambda n_air."""
+    s = 1.e4 / wavelength
+    n_air = (1 + 0.00008336624212083 + 0.02408926869968 / (130.1065924522 - s**2)
+             + 0.0001599740894897 / (38.92568793293 - s**2))
+    return wavelength * n_air
+
+
+def phoenix_temp_grid():
+    """temp_array of phoenix_extraction.py:92-96 (entry 48 is 6800 K)."""
+    return np.array([2300.0 + 100.0 * j if j < 48 else 7000.0 + 200.0 * (j - 49) for j in range(73)])
+
+
+def phoenix_bracket(temp):
+    """(temp_low, temp_high) of phoenix_extraction.py:136-146; None where the
+    reference would index past the grid (or wrap to its last entry)."""
+    g = phoenix_temp_grid()
+    k = int(np.argmin(np.abs(g - temp)))
+    lo, hi = (k, k + 1) if temp > g[k] else (k - 1, k)
+    if lo < 0 or hi >= g.size:
+        return None
+    return g[lo], g[hi]
+
+
+def phoenix_grid(logg, feh=0.0, alpha=0.0):
+    """The nearest library grid values of phoenix_extraction.py:84-104,126-131."""
+    logg_array = np.linspace(0.0, 6.0, 13)
+    feh_array = np.concatenate((np.linspace(-4.0, -3.0, 2), np.linspace(-2.0, 1.0, 7))).round(decimals=2)
+    alpha_array = (-0.2 + np.arange(8) * 0.2).round(decimals=2)
+    lg = logg_array[np.argmin(np.abs(logg_array - logg))]
+    fe = feh_array[np.argmin(np.abs(feh_array - feh))]
+    al = 0.0 if fe > 0.0 else alpha_array[np.argmin(np.abs(alpha_array - alpha))]
+    return float(lg), float(fe), float(al)
+
+
+def phoenix_name(temp, logg, feh, alpha):
+    """PHOENIX_name of phoenix_extraction.py:36-54, restated."""
+    tail = '.PHOENIX-ACES-AGSS-COND-2011-HiRes.fits'
+    if feh <= 0.0:
+        sub = 'Z-%.1f' % abs(feh)
+        if alpha == 0.0:
+            name = '/lte0' + str(int(temp)) + '-%.2f-%.1f' % (logg, abs(feh)) + tail
+        elif alpha > 0.0:
+            name = '/lte0' + str(int(temp)) + '-%.2f-%.1f.Alpha=+%.2f' % (logg, abs(feh), alpha) + tail
+        else:
+            name = '/lte0' + str(int(temp)) + '-%.2f-%.1f.Alpha=%.2f' % (logg, abs(feh), alpha) + tail
+    else:
+        sub = 'Z+%.1f' % abs(feh)
+        name = '/lte0' + str(int(temp)) + '-%.2f+%.1f' % (logg, abs(feh)) + tail
+    return sub, name
+
+
+def phoenix_interp(wave_vac, spec_low, spec_high, temp, t_low, t_high, wmin, wmax):
+    """phoenix_extraction.py:170-185: vacuum window, vac2air, air window, and
+    flux = low + (temp - t_low) / (t_high - t_low) (high - low) per pixel, with the
+    spectra in their own dtype (float32 for the library: the difference is then a
+    float32 difference, as in the reference's scalar loop)."""
+    w = np.asarray(wave_vac, dtype=np.float64)
+    keep = (w >= (wmin - 500)) & (w <= (wmax + 500))
+    wv = w[keep]
+    lo, hi = np.asarray(spec_low)[keep], np.asarray(spec_high)[keep]
+    air = phoenix_vac2air(wv)
+    sel = (air >= wmin) & (air <= wmax)
+    frac = (temp - t_low) / (t_high - t_low)
+    flux = lo[sel].astype(np.float64) + frac * (hi[sel] - lo[sel]).astype(np.float64)
+    return air[sel], flux

@@ -22,7 +22,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-result", "-Wno-unused-variable",
 ]
 SOURCES = [os.path.join(HERE, "csrc", f) for f in ("soap_kernels.hip", "soap_linear.hip", "soap_diff.hip", "soap_inject.hip",
-                                                          "soap_engine.hip")]
+                                                          "soap_phoenix.hip", "soap_engine.hip")]
 
 
 def _run(cmd):
@@ -57,6 +57,7 @@ TOOLS = {
     "SOAP_resolution": ("cli_resolution.cpp", []),
     "BIS_FIT2": ("cli_bisfit.cpp", []),
     "soap_run": ("soap_run.cpp", []),
+    "phoenix_extraction": ("cli_phoenix.cpp", []),
 }
 
 

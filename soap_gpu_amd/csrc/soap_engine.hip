@@ -113,6 +113,7 @@ struct soap_ctx {
   DevBuf acc, taps, krange, kflags, bins, nbins;
   DevBuf taps_g;  // phase-interleaved active taps [n_mu][nd][taps_g_pad(P)][4]
   DevBuf inj;      // bisector injection: wave, flux, moved, sorted x, sorted y, out [6][n], partials, sort scratch
+  DevBuf phx;      // PHOENIX interpolation: wave, low, high, air, flux, out air, out flux [7][n], flags, count, scratch
   // quiet-star tap scratch of its own, so that the quiet star can run beside the
   // active phases (soap_run_series puts it on a second stream)
   DevBuf qacc, qtaps, qkrange, qkflags, qbins, qnbins;
@@ -1745,6 +1746,51 @@ int soap_bis_inject(soap_ctx *c, const soap_bis_inject_params *p, const double *
   HIPCHK(c, launch_inject_interp(d_w, xs, ys, n, d_sum, d_out, c->stream));
   HIPCHK(c, hipMemcpyAsync(out, d_out, n * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return SOAP_OK;
+}
+
+int soap_phoenix_interp(soap_ctx *c, const soap_phoenix_params *p, const double *wave_vac, const double *spec_low,
+                        const double *spec_high, int64_t n, double *wave_air, double *flux, int64_t *n_out) {
+  if (!c || !p || !wave_vac || !spec_low || !spec_high || !wave_air || !flux || !n_out)
+    return fail(c, SOAP_ERR_ARG, "phoenix_interp: null argument");
+  if (n < 1 || n > (int64_t)1 << 30) return fail(c, SOAP_ERR_ARG, "phoenix_interp: n = %lld", (long long)n);
+  if (!(p->t_high != p->t_low) || !std::isfinite(p->temp) || !std::isfinite(p->t_low) || !std::isfinite(p->t_high))
+    return fail(c, SOAP_ERR_ARG, "phoenix_interp: temperatures %g in [%g, %g]", p->temp, p->t_low, p->t_high);
+  HIPCHK(c, hipSetDevice(c->device));
+  PhoenixArgs a{};
+  a.frac = (p->temp - p->t_low) / (p->t_high - p->t_low);
+  a.vlo = p->wmin - 500;
+  a.vhi = p->wmax + 500;
+  a.wmin = p->wmin;
+  a.wmax = p->wmax;
+  a.f32 = p->f32 != 0;
+  size_t sel_bytes = 0;
+  HIPCHK(c, phoenix_select_bytes(n, &sel_bytes));
+  const size_t arrays = (size_t)7 * n * 8, flags = ((size_t)n + 255) / 256 * 256;
+  int rc;
+  if ((rc = ensure(c, c->phx, arrays + flags + 256 + sel_bytes + 256))) return rc;
+  double *d_w = (double *)c->phx.p, *d_lo = d_w + n, *d_hi = d_lo + n, *d_air = d_hi + n, *d_fl = d_air + n,
+         *d_oair = d_fl + n, *d_ofl = d_oair + n;
+  char *d_flag = (char *)(d_ofl + n);
+  int64_t *d_cnt = (int64_t *)(d_flag + flags);
+  void *d_tmp = (void *)(((uintptr_t)(d_cnt + 2) + 255) & ~(uintptr_t)255);
+  HIPCHK(c, hipMemcpyAsync(d_w, wave_vac, n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_lo, spec_low, n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d_hi, spec_high, n * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, launch_phoenix(a, d_w, d_lo, d_hi, n, d_air, d_fl, d_flag, c->stream));
+  HIPCHK(c, launch_phoenix_select(d_tmp, sel_bytes, d_air, d_flag, d_oair, d_cnt, n, c->stream));
+  HIPCHK(c, launch_phoenix_select(d_tmp, sel_bytes, d_fl, d_flag, d_ofl, d_cnt + 1, n, c->stream));
+  int64_t cnt[2] = {0, 0};
+  HIPCHK(c, hipMemcpyAsync(cnt, d_cnt, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (cnt[0] != cnt[1] || cnt[0] < 0 || cnt[0] > n)
+    return fail(c, SOAP_ERR_HIP, "phoenix_interp: selection counts %lld / %lld", (long long)cnt[0], (long long)cnt[1]);
+  if (cnt[0] > 0) {
+    HIPCHK(c, hipMemcpyAsync(wave_air, d_oair, cnt[0] * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(flux, d_ofl, cnt[0] * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  *n_out = cnt[0];
   return SOAP_OK;
 }
 

@@ -405,5 +405,17 @@ hipError_t launch_inject_sort(void *tmp, size_t tmp_bytes, const double *moved, 
                               double *ys, int64_t n, hipStream_t s);
 hipError_t launch_inject_interp(const double *wave, const double *xs, const double *ys, int64_t n,
                                 const double *summary, double *out, hipStream_t s);
+// ------------------------------------------------------------ local PHOENIX spectra (seed preprocessing)
+struct PhoenixArgs {
+  double frac;        // (temp - t_low) / (t_high - t_low)
+  double vlo, vhi;    // vacuum window wmin - 500, wmax + 500
+  double wmin, wmax;  // air window
+  int32_t f32;        // high - low in float32
+};
+hipError_t launch_phoenix(const PhoenixArgs &a, const double *wave, const double *lo, const double *hi, int64_t n,
+                          double *air, double *flux, char *flag, hipStream_t s);
+hipError_t phoenix_select_bytes(int64_t n, size_t *bytes);
+hipError_t launch_phoenix_select(void *tmp, size_t bytes, const double *in, const char *flag, double *out,
+                                 int64_t *count, int64_t n, hipStream_t s);
 
 }  // namespace soap

@@ -49,6 +49,7 @@ EXPORTS = (
     "soap_resolution", "soap_get_lower", "soap_get_norm",
     "soap_set_mask", "soap_ccf", "soap_get_ccf",
     "soap_fit", "soap_get_fit", "soap_fit_ccf_host", "soap_bis_inject",
+    "soap_phoenix_bracket", "soap_phoenix_grid", "soap_phoenix_name", "soap_fits_read", "soap_phoenix_interp",
     "soap_run_series",
     "soap_set_timing", "soap_get_stage_ms", "soap_stage_name", "soap_num_stages",
     "soap_get_integrate_stats",
@@ -77,6 +78,11 @@ class BisInjectParams(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("n_up", ctypes.c_int32), ("n_low", ctypes.c_int32),
                 ("up", ctypes.c_double * 16), ("low", ctypes.c_double * 16), ("dep_cut", ctypes.c_double),
                 ("vel_cut", ctypes.c_double), ("velo_shift", ctypes.c_double), ("scale", ctypes.c_double)]
+
+
+class PhoenixParams(ctypes.Structure):
+    _fields_ = [("temp", ctypes.c_double), ("t_low", ctypes.c_double), ("t_high", ctypes.c_double),
+                ("wmin", ctypes.c_double), ("wmax", ctypes.c_double), ("f32", ctypes.c_int32)]
 
 
 def load(path: str = LIB_PATH):
@@ -120,6 +126,14 @@ def load(path: str = LIB_PATH):
         "soap_get_fit": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp]),
         "soap_fit_ccf_host": (ctypes.c_int, [vp, _dp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, _dp, _dp, _dp]),
         "soap_bis_inject": (ctypes.c_int, [vp, ctypes.POINTER(BisInjectParams), _dp, _dp, ctypes.c_int64, _dp]),
+        "soap_phoenix_bracket": (ctypes.c_int, [ctypes.c_double, _dp, _dp]),
+        "soap_phoenix_grid": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]),
+        "soap_phoenix_name": (ctypes.c_int, [ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_char_p, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32]),
+        "soap_fits_read": (ctypes.c_int, [ctypes.c_char_p, _dp, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
+                                          ctypes.POINTER(ctypes.c_int32)]),
+        "soap_phoenix_interp": (ctypes.c_int, [vp, ctypes.POINTER(PhoenixParams), _dp, _dp, _dp, ctypes.c_int64, _dp,
+                                               _dp, ctypes.POINTER(ctypes.c_int64)]),
         "soap_run_series": (ctypes.c_int, [vp, ctypes.POINTER(Star), ctypes.c_double, ctypes.c_double]),
         "soap_set_timing": (ctypes.c_int, [vp, ctypes.c_int]),
         "soap_get_stage_ms": (ctypes.c_int, [vp, _dp, ctypes.c_int32]),
@@ -383,6 +397,24 @@ class Engine:
         self._chk(self.L.soap_bis_inject(self.ctx, ctypes.byref(prm), wp, fp, ww.size, op), "bis_inject")
         return out
 
+    def phoenix_interp(self, wave_vac, spec_low, spec_high, temp, t_low, t_high, wmin, wmax, f32=True):
+        """The spectrum at temp from the two bracketing PHOENIX spectra on the GPU
+        (phoenix_extraction.py:170-185): returns (wave_air, flux) of the pixels whose
+        vacuum wavelength is in [wmin - 500, wmax + 500] and air wavelength in [wmin, wmax];
+        f32: the spectra are float32 (the library's), their difference taken in float32."""
+        ww, wp = _d(wave_vac)
+        lo, lp = _d(spec_low)
+        hi, hp = _d(spec_high)
+        if not ww.shape == lo.shape == hi.shape:
+            raise SoapError("phoenix_interp: arrays differ in length")
+        prm = PhoenixParams(float(temp), float(t_low), float(t_high), float(wmin), float(wmax), int(bool(f32)))
+        air, ap = _out(ww.size)
+        flux, fp = _out(ww.size)
+        cnt = ctypes.c_int64()
+        self._chk(self.L.soap_phoenix_interp(self.ctx, ctypes.byref(prm), wp, lp, hp, ww.size, ap, fp,
+                                             ctypes.byref(cnt)), "phoenix_interp")
+        return air[:cnt.value].copy(), flux[:cnt.value].copy()
+
     def fit_ccf_host(self, v, y):
         vv, vp = _d(v)
         yy = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
@@ -528,3 +560,49 @@ class Batch:
         arr = (ctypes.c_int64 * self.n_stage)(*[int(grid.get(n, 0)) for n in self.stage_names])
         self._chk(self.L.soap_batch_set_config(self.b, int(taps_cells), int(fir_g), int(fir_b), int(fir_pre), arr,
                                                self.n_stage), "batch_set_config")
+
+
+# ------------------------------------------------------------ local PHOENIX files (host-only, no GPU)
+def phoenix_bracket(temp):
+    """(t_low, t_high): the library temperatures around temp (phoenix_extraction.py:136-146)."""
+    L = load()
+    lo, hi = ctypes.c_double(), ctypes.c_double()
+    rc = L.soap_phoenix_bracket(float(temp), ctypes.byref(lo), ctypes.byref(hi))
+    if rc != 0:
+        raise SoapError(f"phoenix_bracket({temp}) failed ({rc}): outside the library's temperature grid")
+    return lo.value, hi.value
+
+
+def phoenix_grid(logg, feh=0.0, alpha=0.0):
+    """The library grid values nearest to (logg, [Fe/H], [alpha/Fe])."""
+    L = load()
+    a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+    rc = L.soap_phoenix_grid(float(logg), float(feh), float(alpha), ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+    if rc != 0:
+        raise SoapError(f"phoenix_grid failed ({rc})")
+    return a.value, b.value, c.value
+
+
+def phoenix_name(temp, logg, feh, alpha):
+    """PHOENIX_name of phoenix_extraction.py:36-54: (subdir, '/lte0...fits')."""
+    L = load()
+    sd, nm = ctypes.create_string_buffer(64), ctypes.create_string_buffer(256)
+    rc = L.soap_phoenix_name(int(temp), float(logg), float(feh), float(alpha), sd, 64, nm, 256)
+    if rc != 0:
+        raise SoapError(f"phoenix_name failed ({rc})")
+    return sd.value.decode(), nm.value.decode()
+
+
+def fits_read(path):
+    """(data as float64, BITPIX) of the primary image of a FITS file."""
+    L = load()
+    n, bp = ctypes.c_int64(), ctypes.c_int32()
+    p = os.fsencode(path)
+    rc = L.soap_fits_read(p, None, 0, ctypes.byref(n), ctypes.byref(bp))
+    if rc != 0:
+        raise SoapError(f"fits_read({path}) failed ({rc})")
+    out = np.empty(n.value)
+    rc = L.soap_fits_read(p, out.ctypes.data_as(_dp), n.value, ctypes.byref(n), ctypes.byref(bp))
+    if rc != 0:
+        raise SoapError(f"fits_read({path}) failed ({rc})")
+    return out, bp.value
