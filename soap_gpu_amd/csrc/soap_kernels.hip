@@ -478,7 +478,6 @@ __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlan
 template <int G, bool PRE>
 __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job, int n_tiles, int n_groups, int bx_) {
   extern __shared__ double rows[];  // [4][row_len]: quiet, quiet SED, spot, facula rows of one bin
-  __shared__ int s_fl[G][64];
   __shared__ int s_bins[64], s_ufl[64], s_ulo[64], s_uhi[64];
   __shared__ int s_nb;
   // 32-bit block arithmetic (block counts are < 2^31): keeps the division short
@@ -493,38 +492,38 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
   const GLOBAL int32_t *kflags = gl(a.kflags), *krange = gl(a.krange);
   long j0 = (long)tile * kFirTile;
   long src0 = j0 - a.dmin - nd + 1;
-  // group metadata: per-phase flags, union flags and union tap range per bin
-  for (int e = threadIdx.x; e < G * 64; e += blockDim.x) {
-    int g = e / 64, k = e % 64;
-    int fl = 0;
-    if (g < ng && k < n_mu) {
-      long jk = (long)(p0 + g) * n_mu + k;
-      fl = kflags[jk];
-      if (krange[jk * 2] > krange[jk * 2 + 1]) fl = 0;
-    }
-    s_fl[g][k] = fl;
-  }
-  __syncthreads();
+  // group metadata, in the first wave: lane k (mu bin k) loads the flags and tap
+  // ranges of the group's phases (all loads issued together), forms the union flags
+  // and range, and the bins in use are listed in order through a ballot
   if (threadIdx.x < 64) {
-    int k = threadIdx.x;
+    const int k = threadIdx.x;
+    int fl[G], lo[G], hi[G];
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      fl[g] = 0;
+      lo[g] = 0;
+      hi[g] = -1;
+      if (g < ng && k < n_mu) {
+        const long jk = (long)(p0 + g) * n_mu + k;
+        fl[g] = kflags[jk];
+        lo[g] = krange[jk * 2];
+        hi[g] = krange[jk * 2 + 1];
+      }
+    }
     int ufl = 0, ulo = nd, uhi = -1;
-    for (int g = 0; g < ng; g++)
-      if (s_fl[g][k]) {
-        long jk = (long)(p0 + g) * n_mu + k;
-        ufl |= s_fl[g][k] | 4;
-        ulo = min(ulo, krange[jk * 2]);
-        uhi = max(uhi, krange[jk * 2 + 1]);
+#pragma unroll
+    for (int g = 0; g < G; g++)
+      if (fl[g] && lo[g] <= hi[g]) {
+        ufl |= fl[g] | 4;
+        ulo = min(ulo, lo[g]);
+        uhi = max(uhi, hi[g]);
       }
     s_ufl[k] = ufl;
     s_ulo[k] = ulo;
     s_uhi[k] = uhi;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    int c = 0;
-    for (int k = 0; k < n_mu; k++)
-      if (s_ufl[k]) s_bins[c++] = k;
-    s_nb = c;
+    const unsigned long long used = __ballot(ufl != 0);
+    if (ufl) s_bins[__popcll(used & ((1ull << k) - 1ull))] = k;
+    if (k == 0) s_nb = __popcll(used);
   }
   __syncthreads();
   double F[G][7];
