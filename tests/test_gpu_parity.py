@@ -194,6 +194,23 @@ def test_ccf_parity_isolated(case, request):
     np.testing.assert_allclose(g["ccf"], ref["ccf"], rtol=1e-12, atol=1e-13)
 
 
+def test_ccf_long_footprints_segmented(engine):
+    """Line footprints longer than the CCF's 512-pixel LDS segment: 0.06 km/s pixels,
+    so the 40 km/s velocity span covers ~670 pixels and windows straddle segments.
+    The staged CCF against the oracle's CCF of the GPU's lower spectra, and the
+    series (spec_norm and the CCF in one launch) against the staged path bit for bit."""
+    w = synth.small(grid=24, n=60000, n_phase=2, n_lines=40)
+    h = np.log(w.wave[1] / w.wave[0])
+    assert (w.vgrid[-1] - w.vgrid[0]) / (299792458.0 * h) > 600
+    g = run_engine(engine, w)
+    ref = ccf_stage(w, g["lower"])
+    np.testing.assert_allclose(g["ccf"], ref["ccf"], rtol=1e-12, atol=1e-13)
+    engine.run_series(w.star, w.sigma_inst)
+    np.testing.assert_array_equal(engine.get_ccf(), g["ccf"])
+    for t in range(3):
+        np.testing.assert_array_equal(engine.get_lower(0, t), g["lower"][0, t])
+
+
 def test_fit_parity_on_same_ccfs(engine, config0_case):
     """Batched GPU fit vs the oracle fit on identical CCFs."""
     w, g, o = config0_case
