@@ -478,7 +478,7 @@ __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlan
 template <int G, bool PRE>
 __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job, int n_tiles, int n_groups, int bx_) {
   extern __shared__ double rows[];  // [4][row_len]: quiet, quiet SED, spot, facula rows of one bin
-  __shared__ int s_bins[64], s_ufl[64], s_ulo[64], s_uhi[64];
+  __shared__ int4 s_meta[64];  // bins in use, in order: (bin, union flags, union tap range)
   __shared__ int s_nb;
   // 32-bit block arithmetic (block counts are < 2^31): keeps the division short
   const int xcd = bx_ & 7, slot = bx_ >> 3;
@@ -518,11 +518,8 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
         ulo = min(ulo, lo[g]);
         uhi = max(uhi, hi[g]);
       }
-    s_ufl[k] = ufl;
-    s_ulo[k] = ulo;
-    s_uhi[k] = uhi;
     const unsigned long long used = __ballot(ufl != 0);
-    if (ufl) s_bins[__popcll(used & ((1ull << k) - 1ull))] = k;
+    if (ufl) s_meta[__popcll(used & ((1ull << k) - 1ull))] = make_int4(k, ufl, ulo, uhi);
     if (k == 0) s_nb = __popcll(used);
   }
   __syncthreads();
@@ -541,8 +538,9 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
     return gl(r == 0 ? a.q : r == 1 ? a.qs : (r == 2 && has_sp) ? a.sp : a.fa) + (long)k * a.n;
   };
   auto prefetch = [&](int kk) {
-    const int k = uni(s_bins[kk]);
-    const int ufl = uni(s_ufl[k]);
+    const int4 m = s_meta[kk];
+    const int k = uni(m.x);
+    const int ufl = uni(m.y);
     const int has_sp = ufl & 1, nrow = 2 + has_sp + ((ufl >> 1) & 1);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
@@ -558,8 +556,9 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
   };
   if (PRE && nb > 0) prefetch(0);
   for (int kk = 0; kk < nb; kk++) {
-    const int k = uni(s_bins[kk]);
-    const int ufl = uni(s_ufl[k]), ulo = uni(s_ulo[k]), uhi = uni(s_uhi[k]);
+    const int4 m = s_meta[kk];
+    const int k = uni(m.x);
+    const int ufl = uni(m.y), ulo = uni(m.z), uhi = uni(m.w);
     if (kk > 0) __syncthreads();
     // rows present: quiet, quiet SED, then spot (if any spot cell) and facula (if any)
     const int has_sp = ufl & 1, has_fa = (ufl >> 1) & 1;
