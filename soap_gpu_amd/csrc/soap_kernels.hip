@@ -492,6 +492,11 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
   const GLOBAL int32_t *kflags = gl(a.kflags), *krange = gl(a.krange);
   long j0 = (long)tile * kFirTile;
   long src0 = j0 - a.dmin - nd + 1;
+  // the spot and facula continua of this pixel, for the epilogue (loaded now, so
+  // that they arrive during the bins)
+  const long j = j0 + threadIdx.x;
+  const double cs = (j < a.n && a.sps) ? gl(a.sps)[j] : 0.0;
+  const double cf = (j < a.n && a.fas) ? gl(a.fas)[j] : 0.0;
   // group metadata, in the first wave: lane k (mu bin k) loads the flags and tap
   // ranges of the group's phases (all loads issued together), forms the union flags
   // and range, and the bins in use are listed in order through a ballot
@@ -617,10 +622,7 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
       }
     }
   }
-  long j = j0 + threadIdx.x;
   if (j < a.n) {
-    double cs = a.sps ? gl(a.sps)[j] : 0.0;
-    double cf = a.fas ? gl(a.fas)[j] : 0.0;
 #pragma unroll
     for (int g = 0; g < G; g++) {
       if (g >= ng) break;
