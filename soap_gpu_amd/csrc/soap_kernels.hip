@@ -636,7 +636,7 @@ __device__ __forceinline__ void fir_active_grp_body(const FirArgs &a, int n_job,
 }
 
 template <bool PRE>
-__global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_fir_active_grp(FirArgs a, int n_job, int n_tiles, int n_groups) {
+__global__ __launch_bounds__(kTPB) void k_fir_active_grp(FirArgs a, int n_job, int n_tiles, int n_groups) {
   fir_active_grp_body<kFirG, PRE>(a, n_job, n_tiles, n_groups, blockIdx.x);
 }
 
