@@ -1595,8 +1595,11 @@ __device__ __forceinline__ void ccf_lds_body(const double *spec_, int64_t spec_s
       }
     }
   }
-  if (act)
-    for (int k = 0; k < ns; k++) part[((long)chunk * n_spec + s0 + k) * n_v + iv] = acc[k];
+  if (act) {
+#pragma unroll
+    for (int k = 0; k < kCcfSpec; k++)  // fixed trip count: the accumulators stay in registers
+      if (k < ns) part[((long)chunk * n_spec + s0 + k) * n_v + iv] = acc[k];
+  }
 }
 
 __global__ void k_ccf_lds(const double *spec, int64_t spec_stride, int64_t trim, int n_spec, int n_lines, int n_v,
@@ -2111,7 +2114,7 @@ __global__ __launch_bounds__(64) void kb_finalize(SOAP_BATCH_ARGS(FinalizeJob)) 
 // (the row-prefetching form holds 16 more registers per thread)
 template <bool P, int G, bool PRE>
 struct FirWaves {
-  static constexpr int v = PRE ? (G == 2 ? 6 : 4) : G == 2 ? 8 : G == 4 ? (P ? 5 : 6) : 4;
+  static constexpr int v = PRE ? (G == 2 ? (P ? 4 : 6) : G == 4 ? 4 : 3) : G == 2 ? (P ? 7 : 8) : G == 4 ? (P ? 4 : 6) : 4;
 };
 
 template <bool P, int G, bool PRE>
