@@ -87,3 +87,23 @@ def test_linear_solar_mode(engine):
             if np.max(np.abs(ref[t])) == 0:
                 continue
             assert normwise(got[t], ref[t]) <= SPEC_TOL, (p, t)
+
+
+def test_linear_series_equals_staged(linear_case):
+    """soap_run_series on a linear grid (spec_norm and the CCF in one launch) against
+    the stage-by-stage outputs of the same workload, bit for bit."""
+    from soap_gpu_amd.engine import Engine
+    w, g, o = linear_case
+    e = Engine(0)
+    try:
+        setup_workload(e, w)
+        e.run_series(w.star, w.sigma_inst)
+        P = len(w.phases)
+        lower = np.stack([np.stack([e.get_lower(p, t) for t in range(3)]) for p in list(range(P)) + [-1]])
+        np.testing.assert_array_equal(lower, g["lower"])
+        np.testing.assert_array_equal(e.get_ccf(), g["ccf"])
+        fit = e.get_fit()
+        for k in fit:
+            np.testing.assert_array_equal(fit[k], g["fit"][k])
+    finally:
+        e.close()
