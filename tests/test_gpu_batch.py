@@ -85,6 +85,36 @@ def test_batch_eager_equals_plain(batch_case):
     _check_all(ws, engines, refs, "eager")
 
 
+def test_batch_ccf_mixed_footprints():
+    """One slow and one fast rotator in a batch: the fast one's velocity window
+    (+-100 km/s at 0.3 km/s pixels) gives line footprints past the CCF's 512-pixel
+    LDS segment, the slow one's fit in one; both equal their plain runs."""
+    rng = np.random.default_rng(41)
+    ws = [synth.batch_instance(rng, 3000, 24, 3, 2, 3.0, 60.0, name="slow", n_mu=6),
+          synth.batch_instance(rng, 3000, 24, 2, 2, 40.0, 80.0, name="fast", n_mu=6)]
+    h = np.log(ws[1].wave[1] / ws[1].wave[0])
+    assert (ws[1].vgrid[-1] - ws[1].vgrid[0]) / (299792458.0 * h) > 600
+    refs = []
+    for w in ws:
+        r = _prepare(w)
+        r.integrate()
+        r.resolution(w.sigma_inst)
+        r.ccf()
+        r.fit()
+        refs.append(_outputs(r, w))
+        r.close()
+    engines = [_prepare(w) for w in ws]
+    b = Batch(engines, [w.sigma_inst for w in ws])
+    try:
+        b.step(graph=False)
+        b.sync()
+        _check_all(ws, engines, refs, "mixed footprints")
+    finally:
+        b.close()
+        for e in engines:
+            e.close()
+
+
 def test_batch_graph_replays(batch_case):
     """Graph replays, each after eager work that overwrites every output."""
     ws, engines, b, refs = batch_case
