@@ -25,6 +25,25 @@ def test_library_exports_every_symbol():
     assert not missing, missing
 
 
+def test_header_is_plain_c(tmp_path):
+    """The boundary is C: the header compiles as C99 with gcc -pedantic (no C++ or
+    HIP types), every declared struct usable from C."""
+    import shutil
+    import subprocess
+    import pytest
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    src = tmp_path / "use_header.c"
+    src.write_text('#include "soap_gpu.h"\n'
+                   "int main(void) { soap_star s; soap_activity a; soap_region r; soap_diff_params d;\n"
+                   "  soap_bis_inject_params b; soap_phoenix_params p;\n"
+                   "  (void)s; (void)a; (void)r; (void)d; (void)b; (void)p; return 0; }\n")
+    r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-I",
+                        os.path.join(ROOT, "include"), "-c", str(src), "-o", str(tmp_path / "use_header.o")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_version_string_without_gpu():
     L = E.load()
     assert b"gfx950" in L.soap_version()
