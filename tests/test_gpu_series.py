@@ -36,10 +36,12 @@ def _same(a, b, what):
 # an odd pixel count puts every odd spectrum off the 16-byte boundary, which
 # spec_norm (its own launch when staged, a part of the CCF launch in the series)
 # handles with a one-pixel head; spectrum counts 114, 51 and 18 leave a partial
-# group of 8 in the CCF
-@pytest.mark.parametrize("n_phase,n", [(37, 6000), (16, 6000), (5, 6001)])
-def test_series_pipeline_equals_staged_calls(n_phase, n):
-    w = synth.batch_instance(np.random.default_rng(29), n, 64, n_phase, 3, 6.0, 70.0, name="series", n_mu=8)
+# group of 8 in the CCF; vsini 40 km/s widens the velocity window to +-100 km/s
+# (401 velocities: 448-thread blocks, line footprints past the 512-pixel LDS
+# segment of the CCF)
+@pytest.mark.parametrize("n_phase,n,vsini", [(37, 6000, 6.0), (16, 6000, 6.0), (5, 6001, 6.0), (4, 5000, 40.0)])
+def test_series_pipeline_equals_staged_calls(n_phase, n, vsini):
+    w = synth.batch_instance(np.random.default_rng(29), n, 64, n_phase, 3, vsini, 70.0, name="series", n_mu=8)
     staged = Engine(0)
     setup_workload(staged, w)
     staged.disk_init(w.star)
