@@ -68,6 +68,8 @@ def lib():
                                            _dp, _dp, _ip, _dp, _dp, _dp] + [_dp] * 6
                                           + [ctypes.c_int] * 4 + [_dp, _dp, _dp, ctypes.c_int])
         L.orc_conv_window.argtypes = [_dp, ctypes.c_double]
+        L.orc_conv_raw.argtypes = [ctypes.c_int, _dp, ctypes.c_long, ctypes.c_double, _dp, ctypes.c_int,
+                                   ctypes.c_double, _dp, ctypes.c_int]
         L.orc_lower_resolution.argtypes = [ctypes.c_int, _dp, ctypes.c_long, ctypes.c_double, _dp, _dp,
                                            ctypes.c_int, ctypes.c_double, _dp, _dp, ctypes.c_int]
         L.orc_select_lines.restype = ctypes.c_long
@@ -77,6 +79,7 @@ def lib():
         L.orc_appodisation.argtypes = [ctypes.c_int, _dp]
         L.orc_ccf_finish.argtypes = [ctypes.c_int, _dp, _dp]
         L.orc_fit_ccf.argtypes = [_dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]
+        L.orc_fit_ccf_err.argtypes = [_dp, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _ip]
         L.orc_gauss_hess.argtypes = [_dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
         L.orc_gauss_polish.argtypes = [_dp, _dp, ctypes.c_int, _dp]
         _d_ = ctypes.c_double
@@ -245,6 +248,16 @@ def conv_window(wave, sigma_ms):
     return lib().orc_conv_window(wp, float(sigma_ms))
 
 
+def conv_raw(kind, wave, sigma_ms, A, variant, weight=0.005, nthreads=None):
+    """conv_inst alone: the unnormalised convolution of 1 - A (zero outside the valid range)."""
+    w, wp = _d(wave)
+    a, ap = _d(A)
+    out = np.empty(len(w))
+    lib().orc_conv_raw(int(kind), wp, len(w), float(sigma_ms), ap, int(variant), float(weight),
+                       out.ctypes.data_as(_dp), nthreads or nthreads_default())
+    return out
+
+
 def lower_resolution(kind, wave, sigma_ms, quiet, spec, variant, weight=0.005, nthreads=None):
     w, wp = _d(wave)
     q, qp = _d(quiet if quiet is not None else spec)
@@ -293,7 +306,8 @@ def ccf_finish(ccf_raw, appod):
 
 
 # ------------------------------------------------------------------- a9
-def fit_ccf(v, y):
+def fit_ccf(v, y, err=None):
+    """BIS_FIT2's fit + bisector; err: per-point errors (CCF_error.txt), None = ones."""
     vv, vp = _d(v)
     yy, yp = _d(y)
     n = len(vv)
@@ -302,8 +316,12 @@ def fit_ccf(v, y):
     depth = np.zeros(91)
     bis = np.zeros(91)
     it = np.zeros(1, dtype=np.int32)
-    st = lib().orc_fit_ccf(vp, yp, n, par.ctypes.data_as(_dp), model.ctypes.data_as(_dp),
-                           depth.ctypes.data_as(_dp), bis.ctypes.data_as(_dp), it.ctypes.data_as(_ip))
+    if err is None:
+        ep = None
+    else:
+        ee, ep = _d(err)
+    st = lib().orc_fit_ccf_err(vp, yp, ep, n, par.ctypes.data_as(_dp), model.ctypes.data_as(_dp),
+                               depth.ctypes.data_as(_dp), bis.ctypes.data_as(_dp), it.ctypes.data_as(_ip))
     return dict(par=par, model=model, depth=depth, bis=bis, iters=int(it[0]), status=int(st))
 
 

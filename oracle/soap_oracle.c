@@ -540,23 +540,20 @@ int orc_conv_window(const double *wave, double sigma_ms) {
   return (int)orc_rndup((window_width * orc_delta_lambda_cpu(sigma_ms, wave[0])) / wavelength_step, 0);
 }
 
-/* lower_resolution_gpu.
- *  variant 0: quiet-star form, SOAP_initialize.cu:446-505 (A = spec, weight = argument)
- *  variant 1: active form, SOAP_resolution.cu:235-310 (A = quiet - spec, weight = 0.005)
- * Linear grids use conv_inst literally (SOAP_initialize.cu:78-96, SOAP_resolution.cu:76-95).
+/* conv_inst (SOAP_initialize.cu:78-96, SOAP_resolution.cu:76-95): the
+ * unnormalised output o = sum_{m=i-w}^{i+w-2} weight (1 - A[m]) G_i(m - i),
+ * zero outside w < i < n-1-w (A = spec for variant 0, quiet - spec for
+ * variant 1; weight = the argument for variant 0, 0.005 for variant 1).
+ * Linear grids use conv_inst literally, sigma_i = Delta_lambda(sigma, lambda_i).
  * Log grids replace the hard-coded 0.005 A pixel by the exact geometry of the
  * grid: offset lambda_{i+d} - lambda_i = lambda_i expm1(d h), sigma_i = lambda_i fs,
  * quadrature weight lambda_{i+d} expm1(h); every factor lambda_i cancels and the
- * tap is c_d = exp(d h) expm1(h)/(fs sqrt(2 pi)) exp(-(expm1(d h)/fs)^2/2).
- * minmax2 (optional) receives [min, max] as used by spec_norm. */
-void orc_lower_resolution(int kind, const double *wave, long n, double sigma_ms,
-                          const double *quiet, const double *spec, int variant, double weight,
-                          double *out, double *minmax2, int nthreads) {
+ * tap is c_d = exp(d h) expm1(h)/(fs sqrt(2 pi)) exp(-(expm1(d h)/fs)^2/2). */
+void orc_conv_raw(int kind, const double *wave, long n, double sigma_ms, const double *A, int variant,
+                  double weight, double *o, int nthreads) {
   const double pi = 3.141592653589793238463;
   int wid = orc_conv_window(wave, sigma_ms);
-  double *A = (double *)malloc(sizeof(double) * n);
-  double *o = (double *)calloc((size_t)n, sizeof(double));
-  for (long m = 0; m < n; m++) A[m] = variant == 0 ? spec[m] : quiet[m] - spec[m];
+  for (long i = 0; i < n; i++) o[i] = 0.0;
   double w = variant == 0 ? weight : 0.005;
   double *ctap = NULL;
   if (kind == ORC_GRID_LOG) {
@@ -583,6 +580,22 @@ void orc_lower_resolution(int kind, const double *wave, long n, double sigma_ms,
     }
     o[i] = acc;
   }
+  free(ctap);
+}
+
+/* lower_resolution_gpu: conv_inst, then spec_norm with the min of A and the max
+ * of the non-zero outputs.
+ *  variant 0: quiet-star form, SOAP_initialize.cu:446-505 (A = spec, weight = argument)
+ *  variant 1: active form, SOAP_resolution.cu:235-310 (A = quiet - spec, weight = 0.005)
+ * minmax2 (optional) receives [min, max] as used by spec_norm. */
+void orc_lower_resolution(int kind, const double *wave, long n, double sigma_ms,
+                          const double *quiet, const double *spec, int variant, double weight,
+                          double *out, double *minmax2, int nthreads) {
+  if (n <= 0) return;
+  double *A = (double *)malloc(sizeof(double) * n);
+  double *o = (double *)calloc((size_t)n, sizeof(double));
+  for (long m = 0; m < n; m++) A[m] = variant == 0 ? spec[m] : quiet[m] - spec[m];
+  orc_conv_raw(kind, wave, n, sigma_ms, A, variant, weight, o, nthreads);
   /* reduceMinIdxOptimizedShared over A, reduceMax over o with the -60000 seed
    * and zero filter: SOAP_resolution.cu:123-214 */
   double mn = A[0];
@@ -598,7 +611,6 @@ void orc_lower_resolution(int kind, const double *wave, long n, double sigma_ms,
   }
   free(A);
   free(o);
-  free(ctap);
 }
 
 /* ---------------------------------------------------------------- a8: CCF */
@@ -713,21 +725,24 @@ void orc_ccf_finish(int n_v, const double *appod, double *ccf) {
 
 /* --------------------------------------------------- a9: Gaussian + BIS */
 
-/* gauss_f / gauss_df, BIS_FIT2.cpp:31-81 (err = 1) */
-static void orc_gauss(const double *px, const double *y, int n, const double *v, double *f, double *fit,
-                      double (*J)[4]) {
+/* gauss_f / gauss_df, BIS_FIT2.cpp:31-81: residual (fit - y)/err, Jacobian rows
+ * divided by err; err NULL = unit errors (the driver writes ones,
+ * run_all_SOAP.py:317) */
+static void orc_gauss(const double *px, const double *y, const double *err, int n, const double *v, double *f,
+                      double *fit, double (*J)[4]) {
   double c = v[0], k = v[1], x0 = v[2], fwhm = v[3];
   double sigma = fwhm / 2 / sqrt(2 * log(2));
   for (int i = 0; i < n; i++) {
+    double ei = err ? err[i] : 1.0;
     double e = exp(-(px[i] - x0) * (px[i] - x0) / 2 / sigma / sigma);
     double fi = c + k * e;
     if (fit) fit[i] = fi;
-    if (f) f[i] = (fi - y[i]) / 1.0;
+    if (f) f[i] = (fi - y[i]) / ei;
     if (J) {
-      J[i][0] = 1 / 1.0;
-      J[i][1] = e / 1.0;
-      J[i][2] = k / 1.0 * e * (px[i] - x0) / sigma / sigma;
-      J[i][3] = k / 1.0 * e * (px[i] - x0) * (px[i] - x0) / sigma / sigma / sigma / 2 / sqrt(2 * log(2));
+      J[i][0] = 1 / ei;
+      J[i][1] = e / ei;
+      J[i][2] = k / ei * e * (px[i] - x0) / sigma / sigma;
+      J[i][3] = k / ei * e * (px[i] - x0) * (px[i] - x0) / sigma / sigma / sigma / 2 / sqrt(2 * log(2));
     }
   }
 }
@@ -832,8 +847,8 @@ double orc_bisector(const double *v, const double *y, int n, double c, double k,
  * GSL's xtol/gtol tests (xtol = gtol = 1e-8, ftol = 0, max 10000 iterations).
  * par5 = [c, |k|, span, v0, fwhm]; model = fit at the final parameters;
  * depth91 = 1 - |k| depth; bis91 = bisector. */
-int orc_fit_ccf(const double *v, const double *y, int n, double *par5, double *model, double *depth91,
-                double *bis91, int *n_iter) {
+int orc_fit_ccf_err(const double *v, const double *y, const double *err, int n, double *par5, double *model,
+                    double *depth91, double *bis91, int *n_iter) {
   const double xtol = 1.0e-8, gtol = 1.0e-8;
   const int max_iter = 10000;
   double *px = (double *)malloc(sizeof(double) * n);
@@ -850,7 +865,7 @@ int orc_fit_ccf(const double *v, const double *y, int n, double *par5, double *m
       x0 = px[i];
     }
   double x[4] = {c, k, x0, fwhm};
-  orc_gauss(px, y, n, x, f, NULL, J);
+  orc_gauss(px, y, err, n, x, f, NULL, J);
   double D[4];
   for (int j = 0; j < 4; j++) {
     double s = 0;
@@ -882,7 +897,7 @@ int orc_fit_ccf(const double *v, const double *y, int n, double *par5, double *m
         continue;
       }
       for (int a = 0; a < 4; a++) xt[a] = x[a] + dx[a];
-      orc_gauss(px, y, n, xt, ft, NULL, NULL);
+      orc_gauss(px, y, err, n, xt, ft, NULL, NULL);
       double ft2 = 0;
       for (int i = 0; i < n; i++) ft2 += ft[i] * ft[i];
       double pred = 0;
@@ -909,7 +924,7 @@ int orc_fit_ccf(const double *v, const double *y, int n, double *par5, double *m
       break;
     }
     for (int a = 0; a < 4; a++) x[a] = xt[a];
-    orc_gauss(px, y, n, x, f, NULL, J);
+    orc_gauss(px, y, err, n, x, f, NULL, J);
     for (int j = 0; j < 4; j++) {
       double s = 0;
       for (int i = 0; i < n; i++) s += J[i][j] * J[i][j];
@@ -941,7 +956,7 @@ int orc_fit_ccf(const double *v, const double *y, int n, double *par5, double *m
   k = x[1];
   x0 = x[2];
   fwhm = x[3];
-  if (model) orc_gauss(px, y, n, x, NULL, model, NULL);
+  if (model) orc_gauss(px, y, err, n, x, NULL, model, NULL);
 
   double span = orc_bisector(v, y, n, c, k, x0 + v[0], fwhm, depth91, bis91);
   par5[0] = c;
@@ -954,6 +969,11 @@ int orc_fit_ccf(const double *v, const double *y, int n, double *par5, double *m
   free(ft);
   free(J);
   return status;
+}
+
+int orc_fit_ccf(const double *v, const double *y, int n, double *par5, double *model, double *depth91,
+                double *bis91, int *n_iter) {
+  return orc_fit_ccf_err(v, y, NULL, n, par5, model, depth91, bis91, n_iter);
 }
 
 /* ----------------------------------- a11: differentiable model (no reference counterpart) */

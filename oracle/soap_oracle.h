@@ -74,6 +74,8 @@ void orc_integrate_phase(int kind, const double *wave, long n, double step, int 
 
 /* a7: instrumental convolution + normalisation */
 int orc_conv_window(const double *wave, double sigma_ms);
+void orc_conv_raw(int kind, const double *wave, long n, double sigma_ms, const double *A, int variant,
+                  double weight, double *o, int nthreads);
 void orc_lower_resolution(int kind, const double *wave, long n, double sigma_ms,
                           const double *quiet, const double *spec, int variant, double weight,
                           double *out, double *minmax2, int nthreads);
@@ -90,6 +92,9 @@ void orc_ccf_finish(int n_v, const double *appod, double *ccf);
 /* a9: Gaussian fit + bisector (BIS_FIT2) */
 int orc_fit_ccf(const double *v, const double *y, int n, double *par5, double *model, double *depth91,
                 double *bis91, int *n_iter);
+/* the same with per-point errors err[n] (CCF_error.txt), NULL = unit errors */
+int orc_fit_ccf_err(const double *v, const double *y, const double *err, int n, double *par5, double *model,
+                    double *depth91, double *bis91, int *n_iter);
 
 /* a11: differentiable model (no reference counterpart): Gaussian-fit Hessian,
  * Newton polish, forward model and loss (definitions in soap_oracle.c) */
