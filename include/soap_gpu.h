@@ -178,6 +178,12 @@ int soap_get_fit(soap_ctx *ctx, double *par, double *model, double *depth, doubl
  * in run_all_SOAP.py:316-394): v[n_v], y[n_ccf*n_v] */
 int soap_fit_ccf_host(soap_ctx *ctx, const double *v, int32_t n_v, const double *y, int32_t n_ccf,
                       double *par, double *model, double *depth, double *bis);
+/* the same with per-point errors (CCF_error.txt, BIS_FIT2.cpp:114-119): residuals
+ * (fit - y)/err and Jacobian rows divided by err (BIS_FIT2.cpp:49,74-77);
+ * err[n_ccf*n_v], NULL = unit errors.  Errors must be finite and non-zero
+ * (SOAP_ERR_ARG otherwise; the reference would divide by zero). */
+int soap_fit_ccf_host_err(soap_ctx *ctx, const double *v, int32_t n_v, const double *y, const double *err,
+                          int32_t n_ccf, double *par, double *model, double *depth, double *bis);
 
 /* ------------------------------------------------------------ seed preprocessing: bisector injection */
 /* The convective-blueshift injection of CB_injection_faculae.py:191-244 on one

@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <cstdarg>
 #include <cstdio>
@@ -1680,7 +1681,7 @@ int soap_fit(soap_ctx *c) {
   if (rc) return rc;
   {
     StageTimer tm(c, ST_FIT);
-    HIPCHK(c, launch_fit(c->d_vgrid, c->n_v, c->d_ccf, n_ccf, (double *)c->f_par.p, (double *)c->f_model.p,
+    HIPCHK(c, launch_fit(c->d_vgrid, c->n_v, c->d_ccf, nullptr, n_ccf, (double *)c->f_par.p, (double *)c->f_model.p,
                          (double *)c->f_depth.p, (double *)c->f_bis.p, (int32_t *)c->f_iter.p, c->stream));
   }
   c->have_fit = true;
@@ -1794,29 +1795,40 @@ int soap_phoenix_interp(soap_ctx *c, const soap_phoenix_params *p, const double 
   return SOAP_OK;
 }
 
-int soap_fit_ccf_host(soap_ctx *c, const double *v, int32_t n_v, const double *y, int32_t n_ccf, double *par,
-                      double *model, double *depth, double *bis) {
+int soap_fit_ccf_host_err(soap_ctx *c, const double *v, int32_t n_v, const double *y, const double *err,
+                          int32_t n_ccf, double *par, double *model, double *depth, double *bis) {
   if (!c || !v || !y || n_v < 3 || n_v > 1024 || n_ccf < 1 || !par) return fail(c, SOAP_ERR_ARG, "fit_ccf_host: bad arguments");
+  if (err)
+    for (size_t i = 0; i < (size_t)n_ccf * n_v; i++)
+      if (!std::isfinite(err[i]) || err[i] == 0.0)
+        return fail(c, SOAP_ERR_ARG, "fit_ccf_host: error %zu is %g (must be finite and non-zero)", i, err[i]);
   HIPCHK(c, hipSetDevice(c->device));
-  double *dv = nullptr, *dy = nullptr, *dpar = nullptr, *dmod = nullptr, *ddep = nullptr, *dbis = nullptr;
+  double *dv = nullptr, *dy = nullptr, *de = nullptr, *dpar = nullptr, *dmod = nullptr, *ddep = nullptr, *dbis = nullptr;
   int rc = SOAP_OK;
-  if ((rc = dalloc(c, &dv, n_v)) || (rc = dalloc(c, &dy, (size_t)n_ccf * n_v)) || (rc = dalloc(c, &dpar, (size_t)n_ccf * 5)) ||
+  if ((rc = dalloc(c, &dv, n_v)) || (rc = dalloc(c, &dy, (size_t)n_ccf * n_v)) ||
+      (err && (rc = dalloc(c, &de, (size_t)n_ccf * n_v))) || (rc = dalloc(c, &dpar, (size_t)n_ccf * 5)) ||
       (rc = dalloc(c, &dmod, (size_t)n_ccf * n_v)) || (rc = dalloc(c, &ddep, (size_t)n_ccf * 91)) ||
       (rc = dalloc(c, &dbis, (size_t)n_ccf * 91))) {
-    for (double *p : {dv, dy, dpar, dmod, ddep, dbis}) dfree_c(c, p);
+    for (double *p : {dv, dy, de, dpar, dmod, ddep, dbis}) dfree_c(c, p);
     return rc;
   }
   hipError_t e = hipMemcpyAsync(dv, v, n_v * 8, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(dy, y, (size_t)n_ccf * n_v * 8, hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) e = launch_fit(dv, n_v, dy, n_ccf, dpar, dmod, ddep, dbis, nullptr, c->stream);
+  if (e == hipSuccess && err) e = hipMemcpyAsync(de, err, (size_t)n_ccf * n_v * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = launch_fit(dv, n_v, dy, de, n_ccf, dpar, dmod, ddep, dbis, nullptr, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(par, dpar, (size_t)n_ccf * 5 * 8, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess && model) e = hipMemcpyAsync(model, dmod, (size_t)n_ccf * n_v * 8, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess && depth) e = hipMemcpyAsync(depth, ddep, (size_t)n_ccf * 91 * 8, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess && bis) e = hipMemcpyAsync(bis, dbis, (size_t)n_ccf * 91 * 8, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  for (double *p : {dv, dy, dpar, dmod, ddep, dbis}) dfree_c(c, p);
+  for (double *p : {dv, dy, de, dpar, dmod, ddep, dbis}) dfree_c(c, p);
   if (e != hipSuccess) return fail(c, SOAP_ERR_HIP, "fit_ccf_host: %s", hipGetErrorString(e));
   return SOAP_OK;
+}
+
+int soap_fit_ccf_host(soap_ctx *c, const double *v, int32_t n_v, const double *y, int32_t n_ccf, double *par,
+                      double *model, double *depth, double *bis) {
+  return soap_fit_ccf_host_err(c, v, n_v, y, nullptr, n_ccf, par, model, depth, bis);
 }
 
 // ------------------------------------------------------------------ a11: differentiable path
@@ -1959,7 +1971,7 @@ int soap_diff_forward(soap_ctx *c, const soap_diff_params *prm, const double *d_
   }
   {
     StageTimer tm(c, ST_FIT);
-    HIPCHK(c, launch_fit(c->d_vgrid, c->n_v, (const double *)d.ccf.p, d.P, (double *)d.par.p, (double *)d.model.p,
+    HIPCHK(c, launch_fit(c->d_vgrid, c->n_v, (const double *)d.ccf.p, nullptr, d.P, (double *)d.par.p, (double *)d.model.p,
                          (double *)d.depth.p, (double *)d.bis.p, (int32_t *)d.iters.p, c->stream));
     HIPCHK(c, launch_fit_polish(c->d_vgrid, c->n_v, (const double *)d.ccf.p, (const double *)d.par.p, d.P,
                                 (double *)d.th.p, c->stream));
@@ -2691,7 +2703,7 @@ hipError_t dbatch_enqueue_one(soap_diffbatch::Item &it, hipStream_t st, bool adj
       e = launch_ccf_sum((const double *)c->ccf_part.p, ccf_chunks(c->n_lines), d.P, c->n_v, (double *)d.ccf.p,
                          st);
     if (e == hipSuccess)
-      e = launch_fit(c->d_vgrid, c->n_v, (const double *)d.ccf.p, d.P, (double *)d.par.p, (double *)d.model.p,
+      e = launch_fit(c->d_vgrid, c->n_v, (const double *)d.ccf.p, nullptr, d.P, (double *)d.par.p, (double *)d.model.p,
                      (double *)d.depth.p, (double *)d.bis.p, (int32_t *)d.iters.p, st);
     if (e == hipSuccess)
       e = launch_fit_polish(c->d_vgrid, c->n_v, (const double *)d.ccf.p, (const double *)d.par.p, d.P,

@@ -317,8 +317,9 @@ int ccf_chunks(int n_lines);
 // part: ccf_chunks(n_lines) * n_spec * n_v doubles; w_sorted[q] = weight of line sorted[q]
 hipError_t launch_ccf_reduce(const double *part, int nc, int n_spec, int n_v, const double *appod, const double *mm,
                              const double *wsum, double *ccf, hipStream_t s);
-hipError_t launch_fit(const double *v, int n_v, const double *y, int n_ccf, double *par, double *model,
-                      double *depth, double *bis, int32_t *iters, hipStream_t s);
+// err: per-point errors [n_ccf][n_v] or NULL (unit errors)
+hipError_t launch_fit(const double *v, int n_v, const double *y, const double *err, int n_ccf, double *par,
+                      double *model, double *depth, double *bis, int32_t *iters, hipStream_t s);
 // linear grids
 hipError_t launch_disk_cells_by_bin(int ncell, int n_mu, const double *limb, const double *delta,
                                     const int32_t *region, int count_only, int32_t *counts, const int32_t *offsets,

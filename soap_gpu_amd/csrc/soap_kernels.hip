@@ -1791,19 +1791,33 @@ struct GaussEval {
   double colsq[4];
 };
 
-// gauss_f / gauss_df of BIS_FIT2.cpp:31-81 (err = 1), lane-strided over points;
+// gauss_f / gauss_df of BIS_FIT2.cpp:31-81, lane-strided over points: residual
+// (fit - y) / err and Jacobian rows divided by err (BIS_FIT2.cpp:49,74-77); err
+// NULL = unit errors (the driver's CCF_error.txt of ones, run_all_SOAP.py:317).
 // with_jac = false evaluates only sum f^2.
-__device__ void gauss_eval(const double *px, const double *y, int n, const double *x, bool with_jac, GaussEval &r) {
+__device__ void gauss_eval(const double *px, const double *y, const double *err, int n, const double *x,
+                           bool with_jac, GaussEval &r) {
   double c = x[0], k = x[1], x0 = x[2], fwhm = x[3];
   double sigma = fwhm / 2 / sqrt(2 * log(2.0));
   double f2 = 0, A[10] = {0}, g[4] = {0};
   for (int i = threadIdx.x; i < n; i += 64) {
     double e = exp(-(px[i] - x0) * (px[i] - x0) / 2 / sigma / sigma);
-    double fi = (c + k * e) - y[i];
+    double fi = err ? ((c + k * e) - y[i]) / err[i] : (c + k * e) - y[i];
     f2 += fi * fi;
     if (with_jac) {
-      double J[4] = {1.0, e, k * e * (px[i] - x0) / sigma / sigma,
-                     k * e * (px[i] - x0) * (px[i] - x0) / sigma / sigma / sigma / 2 / sqrt(2 * log(2.0))};
+      double J[4];
+      if (err) {
+        double ei = err[i];
+        J[0] = 1 / ei;
+        J[1] = e / ei;
+        J[2] = k / ei * e * (px[i] - x0) / sigma / sigma;
+        J[3] = k / ei * e * (px[i] - x0) * (px[i] - x0) / sigma / sigma / sigma / 2 / sqrt(2 * log(2.0));
+      } else {
+        J[0] = 1.0;
+        J[1] = e;
+        J[2] = k * e * (px[i] - x0) / sigma / sigma;
+        J[3] = k * e * (px[i] - x0) * (px[i] - x0) / sigma / sigma / sigma / 2 / sqrt(2 * log(2.0));
+      }
       int q = 0;
       for (int a = 0; a < 4; a++) {
         g[a] += J[a] * fi;
@@ -1863,8 +1877,11 @@ __device__ int solve4(const double *A, const double *D, double mu, const double 
 // BIS_FIT2 (BIS_FIT2.cpp:97-320) for one CCF per 64-lane block: the same
 // Levenberg-Marquardt restatement of GSL's trust-region solver as the oracle
 // (More scaling, Nielsen damping, xtol = gtol = 1e-8), then the bisector.
-__device__ __forceinline__ void fit_body(const double *v_, int n, const double *Y_, double *par_, double *model_,
-                                         double *depth_out_, double *bis_out_, int32_t *iters_, int bx_) {
+// E_: per-point errors [n_ccf][n] (CCF_error.txt), NULL = unit errors; held in
+// s_p0 during the fit (s_p0 is bisector scratch afterwards).
+__device__ __forceinline__ void fit_body(const double *v_, int n, const double *Y_, const double *E_, double *par_,
+                                         double *model_, double *depth_out_, double *bis_out_, int32_t *iters_,
+                                         int bx_) {
   const GLOBAL double *v = gl(v_), *Y = gl(Y_);
   GLOBAL double *par = gw(par_), *model = gw(model_), *depth_out = gw(depth_out_), *bis_out = gw(bis_out_);
   GLOBAL int32_t *iters = gw(iters_);
@@ -1876,6 +1893,12 @@ __device__ __forceinline__ void fit_body(const double *v_, int n, const double *
   for (int i = threadIdx.x; i < n; i += 64) {
     s_px[i] = v[i] - v[0];
     s_y[i] = y[i];
+  }
+  const double *s_err = nullptr;
+  if (E_) {
+    const GLOBAL double *er = gl(E_) + (long)cidx * n;
+    for (int i = threadIdx.x; i < n; i += 64) s_p0[i] = er[i];
+    s_err = s_p0;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -1896,7 +1919,7 @@ __device__ __forceinline__ void fit_body(const double *v_, int n, const double *
   double x[4] = {s_init[0], s_init[1], s_init[2], s_init[3]};
   const double xtol = 1.0e-8, gtol = 1.0e-8;
   GaussEval ev;
-  gauss_eval(s_px, s_y, n, x, true, ev);
+  gauss_eval(s_px, s_y, s_err, n, x, true, ev);
   double D[4];
   for (int a = 0; a < 4; a++) D[a] = sqrt(ev.colsq[a]) > 0 ? sqrt(ev.colsq[a]) : 1.0;
   double mu = 1.0e-3, nu = 2.0;
@@ -1914,7 +1937,7 @@ __device__ __forceinline__ void fit_body(const double *v_, int n, const double *
       }
       for (int a = 0; a < 4; a++) xt[a] = x[a] + dx[a];
       GaussEval et;
-      gauss_eval(s_px, s_y, n, xt, false, et);
+      gauss_eval(s_px, s_y, s_err, n, xt, false, et);
       double pred = 0;
       for (int a = 0; a < 4; a++) {
         double Adx = 0;
@@ -1936,7 +1959,7 @@ __device__ __forceinline__ void fit_body(const double *v_, int n, const double *
     }
     if (!accepted) break;
     for (int a = 0; a < 4; a++) x[a] = xt[a];
-    gauss_eval(s_px, s_y, n, x, true, ev);
+    gauss_eval(s_px, s_y, s_err, n, x, true, ev);
     for (int a = 0; a < 4; a++)
       if (sqrt(ev.colsq[a]) > D[a]) D[a] = sqrt(ev.colsq[a]);
     bool small = true;
@@ -1966,7 +1989,7 @@ __device__ __forceinline__ void fit_body(const double *v_, int n, const double *
   double sigma = fwhm / 2. / sqrt(2. * log(2.));
   double v0 = x0 + v[0];
   for (int i = threadIdx.x; i < n; i += 64) s_nc[i] = -c / k * (1. - s_y[i] / c);
-  __syncthreads();
+  __syncthreads();  // also: every lane is done with s_err (s_p0) before s_p0 is rewritten
   for (int i = threadIdx.x; i < n - 1; i += 64) {
     double vr = (v[i] + v[i + 1]) / 2.;
     double s2 = sigma * sigma;
@@ -2033,16 +2056,16 @@ __device__ __forceinline__ void fit_body(const double *v_, int n, const double *
   }
 }
 
-__global__ __launch_bounds__(64) void k_fit(const double *v, int n, const double *Y, double *par, double *model,
-                                            double *depth_out, double *bis_out, int32_t *iters) {
-  fit_body(v, n, Y, par, model, depth_out, bis_out, iters, blockIdx.x);
+__global__ __launch_bounds__(64) void k_fit(const double *v, int n, const double *Y, const double *E, double *par,
+                                            double *model, double *depth_out, double *bis_out, int32_t *iters) {
+  fit_body(v, n, Y, E, par, model, depth_out, bis_out, iters, blockIdx.x);
 }
 
-hipError_t launch_fit(const double *v, int n_v, const double *y, int n_ccf, double *par, double *model, double *depth,
-                      double *bis, int32_t *iters, hipStream_t s) {
+hipError_t launch_fit(const double *v, int n_v, const double *y, const double *err, int n_ccf, double *par,
+                      double *model, double *depth, double *bis, int32_t *iters, hipStream_t s) {
   if (n_ccf == 0) return hipSuccess;
   if (n_v > kMaxV || n_v < 3) return hipErrorInvalidValue;
-  k_fit<<<n_ccf, 64, 0, s>>>(v, n_v, y, par, model, depth, bis, iters);
+  k_fit<<<n_ccf, 64, 0, s>>>(v, n_v, y, err, par, model, depth, bis, iters);
   return hipGetLastError();
 }
 
@@ -2179,7 +2202,7 @@ __global__ __launch_bounds__(1024) void kb_ccf_reduce(SOAP_BATCH_ARGS(CcfRedJob)
 template <bool P>
 __global__ __launch_bounds__(64) void kb_fit(SOAP_BATCH_ARGS(FitJob)) {
   batch_loop<P>(jobs, prefix, n_inst, total, [&](const FitJob &j, int local) {
-    fit_body(j.v, j.n_v, j.y, j.par, j.model, j.depth, j.bis, j.iters, local);
+    fit_body(j.v, j.n_v, j.y, nullptr, j.par, j.model, j.depth, j.bis, j.iters, local);
   });
 }
 

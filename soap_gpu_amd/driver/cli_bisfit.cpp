@@ -13,17 +13,18 @@ int main(int argc, char **argv) {
   int n = std::atoi(argv[1]);
   std::vector<double> d = read_numbers("CCF_data.txt"), e = read_numbers("CCF_error.txt");
   if ((int)d.size() < 2 * n || (int)e.size() < n) die("CCF_data.txt / CCF_error.txt shorter than n");
-  std::vector<double> v(n), y(n);
+  std::vector<double> v(n), y(n), err(n);
   for (int i = 0; i < n; i++) {
     v[i] = d[2 * i];
     y[i] = d[2 * i + 1];
-    if (e[i] != 1.0) die("CCF_error.txt: only unit errors are supported (the reference driver writes ones)");
+    err[i] = e[i];  // residuals and Jacobian rows are divided by these (BIS_FIT2.cpp:49,74-77)
   }
   soap_ctx *ctx = nullptr;
   if (soap_ctx_create(0, &ctx) != SOAP_OK) die("no HIP device");
   std::vector<double> par(5), model(n), depth(91), bis(91);
-  check(ctx, soap_fit_ccf_host(ctx, v.data(), n, y.data(), 1, par.data(), model.data(), depth.data(), bis.data()),
-        "fit_ccf_host");
+  check(ctx, soap_fit_ccf_host_err(ctx, v.data(), n, y.data(), err.data(), 1, par.data(), model.data(), depth.data(),
+                                   bis.data()),
+        "fit_ccf_host_err");
   soap_ctx_destroy(ctx);
   write_bin("ccf_model.bin", model.data(), n);
   write_bin("ccf_parameter.bin", par.data(), 5);

@@ -48,7 +48,7 @@ EXPORTS = (
     "soap_set_activity", "soap_integrate", "soap_get_raw", "soap_set_raw", "soap_alloc_phases", "soap_phase_maps",
     "soap_resolution", "soap_get_lower", "soap_get_norm",
     "soap_set_mask", "soap_ccf", "soap_get_ccf",
-    "soap_fit", "soap_get_fit", "soap_fit_ccf_host", "soap_bis_inject",
+    "soap_fit", "soap_get_fit", "soap_fit_ccf_host", "soap_fit_ccf_host_err", "soap_bis_inject",
     "soap_phoenix_bracket", "soap_phoenix_grid", "soap_phoenix_name", "soap_fits_read", "soap_phoenix_interp",
     "soap_run_series",
     "soap_set_timing", "soap_get_stage_ms", "soap_stage_name", "soap_num_stages",
@@ -125,6 +125,8 @@ def load(path: str = LIB_PATH):
         "soap_fit": (ctypes.c_int, [vp]),
         "soap_get_fit": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp]),
         "soap_fit_ccf_host": (ctypes.c_int, [vp, _dp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, _dp, _dp, _dp]),
+        "soap_fit_ccf_host_err": (ctypes.c_int, [vp, _dp, ctypes.c_int32, _dp, _dp, ctypes.c_int32, _dp, _dp, _dp,
+                                                 _dp]),
         "soap_bis_inject": (ctypes.c_int, [vp, ctypes.POINTER(BisInjectParams), _dp, _dp, ctypes.c_int64, _dp]),
         "soap_phoenix_bracket": (ctypes.c_int, [ctypes.c_double, _dp, _dp]),
         "soap_phoenix_grid": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]),
@@ -415,7 +417,8 @@ class Engine:
                                              ctypes.byref(cnt)), "phoenix_interp")
         return air[:cnt.value].copy(), flux[:cnt.value].copy()
 
-    def fit_ccf_host(self, v, y):
+    def fit_ccf_host(self, v, y, err=None):
+        """BIS_FIT2 on host CCFs y[n_ccf][n_v]; err: per-point errors of the same shape (None = ones)."""
         vv, vp = _d(v)
         yy = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
         n_ccf = yy.shape[0]
@@ -423,8 +426,13 @@ class Engine:
         mod, mp = _out(n_ccf * len(vv))
         dep, dp = _out(n_ccf * 91)
         bis, bp = _out(n_ccf * 91)
-        self._chk(self.L.soap_fit_ccf_host(self.ctx, vp, len(vv), yy.ctypes.data_as(_dp), n_ccf, pp, mp, dp, bp),
-                  "fit_ccf_host")
+        if err is None:
+            self._chk(self.L.soap_fit_ccf_host(self.ctx, vp, len(vv), yy.ctypes.data_as(_dp), n_ccf, pp, mp, dp,
+                                               bp), "fit_ccf_host")
+        else:
+            ee = np.ascontiguousarray(np.broadcast_to(np.atleast_2d(err), yy.shape), dtype=np.float64)
+            self._chk(self.L.soap_fit_ccf_host_err(self.ctx, vp, len(vv), yy.ctypes.data_as(_dp),
+                                                   ee.ctypes.data_as(_dp), n_ccf, pp, mp, dp, bp), "fit_ccf_host_err")
         return dict(par=par.reshape(n_ccf, 5), model=mod.reshape(n_ccf, -1), depth=dep.reshape(n_ccf, 91),
                     bis=bis.reshape(n_ccf, 91))
 

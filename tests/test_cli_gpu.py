@@ -265,6 +265,48 @@ def test_bis_fit2_executable(driver_run, tmp_path):
     np.testing.assert_allclose(load_bin(str(tmp_path / "ccf_bis.bin")), d["bis"][i], rtol=0, atol=1e-6)
 
 
+def test_bis_fit2_executable_with_errors(driver_run, tmp_path):
+    """BIS_FIT2 with a non-unit CCF_error.txt (BIS_FIT2.cpp:114-119): the fit
+    weights residuals by 1/err, as the oracle's fit with the same errors."""
+    root, w, ref = driver_run
+    d = np.load(os.path.join(out_prefix(root), "test_RV_tot_GPU.npz"))
+    y = d["ccf"][1]
+    err = 1e-3 * (1 + (w.vgrid / 8000.0) ** 2) * np.random.default_rng(7).uniform(0.5, 2.0, w.vgrid.size)
+    np.savetxt(tmp_path / "CCF_data.txt", np.array([w.vgrid, y]).T)
+    np.savetxt(tmp_path / "CCF_error.txt", err.T)
+    run([tool("BIS_FIT2"), w.vgrid.size], cwd=str(tmp_path))
+    par = load_bin(str(tmp_path / "ccf_parameter.bin"))
+    v_txt = np.loadtxt(tmp_path / "CCF_data.txt")
+    ro = O.fit_ccf(v_txt[:, 0], v_txt[:, 1], np.loadtxt(tmp_path / "CCF_error.txt"))
+    assert abs(par[3] - ro["par"][3]) <= 1e-4 and abs(par[4] - ro["par"][4]) <= 1e-4
+    assert abs(par[2] - ro["par"][2]) <= 1e-4
+    np.testing.assert_allclose(par[:2], ro["par"][:2], rtol=1e-9)
+
+
+def test_bis_fit2_executable_bad_inputs(tmp_path):
+    """Host-side failures of the BIS_FIT2 shim end with a message and a non-zero
+    status before any GPU work: wrong argc, missing files, short files, and a
+    zero error (the reference would divide by it)."""
+    exe = tool("BIS_FIT2")
+
+    def status(args):
+        r = subprocess.run([exe, *[str(a) for a in args]], cwd=str(tmp_path), capture_output=True, text=True,
+                           timeout=60)
+        return r.returncode, r.stderr
+
+    assert status([])[0] != 0
+    assert status([5, 6])[0] != 0
+    rc, err = status([5])
+    assert rc != 0 and "CCF_data.txt" in err
+    np.savetxt(tmp_path / "CCF_data.txt", np.array([np.arange(5.0), np.ones(5)]).T)
+    np.savetxt(tmp_path / "CCF_error.txt", np.ones(3))
+    rc, err = status([5])
+    assert rc != 0 and "shorter" in err
+    np.savetxt(tmp_path / "CCF_error.txt", np.array([1, 1, 0.0, 1, 1]))
+    rc, err = status([5])
+    assert rc != 0 and "non-zero" in err
+
+
 def test_driver_real_keyword_ragged_regions(tmp_path, tables):
     """keyword 'real': phases and per-phase region lists of different lengths (one
     phase with none) are read from the files instead of the configuration."""

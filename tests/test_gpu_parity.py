@@ -224,6 +224,28 @@ def test_fit_parity_on_same_ccfs(engine, config0_case):
     np.testing.assert_allclose(r["bis"], o["bis"].reshape(-1, 91), rtol=0, atol=1e-3)
 
 
+def test_fit_parity_with_errors(engine, config0_case):
+    """Per-point errors (CCF_error.txt, BIS_FIT2.cpp:49,74-77): the GPU fit
+    with non-unit errors vs the oracle fit with the same errors; unit errors
+    give the plain fit bit for bit."""
+    w, g, o = config0_case
+    y = o["ccf"].reshape(-1, len(w.vgrid))
+    rng = np.random.default_rng(42)
+    err = 1e-3 * (1 + (w.vgrid / 8000.0) ** 2)[None, :] * rng.uniform(0.5, 2.0, y.shape)
+    r = engine.fit_ccf_host(w.vgrid, y, err)
+    for i in range(y.shape[0]):
+        ro = O.fit_ccf(w.vgrid, y[i], err[i])
+        assert abs(r["par"][i, 3] - ro["par"][3]) < 1e-4  # RV
+        assert abs(r["par"][i, 4] - ro["par"][4]) < 1e-4  # FWHM
+        assert abs(r["par"][i, 2] - ro["par"][2]) < 1e-4  # span
+        np.testing.assert_allclose(r["par"][i, :2], ro["par"][:2], rtol=1e-9, atol=1e-12)
+    r1 = engine.fit_ccf_host(w.vgrid, y, np.ones_like(y))
+    r0 = engine.fit_ccf_host(w.vgrid, y)
+    np.testing.assert_array_equal(r1["par"], r0["par"])
+    with pytest.raises(RuntimeError):
+        engine.fit_ccf_host(w.vgrid, y[:1], np.zeros_like(y[:1]))
+
+
 @pytest.mark.parametrize("case", ["small_case", "config0_case"])
 def test_end_to_end_rv_fwhm_bis(case, request):
     w, g, o = request.getfixturevalue(case)

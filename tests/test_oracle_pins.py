@@ -475,6 +475,33 @@ def test_fit_matches_scipy_least_squares(seed):
     assert r["par"][4] == pytest.approx(sol.x[3], abs=1e-4)
 
 
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_fit_weighted_matches_scipy_least_squares(seed):
+    """Per-point errors (CCF_error.txt): the optimum of sum ((fit - y)/err)^2,
+    as scipy's MINPACK LM finds it.  Errors that grow into the wings move the
+    optimum away from the unit-error one (so dropping the weights fails), and
+    scaling every error by a constant leaves it unchanged."""
+    from scipy.optimize import least_squares
+    rng = np.random.default_rng(10 + seed)
+    v = np.arange(-20000, 20001, 250.0)
+    y = _gauss(v, 1.0, -0.5, rng.uniform(-800, 800), rng.uniform(6000, 9000))
+    y += 0.03 * np.exp(-((v - 2500) / 3000) ** 2) + 1e-3 * rng.standard_normal(v.size)
+    err = 1e-3 * (1 + (v / 8000.0) ** 2) * rng.uniform(0.5, 2.0, v.size)
+    r = O.fit_ccf(v, y, err)
+    px = v - v[0]
+    x0 = np.array([r["par"][0], -r["par"][1], r["par"][3] - v[0], r["par"][4]])
+    sol = least_squares(lambda p: (_gauss(px, *p) - y) / err, x0 * (1 + 1e-3), method="lm", xtol=1e-15,
+                        ftol=1e-15, gtol=1e-15)
+    assert r["par"][3] == pytest.approx(sol.x[2] + v[0], abs=1e-4)
+    assert r["par"][4] == pytest.approx(sol.x[3], abs=1e-4)
+    assert r["par"][0] == pytest.approx(sol.x[0], abs=1e-9)
+    r1 = O.fit_ccf(v, y)
+    assert abs(r1["par"][3] - r["par"][3]) > 1.0  # the weights matter here (m/s)
+    r2 = O.fit_ccf(v, y, err * 7.0)
+    assert r2["par"][3] == pytest.approx(r["par"][3], abs=1e-6)
+    assert np.array_equal(O.fit_ccf(v, y, np.ones(v.size))["par"], r1["par"])
+
+
 def test_bisector_shape_and_depths():
     """Bisector of a skewed CCF: depth grid 1 - |k| (i+5)/100, and a line with a
     red wing gives a positive-slope bisector (span sign)."""
