@@ -280,7 +280,7 @@ def test_bis_fit2_executable_with_errors(driver_run, tmp_path):
     ro = O.fit_ccf(v_txt[:, 0], v_txt[:, 1], np.loadtxt(tmp_path / "CCF_error.txt"))
     assert abs(par[3] - ro["par"][3]) <= 1e-4 and abs(par[4] - ro["par"][4]) <= 1e-4
     assert abs(par[2] - ro["par"][2]) <= 1e-4
-    np.testing.assert_allclose(par[:2], ro["par"][:2], rtol=1e-9)
+    np.testing.assert_allclose(par[:2], ro["par"][:2], rtol=1e-7)  # both fits stop at xtol 1e-8
 
 
 def test_bis_fit2_executable_bad_inputs(tmp_path):

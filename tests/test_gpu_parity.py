@@ -238,7 +238,9 @@ def test_fit_parity_with_errors(engine, config0_case):
         assert abs(r["par"][i, 3] - ro["par"][3]) < 1e-4  # RV
         assert abs(r["par"][i, 4] - ro["par"][4]) < 1e-4  # FWHM
         assert abs(r["par"][i, 2] - ro["par"][2]) < 1e-4  # span
-        np.testing.assert_allclose(r["par"][i, :2], ro["par"][:2], rtol=1e-9, atol=1e-12)
+        # continuum and contrast: both fits stop at the xtol test |dx| < 1e-8 (1e-8 + |x|),
+        # so their last iterates agree to that order, not to rounding
+        np.testing.assert_allclose(r["par"][i, :2], ro["par"][:2], rtol=1e-7, atol=1e-12)
     r1 = engine.fit_ccf_host(w.vgrid, y, np.ones_like(y))
     r0 = engine.fit_ccf_host(w.vgrid, y)
     np.testing.assert_array_equal(r1["par"], r0["par"])
