@@ -86,7 +86,8 @@ def lib():
         L.orc_bisector.argtypes = [_dp, _dp, ctypes.c_int, _d_, _d_, _d_, _d_, _dp, _dp]
         L.orc_bisector.restype = _d_
         L.orc_diff_forward.argtypes = ([_dp, ctypes.c_long, ctypes.c_int, ctypes.c_int, _dp, _d_, _d_, _d_, _d_,
-                                        _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _d_, ctypes.c_int, _dp,
+                                        _d_, _dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp, _d_,
+                                        ctypes.c_int, _dp,
                                         ctypes.c_int, _dp, _dp, ctypes.c_long, _dp, _dp, _dp, _dp, _d_, _d_, _d_,
                                         _d_, _dp, _dp, _dp, _dp, _dp, ctypes.c_int])
         _lib = L
@@ -360,14 +361,21 @@ def bisector(v, y, theta):
 
 
 def diff_forward(wave, grid, mu_bounds, vsini, u1, u2, kappa, cube_q, cube_s, maps, sigma_ms, trim, vgrid,
-                 line_l, line_w, target, rv_t, fw_t, w_spec, w_rv, w_fw, bis_t=None, w_bis=0.0, nthreads=None):
+                 line_l, line_w, target, rv_t, fw_t, w_spec, w_rv, w_fw, bis_t=None, w_bis=0.0, nthreads=None,
+                 kappa_f=0.0, lb=None, cube_f=None, fmaps=None):
     """Forward model + loss of the differentiable path; returns dict(loss, F[P,n], ccf[P,nv], theta[P,4],
-    span[P])."""
+    span[P]).  cube_f / fmaps (both or neither): facula seed rows and facula filling-factor maps,
+    weighted by kappa_f (1 + LB(mu)) with LB the polynomial lb (constant term first)."""
     w, wp = _d(wave)
     n = len(w)
     mb, mbp = _d(mu_bounds)
     q, qp = _d(cube_q)
     s, sp = _d(cube_s)
+    if (cube_f is None) != (fmaps is None):
+        raise ValueError("cube_f and fmaps go together")
+    f_, fp_ = (None, None) if cube_f is None else _d(cube_f)
+    fm_, fmp_ = (None, None) if fmaps is None else _d(fmaps)
+    lb_, lbp_ = (None, None) if lb is None else _d(lb)
     mp_, mpp = _d(maps)
     P, M = mp_.shape[0], mp_.shape[1]
     v, vp = _d(vgrid)
@@ -383,7 +391,8 @@ def diff_forward(wave, grid, mu_bounds, vsini, u1, u2, kappa, cube_q, cube_s, ma
     span = np.zeros(P)
     loss = np.zeros(1)
     lib().orc_diff_forward(wp, n, int(grid), len(mb) + 1, mbp, float(vsini), float(u1), float(u2), float(kappa),
-                           qp, sp, P, M, mpp, float(sigma_ms), int(trim), vp, len(v), llp, lwp, len(ll), tp, rtp,
+                           float(kappa_f), lbp_, qp, sp, fp_, P, M, mpp, fmp_, float(sigma_ms), int(trim), vp,
+                           len(v), llp, lwp, len(ll), tp, rtp,
                            ftp, btp, float(w_spec), float(w_rv), float(w_fw), float(w_bis), F.ctypes.data_as(_dp),
                            cc.ctypes.data_as(_dp), th.ctypes.data_as(_dp), span.ctypes.data_as(_dp),
                            loss.ctypes.data_as(_dp), nthreads or nthreads_default())

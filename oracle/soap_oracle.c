@@ -1100,8 +1100,12 @@ int orc_gauss_polish(const double *x, const double *y, int n, double *th) {
  *   cells of the G x G disk grid on the disk (y^2 + z^2 <= 1), y = -1 + 2 iy/G,
  *   z = -1 + 2 iz/G, mu = sqrt(1 - y^2 - z^2), mu bin as orc_disk_init,
  *   limb L = 1 - u1 (1-mu) - u2 (1-mu)^2, rigid rotation delta = vsini y [km/s];
- *   filling factor phi_pc = bilinear sample of map p (M x M over [-1, 1]^2);
- *   I_p = sum_c [(1 - phi) Shift(L Q_k) + kappa phi Shift(L S_k)] with the log-grid
+ *   filling factors phi_pc (spots) and psi_pc (faculae) = bilinear samples of map p
+ *   and facula map p (M x M over [-1, 1]^2; psi = 0 without facula maps);
+ *   I_p = sum_c [(1 - phi - psi) Shift(L Q_k) + kappa phi Shift(L S_k)
+ *                + kappa_f (1 + LB(mu)) psi Shift(L Fa_k)]
+ *   with LB(mu) = lb0 + lb1 mu + lb2 mu^2 + lb3 mu^3 and (1 + LB) the facula intensity
+ *   factor of SOAP_initialize.cu:367 / SOAP_integration.cu:317-323, and the log-grid
  *   shift of orc_shift_value (SOAP_integration.cu:232-281);
  *   F_p = conv(I_p) with the log-grid taps of orc_lower_resolution (no spec_norm),
  *   zero outside wid < i < n-1-wid;
@@ -1114,8 +1118,9 @@ int orc_gauss_polish(const double *x, const double *y, int n, double *th) {
  *        + w_bis sum_p (BIS_p - bis_t_p)^2   (the last term only when w_bis != 0).
  * Outputs F [P][n], ccf [P][n_v], theta [P][4], span [P] (each optional) and loss. */
 int orc_diff_forward(const double *wave, long n, int grid, int n_mu, const double *mu_bounds, double vsini,
-                     double u1, double u2, double kappa, const double *cube_q, const double *cube_s, int n_phase,
-                     int map_res, const double *maps, double sigma_ms, int trim, const double *vgrid, int n_v,
+                     double u1, double u2, double kappa, double kappa_f, const double *lb, const double *cube_q,
+                     const double *cube_s, const double *cube_f, int n_phase, int map_res, const double *maps,
+                     const double *fmaps, double sigma_ms, int trim, const double *vgrid, int n_v,
                      const double *line_l, const double *line_w, long n_lines, const double *target,
                      const double *rv_t, const double *fw_t, const double *bis_t, double w_spec, double w_rv,
                      double w_fw, double w_bis, double *F_out, double *ccf_out, double *theta_out, double *span_out,
@@ -1135,6 +1140,7 @@ int orc_diff_forward(const double *wave, long n, int grid, int n_mu, const doubl
   double *cL = (double *)malloc(sizeof(double) * (non > 0 ? non : 1));
   double *cy = (double *)malloc(sizeof(double) * (non > 0 ? non : 1));
   double *cz = (double *)malloc(sizeof(double) * (non > 0 ? non : 1));
+  double *clb = (double *)malloc(sizeof(double) * (non > 0 ? non : 1));
   orc_shift *csh = (orc_shift *)malloc(sizeof(orc_shift) * (non > 0 ? non : 1));
   non = 0;
   for (long idx = 0; idx < ncell; idx++) {
@@ -1153,6 +1159,7 @@ int orc_diff_forward(const double *wave, long n, int grid, int n_mu, const doubl
     cL[non] = 1. - u1 * (1 - mu) - u2 * (1 - mu) * (1 - mu);
     cy[non] = y;
     cz[non] = z;
+    clb[non] = lb ? 1. + (lb[0] + lb[1] * mu + lb[2] * mu * mu + lb[3] * mu * mu * mu) : 1.;
     csh[non] = orc_shift_setup(ORC_GRID_LOG, wave, n, 0.0, vsini * y);
     non++;
   }
@@ -1170,8 +1177,11 @@ int orc_diff_forward(const double *wave, long n, int grid, int n_mu, const doubl
   double *F = (double *)malloc(sizeof(double) * n);
   double *cc = (double *)malloc(sizeof(double) * n_v);
   double *phi = (double *)malloc(sizeof(double) * (non > 0 ? non : 1));
+  double *psi = (double *)malloc(sizeof(double) * (non > 0 ? non : 1));
+  const int fac = cube_f && fmaps;
   for (int p = 0; p < n_phase; p++) {
     const double *mp = maps + (size_t)p * M * M;
+    const double *fp = fac ? fmaps + (size_t)p * M * M : NULL;
     for (long c = 0; c < non; c++) {
       double gy = (cy[c] + 1) * 0.5 * (M - 1), gz = (cz[c] + 1) * 0.5 * (M - 1);
       int i0 = (int)floor(gy), j0 = (int)floor(gz);
@@ -1182,6 +1192,9 @@ int orc_diff_forward(const double *wave, long n, int grid, int n_mu, const doubl
       double fy = gy - i0, fz = gz - j0;
       phi[c] = (1 - fy) * (1 - fz) * mp[i0 * M + j0] + fy * (1 - fz) * mp[(i0 + 1) * M + j0] +
                (1 - fy) * fz * mp[i0 * M + j0 + 1] + fy * fz * mp[(i0 + 1) * M + j0 + 1];
+      psi[c] = fac ? (1 - fy) * (1 - fz) * fp[i0 * M + j0] + fy * (1 - fz) * fp[(i0 + 1) * M + j0] +
+                         (1 - fy) * fz * fp[i0 * M + j0 + 1] + fy * fz * fp[(i0 + 1) * M + j0 + 1]
+                   : 0.0;
     }
     int nchunk = nthreads > 0 ? nthreads * 4 : 32;
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
@@ -1191,10 +1204,19 @@ int orc_diff_forward(const double *wave, long n, int grid, int n_mu, const doubl
       for (long j = j0; j < j1; j++) I[j] = 0.0;
       for (long c = 0; c < non; c++) {
         const double *rq = cube_q + (long)ck[c] * n, *rs = cube_s + (long)ck[c] * n;
-        double a = 1 - phi[c], b = kappa * phi[c];
-        for (long j = j0; j < j1; j++)
-          I[j] += a * orc_shift_value(&csh[c], wave, n, rq, j, cL[c]) +
-                  b * orc_shift_value(&csh[c], wave, n, rs, j, cL[c]);
+        double a = 1 - phi[c] - psi[c], b = kappa * phi[c];
+        if (fac) {
+          const double *rf = cube_f + (long)ck[c] * n;
+          double e = kappa_f * clb[c] * psi[c];
+          for (long j = j0; j < j1; j++)
+            I[j] += a * orc_shift_value(&csh[c], wave, n, rq, j, cL[c]) +
+                    b * orc_shift_value(&csh[c], wave, n, rs, j, cL[c]) +
+                    e * orc_shift_value(&csh[c], wave, n, rf, j, cL[c]);
+        } else {
+          for (long j = j0; j < j1; j++)
+            I[j] += a * orc_shift_value(&csh[c], wave, n, rq, j, cL[c]) +
+                    b * orc_shift_value(&csh[c], wave, n, rs, j, cL[c]);
+        }
       }
     }
 #pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1)
@@ -1230,11 +1252,13 @@ int orc_diff_forward(const double *wave, long n, int grid, int n_mu, const doubl
   free(cL);
   free(cy);
   free(cz);
+  free(clb);
   free(csh);
   free(ctap);
   free(I);
   free(F);
   free(cc);
   free(phi);
+  free(psi);
   return 0;
 }

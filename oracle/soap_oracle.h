@@ -105,8 +105,9 @@ int orc_gauss_polish(const double *x, const double *y, int n, double *th);
 double orc_bisector(const double *v, const double *y, int n, double c, double k, double v0, double fwhm,
                     double *depth91, double *bis91);
 int orc_diff_forward(const double *wave, long n, int grid, int n_mu, const double *mu_bounds, double vsini,
-                     double u1, double u2, double kappa, const double *cube_q, const double *cube_s, int n_phase,
-                     int map_res, const double *maps, double sigma_ms, int trim, const double *vgrid, int n_v,
+                     double u1, double u2, double kappa, double kappa_f, const double *lb, const double *cube_q,
+                     const double *cube_s, const double *cube_f, int n_phase, int map_res, const double *maps,
+                     const double *fmaps, double sigma_ms, int trim, const double *vgrid, int n_v,
                      const double *line_l, const double *line_w, long n_lines, const double *target,
                      const double *rv_t, const double *fw_t, const double *bis_t, double w_spec, double w_rv,
                      double w_fw, double w_bis, double *F_out, double *ccf_out, double *theta_out, double *span_out,

@@ -159,3 +159,60 @@ def test_diff_forward_rv_follows_a_dark_spot():
                         np.zeros(2), 1.0, 0.0, 0.0)
     rv0 = q0["theta"][0, 2]
     assert r["theta"][0, 2] > rv0 + 10 and r["theta"][1, 2] < rv0 - 10
+
+
+def _fac_args(w, maps, sl, sw, vsini=2.5, u=(0.3, 0.2), kappa=0.4):
+    head = (w.wave, 20, w.mu_bounds, vsini, u[0], u[1], kappa, w.cubes["q"], w.cubes["sp"], maps)
+    tail = (w.sigma_inst, TRIM, w.vgrid, sl, sw, np.zeros((2, w.n)), np.zeros(2), np.zeros(2), 1.0, 1.0, 1.0)
+    return head, tail
+
+
+def test_diff_forward_empty_facula_maps_equal_the_spot_model():
+    """Facula maps of zeros leave the model as the two-population one, bit for bit
+    (psi = 0 adds zero terms only)."""
+    w, maps, sl, sw = _problem()
+    head, tail = _fac_args(w, maps, sl, sw)
+    a = O.diff_forward(*head, *tail)
+    b = O.diff_forward(*head, *tail, kappa_f=1.3, lb=[0.2, -0.1, 0.3, 0.05], cube_f=w.cubes["fac"],
+                       fmaps=np.zeros_like(maps))
+    np.testing.assert_array_equal(a["F"], b["F"])
+    assert a["loss"] == b["loss"]
+
+
+def test_diff_forward_facula_equal_to_quiet_ignores_facula_maps():
+    """Fa = Q, kappa_f = 1 and LB = 0: the facula term kappa_f (1 + LB) psi Q cancels
+    the -psi Q of the quiet weight for any facula map."""
+    w, maps, sl, sw = _problem()
+    head, tail = _fac_args(w, maps, sl, sw)
+    a = O.diff_forward(*head, *tail)
+    fm = np.clip(np.random.default_rng(9).uniform(-0.2, 0.6, maps.shape), 0, 1)
+    b = O.diff_forward(*head, *tail, kappa_f=1.0, lb=[0, 0, 0, 0], cube_f=w.cubes["q"], fmaps=fm)
+    np.testing.assert_allclose(b["F"], a["F"], rtol=0, atol=1e-12 * np.abs(a["F"]).max())
+
+
+def test_diff_forward_facula_intensity_is_one_plus_lb_of_mu():
+    """A disk fully covered by faculae (psi = 1, phi = 0) with Fa = Q is the quiet
+    star of the reference's quiet-star path (itot, SOAP_initialize.cu:339-395) with
+    the limb weight multiplied by kappa_f (1 + LB(mu)) (SOAP_integration.cu:317-323),
+    LB = lb0 + lb1 mu + lb2 mu^2 + lb3 mu^3 with mu the cell's own mu; the quiet weight
+    1 - psi vanishes.  A polynomial in mu of every degree up to 3 (a wrong power or
+    a dropped 1 + fails), checked against an independent path: disk_init's limb
+    map scaled and fed to quiet_star, then the log-grid convolution."""
+    w, maps, sl, sw = _problem()
+    vsini, u1, u2, kf = 3.0, 0.29, 0.34, 0.7
+    lb = np.array([0.23, -0.4, 0.5, 0.25])
+    zero = np.zeros_like(maps)
+    r = O.diff_forward(w.wave, 20, w.mu_bounds, vsini, u1, u2, 0.3, w.cubes["q"], w.cubes["sp"], zero,
+                       w.sigma_inst, TRIM, w.vgrid, sl, sw, np.zeros((2, w.n)), np.zeros(2), np.zeros(2),
+                       1.0, 0.0, 0.0, kappa_f=kf, lb=lb, cube_f=w.cubes["q"], fmaps=np.ones_like(maps))
+    prot = synth.prot_for_vsini(vsini)
+    dm = O.disk_init(20, prot, 0.0, 90.0, synth.R_SUN_KM, u1, u2, [0, 0, 0, 0], w.mu_bounds)
+    mu = dm["mu"]
+    on = mu >= 0.0  # off-disk cells hold -999
+    dm["limb"] = np.where(on, dm["limb"] * kf * (1 + lb[0] + lb[1] * mu + lb[2] * mu ** 2 + lb[3] * mu ** 3),
+                          dm["limb"])
+    q = O.quiet_star(O.GRID_LOG, w.wave, dm, w.cubes["q"], 20)
+    ref = O.conv_raw(O.GRID_LOG, w.wave, w.sigma_inst, 1.0 - q, 0)
+    sl_ = slice(TRIM, w.n - TRIM)
+    for p in range(2):
+        assert np.max(np.abs(r["F"][p][sl_] - ref[sl_])) <= 1e-11 * np.max(np.abs(ref))
