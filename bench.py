@@ -239,10 +239,11 @@ def batch_leg(n_inst: int, steps: int, warmup: int, workers: int = 8):
 
 def adjoint_leg(steps: int):
     """Differentiable model on the adjoint shape (grid 600, 100k-pt lambda
-    5000-5500 A, 50 phases, 8000-line mask, 512 x 512 smooth-blob filling-factor
-    map per phase): time of the forward (loss) and of forward + adjoint (loss and
-    every gradient), and a directional check of the vsini and contrast gradients
-    against central differences (h = 1e-6 relative) of the same forward."""
+    5000-5500 A, 50 phases, 8000-line mask, 512 x 512 smooth-blob spot and facula
+    filling-factor maps per phase): time of the forward (loss) and of forward +
+    adjoint (loss and every gradient), and a directional check of the vsini and
+    both contrast gradients against central differences (h = 1e-6 relative) of the
+    same forward."""
     import torch
 
     from soap_gpu_amd import synth
@@ -250,31 +251,35 @@ def adjoint_leg(steps: int):
     from soap_gpu_amd.engine import Engine, select_mask_lines
 
     t0 = time.time()
-    w, maps = synth.config3()
+    w, maps, fmaps = synth.config3()
     P, M = maps.shape[0], maps.shape[1]
     trim = 210
     sl, sw = select_mask_lines(w.wave, w.mask_l, w.mask_w, trim)
     eng = Engine(0)
-    model = DiffModel(eng, w.wave, w.mu_bounds, sl, sw, w.vgrid, w.star["grid"], P, M, w.sigma_inst, trim)
+    model = DiffModel(eng, w.wave, w.mu_bounds, sl, sw, w.vgrid, w.star["grid"], P, M, w.sigma_inst, trim,
+                      lb=w.star["lb"])
     dev = torch.device("cuda")
     tq = torch.tensor(w.cubes["q"], dtype=torch.float64, device=dev)
     ts = torch.tensor(w.cubes["sp"], dtype=torch.float64, device=dev)
+    tf = torch.tensor(w.cubes["fac"], dtype=torch.float64, device=dev)
     tm = torch.tensor(maps, dtype=torch.float64, device=dev)
+    tfm = torch.tensor(fmaps, dtype=torch.float64, device=dev)
     z = torch.zeros((P, w.n), dtype=torch.float64, device=dev)
     log(f"[bench:adjoint] set up in {time.time() - t0:.1f} s: n={w.n} grid={w.star['grid']} P={P} map={M}x{M} "
         f"mask={len(sl)} lines")
     # target: the model at perturbed parameters and maps, plus seeded noise
-    model.forward((2.3, 0.31, 0.30, 0.35), tq, ts, tm * 0.8, z, np.zeros(P), np.zeros(P), 1.0, 0.0, 0.0)
+    model.forward((2.3, 0.31, 0.30, 0.35, 1.15), tq, ts, tm * 0.8, z, np.zeros(P), np.zeros(P), 1.0, 0.0, 0.0,
+                  f=tf, fmaps=tfm * 0.8)
     F = model.get("F")
     th = model.get("theta")
     rng = np.random.default_rng(1237)
     T = torch.tensor(F + 1e-4 * np.abs(F).max() * rng.standard_normal(F.shape), dtype=torch.float64, device=dev)
     rv_t = th[:, 2] + rng.uniform(-5.0, 5.0, P)
     wts = (1.0 / float(torch.sum(T[:, trim:w.n - trim] ** 2)), 1e-3, 0.0)
-    prm = np.array([2.0, 0.29, 0.34, 0.3])
+    prm = np.array([2.0, 0.29, 0.34, 0.3, 1.1])
 
     def fwd(p=prm):
-        return model.forward(p, tq, ts, tm, T, rv_t, np.zeros(P), *wts)[0]
+        return model.forward(p, tq, ts, tm, T, rv_t, np.zeros(P), *wts, f=tf, fmaps=tfm)[0]
 
     for _ in range(3):  # warm: the adjoint's mask table is built by the first backward
         fwd()
@@ -294,31 +299,33 @@ def adjoint_leg(steps: int):
 
     ms_f = timed(fwd)
     ms_fb = timed(lambda: (fwd(), model.backward()))
-    g4, gq, gs, gm = model.backward()
+    g5 = model.backward()[0]
     log(f"[bench:adjoint] forward {ms_f:.2f} ms, forward + adjoint {ms_fb:.2f} ms")
     fd = {}
-    for i, name in ((0, "vsini"), (3, "contrast")):
+    for i, name in ((0, "vsini"), (3, "contrast"), (4, "contrast_fac")):
         h = 1e-6 * prm[i]
-        e = np.eye(4)[i] * h
+        e = np.eye(5)[i] * h
         d = (fwd(prm + e) - fwd(prm - e)) / (2 * h)
-        fd[name] = {"adjoint": float(g4[i]), "central_fd": float(d), "rel_err": float(abs(g4[i] - d) / abs(d))}
+        fd[name] = {"adjoint": float(g5[i]), "central_fd": float(d), "rel_err": float(abs(g5[i] - d) / abs(d))}
     eng.close()
     return dict(config={"workload": f"differentiable model: grid {w.star['grid']}, {w.n}-pt log lambda 5000-5500 A, "
-                                    f"{P} phases, {len(sl)}-line mask, {M}x{M} filling-factor maps, "
-                                    f"{w.n_mu} mu bins"},
+                                    f"{P} phases, {len(sl)}-line mask, {M}x{M} spot and facula filling-factor "
+                                    f"maps, {w.n_mu} mu bins"},
                 ms_forward=round(ms_f, 3), ms_forward_adjoint=round(ms_fb, 3),
                 adjoint_over_forward=round(ms_fb / ms_f, 3),
-                gradients=["vsini", "u1", "u2", "contrast", "quiet seed rows", "spot seed rows", "map pixels"],
+                gradients=["vsini", "u1", "u2", "spot contrast", "facula contrast", "quiet seed rows",
+                           "spot seed rows", "facula seed rows", "spot map pixels", "facula map pixels"],
                 fd_check_vs_gpu_forward=fd)
 
 
 def fit_leg(n_inst: int, iters: int, workers: int = 8):
     """Fitting loop on the fitting workload: n_inst uneven differentiable instances
-    (grid 300-500, 30k-80k lambda points, 20-60 phases, 256x256 filling-factor maps
-    of 1-4 blobs), targets = RV, FWHM and BIS series of the model at the true
-    (vsini, contrast, maps); L-BFGS from perturbed vsini and contrast and smoothed
-    maps.  Reports the time per objective evaluation (forward + adjoint of every
-    instance), the forward alone, and the loss reduction."""
+    (grid 300-500, 30k-80k lambda points, 20-60 phases, 1-4 discrete spot / facula
+    regions rendered as 256x256 spot and facula filling-factor maps), targets = RV,
+    FWHM and BIS series of the model at the true (vsini, contrasts, maps); L-BFGS
+    from perturbed vsini and contrasts and smoothed maps.  Reports the time per
+    objective evaluation (forward + adjoint of every instance), the forward alone,
+    and the loss reduction."""
     from concurrent.futures import ThreadPoolExecutor
 
     import torch
@@ -331,12 +338,13 @@ def fit_leg(n_inst: int, iters: int, workers: int = 8):
     with ThreadPoolExecutor(max_workers=workers) as ex:
         made = list(ex.map(lambda ip: synth.config4_instance(ip[1], 256, ip[0]), list(enumerate(prm))))
     insts, truth = [], []
-    for (w, maps, tr), p_ in zip(made, prm):
+    for (w, maps, fmaps, tr), p_ in zip(made, prm):
         P = len(w.phases)
         start = np.clip(0.5 * maps + 0.25 * maps.mean(), 0.0, 1.0)
+        fstart = np.clip(0.5 * fmaps + 0.25 * fmaps.mean(), 0.0, 1.0)
         inst = Instance(w, start, P, 256, vsini0=tr["vsini"] * 1.2, contrast0=0.5 * (tr["contrast"] + 0.3),
-                        weights=(0.0, 1e-2, 1e-4, 1e-3))
-        rv, fw, bis = inst.observe((tr["vsini"], 0.29, 0.34, tr["contrast"]), maps)
+                        weights=(0.0, 1e-2, 1e-4, 1e-3), fmaps0=fstart, contrast_fac0=1.15)
+        rv, fw, bis = inst.observe((tr["vsini"], 0.29, 0.34, tr["contrast"], tr["contrast_fac"]), maps, fmaps)
         inst.rv_t, inst.fw_t, inst.bis_t = rv, fw, bis
         insts.append(inst)
         truth.append(tr)
@@ -351,11 +359,11 @@ def fit_leg(n_inst: int, iters: int, workers: int = 8):
     torch.cuda.synchronize()
     ms_per_call = (time.perf_counter() - t1) * 1e3
     for inst in insts:
-        inst.theta.grad = None
-        inst.z.grad = None
+        for v in inst.variables():
+            v.grad = None
     r = fit_graph(insts, iters=iters, log=log)
     vs_err = [abs(float(i.theta[0].detach()) - t["vsini"]) / t["vsini"] for i, t in zip(insts, truth)]
-    n_par = sum(2 + i.z.numel() for i in insts)
+    n_par = sum(sum(v.numel() for v in i.variables()) for i in insts)
     out = dict(instances=n_inst, iterations=r["iters"], evaluations=r["evals"], seconds=round(r["seconds"], 2),
                ms_per_evaluation=round(r["ms_per_eval"], 2), ms_per_evaluation_per_call_path=round(ms_per_call, 2),
                ms_per_graph_replay=round(r["graph_ms"], 2), ms_per_forward_graph_replay=round(r["graph_fwd_ms"], 2),

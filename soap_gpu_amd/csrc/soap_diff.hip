@@ -3,9 +3,12 @@
 // Model (log-uniform grid): disk cell c with bin k, limb L = 1 - u1(1-mu) -
 // u2(1-mu)^2 and rigid-rotation velocity delta = vsini y has the integer shift m
 // and weight t of the forward engine; phi_pc is the bilinear sample of phase p's
-// M x M filling-factor map over [-1, 1]^2; the phase spectrum is
-//   I_p = sum_k sum_d HQ_pk[d] Q_k[j - d] + HS_pk[d] S_k[j - d],
-//   HQ_pk = sum_c L (1 - phi) {(1-t) at d = m, t at d = m-1},  HS_pk likewise with kappa phi,
+// M x M filling-factor map over [-1, 1]^2 (psi_pc likewise from the facula map,
+// when there is one); the phase spectrum is
+//   I_p = sum_k sum_d HQ_pk[d] Q_k[j - d] + HS_pk[d] S_k[j - d] + HF_pk[d] Fa_k[j - d],
+//   HQ_pk = sum_c L (1 - phi - psi) {(1-t) at d = m, t at d = m-1},  HS_pk likewise with
+//   kappa phi, HF_pk with kappa_f (1 + LB(mu)) psi (the reference's facula intensity
+//   factor, SOAP_integration.cu:317-323),
 // F_p = conv(I_p) with the resolution taps, CCF_p = raw mask CCF of F_p, and the
 // Gaussian fit of CCF_p polished to the least-squares optimum gives RV_p, FWHM_p.
 // The adjoint runs the same structures backwards: the implicit derivative of
@@ -81,19 +84,27 @@ __device__ __forceinline__ void cell_yz(long idx, int grid, double &y, double &z
   z = -1. + iz * (2. / grid);
 }
 
+// facula intensity factor 1 + LB(mu) (SOAP_initialize.cu:367, SOAP_integration.cu:317-323)
+__device__ __forceinline__ double lb_factor(const DiffArgs &a, double mu) {
+#pragma clang fp contract(off)
+  return 1. + (a.lb[0] + a.lb[1] * mu + a.lb[2] * mu * mu + a.lb[3] * mu * mu * mu);
+}
+
 // taps of every phase (blockIdx.y): fixed-point sums in LDS when they fit, then global.
 // All cells of one disk row share the shift (m, t), and neighbouring cells of a row
 // share the mu bin, so each wave first sums its lanes' weights over runs of equal
 // (row, bin) (a segmented shuffle scan, lanes = consecutive cells); the last lane of
 // each run adds the run's four tap terms.  This removes the many-way atomic
 // contention of one atomic per cell on the same few tap addresses.
+template <int NS>
 __global__ __launch_bounds__(kDT) void k_diff_taps(DiffArgs a, int cells_per_block, int use_lds) {
   const double kappa = a.prm ? a.prm[3] : a.kappa;
-  extern __shared__ unsigned long long lacc[];  // [n_mu][nd][2 sets][2]
+  const double kappa_f = a.prm ? a.prm[4] : a.kappa_f;
+  extern __shared__ unsigned long long lacc[];  // [n_mu][nd][NS sets][2]
   int p = blockIdx.y;
   long ncell = (long)a.grid * a.grid;
   long c0 = (long)blockIdx.x * cells_per_block, c1 = min(ncell, c0 + cells_per_block);
-  long nacc = (long)a.n_mu * a.nd * 4;
+  long nacc = (long)a.n_mu * a.nd * NS * 2;
   unsigned long long *gacc = a.acc + (long)p * nacc;
   if (use_lds) {
     for (long e = threadIdx.x; e < nacc; e += blockDim.x) lacc[e] = 0ull;
@@ -101,12 +112,15 @@ __global__ __launch_bounds__(kDT) void k_diff_taps(DiffArgs a, int cells_per_blo
   }
   unsigned long long *dst = use_lds ? lacc : gacc;
   const double *mp = a.maps + (long)p * a.map_res * a.map_res;
+  const double *fp = NS == 3 ? a.fmaps + (long)p * a.map_res * a.map_res : nullptr;
   const int lane = threadIdx.x & 63;
   // every lane of a wave runs every iteration (the shuffles need the whole wave)
   for (long base = c0; base < c1; base += blockDim.x) {
     long idx = base + threadIdx.x;
     int key = -1, k = -1, d0 = 0;
-    double wq = 0.0, ws = 0.0, t = 0.0;
+    double w[NS], t = 0.0;
+#pragma unroll
+    for (int q = 0; q < NS; q++) w[q] = 0.0;
     if (idx < c1) k = a.ck[idx];
     if (k >= 0) {
       d0 = a.cm[idx] - a.dmin;  // tap index of offset m; m-1 at d0-1
@@ -115,31 +129,36 @@ __global__ __launch_bounds__(kDT) void k_diff_taps(DiffArgs a, int cells_per_blo
         int i0, j0;
         cell_yz(idx, a.grid, y, z);
         double phi = map_sample(mp, a.map_res, y, z, i0, j0, fy, fz);
+        double psi = NS == 3 ? map_sample(fp, a.map_res, y, z, i0, j0, fy, fz) : 0.0;
         double L = a.cL[idx];
         t = a.ct[idx];
-        wq = L * (1 - phi);
-        ws = kappa * L * phi;
+        w[0] = L * (1 - phi - psi);
+        w[1] = kappa * L * phi;
+        if (NS == 3) w[NS - 1] = kappa_f * lb_factor(a, a.cmu[idx]) * L * psi;
         key = (int)(idx / a.grid) * 1024 + k;  // (row, bin): rows share (m, t); k < 1024
       }
     }
     const int kprev = __shfl_up(key, 1), knext = __shfl_down(key, 1);
     bool f = lane == 0 || kprev != key;  // run head
     const bool tail = lane == 63 || knext != key;
-    for (int off = 1; off < 64; off <<= 1) {  // inclusive segmented scan of (wq, ws)
-      double xq = __shfl_up(wq, off), xs = __shfl_up(ws, off);
+    for (int off = 1; off < 64; off <<= 1) {  // inclusive segmented scan of the weights
+      double x[NS];
+#pragma unroll
+      for (int q = 0; q < NS; q++) x[q] = __shfl_up(w[q], off);
       int fo = __shfl_up((int)f, off);
       if (lane >= off && !f) {
-        wq += xq;
-        ws += xs;
+#pragma unroll
+        for (int q = 0; q < NS; q++) w[q] += x[q];
         f = fo != 0;
       }
     }
     if (tail && key >= 0) {
-      unsigned long long *row = dst + (long)k * a.nd * 4;
-      fx_add(row + (long)d0 * 4 + 0, wq * (1.0 - t));
-      fx_add(row + (long)(d0 - 1) * 4 + 0, wq * t);
-      fx_add(row + (long)d0 * 4 + 2, ws * (1.0 - t));
-      fx_add(row + (long)(d0 - 1) * 4 + 2, ws * t);
+      unsigned long long *row = dst + (long)k * a.nd * NS * 2;
+#pragma unroll
+      for (int q = 0; q < NS; q++) {
+        fx_add(row + ((long)d0 * NS + q) * 2, w[q] * (1.0 - t));
+        fx_add(row + ((long)(d0 - 1) * NS + q) * 2, w[q] * t);
+      }
     }
   }
   if (use_lds) {
@@ -151,18 +170,22 @@ __global__ __launch_bounds__(kDT) void k_diff_taps(DiffArgs a, int cells_per_blo
   }
 }
 
-// fixed point -> taps [p][k][d][2] and the nonzero range per (p, k)
+// fixed point -> taps [p][k][d][ns] and the nonzero range per (p, k)
 __global__ void k_diff_taps_finalize(DiffArgs a) {
   int p = blockIdx.x;
+  const int ns = a.ns;
   for (int k = threadIdx.x; k < a.n_mu; k += blockDim.x) {
     int lo = a.nd, hi = -1;
     for (int d = 0; d < a.nd; d++) {
-      const unsigned long long *e = a.acc + (((long)p * a.n_mu + k) * a.nd + d) * 4;
-      double vq = fx_value(e[0], e[1]), vs = fx_value(e[2], e[3]);
-      double *o = a.taps + (((long)p * a.n_mu + k) * a.nd + d) * 2;
-      o[0] = vq;
-      o[1] = vs;
-      if (vq != 0.0 || vs != 0.0) {
+      const unsigned long long *e = a.acc + (((long)p * a.n_mu + k) * a.nd + d) * ns * 2;
+      double *o = a.taps + (((long)p * a.n_mu + k) * a.nd + d) * ns;
+      bool nz = false;
+      for (int q = 0; q < ns; q++) {
+        double v = fx_value(e[q * 2], e[q * 2 + 1]);
+        o[q] = v;
+        nz |= v != 0.0;
+      }
+      if (nz) {
         lo = d < lo ? d : lo;
         hi = d;
       }
@@ -172,9 +195,11 @@ __global__ void k_diff_taps_finalize(DiffArgs a) {
   }
 }
 
-// I_p[j] = sum_k sum_d HQ[d] Q_k[j - dmin - d] + HS[d] S_k[j - dmin - d]; block = (tile, p)
+// I_p[j] = sum_k sum_d HQ[d] Q_k[j - dmin - d] + HS[d] S_k[j - dmin - d] (+ HF[d] Fa_k[j - dmin - d]);
+// block = (tile, p)
+template <int NS>
 __global__ __launch_bounds__(kDT) void k_diff_fir(DiffArgs a) {
-  extern __shared__ double rows[];  // [2][row_len]
+  extern __shared__ double rows[];  // [NS][row_len]
   int p = blockIdx.y;
   long j0 = (long)blockIdx.x * kDT;
   int nd = a.nd, row_len = kDT + nd - 1;
@@ -189,13 +214,14 @@ __global__ __launch_bounds__(kDT) void k_diff_fir(DiffArgs a) {
       bool in = i >= 0 && i < a.n;
       rows[e] = in ? a.q[(long)k * a.n + i] : 0.0;
       rows[row_len + e] = in ? a.s[(long)k * a.n + i] : 0.0;
+      if (NS == 3) rows[2 * row_len + e] = in ? a.f[(long)k * a.n + i] : 0.0;
     }
     __syncthreads();
-    const double *T = a.taps + ((long)p * a.n_mu + k) * nd * 2;
+    const double *T = a.taps + ((long)p * a.n_mu + k) * nd * NS;
     for (int di = lo; di <= hi; di++) {
       int e = threadIdx.x + nd - 1 - di;
-      acc = fma(T[di * 2], rows[e], acc);
-      acc = fma(T[di * 2 + 1], rows[row_len + e], acc);
+#pragma unroll
+      for (int q = 0; q < NS; q++) acc = fma(T[di * NS + q], rows[q * row_len + e], acc);
     }
   }
   long j = j0 + threadIdx.x;
@@ -628,18 +654,22 @@ __device__ __forceinline__ void diff_lag_range(const DiffArgs &a, int &lo, int &
   hi = min(a.nd - 1, m(vs) - a.dmin + 2);
 }
 
-__global__ __launch_bounds__(kDT) void k_diff_corr(DiffArgs a, const double *GI, double *Cpart) {
-  extern __shared__ double lds[];  // Q and S rows [kCorrT + nd - 1 + kCorrL] each, then [kCorrL][kDT] partials
+// One pass covers nsp <= 2 seed cubes (x0, x1), written to sets soff, soff + 1 of
+// the ns-set tables; the facula set, when present, is a second pass (three staged
+// rows would not fit the LDS budget).
+__global__ __launch_bounds__(kDT) void k_diff_corr(DiffArgs a, const double *GI, double *Cpart, const double *x0,
+                                                   const double *x1, int nsp, int soff) {
+  extern __shared__ double lds[];  // row(s) [kCorrT + nd - 1 + kCorrL] each, then [kCorrL][kDT] partials
   const int k = blockIdx.y, chunk = blockIdx.x, nchunk = gridDim.x;
   const long j0 = (long)chunk * kCorrT;
-  const int nd = a.nd, rl = kCorrT + nd - 1 + kCorrL;
-  double *rq = lds, *rs = rq + rl, *red = rs + rl;
+  const int nd = a.nd, rl = kCorrT + nd - 1 + kCorrL, ns = a.ns;
+  double *rq = lds, *rs = rq + rl, *red = rq + nsp * rl;
   const long src0 = j0 - a.dmin - nd + 1;
   for (int e = threadIdx.x; e < rl; e += blockDim.x) {
     long i = src0 + e;
     bool in = i >= 0 && i < a.n && e < kCorrT + nd - 1;
-    rq[e] = in ? a.q[(long)k * a.n + i] : 0.0;
-    rs[e] = in ? a.s[(long)k * a.n + i] : 0.0;
+    rq[e] = in ? x0[(long)k * a.n + i] : 0.0;
+    if (nsp == 2) rs[e] = in ? x1[(long)k * a.n + i] : 0.0;
   }
   __syncthreads();
   const int p1 = min(a.n_phase, (int)(blockIdx.z + 1) * kCorrPG);
@@ -650,10 +680,10 @@ __global__ __launch_bounds__(kDT) void k_diff_corr(DiffArgs a, const double *GI,
       long j = j0 + threadIdx.x * kCorrPer + u;
       g[u] = j < a.n ? GI[(long)p * a.n + j] : 0.0;
     }
-    double *out = Cpart + (((long)p * a.n_mu + k) * nchunk + chunk) * nd * 2;
+    double *out = Cpart + (((long)p * a.n_mu + k) * nchunk + chunk) * nd * ns;
     int dlo, dhi;
     diff_lag_range(a, dlo, dhi);  // uniform
-    for (int set = 0; set < 2; set++) {
+    for (int set = 0; set < nsp; set++) {
       const double *row = (set ? rs : rq) + threadIdx.x * kCorrPer;
       for (int d0 = dlo; d0 <= dhi; d0 += kCorrL) {
         // lag d = d0 + l reads X[j - dmin - d] = row[u + nd - 1 - d]; window base nd - d0 - kCorrL
@@ -681,34 +711,37 @@ __global__ __launch_bounds__(kDT) void k_diff_corr(DiffArgs a, const double *GI,
 #pragma unroll
         for (int r = 0; r < kDT / 32; r++) x += red[l * kDT + r * 32 + part];
         for (int off = 16; off > 0; off >>= 1) x += __shfl_xor(x, off);
-        if (part == 0 && d0 + l < nd) out[(d0 + l) * 2 + set] = x;
+        if (part == 0 && d0 + l < nd) out[(d0 + l) * ns + soff + set] = x;
       }
     }
   }
 }
 
 __global__ void k_diff_corr_reduce(DiffArgs a, int n_pk, int nd, int nchunk, const double *Cpart, double *C) {
+  const int ns = a.ns;
   long e = (long)blockIdx.x * blockDim.x + threadIdx.x;  // (p, k) x (d, set)
-  if (e >= (long)n_pk * nd * 2) return;
-  long pk = e / (nd * 2), r = e % (nd * 2);
+  if (e >= (long)n_pk * nd * ns) return;
+  long pk = e / (nd * ns), r = e % (nd * ns);
   int dlo, dhi;
   diff_lag_range(a, dlo, dhi);
-  const int d = (int)(r / 2);
+  const int d = (int)(r / ns);
   double x = 0.0;
   if (d >= dlo && d <= dhi)  // k_diff_corr computes only these lags
-    for (int c = 0; c < nchunk; c++) x += Cpart[(pk * nchunk + c) * nd * 2 + r];
+    for (int c = 0; c < nchunk; c++) x += Cpart[(pk * nchunk + c) * nd * ns + r];
   C[e] = x;
 }
 
 int diff_corr_chunks(int64_t n) { return (int)((n + kCorrT - 1) / kCorrT); }
 
 // per-cell derivatives from the correlations, phases summed per cell; scalar
-// partials per block: [gvsini, gu1, gu2, gkappa]
+// partials per block: [gvsini, gu1, gu2, gkappa, gkappa_f]
 __global__ __launch_bounds__(kDT) void k_diff_cellgrad(DiffArgs a, const double *C, double *part) {
   const double kappa = a.prm ? a.prm[3] : a.kappa;
+  const double kappa_f = a.prm ? a.prm[4] : a.kappa_f;
+  const int ns = a.ns;
   long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   long ncell = (long)a.grid * a.grid;
-  double sv = 0, su1 = 0, su2 = 0, sk = 0;
+  double sv = 0, su1 = 0, su2 = 0, sk = 0, sf = 0;
   if (idx < ncell && a.ck[idx] >= 0) {
     int k = a.ck[idx];
     int d0 = a.cm[idx] - a.dmin;
@@ -716,18 +749,29 @@ __global__ __launch_bounds__(kDT) void k_diff_cellgrad(DiffArgs a, const double 
       double y, z;
       cell_yz(idx, a.grid, y, z);
       double L = a.cL[idx], t = a.ct[idx], mu = a.cmu[idx];
+      const double lbf = ns == 3 ? lb_factor(a, mu) : 0.0;
       double gL = 0, gt = 0;
       int M = a.map_res;
       for (int p = 0; p < a.n_phase; p++) {
-        const double *Cp = C + (((long)p * a.n_mu + k) * a.nd) * 2;
-        double cq0 = Cp[d0 * 2], cq1 = Cp[(d0 - 1) * 2], cs0 = Cp[d0 * 2 + 1], cs1 = Cp[(d0 - 1) * 2 + 1];
+        const double *Cp = C + (((long)p * a.n_mu + k) * a.nd) * ns;
+        double cq0 = Cp[d0 * ns], cq1 = Cp[(d0 - 1) * ns], cs0 = Cp[d0 * ns + 1], cs1 = Cp[(d0 - 1) * ns + 1];
         double aq = (1 - t) * cq0 + t * cq1, as = (1 - t) * cs0 + t * cs1;
         double bq = cq1 - cq0, bs = cs1 - cs0;
         int i0, j0;
         double fy, fz;
         double phi = map_sample(a.maps + (long)p * M * M, M, y, z, i0, j0, fy, fz);
-        gL += (1 - phi) * aq + kappa * phi * as;
-        gt += L * ((1 - phi) * bq + kappa * phi * bs);
+        if (ns == 3) {
+          double cf0 = Cp[d0 * ns + 2], cf1 = Cp[(d0 - 1) * ns + 2];
+          double af = (1 - t) * cf0 + t * cf1, bf = cf1 - cf0;
+          double psi = map_sample(a.fmaps + (long)p * M * M, M, y, z, i0, j0, fy, fz);
+          double ef = kappa_f * lbf * psi;
+          gL += (1 - phi - psi) * aq + kappa * phi * as + ef * af;
+          gt += L * ((1 - phi - psi) * bq + kappa * phi * bs + ef * bf);
+          sf += L * lbf * psi * af;
+        } else {
+          gL += (1 - phi) * aq + kappa * phi * as;
+          gt += L * ((1 - phi) * bq + kappa * phi * bs);
+        }
         sk += L * phi * as;
       }
       su1 = -gL * (1 - mu);
@@ -735,30 +779,33 @@ __global__ __launch_bounds__(kDT) void k_diff_cellgrad(DiffArgs a, const double 
       sv = gt * a.cdt[idx];  // dt/d(delta) * d(delta)/d(vsini)
     }
   }
-  __shared__ double s_p[4][kDT / 64];
-  double v4[4] = {sv, su1, su2, sk};
-  for (int q = 0; q < 4; q++) {
-    double x = v4[q];
+  __shared__ double s_p[kDiffScalars][kDT / 64];
+  double v5[kDiffScalars] = {sv, su1, su2, sk, sf};
+  for (int q = 0; q < kDiffScalars; q++) {
+    double x = v5[q];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
     if ((threadIdx.x & 63) == 0) s_p[q][threadIdx.x >> 6] = x;
   }
   __syncthreads();
-  if (threadIdx.x < 4) {
+  if (threadIdx.x < kDiffScalars) {
     double s = 0;
     for (int w = 0; w < kDT / 64; w++) s += s_p[threadIdx.x][w];
-    part[(long)blockIdx.x * 4 + threadIdx.x] = s;
+    part[(long)blockIdx.x * kDiffScalars + threadIdx.x] = s;
   }
 }
 
 // map-pixel gradients as a gather (no atomics): node (i, j) of phase p sums, over
 // the cells whose bilinear stencil in map_sample holds it, weight x L (kappa a_S -
-// a_Q), with a_Q, a_S the cell's interpolated correlations.  A cell at map
+// a_Q) for the spot map and weight x L (kappa_f (1 + LB) a_F - a_Q) for the facula
+// map, with a_Q, a_S, a_F the cell's interpolated correlations.  A cell at map
 // coordinate gy = iy (M - 1) / G touches nodes floor(gy) and floor(gy) + 1, so
 // node i looks at the rows with floor(gy) in {i - 1, i} (a superset is scanned
 // and filtered with the same arithmetic as map_sample).
-__global__ __launch_bounds__(kDT) void k_diff_mapgrad(DiffArgs a, const double *C, double *gmaps) {
+__global__ __launch_bounds__(kDT) void k_diff_mapgrad(DiffArgs a, const double *C, double *gmaps, double *gfmaps) {
 #pragma clang fp contract(off)
   const double kappa = a.prm ? a.prm[3] : a.kappa;
+  const double kappa_f = a.prm ? a.prm[4] : a.kappa_f;
+  const int ns = a.ns;
   const int M = a.map_res, G = a.grid;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;  // map column
   const int i = blockIdx.y;                              // map row
@@ -767,7 +814,7 @@ __global__ __launch_bounds__(kDT) void k_diff_mapgrad(DiffArgs a, const double *
   const double sc = (double)G / (M - 1), dg = 2. / G;
   const int y_lo = max(0, (int)floor((i - 1) * sc) - 1), y_hi = min(G - 1, (int)ceil((i + 1) * sc) + 1);
   const int z_lo = max(0, (int)floor((j - 1) * sc) - 1), z_hi = min(G - 1, (int)ceil((j + 1) * sc) + 1);
-  double acc = 0.0;
+  double acc = 0.0, accf = 0.0;
   for (int iy = y_lo; iy <= y_hi; iy++) {
     // the row half of map_sample: gy, its clamped floor and fraction
     double y = -1. + iy * dg;
@@ -790,23 +837,32 @@ __global__ __launch_bounds__(kDT) void k_diff_mapgrad(DiffArgs a, const double *
       if (d0 < 1 || d0 >= a.nd) continue;
       double fz = gz - j0;
       double wz = j0 == j ? 1 - fz : fz;
-      const double *Cp = C + (((long)p * a.n_mu + k) * a.nd) * 2;
+      const double *Cp = C + (((long)p * a.n_mu + k) * a.nd) * ns;
       double t = a.ct[idx];
-      double aq = (1 - t) * Cp[d0 * 2] + t * Cp[(d0 - 1) * 2];
-      double as = (1 - t) * Cp[d0 * 2 + 1] + t * Cp[(d0 - 1) * 2 + 1];
+      double aq = (1 - t) * Cp[d0 * ns] + t * Cp[(d0 - 1) * ns];
+      double as = (1 - t) * Cp[d0 * ns + 1] + t * Cp[(d0 - 1) * ns + 1];
       acc += wy * wz * (a.cL[idx] * (kappa * as - aq));
+      if (ns == 3) {
+        double af = (1 - t) * Cp[d0 * ns + 2] + t * Cp[(d0 - 1) * ns + 2];
+        accf += wy * wz * (a.cL[idx] * (kappa_f * lb_factor(a, a.cmu[idx]) * af - aq));
+      }
     }
   }
   gmaps[((long)p * M + i) * M + j] = acc;
+  if (ns == 3) gfmaps[((long)p * M + i) * M + j] = accf;
 }
 
 // seed gradients gX_k[i] = sum_p sum_d HX_pk[d] G_I,p[i + dmin + d]; block = (tile, bin)
-__global__ __launch_bounds__(kDT) void k_diff_seedgrad(DiffArgs a, const double *GI, double *gq, double *gs) {
+template <int NS>
+__global__ __launch_bounds__(kDT) void k_diff_seedgrad(DiffArgs a, const double *GI, double *gq, double *gs,
+                                                       double *gf) {
   extern __shared__ double gl[];  // G_I,p over [i0 + dmin, i0 + kDT + dmin + nd - 1)
   int k = blockIdx.y;
   long i0 = (long)blockIdx.x * kDT;
   int nd = a.nd, len = kDT + nd - 1;
-  double aq = 0.0, as = 0.0;
+  double ax[NS];
+#pragma unroll
+  for (int q = 0; q < NS; q++) ax[q] = 0.0;
   for (int p = 0; p < a.n_phase; p++) {
     int lo = a.krange[((long)p * a.n_mu + k) * 2], hi = a.krange[((long)p * a.n_mu + k) * 2 + 1];
     if (lo > hi) continue;  // uniform
@@ -816,17 +872,18 @@ __global__ __launch_bounds__(kDT) void k_diff_seedgrad(DiffArgs a, const double 
       gl[e] = (j >= 0 && j < a.n) ? GI[(long)p * a.n + j] : 0.0;
     }
     __syncthreads();
-    const double *T = a.taps + ((long)p * a.n_mu + k) * nd * 2;
+    const double *T = a.taps + ((long)p * a.n_mu + k) * nd * NS;
     for (int d = lo; d <= hi; d++) {
       double g = gl[threadIdx.x + d];
-      aq = fma(T[d * 2], g, aq);
-      as = fma(T[d * 2 + 1], g, as);
+#pragma unroll
+      for (int q = 0; q < NS; q++) ax[q] = fma(T[d * NS + q], g, ax[q]);
     }
   }
   long i = i0 + threadIdx.x;
   if (i < a.n) {
-    gq[(long)k * a.n + i] = aq;
-    gs[(long)k * a.n + i] = as;
+    gq[(long)k * a.n + i] = ax[0];
+    gs[(long)k * a.n + i] = ax[1];
+    if (NS == 3) gf[(long)k * a.n + i] = ax[NS - 1];
   }
 }
 
@@ -837,20 +894,28 @@ hipError_t launch_diff_forward_spectra(const DiffArgs &a, const ConvArgs &c, con
   k_diff_cells<<<(unsigned)((ncell + kDT - 1) / kDT), kDT, 0, s>>>(a);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  long nacc = (long)a.n_mu * a.nd * 4;
+  if (a.ns != 2 && a.ns != 3) return hipErrorInvalidValue;
+  if (a.ns == 3 && (!a.f || !a.fmaps)) return hipErrorInvalidValue;
+  long nacc = (long)a.n_mu * a.nd * a.ns * 2;
   e = launch_fill_u64(a.acc, (size_t)a.n_phase * nacc, s);
   if (e != hipSuccess) return e;
   int use_lds = nacc * 8 <= 49152;
   int cpb = 4096;
   dim3 gt((unsigned)((ncell + cpb - 1) / cpb), a.n_phase);
-  k_diff_taps<<<gt, kDT, use_lds ? nacc * 8 : 0, s>>>(a, cpb, use_lds);
+  if (a.ns == 3)
+    k_diff_taps<3><<<gt, kDT, use_lds ? nacc * 8 : 0, s>>>(a, cpb, use_lds);
+  else
+    k_diff_taps<2><<<gt, kDT, use_lds ? nacc * 8 : 0, s>>>(a, cpb, use_lds);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   k_diff_taps_finalize<<<a.n_phase, 64, 0, s>>>(a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  size_t lds_fir = (size_t)2 * (kDT + a.nd - 1) * sizeof(double);
+  size_t lds_fir = (size_t)a.ns * (kDT + a.nd - 1) * sizeof(double);
   if (lds_fir > 65536) return hipErrorInvalidValue;
   dim3 gf((unsigned)((a.n + kDT - 1) / kDT), a.n_phase);
-  k_diff_fir<<<gf, kDT, lds_fir, s>>>(a);
+  if (a.ns == 3)
+    k_diff_fir<3><<<gf, kDT, lds_fir, s>>>(a);
+  else
+    k_diff_fir<2><<<gf, kDT, lds_fir, s>>>(a);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   size_t lds_conv = (size_t)(4 * kDT + 2 * c.ntp) * sizeof(double);
   if (lds_conv > 65536) return hipErrorInvalidValue;
@@ -865,11 +930,16 @@ int diff_conv_blocks(int64_t n) { return (int)((n + 4 * kDT - 1) / (4 * kDT)); }
 
 // loss = sum_p [w_spec r2_p + w_rv (RV_p - rv_t)^2 + w_fw (FWHM_p - fw_t)^2], r2_p the
 // residual partials of phase p summed in order; the same order as the host sum
+// With device parameters (graph replays) the loss is NaN when vsini leaves the
+// bound tap range or when the cell weights could exceed the fixed-point tap
+// capacity: n_cells max|L| max(3, |kappa|, |kappa_f| lbmax) >= 2^30, lbmax =
+// max |1 + LB(mu)| (maps are filling factors in [0, 1]).
 __global__ __launch_bounds__(kDT) void k_diff_loss(int n_phase, int res_blocks, const double *res_part,
                                                    const double *theta, const double *rv_t, const double *fw_t,
                                                    const double *bis_t, double w_spec, double w_rv, double w_fw,
                                                    double w_bis, double *bisd, int n_v, const double *prm,
-                                                   double vsini_max, double *loss, double *dth) {
+                                                   double vsini_max, double n_cells, double lbmax, double *loss,
+                                                   double *dth) {
   extern __shared__ double term[];  // [n_phase]
   for (int p = threadIdx.x; p < n_phase; p += blockDim.x) {
     double r2 = 0;
@@ -894,43 +964,57 @@ __global__ __launch_bounds__(kDT) void k_diff_loss(int n_phase, int res_blocks, 
     for (int p = 0; p < n_phase; p++) l += term[p];
     double vs = prm ? prm[0] : 0.0;
     if (prm && !(vs > 0 && vs <= vsini_max)) l = __longlong_as_double(0x7ff8000000000000LL);  // outside the tap range
+    if (prm) {
+      // max |L| over mu in [0, 1]: the ends and the vertex of the quadratic in (1 - mu)
+      const double u1 = prm[1], u2 = prm[2];
+      double lmax = fmax(1.0, fabs(1 - u1 - u2));
+      if (u2 != 0.0) {
+        double x = -u1 / (2 * u2);
+        if (x > 0 && x < 1) lmax = fmax(lmax, fabs(1 - u1 * x - u2 * x * x));
+      }
+      double w = lmax * fmax(3.0, fmax(fabs(prm[3]), fabs(prm[4]) * lbmax));
+      if (!(n_cells * w < 1073741824.0)) l = __longlong_as_double(0x7ff8000000000000LL);
+    }
     loss[0] = l;
   }
 }
 
-// g4[q] = sum over blocks of cell_part[b][q]: thread t sums blocks t, t + kDT, ...
+// g[q] = sum over blocks of cell_part[b][q]: thread t sums blocks t, t + kDT, ...
 // (independent loads), then a fixed-order tree over the threads
-__global__ __launch_bounds__(kDT) void k_diff_grad4(int n_blocks, const double *cell_part, double *g4) {
-  __shared__ double red[4][kDT];
-  double s4[4] = {0, 0, 0, 0};
+__global__ __launch_bounds__(kDT) void k_diff_grad(int n_blocks, const double *cell_part, double *g) {
+  __shared__ double red[kDiffScalars][kDT];
+  double sq[kDiffScalars];
+#pragma unroll
+  for (int q = 0; q < kDiffScalars; q++) sq[q] = 0.0;
   for (int b = threadIdx.x; b < n_blocks; b += kDT)
 #pragma unroll
-    for (int q = 0; q < 4; q++) s4[q] += cell_part[(long)b * 4 + q];
+    for (int q = 0; q < kDiffScalars; q++) sq[q] += cell_part[(long)b * kDiffScalars + q];
 #pragma unroll
-  for (int q = 0; q < 4; q++) red[q][threadIdx.x] = s4[q];
+  for (int q = 0; q < kDiffScalars; q++) red[q][threadIdx.x] = sq[q];
   __syncthreads();
   for (int off = kDT / 2; off > 0; off >>= 1) {
     if (threadIdx.x < off)
 #pragma unroll
-      for (int q = 0; q < 4; q++) red[q][threadIdx.x] += red[q][threadIdx.x + off];
+      for (int q = 0; q < kDiffScalars; q++) red[q][threadIdx.x] += red[q][threadIdx.x + off];
     __syncthreads();
   }
-  if (threadIdx.x < 4) g4[threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < kDiffScalars) g[threadIdx.x] = red[threadIdx.x][0];
 }
 
 hipError_t launch_diff_loss(int n_phase, int res_blocks, const double *res_part, const double *theta,
                             const double *rv_t, const double *fw_t, const double *bis_t, double w_spec, double w_rv,
                             double w_fw, double w_bis, double *bisd, int n_v, const double *prm, double vsini_max,
-                            double *loss, double *dth, hipStream_t s) {
+                            double n_cells, double lbmax, double *loss, double *dth, hipStream_t s) {
   if (n_phase < 1 || (size_t)n_phase * 8 > 65536) return hipErrorInvalidValue;
   if (w_bis != 0.0 && (!bisd || !bis_t)) return hipErrorInvalidValue;
   k_diff_loss<<<1, kDT, (size_t)n_phase * 8, s>>>(n_phase, res_blocks, res_part, theta, rv_t, fw_t, bis_t, w_spec,
-                                                  w_rv, w_fw, w_bis, bisd, n_v, prm, vsini_max, loss, dth);
+                                                  w_rv, w_fw, w_bis, bisd, n_v, prm, vsini_max, n_cells, lbmax, loss,
+                                                  dth);
   return hipGetLastError();
 }
 
-hipError_t launch_diff_grad4(int n_blocks, const double *cell_part, double *g4, hipStream_t s) {
-  k_diff_grad4<<<1, kDT, 0, s>>>(n_blocks, cell_part, g4);
+hipError_t launch_diff_grad(int n_blocks, const double *cell_part, double *g, hipStream_t s) {
+  k_diff_grad<<<1, kDT, 0, s>>>(n_blocks, cell_part, g);
   return hipGetLastError();
 }
 int diff_cell_blocks(int64_t ncell) { return (int)((ncell + kDT - 1) / kDT); }
@@ -946,9 +1030,10 @@ hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const doub
                                 const double *v, int n_v, const double *ccf, const double *theta, const double *dth,
                                 double *gy, int64_t n_t, const int64_t *mt_ptr, const int32_t *mt_w,
                                 const int32_t *mt_col, const double *mt_val, double *GF, double *GI, double *Cpart,
-                                double *C, double *gmaps, double *cell_part, double *gq, double *gs,
-                                const double *bisd, hipStream_t s) {
+                                double *C, double *gmaps, double *gfmaps, double *cell_part, double *gq, double *gs,
+                                double *gf, const double *bisd, hipStream_t s) {
   hipError_t e;
+  if (a.ns == 3 && (!gf || !gfmaps)) return hipErrorInvalidValue;
   k_fit_adjoint<<<a.n_phase, 64, 0, s>>>(v, n_v, ccf, theta, dth, bisd, gy);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   long tot = (long)a.n_phase * a.n;
@@ -970,20 +1055,27 @@ hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const doub
   if (lds_corr > 65536) return hipErrorInvalidValue;
   const int nchunk = diff_corr_chunks(a.n);
   dim3 gr((unsigned)nchunk, a.n_mu, (a.n_phase + kCorrPG - 1) / kCorrPG);
-  k_diff_corr<<<gr, kDT, lds_corr, s>>>(a, GI, Cpart);
+  k_diff_corr<<<gr, kDT, lds_corr, s>>>(a, GI, Cpart, a.q, a.s, 2, 0);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  long nc = (long)a.n_phase * a.n_mu * a.nd * 2;
+  if (a.ns == 3) {
+    k_diff_corr<<<gr, kDT, lds_corr, s>>>(a, GI, Cpart, a.f, nullptr, 1, 2);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  long nc = (long)a.n_phase * a.n_mu * a.nd * a.ns;
   k_diff_corr_reduce<<<(unsigned)((nc + kDT - 1) / kDT), kDT, 0, s>>>(a, a.n_phase * a.n_mu, a.nd, nchunk, Cpart, C);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   long ncell = (long)a.grid * a.grid;
   k_diff_cellgrad<<<(unsigned)((ncell + kDT - 1) / kDT), kDT, 0, s>>>(a, C, cell_part);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   dim3 gmg((unsigned)((a.map_res + kDT - 1) / kDT), a.map_res, a.n_phase);
-  k_diff_mapgrad<<<gmg, kDT, 0, s>>>(a, C, gmaps);
+  k_diff_mapgrad<<<gmg, kDT, 0, s>>>(a, C, gmaps, gfmaps);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   size_t lds_seed = (size_t)(kDT + a.nd - 1) * sizeof(double);
   dim3 gsd((unsigned)((a.n + kDT - 1) / kDT), a.n_mu);
-  k_diff_seedgrad<<<gsd, kDT, lds_seed, s>>>(a, GI, gq, gs);
+  if (a.ns == 3)
+    k_diff_seedgrad<3><<<gsd, kDT, lds_seed, s>>>(a, GI, gq, gs, gf);
+  else
+    k_diff_seedgrad<2><<<gsd, kDT, lds_seed, s>>>(a, GI, gq, gs, gf);
   return hipGetLastError();
 }
 

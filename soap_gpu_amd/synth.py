@@ -351,21 +351,61 @@ def smooth_blob_maps(rng: np.random.Generator, n_phase: int, res: int, n_blobs: 
     return out
 
 
+def region_maps(regions, phases, incl_deg: float, res: int):
+    """Discrete regions rendered as per-phase spot and facula filling-factor maps
+    [P][res][res] (values 0 / 1 at the bilinear nodes y, z = -1 + 2 i / (res - 1)).
+    A region (lat, lon, size, type) is the cap u . c >= sqrt(1 - size^2) of the
+    unit sphere around its centre c (the membership test x^2 >= 1 - s^2 of
+    SOAP_integration.cu:364, in the region's own frame), c at longitude
+    lon + 360 phase on the star rotating about its axis, inclined by incl_deg to
+    the line of sight; later regions win shared nodes (SOAP_integration.cu:364-381).
+    Observer frame: x toward the observer, y along the rotation velocity of the
+    disk centre (the model's delta = vsini y), z up."""
+    g = np.linspace(-1.0, 1.0, res)
+    Y, Z = np.meshgrid(g, g, indexing="ij")
+    spot = np.zeros((len(phases), res, res))
+    fac = np.zeros_like(spot)
+    for p, ph in enumerate(phases):
+        owner = region_owner(regions, ph, incl_deg, Y, Z)
+        for j, (_, _, _, typ) in enumerate(regions):
+            (spot if typ == 0 else fac)[p][owner == j] = 1.0
+    return spot, fac
+
+
+def region_owner(regions, phase: float, incl_deg: float, Y, Z):
+    """Index of the region owning each disk point (Y, Z) at a phase (the last
+    region containing it), -1 where none does or off the disk (geometry of
+    region_maps)."""
+    r2 = Y * Y + Z * Z
+    on = r2 <= 1.0
+    X = np.sqrt(np.clip(1.0 - r2, 0.0, None))
+    si, ci = np.sin(np.deg2rad(incl_deg)), np.cos(np.deg2rad(incl_deg))
+    owner = np.full(np.shape(Y), -1)
+    for j, (lat, lon, size, _typ) in enumerate(regions):
+        la, lo = np.deg2rad(lat), np.deg2rad(lon + 360.0 * phase)
+        cx, cy, cz = np.cos(la) * np.cos(lo), np.cos(la) * np.sin(lo), np.sin(la)
+        ox, oy, oz = cx * si + cz * ci, cy, -cx * ci + cz * si
+        owner[on & (X * ox + Y * oy + Z * oz >= np.sqrt(1.0 - size * size))] = j
+    return owner
+
+
 def config3(seed: int = 1237, n_phase: int = 50, grid: int = 600, map_res: int = 512, n_mu: int = 10):
     """Adjoint shape: grid 600, lambda 5000-5500 A log-uniform (~100k points), 50
-    phases, 8000 seeded lines (all in the mask), a 512 x 512 smooth-blob filling
-    factor map per phase.  Returns (workload, maps)."""
+    phases, 8000 seeded lines (all in the mask), 512 x 512 smooth-blob spot and
+    facula filling-factor maps per phase (facula blobs scaled to 0.6).  Returns
+    (workload, spot maps, facula maps)."""
     rng = np.random.default_rng(seed)
     wave = log_grid(5000.0, 5500.0, n=100000)
     lines = draw_lines(rng, 8000, 5000.0, 5500.0)
     bounds, _ = mu_bins(n_mu)
     cubes = make_cubes(wave, lines, n_mu)
     maps = smooth_blob_maps(rng, n_phase, map_res)
+    fmaps = 0.6 * smooth_blob_maps(rng, n_phase, map_res)
     phases = np.arange(n_phase, dtype=np.float64) / n_phase
     w = Workload("config3", wave, 1, bounds, cubes, lines, lines["center"].copy(), lines["depth"].copy(),
                  _star(grid, 2.0), _act(), phases, [[] for _ in phases], sigma_inst_ms(115000),
                  vel_grid(20000.0, 250.0), meta={"map_res": map_res})
-    return w, maps
+    return w, maps, fmaps
 
 
 def config4_params(seed: int = 1238, n_inst: int = 128):
@@ -379,11 +419,14 @@ def config4_params(seed: int = 1238, n_inst: int = 128):
 
 
 def config4_instance(params, map_res: int = 256, i: int = 0):
-    """(workload, true maps, truth) of one fitting instance: a log-grid instance as
-    in the batched workload, and true filling-factor maps of n_reg smooth blobs;
-    truth = dict(vsini, contrast)."""
+    """(workload, spot maps, facula maps, truth) of one fitting instance: a log-grid
+    instance as in the batched workload with its 1-4 discrete regions (faculae,
+    each with a spot inside, last region wins), rendered per phase as true spot and
+    facula filling-factor maps (region_maps, inclination 90 deg); truth =
+    dict(vsini, contrast, contrast_fac)."""
     cs, n, grid, n_phase, n_reg, vsini = params
     rng = np.random.default_rng(cs)
     w = batch_instance(rng, n, grid, n_phase, n_reg, vsini, 90.0, name=f"config4[{i}]")
-    maps = smooth_blob_maps(rng, n_phase, map_res, n_blobs=n_reg)
-    return w, maps, dict(vsini=vsini, contrast=float(rng.uniform(0.2, 0.5)))
+    spot, fac = region_maps(w.regions[0], w.phases, 90.0, map_res)
+    return w, spot, fac, dict(vsini=vsini, contrast=float(rng.uniform(0.2, 0.5)),
+                              contrast_fac=float(rng.uniform(1.05, 1.3)))

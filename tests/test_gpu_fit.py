@@ -44,29 +44,31 @@ def test_lbfgs_fit_reduces_loss_and_recovers_vsini():
 
 
 def test_graph_evaluation_matches_per_call():
-    """One graph replay (forward + adjoint of two uneven instances, parameters,
-    maps and results in device buffers) against the per-call path at two
-    parameter sets, replaying the same graph: loss, the four scalar gradients and
-    the seed / map gradients agree to rounding."""
+    """One graph replay (forward + adjoint of two uneven instances, the first with
+    the facula population and the second without; parameters, maps and results in
+    device buffers) against the per-call path at two parameter sets, replaying
+    the same graph: loss, the scalar gradients and the seed / map gradients agree
+    to rounding."""
     import torch
 
     from soap_gpu_amd.diff import DiffBatch
 
     rng = np.random.default_rng(12)
-    specs = ((3100, 36, 3, 2, 3.5), (4300, 48, 4, 1, 6.0))
+    specs = ((3100, 36, 3, 2, 3.5, True), (4300, 48, 4, 1, 6.0, False))
     made = []
-    for n, grid, P, nreg, vs in specs:
+    for n, grid, P, nreg, vs, fac in specs:
         w = synth.batch_instance(rng, n, grid, P, nreg, vs, 90.0, n_mu=5)
         maps = synth.smooth_blob_maps(rng, P, 12, n_blobs=nreg)
-        made.append((w, maps, vs, P))
+        fmaps = synth.smooth_blob_maps(rng, P, 12, n_blobs=nreg) * 0.6 if fac else None
+        made.append((w, maps, fmaps, vs, P))
     weights = (1e-3, 1e-2, 1e-4, 1e-2)
-    graph_insts = [Instance(w, maps, P, 12, weights=weights) for w, maps, vs, P in made]
-    call_insts = [Instance(w, maps, P, 12, weights=weights) for w, maps, vs, P in made]
+    graph_insts = [Instance(w, maps, P, 12, weights=weights, fmaps0=fm) for w, maps, fm, vs, P in made]
+    call_insts = [Instance(w, maps, P, 12, weights=weights, fmaps0=fm) for w, maps, fm, vs, P in made]
     try:
         dev = torch.device("cuda")
         bufs = []
         batch = DiffBatch([g.model for g in graph_insts])
-        for i, (g, (w, maps, vs, P)) in enumerate(zip(graph_insts, made)):
+        for i, (g, (w, maps, fm, vs, P)) in enumerate(zip(graph_insts, made)):
             rv_t = rng.uniform(-50, 50, P)
             fw_t = rng.uniform(7000, 9000, P)
             bis_t = rng.uniform(-30, 30, P)
@@ -75,43 +77,52 @@ def test_graph_evaluation_matches_per_call():
             tgt = torch.tensor(rng.uniform(0.9, 1.0, (P, w.n)), dtype=torch.float64, device=dev)
             g.target = tgt
             call_insts[i].target = tgt.clone()
-            b = dict(prm=torch.empty(4, dtype=torch.float64, device=dev),
-                     maps=torch.empty((P, 12, 12), dtype=torch.float64, device=dev),
+            mk = lambda: torch.empty((P, 12, 12), dtype=torch.float64, device=dev)  # noqa: E731
+            b = dict(prm=torch.empty(5, dtype=torch.float64, device=dev), maps=mk(),
+                     fmaps=mk() if g.facula else None,
                      rv=torch.tensor(rv_t, dtype=torch.float64, device=dev),
                      fw=torch.tensor(fw_t, dtype=torch.float64, device=dev),
                      bis=torch.tensor(bis_t, dtype=torch.float64, device=dev),
                      loss=torch.empty(1, dtype=torch.float64, device=dev),
-                     g4=torch.empty(4, dtype=torch.float64, device=dev),
+                     g5=torch.empty(5, dtype=torch.float64, device=dev),
                      gq=torch.empty_like(g.q), gs=torch.empty_like(g.s),
-                     gm=torch.empty((P, 12, 12), dtype=torch.float64, device=dev))
+                     gf=torch.empty_like(g.q) if g.facula else None, gm=mk(), gfm=mk() if g.facula else None)
             batch.bind(i, 15.0, b["prm"], g.q, g.s, b["maps"], g.target, b["rv"], b["fw"], b["bis"], weights,
-                       b["loss"], b["g4"], b["gq"], b["gs"], b["gm"])
+                       b["loss"], b["g5"], b["gq"], b["gs"], b["gm"], f=g.f, fmaps=b["fmaps"], gf=b["gf"],
+                       gfm=b["gfm"])
             bufs.append(b)
         for rep, scale in enumerate((1.0, 1.3)):
-            for (w, maps, vs, P), b in zip(made, bufs):
-                b["prm"].copy_(torch.tensor([vs * scale, 0.29, 0.34, 0.3 + 0.1 * rep], dtype=torch.float64,
-                                            device=dev))
+            for (w, maps, fm, vs, P), b in zip(made, bufs):
+                b["prm"].copy_(torch.tensor([vs * scale, 0.29, 0.34, 0.3 + 0.1 * rep, 1.1 + 0.05 * rep],
+                                            dtype=torch.float64, device=dev))
                 b["maps"].copy_(torch.tensor(np.clip(maps * scale, 0, 1), device=dev))
+                if fm is not None:
+                    b["fmaps"].copy_(torch.tensor(np.clip(fm * scale, 0, 1), device=dev))
             torch.cuda.synchronize()
             batch.step(graph=True)
             batch.sync()
-            for (w, maps, vs, P), b, c in zip(made, bufs, call_insts):
-                prm = (vs * scale, 0.29, 0.34, 0.3 + 0.1 * rep)
+            for (w, maps, fm, vs, P), b, c in zip(made, bufs, call_insts):
+                prm = (vs * scale, 0.29, 0.34, 0.3 + 0.1 * rep, 1.1 + 0.05 * rep)
                 mp = torch.tensor(np.clip(maps * scale, 0, 1), device=dev)
+                fmp = None if fm is None else torch.tensor(np.clip(fm * scale, 0, 1), device=dev)
                 loss, _, _, _ = c.model.forward(prm, c.q, c.s, mp, c.target, c.rv_t, c.fw_t, *weights[:3],
-                                                bis_t=c.bis_t, w_bis=weights[3])
-                g4, gq, gs, gm = c.model.backward()
+                                                bis_t=c.bis_t, w_bis=weights[3], f=c.f, fmaps=fmp)
+                g5, gq, gs, gm, gf, gfm = c.model.backward()
                 # the same kernels in the same order on the same inputs
                 assert float(b["loss"][0]) == loss, rep
-                np.testing.assert_array_equal(b["g4"].cpu().numpy(), g4)
-                for got, ref in ((b["gq"], gq), (b["gs"], gs), (b["gm"], gm)):
+                np.testing.assert_array_equal(b["g5"].cpu().numpy(), g5)
+                pairs = [(b["gq"], gq), (b["gs"], gs), (b["gm"], gm)]
+                if fm is not None:
+                    pairs += [(b["gf"], gf), (b["gfm"], gfm)]
+                    assert g5[4] != 0.0
+                for got, ref in pairs:
                     torch.testing.assert_close(got, ref, rtol=1e-9, atol=1e-12 * float(ref.abs().max()))
         # the forward-only graph gives the same losses and leaves the gradients alone
         losses = [float(b["loss"][0]) for b in bufs]
-        g4s = [b["g4"].clone() for b in bufs]
+        g5s = [b["g5"].clone() for b in bufs]
         for b in bufs:
             b["loss"].fill_(0.0)
-            b["g4"].fill_(7.0)
+            b["g5"].fill_(7.0)
         torch.cuda.synchronize()
         batch.step(graph=True, adjoint=False)
         batch.sync()
@@ -119,9 +130,9 @@ def test_graph_evaluation_matches_per_call():
         batch.sync()
         for b, l0 in zip(bufs, losses):
             assert float(b["loss"][0]) == l0
-            assert bool(torch.all(b["g4"] == 7.0))
-        for b, g in zip(bufs, g4s):
-            b["g4"].copy_(g)
+            assert bool(torch.all(b["g5"] == 7.0))
+        for b, g in zip(bufs, g5s):
+            b["g5"].copy_(g)
         # a bound vsini beyond the graph's tap range gives NaN, not a silently clipped model
         bufs[0]["prm"][0] = 20.0
         torch.cuda.synchronize()
