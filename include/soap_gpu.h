@@ -100,7 +100,8 @@ const char *soap_version(void);
 int soap_set_wave(soap_ctx *ctx, int kind, const double *wave, int64_t n, double step);
 /* mu boundaries (mu_boundary_array.bin, SOAP_initialize.cu:567-574); n_mu = n_bounds + 1 */
 int soap_set_mu_bounds(soap_ctx *ctx, const double *bounds, int32_t n_bounds);
-/* seed cube [n_mu][n] host -> device (the fread + cudaMemcpy of SOAP_integration.cu:504-580) */
+/* seed cube [n_mu][n] host -> device (the fread + cudaMemcpy of SOAP_integration.cu:504-580);
+ * SOAP_ERR_ARG if a value is not finite */
 int soap_set_cube(soap_ctx *ctx, int which, const double *cube, int32_t n_mu);
 /* seed cube already resident on the device (not copied; must outlive its use) */
 int soap_set_cube_device(soap_ctx *ctx, int which, const double *dev_cube, int32_t n_mu);
@@ -181,9 +182,14 @@ int soap_fit_ccf_host(soap_ctx *ctx, const double *v, int32_t n_v, const double 
 /* the same with per-point errors (CCF_error.txt, BIS_FIT2.cpp:114-119): residuals
  * (fit - y)/err and Jacobian rows divided by err (BIS_FIT2.cpp:49,74-77);
  * err[n_ccf*n_v], NULL = unit errors.  Errors must be finite and non-zero
- * (SOAP_ERR_ARG otherwise; the reference would divide by zero). */
+ * (SOAP_ERR_ARG otherwise; the reference would divide by zero).  status[n_ccf]
+ * (may be NULL) receives the solver's outcome per CCF, in GSL's codes: 0 = the
+ * xtol or gtol test ended the fit, 27 (GSL_ENOPROG) = no step reduced the
+ * residual (a flat or degenerate CCF; the outputs hold the starting point),
+ * 11 (GSL_EMAXITER) = 10000 iterations.  The reference ignores this outcome
+ * (BIS_FIT2.cpp:191) and so does the call: it returns SOAP_OK either way. */
 int soap_fit_ccf_host_err(soap_ctx *ctx, const double *v, int32_t n_v, const double *y, const double *err,
-                          int32_t n_ccf, double *par, double *model, double *depth, double *bis);
+                          int32_t n_ccf, double *par, double *model, double *depth, double *bis, int32_t *status);
 
 /* ------------------------------------------------------------ seed preprocessing: bisector injection */
 /* The convective-blueshift injection of CB_injection_faculae.py:191-244 on one
@@ -260,31 +266,46 @@ int soap_set_fir_shape(soap_ctx *ctx, int32_t phases_per_block, int32_t pixels_p
  * No counterpart in the reference (a differentiable forward model with its FP64
  * adjoint, for fitting).  Model on the context's log-uniform grid, mu bounds
  * and mask (soap_set_wave, soap_set_mu_bounds, soap_set_mask): disk grid G,
- * rigid rotation delta = vsini y, quadratic limb darkening, per-phase filling
- * factor maps f_p (M x M over [-1, 1]^2, bilinear) blending the quiet seed rows
- * Q and the spot rows S scaled by the contrast:
- *   I_p = sum_c L_c [(1 - f) Shift(Q_k) + contrast f Shift(S_k)],
+ * rigid rotation delta = vsini y, quadratic limb darkening L, per-phase spot and
+ * facula filling-factor maps phi_p, psi_p (M x M over [-1, 1]^2, bilinear)
+ * blending the quiet seed rows Q, the spot rows S scaled by the spot contrast
+ * and the facula rows Fa scaled by the facula contrast and the reference's
+ * facula intensity factor 1 + LB(mu) (SOAP_integration.cu:317-323,
+ * LB = lb0 + lb1 mu + lb2 mu^2 + lb3 mu^3 as in SOAP_initialize.cu:367):
+ *   I_p = sum_c L_c [(1 - phi - psi) Shift(Q_k) + contrast phi Shift(S_k)
+ *                    + contrast_fac (1 + LB(mu_c)) psi Shift(Fa_k)],
  *   F_p = instrumental convolution of I_p, CCF_p = raw mask CCF of F_p,
  *   (RV_p, FWHM_p) = Gaussian fit of CCF_p polished to the least-squares optimum,
  *   BIS_p = bisector span of CCF_p at that Gaussian (BIS_FIT2.cpp:214-268),
  *   loss = w_spec sum (F_p - T_p)^2 over the trimmed pixels
  *        + w_rv sum (RV_p - rv_target_p)^2 + w_fwhm sum (FWHM_p - fwhm_target_p)^2
  *        + w_bis sum (BIS_p - bis_target_p)^2.
+ * The facula population is optional: without facula seeds and maps psi = 0.
+ * Maps are filling factors in [0, 1].  The per-cell weights enter fixed-point tap
+ * sums: G^2 max|L| max(3, |contrast|, |contrast_fac| max|1 + LB|) must stay below
+ * 2^30 (SOAP_ERR_LIMIT from soap_diff_forward, a NaN loss in graph replays).
  * Array arguments named d_* are device (HIP) pointers: seeds [n_mu][n], maps
  * [P][M][M], target [P][n]; they must stay valid until soap_diff_backward. */
 typedef struct soap_diff_params {
-  double vsini_kms, u1, u2, contrast;
+  double vsini_kms, u1, u2, contrast, contrast_fac;
 } soap_diff_params;
-int soap_diff_setup(soap_ctx *ctx, int32_t grid, int32_t n_phase, int32_t map_res, double sigma_ms);
-/* rv_target, fwhm_target, bis_target [P], loss (scalar), rv, fwhm, bis [P]: host
- * arrays (rv, fwhm, bis may be NULL) */
+/* lb: the facula intensity polynomial (4 values, constant term first), NULL = zeros */
+int soap_diff_setup(soap_ctx *ctx, int32_t grid, int32_t n_phase, int32_t map_res, double sigma_ms,
+                    const double *lb);
+/* d_seed_f and d_fac_maps: both NULL (no faculae) or both set.  rv_target,
+ * fwhm_target, bis_target [P], loss (scalar), rv, fwhm, bis [P]: host arrays
+ * (rv, fwhm, bis may be NULL) */
 int soap_diff_forward(soap_ctx *ctx, const soap_diff_params *prm, const double *d_seed_q, const double *d_seed_s,
-                      const double *d_maps, const double *d_target, const double *rv_target,
-                      const double *fwhm_target, const double *bis_target, double w_spec, double w_rv, double w_fwhm,
-                      double w_bis, double *loss, double *rv, double *fwhm, double *bis);
-/* gradients of the last forward's loss: grad4 (host) = d/d(vsini, u1, u2, contrast);
- * d_grad_q, d_grad_s [n_mu][n] and d_grad_maps [P][M][M] on the device (overwritten) */
-int soap_diff_backward(soap_ctx *ctx, double *grad4, double *d_grad_q, double *d_grad_s, double *d_grad_maps);
+                      const double *d_seed_f, const double *d_maps, const double *d_fac_maps, const double *d_target,
+                      const double *rv_target, const double *fwhm_target, const double *bis_target, double w_spec,
+                      double w_rv, double w_fwhm, double w_bis, double *loss, double *rv, double *fwhm, double *bis);
+/* gradients of the last forward's loss: grad5 (host) = d/d(vsini, u1, u2, contrast,
+ * contrast_fac) (the last is 0 without faculae); d_grad_q, d_grad_s, d_grad_f
+ * [n_mu][n] and d_grad_maps, d_grad_fac_maps [P][M][M] on the device (overwritten;
+ * d_grad_f and d_grad_fac_maps are required when the forward had faculae and
+ * ignored otherwise) */
+int soap_diff_backward(soap_ctx *ctx, double *grad5, double *d_grad_q, double *d_grad_s, double *d_grad_f,
+                       double *d_grad_maps, double *d_grad_fac_maps);
 /* host copies of the last forward: what 0 = F [P][n], 1 = raw CCF [P][n_v], 2 = (c, k, RV, FWHM) [P][4],
  * 3 = BIS span [P] */
 int soap_diff_get(soap_ctx *ctx, int32_t what, double *out);
@@ -340,9 +361,12 @@ int soap_batch_tune(soap_batch *b, int32_t reps);
  * No counterpart in the reference.  Forward + adjoint of many instances (each a
  * soap_ctx after soap_diff_setup) with every input, parameter and result in
  * device memory, so that one evaluation is one HIP graph launch:
- *   d_params [4] = (vsini, u1, u2, contrast), d_q / d_s [n_mu][n], d_maps [P][M][M],
- *   d_target [P][n], d_rv_t / d_fw_t / d_bis_t [P]  ->  d_loss [1], d_grad4 [4],
- *   d_grad_q / d_grad_s [n_mu][n], d_grad_maps [P][M][M].
+ *   d_params [5] = (vsini, u1, u2, contrast, contrast_fac), d_q / d_s / d_f [n_mu][n],
+ *   d_maps / d_fac_maps [P][M][M], d_target [P][n], d_rv_t / d_fw_t / d_bis_t [P]
+ *   ->  d_loss [1], d_grad5 [5], d_grad_q / d_grad_s / d_grad_f [n_mu][n],
+ *   d_grad_maps / d_grad_fac_maps [P][M][M].
+ * The facula pointers d_f, d_fac_maps, d_grad_f, d_grad_fac_maps are all NULL (an
+ * instance without faculae) or all set.
  * The tap range is fixed at bind time by vsini_max; an evaluation with vsini
  * outside (0, vsini_max] returns loss = NaN.  The graph is captured on the first
  * step after a bind and replayed afterwards (values at the bound addresses may
@@ -353,10 +377,11 @@ int soap_diff_batch_create(soap_ctx *const *ctxs, int32_t n_inst, soap_diffbatch
 void soap_diff_batch_destroy(soap_diffbatch *b);
 const char *soap_diff_batch_last_error(const soap_diffbatch *b);
 int soap_diff_batch_bind(soap_diffbatch *b, int32_t inst, double vsini_max, const double *d_params,
-                         const double *d_seed_q, const double *d_seed_s, const double *d_maps, const double *d_target,
-                         const double *d_rv_target, const double *d_fwhm_target, const double *d_bis_target,
-                         double w_spec, double w_rv, double w_fwhm, double w_bis, double *d_loss, double *d_grad4,
-                         double *d_grad_q, double *d_grad_s, double *d_grad_maps);
+                         const double *d_seed_q, const double *d_seed_s, const double *d_seed_f, const double *d_maps,
+                         const double *d_fac_maps, const double *d_target, const double *d_rv_target,
+                         const double *d_fwhm_target, const double *d_bis_target, double w_spec, double w_rv,
+                         double w_fwhm, double w_bis, double *d_loss, double *d_grad5, double *d_grad_q,
+                         double *d_grad_s, double *d_grad_f, double *d_grad_maps, double *d_grad_fac_maps);
 /* enqueue forward + loss of every instance, and the adjoint when with_adjoint != 0
  * (without it d_grad* are left as they are); use_graph != 0 replays a captured
  * graph (one per with_adjoint value) */

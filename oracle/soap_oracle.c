@@ -873,7 +873,7 @@ int orc_fit_ccf_err(const double *v, const double *y, const double *err, int n, 
     D[j] = sqrt(s) > 0 ? sqrt(s) : 1.0;
   }
   double mu = 1.0e-3, nu = 2.0;
-  int iter = 0, status = 0;
+  int iter = 0, status = 0, done = 0;
   for (iter = 0; iter < max_iter; iter++) {
     double A[4][4], g[4];
     for (int a = 0; a < 4; a++) {
@@ -935,6 +935,7 @@ int orc_fit_ccf_err(const double *v, const double *y, const double *err, int n, 
       if (!(fabs(dx[a]) < xtol * xtol + xtol * fabs(x[a]))) small = 0;
     if (small) {
       iter++;
+      done = 1;
       break;
     }
     double gn = 0, f2 = 0;
@@ -948,9 +949,11 @@ int orc_fit_ccf_err(const double *v, const double *y, const double *err, int n, 
     double phi = 0.5 * f2;
     if (gn <= gtol * (phi > 1.0 ? phi : 1.0)) {
       iter++;
+      done = 1;
       break;
     }
   }
+  if (status == 0 && !done) status = 11; /* GSL_EMAXITER */
   if (n_iter) *n_iter = iter;
   c = x[0];
   k = x[1];

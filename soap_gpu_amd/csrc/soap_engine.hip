@@ -88,17 +88,21 @@ struct soap_ctx {
   int32_t *d_geo_off = nullptr, *d_box = nullptr;
   int max_col_blocks = 0;
   int fir_g = 0, fir_b = -1;  // FIR launch shape (phases per block, pixels per thread); -1 = automatic
+  double tap_wmax = 0;        // bound on |limb| and |limb (1 + LB)| over the disk (fixed-point tap capacity)
+  double solar_leb = 1;       // max over the facula temperatures of sum_q |l_q(T)| (solar Lagrange weights)
   // differentiable path
   struct {
     bool ready = false, have_fwd = false;
     int G = 0, P = 0, M = 0, dmin = 0, nd = 0, res_blocks = 0;
     double sigma = 0, w_spec = 0;
     soap_diff_params prm{};
-    const double *q = nullptr, *s = nullptr, *maps = nullptr, *target = nullptr;
+    double lb[4] = {0, 0, 0, 0};  // facula intensity polynomial, constant term first
+    int ns = 2;                   // tap sets of the last forward: 2, or 3 with the facula population
+    const double *q = nullptr, *s = nullptr, *f = nullptr, *maps = nullptr, *fmaps = nullptr, *target = nullptr;
     DevBuf ck, cm, ct, cL, cmu, cdt, acc, taps, krange, I, F, res_part, ccf, par, model, depth, bis, iters, th,
         dth, gy, GF, GI, C, Cpart, cell_part;
     DevBuf bisd;  // [P][n_v + kBisHead]: span, loss coefficient, dspan/dtheta, dspan/dy
-    DevBuf tg;    // per-call targets [3][P] (RV, FWHM, BIS span), the loss [1], the scalar gradients [4]
+    DevBuf tg;    // per-call targets [3][P] (RV, FWHM, BIS span), the loss [1], the scalar gradients [5]
     DevBuf mt_ptr, mt_w, mt_col, mt_val;  // the mask in sliced pixel-major form (adjoint of the CCF)
     bool mt_ready = false;
   } diff;
@@ -458,6 +462,12 @@ int soap_set_cube(soap_ctx *c, int which, const double *cube, int32_t n_mu) {
   if (!c || !cube || which < 0 || which > 5) return fail(c, SOAP_ERR_ARG, "set_cube: bad arguments");
   if (c->n == 0) return fail(c, SOAP_ERR_STATE, "set_cube: set the wavelength grid first");
   if (n_mu < 1) return fail(c, SOAP_ERR_ARG, "set_cube: n_mu < 1");
+  // the reference reads seed cubes without checks and carries a NaN into every
+  // spectrum; here a non-finite value is refused with its position
+  for (size_t i = 0, cnt = (size_t)n_mu * c->n; i < cnt; i++)
+    if (!std::isfinite(cube[i]))
+      return fail(c, SOAP_ERR_ARG, "set_cube: cube %d holds %g at mu row %zu, pixel %zu", which, cube[i],
+                  i / (size_t)c->n, i % (size_t)c->n);
   HIPCHK(c, hipSetDevice(c->device));
   if (!c->cube_owned[which]) c->d_cube[which] = nullptr;
   int rc = dalloc(c, &c->d_cube[which], (size_t)n_mu * c->n);
@@ -561,6 +571,29 @@ static LinArgs lin_args(soap_ctx *c) {
   return a;
 }
 
+// max over mu in [0, 1] of |L| and |L (1 + LB)|, L = 1 - u1 (1-mu) - u2 (1-mu)^2,
+// LB = lb0 + lb1 mu + lb2 mu^2 + lb3 mu^3 (sampled finely, with a 1 % margin)
+static double limb_weight_bound(double u1, double u2, const double *lb) {
+  double w = 0;
+  for (int i = 0; i <= 4096; i++) {
+    double mu = i / 4096.0, L = 1 - u1 * (1 - mu) - u2 * (1 - mu) * (1 - mu);
+    double f = 1 + (lb[0] + lb[1] * mu + lb[2] * mu * mu + lb[3] * mu * mu * mu);
+    w = std::max(w, std::max(std::fabs(L), std::fabs(L * f)));
+  }
+  return 1.01 * w;
+}
+
+// every contribution to a tap is a cell weight times a fraction in [0, 1]; with
+// at most G^2 cells per tap the fixed-point sums need G^2 w_max < 2^30
+static int check_tap_capacity(soap_ctx *c, double w_max, const char *what) {
+  double ncell = (double)c->G * c->G;
+  if (!fx_capacity_ok(ncell, w_max))
+    return fail(c, SOAP_ERR_LIMIT,
+                "%s: %d^2 cells with weights up to %g exceed the fixed-point tap capacity (cells x weight < 2^30)",
+                what, c->G, w_max);
+  return SOAP_OK;
+}
+
 int soap_disk_init(soap_ctx *c, const soap_star *st) {
   if (!c || !st) return fail(c, SOAP_ERR_ARG, "disk_init: null argument");
   if (c->n == 0 || c->n_mu == 0) return fail(c, SOAP_ERR_STATE, "disk_init: set wave and mu bounds first");
@@ -570,6 +603,7 @@ int soap_disk_init(soap_ctx *c, const soap_star *st) {
   int rc = alloc_maps(c, st->grid);
   if (rc) return rc;
   c->star = *st;
+  c->tap_wmax = limb_weight_bound(st->u1, st->u2, st->lb);
   DiskArgs a{};
   a.grid = st->grid;
   a.n_mu = c->n_mu;
@@ -651,6 +685,11 @@ int soap_set_maps(soap_ctx *c, const soap_star *st, const double *delta, const d
     m[i] = (int32_t)mc;
     t[i] = expm1((mc - sh) * c->h) / c->expm1_h;
   }
+  double wmax = 0;
+  for (size_t i = 0; i < g2; i++)
+    if (!(limb[i] == -999.0 && delta[i] == -999.0))
+      wmax = std::max(wmax, std::max(std::fabs(limb[i]), std::fabs(limb[i] * (1 + lb[i]))));
+  c->tap_wmax = wmax;
   HIPCHK(c, hipMemcpyAsync(c->d_delta, delta, g2 * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_limb, limb, g2 * 8, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(c->d_region, region, g2 * 4, hipMemcpyHostToDevice, c->stream));
@@ -714,6 +753,7 @@ int soap_quiet_star(soap_ctx *c) {
     c->have_lower = c->have_ccf = c->have_fit = false;
     return SOAP_OK;
   }
+  if ((rc = check_tap_capacity(c, c->tap_wmax, "quiet_star"))) return rc;
   {
     size_t per = (size_t)c->n_mu * c->nd;
     if ((rc = ensure(c, c->qacc, per * 2 * 8)) || (rc = ensure(c, c->qtaps, per * 8)) ||
@@ -832,6 +872,18 @@ static int solar_nodes(soap_ctx *c, int t_sum) {
       }
     }
     if (err <= 2e-14) {
+      double leb = 0;  // Lebesgue function max: bounds the Lagrange weights of a cell
+      for (int T = t_sum - 216; T <= t_sum; T++) {
+        double sum = 0;
+        for (int q = 0; q < K; q++) {
+          double l = w[q];
+          for (int r = 0; r < K; r++)
+            if (r != q) l *= T - x[r];
+          sum += std::fabs(l);
+        }
+        leb = std::max(leb, sum);
+      }
+      c->solar_leb = leb;
       c->solar_k = K;
       c->t_norm_fac = t_norm;
       std::copy(x, x + K, c->solar_t);
@@ -986,6 +1038,8 @@ int soap_integrate(soap_ctx *c) {
   const bool solar = c->act.solar_mode != 0;
   const int nsets = solar ? 2 + c->solar_k : 3;
   if (solar && c->solar_k == 0) return fail(c, SOAP_ERR_STATE, "integrate: solar nodes missing");
+  if ((rc = check_tap_capacity(c, c->tap_wmax * (solar ? std::max(1.0, c->solar_leb) : 1.0), "integrate")))
+    return rc;
   if ((rc = ensure_taps(c, c->P, nsets, nsets + 1))) return rc;
   TapAccArgs a{};
   a.grid = c->G;
@@ -1052,8 +1106,10 @@ int soap_integrate(soap_ctx *c) {
       HIPCHK(c, e);
     } else {
       int fg = c->fir_g, fb = c->fir_b;
-      if (fb < 0) fg = 4, fb = 0;
-      hipError_t e = fb == 0 ? launch_fir_active_grp(f, c->P, c->stream) : launch_fir2(f, c->P, fg, fb, c->stream);
+      if (fb == -1) fg = 4, fb = 0;
+      hipError_t e = fb == -2  ? launch_fir_direct(f, c->P, fg, c->stream)
+                     : fb == 0 ? launch_fir_active_grp(f, c->P, c->stream)
+                               : launch_fir2(f, c->P, fg, fb, c->stream);
       if (e == hipErrorInvalidValue) e = launch_fir_active(f, c->P, c->stream);  // very wide shift ranges
       HIPCHK(c, e);
     }
@@ -1066,7 +1122,7 @@ int soap_integrate(soap_ctx *c) {
 
 int soap_set_fir_shape(soap_ctx *c, int32_t phases_per_block, int32_t pixels_per_thread) {
   if (!c) return SOAP_ERR_ARG;
-  const int ok[][2] = {{-1, -1}, {4, 0}, {1, 3}, {1, 5}, {2, 3}, {2, 1}, {4, 1}};
+  const int ok[][2] = {{-1, -1}, {4, 0}, {1, 3}, {1, 5}, {2, 3}, {2, 1}, {4, 1}, {1, -2}, {2, -2}, {4, -2}};
   for (auto &s : ok)
     if (s[0] == phases_per_block && s[1] == pixels_per_thread) {
       c->fir_g = phases_per_block;
@@ -1796,7 +1852,7 @@ int soap_phoenix_interp(soap_ctx *c, const soap_phoenix_params *p, const double 
 }
 
 int soap_fit_ccf_host_err(soap_ctx *c, const double *v, int32_t n_v, const double *y, const double *err,
-                          int32_t n_ccf, double *par, double *model, double *depth, double *bis) {
+                          int32_t n_ccf, double *par, double *model, double *depth, double *bis, int32_t *status) {
   if (!c || !v || !y || n_v < 3 || n_v > 1024 || n_ccf < 1 || !par) return fail(c, SOAP_ERR_ARG, "fit_ccf_host: bad arguments");
   if (err)
     for (size_t i = 0; i < (size_t)n_ccf * n_v; i++)
@@ -1804,35 +1860,39 @@ int soap_fit_ccf_host_err(soap_ctx *c, const double *v, int32_t n_v, const doubl
         return fail(c, SOAP_ERR_ARG, "fit_ccf_host: error %zu is %g (must be finite and non-zero)", i, err[i]);
   HIPCHK(c, hipSetDevice(c->device));
   double *dv = nullptr, *dy = nullptr, *de = nullptr, *dpar = nullptr, *dmod = nullptr, *ddep = nullptr, *dbis = nullptr;
+  int32_t *dst = nullptr;
   int rc = SOAP_OK;
   if ((rc = dalloc(c, &dv, n_v)) || (rc = dalloc(c, &dy, (size_t)n_ccf * n_v)) ||
       (err && (rc = dalloc(c, &de, (size_t)n_ccf * n_v))) || (rc = dalloc(c, &dpar, (size_t)n_ccf * 5)) ||
       (rc = dalloc(c, &dmod, (size_t)n_ccf * n_v)) || (rc = dalloc(c, &ddep, (size_t)n_ccf * 91)) ||
-      (rc = dalloc(c, &dbis, (size_t)n_ccf * 91))) {
+      (rc = dalloc(c, &dbis, (size_t)n_ccf * 91)) || (status && (rc = dalloc(c, &dst, (size_t)n_ccf)))) {
     for (double *p : {dv, dy, de, dpar, dmod, ddep, dbis}) dfree_c(c, p);
+    dfree_c(c, dst);
     return rc;
   }
   hipError_t e = hipMemcpyAsync(dv, v, n_v * 8, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(dy, y, (size_t)n_ccf * n_v * 8, hipMemcpyHostToDevice, c->stream);
   if (e == hipSuccess && err) e = hipMemcpyAsync(de, err, (size_t)n_ccf * n_v * 8, hipMemcpyHostToDevice, c->stream);
-  if (e == hipSuccess) e = launch_fit(dv, n_v, dy, de, n_ccf, dpar, dmod, ddep, dbis, nullptr, c->stream);
+  if (e == hipSuccess) e = launch_fit(dv, n_v, dy, de, n_ccf, dpar, dmod, ddep, dbis, nullptr, c->stream, dst);
   if (e == hipSuccess) e = hipMemcpyAsync(par, dpar, (size_t)n_ccf * 5 * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && status) e = hipMemcpyAsync(status, dst, (size_t)n_ccf * 4, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess && model) e = hipMemcpyAsync(model, dmod, (size_t)n_ccf * n_v * 8, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess && depth) e = hipMemcpyAsync(depth, ddep, (size_t)n_ccf * 91 * 8, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess && bis) e = hipMemcpyAsync(bis, dbis, (size_t)n_ccf * 91 * 8, hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   for (double *p : {dv, dy, de, dpar, dmod, ddep, dbis}) dfree_c(c, p);
+  dfree_c(c, dst);
   if (e != hipSuccess) return fail(c, SOAP_ERR_HIP, "fit_ccf_host: %s", hipGetErrorString(e));
   return SOAP_OK;
 }
 
 int soap_fit_ccf_host(soap_ctx *c, const double *v, int32_t n_v, const double *y, int32_t n_ccf, double *par,
                       double *model, double *depth, double *bis) {
-  return soap_fit_ccf_host_err(c, v, n_v, y, nullptr, n_ccf, par, model, depth, bis);
+  return soap_fit_ccf_host_err(c, v, n_v, y, nullptr, n_ccf, par, model, depth, bis, nullptr);
 }
 
 // ------------------------------------------------------------------ a11: differentiable path
-int soap_diff_setup(soap_ctx *c, int32_t grid, int32_t n_phase, int32_t map_res, double sigma_ms) {
+int soap_diff_setup(soap_ctx *c, int32_t grid, int32_t n_phase, int32_t map_res, double sigma_ms, const double *lb) {
   if (!c || grid < 2 || grid > 8192 || n_phase < 1 || map_res < 2 || map_res > 8192 || !(sigma_ms > 0))
     return fail(c, SOAP_ERR_ARG, "diff_setup: bad arguments");
   if (c->kind != SOAP_GRID_LOG) return fail(c, SOAP_ERR_STATE, "diff_setup: the differentiable path needs a log grid");
@@ -1855,8 +1915,8 @@ int soap_diff_setup(soap_ctx *c, int32_t grid, int32_t n_phase, int32_t map_res,
       (rc = ensure(c, d.th, (size_t)n_phase * 4 * 8)) || (rc = ensure(c, d.dth, (size_t)n_phase * 4 * 8)) ||
       (rc = ensure(c, d.gy, (size_t)n_phase * c->n_v * 8)) ||
       (rc = ensure(c, d.bisd, (size_t)n_phase * (c->n_v + kBisHead) * 8)) ||
-      (rc = ensure(c, d.tg, ((size_t)n_phase * 3 + 5) * 8)) ||
-      (rc = ensure(c, d.cell_part, (size_t)diff_cell_blocks((int64_t)g2) * 4 * 8)) ||
+      (rc = ensure(c, d.tg, ((size_t)n_phase * 3 + 1 + kDiffScalars) * 8)) ||
+      (rc = ensure(c, d.cell_part, (size_t)diff_cell_blocks((int64_t)g2) * kDiffScalars * 8)) ||
       (rc = ensure(c, c->ccf_part, (size_t)ccf_chunks(c->n_lines) * n_phase * c->n_v * 8)))
     return rc;
   int wid = conv_window(c->wave, sigma_ms);
@@ -1866,9 +1926,20 @@ int soap_diff_setup(soap_ctx *c, int32_t grid, int32_t n_phase, int32_t map_res,
   d.P = n_phase;
   d.M = map_res;
   d.sigma = sigma_ms;
+  for (int q = 0; q < 4; q++) d.lb[q] = lb ? lb[q] : 0.0;
   d.ready = true;
   d.have_fwd = false;
   return SOAP_OK;
+}
+
+// max over mu in [0, 1] of |1 + LB(mu)| (sampled, 1 % margin)
+static double lb_factor_bound(const double *lb) {
+  double w = 0;
+  for (int i = 0; i <= 4096; i++) {
+    double mu = i / 4096.0;
+    w = std::max(w, std::fabs(1 + (lb[0] + lb[1] * mu + lb[2] * mu * mu + lb[3] * mu * mu * mu)));
+  }
+  return 1.01 * w;
 }
 
 static DiffArgs diff_args(soap_ctx *c) {
@@ -1887,6 +1958,9 @@ static DiffArgs diff_args(soap_ctx *c) {
   a.u1 = d.prm.u1;
   a.u2 = d.prm.u2;
   a.kappa = d.prm.contrast;
+  a.kappa_f = d.prm.contrast_fac;
+  a.ns = d.ns;
+  for (int q = 0; q < 4; q++) a.lb[q] = d.lb[q];
   a.mu_bounds = c->d_mub;
   a.ck = (int32_t *)d.ck.p;
   a.cm = (int32_t *)d.cm.p;
@@ -1895,11 +1969,13 @@ static DiffArgs diff_args(soap_ctx *c) {
   a.cmu = (double *)d.cmu.p;
   a.cdt = (double *)d.cdt.p;
   a.maps = d.maps;
+  a.fmaps = d.fmaps;
   a.acc = (unsigned long long *)d.acc.p;
   a.taps = (double *)d.taps.p;
   a.krange = (int32_t *)d.krange.p;
   a.q = d.q;
   a.s = d.s;
+  a.f = d.f;
   a.I = (double *)d.I.p;
   a.F = (double *)d.F.p;
   return a;
@@ -1916,15 +1992,26 @@ static ConvArgs diff_conv(soap_ctx *c) {
 }
 
 int soap_diff_forward(soap_ctx *c, const soap_diff_params *prm, const double *d_q, const double *d_s,
-                      const double *d_maps, const double *d_target, const double *rv_t, const double *fw_t,
-                      const double *bis_t, double w_spec, double w_rv, double w_fw, double w_bis, double *loss,
-                      double *rv, double *fw, double *bis) {
+                      const double *d_f, const double *d_maps, const double *d_fmaps, const double *d_target,
+                      const double *rv_t, const double *fw_t, const double *bis_t, double w_spec, double w_rv,
+                      double w_fw, double w_bis, double *loss, double *rv, double *fw, double *bis) {
   if (!c || !prm || !d_q || !d_s || !d_maps || !d_target || !rv_t || !fw_t || !bis_t || !loss)
     return fail(c, SOAP_ERR_ARG, "diff_forward: null argument");
+  if (!d_f != !d_fmaps) return fail(c, SOAP_ERR_ARG, "diff_forward: facula seeds and facula maps go together");
   auto &d = c->diff;
   if (!d.ready) return fail(c, SOAP_ERR_STATE, "diff_forward: run diff_setup first");
   if (!(prm->vsini_kms > 0 && prm->vsini_kms < 2000))
     return fail(c, SOAP_ERR_ARG, "diff_forward: vsini %g km/s out of range", prm->vsini_kms);
+  {
+    // cell weights L (1 - phi - psi), contrast L phi, contrast_fac L (1 + LB) psi, maps in [0, 1]
+    const double zero4[4] = {0, 0, 0, 0};
+    double lmax = limb_weight_bound(prm->u1, prm->u2, zero4);
+    double w = lmax * std::max(3.0, std::max(std::fabs(prm->contrast),
+                                             d_f ? std::fabs(prm->contrast_fac) * lb_factor_bound(d.lb) : 0.0));
+    if (!fx_capacity_ok((double)d.G * d.G, w))
+      return fail(c, SOAP_ERR_LIMIT, "diff_forward: %d^2 cells with weights up to %g exceed the fixed-point tap "
+                  "capacity (cells x weight < 2^30)", d.G, w);
+  }
   HIPCHK(c, hipSetDevice(c->device));
   // taps span the shifts of delta in [-vsini, vsini] (m monotone in delta), plus margins
   auto mshift = [&](double v) {
@@ -1936,18 +2023,22 @@ int soap_diff_forward(soap_ctx *c, const soap_diff_params *prm, const double *d_
   d.dmin = mlo - 2;
   d.nd = mhi - d.dmin + 3;
   if (d.nd > 2048) return fail(c, SOAP_ERR_LIMIT, "diff_forward: shift range of %d pixels", d.nd);
+  const int ns = d_f ? 3 : 2;
   int rc;
-  if ((rc = ensure(c, d.acc, (size_t)d.P * c->n_mu * d.nd * 4 * 8)) ||
-      (rc = ensure(c, d.taps, (size_t)d.P * c->n_mu * d.nd * 2 * 8)) ||
+  if ((rc = ensure(c, d.acc, (size_t)d.P * c->n_mu * d.nd * ns * 2 * 8)) ||
+      (rc = ensure(c, d.taps, (size_t)d.P * c->n_mu * d.nd * ns * 8)) ||
       (rc = ensure(c, d.krange, (size_t)d.P * c->n_mu * 2 * 4)) ||
-      (rc = ensure(c, d.C, (size_t)d.P * c->n_mu * d.nd * 2 * 8)) ||
-      (rc = ensure(c, d.Cpart, (size_t)d.P * c->n_mu * diff_corr_chunks(c->n) * d.nd * 2 * 8)))
+      (rc = ensure(c, d.C, (size_t)d.P * c->n_mu * d.nd * ns * 8)) ||
+      (rc = ensure(c, d.Cpart, (size_t)d.P * c->n_mu * diff_corr_chunks(c->n) * d.nd * ns * 8)))
     return rc;
   if (!c->mask_built && (rc = build_mask(c))) return rc;
   d.prm = *prm;
+  d.ns = ns;
   d.q = d_q;
   d.s = d_s;
+  d.f = d_f;
   d.maps = d_maps;
+  d.fmaps = d_fmaps;
   d.target = d_target;
   d.w_spec = w_spec;
   double *tg = (double *)d.tg.p;  // [rv_t | fw_t | bis_t | loss]
@@ -1980,7 +2071,7 @@ int soap_diff_forward(soap_ctx *c, const soap_diff_params *prm, const double *d_
     // the loss and the fit-parameter derivatives for the backward stay on the device
     HIPCHK(c, launch_diff_loss(d.P, d.res_blocks, (const double *)d.res_part.p, (const double *)d.th.p, tg, tg + d.P,
                                tg + 2 * d.P, w_spec, w_rv, w_fw, w_bis, (double *)d.bisd.p, c->n_v, nullptr, 0.0,
-                               tg + 3 * d.P, (double *)d.dth.p, c->stream));
+                               0.0, 0.0, tg + 3 * d.P, (double *)d.dth.p, c->stream));
   }
   std::vector<double> th((size_t)d.P * 4), bd((size_t)d.P * (c->n_v + kBisHead));
   HIPCHK(c, hipMemcpyAsync(loss, tg + 3 * d.P, 8, hipMemcpyDeviceToHost, c->stream));
@@ -1996,10 +2087,13 @@ int soap_diff_forward(soap_ctx *c, const soap_diff_params *prm, const double *d_
   return SOAP_OK;
 }
 
-int soap_diff_backward(soap_ctx *c, double *grad4, double *d_gq, double *d_gs, double *d_gmaps) {
-  if (!c || !grad4 || !d_gq || !d_gs || !d_gmaps) return fail(c, SOAP_ERR_ARG, "diff_backward: null argument");
+int soap_diff_backward(soap_ctx *c, double *grad5, double *d_gq, double *d_gs, double *d_gf, double *d_gmaps,
+                       double *d_gfmaps) {
+  if (!c || !grad5 || !d_gq || !d_gs || !d_gmaps) return fail(c, SOAP_ERR_ARG, "diff_backward: null argument");
   auto &d = c->diff;
   if (!d.have_fwd) return fail(c, SOAP_ERR_STATE, "diff_backward: run diff_forward first");
+  if (d.ns == 3 && (!d_gf || !d_gfmaps))
+    return fail(c, SOAP_ERR_ARG, "diff_backward: the forward had faculae: facula gradients need buffers");
   HIPCHK(c, hipSetDevice(c->device));
   int rc0 = diff_mask_transpose(c);
   if (rc0) return rc0;
@@ -2011,16 +2105,16 @@ int soap_diff_backward(soap_ctx *c, double *grad4, double *d_gq, double *d_gs, d
         a, ca, d.target, c->trim, d.w_spec, c->d_vgrid, c->n_v, (const double *)d.ccf.p, (const double *)d.th.p,
         (const double *)d.dth.p, (double *)d.gy.p, c->n - 2 * (int64_t)c->trim, (const int64_t *)d.mt_ptr.p,
         (const int32_t *)d.mt_w.p, (const int32_t *)d.mt_col.p, (const double *)d.mt_val.p, (double *)d.GF.p,
-        (double *)d.GI.p, (double *)d.Cpart.p, (double *)d.C.p, d_gmaps, (double *)d.cell_part.p, d_gq, d_gs,
-        (const double *)d.bisd.p, c->stream);
+        (double *)d.GI.p, (double *)d.Cpart.p, (double *)d.C.p, d_gmaps, d.ns == 3 ? d_gfmaps : nullptr,
+        (double *)d.cell_part.p, d_gq, d_gs, d.ns == 3 ? d_gf : nullptr, (const double *)d.bisd.p, c->stream);
     if (e == hipErrorInvalidValue) return fail(c, SOAP_ERR_LIMIT, "diff_backward: window or shift range too wide");
     HIPCHK(c, e);
   }
   // the scalar gradients: the same device reduction as the graph path; the sums
   // land in the tail of the loss slot buffer
-  double *g4 = (double *)d.tg.p + 3 * d.P + 1;
-  HIPCHK(c, launch_diff_grad4(diff_cell_blocks((int64_t)d.G * d.G), (const double *)d.cell_part.p, g4, c->stream));
-  HIPCHK(c, hipMemcpyAsync(grad4, g4, 4 * 8, hipMemcpyDeviceToHost, c->stream));
+  double *g5 = (double *)d.tg.p + 3 * d.P + 1;
+  HIPCHK(c, launch_diff_grad(diff_cell_blocks((int64_t)d.G * d.G), (const double *)d.cell_part.p, g5, c->stream));
+  HIPCHK(c, hipMemcpyAsync(grad5, g5, kDiffScalars * 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return SOAP_OK;
 }
@@ -2587,10 +2681,11 @@ struct soap_diffbatch {
     soap_ctx *c = nullptr;
     bool bound = false;
     double vsini_max = 0;
-    const double *prm = nullptr, *q = nullptr, *s = nullptr, *maps = nullptr, *target = nullptr, *rv_t = nullptr,
-                 *fw_t = nullptr, *bis_t = nullptr;
+    const double *prm = nullptr, *q = nullptr, *s = nullptr, *f = nullptr, *maps = nullptr, *fmaps = nullptr,
+                 *target = nullptr, *rv_t = nullptr, *fw_t = nullptr, *bis_t = nullptr;
     double w_spec = 0, w_rv = 0, w_fw = 0, w_bis = 0;
-    double *loss = nullptr, *g4 = nullptr, *gq = nullptr, *gs = nullptr, *gmaps = nullptr;
+    double *loss = nullptr, *g5 = nullptr, *gq = nullptr, *gs = nullptr, *gf = nullptr, *gmaps = nullptr,
+           *gfmaps = nullptr;
     int dmin = 0, nd = 0;
     std::vector<uintptr_t> sig;  // buffers and sizes the enqueued launches use
   };
@@ -2616,7 +2711,8 @@ std::vector<uintptr_t> dbatch_signature(const soap_ctx *c) {
           u(d.gy.p),   u(d.GF.p),    u(d.GI.p),     u(d.cell_part.p), u(d.I.p),  u(d.F.p),        u(d.ck.p),
           u(d.cm.p),   u(d.ct.p),    u(d.cL.p),     u(d.cmu.p),   u(d.cdt.p),    u(d.mt_ptr.p),   u(d.mt_w.p),
           u(d.mt_col.p), u(d.mt_val.p), u(c->ccf_part.p), u(c->m_start.p), u(c->m_alive.p), u(c->m_val.p),
-          u(c->d_w_sorted), u(c->ctap.p), u(d.bisd.p), u(d.maps), u(d.q), u(d.s), u(d.target), (uintptr_t)d.nd,
+          u(c->d_w_sorted), u(c->ctap.p), u(d.bisd.p), u(d.maps), u(d.q), u(d.s), u(d.target), u(d.f), u(d.fmaps),
+          (uintptr_t)d.ns, (uintptr_t)d.nd,
           (uintptr_t)d.dmin, (uintptr_t)d.P, (uintptr_t)d.G, (uintptr_t)d.M, (uintptr_t)c->n, (uintptr_t)c->res_ntp,
           (uintptr_t)c->n_lines, (uintptr_t)c->n_v};
 }
@@ -2715,17 +2811,19 @@ hipError_t dbatch_enqueue_one(soap_diffbatch::Item &it, hipStream_t st, bool adj
     double *bisd = it.w_bis != 0.0 ? (double *)d.bisd.p : nullptr;  // no span term: no span kernel
     if (e == hipSuccess)
       e = launch_diff_loss(d.P, rb, (const double *)d.res_part.p, (const double *)d.th.p, it.rv_t, it.fw_t, it.bis_t,
-                           it.w_spec, it.w_rv, it.w_fw, it.w_bis, bisd, c->n_v, it.prm, it.vsini_max, it.loss,
-                           (double *)d.dth.p, st);
+                           it.w_spec, it.w_rv, it.w_fw, it.w_bis, bisd, c->n_v, it.prm, it.vsini_max,
+                           (double)d.G * d.G, d.ns == 3 ? lb_factor_bound(d.lb) : 0.0, it.loss, (double *)d.dth.p,
+                           st);
     if (!adjoint) return e;
     if (e == hipSuccess)
       e = launch_diff_backward(a, ca, it.target, c->trim, it.w_spec, c->d_vgrid, c->n_v, (const double *)d.ccf.p,
                                (const double *)d.th.p, (const double *)d.dth.p, (double *)d.gy.p,
                                c->n - 2 * (int64_t)c->trim, (const int64_t *)d.mt_ptr.p, (const int32_t *)d.mt_w.p, (const int32_t *)d.mt_col.p,
                                (const double *)d.mt_val.p, (double *)d.GF.p, (double *)d.GI.p, (double *)d.Cpart.p,
-                               (double *)d.C.p, it.gmaps, (double *)d.cell_part.p, it.gq, it.gs, bisd, st);
-    if (e == hipSuccess) e = launch_diff_grad4(diff_cell_blocks((int64_t)d.G * d.G), (const double *)d.cell_part.p,
-                                               it.g4, st);
+                               (double *)d.C.p, it.gmaps, it.gfmaps, (double *)d.cell_part.p, it.gq, it.gs, it.gf,
+                               bisd, st);
+    if (e == hipSuccess) e = launch_diff_grad(diff_cell_blocks((int64_t)d.G * d.G), (const double *)d.cell_part.p,
+                                              it.g5, st);
     return e;
   }
 }
@@ -2782,15 +2880,18 @@ void soap_diff_batch_destroy(soap_diffbatch *b) {
 const char *soap_diff_batch_last_error(const soap_diffbatch *b) { return b ? b->err.c_str() : "null batch"; }
 
 int soap_diff_batch_bind(soap_diffbatch *b, int32_t i, double vsini_max, const double *d_params, const double *d_q,
-                         const double *d_s, const double *d_maps, const double *d_target, const double *d_rv_t,
-                         const double *d_fw_t, const double *d_bis_t, double w_spec, double w_rv, double w_fw,
-                         double w_bis, double *d_loss, double *d_grad4, double *d_gq, double *d_gs,
-                         double *d_gmaps) {
+                         const double *d_s, const double *d_f, const double *d_maps, const double *d_fmaps,
+                         const double *d_target, const double *d_rv_t, const double *d_fw_t, const double *d_bis_t,
+                         double w_spec, double w_rv, double w_fw, double w_bis, double *d_loss, double *d_grad5,
+                         double *d_gq, double *d_gs, double *d_gf, double *d_gmaps, double *d_gfmaps) {
   if (!b || i < 0 || i >= (int32_t)b->items.size()) return SOAP_ERR_ARG;
-  if (!d_params || !d_q || !d_s || !d_maps || !d_target || !d_rv_t || !d_fw_t || !d_bis_t || !d_loss || !d_grad4 ||
-      !d_gq ||
-      !d_gs || !d_gmaps)
+  if (!d_params || !d_q || !d_s || !d_maps || !d_target || !d_rv_t || !d_fw_t || !d_bis_t || !d_loss || !d_grad5 ||
+      !d_gq || !d_gs || !d_gmaps)
     return dfail(b, SOAP_ERR_ARG, "diff_batch_bind: null pointer");
+  const bool fac = d_f != nullptr;
+  if (fac != (d_fmaps != nullptr) || fac != (d_gf != nullptr) || fac != (d_gfmaps != nullptr))
+    return dfail(b, SOAP_ERR_ARG, "diff_batch_bind: facula seeds, maps and their gradients go together");
+  const int ns = fac ? 3 : 2;
   if (!(vsini_max > 0 && vsini_max < 2000)) return dfail(b, SOAP_ERR_ARG, "diff_batch_bind: vsini_max out of range");
   auto &it = b->items[i];
   soap_ctx *c = it.c;
@@ -2806,18 +2907,21 @@ int soap_diff_batch_bind(soap_diffbatch *b, int32_t i, double vsini_max, const d
   int nd = mshift(vsini_max) - dmin + 3;
   if (nd > 2048) return dfail(b, SOAP_ERR_LIMIT, "diff_batch_bind: shift range of %d pixels", nd);
   int rc;
-  if ((rc = ensure(c, d.acc, (size_t)d.P * c->n_mu * nd * 4 * 8)) ||
-      (rc = ensure(c, d.taps, (size_t)d.P * c->n_mu * nd * 2 * 8)) ||
+  if ((rc = ensure(c, d.acc, (size_t)d.P * c->n_mu * nd * ns * 2 * 8)) ||
+      (rc = ensure(c, d.taps, (size_t)d.P * c->n_mu * nd * ns * 8)) ||
       (rc = ensure(c, d.krange, (size_t)d.P * c->n_mu * 2 * 4)) ||
-      (rc = ensure(c, d.C, (size_t)d.P * c->n_mu * nd * 2 * 8)) ||
-      (rc = ensure(c, d.Cpart, (size_t)d.P * c->n_mu * diff_corr_chunks(c->n) * nd * 2 * 8)) ||
+      (rc = ensure(c, d.C, (size_t)d.P * c->n_mu * nd * ns * 8)) ||
+      (rc = ensure(c, d.Cpart, (size_t)d.P * c->n_mu * diff_corr_chunks(c->n) * nd * ns * 8)) ||
       (!c->mask_built && (rc = build_mask(c))) || (rc = diff_mask_transpose(c)))
     return dfail(b, rc, "diff_batch_bind: instance %d: %s", i, c->err.c_str());
   d.dmin = dmin;
   d.nd = nd;
+  d.ns = ns;
   d.q = d_q;
   d.s = d_s;
+  d.f = d_f;
   d.maps = d_maps;
+  d.fmaps = d_fmaps;
   d.target = d_target;
   d.have_fwd = false;  // the per-call forward state no longer matches the buffers
   it.vsini_max = vsini_max;
@@ -2826,7 +2930,9 @@ int soap_diff_batch_bind(soap_diffbatch *b, int32_t i, double vsini_max, const d
   it.prm = d_params;
   it.q = d_q;
   it.s = d_s;
+  it.f = d_f;
   it.maps = d_maps;
+  it.fmaps = d_fmaps;
   it.target = d_target;
   it.rv_t = d_rv_t;
   it.fw_t = d_fw_t;
@@ -2836,10 +2942,12 @@ int soap_diff_batch_bind(soap_diffbatch *b, int32_t i, double vsini_max, const d
   it.w_rv = w_rv;
   it.w_fw = w_fw;
   it.loss = d_loss;
-  it.g4 = d_grad4;
+  it.g5 = d_grad5;
   it.gq = d_gq;
   it.gs = d_gs;
+  it.gf = d_gf;
   it.gmaps = d_gmaps;
+  it.gfmaps = d_gfmaps;
   it.bound = true;
   it.sig = dbatch_signature(c);
   dbatch_drop_graph(b);

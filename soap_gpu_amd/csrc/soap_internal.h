@@ -90,15 +90,19 @@ struct ConvArgs {
 
 // ------------------------------------------------------------ device helpers
 // ------------------------------------------------------------ fixed point taps
-// A contribution x is split into hi = floor(x 2^32) and lo = (x 2^32 - hi) 2^40
+// A contribution x is split into hi = floor(x 2^32) and lo = (x 2^32 - hi) 2^32
 // (exact: power-of-two scalings and the subtraction of an integer part do not
 // round); sums of 64-bit integers are associative, so atomics give the exact,
-// order-independent sum of all contributions truncated at 2^-72.
+// order-independent sum of all contributions truncated at 2^-64 each.
+// Capacity: lo < 2^32 per contribution, so the lo word holds 2^32 contributions
+// to one tap before it could wrap (a disk of 8192^2 cells has 5.3e7); the hi word
+// holds |sum| < 2^63 / 2^32 = 2^31, and the host keeps the summed weights of one
+// tap below 2^30 (fx_capacity_ok).
 __device__ __forceinline__ void fx_split(double x, unsigned long long &hi, unsigned long long &lo) {
   double y = x * 4294967296.0;
   double fl = floor(y);
   hi = (unsigned long long)(long long)fl;
-  lo = (unsigned long long)((y - fl) * 1099511627776.0);
+  lo = (unsigned long long)((y - fl) * 4294967296.0);
 }
 
 // P: a pointer to unsigned long long in any address space (LDS, global, generic)
@@ -111,10 +115,14 @@ __device__ __forceinline__ void fx_add(P p, double x) {
 }
 
 __device__ __forceinline__ double fx_value(unsigned long long hi, unsigned long long lo) {
-  long long h = (long long)hi + (long long)(lo >> 40);
-  unsigned long long r = lo & ((1ull << 40) - 1);
-  return (double)h * 2.3283064365386963e-10 + (double)r * 2.117582368135751e-22;  // 2^-32, 2^-72
+  long long h = (long long)hi + (long long)(lo >> 32);
+  unsigned long long r = lo & ((1ull << 32) - 1);
+  return (double)h * 2.3283064365386963e-10 + (double)r * 5.421010862427522e-20;  // 2^-32, 2^-64
 }
+
+// Host bound for the fixed-point tap sums: every tap receives at most n_contrib
+// contributions of magnitude <= w_max; the hi word's total stays below 2^62.
+inline bool fx_capacity_ok(double n_contrib, double w_max) { return n_contrib * w_max < 1073741824.0; }
 
 // Cell classification of one phase: loop over the phase's visible regions in
 // order, later ones overwriting earlier ones (SOAP_integration.cu:724-785,
@@ -163,18 +171,21 @@ struct LinSolar {
 struct DiffArgs {
   int64_t n;
   int32_t grid, n_mu, n_phase, map_res, dmin, nd;
-  double h, expm1_h, vsini, u1, u2, kappa;
-  const double *prm;             // device (vsini, u1, u2, kappa) if not null (graph replays), else the fields above
+  int32_t ns;                    // tap sets: 2 (quiet, spot) or 3 (quiet, spot, facula)
+  double h, expm1_h, vsini, u1, u2, kappa, kappa_f;
+  double lb[4];                  // facula intensity 1 + LB(mu), LB = lb0 + lb1 mu + lb2 mu^2 + lb3 mu^3
+  const double *prm;             // device (vsini, u1, u2, kappa, kappa_f) if not null (graph replays), else the fields above
   const double *mu_bounds;       // n_mu - 1
   int32_t *ck, *cm;              // per cell [G*G]: bin (-1 off disk), shift m
   double *ct, *cL, *cmu, *cdt;   // weight t, limb, mu, dt/d(vsini)
-  const double *maps;            // [P][M][M]
-  unsigned long long *acc;       // [P][n_mu][nd][2 sets][hi, lo]
-  double *taps;                  // [P][n_mu][nd][2]
+  const double *maps, *fmaps;    // spot / facula filling factors [P][M][M] (fmaps null when ns == 2)
+  unsigned long long *acc;       // [P][n_mu][nd][ns][hi, lo]
+  double *taps;                  // [P][n_mu][nd][ns]
   int32_t *krange;               // [P][n_mu][2]
-  const double *q, *s;           // seed rows [n_mu][n]
+  const double *q, *s, *f;       // quiet / spot / facula seed rows [n_mu][n] (f null when ns == 2)
   double *I, *F;                 // [P][n]
 };
+constexpr int kDiffScalars = 5;  // gradients of (vsini, u1, u2, kappa, kappa_f)
 
 struct ConvLinArgs {
   int64_t n;
@@ -259,6 +270,8 @@ hipError_t launch_fir_active_grp(const FirArgs &a, int n_job, hipStream_t s);
 // register-blocked form: G phases per block, B pixels per thread (G, B in
 // {(1,3), (1,5), (2,3), (2,1), (4,1)})
 hipError_t launch_fir2(const FirArgs &a, int n_job, int G, int B, hipStream_t s);
+// direct form (no LDS staging): G phases per block, G in {1, 2, 4}
+hipError_t launch_fir_direct(const FirArgs &a, int n_job, int G, hipStream_t s);
 size_t fir2_lds(int nd, int G, int B);
 int fir2_tiles(int64_t n, int B);
 // solar mode: taps [job][n_mu][nd][K+3] = H1s, H1f, L_0..L_{K-1}, H1a
@@ -317,9 +330,11 @@ int ccf_chunks(int n_lines);
 // part: ccf_chunks(n_lines) * n_spec * n_v doubles; w_sorted[q] = weight of line sorted[q]
 hipError_t launch_ccf_reduce(const double *part, int nc, int n_spec, int n_v, const double *appod, const double *mm,
                              const double *wsum, double *ccf, hipStream_t s);
-// err: per-point errors [n_ccf][n_v] or NULL (unit errors)
+// err: per-point errors [n_ccf][n_v] or NULL (unit errors); status (optional): per
+// CCF 0 (converged), 27 (no progress), 11 (iteration limit)
 hipError_t launch_fit(const double *v, int n_v, const double *y, const double *err, int n_ccf, double *par,
-                      double *model, double *depth, double *bis, int32_t *iters, hipStream_t s);
+                      double *model, double *depth, double *bis, int32_t *iters, hipStream_t s,
+                      int32_t *status = nullptr);
 // linear grids
 hipError_t launch_disk_cells_by_bin(int ncell, int n_mu, const double *limb, const double *delta,
                                     const int32_t *region, int count_only, int32_t *counts, const int32_t *offsets,
@@ -353,28 +368,29 @@ hipError_t launch_diff_backward(const DiffArgs &a, const ConvArgs &c, const doub
                                 const double *v, int n_v, const double *ccf, const double *theta, const double *dth,
                                 double *gy, int64_t n_t, const int64_t *mt_ptr, const int32_t *mt_w,
                                 const int32_t *mt_col, const double *mt_val, double *GF, double *GI, double *Cpart,
-                                double *C, double *gmaps, double *cell_part, double *gq, double *gs,
-                                const double *bisd, hipStream_t s);
+                                double *C, double *gmaps, double *gfmaps, double *cell_part, double *gq, double *gs,
+                                double *gf, const double *bisd, hipStream_t s);
 // bisector span of each CCF at the polished Gaussian and its partials (k_diff_bis):
 // bisd [n_ccf][n_v + kBisHead] = (span, loss coefficient, dspan/dtheta[4], dspan/dy[n_v])
 constexpr int kBisHead = 6;
 constexpr int kMaxVDiff = 1024;
 hipError_t launch_diff_bis(const double *v, int n_v, const double *ccf, const double *theta, int n_ccf, double *bisd,
                            hipStream_t s);
-// chunks of the lag-correlation pass (Cpart holds [P][n_mu][chunks][nd][2])
+// chunks of the lag-correlation pass (Cpart holds [P][n_mu][chunks][nd][ns])
 int diff_corr_chunks(int64_t n);
 hipError_t launch_ccf_sum(const double *part, int nc, int n_spec, int n_v, double *ccf, hipStream_t s);
 // device-side loss of the differentiable model: per phase the residual partials
 // (res_blocks per phase, in order), the RV / FWHM terms and, when w_bis != 0, the
 // bisector-span term from bisd; writes loss[0], the fit-parameter derivatives
 // dth [P][4] and the span coefficients of bisd; loss = NaN when prm (optional) has
-// vsini outside (0, vsini_max] (the fixed tap range of a graph)
+// vsini outside (0, vsini_max] (the fixed tap range of a graph) or weights beyond
+// the fixed-point tap capacity of n_cells cells (lbmax = max |1 + LB(mu)|)
 hipError_t launch_diff_loss(int n_phase, int res_blocks, const double *res_part, const double *theta,
                             const double *rv_t, const double *fw_t, const double *bis_t, double w_spec, double w_rv,
                             double w_fw, double w_bis, double *bisd, int n_v, const double *prm, double vsini_max,
-                            double *loss, double *dth, hipStream_t s);
-// g4[q] = sum over blocks (in order) of cell_part[b][q], q < 4
-hipError_t launch_diff_grad4(int n_blocks, const double *cell_part, double *g4, hipStream_t s);
+                            double n_cells, double lbmax, double *loss, double *dth, hipStream_t s);
+// g[q] = sum over blocks (in order) of cell_part[b][q], q < kDiffScalars
+hipError_t launch_diff_grad(int n_blocks, const double *cell_part, double *g, hipStream_t s);
 // batched instances: stage st over jobs[n_inst] (device), prefix[n_inst + 1]
 // (device, prefix[n_inst] = total); grid = persistent block count (0: one block
 // per work item), threads / lds as the stage needs; FIR shape: fir_g phases per

@@ -659,6 +659,153 @@ hipError_t launch_fir_active_grp(const FirArgs &a, int n_job, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Direct form of the phase-group FIR: no LDS staging and no barrier after the
+// group metadata.  Each thread reads the seed-row values of its pixel straight
+// from global memory (a wave's loads at one offset are 64 consecutive doubles;
+// the rows of a tile are shared by all its phase groups, whose blocks run back
+// to back on one XCD, so they come from L2 / the vector L1), and the inner loop
+// is specialised by the row kinds a bin uses for the group (spot rows, facula
+// rows, both), so absent rows cost neither loads nor FMAs.  Blocks whose source
+// range is inside [0, n) skip the per-load bounds selects.
+template <int G, bool SP, bool FA, bool CHECK>
+__device__ __forceinline__ void fir_direct_bin(const GLOBAL double *rq, const GLOBAL double *rs,
+                                               const GLOBAL double *rp, const GLOBAL double *rf,
+                                               const GLOBAL double *Tg, long dstride, long i0, long n, int n_off,
+                                               double (&F)[G][7]) {
+  // unrolled so that the loads of several offsets are in flight together; the
+  // unroll is bounded by the scalar registers holding the group's taps
+  constexpr int U = G >= 2 ? 2 : 4;
+#pragma unroll U
+  for (int i = 0; i < n_off; i++) {
+    const long x = i0 - i;
+    bool in = true;
+    long xc = x;
+    if (CHECK) {
+      in = x >= 0 && x < n;
+      xc = in ? x : 0;
+    }
+    double vq = rq[xc], vs = rs[xc];
+    double vp = SP ? rp[xc] : 0.0, vf = FA ? rf[xc] : 0.0;
+    if (CHECK) {
+      vq = in ? vq : 0.0;
+      vs = in ? vs : 0.0;
+      if (SP) vp = in ? vp : 0.0;
+      if (FA) vf = in ? vf : 0.0;
+    }
+    const GLOBAL double *T = Tg + (long)i * dstride;
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      // taps per phase: 0 = H1s, 1 = H1f, 2 = H2f, 3 = H1a (zero where a phase has no cells)
+      const double t0 = T[g * 4 + 0], t1 = T[g * 4 + 1], t2 = T[g * 4 + 2], t3 = T[g * 4 + 3];
+      F[g][0] = fma(t3, vs, F[g][0]);
+      if (SP) {
+        F[g][1] = fma(t0, vq, F[g][1]);
+        F[g][4] = fma(t0, vp, F[g][4]);
+      }
+      if (FA) {
+        F[g][2] = fma(t2, vq, F[g][2]);
+        F[g][3] = fma(t1, vq, F[g][3]);
+        F[g][5] = fma(t1, vf, F[g][5]);
+        F[g][6] = fma(t2, vf, F[g][6]);
+      }
+    }
+  }
+}
+
+template <int G>
+__global__ __launch_bounds__(kTPB) void k_fir_direct(FirArgs a, int n_job, int n_tiles, int n_groups) {
+  __shared__ int4 s_meta[64];
+  __shared__ int s_nb;
+  const int bx = blockIdx.x;
+  const int xcd = bx & 7, slot = bx >> 3;
+  const int tile = (slot / n_groups) * 8 + xcd;
+  const int grp = slot % n_groups;
+  if (tile >= n_tiles) return;  // uniform over the block
+  const int p0 = grp * G;
+  const int ng = min(G, n_job - p0);
+  const int nd = a.nd, n_mu = a.n_mu;
+  const GLOBAL int32_t *kflags = gl(a.kflags), *krange = gl(a.krange);
+  const long j0 = (long)tile * kFirTile;
+  const long j = j0 + threadIdx.x;
+  const double cs = (j < a.n && a.sps) ? gl(a.sps)[j] : 0.0;
+  const double cf = (j < a.n && a.fas) ? gl(a.fas)[j] : 0.0;
+  if (threadIdx.x < 64) {  // group metadata: union flags and tap range per bin, bins in use in order
+    const int k = threadIdx.x;
+    int ufl = 0, ulo = nd, uhi = -1;
+#pragma unroll
+    for (int g = 0; g < G; g++)
+      if (g < ng && k < n_mu) {
+        const long jk = (long)(p0 + g) * n_mu + k;
+        const int fl = kflags[jk], lo = krange[jk * 2], hi = krange[jk * 2 + 1];
+        if (fl && lo <= hi) {
+          ufl |= fl | 4;
+          ulo = min(ulo, lo);
+          uhi = max(uhi, hi);
+        }
+      }
+    const unsigned long long used = __ballot(ufl != 0);
+    if (ufl) s_meta[__popcll(used & ((1ull << k) - 1ull))] = make_int4(k, ufl, ulo, uhi);
+    if (k == 0) s_nb = __popcll(used);
+  }
+  __syncthreads();
+  double F[G][7];
+#pragma unroll
+  for (int g = 0; g < G; g++)
+#pragma unroll
+    for (int f = 0; f < 7; f++) F[g][f] = 0.0;
+  const int nb = uni(s_nb);
+  const long dstride = (long)a.p_pad * 4;
+  for (int kk = 0; kk < nb; kk++) {
+    const int4 m = s_meta[kk];
+    const int k = uni(m.x), ufl = uni(m.y), ulo = uni(m.z), uhi = uni(m.w);
+    const long roff = (long)k * a.n;
+    const GLOBAL double *rq = gl(a.q) + roff, *rs = gl(a.qs) + roff;
+    const GLOBAL double *rp = (ufl & 1) ? gl(a.sp) + roff : rq, *rf = (ufl & 2) ? gl(a.fa) + roff : rq;
+    const GLOBAL double *Tg = gl(a.taps_g) + (((long)k * nd + ulo) * a.p_pad + p0) * 4;
+    const int n_off = uhi - ulo + 1;
+    // source index of offset ulo + i: x = j - dmin - ulo - i
+    const long i0 = j - a.dmin - ulo;
+    const bool inside = (j0 - a.dmin - uhi >= 0) && (j0 + kFirTile - 1 - a.dmin - ulo < a.n);  // uniform
+    const int kind = ufl & 3;
+    if (inside) {
+      if (kind == 3) fir_direct_bin<G, true, true, false>(rq, rs, rp, rf, Tg, dstride, i0, a.n, n_off, F);
+      else if (kind == 1) fir_direct_bin<G, true, false, false>(rq, rs, rp, rf, Tg, dstride, i0, a.n, n_off, F);
+      else fir_direct_bin<G, false, true, false>(rq, rs, rp, rf, Tg, dstride, i0, a.n, n_off, F);
+    } else {
+      if (kind == 3) fir_direct_bin<G, true, true, true>(rq, rs, rp, rf, Tg, dstride, i0, a.n, n_off, F);
+      else if (kind == 1) fir_direct_bin<G, true, false, true>(rq, rs, rp, rf, Tg, dstride, i0, a.n, n_off, F);
+      else fir_direct_bin<G, false, true, true>(rq, rs, rp, rf, Tg, dstride, i0, a.n, n_off, F);
+    }
+  }
+  if (j < a.n) {
+#pragma unroll
+    for (int g = 0; g < G; g++) {
+      if (g >= ng) break;
+      GLOBAL double *o = gw(a.out) + (long)(p0 + g) * 3 * a.n;
+      o[j] = F[g][0] - cs * F[g][1] - cf * F[g][2];
+      o[a.n + j] = (F[g][1] + F[g][3]) - F[g][4] - F[g][5];
+      o[2 * a.n + j] = F[g][0] - cs * F[g][4] - cf * F[g][6];
+    }
+  }
+}
+
+hipError_t launch_fir_direct(const FirArgs &a, int n_job, int G, hipStream_t s) {
+  if (n_job == 0) return hipSuccess;
+  if (a.n_mu > 64) return hipErrorInvalidValue;
+  int n_tiles = (int)((a.n + kFirTile - 1) / kFirTile);
+  int n_groups = (n_job + G - 1) / G;
+  long tiles8 = ((long)n_tiles + 7) / 8 * 8;
+  long blocks = tiles8 * n_groups;
+  if (blocks >= (1L << 31)) return hipErrorInvalidValue;
+  switch (G) {
+    case 1: k_fir_direct<1><<<(unsigned)blocks, kTPB, 0, s>>>(a, n_job, n_tiles, n_groups); break;
+    case 2: k_fir_direct<2><<<(unsigned)blocks, kTPB, 0, s>>>(a, n_job, n_tiles, n_groups); break;
+    case 4: k_fir_direct<4><<<(unsigned)blocks, kTPB, 0, s>>>(a, n_job, n_tiles, n_groups); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 // Register-blocked form of the active FIR.  Per block: a lambda tile of
 // kTPB * B pixels and a group of G phases; each thread owns B consecutive
 // output pixels of every phase of the group.  For each mu bin used by the
@@ -1879,9 +2026,12 @@ __device__ int solve4(const double *A, const double *D, double mu, const double 
 // (More scaling, Nielsen damping, xtol = gtol = 1e-8), then the bisector.
 // E_: per-point errors [n_ccf][n] (CCF_error.txt), NULL = unit errors; held in
 // s_p0 during the fit (s_p0 is bisector scratch afterwards).
+// status_ (optional): 0 when the xtol or gtol test ended the fit, 27 when no step
+// reduced the residual (GSL_ENOPROG: a flat or degenerate CCF), 11 after 10000
+// iterations (GSL_EMAXITER).
 __device__ __forceinline__ void fit_body(const double *v_, int n, const double *Y_, const double *E_, double *par_,
                                          double *model_, double *depth_out_, double *bis_out_, int32_t *iters_,
-                                         int bx_) {
+                                         int bx_, int32_t *status_ = nullptr) {
   const GLOBAL double *v = gl(v_), *Y = gl(Y_);
   GLOBAL double *par = gw(par_), *model = gw(model_), *depth_out = gw(depth_out_), *bis_out = gw(bis_out_);
   GLOBAL int32_t *iters = gw(iters_);
@@ -1923,7 +2073,7 @@ __device__ __forceinline__ void fit_body(const double *v_, int n, const double *
   double D[4];
   for (int a = 0; a < 4; a++) D[a] = sqrt(ev.colsq[a]) > 0 ? sqrt(ev.colsq[a]) : 1.0;
   double mu = 1.0e-3, nu = 2.0;
-  int iter = 0;
+  int iter = 0, fit_status = 11;
   for (iter = 0; iter < 10000; iter++) {
     double dx[4], xt[4];
     int bad = 0;
@@ -1957,7 +2107,10 @@ __device__ __forceinline__ void fit_body(const double *v_, int n, const double *
         if (++bad > 15) break;
       }
     }
-    if (!accepted) break;
+    if (!accepted) {
+      fit_status = 27;
+      break;
+    }
     for (int a = 0; a < 4; a++) x[a] = xt[a];
     gauss_eval(s_px, s_y, s_err, n, x, true, ev);
     for (int a = 0; a < 4; a++)
@@ -1967,6 +2120,7 @@ __device__ __forceinline__ void fit_body(const double *v_, int n, const double *
       if (!(fabs(dx[a]) < xtol * xtol + xtol * fabs(x[a]))) small = false;
     if (small) {
       iter++;
+      fit_status = 0;
       break;
     }
     double gn = 0;
@@ -1977,6 +2131,7 @@ __device__ __forceinline__ void fit_body(const double *v_, int n, const double *
     double phi = 0.5 * ev.sum_f2;
     if (gn <= gtol * (phi > 1.0 ? phi : 1.0)) {
       iter++;
+      fit_status = 0;
       break;
     }
   }
@@ -2053,19 +2208,21 @@ __device__ __forceinline__ void fit_body(const double *v_, int n, const double *
     pp[3] = v0;
     pp[4] = fwhm;
     if (iters) iters[cidx] = iter;
+    if (status_) gw(status_)[cidx] = fit_status;
   }
 }
 
 __global__ __launch_bounds__(64) void k_fit(const double *v, int n, const double *Y, const double *E, double *par,
-                                            double *model, double *depth_out, double *bis_out, int32_t *iters) {
-  fit_body(v, n, Y, E, par, model, depth_out, bis_out, iters, blockIdx.x);
+                                            double *model, double *depth_out, double *bis_out, int32_t *iters,
+                                            int32_t *status) {
+  fit_body(v, n, Y, E, par, model, depth_out, bis_out, iters, blockIdx.x, status);
 }
 
 hipError_t launch_fit(const double *v, int n_v, const double *y, const double *err, int n_ccf, double *par,
-                      double *model, double *depth, double *bis, int32_t *iters, hipStream_t s) {
+                      double *model, double *depth, double *bis, int32_t *iters, hipStream_t s, int32_t *status) {
   if (n_ccf == 0) return hipSuccess;
   if (n_v > kMaxV || n_v < 3) return hipErrorInvalidValue;
-  k_fit<<<n_ccf, 64, 0, s>>>(v, n_v, y, err, par, model, depth, bis, iters);
+  k_fit<<<n_ccf, 64, 0, s>>>(v, n_v, y, err, par, model, depth, bis, iters, status);
   return hipGetLastError();
 }
 

@@ -22,9 +22,13 @@ int main(int argc, char **argv) {
   soap_ctx *ctx = nullptr;
   if (soap_ctx_create(0, &ctx) != SOAP_OK) die("no HIP device");
   std::vector<double> par(5), model(n), depth(91), bis(91);
+  int32_t status = 0;
   check(ctx, soap_fit_ccf_host_err(ctx, v.data(), n, y.data(), err.data(), 1, par.data(), model.data(), depth.data(),
-                                   bis.data()),
+                                   bis.data(), &status),
         "fit_ccf_host_err");
+  // the reference writes its outputs whatever the solver's outcome (BIS_FIT2.cpp:191); so does this tool
+  if (status == 27) std::cerr << "BIS_FIT2: the fit made no progress (flat or degenerate CCF)\n";
+  if (status == 11) std::cerr << "BIS_FIT2: the fit stopped at 10000 iterations\n";
   soap_ctx_destroy(ctx);
   write_bin("ccf_model.bin", model.data(), n);
   write_bin("ccf_parameter.bin", par.data(), 5);

@@ -71,7 +71,7 @@ class SoapError(RuntimeError):
 
 class DiffParams(ctypes.Structure):
     _fields_ = [("vsini_kms", ctypes.c_double), ("u1", ctypes.c_double), ("u2", ctypes.c_double),
-                ("contrast", ctypes.c_double)]
+                ("contrast", ctypes.c_double), ("contrast_fac", ctypes.c_double)]
 
 
 class BisInjectParams(ctypes.Structure):
@@ -126,7 +126,7 @@ def load(path: str = LIB_PATH):
         "soap_get_fit": (ctypes.c_int, [vp, _dp, _dp, _dp, _dp]),
         "soap_fit_ccf_host": (ctypes.c_int, [vp, _dp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, _dp, _dp, _dp]),
         "soap_fit_ccf_host_err": (ctypes.c_int, [vp, _dp, ctypes.c_int32, _dp, _dp, ctypes.c_int32, _dp, _dp, _dp,
-                                                 _dp]),
+                                                 _dp, _ip]),
         "soap_bis_inject": (ctypes.c_int, [vp, ctypes.POINTER(BisInjectParams), _dp, _dp, ctypes.c_int64, _dp]),
         "soap_phoenix_bracket": (ctypes.c_int, [ctypes.c_double, _dp, _dp]),
         "soap_phoenix_grid": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]),
@@ -144,11 +144,12 @@ def load(path: str = LIB_PATH):
         "soap_get_integrate_stats": (ctypes.c_int, [vp, _dp, _dp, ctypes.POINTER(ctypes.c_int64)]),
         "soap_get_stage_work": (ctypes.c_int, [vp, ctypes.c_int32, _dp, _dp]),
         "soap_set_fir_shape": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32]),
-        "soap_diff_setup": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double]),
-        "soap_diff_forward": (ctypes.c_int, [vp, ctypes.POINTER(DiffParams), vp, vp, vp, vp, _dp, _dp, _dp,
-                                             ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                             _dp, _dp, _dp, _dp]),
-        "soap_diff_backward": (ctypes.c_int, [vp, _dp, vp, vp, vp]),
+        "soap_diff_setup": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                           _dp]),
+        "soap_diff_forward": (ctypes.c_int, [vp, ctypes.POINTER(DiffParams), vp, vp, vp, vp, vp, vp, _dp, _dp,
+                                             _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, _dp, _dp, _dp, _dp]),
+        "soap_diff_backward": (ctypes.c_int, [vp, _dp, vp, vp, vp, vp, vp]),
         "soap_diff_get": (ctypes.c_int, [vp, ctypes.c_int32, _dp]),
         "soap_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int32, _dp, _dp, ctypes.POINTER(vp)]),
         "soap_batch_destroy": (None, [vp]),
@@ -167,8 +168,8 @@ def load(path: str = LIB_PATH):
         "soap_diff_batch_destroy": (None, [vp]),
         "soap_diff_batch_last_error": (ctypes.c_char_p, [vp]),
         "soap_diff_batch_bind": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_double, vp, vp, vp, vp, vp, vp, vp,
-                                                vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
-                                                ctypes.c_double, vp, vp, vp, vp, vp]),
+                                                vp, vp, vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                                ctypes.c_double, vp, vp, vp, vp, vp, vp, vp]),
         "soap_diff_batch_step": (ctypes.c_int, [vp, ctypes.c_int32, ctypes.c_int32]),
         "soap_diff_batch_sync": (ctypes.c_int, [vp]),
     }
@@ -418,7 +419,8 @@ class Engine:
         return air[:cnt.value].copy(), flux[:cnt.value].copy()
 
     def fit_ccf_host(self, v, y, err=None):
-        """BIS_FIT2 on host CCFs y[n_ccf][n_v]; err: per-point errors of the same shape (None = ones)."""
+        """BIS_FIT2 on host CCFs y[n_ccf][n_v]; err: per-point errors of the same shape (None = ones).
+        With errors the result also holds the solver outcome per CCF ('status', GSL codes)."""
         vv, vp = _d(v)
         yy = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
         n_ccf = yy.shape[0]
@@ -426,15 +428,20 @@ class Engine:
         mod, mp = _out(n_ccf * len(vv))
         dep, dp = _out(n_ccf * 91)
         bis, bp = _out(n_ccf * 91)
+        out = {}
         if err is None:
             self._chk(self.L.soap_fit_ccf_host(self.ctx, vp, len(vv), yy.ctypes.data_as(_dp), n_ccf, pp, mp, dp,
                                                bp), "fit_ccf_host")
         else:
             ee = np.ascontiguousarray(np.broadcast_to(np.atleast_2d(err), yy.shape), dtype=np.float64)
+            st, sp = _out(n_ccf, np.int32)
             self._chk(self.L.soap_fit_ccf_host_err(self.ctx, vp, len(vv), yy.ctypes.data_as(_dp),
-                                                   ee.ctypes.data_as(_dp), n_ccf, pp, mp, dp, bp), "fit_ccf_host_err")
-        return dict(par=par.reshape(n_ccf, 5), model=mod.reshape(n_ccf, -1), depth=dep.reshape(n_ccf, 91),
-                    bis=bis.reshape(n_ccf, 91))
+                                                   ee.ctypes.data_as(_dp), n_ccf, pp, mp, dp, bp, sp),
+                      "fit_ccf_host_err")
+            out["status"] = st
+        out.update(par=par.reshape(n_ccf, 5), model=mod.reshape(n_ccf, -1), depth=dep.reshape(n_ccf, 91),
+                   bis=bis.reshape(n_ccf, 91))
+        return out
 
     def run_series(self, star: dict, sigma_ms, quiet_weight=0.005):
         self._star = make_star(star)

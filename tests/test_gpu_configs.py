@@ -295,3 +295,180 @@ def test_config2_batch_equals_plain_and_oracle():
     finally:
         for e in engines:
             e.close()
+
+
+# ----------------------------------------------------------------- adjoint workload
+@pytest.fixture(scope="module")
+def config3_model():
+    import torch
+
+    from soap_gpu_amd.diff import DiffModel
+    from soap_gpu_amd.engine import select_mask_lines
+
+    w, maps, fmaps = synth.config3()
+    P, M = maps.shape[0], maps.shape[1]
+    sl, sw = select_mask_lines(w.wave, w.mask_l, w.mask_w, TRIM)
+    eng = Engine(0)
+    model = DiffModel(eng, w.wave, w.mu_bounds, sl, sw, w.vgrid, w.star["grid"], P, M, w.sigma_inst, TRIM,
+                      lb=w.star["lb"])
+    dev = torch.device("cuda")
+    t_ = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)  # noqa: E731
+    d = dict(w=w, model=model, eng=eng, P=P, M=M, q=t_(w.cubes["q"]), s=t_(w.cubes["sp"]), f=t_(w.cubes["fac"]),
+             maps=t_(maps), fmaps=t_(fmaps), zero=torch.zeros((P, w.n), dtype=torch.float64, device=dev))
+    model.forward((2.3, 0.31, 0.30, 0.35, 1.15), d["q"], d["s"], d["maps"] * 0.8, d["zero"], np.zeros(P),
+                  np.zeros(P), 1.0, 0.0, 0.0, f=d["f"], fmaps=d["fmaps"] * 0.8)
+    F = model.get("F")
+    th = model.get("theta")
+    rng = np.random.default_rng(1237)
+    d["T"] = t_(F + 1e-4 * np.abs(F).max() * rng.standard_normal(F.shape))
+    d["rv_t"] = th[:, 2] + rng.uniform(-5.0, 5.0, P)
+    d["fw_t"] = th[:, 3] + rng.uniform(-20.0, 20.0, P)
+    d["bis_t"] = model.get("bis") + rng.uniform(-5.0, 5.0, P)
+    d["wts"] = (1.0 / float(torch.sum(d["T"][:, TRIM:w.n - TRIM] ** 2)), 1e-3, 1e-5, 1e-3)
+    d["prm"] = np.array([2.0, 0.29, 0.34, 0.3, 1.1])
+    _log(f"[config3] model set up: n={w.n} grid={w.star['grid']} P={P} maps {M}x{M}")
+    yield d
+    eng.close()
+
+
+def _c3_loss(d, prm=None, **over):
+    a = dict(q=d["q"], s=d["s"], f=d["f"], maps=d["maps"], fmaps=d["fmaps"])
+    a.update(over)
+    w = d["wts"]
+    return d["model"].forward(d["prm"] if prm is None else prm, a["q"], a["s"], a["maps"], d["T"], d["rv_t"],
+                              d["fw_t"], *w[:3], bis_t=d["bis_t"], w_bis=w[3], f=a["f"], fmaps=a["fmaps"])[0]
+
+
+@pytest.mark.timeout(900)
+def test_config3_adjoint_vs_central_differences(config3_model):
+    """At the adjoint shape (grid 600, 100k pixels, 50 phases, 512 x 512 spot and
+    facula maps, 7473-line mask), every gradient of the full loss (spectra, RV,
+    FWHM, bisector span) along its own direction -- the five scalars, and one
+    random direction each of the quiet / spot / facula seed rows and of both maps
+    -- against central differences (h = 1e-6 relative) of the GPU forward, whose
+    parity with the oracle is held at small shapes in test_gpu_diff."""
+    import torch
+
+    d = config3_model
+    _c3_loss(d)
+    g5, gq, gs, gm, gf, gfm = d["model"].backward()
+    p0 = d["prm"]
+    for i, name in enumerate(("vsini", "u1", "u2", "contrast", "contrast_fac")):
+        h = 1e-6 * abs(p0[i])
+        e = np.eye(5)[i] * h
+        fd = (_c3_loss(d, p0 + e) - _c3_loss(d, p0 - e)) / (2 * h)
+        assert g5[i] == pytest.approx(fd, rel=2e-5, abs=1e-9 * np.abs(g5).max()), name
+        _log(f"[config3] d/d{name}: adjoint {g5[i]:.9e} central {fd:.9e}")
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    for name, g in (("q", gq), ("s", gs), ("f", gf), ("maps", gm), ("fmaps", gfm)):
+        base = d[name]
+        v = torch.randn(base.shape, dtype=torch.float64, device="cuda", generator=gen)
+        h = 1e-6 * float(torch.linalg.vector_norm(base)) / float(torch.linalg.vector_norm(v))
+        fd = (_c3_loss(d, **{name: base + h * v}) - _c3_loss(d, **{name: base - h * v})) / (2 * h)
+        ad = float(torch.sum(g * v))
+        assert ad == pytest.approx(fd, rel=2e-5, abs=1e-12), name
+        _log(f"[config3] directional d/d{name}: adjoint {ad:.9e} central {fd:.9e}")
+
+
+@pytest.mark.timeout(900)
+def test_config3_empty_maps_give_the_convolved_quiet_star(config3_model):
+    """With both maps empty the model spectrum is the instrumental convolution of
+    the plain engine's quiet star (itot, SOAP_initialize.cu:297-395) at the same
+    disk grid, limb darkening and vsini: F = conv(1) - o with o the plain path's
+    unnormalised quiet-frame convolution (conv(1) from the oracle's taps)."""
+    import torch
+
+    d = config3_model
+    w = d["w"]
+    z = torch.zeros_like(d["maps"])
+    d["model"].forward((2.0, 0.29, 0.34, 0.3, 1.1), d["q"], d["s"], z, d["zero"], np.zeros(d["P"]),
+                       np.zeros(d["P"]), 1.0, 0.0, 0.0, f=d["f"], fmaps=z)
+    F = d["model"].get("F")
+    eng = Engine(0)
+    try:
+        eng.set_wave(w.wave)
+        eng.set_mu_bounds(w.mu_bounds)
+        eng.set_cubes(w.cubes)
+        st = dict(w.star, lb=np.zeros(4))
+        eng.disk_init(st)
+        eng.quiet_star()
+        eng.alloc_phases(1)
+        eng.set_raw(0, *(np.zeros(w.n) for _ in range(3)))
+        eng.resolution(w.sigma_inst)
+        out = eng.get_lower(-1, BCONV)
+        mn, mx = eng.get_norm(-1, BCONV)
+    finally:
+        eng.close()
+    o = (1.0 - out) * mx / (1.0 - mn)
+    one = O.conv_raw(O.GRID_LOG, w.wave, w.sigma_inst, np.zeros(w.n), 0)
+    ref = np.where(one != 0.0, one - o, 0.0)
+    sl_ = slice(TRIM, w.n - TRIM)
+    for p in (0, d["P"] - 1):
+        e = float(np.max(np.abs(F[p][sl_] - ref[sl_])) / np.max(np.abs(ref[sl_])))
+        assert e <= 1e-11, (p, e)
+
+
+# ----------------------------------------------------------------- fitting workload
+@pytest.mark.timeout(900)
+def test_config4_graph_equals_per_call_and_fit_descends():
+    """Two instances of the fitting workload at full shape (grid 300-500,
+    30k-80k pixels, 20-60 phases, 256 x 256 spot and facula maps rendered from
+    their discrete regions): one graph replay of forward + adjoint equals the
+    per-call path (loss bit for bit, gradients to rounding), and five L-BFGS
+    iterations of the graph-captured loop lower the summed loss."""
+    import torch
+
+    from soap_gpu_amd.diff import DiffBatch
+    from soap_gpu_amd.fitting import Instance, fit_graph
+
+    prm = synth.config4_params()[:2]
+    made = [synth.config4_instance(p, 256, i) for i, p in enumerate(prm)]
+    insts, calls = [], []
+    for w, maps, fmaps, tr in made:
+        P = len(w.phases)
+        start = np.clip(0.5 * maps + 0.25 * maps.mean(), 0.0, 1.0)
+        fstart = np.clip(0.5 * fmaps + 0.25 * fmaps.mean(), 0.0, 1.0)
+        for lst in (insts, calls):
+            inst = Instance(w, start, P, 256, vsini0=tr["vsini"] * 1.2, contrast0=0.5 * (tr["contrast"] + 0.3),
+                            weights=(0.0, 1e-2, 1e-4, 1e-3), fmaps0=fstart, contrast_fac0=1.15)
+            rv, fw, bis = inst.observe((tr["vsini"], 0.29, 0.34, tr["contrast"], tr["contrast_fac"]), maps, fmaps)
+            inst.rv_t, inst.fw_t, inst.bis_t = rv, fw, bis
+            lst.append(inst)
+        _log(f"[config4] instance n={w.n} grid={w.star['grid']} P={P} regions={len(w.regions[0])}")
+    try:
+        dev = torch.device("cuda")
+        batch = DiffBatch([g.model for g in insts])
+        bufs = []
+        for i, g in enumerate(insts):
+            t = dict(prm=torch.tensor(g.params().detach().cpu().numpy(), dtype=torch.float64, device=dev),
+                     maps=g.maps().detach().contiguous(), fmaps=g.fmaps().detach().contiguous(),
+                     rv=torch.tensor(g.rv_t, dtype=torch.float64, device=dev),
+                     fw=torch.tensor(g.fw_t, dtype=torch.float64, device=dev),
+                     bis=torch.tensor(g.bis_t, dtype=torch.float64, device=dev),
+                     loss=torch.empty(1, dtype=torch.float64, device=dev),
+                     g5=torch.empty(5, dtype=torch.float64, device=dev), gq=torch.empty_like(g.q),
+                     gs=torch.empty_like(g.s), gf=torch.empty_like(g.q), gm=torch.empty_like(g.maps()),
+                     gfm=torch.empty_like(g.maps()))
+            batch.bind(i, 2.0 * float(g.theta[0]) + 5.0, t["prm"], g.q, g.s, t["maps"], g.target, t["rv"], t["fw"],
+                       t["bis"], g.weights, t["loss"], t["g5"], t["gq"], t["gs"], t["gm"], f=g.f,
+                       fmaps=t["fmaps"], gf=t["gf"], gfm=t["gfm"])
+            bufs.append(t)
+        batch.step(graph=True)
+        batch.step(graph=True)  # a replay after the capturing launch
+        batch.sync()
+        for t, c in zip(bufs, calls):
+            loss, _, _, _ = c.model.forward(c.params().detach().cpu().numpy(), c.q, c.s, c.maps().detach(),
+                                            c.target, c.rv_t, c.fw_t, *c.weights[:3], bis_t=c.bis_t,
+                                            w_bis=c.weights[3], f=c.f, fmaps=c.fmaps().detach())
+            g5, gq, gs, gm, gf, gfm = c.model.backward()
+            assert float(t["loss"][0]) == loss
+            np.testing.assert_array_equal(t["g5"].cpu().numpy(), g5)
+            for got, ref in ((t["gq"], gq), (t["gs"], gs), (t["gf"], gf), (t["gm"], gm), (t["gfm"], gfm)):
+                torch.testing.assert_close(got, ref, rtol=1e-9, atol=1e-12 * float(ref.abs().max()))
+        batch.close()
+        r = fit_graph(insts, iters=5)
+        _log(f"[config4] L-BFGS: loss {r['loss0']:.6e} -> best {r['best']:.6e} in {r['evals']} evaluations")
+        assert r["best"] < r["loss0"]
+    finally:
+        for x in insts + calls:
+            x.close()

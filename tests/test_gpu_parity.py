@@ -287,3 +287,38 @@ def test_errors_are_reported(engine):
     with pytest.raises(SoapError, match="run disk_init first"):
         Engine(0).quiet_star()
     e2.close()
+
+
+def test_fit_status_flat_and_normal_ccf(engine, config0_case):
+    """The solver outcome per CCF (GSL codes): a flat CCF makes no progress (27, as
+    the oracle's restatement of the trust-region loop reports), a real CCF
+    converges (0); the outputs are written in both cases, as the reference does."""
+    w, g, o = config0_case
+    y = np.stack([np.ones(len(w.vgrid)), o["ccf"].reshape(-1, len(w.vgrid))[0]])
+    r = engine.fit_ccf_host(w.vgrid, y, np.ones_like(y))
+    assert list(r["status"]) == [O.fit_ccf(w.vgrid, y[0])["status"], 0] == [27, 0]
+    assert r["par"][0, 0] == 1.0 and r["par"][0, 1] == 0.0
+
+
+def test_host_forced_failures(engine):
+    """Inputs the reference would carry on with silently end in a status here: a
+    NaN in a seed cube; a disk grid of 1 (no cell on the disk) runs through with
+    zero spectra and a fit that reports no progress."""
+    w = synth.small(n_phase=1)
+    eng = Engine(0)
+    try:
+        eng.set_wave(w.wave)
+        eng.set_mu_bounds(w.mu_bounds)
+        bad = w.cubes["q"].copy()
+        bad[2, 1000] = np.nan
+        with pytest.raises(RuntimeError, match="mu row 2, pixel 1000"):
+            eng.set_cube(0, bad)
+        setup_workload(eng, w)
+        st = dict(w.star, grid=1)
+        eng.disk_init(st)
+        eng.quiet_star()
+        assert not np.any(eng.get_quiet(0))
+        eng.integrate()
+        assert not any(np.any(x) for x in eng.get_raw(0))
+    finally:
+        eng.close()
