@@ -15,6 +15,9 @@ def pytest_configure(config):
 
 @pytest.fixture(scope="session")
 def engine():
+    # torch's HIP runtime must come up before the library's: a process whose first
+    # HIP initialisation was the library's sees no GPU through torch afterwards
+    import torch  # noqa: F401
     from soap_gpu_amd.engine import Engine
     eng = Engine(0)
     yield eng

@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import numpy as np
 import pytest
+import torch  # noqa: F401  (before the library initialises HIP)
 
 import oracle as O
 from soap_gpu_amd import synth
