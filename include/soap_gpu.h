@@ -190,6 +190,23 @@ int soap_fit_ccf_host(soap_ctx *ctx, const double *v, int32_t n_v, const double 
  * (BIS_FIT2.cpp:191) and so does the call: it returns SOAP_OK either way. */
 int soap_fit_ccf_host_err(soap_ctx *ctx, const double *v, int32_t n_v, const double *y, const double *err,
                           int32_t n_ccf, double *par, double *model, double *depth, double *bis, int32_t *status);
+/* soap_fit_ccf_host_err without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (tight serving or
+ * fitting loops, harnesses that forbid hipMalloc).  Inputs and outputs are the
+ * caller's host arrays as above; the device staging (inputs, fit parameters,
+ * model, depth, bisector) lives in a fixed 16 MiB array per GPU held by the
+ * library until the process ends, so a call needs
+ *   n_v + n_ccf (n_v [y] + n_v [err, if given] + 5 + n_v + 182) doubles
+ * (each part rounded up to 32; + n_ccf / 2 with status) <= 2^21, else
+ * SOAP_ERR_LIMIT and nothing runs.
+ * Every call on one GPU uses that GPU's one array (each process holds its own)
+ * and it keeps the last call's contents: calls that overlapped on one GPU would
+ * race on it, so the library serialises them with a lock (a call waits for the
+ * one before it) and each returns after its results are on the host.  Results
+ * equal soap_fit_ccf_host_err's bit for bit. */
+int soap_fit_ccf_host_noalloc(soap_ctx *ctx, const double *v, int32_t n_v, const double *y, const double *err,
+                              int32_t n_ccf, double *par, double *model, double *depth, double *bis,
+                              int32_t *status);
 
 /* ------------------------------------------------------------ seed preprocessing: bisector injection */
 /* The convective-blueshift injection of CB_injection_faculae.py:191-244 on one

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <mutex>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -332,6 +333,31 @@ static void inverse_rotation_row0(double lon, double lat, double incl_deg, doubl
   for (int c = 0; c < 3; c++) r0[c] = R2r0[0] * R[0][c] + R2r0[1] * R[1][c] + R2r0[2] * R[2][c];
 }
 
+// ------------------------------------------------------------------ allocation-free staging
+// Staging for soap_fit_ccf_host_noalloc: a file-scope device array, one copy per
+// GPU, held until the process ends.  Host code never uses its name as an address
+// (that is a host-side stand-in); each device's address comes from
+// hipGetSymbolAddress, taken once per device at context creation with that
+// device current.
+constexpr size_t kScratchDoubles = size_t(2) << 20;  // 16 MiB per GPU
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_lock[kMaxDevices];  // one call at a time per device in this process
+static std::mutex g_scratch_init;
+
+// called with `device` current
+static hipError_t scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(g_scratch_init);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) return e != hipSuccess ? e : hipErrorInvalidSymbol;
+  g_scratch_ptr[device] = (double *)ptr;
+  return hipSuccess;
+}
+
 // ------------------------------------------------------------------ API
 extern "C" {
 
@@ -344,6 +370,7 @@ int soap_ctx_create(int device, soap_ctx **out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return SOAP_ERR_HIP;
   if (device < 0 || device >= ndev) return SOAP_ERR_ARG;
   if (hipSetDevice(device) != hipSuccess) return SOAP_ERR_HIP;
+  if (scratch_setup(device) != hipSuccess) return SOAP_ERR_HIP;
   soap_ctx *c = new soap_ctx();
   c->device = device;
   if (hipStreamCreate(&c->stream) != hipSuccess) {
@@ -1889,6 +1916,49 @@ int soap_fit_ccf_host_err(soap_ctx *c, const double *v, int32_t n_v, const doubl
 int soap_fit_ccf_host(soap_ctx *c, const double *v, int32_t n_v, const double *y, int32_t n_ccf, double *par,
                       double *model, double *depth, double *bis) {
   return soap_fit_ccf_host_err(c, v, n_v, y, nullptr, n_ccf, par, model, depth, bis, nullptr);
+}
+
+int soap_fit_ccf_host_noalloc(soap_ctx *c, const double *v, int32_t n_v, const double *y, const double *err,
+                              int32_t n_ccf, double *par, double *model, double *depth, double *bis,
+                              int32_t *status) {
+  if (!c || !v || !y || n_v < 3 || n_v > 1024 || n_ccf < 1 || !par)
+    return fail(c, SOAP_ERR_ARG, "fit_ccf_host_noalloc: bad arguments");
+  if (err)
+    for (size_t i = 0; i < (size_t)n_ccf * n_v; i++)
+      if (!std::isfinite(err[i]) || err[i] == 0.0)
+        return fail(c, SOAP_ERR_ARG, "fit_ccf_host_noalloc: error %zu is %g (must be finite and non-zero)", i,
+                    err[i]);
+  // staging layout, each part rounded up to 32 doubles (256 bytes)
+  auto up = [](size_t d) { return (d + 31) / 32 * 32; };
+  const size_t nv = (size_t)n_v, nc = (size_t)n_ccf;
+  const size_t o_v = 0, o_y = o_v + up(nv), o_e = o_y + up(nc * nv), o_par = o_e + (err ? up(nc * nv) : 0),
+               o_mod = o_par + up(nc * 5), o_dep = o_mod + up(nc * nv), o_bis = o_dep + up(nc * 91),
+               o_st = o_bis + up(nc * 91), total = o_st + (status ? up((nc + 1) / 2) : 0);
+  if (total > kScratchDoubles)
+    return fail(c, SOAP_ERR_LIMIT,
+                "fit_ccf_host_noalloc: %d CCFs of %d points need %zu staging doubles, the limit is %zu "
+                "(16 MiB per GPU); use soap_fit_ccf_host_err or fewer CCFs per call",
+                n_ccf, n_v, total, kScratchDoubles);
+  double *base = g_scratch_ptr[c->device];
+  if (!base) return fail(c, SOAP_ERR_STATE, "fit_ccf_host_noalloc: no staging address for device %d", c->device);
+  std::lock_guard<std::mutex> g(g_scratch_lock[c->device]);
+  HIPCHK(c, hipSetDevice(c->device));
+  hipError_t e = hipMemcpyAsync(base + o_v, v, nv * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(base + o_y, y, nc * nv * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && err) e = hipMemcpyAsync(base + o_e, err, nc * nv * 8, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess)
+    e = launch_fit(base + o_v, n_v, base + o_y, err ? base + o_e : nullptr, n_ccf, base + o_par, base + o_mod,
+                   base + o_dep, base + o_bis, nullptr, c->stream, status ? (int32_t *)(base + o_st) : nullptr);
+  if (e == hipSuccess) e = hipMemcpyAsync(par, base + o_par, nc * 5 * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && model) e = hipMemcpyAsync(model, base + o_mod, nc * nv * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && depth) e = hipMemcpyAsync(depth, base + o_dep, nc * 91 * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && bis) e = hipMemcpyAsync(bis, base + o_bis, nc * 91 * 8, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess && status) e = hipMemcpyAsync(status, base + o_st, nc * 4, hipMemcpyDeviceToHost, c->stream);
+  // the staging is reused by the next call on this device: wait before releasing it
+  hipError_t e2 = hipStreamSynchronize(c->stream);
+  if (e == hipSuccess) e = e2;
+  if (e != hipSuccess) return fail(c, SOAP_ERR_HIP, "fit_ccf_host_noalloc: %s", hipGetErrorString(e));
+  return SOAP_OK;
 }
 
 // ------------------------------------------------------------------ a11: differentiable path

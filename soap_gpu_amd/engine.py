@@ -48,7 +48,8 @@ EXPORTS = (
     "soap_set_activity", "soap_integrate", "soap_get_raw", "soap_set_raw", "soap_alloc_phases", "soap_phase_maps",
     "soap_resolution", "soap_get_lower", "soap_get_norm",
     "soap_set_mask", "soap_ccf", "soap_get_ccf",
-    "soap_fit", "soap_get_fit", "soap_fit_ccf_host", "soap_fit_ccf_host_err", "soap_bis_inject",
+    "soap_fit", "soap_get_fit", "soap_fit_ccf_host", "soap_fit_ccf_host_err", "soap_fit_ccf_host_noalloc",
+    "soap_bis_inject",
     "soap_phoenix_bracket", "soap_phoenix_grid", "soap_phoenix_name", "soap_fits_read", "soap_phoenix_interp",
     "soap_run_series",
     "soap_set_timing", "soap_get_stage_ms", "soap_stage_name", "soap_num_stages",
@@ -127,6 +128,8 @@ def load(path: str = LIB_PATH):
         "soap_fit_ccf_host": (ctypes.c_int, [vp, _dp, ctypes.c_int32, _dp, ctypes.c_int32, _dp, _dp, _dp, _dp]),
         "soap_fit_ccf_host_err": (ctypes.c_int, [vp, _dp, ctypes.c_int32, _dp, _dp, ctypes.c_int32, _dp, _dp, _dp,
                                                  _dp, _ip]),
+        "soap_fit_ccf_host_noalloc": (ctypes.c_int, [vp, _dp, ctypes.c_int32, _dp, _dp, ctypes.c_int32, _dp, _dp,
+                                                     _dp, _dp, _ip]),
         "soap_bis_inject": (ctypes.c_int, [vp, ctypes.POINTER(BisInjectParams), _dp, _dp, ctypes.c_int64, _dp]),
         "soap_phoenix_bracket": (ctypes.c_int, [ctypes.c_double, _dp, _dp]),
         "soap_phoenix_grid": (ctypes.c_int, [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]),
@@ -442,6 +445,24 @@ class Engine:
         out.update(par=par.reshape(n_ccf, 5), model=mod.reshape(n_ccf, -1), depth=dep.reshape(n_ccf, 91),
                    bis=bis.reshape(n_ccf, 91))
         return out
+
+    def fit_ccf_host_noalloc(self, v, y, err=None):
+        """fit_ccf_host through the allocation-free entry point (fixed per-GPU staging)."""
+        vv, vp = _d(v)
+        yy = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
+        n_ccf = yy.shape[0]
+        par, pp = _out(n_ccf * 5)
+        mod, mp = _out(n_ccf * len(vv))
+        dep, dp = _out(n_ccf * 91)
+        bis, bp = _out(n_ccf * 91)
+        ee = None if err is None else np.ascontiguousarray(np.broadcast_to(np.atleast_2d(err), yy.shape),
+                                                           dtype=np.float64)
+        st, sp = _out(n_ccf, np.int32)
+        self._chk(self.L.soap_fit_ccf_host_noalloc(self.ctx, vp, len(vv), yy.ctypes.data_as(_dp),
+                                                   None if ee is None else ee.ctypes.data_as(_dp), n_ccf, pp, mp,
+                                                   dp, bp, sp), "fit_ccf_host_noalloc")
+        return dict(par=par.reshape(n_ccf, 5), model=mod.reshape(n_ccf, -1), depth=dep.reshape(n_ccf, 91),
+                    bis=bis.reshape(n_ccf, 91), status=st)
 
     def run_series(self, star: dict, sigma_ms, quiet_weight=0.005):
         self._star = make_star(star)

@@ -298,6 +298,8 @@ def test_fit_status_flat_and_normal_ccf(engine, config0_case):
     r = engine.fit_ccf_host(w.vgrid, y, np.ones_like(y))
     assert list(r["status"]) == [O.fit_ccf(w.vgrid, y[0])["status"], 0] == [27, 0]
     assert r["par"][0, 0] == 1.0 and r["par"][0, 1] == 0.0
+    rn = engine.fit_ccf_host_noalloc(w.vgrid, y)
+    assert list(rn["status"]) == [27, 0]
 
 
 def test_host_forced_failures(engine):
