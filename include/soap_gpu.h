@@ -274,10 +274,15 @@ int32_t soap_num_stages(void);
  * counted per phase, sum_psi 8 n (R_psi + continua + 3); rows_sum = sum_psi R_psi */
 int soap_get_integrate_stats(soap_ctx *ctx, double *bytes_min, double *bytes_phase_sum, int64_t *rows_sum);
 /* Launch shape of the active-phase FIR of soap_integrate (log grids, non-solar):
- * (4, 0) = four phases per block, one pixel per thread; (G, B) with B > 0 =
- * G phases per block, B pixels per thread (register-blocked), one of (1, 3),
- * (1, 5), (2, 3), (2, 1), (4, 1); (-1, -1) = automatic.  Results agree to
- * rounding across shapes.  No counterpart in the reference (tuning support). */
+ * (4, -7) = a 64-pixel tile and every phase per block, all seed rows of the tile
+ * staged once, taps broadcast across lanes (the automatic choice); (4, -4) = the
+ * same with the taps in scalar registers; (4, 0) = four phases per block, one
+ * pixel per thread, rows staged per mu bin; (G, -2) = G in {1, 2, 4} phases per
+ * block reading the rows from global memory; (G, B) with B > 0 = G phases per
+ * block, B pixels per thread (register-blocked), one of (1, 3), (1, 5), (2, 3),
+ * (2, 1), (4, 1); (-1, -1) = automatic.  Results agree to rounding across shapes
+ * (bit for bit among the one-pixel forms).  No counterpart in the reference
+ * (tuning support). */
 int soap_set_fir_shape(soap_ctx *ctx, int32_t phases_per_block, int32_t pixels_per_thread);
 /* ------------------------------------------------------------ a11: differentiable path
  * No counterpart in the reference (a differentiable forward model with its FP64

@@ -88,6 +88,7 @@ struct soap_ctx {
   RegionDev *d_geo = nullptr;
   int32_t *d_geo_off = nullptr, *d_box = nullptr;
   int max_col_blocks = 0;
+  bool ccf_fused = false;     // the last CCF ran with spec_norm in its launch (k_norm_ccf)
   int fir_g = 0, fir_b = -1;  // FIR launch shape (phases per block, pixels per thread); -1 = automatic
   double tap_wmax = 0;        // bound on |limb| and |limb (1 + LB)| over the disk (fixed-point tap capacity)
   double solar_leb = 1;       // max over the facula temperatures of sum_q |l_q(T)| (solar Lagrange weights)
@@ -118,6 +119,7 @@ struct soap_ctx {
   // tap scratch
   DevBuf acc, taps, krange, kflags, bins, nbins;
   DevBuf taps_g;  // phase-interleaved active taps [n_mu][nd][taps_g_pad(P)][4]
+  DevBuf prog_t, prog_x, prog_n, prog_u;  // entry streams of the tile-stationary FIR
   DevBuf inj;      // bisector injection: wave, flux, moved, sorted x, sorted y, out [6][n], partials, sort scratch
   DevBuf phx;      // PHOENIX interpolation: wave, low, high, air, flux, out air, out flux [7][n], flags, count, scratch
   // quiet-star tap scratch of its own, so that the quiet star can run beside the
@@ -1133,11 +1135,25 @@ int soap_integrate(soap_ctx *c) {
       HIPCHK(c, e);
     } else {
       int fg = c->fir_g, fb = c->fir_b;
-      if (fb == -1) fg = 4, fb = 0;
+      const bool automatic = fb == -1;
+      if (automatic) fg = 0, fb = -7;  // the DPP tile form, else the grouped form
+      const bool tile = fb == -4 || fb == -7 || fb == -8;
+      FirProg pg{};
+      if (tile) {
+        const int ng = fir_tile_groups(c->P), cap = c->n_mu * c->nd + 6;
+        int rc;
+        if ((rc = ensure(c, c->prog_t, (size_t)ng * cap * 16 * 8)) || (rc = ensure(c, c->prog_x, (size_t)ng * cap * 4)) ||
+            (rc = ensure(c, c->prog_n, (size_t)ng * 4)) || (rc = ensure(c, c->prog_u, (size_t)c->n_mu * 4)))
+          return rc;
+        pg = FirProg{(double *)c->prog_t.p, (int32_t *)c->prog_x.p, (int32_t *)c->prog_n.p, (int32_t *)c->prog_u.p, cap};
+      }
       hipError_t e = fb == -2  ? launch_fir_direct(f, c->P, fg, c->stream)
+                     : tile ? launch_fir_tile(f, c->P, pg, fb != -4, fb == -4 ? 8 : fg, fb == -8 ? 6 : 4, c->stream)
                      : fb == 0 ? launch_fir_active_grp(f, c->P, c->stream)
                                : launch_fir2(f, c->P, fg, fb, c->stream);
-      if (e == hipErrorInvalidValue) e = launch_fir_active(f, c->P, c->stream);  // very wide shift ranges
+      // shift ranges too wide for a form's LDS: the grouped form, then the plain one
+      if (e == hipErrorInvalidValue && automatic) e = launch_fir_active_grp(f, c->P, c->stream);
+      if (e == hipErrorInvalidValue) e = launch_fir_active(f, c->P, c->stream);
       HIPCHK(c, e);
     }
   }
@@ -1149,7 +1165,7 @@ int soap_integrate(soap_ctx *c) {
 
 int soap_set_fir_shape(soap_ctx *c, int32_t phases_per_block, int32_t pixels_per_thread) {
   if (!c) return SOAP_ERR_ARG;
-  const int ok[][2] = {{-1, -1}, {4, 0}, {1, 3}, {1, 5}, {2, 3}, {2, 1}, {4, 1}, {1, -2}, {2, -2}, {4, -2}};
+  const int ok[][2] = {{-1, -1}, {4, 0}, {1, 3}, {1, 5}, {2, 3}, {2, 1}, {4, 1}, {1, -2}, {2, -2}, {4, -2}, {4, -4}, {0, -7}, {4, -7}, {5, -7}, {6, -7}, {7, -7}, {8, -7}, {0, -8}, {5, -8}, {8, -8}};
   for (auto &s : ok)
     if (s[0] == phases_per_block && s[1] == pixels_per_thread) {
       c->fir_g = phases_per_block;
@@ -1248,7 +1264,9 @@ int soap_get_stage_work(soap_ctx *c, int32_t stage, double *bytes, double *flops
       double entries = 0;
       for (int32_t v : len) entries += v > 0 ? v : 0;
       double n_spec = 3.0 * (c->P + 1);
-      *bytes = 8 * (n - 2.0 * c->trim) * n_spec;  // every trimmed spectrum read once
+      // every trimmed spectrum read once; with spec_norm in the same launch, every
+      // spectrum read and its normalised form written instead
+      *bytes = c->ccf_fused ? 2 * 8 * n * n_spec : 8 * (n - 2.0 * c->trim) * n_spec;
       *flops = 2 * entries * n_spec;
       return SOAP_OK;
     }
@@ -1721,7 +1739,9 @@ static int ccf_impl(soap_ctx *c, bool norm) {
     StageTimer tm(c, ST_CCF);
     HIPCHK(c, launch_norm_ccf(a, c->stream));
     c->have_lower = true;
+    c->ccf_fused = true;
   } else {
+    c->ccf_fused = false;
     StageTimer tm(c, ST_CCF);
     HIPCHK(c, ctx_ccf(c, c->d_lraw, n_spec, (double *)c->ccf_part.p, c->stream));
   }
@@ -2240,8 +2260,9 @@ int soap_run_series(soap_ctx *c, const soap_star *star, double sigma_ms, double 
     if ((rc = soap_quiet_star(c))) return rc;
     if ((rc = soap_integrate(c))) return rc;
   }
-  // spec_norm shares its launch with the CCF unless stages are timed one by one
-  const bool fuse = !c->timing && c->have_mask;
+  // spec_norm shares its launch with the CCF (stage timing then reports the pair
+  // under the CCF stage, as it runs)
+  const bool fuse = c->have_mask;
   if ((rc = resolution_impl(c, sigma_ms, quiet_weight, !fuse))) return rc;
   if (c->have_mask) {
     if ((rc = ccf_impl(c, fuse))) return rc;

@@ -70,6 +70,15 @@ struct FirArgs {
   double *out;           // quiet: [2][n]; active: [job][3][n]
 };
 
+// Entry streams of the tile-stationary FIR (built on the device per launch).
+struct FirProg {
+  double *taps;    // [n_groups][cap][16]: the group's 4 phases x (H1s, H1f, H2f, H1a)
+  int32_t *xoff;   // [n_groups][cap]: LDS byte offset of lane 0's source pixel
+  int32_t *count;  // [n_groups]: entries in use, padded with zero entries to the consumer's step
+  int32_t *ufl;    // [n_mu]: bin flags over every phase (bit 0 spot, 1 facula, 2 used)
+  int32_t cap;     // entries per group, >= n_mu * nd + 6
+};
+
 // Solar-mode epilogue: the Planck ratio B(lambda, T)/B(lambda, t_norm) at the
 // K temperature nodes multiplies the Lagrange-set FIR sums.
 struct SolarFir {
@@ -272,6 +281,14 @@ hipError_t launch_fir_active_grp(const FirArgs &a, int n_job, hipStream_t s);
 hipError_t launch_fir2(const FirArgs &a, int n_job, int G, int B, hipStream_t s);
 // direct form (no LDS staging): G phases per block, G in {1, 2, 4}
 hipError_t launch_fir_direct(const FirArgs &a, int n_job, int G, hipStream_t s);
+// tile-stationary form: 64 pixels and every phase per block; the taps in scalar
+// registers (8 waves), or (dpp) in vector registers broadcast by DPP with `waves`
+// waves per block (4..8; 0: chosen from the phase-group count) and `ahead` (4 or
+// 6) stream entries in flight per wave
+hipError_t launch_fir_tile(const FirArgs &a, int n_job, const FirProg &pg, bool dpp, int waves, int ahead,
+                           hipStream_t s);
+size_t fir_tile_lds(int n_mu, int nd, int t64);
+int fir_tile_groups(int n_job);
 size_t fir2_lds(int nd, int G, int B);
 int fir2_tiles(int64_t n, int B);
 // solar mode: taps [job][n_mu][nd][K+3] = H1s, H1f, L_0..L_{K-1}, H1a

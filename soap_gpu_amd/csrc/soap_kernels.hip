@@ -659,6 +659,414 @@ hipError_t launch_fir_active_grp(const FirArgs &a, int n_job, hipStream_t s) {
   return hipGetLastError();
 }
 
+// Tile-stationary FIR.  Block = a tile of 64 x T64 pixels and every phase.  The
+// seed rows of all mu bins that any phase uses are staged in LDS once per block,
+// interleaved per source pixel as (quiet, quiet SED, spot, facula) so that one
+// pixel's four values are two 16-byte reads; after one barrier each wave walks its
+// own phase groups (kTileG phases each) without further synchronisation.  A
+// group's work is a stream of (bin, offset) entries in bin-then-offset order (the
+// order of every other FIR form, so the sums are the same bit for bit), built once
+// per launch by k_fir_prog: the group's 4 x kTileG taps at that (bin, offset) and
+// the LDS byte offset of lane 0's source pixel.  Scalar (SMEM) and LDS loads share
+// one counter and scalar loads return out of order, so each entry is waited for in
+// full at the top of its step, the next entry's loads are issued, and the FMAs of
+// the current one then run while those loads are in flight (registers A/B
+// alternate).  Blocks are numbered so that each XCD takes a contiguous run of
+// tiles, whose halos then come from its L2.
+constexpr int kTileG = 4;      // phases per group
+
+constexpr int kProgThreads = 1024;
+
+// xscale: bytes per staged source pixel and bin row (32: the four rows interleaved;
+// 16: two planes of row pairs); mult: the stream length is padded with zero entries
+// (reading the last entry's staged pixels) to a multiple of mult, the entries a
+// consuming loop takes per step
+__global__ __launch_bounds__(kProgThreads) void k_fir_prog(FirArgs a, FirProg pg, int n_job, int n_groups, int lmax,
+                                                           int xscale, int mult) {
+  const int g = blockIdx.x;
+  const int n_mu = a.n_mu, nd = a.nd;
+  const GLOBAL int32_t *kflags = gl(a.kflags), *krange = gl(a.krange);
+  if (g == n_groups) {  // union over every phase: which rows the tiles stage
+    __shared__ int s_u[64];
+    const int k = threadIdx.x & 63, slice = threadIdx.x >> 6;  // 16 phase slices per bin
+    if (threadIdx.x < 64) s_u[k] = 0;
+    __syncthreads();
+    int u = 0;
+    if (k < n_mu)
+      for (int p = slice; p < n_job; p += kProgThreads / 64) {
+        const long jk = (long)p * n_mu + k;
+        const int fl = kflags[jk];
+        if (fl && krange[jk * 2] <= krange[jk * 2 + 1]) u |= fl | 4;
+      }
+    if (u) atomicOr(&s_u[k], u);
+    __syncthreads();
+    if (threadIdx.x < n_mu) gw(pg.ufl)[threadIdx.x] = s_u[threadIdx.x];
+    return;
+  }
+  if (threadIdx.x >= 64) return;  // one wave per group; lane = mu bin
+  const int k = threadIdx.x;
+  const int p0 = g * kTileG;
+  int ufl = 0, ulo = nd, uhi = -1;
+  if (k < n_mu)
+    for (int q = 0; q < kTileG; q++) {
+      const int p = p0 + q;
+      if (p >= n_job) break;
+      const long jk = (long)p * n_mu + k;
+      const int fl = kflags[jk], lo = krange[jk * 2], hi = krange[jk * 2 + 1];
+      if (fl && lo <= hi) {
+        ufl |= fl;
+        ulo = min(ulo, lo);
+        uhi = max(uhi, hi);
+      }
+    }
+  const int cnt = ufl ? uhi - ulo + 1 : 0;
+  int pos = cnt;  // inclusive prefix over the lanes (bins in order)
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(pos, o);
+    if (k >= o) pos += v;
+  }
+  const int total = __shfl(pos, 63);
+  pos -= cnt;
+  GLOBAL double *T = gw(pg.taps) + (long)g * pg.cap * (4 * kTileG);
+  GLOBAL int32_t *X = gw(pg.xoff) + (long)g * pg.cap;
+  const GLOBAL double *taps = gl(a.taps);
+  const unsigned long long used = __ballot(cnt > 0);
+  const int klast = used ? 63 - __clzll(used) : 0;
+  const int hlast = __shfl(uhi, klast);
+  const int padded = total ? (total + mult - 1) / mult * mult : 0;
+  if (k == 0) gw(pg.count)[g] = padded;
+  if (k < padded - total) {  // zero entries
+    X[total + k] = (klast * lmax + nd - 1 - hlast) * xscale;
+    for (int t = 0; t < 4 * kTileG; t++) T[(long)(total + k) * (4 * kTileG) + t] = 0.0;
+  }
+  for (int di = ulo, e = pos; di <= uhi; di++, e++) {
+    X[e] = (k * lmax + nd - 1 - di) * xscale;
+    for (int q = 0; q < kTileG; q++) {
+      const int p = p0 + q;
+      const GLOBAL double *src = taps + (((long)p * n_mu + k) * nd + di) * 4;
+      for (int t = 0; t < 4; t++) T[(long)e * (4 * kTileG) + q * 4 + t] = p < n_job ? src[t] : 0.0;
+    }
+  }
+}
+
+__device__ __forceinline__ void lgkm_wait_all() { __builtin_amdgcn_s_waitcnt(0xc07f); }  // lgkmcnt(0)
+
+template <int T64, int NW>
+__device__ __forceinline__ void fir_tile_body(const FirArgs &a, const double *__restrict__ ptaps,
+                                              const int32_t *__restrict__ pxoff, const int32_t *__restrict__ pcount,
+                                              const int32_t *__restrict__ pufl, double *__restrict__ out, int cap,
+                                              int n_job, int n_groups, int n_tiles, int tiles_per_xcd) {
+  extern __shared__ double2 srow[];  // [n_mu][lmax][2]: (q, qs), (sp, fa)
+  constexpr int TP = 64 * T64;
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * tiles_per_xcd + (b >> 3);
+  if (tile >= n_tiles) return;  // uniform over the block
+  const int nd = a.nd, n_mu = a.n_mu, lmax = TP + nd - 1;
+  const long n = a.n;
+  const long j0 = (long)tile * TP;
+  const long s0 = j0 - a.dmin - (nd - 1);
+  // staging: every (bin, source pixel) of the bins in use; rows a bin does not use
+  // are zero (their taps are zero too)
+  {
+    const GLOBAL double *q = gl(a.q), *qs = gl(a.qs), *sp = gl(a.sp), *fa = gl(a.fa);
+    const int tot = n_mu * lmax;
+    for (int e = threadIdx.x; e < tot; e += 64 * NW) {
+      const int k = e / lmax, x = e - k * lmax;
+      const int u = pufl[k];
+      if (!u) continue;
+      const long i = s0 + x;
+      const bool in = i >= 0 && i < n;
+      const long o = (long)k * n + i;
+      const double vq = in ? q[o] : 0.0, vs = in ? qs[o] : 0.0;
+      const double vp = (in && (u & 1)) ? sp[o] : 0.0, vf = (in && (u & 2)) ? fa[o] : 0.0;
+      srow[2 * e] = make_double2(vq, vs);
+      srow[2 * e + 1] = make_double2(vp, vf);
+    }
+  }
+  __syncthreads();
+  const int w = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;  // w: the same in every lane of a wave
+  const int n_items = T64 * n_groups;
+  for (int it = w; it < n_items; it += NW) {
+    const int sub = T64 == 1 ? 0 : it % T64, g = T64 == 1 ? it : it / T64;
+    const int px = sub * 64 + lane;
+    const long j = j0 + px;
+    const double cs = (j < n && a.sps) ? gl(a.sps)[j] : 0.0;
+    const double cf = (j < n && a.fas) ? gl(a.fas)[j] : 0.0;
+    const char *lb = (const char *)srow + px * 32;
+    const double *T = ptaps + (long)g * cap * (4 * kTileG);
+    const int32_t *X = pxoff + (long)g * cap;
+    const int ne = pcount[g];
+    double F[kTileG][7];
+#pragma unroll
+    for (int q = 0; q < kTileG; q++)
+#pragma unroll
+      for (int f = 0; f < 7; f++) F[q][f] = 0.0;
+    auto loadt = [&](double(&t)[4 * kTileG], int m) {
+      const double *p = T + (long)m * (4 * kTileG);
+#pragma unroll
+      for (int i = 0; i < 4 * kTileG; i++) t[i] = p[i];
+    };
+    auto loadr = [&](double2 &r0, double2 &r1, int x) {
+      const double2 *p = (const double2 *)(lb + x);
+      r0 = p[0];
+      r1 = p[1];
+    };
+    auto fmas = [&](const double(&t)[4 * kTileG], const double2 r0, const double2 r1) {
+      const double vq = r0.x, vs = r0.y, vp = r1.x, vf = r1.y;
+#pragma unroll
+      for (int q = 0; q < kTileG; q++) {
+        const double t0 = t[q * 4], t1 = t[q * 4 + 1], t2 = t[q * 4 + 2], t3 = t[q * 4 + 3];
+        F[q][0] = fma(t3, vs, F[q][0]);
+        F[q][1] = fma(t0, vq, F[q][1]);
+        F[q][4] = fma(t0, vp, F[q][4]);
+        F[q][2] = fma(t2, vq, F[q][2]);
+        F[q][3] = fma(t1, vq, F[q][3]);
+        F[q][5] = fma(t1, vf, F[q][5]);
+        F[q][6] = fma(t2, vf, F[q][6]);
+      }
+    };
+    if (ne > 0) {  // ne is even
+      double ta[4 * kTileG], tb[4 * kTileG];
+      double2 a0, a1, b0, b1;
+      const int last = ne - 1;
+      loadt(ta, 0);
+      loadr(a0, a1, X[0]);
+      int xn = X[1];
+      for (int m = 0; m < ne; m += 2) {
+        lgkm_wait_all();
+        loadt(tb, m + 1);
+        loadr(b0, b1, xn);
+        const int xn2 = X[min(m + 2, last)];
+        __builtin_amdgcn_sched_barrier(0);
+        fmas(ta, a0, a1);
+        lgkm_wait_all();
+        loadt(ta, min(m + 2, last));
+        loadr(a0, a1, xn2);
+        xn = X[min(m + 3, last)];
+        __builtin_amdgcn_sched_barrier(0);
+        fmas(tb, b0, b1);
+      }
+    }
+    if (j < n) {
+      const int p0 = g * kTileG;
+#pragma unroll
+      for (int q = 0; q < kTileG; q++) {
+        if (p0 + q >= n_job) break;
+        double *o = out + (long)(p0 + q) * 3 * n;
+        // F: 0 = H1a*SEDq, 1 = H1s*Sq, 2 = H2f*Sq, 3 = H1f*Sq, 4 = H1s*Ssp, 5 = H1f*Sfac, 6 = H2f*Sfac
+        o[j] = F[q][0] - cs * F[q][1] - cf * F[q][2];
+        o[n + j] = (F[q][1] + F[q][3]) - F[q][4] - F[q][5];
+        o[2 * n + j] = F[q][0] - cs * F[q][4] - cf * F[q][6];
+      }
+    }
+  }
+}
+
+template <int T64, int NW>
+__global__ __launch_bounds__(64 * NW) void k_fir_tile(FirArgs a, const double *__restrict__ ptaps,
+                                                      const int32_t *__restrict__ pxoff,
+                                                      const int32_t *__restrict__ pcount,
+                                                      const int32_t *__restrict__ pufl, double *__restrict__ out,
+                                                      int cap, int n_job, int n_groups, int n_tiles,
+                                                      int tiles_per_xcd) {
+  fir_tile_body<T64, NW>(a, ptaps, pxoff, pcount, pufl, out, cap, n_job, n_groups, n_tiles, tiles_per_xcd);
+}
+
+// Tile-stationary FIR with the taps in vector registers.  Scalar loads of the
+// taps from L2 cost about a thousand cycles each and the scalar registers hold one
+// entry ahead at most, so here every lane loads one tap of the entry (lane l: tap
+// l mod 16, the four 16-lane rows of a wave alike) with vector loads issued four
+// entries ahead, and each FMA broadcasts its tap from lane n of the lane's row
+// (v_fmac_f64 with DPP row_newbcast:n, a single rounding like fma()).  The LDS
+// holds two planes per bin: (quiet, quiet SED) and (spot, facula) pairs, so the
+// two 16-byte reads of a step are each 64 consecutive pairs.
+template <int Q>
+__device__ __forceinline__ void fir_dpp_phase(double (&F)[7], double t, double vq, double vs, double vp, double vf) {
+  // F: 0 = H1a*SEDq, 1 = H1s*Sq, 2 = H2f*Sq, 3 = H1f*Sq, 4 = H1s*Ssp, 5 = H1f*Sfac, 6 = H2f*Sfac;
+  // taps of phase Q at lanes 4Q + (0 H1s, 1 H1f, 2 H2f, 3 H1a) of each row
+  asm("v_fmac_f64_dpp %0, %7, %9 row_newbcast:%[n3] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, %7, %8 row_newbcast:%[n0] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %4, %7, %10 row_newbcast:%[n0] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %2, %7, %8 row_newbcast:%[n2] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %3, %7, %8 row_newbcast:%[n1] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %5, %7, %11 row_newbcast:%[n1] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %6, %7, %11 row_newbcast:%[n2] row_mask:0xf bank_mask:0xf"
+      : "+v"(F[0]), "+v"(F[1]), "+v"(F[2]), "+v"(F[3]), "+v"(F[4]), "+v"(F[5]), "+v"(F[6])
+      : "v"(t), "v"(vq), "v"(vs), "v"(vp), "v"(vf), [n0] "i"(4 * Q), [n1] "i"(4 * Q + 1), [n2] "i"(4 * Q + 2),
+        [n3] "i"(4 * Q + 3));
+}
+
+__device__ __forceinline__ void fir_dpp_entry(double (&F)[kTileG][7], double t, const double2 r0, const double2 r1) {
+  // two wait states between a VALU write of the tap register and its DPP read
+  asm volatile("s_nop 1" ::: "memory");
+  fir_dpp_phase<0>(F[0], t, r0.x, r0.y, r1.x, r1.y);
+  fir_dpp_phase<1>(F[1], t, r0.x, r0.y, r1.x, r1.y);
+  fir_dpp_phase<2>(F[2], t, r0.x, r0.y, r1.x, r1.y);
+  fir_dpp_phase<3>(F[3], t, r0.x, r0.y, r1.x, r1.y);
+}
+
+constexpr int kDppAheadMax = 6;  // entries in flight per wave, at most (the stream is padded to a multiple)
+
+// NW waves per block, each taking phase groups w, w + NW, ...; D entries in flight
+template <int NW, int D>
+__global__ __launch_bounds__(64 * NW) void k_fir_tile_dpp(FirArgs a, const double *__restrict__ ptaps,
+                                                          const int32_t *__restrict__ pxoff,
+                                                          const int32_t *__restrict__ pcount,
+                                                          const int32_t *__restrict__ pufl, double *__restrict__ out,
+                                                          int cap, int n_job, int n_groups, int n_tiles,
+                                                          int tiles_per_xcd) {
+  extern __shared__ double2 srow[];  // plane A [n_mu][lmax] (q, qs), then plane B (sp, fa)
+  constexpr int TP = 64, kDppAhead = D;
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * tiles_per_xcd + (b >> 3);
+  if (tile >= n_tiles) return;  // uniform over the block
+  const int nd = a.nd, n_mu = a.n_mu, lmax = TP + nd - 1;
+  const long n = a.n;
+  const long j0 = (long)tile * TP;
+  const long s0 = j0 - a.dmin - (nd - 1);
+  const int tot = n_mu * lmax;
+  {
+    const GLOBAL double *q = gl(a.q), *qs = gl(a.qs), *sp = gl(a.sp), *fa = gl(a.fa);
+    for (int e = threadIdx.x; e < tot; e += 64 * NW) {
+      const int k = e / lmax, x = e - k * lmax;
+      const int u = pufl[k];
+      if (!u) continue;
+      const long i = s0 + x;
+      const bool in = i >= 0 && i < n;
+      const long o = (long)k * n + i;
+      const double vq = in ? q[o] : 0.0, vs = in ? qs[o] : 0.0;
+      const double vp = (in && (u & 1)) ? sp[o] : 0.0, vf = (in && (u & 2)) ? fa[o] : 0.0;
+      srow[e] = make_double2(vq, vs);
+      srow[tot + e] = make_double2(vp, vf);
+    }
+  }
+  __syncthreads();
+  const int w = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const char *la = (const char *)srow + lane * 16, *lb = la + (size_t)tot * 16;
+  for (int g = w; g < n_groups; g += NW) {
+    const long j = j0 + lane;
+    const double cs = (j < n && a.sps) ? gl(a.sps)[j] : 0.0;
+    const double cf = (j < n && a.fas) ? gl(a.fas)[j] : 0.0;
+    const double *T = ptaps + (long)g * cap * (4 * kTileG) + (lane & 15);
+    // the offsets are uniform, but are loaded per lane through the vector unit: a
+    // scalar load would share the LDS reads' wait counter and, returning out of
+    // order, force a full wait each step
+    int vzero;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(vzero));
+    const int32_t *X = pxoff + (long)g * cap + vzero;
+    const int ne = pcount[g];  // a multiple of kDppAhead
+    double F[kTileG][7];
+#pragma unroll
+    for (int q = 0; q < kTileG; q++)
+#pragma unroll
+      for (int f = 0; f < 7; f++) F[q][f] = 0.0;
+    if (ne > 0) {
+      const int last = ne - 1;
+      double t[kDppAhead];
+      int x[kDppAhead];
+      // slot s holds the taps of entry m + s and the offset of entry m + s + 1 (the
+      // rows are read one entry ahead); rows in two buffers by entry parity.  A slot
+      // is refilled after its last use, so each load writes the register that its
+      // use reads and nothing waits for it before then
+      const int x0 = X[0];
+#pragma unroll
+      for (int s = 0; s < kDppAhead; s++) {
+        t[s] = T[(long)min(s, last) * (4 * kTileG)];
+        x[s] = X[min(s + 1, last)];
+      }
+      double2 r0[2], r1[2];
+      r0[0] = *(const double2 *)(la + x0);
+      r1[0] = *(const double2 *)(lb + x0);
+      for (int m = 0; m < ne; m += kDppAhead) {
+#pragma unroll
+        for (int s = 0; s < kDppAhead; s++) {
+          r0[(s + 1) & 1] = *(const double2 *)(la + x[s]);  // rows of entry m + s + 1
+          r1[(s + 1) & 1] = *(const double2 *)(lb + x[s]);
+          fir_dpp_entry(F, t[s], r0[s & 1], r1[s & 1]);
+          t[s] = T[(long)min(m + s + kDppAhead, last) * (4 * kTileG)];
+          x[s] = X[min(m + s + kDppAhead + 1, last)];
+        }
+      }
+    }
+    if (j < n) {
+      const int p0 = g * kTileG;
+#pragma unroll
+      for (int q = 0; q < kTileG; q++) {
+        if (p0 + q >= n_job) break;
+        double *o = out + (long)(p0 + q) * 3 * n;
+        o[j] = F[q][0] - cs * F[q][1] - cf * F[q][2];
+        o[n + j] = (F[q][1] + F[q][3]) - F[q][4] - F[q][5];
+        o[2 * n + j] = F[q][0] - cs * F[q][4] - cf * F[q][6];
+      }
+    }
+  }
+}
+
+size_t fir_tile_lds(int n_mu, int nd, int t64) { return (size_t)n_mu * (64 * t64 + nd - 1) * 32; }
+
+int fir_tile_groups(int n_job) { return (n_job + kTileG - 1) / kTileG; }
+
+// waves per block for n_groups phase groups: eight (at 100 phases, 25 groups, eight
+// waves ran 1.09 ms against 1.29-1.30 ms for five or six, whose last round leaves no
+// slot idle: the kernel is latency-bound and gains more from resident waves than it
+// loses to idle ones), fewer only when there are fewer groups
+static int tile_waves(int n_groups) { return n_groups >= 8 ? 8 : (n_groups > 4 ? n_groups : 4); }
+
+template <int D>
+static void launch_tile_dpp(int nw, unsigned blocks, size_t lds, hipStream_t s, const FirArgs &a, const FirProg &pg,
+                            int n_job, int n_groups, int nt, int tx) {
+#define SOAP_TILE_DPP(W)                                                                                        \
+  k_fir_tile_dpp<W, D><<<blocks, 64 * W, lds, s>>>(a, pg.taps, pg.xoff, pg.count, pg.ufl, a.out, pg.cap, n_job, \
+                                                   n_groups, nt, tx)
+  switch (nw) {
+    case 4: SOAP_TILE_DPP(4); break;
+    case 5: SOAP_TILE_DPP(5); break;
+    case 6: SOAP_TILE_DPP(6); break;
+    case 7: SOAP_TILE_DPP(7); break;
+    default: SOAP_TILE_DPP(8); break;
+  }
+#undef SOAP_TILE_DPP
+}
+
+hipError_t launch_fir_tile(const FirArgs &a, int n_job, const FirProg &pg, bool dpp, int waves, int ahead,
+                           hipStream_t s) {
+  if (n_job == 0) return hipSuccess;
+  if (a.n_mu > 64 || pg.cap < a.n_mu * a.nd + kDppAheadMax || (waves != 0 && (waves < 4 || waves > 8)) ||
+      (ahead != 4 && ahead != 6))
+    return hipErrorInvalidValue;
+  const int t64 = 1;
+  const size_t lds = fir_tile_lds(a.n_mu, a.nd, t64);
+  // the device's LDS limit per block, read once per device
+  static int lds_max[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return hipErrorInvalidValue;
+  if (!lds_max[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return hipErrorInvalidValue;
+    lds_max[dev] = v;
+  }
+  if (lds > (size_t)lds_max[dev]) return hipErrorInvalidValue;
+  const int TP = 64 * t64, lmax = TP + a.nd - 1;
+  const int n_groups = fir_tile_groups(n_job);
+  const long n_tiles = (a.n + TP - 1) / TP;
+  const long tiles_per_xcd = (n_tiles + 7) / 8;
+  if (8 * tiles_per_xcd >= (1L << 31)) return hipErrorInvalidValue;
+  k_fir_prog<<<n_groups + 1, kProgThreads, 0, s>>>(a, pg, n_job, n_groups, lmax, dpp ? 16 : 32, dpp ? ahead : 2);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const unsigned blocks = (unsigned)(8 * tiles_per_xcd);
+  const int nt = (int)n_tiles, tx = (int)tiles_per_xcd;
+  const int nw = waves ? waves : tile_waves(n_groups);
+  if (dpp && ahead == 6)
+    launch_tile_dpp<6>(nw, blocks, lds, s, a, pg, n_job, n_groups, nt, tx);
+  else if (dpp)
+    launch_tile_dpp<4>(nw, blocks, lds, s, a, pg, n_job, n_groups, nt, tx);
+  else
+    k_fir_tile<1, 8><<<blocks, 512, lds, s>>>(a, pg.taps, pg.xoff, pg.count, pg.ufl, a.out, pg.cap, n_job, n_groups,
+                                             nt, tx);
+  return hipGetLastError();
+}
+
 // Direct form of the phase-group FIR: no LDS staging and no barrier after the
 // group metadata.  Each thread reads the seed-row values of its pixel straight
 // from global memory (a wave's loads at one offset are 64 consecutive doubles;
@@ -1213,20 +1621,19 @@ hipError_t launch_phase_maps(int grid, int t_star, int t_diff_spot, int t_diff_f
 }
 
 // ------------------------------------------------------------ a7: resolution
-// conv_inst (SOAP_resolution.cu:76-95) is linear in the spectrum, so with
-// A = Q - D:  o_i = sum_d c_d (1 - Q_{i+d}) + sum_d c_d D_{i+d}.  The quiet
-// term Qc = sum_d c_d (1 - Q_{i+d}) is computed once per quiet spectrum and is
-// itself the quiet-star frame (A = Q, SOAP_initialize.cu:78-96); every active
-// spectrum then needs one FIR of its delta.  Outputs exist for wid < i < n-1-wid
-// only (zero elsewhere).  Tiles: kConvB outputs per thread (odd, so the per-lane
-// LDS stride is bank-conflict free for 64-bit reads), taps in register chunks of
-// kConvCH (the host pads the tap vector with zeros to a multiple of kConvCH).
+// conv_inst (SOAP_resolution.cu:76-95): o_i = sum_d c_d (1 - A_{i+d}) with A = Q
+// for the quiet star (SOAP_initialize.cu:78-96) and A = Q - D for an active
+// spectrum, as the reference forms it; outputs exist for wid < i < n-1-wid only
+// (zero elsewhere).  Tiles: kConvB outputs per thread (odd, so the per-lane LDS
+// stride is bank-conflict free for 64-bit reads), the window in register slots
+// and the taps in scalar registers, kConvCH at a time (the host pads the tap
+// vector with zeros to a multiple of kConvCH).
 #ifndef SOAP_CONV_B
 #define SOAP_CONV_B 11
 #endif
 constexpr int kConvB = SOAP_CONV_B;  // outputs per thread (odd: conflict-free LDS stride)
 #ifndef SOAP_CONV_CH
-#define SOAP_CONV_CH 9
+#define SOAP_CONV_CH 12
 #endif
 constexpr int kConvCH = SOAP_CONV_CH;
 constexpr int kConvTile = kTPB * kConvB;
@@ -1256,21 +1663,33 @@ __device__ __forceinline__ void conv_stage(double *xs, const GLOBAL double *X, l
   }
 }
 
-// acc[b] = sum_k ctap[k] xs[tid*B + b + k], k in [0, ntp); the taps and the
-// window are both in LDS (the taps are broadcast reads)
-__device__ __forceinline__ void conv_accumulate(const double *xs, const double *ctap, int ntp, double acc[kConvB]) {
+// acc[b] = sum_k c_k xs[tid*B + b + k], k in [0, ntp), in increasing k.  The
+// window x[k .. k+B-1] of tap k lives in kConvCH > B register slots (x[m] in
+// slot m mod kConvCH): after tap k, x[k] is dead and its slot is refilled with
+// x[k + kConvCH], first needed kConvCH - B + 1 taps later, so each tap costs one
+// LDS read for B FMAs.  The taps go through the scalar unit, kConvCH at a time
+// (uniform loads; ntp is a multiple of kConvCH).  The refills run up to kConvCH
+// values past the staged window: the LDS allocation holds that slack.
+__device__ __forceinline__ void conv_accumulate(const double *xs, const GLOBAL double *ctg, int ntp,
+                                                double acc[kConvB]) {
+  static_assert(kConvCH > kConvB, "the window needs more slots than outputs");
 #pragma unroll
   for (int b = 0; b < kConvB; b++) acc[b] = 0.0;
   const double *base = xs + threadIdx.x * kConvB;
-  for (int k0 = 0; k0 < ntp; k0 += kConvCH) {
-    double win[kConvCH + kConvB - 1];
+  double w[kConvCH];
 #pragma unroll
-    for (int e = 0; e < kConvCH + kConvB - 1; e++) win[e] = base[k0 + e];
+  for (int e = 0; e < kConvCH; e++) w[e] = base[e];
+  for (int k0 = 0; k0 < ntp; k0 += kConvCH) {
+    const GLOBAL double *cp = ctg + uni(k0);  // uniform: the taps go through the scalar unit
+    double cv[kConvCH];
+#pragma unroll
+    for (int k = 0; k < kConvCH; k++) cv[k] = cp[k];
+    const double *nx = base + k0 + kConvCH;
 #pragma unroll
     for (int k = 0; k < kConvCH; k++) {
-      double cv = ctap[k0 + k];
 #pragma unroll
-      for (int b = 0; b < kConvB; b++) acc[b] = fma(cv, win[b + k], acc[b]);
+      for (int b = 0; b < kConvB; b++) acc[b] = fma(cv[k], w[(k + b) % kConvCH], acc[b]);
+      w[k] = nx[k];
     }
   }
 }
@@ -1305,7 +1724,8 @@ __device__ __forceinline__ void block_minmax_store(double amin, double omax, GLO
 // written to the frame slots of the lower-spectrum layout [flux, bconv, tot] =
 // [SED, normalised, SED] (run_all_SOAP.py:359-362), with the partial min(Q) and
 // max(o != 0) (seed -60000) of each tile per slot.
-__device__ __forceinline__ void conv_quiet_body(const ConvArgs &c, const double *quiet_, double *frame_, double *part_, int bx_, int by_) {
+__device__ __forceinline__ void conv_quiet_body(const ConvArgs &c, const double *ctap, const double *quiet_, double *frame_,
+                                                double *part_, int bx_, int by_) {
   const GLOBAL double *quiet = gl(quiet_);
   GLOBAL double *frame = gw(frame_), *part = gw(part_);
   extern __shared__ double xs[];
@@ -1313,12 +1733,10 @@ __device__ __forceinline__ void conv_quiet_body(const ConvArgs &c, const double 
   long i0 = (long)bx_ * kConvTile;
   int span = kConvTile + c.ntp - 1;
   const GLOBAL double *Q = quiet + (long)q * c.n;
-  double *ct = xs + span + (span & 1);  // taps, 16-byte aligned
-  for (int e = threadIdx.x; e < c.ntp; e += blockDim.x) ct[e] = gl(c.ctap)[e];
   conv_stage<true>(xs, Q, i0, c.dlo, span, c.n);
   __syncthreads();
   double acc[kConvB];
-  conv_accumulate(xs, ct, c.ntp, acc);
+  conv_accumulate(xs, gl(ctap), c.ntp, acc);
   __syncthreads();
   // results through LDS so the global stores and loads below are coalesced
 #pragma unroll
@@ -1343,8 +1761,11 @@ __device__ __forceinline__ void conv_quiet_body(const ConvArgs &c, const double 
   }
 }
 
-__global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *quiet, double *frame, double *part) {
-  conv_quiet_body(c, quiet, frame, part, blockIdx.x, blockIdx.y);
+// the tap vector is a separate restrict argument: no store of the kernel can reach
+// it, so its uniform loads go through the scalar unit
+__global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *__restrict__ ctap, const double *quiet,
+                                                     double *frame, double *part) {
+  conv_quiet_body(c, ctap, quiet, frame, part, blockIdx.x, blockIdx.y);
 }
 
 // Active spectra of phase p = blockIdx.y, types 0..2 (flux, bconv, tot):
@@ -1359,7 +1780,9 @@ __global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *q
 // lambda tile run back to back on one XCD (block b -> XCD b % 8): the tile's
 // quiet rows stay in that XCD's L2.  Staging issues kStageU loads of Q and D per
 // thread before their first use.
-__device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double *quiet_, const double *raw_, double *o_out_, double *part_, int n_phase, int bx_, int by_) {
+__device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double *ctap, const double *quiet_,
+                                                 const double *raw_, double *o_out_, double *part_, int n_phase,
+                                                 int bx_, int by_) {
   const GLOBAL double *quiet = gl(quiet_), *raw = gl(raw_);
   GLOBAL double *o_out = gw(o_out_), *part = gw(part_);
   extern __shared__ double xs[];
@@ -1369,8 +1792,6 @@ __device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double
   if (tile >= c.n_tiles) return;  // uniform over the block
   long i0 = (long)tile * kConvTile;
   int span = kConvTile + c.ntp - 1;
-  double *ct = xs + span + (span & 1);  // taps, 16-byte aligned
-  for (int e = threadIdx.x; e < c.ntp; e += blockDim.x) ct[e] = gl(c.ctap)[e];
   {
     int sel = t == 1 ? 0 : 1;
     long sp = (long)p * 3 + t;
@@ -1402,7 +1823,7 @@ __device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double
     }
     __syncthreads();
     double acc[kConvB];
-    conv_accumulate(xs, ct, c.ntp, acc);
+    conv_accumulate(xs, gl(ctap), c.ntp, acc);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kConvB; k++) xs[threadIdx.x * kConvB + k] = acc[k];
@@ -1418,9 +1839,9 @@ __device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double
   }
 }
 
-__global__ __launch_bounds__(kTPB) void k_conv_active(ConvArgs c, const double *quiet, const double *raw,
-                                                      double *o_out, double *part, int n_phase) {
-  conv_active_body(c, quiet, raw, o_out, part, n_phase, blockIdx.x, blockIdx.y);
+__global__ __launch_bounds__(kTPB) void k_conv_active(ConvArgs c, const double *__restrict__ ctap, const double *quiet,
+                                                      const double *raw, double *o_out, double *part, int n_phase) {
+  conv_active_body(c, ctap, quiet, raw, o_out, part, n_phase, blockIdx.x, blockIdx.y);
 }
 
 static size_t conv_lds(const ConvArgs &c) { return (size_t)(kConvTile + 2 * c.ntp + 1) * sizeof(double); }
@@ -1430,7 +1851,7 @@ hipError_t launch_conv_quiet(const ConvArgs &c, const double *quiet, double *fra
   size_t lds = conv_lds(c);
   if (lds > 65536) return hipErrorInvalidValue;
   dim3 grid(c.n_tiles, 2);
-  k_conv_quiet<<<grid, kTPB, lds, s>>>(c, quiet, frame, part);
+  k_conv_quiet<<<grid, kTPB, lds, s>>>(c, c.ctap, quiet, frame, part);
   return hipGetLastError();
 }
 
@@ -1442,7 +1863,7 @@ hipError_t launch_conv_active(const ConvArgs &c, const double *quiet, const doub
   long tiles8 = ((long)c.n_tiles + 7) / 8 * 8;
   long blocks = tiles8 * n_phase * 3;
   if (blocks > 0x7fffffffL || lds > 65536) return hipErrorInvalidValue;
-  k_conv_active<<<(unsigned)blocks, kTPB, lds, s>>>(c, quiet, raw, o_out, part, n_phase);
+  k_conv_active<<<(unsigned)blocks, kTPB, lds, s>>>(c, c.ctap, quiet, raw, o_out, part, n_phase);
   return hipGetLastError();
 }
 
@@ -2316,14 +2737,14 @@ template <bool P>
 __global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(6, 8))) void kb_conv_quiet(
     SOAP_BATCH_ARGS(ConvQJob)) {
   batch_loop<P>(jobs, prefix, n_inst, total, [&](const ConvQJob &j, int local) {
-    conv_quiet_body(j.c, j.quiet, j.frame, j.part, local % j.c.n_tiles, local / j.c.n_tiles);
+    conv_quiet_body(j.c, j.c.ctap, j.quiet, j.frame, j.part, local % j.c.n_tiles, local / j.c.n_tiles);
   });
 }
 
 template <bool P>
 __global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(6, 8))) void kb_conv(SOAP_BATCH_ARGS(ConvAJob)) {
   batch_loop<P>(jobs, prefix, n_inst, total, [&](const ConvAJob &j, int local) {
-    conv_active_body(j.c, j.quiet, j.raw, j.out, j.part, j.n_phase, local, 0);
+    conv_active_body(j.c, j.c.ctap, j.quiet, j.raw, j.out, j.part, j.n_phase, local, 0);
   });
 }
 

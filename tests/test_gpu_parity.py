@@ -95,7 +95,7 @@ def test_integrate_parity(case, request):
             assert normwise(g["raw"][p, t], ref) <= SPEC_TOL, (p, t)
 
 
-FIR_SHAPES = [(4, 0), (1, 3), (1, 5), (2, 3), (2, 1), (4, 1)]
+FIR_SHAPES = [(4, 0), (1, 3), (1, 5), (2, 3), (2, 1), (4, 1), (1, -2), (2, -2), (4, -2), (4, -4), (0, -7), (5, -7), (8, -7), (0, -8), (8, -8)]
 
 
 @pytest.mark.parametrize("shape", FIR_SHAPES)
