@@ -368,9 +368,11 @@ const char *soap_batch_stage_name(int32_t stage);
 int32_t soap_batch_num_stages(void);
 /* launch shapes: disk cells per taps block; FIR phases per block (fir_g) and
  * pixels per thread (fir_b, 0 = one-pixel form; (g, b) one of (2,0) (4,0) (6,0)
- * (1,3) (1,5) (2,3)); fir_pre = 1: the one-pixel form prefetches the next bin's
- * seed rows into registers; per stage the persistent block count (0 = one block
- * per work item, else a multiple of 8) and the work items */
+ * (1,3) (1,5) (2,3), or (4,-7) = the tile-stationary form of soap_set_fir_shape,
+ * the default, which adds the work of stage "fir_prog"); fir_pre = 1: the
+ * one-pixel form prefetches the next bin's seed rows into registers; per stage
+ * the persistent block count (0 = one block per work item, else a multiple of
+ * 8) and the work items */
 int soap_batch_get_config(soap_batch *b, int32_t *taps_cells, int32_t *fir_g, int32_t *fir_b, int32_t *fir_pre,
                           int64_t *grid, int64_t *work, int32_t n_stage);
 int soap_batch_set_config(soap_batch *b, int32_t taps_cells, int32_t fir_g, int32_t fir_b, int32_t fir_pre,

@@ -2275,7 +2275,7 @@ int soap_run_series(soap_ctx *c, const soap_star *star, double sigma_ms, double 
 }  // extern "C"
 
 namespace {
-const char *kBatchStageNames[BS_COUNT] = {"fill",  "taps",   "finalize", "fir", "conv_quiet", "conv",
+const char *kBatchStageNames[BS_COUNT] = {"fill", "taps", "finalize", "fir_prog", "fir", "conv_quiet", "conv",
                                           "minmax", "norm", "ccf",      "ccf_reduce", "fit"};
 
 struct BatchStageBuf {
@@ -2298,7 +2298,7 @@ struct soap_batch {
   BatchStageBuf st[BS_COUNT];
   int taps_cells = 4096;  // disk cells per taps block (cells-per-block shape)
   int fir_g = 4;          // phases per FIR block
-  int fir_b = 0;          // pixels per FIR thread (0: one-pixel form)
+  int fir_b = -7;         // pixels per FIR thread (0: one-pixel form; -7: the tile-stationary DPP form)
   int fir_pre = 0;        // one-pixel form: prefetch the next bin's seed rows into registers
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -2362,7 +2362,8 @@ std::vector<uintptr_t> batch_signature(const soap_ctx *c) {
           u(c->kflags.p), u(c->bins.p),  u(c->nbins.p),   u(c->part.p),   u(c->mm.p),     u(c->ctap.p),
           u(c->d_ccf),    u(c->ccf_part.p), u(c->f_par.p), u(c->f_model.p), u(c->f_depth.p), u(c->f_bis.p),
           u(c->f_iter.p), u(c->m_start.p), u(c->m_alive.p), u(c->m_val.p), u(c->d_geo),   u(c->d_box),
-          u(c->d_region), u(c->d_m),     u(c->d_t),       (uintptr_t)c->P, (uintptr_t)c->n, (uintptr_t)c->G,
+          u(c->d_region), u(c->d_m),     u(c->d_t),       u(c->prog_t.p), u(c->prog_x.p), u(c->prog_n.p),
+          u(c->prog_u.p), (uintptr_t)c->P, (uintptr_t)c->n, (uintptr_t)c->G,
           (uintptr_t)c->nd, (uintptr_t)c->res_ntp, (uintptr_t)c->res_wid, (uintptr_t)c->n_lines,
           (uintptr_t)c->n_v, (uintptr_t)c->geo.size(), (uintptr_t)c->act.solar_mode};
 }
@@ -2381,6 +2382,9 @@ int batch_build(soap_batch *b) {
   std::vector<TapsJob> taps(ni);
   std::vector<FinalizeJob> fin(ni);
   std::vector<FirJob> fir(ni);
+  std::vector<FirProgJob> fprog(ni);
+  std::vector<FirTileJob> ftile(ni);
+  const bool tile = b->fir_b == -7;
   std::vector<ConvQJob> cq(ni);
   std::vector<ConvAJob> ca(ni);
   std::vector<MinmaxJob> mm(ni);
@@ -2444,11 +2448,27 @@ int batch_build(soap_batch *b) {
     f.bins = (const int32_t *)c->bins.p;
     f.nbins = (const int32_t *)c->nbins.p;
     f.out = c->d_raw;
-    int fir_tiles = b->fir_b > 0 ? fir2_tiles(c->n, b->fir_b) : (int)((c->n + fir_tile() - 1) / fir_tile());
-    int groups = (P + b->fir_g - 1) / b->fir_g;
-    fir[i] = FirJob{f, P, fir_tiles, groups};
-    w[BS_FIR][i] = (int64_t)(fir_tiles + 7) / 8 * 8 * groups;
-    lds_fir = std::max(lds_fir, b->fir_b > 0 ? fir2_lds(c->nd, b->fir_g, b->fir_b) : fir_lds(c->nd));
+    if (tile) {  // the tile-stationary DPP form: entry streams per phase group, then 64-pixel tiles
+      const int ng = fir_tile_groups(P), cap = c->n_mu * c->nd + 6;
+      int rc;
+      if ((rc = ensure(c, c->prog_t, (size_t)ng * cap * 16 * 8)) || (rc = ensure(c, c->prog_x, (size_t)ng * cap * 4)) ||
+          (rc = ensure(c, c->prog_n, (size_t)ng * 4)) || (rc = ensure(c, c->prog_u, (size_t)c->n_mu * 4)))
+        return bfail(b, rc, "batch: instance %zu: %s", i, c->err.c_str());
+      const FirProg pg{(double *)c->prog_t.p, (int32_t *)c->prog_x.p, (int32_t *)c->prog_n.p, (int32_t *)c->prog_u.p,
+                       cap};
+      const int n_tiles = (int)((c->n + 63) / 64), tpx = (n_tiles + 7) / 8;
+      fprog[i] = FirProgJob{f, pg, P, ng, 64 + c->nd - 1};
+      w[BS_FIR_PROG][i] = ng + 1;
+      ftile[i] = FirTileJob{f, pg, P, ng, n_tiles, tpx};
+      w[BS_FIR][i] = 8L * tpx;
+      lds_fir = std::max(lds_fir, fir_tile_lds(c->n_mu, c->nd, 1));
+    } else {
+      int fir_tiles = b->fir_b > 0 ? fir2_tiles(c->n, b->fir_b) : (int)((c->n + fir_tile() - 1) / fir_tile());
+      int groups = (P + b->fir_g - 1) / b->fir_g;
+      fir[i] = FirJob{f, P, fir_tiles, groups};
+      w[BS_FIR][i] = (int64_t)(fir_tiles + 7) / 8 * 8 * groups;
+      lds_fir = std::max(lds_fir, b->fir_b > 0 ? fir2_lds(c->nd, b->fir_g, b->fir_b) : fir_lds(c->nd));
+    }
     // resolution (log grid)
     ConvArgs cv{};
     cv.n = c->n;
@@ -2489,14 +2509,17 @@ int batch_build(soap_batch *b) {
   for (auto &v : w)
     for (int64_t x : v)
       if (x > 0x7fffffffL) return bfail(b, SOAP_ERR_LIMIT, "batch: an instance has more than 2^31 blocks in a stage");
-  if (lds_fir > 65536 || lds_conv > 65536 || lds_taps > 65536 || ccf_threads > 1024 || red_threads > 1024)
+  if (lds_fir > (tile ? 163840u : 65536u) || lds_conv > 65536 || lds_taps > 65536 || ccf_threads > 1024 ||
+      red_threads > 1024)
     return bfail(b, SOAP_ERR_LIMIT, "batch: an instance exceeds a per-block limit (lds fir %zu conv %zu taps %zu)",
                  lds_fir, lds_conv, lds_taps);
   int rc;
   if ((rc = batch_upload(b, BS_FILL, fill, w[BS_FILL], 256, 0)) ||
       (rc = batch_upload(b, BS_TAPS, taps, w[BS_TAPS], 256, lds_taps)) ||
       (rc = batch_upload(b, BS_FINALIZE, fin, w[BS_FINALIZE], 64, 0)) ||
-      (rc = batch_upload(b, BS_FIR, fir, w[BS_FIR], fir_tile(), lds_fir)) ||
+      (rc = batch_upload(b, BS_FIR_PROG, fprog, w[BS_FIR_PROG], 1024, 0)) ||
+      (rc = tile ? batch_upload(b, BS_FIR, ftile, w[BS_FIR], 512, lds_fir)
+                 : batch_upload(b, BS_FIR, fir, w[BS_FIR], fir_tile(), lds_fir)) ||
       (rc = batch_upload(b, BS_CONV_QUIET, cq, w[BS_CONV_QUIET], 256, lds_conv)) ||
       (rc = batch_upload(b, BS_CONV, ca, w[BS_CONV], 256, lds_conv)) ||
       (rc = batch_upload(b, BS_MINMAX, mm, w[BS_MINMAX], 256, 0)) ||
@@ -2664,7 +2687,7 @@ int soap_batch_get_config(soap_batch *b, int32_t *taps_cells, int32_t *fir_g, in
 }
 
 static bool batch_fir_shape_ok(int g, int bb) {
-  const int ok[][2] = {{2, 0}, {4, 0}, {6, 0}, {1, 3}, {1, 5}, {2, 3}};
+  const int ok[][2] = {{2, 0}, {4, 0}, {6, 0}, {1, 3}, {1, 5}, {2, 3}, {4, -7}};
   for (auto &s : ok)
     if (s[0] == g && s[1] == bb) return true;
   return false;
@@ -2675,8 +2698,8 @@ int soap_batch_set_config(soap_batch *b, int32_t taps_cells, int32_t fir_g, int3
   if (!b || n_stage < BS_COUNT || !grid) return SOAP_ERR_ARG;
   if (taps_cells < 64 || taps_cells > (1 << 20)) return bfail(b, SOAP_ERR_ARG, "set_config: taps_cells out of range");
   if (!batch_fir_shape_ok(fir_g, fir_b))
-    return bfail(b, SOAP_ERR_ARG, "set_config: FIR shape (%d, %d) not one of (2,0) (4,0) (6,0) (1,3) (1,5) (2,3)",
-                 fir_g, fir_b);
+    return bfail(b, SOAP_ERR_ARG,
+                 "set_config: FIR shape (%d, %d) not one of (2,0) (4,0) (6,0) (1,3) (1,5) (2,3) (4,-7)", fir_g, fir_b);
   if (fir_pre != 0 && (fir_pre != 1 || fir_b != 0))
     return bfail(b, SOAP_ERR_ARG, "set_config: fir_pre is 0 or 1, and 1 only with the one-pixel form (fir_b 0)");
   for (int s = 0; s < BS_COUNT; s++)
@@ -2725,14 +2748,15 @@ int soap_batch_tune(soap_batch *b, int32_t reps) {
   b->taps_cells = best_cells;
   // FIR shape: phases per block, pixels per thread (the lambda tile) and, for the
   // one-pixel form, the seed-row prefetch
-  const int g_c[][3] = {{2, 0, 0}, {4, 0, 0}, {6, 0, 0}, {2, 0, 1}, {4, 0, 1}, {6, 0, 1}, {1, 3, 0}, {1, 5, 0}, {2, 3, 0}};
+  const int g_c[][3] = {{2, 0, 0}, {4, 0, 0}, {6, 0, 0}, {2, 0, 1}, {4, 0, 1},
+                        {6, 0, 1}, {1, 3, 0}, {1, 5, 0}, {2, 3, 0}, {4, -7, 0}};
   int best_g = b->fir_g, best_b = b->fir_b, best_pre = b->fir_pre;
   best = HUGE_VAL;
   for (auto &gb : g_c) {
     b->fir_g = gb[0];
     b->fir_b = gb[1];
     b->fir_pre = gb[2];
-    if ((rc = batch_build(b)) || (rc = score(BS_FIR, BS_FIR, &t))) return rc;
+    if ((rc = batch_build(b)) || (rc = score(BS_FIR_PROG, BS_FIR, &t))) return rc;
     if (t < best) best = t, best_g = gb[0], best_b = gb[1], best_pre = gb[2];
   }
   b->fir_g = best_g;

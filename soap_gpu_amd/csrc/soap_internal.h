@@ -212,8 +212,8 @@ struct ConvLinArgs {
 // single-instance kernel would have had.  Counts of the XCD-ordered stages
 // (FIR, active convolution) are multiples of 8, so local % 8 == w % 8.
 enum BatchStage {
-  BS_FILL = 0, BS_TAPS, BS_FINALIZE, BS_FIR, BS_CONV_QUIET, BS_CONV, BS_MINMAX, BS_NORM, BS_CCF, BS_CCF_REDUCE,
-  BS_FIT, BS_COUNT
+  BS_FILL = 0, BS_TAPS, BS_FINALIZE, BS_FIR_PROG, BS_FIR, BS_CONV_QUIET, BS_CONV, BS_MINMAX, BS_NORM, BS_CCF,
+  BS_CCF_REDUCE, BS_FIT, BS_COUNT
 };
 struct FillJob { unsigned long long *p; int64_t count; };           // work: ceil(count / batch_fill_chunk())
 struct TapsJob { TapAccArgs a; int32_t mcb; };                        // work: mcb * P, local = p * mcb + col block
@@ -226,6 +226,8 @@ struct FinalizeJob {                                                  // work: p
   int32_t n_job, p_pad;
 };
 struct FirJob { FirArgs a; int32_t n_job, n_tiles, n_groups; };      // work: tiles8 * n_groups
+struct FirProgJob { FirArgs a; FirProg pg; int32_t n_job, n_groups, lmax; };  // work: n_groups + 1
+struct FirTileJob { FirArgs a; FirProg pg; int32_t n_job, n_groups, n_tiles, tiles_per_xcd; };  // work: 8 tiles_per_xcd
 struct ConvQJob { ConvArgs c; const double *quiet; double *frame, *part; };  // work: 2 n_tiles
 struct ConvAJob { ConvArgs c; const double *quiet, *raw; double *out, *part; int32_t n_phase; };  // tiles8 * P
 struct MinmaxJob { const double *part; int32_t n_tiles; double *mm; };    // work: n_spec
@@ -288,6 +290,7 @@ hipError_t launch_fir_direct(const FirArgs &a, int n_job, int G, hipStream_t s);
 hipError_t launch_fir_tile(const FirArgs &a, int n_job, const FirProg &pg, bool dpp, int waves, int ahead,
                            hipStream_t s);
 size_t fir_tile_lds(int n_mu, int nd, int t64);
+int fir_tile_ahead();  // entries in flight per wave of the batched DPP form (stream padding)
 int fir_tile_groups(int n_job);
 size_t fir2_lds(int nd, int G, int B);
 int fir2_tiles(int64_t n, int B);

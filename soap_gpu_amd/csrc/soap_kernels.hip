@@ -681,9 +681,8 @@ constexpr int kProgThreads = 1024;
 // 16: two planes of row pairs); mult: the stream length is padded with zero entries
 // (reading the last entry's staged pixels) to a multiple of mult, the entries a
 // consuming loop takes per step
-__global__ __launch_bounds__(kProgThreads) void k_fir_prog(FirArgs a, FirProg pg, int n_job, int n_groups, int lmax,
-                                                           int xscale, int mult) {
-  const int g = blockIdx.x;
+__device__ __forceinline__ void fir_prog_body(const FirArgs &a, const FirProg &pg, int n_job, int n_groups, int lmax,
+                                              int xscale, int mult, int g) {
   const int n_mu = a.n_mu, nd = a.nd;
   const GLOBAL int32_t *kflags = gl(a.kflags), *krange = gl(a.krange);
   if (g == n_groups) {  // union over every phase: which rows the tiles stage
@@ -747,6 +746,11 @@ __global__ __launch_bounds__(kProgThreads) void k_fir_prog(FirArgs a, FirProg pg
       for (int t = 0; t < 4; t++) T[(long)e * (4 * kTileG) + q * 4 + t] = p < n_job ? src[t] : 0.0;
     }
   }
+}
+
+__global__ __launch_bounds__(kProgThreads) void k_fir_prog(FirArgs a, FirProg pg, int n_job, int n_groups, int lmax,
+                                                           int xscale, int mult) {
+  fir_prog_body(a, pg, n_job, n_groups, lmax, xscale, mult, blockIdx.x);
 }
 
 __device__ __forceinline__ void lgkm_wait_all() { __builtin_amdgcn_s_waitcnt(0xc07f); }  // lgkmcnt(0)
@@ -908,16 +912,70 @@ __device__ __forceinline__ void fir_dpp_entry(double (&F)[kTileG][7], double t, 
 constexpr int kDppAheadMax = 6;  // entries in flight per wave, at most (the stream is padded to a multiple)
 
 // NW waves per block, each taking phase groups w, w + NW, ...; D entries in flight
+// one phase group's stream over the staged tile: the FMAs of each entry, then the
+// three outputs of each phase for this lane's pixel
+template <int D>
+__device__ __forceinline__ void fir_dpp_group(const FirArgs &a, const GLOBAL double *T, const GLOBAL int32_t *X,
+                                              int ne, const char *la, const char *lb, int p0, int n_job, long j,
+                                              GLOBAL double *out) {
+  constexpr int kDppAhead = D;
+  const long n = a.n;
+  const double cs = (j < n && a.sps) ? gl(a.sps)[j] : 0.0;
+  const double cf = (j < n && a.fas) ? gl(a.fas)[j] : 0.0;
+  double F[kTileG][7];
+#pragma unroll
+  for (int q = 0; q < kTileG; q++)
+#pragma unroll
+    for (int f = 0; f < 7; f++) F[q][f] = 0.0;
+  if (ne > 0) {
+    const int last = ne - 1;
+    double t[kDppAhead];
+    int x[kDppAhead];
+    // slot s holds the taps of entry m + s and the offset of entry m + s + 1 (the
+    // rows are read one entry ahead); rows in two buffers by entry parity.  A slot
+    // is refilled after its last use, so each load writes the register that its
+    // use reads and nothing waits for it before then
+    const int x0 = X[0];
+#pragma unroll
+    for (int s = 0; s < kDppAhead; s++) {
+      t[s] = T[(long)min(s, last) * (4 * kTileG)];
+      x[s] = X[min(s + 1, last)];
+    }
+    double2 r0[2], r1[2];
+    r0[0] = *(const double2 *)(la + x0);
+    r1[0] = *(const double2 *)(lb + x0);
+    for (int m = 0; m < ne; m += kDppAhead) {
+#pragma unroll
+      for (int s = 0; s < kDppAhead; s++) {
+        r0[(s + 1) & 1] = *(const double2 *)(la + x[s]);  // rows of entry m + s + 1
+        r1[(s + 1) & 1] = *(const double2 *)(lb + x[s]);
+        fir_dpp_entry(F, t[s], r0[s & 1], r1[s & 1]);
+        t[s] = T[(long)min(m + s + kDppAhead, last) * (4 * kTileG)];
+        x[s] = X[min(m + s + kDppAhead + 1, last)];
+      }
+    }
+  }
+  if (j < n) {
+#pragma unroll
+    for (int q = 0; q < kTileG; q++) {
+      if (p0 + q >= n_job) break;
+      GLOBAL double *o = out + (long)(p0 + q) * 3 * n;
+      // F: 0 = H1a*SEDq, 1 = H1s*Sq, 2 = H2f*Sq, 3 = H1f*Sq, 4 = H1s*Ssp, 5 = H1f*Sfac, 6 = H2f*Sfac
+      o[j] = F[q][0] - cs * F[q][1] - cf * F[q][2];
+      o[n + j] = (F[q][1] + F[q][3]) - F[q][4] - F[q][5];
+      o[2 * n + j] = F[q][0] - cs * F[q][4] - cf * F[q][6];
+    }
+  }
+}
+
 template <int NW, int D>
-__global__ __launch_bounds__(64 * NW) void k_fir_tile_dpp(FirArgs a, const double *__restrict__ ptaps,
-                                                          const int32_t *__restrict__ pxoff,
-                                                          const int32_t *__restrict__ pcount,
-                                                          const int32_t *__restrict__ pufl, double *__restrict__ out,
-                                                          int cap, int n_job, int n_groups, int n_tiles,
-                                                          int tiles_per_xcd) {
+__device__ __forceinline__ void fir_tile_dpp_body(const FirArgs &a, const double *__restrict__ ptaps,
+                                                  const int32_t *__restrict__ pxoff,
+                                                  const int32_t *__restrict__ pcount,
+                                                  const int32_t *__restrict__ pufl, double *__restrict__ out, int cap,
+                                                  int n_job, int n_groups, int n_tiles, int tiles_per_xcd, int b) {
   extern __shared__ double2 srow[];  // plane A [n_mu][lmax] (q, qs), then plane B (sp, fa)
-  constexpr int TP = 64, kDppAhead = D;
-  const int b = blockIdx.x;
+  constexpr int TP = 64;
   const int tile = (b & 7) * tiles_per_xcd + (b >> 3);
   if (tile >= n_tiles) return;  // uniform over the block
   const int nd = a.nd, n_mu = a.n_mu, lmax = TP + nd - 1;
@@ -929,7 +987,7 @@ __global__ __launch_bounds__(64 * NW) void k_fir_tile_dpp(FirArgs a, const doubl
     const GLOBAL double *q = gl(a.q), *qs = gl(a.qs), *sp = gl(a.sp), *fa = gl(a.fa);
     for (int e = threadIdx.x; e < tot; e += 64 * NW) {
       const int k = e / lmax, x = e - k * lmax;
-      const int u = pufl[k];
+      const int u = gl(pufl)[k];
       if (!u) continue;
       const long i = s0 + x;
       const bool in = i >= 0 && i < n;
@@ -944,65 +1002,31 @@ __global__ __launch_bounds__(64 * NW) void k_fir_tile_dpp(FirArgs a, const doubl
   const int w = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const char *la = (const char *)srow + lane * 16, *lb = la + (size_t)tot * 16;
   for (int g = w; g < n_groups; g += NW) {
-    const long j = j0 + lane;
-    const double cs = (j < n && a.sps) ? gl(a.sps)[j] : 0.0;
-    const double cf = (j < n && a.fas) ? gl(a.fas)[j] : 0.0;
-    const double *T = ptaps + (long)g * cap * (4 * kTileG) + (lane & 15);
+    const GLOBAL double *T = gl(ptaps) + (long)g * cap * (4 * kTileG) + (lane & 15);
     // the offsets are uniform, but are loaded per lane through the vector unit: a
     // scalar load would share the LDS reads' wait counter and, returning out of
     // order, force a full wait each step
     int vzero;
     asm volatile("v_mov_b32 %0, 0" : "=v"(vzero));
-    const int32_t *X = pxoff + (long)g * cap + vzero;
-    const int ne = pcount[g];  // a multiple of kDppAhead
-    double F[kTileG][7];
-#pragma unroll
-    for (int q = 0; q < kTileG; q++)
-#pragma unroll
-      for (int f = 0; f < 7; f++) F[q][f] = 0.0;
-    if (ne > 0) {
-      const int last = ne - 1;
-      double t[kDppAhead];
-      int x[kDppAhead];
-      // slot s holds the taps of entry m + s and the offset of entry m + s + 1 (the
-      // rows are read one entry ahead); rows in two buffers by entry parity.  A slot
-      // is refilled after its last use, so each load writes the register that its
-      // use reads and nothing waits for it before then
-      const int x0 = X[0];
-#pragma unroll
-      for (int s = 0; s < kDppAhead; s++) {
-        t[s] = T[(long)min(s, last) * (4 * kTileG)];
-        x[s] = X[min(s + 1, last)];
-      }
-      double2 r0[2], r1[2];
-      r0[0] = *(const double2 *)(la + x0);
-      r1[0] = *(const double2 *)(lb + x0);
-      for (int m = 0; m < ne; m += kDppAhead) {
-#pragma unroll
-        for (int s = 0; s < kDppAhead; s++) {
-          r0[(s + 1) & 1] = *(const double2 *)(la + x[s]);  // rows of entry m + s + 1
-          r1[(s + 1) & 1] = *(const double2 *)(lb + x[s]);
-          fir_dpp_entry(F, t[s], r0[s & 1], r1[s & 1]);
-          t[s] = T[(long)min(m + s + kDppAhead, last) * (4 * kTileG)];
-          x[s] = X[min(m + s + kDppAhead + 1, last)];
-        }
-      }
-    }
-    if (j < n) {
-      const int p0 = g * kTileG;
-#pragma unroll
-      for (int q = 0; q < kTileG; q++) {
-        if (p0 + q >= n_job) break;
-        double *o = out + (long)(p0 + q) * 3 * n;
-        o[j] = F[q][0] - cs * F[q][1] - cf * F[q][2];
-        o[n + j] = (F[q][1] + F[q][3]) - F[q][4] - F[q][5];
-        o[2 * n + j] = F[q][0] - cs * F[q][4] - cf * F[q][6];
-      }
-    }
+    const GLOBAL int32_t *X = gl(pxoff) + (long)g * cap + vzero;
+    const int ne = gl(pcount)[g];  // a multiple of D
+    fir_dpp_group<D>(a, T, X, ne, la, lb, g * kTileG, n_job, j0 + lane, gw(out));
   }
 }
 
+template <int NW, int D>
+__global__ __launch_bounds__(64 * NW) void k_fir_tile_dpp(FirArgs a, const double *__restrict__ ptaps,
+                                                          const int32_t *__restrict__ pxoff,
+                                                          const int32_t *__restrict__ pcount,
+                                                          const int32_t *__restrict__ pufl, double *__restrict__ out,
+                                                          int cap, int n_job, int n_groups, int n_tiles,
+                                                          int tiles_per_xcd) {
+  fir_tile_dpp_body<NW, D>(a, ptaps, pxoff, pcount, pufl, out, cap, n_job, n_groups, n_tiles, tiles_per_xcd,
+                           blockIdx.x);
+}
+
 size_t fir_tile_lds(int n_mu, int nd, int t64) { return (size_t)n_mu * (64 * t64 + nd - 1) * 32; }
+int fir_tile_ahead() { return 4; }
 
 int fir_tile_groups(int n_job) { return (n_job + kTileG - 1) / kTileG; }
 
@@ -1048,6 +1072,7 @@ hipError_t launch_fir_tile(const FirArgs &a, int n_job, const FirProg &pg, bool 
   if (lds > (size_t)lds_max[dev]) return hipErrorInvalidValue;
   const int TP = 64 * t64, lmax = TP + a.nd - 1;
   const int n_groups = fir_tile_groups(n_job);
+  const int nw = waves ? waves : tile_waves(n_groups);
   const long n_tiles = (a.n + TP - 1) / TP;
   const long tiles_per_xcd = (n_tiles + 7) / 8;
   if (8 * tiles_per_xcd >= (1L << 31)) return hipErrorInvalidValue;
@@ -1056,7 +1081,6 @@ hipError_t launch_fir_tile(const FirArgs &a, int n_job, const FirProg &pg, bool 
   if (e != hipSuccess) return e;
   const unsigned blocks = (unsigned)(8 * tiles_per_xcd);
   const int nt = (int)n_tiles, tx = (int)tiles_per_xcd;
-  const int nw = waves ? waves : tile_waves(n_groups);
   if (dpp && ahead == 6)
     launch_tile_dpp<6>(nw, blocks, lds, s, a, pg, n_job, n_groups, nt, tx);
   else if (dpp)
@@ -1800,25 +1824,50 @@ __device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double
     double amin = HUGE_VAL, omax = -60000.0;
     // kStageU loads of Q and of D in flight per thread before the first use
     // (clamped addresses, so every load is unconditional)
-    for (int e0 = 0; e0 < span; e0 += kStageU * kTPB) {
-      double qv[kStageU], dv[kStageU];
+    const long s0 = i0 + c.dlo;  // source index of span position 0
+    if (s0 >= 0 && s0 + span <= c.n) {
+      // the whole span inside the spectrum (every tile but the first and last):
+      // 32-bit offsets from the span's start, no range selects
+      const GLOBAL double *Qs = Q + s0, *Ds = D + s0;
+      const int t0 = -c.dlo;  // span position of the tile's first pixel
+      for (int e0 = 0; e0 < span; e0 += kStageU * kTPB) {
+        double qv[kStageU], dv[kStageU];
 #pragma unroll
-      for (int u = 0; u < kStageU; u++) {
-        long m = i0 + c.dlo + e0 + threadIdx.x + u * kTPB;
-        long mc = m < 0 ? 0 : (m >= c.n ? c.n - 1 : m);
-        qv[u] = Q[mc];
-        dv[u] = D[mc];
+        for (int u = 0; u < kStageU; u++) {
+          const int e = min(e0 + (int)threadIdx.x + u * kTPB, span - 1);
+          qv[u] = Qs[e];
+          dv[u] = Ds[e];
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
+#pragma unroll
+        for (int u = 0; u < kStageU; u++) {
+          const int e = e0 + threadIdx.x + u * kTPB;
+          const double A = qv[u] - dv[u];
+          amin = (e >= t0 && e < t0 + kConvTile && A < amin) ? A : amin;
+          if (e < span) xs[e] = 1.0 - A;
+        }
       }
-      __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
+    } else {
+      for (int e0 = 0; e0 < span; e0 += kStageU * kTPB) {
+        double qv[kStageU], dv[kStageU];
 #pragma unroll
-      for (int u = 0; u < kStageU; u++) {  // selects, not branches: the loads stay in flight together
-        int e = e0 + threadIdx.x + u * kTPB;
-        long m = i0 + c.dlo + e;
-        const bool in = m >= 0 && m < c.n;
-        const double A = qv[u] - dv[u];
-        const double v = in ? 1.0 - A : 1.0;
-        amin = (in && m >= i0 && m < i0 + kConvTile && A < amin) ? A : amin;
-        if (e < span) xs[e] = v;
+        for (int u = 0; u < kStageU; u++) {
+          long m = s0 + e0 + threadIdx.x + u * kTPB;
+          long mc = m < 0 ? 0 : (m >= c.n ? c.n - 1 : m);
+          qv[u] = Q[mc];
+          dv[u] = D[mc];
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
+#pragma unroll
+        for (int u = 0; u < kStageU; u++) {  // selects, not branches: the loads stay in flight together
+          int e = e0 + threadIdx.x + u * kTPB;
+          long m = s0 + e;
+          const bool in = m >= 0 && m < c.n;
+          const double A = qv[u] - dv[u];
+          const double v = in ? 1.0 - A : 1.0;
+          amin = (in && m >= i0 && m < i0 + kConvTile && A < amin) ? A : amin;
+          if (e < span) xs[e] = v;
+        }
       }
     }
     __syncthreads();
@@ -1828,12 +1877,21 @@ __device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double
 #pragma unroll
     for (int k = 0; k < kConvB; k++) xs[threadIdx.x * kConvB + k] = acc[k];
     __syncthreads();
-    for (int e = threadIdx.x; e < kConvTile; e += blockDim.x) {
-      long i = i0 + e;
-      if (i >= c.n) break;
-      double o = ((i > c.wid) && (i + c.wid < c.n - 1)) ? xs[e] : 0.0;
-      o_out[sp * c.n + i] = o;
-      if (o != 0.0 && omax < o) omax = o;
+    GLOBAL double *ob = o_out + sp * c.n + i0;
+    if (i0 > c.wid && i0 + kConvTile + c.wid < c.n - 1) {  // every output of the tile exists
+      for (int e = threadIdx.x; e < kConvTile; e += blockDim.x) {
+        const double o = xs[e];
+        ob[e] = o;
+        if (o != 0.0 && omax < o) omax = o;
+      }
+    } else {
+      for (int e = threadIdx.x; e < kConvTile; e += blockDim.x) {
+        long i = i0 + e;
+        if (i >= c.n) break;
+        double o = ((i > c.wid) && (i + c.wid < c.n - 1)) ? xs[e] : 0.0;
+        ob[e] = o;
+        if (o != 0.0 && omax < o) omax = o;
+      }
     }
     block_minmax_store(amin, omax, part + (sp * c.n_tiles + tile) * 2);  // ends with a barrier
   }
@@ -2726,6 +2784,24 @@ __global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(FirWaves<P
   });
 }
 
+// the tile-stationary DPP form: entry streams, then the tiles (8 waves, 4 entries
+// ahead; every instance's block count is a multiple of 8, so a block's local index
+// keeps the XCD order of the single-instance launch)
+template <bool P>
+__global__ __launch_bounds__(kProgThreads) void kb_fir_prog(SOAP_BATCH_ARGS(FirProgJob)) {
+  batch_loop<P>(jobs, prefix, n_inst, total, [&](const FirProgJob &j, int local) {
+    fir_prog_body(j.a, j.pg, j.n_job, j.n_groups, j.lmax, 16, 4, local);
+  });
+}
+
+template <bool P>
+__global__ __launch_bounds__(512) void kb_fir_tile(SOAP_BATCH_ARGS(FirTileJob)) {
+  batch_loop<P>(jobs, prefix, n_inst, total, [&](const FirTileJob &j, int local) {
+    fir_tile_dpp_body<8, 4>(j.a, j.pg.taps, j.pg.xoff, j.pg.count, j.pg.ufl, j.a.out, j.pg.cap, j.n_job, j.n_groups,
+                            j.n_tiles, j.tiles_per_xcd, local);
+  });
+}
+
 template <bool P, int G, int B>
 __global__ __launch_bounds__(kTPB) void kb_fir2(SOAP_BATCH_ARGS(FirJob)) {
   batch_loop<P>(jobs, prefix, n_inst, total, [&](const FirJob &j, int local) {
@@ -2795,7 +2871,8 @@ size_t taps_lds_bytes(const TapAccArgs &a) { return (size_t)a.nsets * a.n_mu * a
 hipError_t launch_batch(int st, const void *jobs, const int64_t *prefix, int n_inst, int64_t total, int64_t grid,
                         int threads, size_t lds, int fir_g, int fir_b, int fir_pre, hipStream_t s) {
   if (total == 0) return hipSuccess;
-  if (n_inst < 1 || total < 0 || grid < 0 || lds > 65536) return hipErrorInvalidValue;
+  const bool tile = st == BS_FIR && fir_b == -7;
+  if (n_inst < 1 || total < 0 || grid < 0 || lds > (tile ? 163840u : 65536u)) return hipErrorInvalidValue;
   int64_t g = grid == 0 ? total : std::min<int64_t>(grid, total);
   if (g > 0x7fffffffL) return hipErrorInvalidValue;
   unsigned nb = (unsigned)g;
@@ -2810,6 +2887,7 @@ hipError_t launch_batch(int st, const void *jobs, const int64_t *prefix, int n_i
     SOAP_BATCH_CASE(BS_FILL, kb_fill, FillJob, kTPB)
     SOAP_BATCH_CASE(BS_TAPS, kb_taps, TapsJob, kTPB)
     SOAP_BATCH_CASE(BS_FINALIZE, kb_finalize, FinalizeJob, 64)
+    SOAP_BATCH_CASE(BS_FIR_PROG, kb_fir_prog, FirProgJob, kProgThreads)
     SOAP_BATCH_CASE(BS_CONV_QUIET, kb_conv_quiet, ConvQJob, kTPB)
     SOAP_BATCH_CASE(BS_CONV, kb_conv, ConvAJob, kTPB)
     SOAP_BATCH_CASE(BS_MINMAX, kb_minmax, MinmaxJob, kTPB)
@@ -2819,6 +2897,12 @@ hipError_t launch_batch(int st, const void *jobs, const int64_t *prefix, int n_i
     SOAP_BATCH_CASE(BS_FIT, kb_fit, FitJob, 64)
 #undef SOAP_BATCH_CASE
     case BS_FIR:
+      if (tile) {
+        if (threads != 512) return hipErrorInvalidValue;
+        if (pers) kb_fir_tile<true><<<nb, threads, lds, s>>>((const FirTileJob *)jobs, prefix, n_inst, total);
+        else kb_fir_tile<false><<<nb, threads, lds, s>>>((const FirTileJob *)jobs, prefix, n_inst, total);
+        break;
+      }
       if (threads != kTPB) return hipErrorInvalidValue;
 #define SOAP_FIR_CASE(G)                                                                                   \
   case G:                                                                                                  \

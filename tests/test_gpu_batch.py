@@ -135,7 +135,7 @@ def test_batch_tuned_shapes(batch_case):
     ws, engines, b, refs = batch_case
     b.tune(reps=1)
     cfg = b.config()
-    assert (cfg["fir_g"], cfg["fir_b"]) in ((2, 0), (4, 0), (6, 0), (1, 3), (1, 5), (2, 3))
+    assert (cfg["fir_g"], cfg["fir_b"]) in ((2, 0), (4, 0), (6, 0), (1, 3), (1, 5), (2, 3), (4, -7))
     assert cfg["fir_pre"] in (0, 1) and (cfg["fir_pre"] == 0 or cfg["fir_b"] == 0)
     assert all(g % 8 == 0 and g >= 0 for g in cfg["grid"].values())
     for name in ("taps", "fir", "conv", "ccf", "fit"):
@@ -143,7 +143,8 @@ def test_batch_tuned_shapes(batch_case):
     rng = np.random.default_rng(5)
     shapes = [(1024, 2, 0, 0, 8), (16384, 6, 0, 0, 16), (4096, 4, 0, 0, None), (2048, 1, 3, 0, None),
               (4096, 1, 5, 0, 4), (8192, 2, 3, 0, None), (4096, 4, 0, 1, None), (2048, 2, 0, 1, 8),
-              (4096, 6, 0, 1, 16), (cfg["taps_cells"], cfg["fir_g"], cfg["fir_b"], cfg["fir_pre"], "tuned")]
+              (4096, 6, 0, 1, 16), (4096, 4, -7, 0, None), (2048, 4, -7, 0, 8),
+              (cfg["taps_cells"], cfg["fir_g"], cfg["fir_b"], cfg["fir_pre"], "tuned")]
     for cells, g, bb, pre, mult in shapes:
         if mult == "tuned":
             grid = cfg["grid"]
@@ -157,7 +158,7 @@ def test_batch_tuned_shapes(batch_case):
         b.step(graph=True)
         b.sync()
         got = [_outputs(e, w) for w, e in zip(ws, engines)]
-        if bb == 0:  # the plain path's FIR form: bit for bit
+        if bb <= 0:  # the plain path's one-pixel FIR forms: bit for bit
             for i, r in enumerate(refs):
                 _same(got[i], r, f"shapes {cells}/{g}/{bb}/{pre}/{mult}[{i}]")
         else:  # register-blocked FIR: the same sums in another order
