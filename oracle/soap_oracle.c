@@ -650,7 +650,8 @@ void orc_ccf(int kind, const double *wave_t, long n_t, double step, const double
              const double *line_l, const double *line_w, long n_lines, const double *spec_t,
              double *ccf, int nthreads) {
   const double c = 299792458.;
-  long no_points = (long)(100.0 / step);
+  /* the extended grid exists for the linear form only (the log form's step is h) */
+  long no_points = kind == ORC_GRID_LOG ? 0 : (long)(100.0 / step);
   long n_ext = n_t + 2 * no_points;
   double begin_wave = wave_t[0] - step * no_points;
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
