@@ -2456,12 +2456,15 @@ int batch_build(soap_batch *b) {
         return bfail(b, rc, "batch: instance %zu: %s", i, c->err.c_str());
       const FirProg pg{(double *)c->prog_t.p, (int32_t *)c->prog_x.p, (int32_t *)c->prog_n.p, (int32_t *)c->prog_u.p,
                        cap};
-      const int n_tiles = (int)((c->n + 63) / 64), tpx = (n_tiles + 7) / 8;
-      fprog[i] = FirProgJob{f, pg, P, ng, 64 + c->nd - 1};
+      // wider tiles for few phase groups only while two blocks still fit per CU:
+      // the launch's LDS is the largest instance's for every block
+      const int lg = fir_tile_t64_log2(P, c->n_mu, c->nd, 81920), tp = 64 << lg;
+      const int n_tiles = (int)((c->n + tp - 1) / tp), tpx = (n_tiles + 7) / 8;
+      fprog[i] = FirProgJob{f, pg, P, ng, tp + c->nd - 1};
       w[BS_FIR_PROG][i] = ng + 1;
-      ftile[i] = FirTileJob{f, pg, P, ng, n_tiles, tpx};
+      ftile[i] = FirTileJob{f, pg, P, ng, n_tiles, tpx, lg};
       w[BS_FIR][i] = 8L * tpx;
-      lds_fir = std::max(lds_fir, fir_tile_lds(c->n_mu, c->nd, 1));
+      lds_fir = std::max(lds_fir, fir_tile_lds(c->n_mu, c->nd, 1 << lg));
     } else {
       int fir_tiles = b->fir_b > 0 ? fir2_tiles(c->n, b->fir_b) : (int)((c->n + fir_tile() - 1) / fir_tile());
       int groups = (P + b->fir_g - 1) / b->fir_g;
@@ -2480,8 +2483,11 @@ int batch_build(soap_batch *b) {
     double *part = (double *)c->part.p;
     cq[i] = ConvQJob{cv, c->d_quiet, c->d_lraw + (size_t)3 * P * c->n, part + (size_t)3 * P * cv.n_tiles * 2};
     w[BS_CONV_QUIET][i] = 2L * cv.n_tiles;
-    ca[i] = ConvAJob{cv, c->d_quiet, c->d_raw, c->d_lraw, part, P};
-    w[BS_CONV][i] = (int64_t)(cv.n_tiles + 7) / 8 * 8 * P * 3;  // one block per (tile, phase, type)
+    // one block per (tile, phase, type); XCD groups of 8 tiles only for spectra of
+    // at least 8 tiles (below that, the padding blocks would outnumber the tiles)
+    const int xcd = cv.n_tiles >= 8;
+    ca[i] = ConvAJob{cv, c->d_quiet, c->d_raw, c->d_lraw, part, P, xcd};
+    w[BS_CONV][i] = xcd ? (int64_t)(cv.n_tiles + 7) / 8 * 8 * P * 3 : ((int64_t)cv.n_tiles * P * 3 + 7) / 8 * 8;
     lds_conv = std::max(lds_conv, conv_lds_bytes(cv.ntp));
     mm[i] = MinmaxJob{part, cv.n_tiles, (double *)c->mm.p};
     w[BS_MINMAX][i] = n_spec;

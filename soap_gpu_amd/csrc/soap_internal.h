@@ -227,9 +227,14 @@ struct FinalizeJob {                                                  // work: p
 };
 struct FirJob { FirArgs a; int32_t n_job, n_tiles, n_groups; };      // work: tiles8 * n_groups
 struct FirProgJob { FirArgs a; FirProg pg; int32_t n_job, n_groups, lmax; };  // work: n_groups + 1
-struct FirTileJob { FirArgs a; FirProg pg; int32_t n_job, n_groups, n_tiles, tiles_per_xcd; };  // work: 8 tiles_per_xcd
+struct FirTileJob {  // work: 8 tiles_per_xcd
+  FirArgs a;
+  FirProg pg;
+  int32_t n_job, n_groups, n_tiles, tiles_per_xcd, t64_log2;
+};
 struct ConvQJob { ConvArgs c; const double *quiet; double *frame, *part; };  // work: 2 n_tiles
-struct ConvAJob { ConvArgs c; const double *quiet, *raw; double *out, *part; int32_t n_phase; };  // tiles8 * P
+// work: 3P tiles8 in XCD groups (xcd), else 3P n_tiles
+struct ConvAJob { ConvArgs c; const double *quiet, *raw; double *out, *part; int32_t n_phase, xcd; };
 struct MinmaxJob { const double *part; int32_t n_tiles; double *mm; };    // work: n_spec
 struct NormJob { int64_t n, bx; const double *mm, *in; double *o; };     // work: bx * n_spec
 struct CcfJob {                                                          // work: nc * ceil(n_spec / ccf_spec_chunk())
@@ -292,6 +297,8 @@ hipError_t launch_fir_tile(const FirArgs &a, int n_job, const FirProg &pg, bool 
 size_t fir_tile_lds(int n_mu, int nd, int t64);
 int fir_tile_ahead();  // entries in flight per wave of the batched DPP form (stream padding)
 int fir_tile_groups(int n_job);
+// sub-tiles of 64 pixels per tile (log2) of the DPP form for this shape and LDS limit
+int fir_tile_t64_log2(int n_job, int n_mu, int nd, size_t lds_max);
 size_t fir2_lds(int nd, int G, int B);
 int fir2_tiles(int64_t n, int B);
 // solar mode: taps [job][n_mu][nd][K+3] = H1s, H1f, L_0..L_{K-1}, H1a

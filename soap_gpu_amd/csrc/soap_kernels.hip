@@ -915,13 +915,11 @@ constexpr int kDppAheadMax = 6;  // entries in flight per wave, at most (the str
 // one phase group's stream over the staged tile: the FMAs of each entry, then the
 // three outputs of each phase for this lane's pixel
 template <int D>
-__device__ __forceinline__ void fir_dpp_group(const FirArgs &a, const GLOBAL double *T, const GLOBAL int32_t *X,
-                                              int ne, const char *la, const char *lb, int p0, int n_job, long j,
-                                              GLOBAL double *out) {
+__device__ __forceinline__ void fir_dpp_group(const FirArgs &a, const double *T, const int32_t *X, int cap, int ne,
+                                              const char *la, const char *lb, int p0, int n_job, long j, double cs,
+                                              double cf, GLOBAL double *out) {
   constexpr int kDppAhead = D;
   const long n = a.n;
-  const double cs = (j < n && a.sps) ? gl(a.sps)[j] : 0.0;
-  const double cf = (j < n && a.fas) ? gl(a.fas)[j] : 0.0;
   double F[kTileG][7];
 #pragma unroll
   for (int q = 0; q < kTileG; q++)
@@ -935,23 +933,42 @@ __device__ __forceinline__ void fir_dpp_group(const FirArgs &a, const GLOBAL dou
     // rows are read one entry ahead); rows in two buffers by entry parity.  A slot
     // is refilled after its last use, so each load writes the register that its
     // use reads and nothing waits for it before then
-    const int x0 = X[0];
+    // buffer loads: the stream bases in scalar registers (resources), the entry's
+    // offset a scalar operand and the lane's tap a constant vector offset, so a step
+    // spends no vector instructions on addresses; the offsets are uniform but come
+    // through the vector unit too (a scalar load would share the LDS reads' wait
+    // counter and, returning out of order, force a full wait each step)
+    const __amdgpu_buffer_rsrc_t rt = __builtin_amdgcn_make_buffer_rsrc((void *)T, 0, cap * 8 * (4 * kTileG), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc((void *)X, 0, cap * 4, 0x00020000);
+    const int tl = (threadIdx.x & 15) * 8;
+    auto ldt = [&](int e) {
+      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rt, tl, e * (8 * 4 * kTileG), 0));
+    };
+    auto ldx = [&](int e) { return __builtin_amdgcn_raw_buffer_load_b32(rx, 0, e * 4, 0); };
+    const int x0 = ldx(0);
+    // issued in the loop's own order (taps then offset, slot by slot), so that the
+    // wait counts at the loop head are the same on entry as around the loop
 #pragma unroll
     for (int s = 0; s < kDppAhead; s++) {
-      t[s] = T[(long)min(s, last) * (4 * kTileG)];
-      x[s] = X[min(s + 1, last)];
+      __builtin_amdgcn_sched_barrier(0);
+      t[s] = ldt(min(s, last));
+      x[s] = ldx(min(s + 1, last));
     }
+    __builtin_amdgcn_sched_barrier(0);
     double2 r0[2], r1[2];
     r0[0] = *(const double2 *)(la + x0);
     r1[0] = *(const double2 *)(lb + x0);
     for (int m = 0; m < ne; m += kDppAhead) {
 #pragma unroll
       for (int s = 0; s < kDppAhead; s++) {
+        // each step's instructions stay in their step: hoisting the next step's LDS
+        // address (a just-loaded offset) above these FMAs would wait for that load
+        __builtin_amdgcn_sched_barrier(0);
         r0[(s + 1) & 1] = *(const double2 *)(la + x[s]);  // rows of entry m + s + 1
         r1[(s + 1) & 1] = *(const double2 *)(lb + x[s]);
         fir_dpp_entry(F, t[s], r0[s & 1], r1[s & 1]);
-        t[s] = T[(long)min(m + s + kDppAhead, last) * (4 * kTileG)];
-        x[s] = X[min(m + s + kDppAhead + 1, last)];
+        t[s] = ldt(min(m + s + kDppAhead, last));
+        x[s] = ldx(min(m + s + kDppAhead + 1, last));
       }
     }
   }
@@ -973,9 +990,10 @@ __device__ __forceinline__ void fir_tile_dpp_body(const FirArgs &a, const double
                                                   const int32_t *__restrict__ pxoff,
                                                   const int32_t *__restrict__ pcount,
                                                   const int32_t *__restrict__ pufl, double *__restrict__ out, int cap,
-                                                  int n_job, int n_groups, int n_tiles, int tiles_per_xcd, int b) {
+                                                  int n_job, int n_groups, int n_tiles, int tiles_per_xcd, int t64_log2,
+                                                  int b) {
   extern __shared__ double2 srow[];  // plane A [n_mu][lmax] (q, qs), then plane B (sp, fa)
-  constexpr int TP = 64;
+  const int TP = 64 << t64_log2;    // tile pixels: 64, 128 or 256 (few groups: waves take (sub-tile, group))
   const int tile = (b & 7) * tiles_per_xcd + (b >> 3);
   if (tile >= n_tiles) return;  // uniform over the block
   const int nd = a.nd, n_mu = a.n_mu, lmax = TP + nd - 1;
@@ -984,33 +1002,58 @@ __device__ __forceinline__ void fir_tile_dpp_body(const FirArgs &a, const double
   const long s0 = j0 - a.dmin - (nd - 1);
   const int tot = n_mu * lmax;
   {
+    // kStage elements per thread with all their loads in flight before the first
+    // use: clamped addresses (a row the bin lacks reads the quiet row instead), so
+    // every load is unconditional, then selects
+    constexpr int kStage = 4;
     const GLOBAL double *q = gl(a.q), *qs = gl(a.qs), *sp = gl(a.sp), *fa = gl(a.fa);
-    for (int e = threadIdx.x; e < tot; e += 64 * NW) {
-      const int k = e / lmax, x = e - k * lmax;
-      const int u = gl(pufl)[k];
-      if (!u) continue;
-      const long i = s0 + x;
-      const bool in = i >= 0 && i < n;
-      const long o = (long)k * n + i;
-      const double vq = in ? q[o] : 0.0, vs = in ? qs[o] : 0.0;
-      const double vp = (in && (u & 1)) ? sp[o] : 0.0, vf = (in && (u & 2)) ? fa[o] : 0.0;
-      srow[e] = make_double2(vq, vs);
-      srow[tot + e] = make_double2(vp, vf);
+    for (int e0 = threadIdx.x; e0 < tot; e0 += kStage * 64 * NW) {
+      double vq[kStage], vs[kStage], vp[kStage], vf[kStage];
+      bool in[kStage];
+      int fl[kStage];
+#pragma unroll
+      for (int u = 0; u < kStage; u++) {
+        const int e = min(e0 + u * 64 * NW, tot - 1);
+        const int k = e / lmax, x = e - k * lmax;
+        fl[u] = gl(pufl)[k];
+        const long i = s0 + x;
+        in[u] = fl[u] && i >= 0 && i < n;
+        const long o = (long)k * n + (in[u] ? i : 0);
+        vq[u] = q[o];
+        vs[u] = qs[o];
+        vp[u] = ((fl[u] & 1) ? sp : q)[o];
+        vf[u] = ((fl[u] & 2) ? fa : q)[o];
+      }
+      __builtin_amdgcn_sched_barrier(0);  // every load above the first use
+#pragma unroll
+      for (int u = 0; u < kStage; u++) {
+        const int e = e0 + u * 64 * NW;
+        if (e < tot) {
+          srow[e] = make_double2(in[u] ? vq[u] : 0.0, in[u] ? vs[u] : 0.0);
+          srow[tot + e] = make_double2((in[u] && (fl[u] & 1)) ? vp[u] : 0.0, (in[u] && (fl[u] & 2)) ? vf[u] : 0.0);
+        }
+      }
     }
   }
   __syncthreads();
   const int w = uni(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const char *la = (const char *)srow + lane * 16, *lb = la + (size_t)tot * 16;
-  for (int g = w; g < n_groups; g += NW) {
-    const GLOBAL double *T = gl(ptaps) + (long)g * cap * (4 * kTileG) + (lane & 15);
-    // the offsets are uniform, but are loaded per lane through the vector unit: a
-    // scalar load would share the LDS reads' wait counter and, returning out of
-    // order, force a full wait each step
-    int vzero;
-    asm volatile("v_mov_b32 %0, 0" : "=v"(vzero));
-    const GLOBAL int32_t *X = gl(pxoff) + (long)g * cap + vzero;
+  // items (64-pixel sub-tile, group) in turn over the waves; the spot and facula
+  // continua of the lane's pixel are loaded when the sub-tile changes
+  int cur = -1;
+  double cs = 0.0, cf = 0.0;
+  for (int it = w; it < (n_groups << t64_log2); it += NW) {
+    const int sub = it & ((1 << t64_log2) - 1), g = it >> t64_log2;
+    const int px = sub * 64 + lane;
+    const long j = j0 + px;
+    if (sub != cur) {
+      cs = (j < n && a.sps) ? gl(a.sps)[j] : 0.0;
+      cf = (j < n && a.fas) ? gl(a.fas)[j] : 0.0;
+      cur = sub;
+    }
+    const char *la = (const char *)srow + px * 16, *lb = la + (size_t)tot * 16;
     const int ne = gl(pcount)[g];  // a multiple of D
-    fir_dpp_group<D>(a, T, X, ne, la, lb, g * kTileG, n_job, j0 + lane, gw(out));
+    fir_dpp_group<D>(a, ptaps + (long)g * cap * (4 * kTileG), pxoff + (long)g * cap, cap, ne, la, lb, g * kTileG,
+                     n_job, j, cs, cf, gw(out));
   }
 }
 
@@ -1020,9 +1063,9 @@ __global__ __launch_bounds__(64 * NW) void k_fir_tile_dpp(FirArgs a, const doubl
                                                           const int32_t *__restrict__ pcount,
                                                           const int32_t *__restrict__ pufl, double *__restrict__ out,
                                                           int cap, int n_job, int n_groups, int n_tiles,
-                                                          int tiles_per_xcd) {
+                                                          int tiles_per_xcd, int t64_log2) {
   fir_tile_dpp_body<NW, D>(a, ptaps, pxoff, pcount, pufl, out, cap, n_job, n_groups, n_tiles, tiles_per_xcd,
-                           blockIdx.x);
+                           t64_log2, blockIdx.x);
 }
 
 size_t fir_tile_lds(int n_mu, int nd, int t64) { return (size_t)n_mu * (64 * t64 + nd - 1) * 32; }
@@ -1030,18 +1073,13 @@ int fir_tile_ahead() { return 4; }
 
 int fir_tile_groups(int n_job) { return (n_job + kTileG - 1) / kTileG; }
 
-// waves per block for n_groups phase groups: eight (at 100 phases, 25 groups, eight
-// waves ran 1.09 ms against 1.29-1.30 ms for five or six, whose last round leaves no
-// slot idle: the kernel is latency-bound and gains more from resident waves than it
-// loses to idle ones), fewer only when there are fewer groups
-static int tile_waves(int n_groups) { return n_groups >= 8 ? 8 : (n_groups > 4 ? n_groups : 4); }
 
 template <int D>
 static void launch_tile_dpp(int nw, unsigned blocks, size_t lds, hipStream_t s, const FirArgs &a, const FirProg &pg,
-                            int n_job, int n_groups, int nt, int tx) {
+                            int n_job, int n_groups, int nt, int tx, int t64_log2) {
 #define SOAP_TILE_DPP(W)                                                                                        \
   k_fir_tile_dpp<W, D><<<blocks, 64 * W, lds, s>>>(a, pg.taps, pg.xoff, pg.count, pg.ufl, a.out, pg.cap, n_job, \
-                                                   n_groups, nt, tx)
+                                                   n_groups, nt, tx, t64_log2)
   switch (nw) {
     case 4: SOAP_TILE_DPP(4); break;
     case 5: SOAP_TILE_DPP(5); break;
@@ -1052,27 +1090,42 @@ static void launch_tile_dpp(int nw, unsigned blocks, size_t lds, hipStream_t s, 
 #undef SOAP_TILE_DPP
 }
 
+// the device's LDS limit per block, read once per device
+static size_t lds_limit() {
+  static int lds_max[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+  if (!lds_max[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+    lds_max[dev] = v;
+  }
+  return (size_t)lds_max[dev];
+}
+
+// sub-tiles of 64 pixels per tile (log2) for the DPP form: enough (sub-tile, group)
+// items for eight waves when the phase groups are few, within the LDS limit
+int fir_tile_t64_log2(int n_job, int n_mu, int nd, size_t lds_max) {
+  const int ng = fir_tile_groups(n_job);
+  int lg = 0;
+  while (lg < 2 && (ng << lg) < 8 && fir_tile_lds(n_mu, nd, 2 << lg) <= lds_max) lg++;
+  return lg;
+}
+
 hipError_t launch_fir_tile(const FirArgs &a, int n_job, const FirProg &pg, bool dpp, int waves, int ahead,
                            hipStream_t s) {
   if (n_job == 0) return hipSuccess;
   if (a.n_mu > 64 || pg.cap < a.n_mu * a.nd + kDppAheadMax || (waves != 0 && (waves < 4 || waves > 8)) ||
       (ahead != 4 && ahead != 6))
     return hipErrorInvalidValue;
-  const int t64 = 1;
+  const size_t lds_max = lds_limit();
+  const int lg = dpp ? fir_tile_t64_log2(n_job, a.n_mu, a.nd, lds_max) : 0;
+  const int t64 = 1 << lg;
   const size_t lds = fir_tile_lds(a.n_mu, a.nd, t64);
-  // the device's LDS limit per block, read once per device
-  static int lds_max[64];
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return hipErrorInvalidValue;
-  if (!lds_max[dev]) {
-    int v = 0;
-    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return hipErrorInvalidValue;
-    lds_max[dev] = v;
-  }
-  if (lds > (size_t)lds_max[dev]) return hipErrorInvalidValue;
+  if (lds > lds_max) return hipErrorInvalidValue;
   const int TP = 64 * t64, lmax = TP + a.nd - 1;
   const int n_groups = fir_tile_groups(n_job);
-  const int nw = waves ? waves : tile_waves(n_groups);
+  const int nw = waves ? waves : 8;
   const long n_tiles = (a.n + TP - 1) / TP;
   const long tiles_per_xcd = (n_tiles + 7) / 8;
   if (8 * tiles_per_xcd >= (1L << 31)) return hipErrorInvalidValue;
@@ -1082,9 +1135,9 @@ hipError_t launch_fir_tile(const FirArgs &a, int n_job, const FirProg &pg, bool 
   const unsigned blocks = (unsigned)(8 * tiles_per_xcd);
   const int nt = (int)n_tiles, tx = (int)tiles_per_xcd;
   if (dpp && ahead == 6)
-    launch_tile_dpp<6>(nw, blocks, lds, s, a, pg, n_job, n_groups, nt, tx);
+    launch_tile_dpp<6>(nw, blocks, lds, s, a, pg, n_job, n_groups, nt, tx, lg);
   else if (dpp)
-    launch_tile_dpp<4>(nw, blocks, lds, s, a, pg, n_job, n_groups, nt, tx);
+    launch_tile_dpp<4>(nw, blocks, lds, s, a, pg, n_job, n_groups, nt, tx, lg);
   else
     k_fir_tile<1, 8><<<blocks, 512, lds, s>>>(a, pg.taps, pg.xoff, pg.count, pg.ufl, a.out, pg.cap, n_job, n_groups,
                                              nt, tx);
@@ -1804,15 +1857,28 @@ __global__ __launch_bounds__(kTPB) void k_conv_quiet(ConvArgs c, const double *_
 // lambda tile run back to back on one XCD (block b -> XCD b % 8): the tile's
 // quiet rows stay in that XCD's L2.  Staging issues kStageU loads of Q and D per
 // thread before their first use.
+// xcd: blocks in groups of 8 tiles (tiles padded to a multiple of 8, block b on XCD
+// b % 8 keeps a tile's spectra on one XCD); otherwise block = tile + n_tiles (3 p + t)
+// (a spectrum of fewer than 8 tiles), the count padded to a multiple of 8 only
 __device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double *ctap, const double *quiet_,
                                                  const double *raw_, double *o_out_, double *part_, int n_phase,
-                                                 int bx_, int by_) {
+                                                 int bx_, int by_, bool xcd = true) {
   const GLOBAL double *quiet = gl(quiet_), *raw = gl(raw_);
   GLOBAL double *o_out = gw(o_out_), *part = gw(part_);
   extern __shared__ double xs[];
-  const int slot = bx_ >> 3;
-  const int tile = (slot / (3 * n_phase)) * 8 + (bx_ & 7);
-  const int p = (slot % (3 * n_phase)) / 3, t = slot % 3;
+  int tile, p, t;
+  if (xcd) {
+    const int slot = bx_ >> 3;
+    tile = (slot / (3 * n_phase)) * 8 + (bx_ & 7);
+    p = (slot % (3 * n_phase)) / 3;
+    t = slot % 3;
+  } else {
+    tile = bx_ % c.n_tiles;
+    const int slot = bx_ / c.n_tiles;
+    p = slot / 3;
+    t = slot % 3;
+    if (p >= n_phase) return;  // the count is padded to a multiple of 8 (uniform over the block)
+  }
   if (tile >= c.n_tiles) return;  // uniform over the block
   long i0 = (long)tile * kConvTile;
   int span = kConvTile + c.ntp - 1;
@@ -2334,9 +2400,7 @@ hipError_t launch_mask_wsum(int n_lines, int n_v, int64_t n_t, const int32_t *m_
 // kinds and the norm's streaming runs beside the CCF's gathers.  blockDim.x is the
 // CCF's thread count (velocities rounded up to a wave); a norm block covers
 // kNormPer * blockDim.x pixels of one spectrum.
-__global__ void k_norm_ccf(NormCcfArgs a) {
-  extern __shared__ double ccf_lds[];
-  const long b = blockIdx.x;
+__device__ __forceinline__ void norm_ccf_body(const NormCcfArgs &a, long b, double *ccf_lds) {
   const long grp = (long)a.ratio + 1;
   const long mixed = (long)a.n_ccf * grp;
   long ci = -1, ni;
@@ -2354,6 +2418,11 @@ __global__ void k_norm_ccf(NormCcfArgs a) {
                  a.cap, a.part, (int)(ci % a.nc), (int)(ci / a.nc), ccf_lds);
   else if (ni < a.n_norm)
     norm_body(a.n, a.mm, a.raw, a.out, (int)(ni % a.nbx), (int)(ni / a.nbx), blockDim.x);
+}
+
+__global__ void k_norm_ccf(NormCcfArgs a) {
+  extern __shared__ double ccf_lds[];
+  norm_ccf_body(a, blockIdx.x, ccf_lds);
 }
 
 hipError_t launch_norm_ccf(NormCcfArgs a, hipStream_t s) {
@@ -2798,7 +2867,7 @@ template <bool P>
 __global__ __launch_bounds__(512) void kb_fir_tile(SOAP_BATCH_ARGS(FirTileJob)) {
   batch_loop<P>(jobs, prefix, n_inst, total, [&](const FirTileJob &j, int local) {
     fir_tile_dpp_body<8, 4>(j.a, j.pg.taps, j.pg.xoff, j.pg.count, j.pg.ufl, j.a.out, j.pg.cap, j.n_job, j.n_groups,
-                            j.n_tiles, j.tiles_per_xcd, local);
+                            j.n_tiles, j.tiles_per_xcd, j.t64_log2, local);
   });
 }
 
@@ -2820,7 +2889,7 @@ __global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(6, 8))) vo
 template <bool P>
 __global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(6, 8))) void kb_conv(SOAP_BATCH_ARGS(ConvAJob)) {
   batch_loop<P>(jobs, prefix, n_inst, total, [&](const ConvAJob &j, int local) {
-    conv_active_body(j.c, j.c.ctap, j.quiet, j.raw, j.out, j.part, j.n_phase, local, 0);
+    conv_active_body(j.c, j.c.ctap, j.quiet, j.raw, j.out, j.part, j.n_phase, local, 0, j.xcd != 0);
   });
 }
 

@@ -9,6 +9,11 @@
         `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE --output-format csv`,
         scales each by (known bytes / counter reading) of the copy kernel and
         writes OUT_PREFIX.json (read by bench.py) and OUT_PREFIX.md;
+    python -m soap_gpu_amd.profiling sq OUT.md TITLE --pass COMMAND DIR [--pass COMMAND DIR ...]
+            [--kernels k1,k2]
+        per-dispatch averages of the SQ counters of each pass (`rocprofv3 --pmc
+        SQ_... --output-format csv`) for the named kernels, with the FP64 FMA
+        share of the VALU instructions and the SALU / VALU ratio;
     python -m soap_gpu_amd.profiling stats OUT.md TITLE COMMAND DB [--top N] [--match REGEX]
         kernel statistics (calls, average / min / max microseconds, share of the
         total) from the SQLite output of `rocprofv3 --kernel-trace --stats`,
@@ -63,7 +68,8 @@ def per_kernel(d, counter):
 
 
 def _short(name):
-    for k in ("k_fir_active_grp", "k_conv_active", "k_norm_ccf", "k_norm", "k_ccf_reduce", "k_ccf_lds",
+    for k in ("k_fir_tile_dpp", "k_fir_prog", "k_fir_tile", "k_fir_active_grp", "k_conv_active", "k_norm_ccf", "k_norm",
+              "k_ccf_reduce", "k_ccf_lds",
               "k_fir_quiet_lds", "k_conv_quiet", "k_taps_active", "k_taps_quiet", "k_minmax_reduce", "k_fit",
               "k_mask_build", "k_mask_overwrite", "k_mask_wsum", "k_mask_footprint", "k_disk_init",
               "k_taps_finalize", "k_fill_u64"):
@@ -106,6 +112,42 @@ def summarize(out_prefix, workload, fetch, write, copy_fetch, copy_write):
     print("\n".join(lines))
 
 
+def sq_summary(out_md, title, passes, kernels):
+    """passes: [(command, dir)]; kernels: short names (None: every kernel seen)"""
+    data = {}
+    for _, d in passes:
+        acc = {}
+        for r in _counter_rows(d):
+            key = (_short(r.get("Kernel_Name", "")), r["Counter_Name"])
+            acc.setdefault(key, []).append(float(r["Counter_Value"]))
+        for (k, c), v in acc.items():
+            data.setdefault(k, {})[c] = sum(v) / len(v)
+    names = kernels or sorted(data)
+    counters = sorted({c for k in names for c in data.get(k, {})})
+    lines = [f"# rocprofv3 SQ counters per dispatch: {title}", "", "Commands (repository root on the MI355X box), one pass each:", ""]
+    lines += [f"    {cmd}" for cmd, _ in passes]
+    lines += ["", "| counter | " + " | ".join(f"`{k}`" for k in names) + " |",
+              "|---|" + "---:|" * len(names)]
+    for c in counters:
+        lines.append(f"| {c} | " + " | ".join(f"{data.get(k, {}).get(c, float('nan')):.4g}" for k in names) + " |")
+    derived = []
+    for k in names:
+        d = data.get(k, {})
+        v, f, sa = d.get("SQ_INSTS_VALU"), d.get("SQ_INSTS_VALU_FMA_F64"), d.get("SQ_INSTS_SALU")
+        parts = []
+        if v and f is not None:
+            parts.append(f"FP64 FMA {100.0 * f / v:.0f} % of VALU")
+        if v and sa is not None:
+            parts.append(f"SALU / VALU {sa / v:.2f}")
+        if parts:
+            derived.append(f"- `{k}`: " + ", ".join(parts))
+    if derived:
+        lines += ["", *derived]
+    with open(out_md, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
 def stats(out_md, title, command, db, top=25, match=None):
     import sqlite3
     con = sqlite3.connect(db)
@@ -142,6 +184,15 @@ def main(argv):
         ap.add_argument("--copy-write", required=True)
         a = ap.parse_args(argv[1:])
         summarize(a.out_prefix, a.workload, a.fetch, a.write, a.copy_fetch, a.copy_write)
+    elif argv[0] == "sq":
+        import argparse
+        ap = argparse.ArgumentParser()
+        ap.add_argument("out_md")
+        ap.add_argument("title")
+        ap.add_argument("--pass", dest="passes", nargs=2, action="append", metavar=("COMMAND", "DIR"), required=True)
+        ap.add_argument("--kernels", default=None)
+        a = ap.parse_args(argv[1:])
+        sq_summary(a.out_md, a.title, [tuple(x) for x in a.passes], a.kernels.split(",") if a.kernels else None)
     elif argv[0] == "stats":
         import argparse
         ap = argparse.ArgumentParser()
