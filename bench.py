@@ -378,6 +378,34 @@ def fit_leg(n_inst: int, iters: int, workers: int = 8):
     return out
 
 
+# launches of one series step (soap_run_series with a mask: quiet star on a second
+# stream, active taps + FIR, resolution, spec_norm + CCF in one launch, fit)
+SERIES_LAUNCHES = {"k_disk_init": 1, "k_fill_u64": 2, "k_taps_quiet": 1, "k_taps_finalize": 2,
+                   "k_fir_quiet_lds": 1, "k_taps_active": 1, "k_fir_prog": 1, "k_fir_tile_dpp": 1,
+                   "k_conv_quiet": 1, "k_conv_active": 1, "k_minmax_reduce": 1, "k_norm_ccf": 1, "k_ccf_reduce": 1,
+                   "k_fit": 1}
+
+
+def series_measured_bytes(workload: str):
+    """HBM bytes of one series step from the committed PMC summary (each launch's
+    scaled FETCH_SIZE + WRITE_SIZE per dispatch), or None when the summary lacks a
+    kernel of the step or is for another workload."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            d = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if d.get("workload") != workload:
+        return None
+    kern = d.get("kernels", {})
+    total = 0.0
+    for k, calls in SERIES_LAUNCHES.items():
+        if k not in kern:
+            return None
+        total += calls * kern[k]["traffic_bytes"]
+    return total
+
+
 def load_traffic(kernel: str, workload: str):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC summary
     (FETCH_SIZE and WRITE_SIZE passes, scaled by the copy-kernel calibration)."""
@@ -498,6 +526,12 @@ def main():
     series = {"bytes_per_step": series_bytes, "rows_sum": rows,
               "GBs": round(series_bytes / (ms_step * 1e-3) / 1e9, 1),
               "hbm_frac": round(series_bytes / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+    # the same step measured: the HBM bytes the rocprofv3 counters saw per launch
+    # (profiles/), summed over the step's launches, against the same step time
+    measured = series_measured_bytes(workload)
+    series["measured_bytes_per_step"] = measured
+    series["measured_traffic_frac"] = (round(measured / (ms_step * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)
+                                       if measured else None)
     log(f"[bench] dominant kernel {k['kernel']}: {roof['achieved']} {roof['unit']} ({roof['frac']:.1%} of peak); "
         f"series {series['GBs']} GB/s algorithmic ({series['hbm_frac']:.1%} of HBM peak)")
 

@@ -739,12 +739,19 @@ __device__ __forceinline__ void fir_prog_body(const FirArgs &a, const FirProg &p
     for (int t = 0; t < 4 * kTileG; t++) T[(long)(total + k) * (4 * kTileG) + t] = 0.0;
   }
   for (int di = ulo, e = pos; di <= uhi; di++, e++) {
-    X[e] = (k * lmax + nd - 1 - di) * xscale;
+    int kind = 0;  // bit 0: a spot tap (H1s) of the group is nonzero, bit 1: a facula tap (H1f, H2f)
     for (int q = 0; q < kTileG; q++) {
       const int p = p0 + q;
       const GLOBAL double *src = taps + (((long)p * n_mu + k) * nd + di) * 4;
-      for (int t = 0; t < 4; t++) T[(long)e * (4 * kTileG) + q * 4 + t] = p < n_job ? src[t] : 0.0;
+      for (int t = 0; t < 4; t++) {
+        const double v = p < n_job ? src[t] : 0.0;
+        T[(long)e * (4 * kTileG) + q * 4 + t] = v;
+        if (v != 0.0) kind |= t == 0 ? 1 : (t == 1 || t == 2) ? 2 : 0;
+      }
     }
+    // the DPP form carries the kind in bits 28-29 of the offset word (the
+    // offsets are below 2^28 bytes; the scalar-tap form reads plain offsets)
+    X[e] = (k * lmax + nd - 1 - di) * xscale | (xscale == 16 ? kind << 28 : 0);
   }
 }
 
@@ -884,29 +891,50 @@ __global__ __launch_bounds__(64 * NW) void k_fir_tile(FirArgs a, const double *_
 // (v_fmac_f64 with DPP row_newbcast:n, a single rounding like fma()).  The LDS
 // holds two planes per bin: (quiet, quiet SED) and (spot, facula) pairs, so the
 // two 16-byte reads of a step are each 64 consecutive pairs.
+// Two phases of one entry in one asm block with the entry's kind (bit 0: a spot
+// tap of the group is nonzero, bit 1: a facula tap) in a scalar register: H1a x
+// quiet SED always, the spot products (F1, F4) and the facula products (F2, F3,
+// F5, F6) behind scalar branches on the kind, so an entry of one population skips
+// the other's FMAs (their taps are zero: the skipped products are exact zeros and
+// every sum is the same).  Inside one block the register allocation is untouched.
 template <int Q>
-__device__ __forceinline__ void fir_dpp_phase(double (&F)[7], double t, double vq, double vs, double vp, double vf) {
-  // F: 0 = H1a*SEDq, 1 = H1s*Sq, 2 = H2f*Sq, 3 = H1f*Sq, 4 = H1s*Ssp, 5 = H1f*Sfac, 6 = H2f*Sfac;
-  // taps of phase Q at lanes 4Q + (0 H1s, 1 H1f, 2 H2f, 3 H1a) of each row
-  asm("v_fmac_f64_dpp %0, %7, %9 row_newbcast:%[n3] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %1, %7, %8 row_newbcast:%[n0] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %4, %7, %10 row_newbcast:%[n0] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %2, %7, %8 row_newbcast:%[n2] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %3, %7, %8 row_newbcast:%[n1] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %5, %7, %11 row_newbcast:%[n1] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %6, %7, %11 row_newbcast:%[n2] row_mask:0xf bank_mask:0xf"
-      : "+v"(F[0]), "+v"(F[1]), "+v"(F[2]), "+v"(F[3]), "+v"(F[4]), "+v"(F[5]), "+v"(F[6])
-      : "v"(t), "v"(vq), "v"(vs), "v"(vp), "v"(vf), [n0] "i"(4 * Q), [n1] "i"(4 * Q + 1), [n2] "i"(4 * Q + 2),
-        [n3] "i"(4 * Q + 3));
+__device__ __forceinline__ void fir_dpp_pair(double (&A)[7], double (&B)[7], double t, double vq, double vs,
+                                             double vp, double vf, int kind) {
+  asm("v_fmac_f64_dpp %0, %14, %16 row_newbcast:%[a3] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %7, %14, %16 row_newbcast:%[b3] row_mask:0xf bank_mask:0xf\n\t"
+      "s_bitcmp1_b32 %19, 0\n\t"
+      "s_cbranch_scc0 1f\n\t"
+      "v_fmac_f64_dpp %1, %14, %15 row_newbcast:%[a0] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %4, %14, %17 row_newbcast:%[a0] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %8, %14, %15 row_newbcast:%[b0] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %11, %14, %17 row_newbcast:%[b0] row_mask:0xf bank_mask:0xf\n"
+      "1:\n\t"
+      "s_bitcmp1_b32 %19, 1\n\t"
+      "s_cbranch_scc0 2f\n\t"
+      "v_fmac_f64_dpp %2, %14, %15 row_newbcast:%[a2] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %3, %14, %15 row_newbcast:%[a1] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %5, %14, %18 row_newbcast:%[a1] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %6, %14, %18 row_newbcast:%[a2] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %9, %14, %15 row_newbcast:%[b2] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %10, %14, %15 row_newbcast:%[b1] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %12, %14, %18 row_newbcast:%[b1] row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %13, %14, %18 row_newbcast:%[b2] row_mask:0xf bank_mask:0xf\n"
+      "2:"
+      : "+v"(A[0]), "+v"(A[1]), "+v"(A[2]), "+v"(A[3]), "+v"(A[4]), "+v"(A[5]), "+v"(A[6]), "+v"(B[0]), "+v"(B[1]),
+        "+v"(B[2]), "+v"(B[3]), "+v"(B[4]), "+v"(B[5]), "+v"(B[6])
+      : "v"(t), "v"(vq), "v"(vs), "v"(vp), "v"(vf), "s"(kind), [a0] "i"(4 * Q), [a1] "i"(4 * Q + 1),
+        [a2] "i"(4 * Q + 2), [a3] "i"(4 * Q + 3), [b0] "i"(4 * Q + 4), [b1] "i"(4 * Q + 5), [b2] "i"(4 * Q + 6),
+        [b3] "i"(4 * Q + 7)
+      : "scc");
 }
 
-__device__ __forceinline__ void fir_dpp_entry(double (&F)[kTileG][7], double t, const double2 r0, const double2 r1) {
+// the FMAs of one entry (kind uniform over the wave)
+__device__ __forceinline__ void fir_dpp_entry(double (&F)[kTileG][7], double t, const double2 r0, const double2 r1,
+                                              int kind) {
   // two wait states between a VALU write of the tap register and its DPP read
   asm volatile("s_nop 1" ::: "memory");
-  fir_dpp_phase<0>(F[0], t, r0.x, r0.y, r1.x, r1.y);
-  fir_dpp_phase<1>(F[1], t, r0.x, r0.y, r1.x, r1.y);
-  fir_dpp_phase<2>(F[2], t, r0.x, r0.y, r1.x, r1.y);
-  fir_dpp_phase<3>(F[3], t, r0.x, r0.y, r1.x, r1.y);
+  fir_dpp_pair<0>(F[0], F[1], t, r0.x, r0.y, r1.x, r1.y, kind);
+  fir_dpp_pair<2>(F[2], F[3], t, r0.x, r0.y, r1.x, r1.y, kind);
 }
 
 constexpr int kDppAheadMax = 6;  // entries in flight per wave, at most (the stream is padded to a multiple)
@@ -955,18 +983,23 @@ __device__ __forceinline__ void fir_dpp_group(const FirArgs &a, const double *T,
       x[s] = ldx(min(s + 1, last));
     }
     __builtin_amdgcn_sched_barrier(0);
+    // the offset word: LDS byte offset in bits 0-27, the entry's kind in bits 28-29
+    constexpr int kOff = (1 << 28) - 1;
     double2 r0[2], r1[2];
-    r0[0] = *(const double2 *)(la + x0);
-    r1[0] = *(const double2 *)(lb + x0);
+    r0[0] = *(const double2 *)(la + (x0 & kOff));
+    r1[0] = *(const double2 *)(lb + (x0 & kOff));
+    int kind = uni(x0) >> 28;  // the kind of entry m + s (uniform: every lane loaded the same word)
     for (int m = 0; m < ne; m += kDppAhead) {
 #pragma unroll
       for (int s = 0; s < kDppAhead; s++) {
         // each step's instructions stay in their step: hoisting the next step's LDS
         // address (a just-loaded offset) above these FMAs would wait for that load
         __builtin_amdgcn_sched_barrier(0);
-        r0[(s + 1) & 1] = *(const double2 *)(la + x[s]);  // rows of entry m + s + 1
-        r1[(s + 1) & 1] = *(const double2 *)(lb + x[s]);
-        fir_dpp_entry(F, t[s], r0[s & 1], r1[s & 1]);
+        r0[(s + 1) & 1] = *(const double2 *)(la + (x[s] & kOff));  // rows of entry m + s + 1
+        r1[(s + 1) & 1] = *(const double2 *)(lb + (x[s] & kOff));
+        const int kind_next = uni(x[s]) >> 28;
+        fir_dpp_entry(F, t[s], r0[s & 1], r1[s & 1], kind);
+        kind = kind_next;
         t[s] = ldt(min(m + s + kDppAhead, last));
         x[s] = ldx(min(m + s + kDppAhead + 1, last));
       }
