@@ -702,53 +702,75 @@ __device__ __forceinline__ void fir_prog_body(const FirArgs &a, const FirProg &p
     if (threadIdx.x < n_mu) gw(pg.ufl)[threadIdx.x] = s_u[threadIdx.x];
     return;
   }
-  if (threadIdx.x >= 64) return;  // one wave per group; lane = mu bin
-  const int k = threadIdx.x;
+  // the group's bins: wave 0, lane = mu bin, forms the union ranges and their
+  // prefix (entry positions, bins in order) in LDS; then every thread of the block
+  // builds whole entries (their 16 taps loaded together, then stored)
+  __shared__ int s_pos[65], s_lo[64];
   const int p0 = g * kTileG;
-  int ufl = 0, ulo = nd, uhi = -1;
-  if (k < n_mu)
-    for (int q = 0; q < kTileG; q++) {
-      const int p = p0 + q;
-      if (p >= n_job) break;
-      const long jk = (long)p * n_mu + k;
-      const int fl = kflags[jk], lo = krange[jk * 2], hi = krange[jk * 2 + 1];
-      if (fl && lo <= hi) {
-        ufl |= fl;
-        ulo = min(ulo, lo);
-        uhi = max(uhi, hi);
+  if (threadIdx.x < 64) {
+    const int k = threadIdx.x;
+    int ufl = 0, ulo = nd, uhi = -1;
+    if (k < n_mu)
+      for (int q = 0; q < kTileG; q++) {
+        const int p = p0 + q;
+        if (p >= n_job) break;
+        const long jk = (long)p * n_mu + k;
+        const int fl = kflags[jk], lo = krange[jk * 2], hi = krange[jk * 2 + 1];
+        if (fl && lo <= hi) {
+          ufl |= fl;
+          ulo = min(ulo, lo);
+          uhi = max(uhi, hi);
+        }
       }
+    const int cnt = ufl ? uhi - ulo + 1 : 0;
+    int pos = cnt;  // inclusive prefix over the lanes (bins in order)
+    for (int o = 1; o < 64; o <<= 1) {
+      const int v = __shfl_up(pos, o);
+      if (k >= o) pos += v;
     }
-  const int cnt = ufl ? uhi - ulo + 1 : 0;
-  int pos = cnt;  // inclusive prefix over the lanes (bins in order)
-  for (int o = 1; o < 64; o <<= 1) {
-    const int v = __shfl_up(pos, o);
-    if (k >= o) pos += v;
+    const int total = __shfl(pos, 63);
+    s_pos[k] = pos - cnt;
+    s_lo[k] = ulo;
+    if (k == 63) s_pos[64] = total;
+    // zero entries up to a multiple of mult, reading the last entry's pixels
+    const unsigned long long used = __ballot(cnt > 0);
+    const int klast = used ? 63 - __clzll(used) : 0;
+    const int hlast = __shfl(uhi, klast);
+    const int padded = total ? (total + mult - 1) / mult * mult : 0;
+    GLOBAL double *T = gw(pg.taps) + (long)g * pg.cap * (4 * kTileG);
+    GLOBAL int32_t *X = gw(pg.xoff) + (long)g * pg.cap;
+    if (k == 0) gw(pg.count)[g] = padded;
+    if (k < padded - total) {
+      X[total + k] = (klast * lmax + nd - 1 - hlast) * xscale;
+      for (int t = 0; t < 4 * kTileG; t++) T[(long)(total + k) * (4 * kTileG) + t] = 0.0;
+    }
   }
-  const int total = __shfl(pos, 63);
-  pos -= cnt;
+  __syncthreads();
+  const int total = s_pos[64];
+  const GLOBAL double *taps = gl(a.taps);
   GLOBAL double *T = gw(pg.taps) + (long)g * pg.cap * (4 * kTileG);
   GLOBAL int32_t *X = gw(pg.xoff) + (long)g * pg.cap;
-  const GLOBAL double *taps = gl(a.taps);
-  const unsigned long long used = __ballot(cnt > 0);
-  const int klast = used ? 63 - __clzll(used) : 0;
-  const int hlast = __shfl(uhi, klast);
-  const int padded = total ? (total + mult - 1) / mult * mult : 0;
-  if (k == 0) gw(pg.count)[g] = padded;
-  if (k < padded - total) {  // zero entries
-    X[total + k] = (klast * lmax + nd - 1 - hlast) * xscale;
-    for (int t = 0; t < 4 * kTileG; t++) T[(long)(total + k) * (4 * kTileG) + t] = 0.0;
-  }
-  for (int di = ulo, e = pos; di <= uhi; di++, e++) {
-    int kind = 0;  // bit 0: a spot tap (H1s) of the group is nonzero, bit 1: a facula tap (H1f, H2f)
+  for (int e = threadIdx.x; e < total; e += blockDim.x) {
+    int k = 0;  // the bin of entry e: the last bin whose first entry is <= e
+    for (int step = 32; step > 0; step >>= 1)
+      if (k + step < 64 && s_pos[k + step] <= e) k += step;
+    const int di = s_lo[k] + (e - s_pos[k]);
+    double v[4 * kTileG];
+#pragma unroll
     for (int q = 0; q < kTileG; q++) {
       const int p = p0 + q;
-      const GLOBAL double *src = taps + (((long)p * n_mu + k) * nd + di) * 4;
-      for (int t = 0; t < 4; t++) {
-        const double v = p < n_job ? src[t] : 0.0;
-        T[(long)e * (4 * kTileG) + q * 4 + t] = v;
-        if (v != 0.0) kind |= t == 0 ? 1 : (t == 1 || t == 2) ? 2 : 0;
-      }
+      const GLOBAL double *src = taps + (((long)(p < n_job ? p : 0) * n_mu + k) * nd + di) * 4;
+#pragma unroll
+      for (int t = 0; t < 4; t++) v[q * 4 + t] = p < n_job ? src[t] : 0.0;
     }
+    int kind = 0;  // bit 0: a spot tap (H1s) of the group is nonzero, bit 1: a facula tap (H1f, H2f)
+#pragma unroll
+    for (int q = 0; q < kTileG; q++) {
+      kind |= v[q * 4] != 0.0 ? 1 : 0;
+      kind |= (v[q * 4 + 1] != 0.0 || v[q * 4 + 2] != 0.0) ? 2 : 0;
+    }
+#pragma unroll
+    for (int t = 0; t < 4 * kTileG; t++) T[(long)e * (4 * kTileG) + t] = v[t];
     // the DPP form carries the kind in bits 28-29 of the offset word (the
     // offsets are below 2^28 bytes; the scalar-tap form reads plain offsets)
     X[e] = (k * lmax + nd - 1 - di) * xscale | (xscale == 16 ? kind << 28 : 0);
