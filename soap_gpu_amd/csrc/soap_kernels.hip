@@ -1033,9 +1033,11 @@ __device__ __forceinline__ void fir_dpp_group(const FirArgs &a, const double *T,
       if (p0 + q >= n_job) break;
       GLOBAL double *o = out + (long)(p0 + q) * 3 * n;
       // F: 0 = H1a*SEDq, 1 = H1s*Sq, 2 = H2f*Sq, 3 = H1f*Sq, 4 = H1s*Ssp, 5 = H1f*Sfac, 6 = H2f*Sfac
-      o[j] = F[q][0] - cs * F[q][1] - cf * F[q][2];
-      o[n + j] = (F[q][1] + F[q][3]) - F[q][4] - F[q][5];
-      o[2 * n + j] = F[q][0] - cs * F[q][4] - cf * F[q][6];
+      // The three planes are written once and read back only by the convolution, which runs after
+      // the whole grid: streaming stores keep them from evicting the tap and offset streams.
+      __builtin_nontemporal_store(F[q][0] - cs * F[q][1] - cf * F[q][2], &o[j]);
+      __builtin_nontemporal_store((F[q][1] + F[q][3]) - F[q][4] - F[q][5], &o[n + j]);
+      __builtin_nontemporal_store(F[q][0] - cs * F[q][4] - cf * F[q][6], &o[2 * n + j]);
     }
   }
 }
@@ -2002,7 +2004,7 @@ __device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double
     if (i0 > c.wid && i0 + kConvTile + c.wid < c.n - 1) {  // every output of the tile exists
       for (int e = threadIdx.x; e < kConvTile; e += blockDim.x) {
         const double o = xs[e];
-        ob[e] = o;
+        __builtin_nontemporal_store(o, &ob[e]);  // read back only by the normalisation pass
         if (o != 0.0 && omax < o) omax = o;
       }
     } else {
