@@ -1772,6 +1772,8 @@ constexpr int kConvB = SOAP_CONV_B;  // outputs per thread (odd: conflict-free L
 constexpr int kConvCH = SOAP_CONV_CH;
 constexpr int kConvTile = kTPB * kConvB;
 constexpr int kStageU = 8;  // global loads per thread issued together when staging
+constexpr int kStageA = 6;  // slots per thread and pass of the active convolution's buffer-load staging
+constexpr long kConvBufN = 1L << 27;  // spectra shorter than this stage through buffer loads (8n < 2^31)
 constexpr int kConvMaxTaps = 4095;
 
 // stage x[i0 + dlo + e], e in [0, kConvTile + ntp - 1), into LDS; g(x) = 1 - x for
@@ -1948,7 +1950,35 @@ __device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double
     // kStageU loads of Q and of D in flight per thread before the first use
     // (clamped addresses, so every load is unconditional)
     const long s0 = i0 + c.dlo;  // source index of span position 0
-    if (s0 >= 0 && s0 + span <= c.n) {
+    if (c.n < kConvBufN) {
+      // buffer loads with 32-bit byte offsets from the spectrum's start: an offset
+      // outside [0, 8n) (a negative index wraps above it) reads 0, so A = 0 and the
+      // staged 1 - A is exactly the 1 the reference pads with; every tile takes this
+      // one path, kStageA slots of kTPB per pass (the LDS holds the rounded span)
+      const __amdgpu_buffer_rsrc_t rq = __builtin_amdgcn_make_buffer_rsrc((void *)Q, 0, (int)(c.n * 8), 0x00020000);
+      const __amdgpu_buffer_rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc((void *)D, 0, (int)(c.n * 8), 0x00020000);
+      const int m0 = (int)s0 + (int)threadIdx.x;     // source index of this thread's slot 0
+      const int lo = (int)i0 - m0;                    // m in the tile <=> (unsigned)(m - i0) < tl
+      const unsigned tl = (unsigned)min((long)kConvTile, c.n - i0);
+      const int nu = (span + kTPB - 1) / kTPB;
+      for (int u0 = 0; u0 < nu; u0 += kStageA) {
+        double qv[kStageA], dv[kStageA];
+#pragma unroll
+        for (int u = 0; u < kStageA; u++) {
+          const int off = (m0 + (u0 + u) * kTPB) * 8;
+          qv[u] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rq, off, 0, 0));
+          dv[u] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rd, off, 0, 0));
+        }
+        __builtin_amdgcn_sched_barrier(0);  // keep every load above the first use
+#pragma unroll
+        for (int u = 0; u < kStageA; u++) {
+          const double A = qv[u] - dv[u];
+          const bool in = (unsigned)((u0 + u) * kTPB - lo) < tl;
+          amin = (in && A < amin) ? A : amin;
+          xs[(u0 + u) * kTPB + threadIdx.x] = 1.0 - A;
+        }
+      }
+    } else if (s0 >= 0 && s0 + span <= c.n) {
       // the whole span inside the spectrum (every tile but the first and last):
       // 32-bit offsets from the span's start, no range selects
       const GLOBAL double *Qs = Q + s0, *Ds = D + s0;
@@ -2002,7 +2032,9 @@ __device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double
     __syncthreads();
     GLOBAL double *ob = o_out + sp * c.n + i0;
     if (i0 > c.wid && i0 + kConvTile + c.wid < c.n - 1) {  // every output of the tile exists
-      for (int e = threadIdx.x; e < kConvTile; e += blockDim.x) {
+#pragma unroll
+      for (int k = 0; k < kConvB; k++) {
+        const int e = k * kTPB + threadIdx.x;
         const double o = xs[e];
         __builtin_nontemporal_store(o, &ob[e]);  // read back only by the normalisation pass
         if (o != 0.0 && omax < o) omax = o;
@@ -2025,7 +2057,14 @@ __global__ __launch_bounds__(kTPB) void k_conv_active(ConvArgs c, const double *
   conv_active_body(c, ctap, quiet, raw, o_out, part, n_phase, blockIdx.x, blockIdx.y);
 }
 
-static size_t conv_lds(const ConvArgs &c) { return (size_t)(kConvTile + 2 * c.ntp + 1) * sizeof(double); }
+// the window slack of conv_accumulate (ntp + 1 past the span), or the span rounded up to
+// whole passes of the buffer-load staging, whichever is larger
+static size_t conv_lds_doubles(int ntp) {
+  const long span = kConvTile + ntp - 1;
+  const long pass = (long)kStageA * kTPB;
+  return (size_t)max((long)kConvTile + 2 * ntp + 1, (span + pass - 1) / pass * pass);
+}
+static size_t conv_lds(const ConvArgs &c) { return conv_lds_doubles(c.ntp) * sizeof(double); }
 
 hipError_t launch_conv_quiet(const ConvArgs &c, const double *quiet, double *frame, double *part, hipStream_t s) {
   if (c.ntp % kConvCH || c.ntp > kConvMaxTaps) return hipErrorInvalidValue;
@@ -2991,7 +3030,7 @@ int norm_px_per_block() { return kTPB * kNormPer; }
 int ccf_spec_chunk() { return kCcfSpec; }
 int fir_tile() { return kFirTile; }
 size_t fir_lds(int nd) { return (size_t)4 * (kFirTile + nd - 1) * sizeof(double); }
-size_t conv_lds_bytes(int ntp) { return (size_t)(kConvTile + 2 * ntp + 1) * sizeof(double); }
+size_t conv_lds_bytes(int ntp) { return conv_lds_doubles(ntp) * sizeof(double); }
 size_t taps_lds_bytes(const TapAccArgs &a) { return (size_t)a.nsets * a.n_mu * a.wd * 2 * sizeof(unsigned long long); }
 
 hipError_t launch_batch(int st, const void *jobs, const int64_t *prefix, int n_inst, int64_t total, int64_t grid,
