@@ -35,6 +35,14 @@ template <class T>
 __device__ __forceinline__ GLOBAL T *gw(T *p) {
   return (GLOBAL T *)p;
 }
+// read-only for the whole kernel (the constant address space): uniform loads from it
+// go through the scalar unit even where the pointer came from memory and the kernel
+// stores elsewhere through pointers that could alias it
+#define CONSTANT __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const CONSTANT T *gc(const T *p) {
+  return (const CONSTANT T *)(uintptr_t)p;
+}
 
 // ------------------------------------------------------------------ utilities
 __global__ void k_fill_u64(unsigned long long *p, size_t count) {
@@ -1806,7 +1814,7 @@ __device__ __forceinline__ void conv_stage(double *xs, const GLOBAL double *X, l
 // LDS read for B FMAs.  The taps go through the scalar unit, kConvCH at a time
 // (uniform loads; ntp is a multiple of kConvCH).  The refills run up to kConvCH
 // values past the staged window: the LDS allocation holds that slack.
-__device__ __forceinline__ void conv_accumulate(const double *xs, const GLOBAL double *ctg, int ntp,
+__device__ __forceinline__ void conv_accumulate(const double *xs, const CONSTANT double *ctg, int ntp,
                                                 double acc[kConvB]) {
   static_assert(kConvCH > kConvB, "the window needs more slots than outputs");
 #pragma unroll
@@ -1816,7 +1824,7 @@ __device__ __forceinline__ void conv_accumulate(const double *xs, const GLOBAL d
 #pragma unroll
   for (int e = 0; e < kConvCH; e++) w[e] = base[e];
   for (int k0 = 0; k0 < ntp; k0 += kConvCH) {
-    const GLOBAL double *cp = ctg + uni(k0);  // uniform: the taps go through the scalar unit
+    const CONSTANT double *cp = ctg + uni(k0);  // uniform: the taps go through the scalar unit
     double cv[kConvCH];
 #pragma unroll
     for (int k = 0; k < kConvCH; k++) cv[k] = cp[k];
@@ -1872,7 +1880,7 @@ __device__ __forceinline__ void conv_quiet_body(const ConvArgs &c, const double 
   conv_stage<true>(xs, Q, i0, c.dlo, span, c.n);
   __syncthreads();
   double acc[kConvB];
-  conv_accumulate(xs, gl(ctap), c.ntp, acc);
+  conv_accumulate(xs, gc(ctap), c.ntp, acc);
   __syncthreads();
   // results through LDS so the global stores and loads below are coalesced
 #pragma unroll
@@ -2025,7 +2033,7 @@ __device__ __forceinline__ void conv_active_body(const ConvArgs &c, const double
     }
     __syncthreads();
     double acc[kConvB];
-    conv_accumulate(xs, gl(ctap), c.ntp, acc);
+    conv_accumulate(xs, gc(ctap), c.ntp, acc);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kConvB; k++) xs[threadIdx.x * kConvB + k] = acc[k];
