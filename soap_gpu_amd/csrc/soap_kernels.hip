@@ -2435,8 +2435,20 @@ __device__ __forceinline__ void ccf_reduce_body(const double *part_, int n_chunk
   const double s_min = mm[sp * 2], s_max = mm[sp * 2 + 1];
   double v = -HUGE_VAL;
   for (int iv = threadIdx.x; iv < n_v; iv += blockDim.x) {
+    // chunks added in order; kRedU loads in flight ahead of their adds
+    constexpr int kRedU = 16;
+    const GLOBAL double *pv = part + (long)sp * n_v + iv;
+    const long cs = (long)n_spec * n_v;
     double a = 0.0;
-    for (int c = 0; c < n_chunks; c++) a += part[((long)c * n_spec + sp) * n_v + iv];
+    int c0 = 0;
+    for (; c0 + kRedU <= n_chunks; c0 += kRedU) {
+      double t[kRedU];
+#pragma unroll
+      for (int u = 0; u < kRedU; u++) t[u] = pv[(c0 + u) * cs];
+#pragma unroll
+      for (int u = 0; u < kRedU; u++) a += t[u];
+    }
+    for (; c0 < n_chunks; c0++) a += pv[c0 * cs];
     ccf[(long)sp * n_v + iv] = wsum[iv] - a * (1 - s_min) / s_max;
   }
   __syncthreads();
