@@ -2522,7 +2522,7 @@ int batch_build(soap_batch *b) {
   int rc;
   if ((rc = batch_upload(b, BS_FILL, fill, w[BS_FILL], 256, 0)) ||
       (rc = batch_upload(b, BS_TAPS, taps, w[BS_TAPS], 256, lds_taps)) ||
-      (rc = batch_upload(b, BS_FINALIZE, fin, w[BS_FINALIZE], 64, 0)) ||
+      (rc = batch_upload(b, BS_FINALIZE, fin, w[BS_FINALIZE], 256, 0)) ||
       (rc = batch_upload(b, BS_FIR_PROG, fprog, w[BS_FIR_PROG], 1024, 0)) ||
       (rc = tile ? batch_upload(b, BS_FIR, ftile, w[BS_FIR], 512, lds_fir)
                  : batch_upload(b, BS_FIR, fir, w[BS_FIR], fir_tile(), lds_fir)) ||

@@ -316,40 +316,48 @@ __device__ __forceinline__ void taps_finalize_body(const unsigned long long *acc
         for (int st = 0; st < nsets_out; st++) gw(taps_g_)[(e * p_pad + job) * nsets_out + st] = 0.0;
     return;  // uniform over the block
   }
+  // one thread per (bin, offset): the nonzero range and the populations of a bin
+  // gathered with LDS atomics (min, max and or do not depend on the order)
   __shared__ int s_lo[256], s_hi[256], s_fl[256];
   for (int k = threadIdx.x; k < n_mu; k += blockDim.x) {
-    int lo = nd, hi = -1, fl = 0;
-    for (int d = 0; d < nd; d++) {
-      bool nz = false;
-      double v01 = 0.0;
-      GLOBAL double *out = taps + (((long)job * n_mu + k) * nd + d) * nsets_out;
-      GLOBAL double *outg = taps_g_ ? gw(taps_g_) + (((long)k * nd + d) * p_pad + job) * nsets_out : nullptr;
-      for (int st = 0; st < nsets_in; st++) {
-        long e = ((((long)job * nsets_in + st) * n_mu + k) * nd + d) * 2;
-        double v = fx_value(acc[e], acc[e + 1]);
-        out[st] = v;
-        if (outg) outg[st] = v;
-        if (st < 2) v01 += v;
-        if (v != 0.0) {
-          nz = true;
-          fl |= (st == 0 || nsets_in == 1) ? 1 : 2;
-        }
-      }
-      if (nsets_out > nsets_in) {
-        out[nsets_in] = v01;
-        if (outg) outg[nsets_in] = v01;
-      }
-      if (nz) {
-        lo = d < lo ? d : lo;
-        hi = d;
+    s_lo[k] = nd;
+    s_hi[k] = -1;
+    s_fl[k] = 0;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < n_mu * nd; e += blockDim.x) {
+    const int k = e / nd, d = e - k * nd;
+    bool nz = false;
+    int fl = 0;
+    double v01 = 0.0;
+    GLOBAL double *out = taps + (((long)job * n_mu + k) * nd + d) * nsets_out;
+    GLOBAL double *outg = taps_g_ ? gw(taps_g_) + (((long)k * nd + d) * p_pad + job) * nsets_out : nullptr;
+    for (int st = 0; st < nsets_in; st++) {
+      long ea = ((((long)job * nsets_in + st) * n_mu + k) * nd + d) * 2;
+      double v = fx_value(acc[ea], acc[ea + 1]);
+      out[st] = v;
+      if (outg) outg[st] = v;
+      if (st < 2) v01 += v;
+      if (v != 0.0) {
+        nz = true;
+        fl |= (st == 0 || nsets_in == 1) ? 1 : 2;
       }
     }
-    s_lo[k] = lo;
-    s_hi[k] = hi;
-    s_fl[k] = fl;
-    krange[((long)job * n_mu + k) * 2] = lo;
-    krange[((long)job * n_mu + k) * 2 + 1] = hi;
-    kflags[(long)job * n_mu + k] = fl;
+    if (nsets_out > nsets_in) {
+      out[nsets_in] = v01;
+      if (outg) outg[nsets_in] = v01;
+    }
+    if (nz) {
+      atomicMin(&s_lo[k], d);
+      atomicMax(&s_hi[k], d);
+      atomicOr(&s_fl[k], fl);
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < n_mu; k += blockDim.x) {
+    krange[((long)job * n_mu + k) * 2] = s_lo[k];
+    krange[((long)job * n_mu + k) * 2 + 1] = s_hi[k];
+    kflags[(long)job * n_mu + k] = s_fl[k];
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -371,7 +379,7 @@ hipError_t launch_taps_finalize(const unsigned long long *acc, int n_job, int ns
                                 int nd, double *taps, int32_t *krange, int32_t *kflags, int32_t *bins,
                                 int32_t *nbins, double *taps_g, int p_pad, hipStream_t s) {
   if (n_mu > 256 || (taps_g && p_pad < n_job + 8)) return hipErrorInvalidValue;
-  k_taps_finalize<<<taps_g ? p_pad : n_job, 64, 0, s>>>(acc, nsets_in, nsets_out, n_mu, nd, taps, krange, kflags,
+  k_taps_finalize<<<taps_g ? p_pad : n_job, 256, 0, s>>>(acc, nsets_in, nsets_out, n_mu, nd, taps, krange, kflags,
                                                         bins, nbins, taps_g, n_job, p_pad);
   return hipGetLastError();
 }
@@ -2946,7 +2954,7 @@ __global__ __launch_bounds__(kTPB) void kb_taps(SOAP_BATCH_ARGS(TapsJob)) {
 }
 
 template <bool P>
-__global__ __launch_bounds__(64) void kb_finalize(SOAP_BATCH_ARGS(FinalizeJob)) {
+__global__ __launch_bounds__(256) void kb_finalize(SOAP_BATCH_ARGS(FinalizeJob)) {
   batch_loop<P>(jobs, prefix, n_inst, total, [&](const FinalizeJob &j, int local) {
     taps_finalize_body(j.acc, j.nsets_in, j.nsets_out, j.n_mu, j.nd, j.taps, j.krange, j.kflags, j.bins, j.nbins,
                        j.taps_g, j.n_job, j.p_pad, local, 0);
@@ -3071,7 +3079,7 @@ hipError_t launch_batch(int st, const void *jobs, const int64_t *prefix, int n_i
     break;
     SOAP_BATCH_CASE(BS_FILL, kb_fill, FillJob, kTPB)
     SOAP_BATCH_CASE(BS_TAPS, kb_taps, TapsJob, kTPB)
-    SOAP_BATCH_CASE(BS_FINALIZE, kb_finalize, FinalizeJob, 64)
+    SOAP_BATCH_CASE(BS_FINALIZE, kb_finalize, FinalizeJob, kTPB)
     SOAP_BATCH_CASE(BS_FIR_PROG, kb_fir_prog, FirProgJob, kProgThreads)
     SOAP_BATCH_CASE(BS_CONV_QUIET, kb_conv_quiet, ConvQJob, kTPB)
     SOAP_BATCH_CASE(BS_CONV, kb_conv, ConvAJob, kTPB)
