@@ -1392,7 +1392,41 @@ static int log_conv_taps(soap_ctx *c, double sigma_ms, int wid) {
 // The convolution writes d_lraw and the tile min / max; spec_norm (d_lraw ->
 // d_lower) runs here when norm is set, else in the same launch as the CCF (the
 // series, ccf_impl).
-static int resolution_impl(soap_ctx *c, double sigma_ms, double quiet_weight, bool norm) {
+// log grids: the convolution's taps (host, cached per sigma), its arguments and the
+// output buffers of every spectrum
+static int log_conv_setup(soap_ctx *c, double sigma_ms, int wid, ConvArgs &ca) {
+  int rc;
+  if ((rc = log_conv_taps(c, sigma_ms, wid))) return rc;
+  ca = ConvArgs{};
+  ca.n = c->n;
+  ca.wid = wid;
+  ca.dlo = c->res_dlo;
+  ca.ntp = c->res_ntp;
+  ca.ctap = (const double *)c->ctap.p;
+  ca.n_tiles = (int)((c->n + conv_tile() - 1) / conv_tile());
+  int n_spec = 3 * (c->P + 1);
+  if ((rc = dalloc(c, &c->d_lower, (size_t)n_spec * c->n)) || (rc = dalloc(c, &c->d_lraw, (size_t)n_spec * c->n)) ||
+      (rc = ensure(c, c->part, (size_t)n_spec * ca.n_tiles * 2 * 8)) || (rc = ensure(c, c->mm, (size_t)n_spec * 2 * 8)))
+    return rc;
+  return SOAP_OK;
+}
+
+// the quiet star's convolution alone, on the ctx stream (the series runs it on the
+// quiet star's stream, right after the quiet star); a log grid with the quiet star set
+static int resolution_quiet(soap_ctx *c, double sigma_ms) {
+  if (!(sigma_ms > 0)) return fail(c, SOAP_ERR_ARG, "resolution: sigma must be > 0");
+  int wid = conv_window(c->wave, sigma_ms);
+  if (wid < 1 || 2 * (int64_t)wid + 2 >= c->n)
+    return fail(c, SOAP_ERR_ARG, "resolution: window %d vs %lld pixels", wid, (long long)c->n);
+  ConvArgs ca{};
+  int rc;
+  if ((rc = log_conv_setup(c, sigma_ms, wid, ca))) return rc;
+  HIPCHK(c, launch_conv_quiet(ca, c->d_quiet, c->d_lraw + (size_t)3 * c->P * c->n,
+                              (double *)c->part.p + (size_t)3 * c->P * ca.n_tiles * 2, c->stream));
+  return SOAP_OK;
+}
+
+static int resolution_impl(soap_ctx *c, double sigma_ms, double quiet_weight, bool norm, bool quiet_conv = true) {
   if (!c) return SOAP_ERR_ARG;
   if (!c->have_quiet) return fail(c, SOAP_ERR_STATE, "resolution: quiet star missing");
   if (c->P > 0 && !c->have_raw) return fail(c, SOAP_ERR_STATE, "resolution: raw spectra missing");
@@ -1438,20 +1472,11 @@ static int resolution_impl(soap_ctx *c, double sigma_ms, double quiet_weight, bo
     c->have_ccf = c->have_fit = false;
     return SOAP_OK;
   }
-  if ((rc = log_conv_taps(c, sigma_ms, wid))) return rc;
   ConvArgs ca{};
-  ca.n = c->n;
-  ca.wid = wid;
-  ca.dlo = c->res_dlo;
-  ca.ntp = c->res_ntp;
-  ca.ctap = (const double *)c->ctap.p;
-  ca.n_tiles = (int)((c->n + conv_tile() - 1) / conv_tile());
+  if ((rc = log_conv_setup(c, sigma_ms, wid, ca))) return rc;
   int n_spec = 3 * (c->P + 1);
-  if ((rc = dalloc(c, &c->d_lower, (size_t)n_spec * c->n)) || (rc = dalloc(c, &c->d_lraw, (size_t)n_spec * c->n)) ||
-      (rc = ensure(c, c->part, (size_t)n_spec * ca.n_tiles * 2 * 8)) || (rc = ensure(c, c->mm, (size_t)n_spec * 2 * 8)))
-    return rc;
   double *part = (double *)c->part.p;
-  {
+  if (quiet_conv) {
     StageTimer tm(c, ST_CONV_QUIET);
     HIPCHK(c, launch_conv_quiet(ca, c->d_quiet, c->d_lraw + (size_t)3 * c->P * c->n,
                                 part + (size_t)3 * c->P * ca.n_tiles * 2, c->stream));
@@ -2245,6 +2270,7 @@ int soap_run_series(soap_ctx *c, const soap_star *star, double sigma_ms, double 
     hipStream_t main_stream = c->stream;
     c->stream = c->aux;
     rc = soap_quiet_star(c);
+    if (!rc) rc = resolution_quiet(c, sigma_ms);  // its convolution too, off the active path
     c->stream = main_stream;
     hipError_t e = hipEventRecord(c->ev_join, c->aux);  // recorded even after a failure, so the wait below ends
     if (rc) {
@@ -2263,7 +2289,7 @@ int soap_run_series(soap_ctx *c, const soap_star *star, double sigma_ms, double 
   // spec_norm shares its launch with the CCF (stage timing then reports the pair
   // under the CCF stage, as it runs)
   const bool fuse = c->have_mask;
-  if ((rc = resolution_impl(c, sigma_ms, quiet_weight, !fuse))) return rc;
+  if ((rc = resolution_impl(c, sigma_ms, quiet_weight, !fuse, !fork))) return rc;
   if (c->have_mask) {
     if ((rc = ccf_impl(c, fuse))) return rc;
     if ((rc = soap_fit(c))) return rc;
