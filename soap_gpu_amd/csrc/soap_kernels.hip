@@ -1732,7 +1732,96 @@ __global__ __launch_bounds__(kTPB) void k_fir_quiet_lds(FirArgs a) {
   }
 }
 
+// The quiet star's FIR with the seed rows of the next bin in flight: a block of
+// kQuietPx * 256 pixels keeps two LDS buffers of the rows of one bin; while it
+// sums bin kk from one, the rows of bin kk + 1 go from registers into the other
+// and the loads of bin kk + 2 are issued, so one bin's load latency hides behind
+// another's FMAs (k_fir_quiet_lds waited on each bin's loads).  The sums per pixel
+// are the same fmas in the same order (bins in list order, offsets ascending).
+constexpr int kQuietPx = 2;
+constexpr int kQuietTile = kTPB * kQuietPx;
+constexpr int kQuietSlots = 3;  // row values per thread: rows of up to 3 * 256 pixels (nd <= 257)
+__global__ __launch_bounds__(kTPB) void k_fir_quiet_pipe(FirArgs a) {
+  extern __shared__ double rows[];  // [2 buffers][q row, qs row] of row_len each
+  const long n = a.n;
+  const long j0 = (long)blockIdx.x * kQuietTile;
+  const int row_len = kQuietTile + a.nd - 1;
+  const long src0 = j0 - a.dmin - a.nd + 1;
+  const CONSTANT int32_t *bins = gc(a.bins), *krange = gc(a.krange);
+  const CONSTANT double *taps = gc(a.taps);
+  const GLOBAL double *q = gl(a.q), *qs = gl(a.qs);
+  const int nb = gc(a.nbins)[0];
+  double rq[kQuietSlots], rs[kQuietSlots];
+  auto load = [&](int kk) {  // bin kk's rows into registers, 0 outside the spectrum
+    const long kb = (long)bins[kk] * n;
+#pragma unroll
+    for (int u = 0; u < kQuietSlots; u++) {
+      const int e = threadIdx.x + u * kTPB;
+      const long i = src0 + e;
+      const bool in = e < row_len && i >= 0 && i < n;
+      const long ic = in ? i : 0;
+      const double vq = q[kb + ic], vs = qs[kb + ic];
+      rq[u] = in ? vq : 0.0;
+      rs[u] = in ? vs : 0.0;
+    }
+  };
+  auto put = [&](int buf) {
+    double *R = rows + (long)buf * 2 * row_len;
+#pragma unroll
+    for (int u = 0; u < kQuietSlots; u++) {
+      const int e = threadIdx.x + u * kTPB;
+      if (e < row_len) {
+        R[e] = rq[u];
+        R[row_len + e] = rs[u];
+      }
+    }
+  };
+  double accq[kQuietPx], accs[kQuietPx];
+#pragma unroll
+  for (int p = 0; p < kQuietPx; p++) accq[p] = accs[p] = 0.0;
+  if (nb > 0) {
+    load(0);
+    put(0);
+  }
+  __syncthreads();
+  if (nb > 1) load(1);
+  for (int kk = 0; kk < nb; kk++) {
+    const int k = bins[kk];
+    const int lo = krange[k * 2], hi = krange[k * 2 + 1];
+    const CONSTANT double *T = taps + (long)k * a.nd;
+    const double *R = rows + (long)(kk & 1) * 2 * row_len;
+#pragma unroll 4
+    for (int di = lo; di <= hi; di++) {
+      const double hv = T[di];
+#pragma unroll
+      for (int p = 0; p < kQuietPx; p++) {
+        const int e = threadIdx.x + p * kTPB + a.nd - 1 - di;
+        accq[p] = fma(hv, R[e], accq[p]);
+        accs[p] = fma(hv, R[row_len + e], accs[p]);
+      }
+    }
+    if (kk + 1 < nb) put((kk + 1) & 1);  // that buffer was last read in iteration kk - 1
+    if (kk + 2 < nb) load(kk + 2);
+    __syncthreads();
+  }
+  GLOBAL double *out = gw(a.out);
+#pragma unroll
+  for (int p = 0; p < kQuietPx; p++) {
+    const long j = j0 + threadIdx.x + p * kTPB;
+    if (j < n) {
+      out[j] = accq[p];
+      out[n + j] = accs[p];
+    }
+  }
+}
+
 hipError_t launch_fir_quiet_lds(const FirArgs &a, hipStream_t s) {
+  const long row_pipe = kQuietTile + a.nd - 1;
+  if (row_pipe <= (long)kQuietSlots * kTPB) {
+    k_fir_quiet_pipe<<<(unsigned)((a.n + kQuietTile - 1) / kQuietTile), kTPB, (size_t)4 * row_pipe * sizeof(double),
+                       s>>>(a);
+    return hipGetLastError();
+  }
   size_t lds = (size_t)2 * (kTPB + a.nd - 1) * sizeof(double);
   if (lds > 65536) return launch_fir_quiet(a, s);
   k_fir_quiet_lds<<<(unsigned)((a.n + kTPB - 1) / kTPB), kTPB, lds, s>>>(a);
