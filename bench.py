@@ -381,7 +381,7 @@ def fit_leg(n_inst: int, iters: int, workers: int = 8):
 # launches of one series step (soap_run_series with a mask: quiet star on a second
 # stream, active taps + FIR, resolution, spec_norm + CCF in one launch, fit)
 SERIES_LAUNCHES = {"k_disk_init": 1, "k_fill_u64": 2, "k_taps_quiet": 1, "k_taps_finalize": 2,
-                   "k_fir_quiet_lds": 1, "k_taps_active": 1, "k_fir_prog": 1, "k_fir_tile_dpp": 1,
+                   "k_fir_quiet_pipe": 1, "k_taps_active": 1, "k_fir_prog": 1, "k_fir_tile_dpp": 1,
                    "k_conv_quiet": 1, "k_conv_active": 1, "k_minmax_reduce": 1, "k_norm_ccf": 1, "k_ccf_reduce": 1,
                    "k_fit": 1}
 

@@ -70,7 +70,7 @@ def per_kernel(d, counter):
 def _short(name):
     for k in ("k_fir_tile_dpp", "k_fir_prog", "k_fir_tile", "k_fir_active_grp", "k_conv_active", "k_norm_ccf", "k_norm",
               "k_ccf_reduce", "k_ccf_lds",
-              "k_fir_quiet_lds", "k_conv_quiet", "k_taps_active", "k_taps_quiet", "k_minmax_reduce", "k_fit",
+              "k_fir_quiet_pipe", "k_fir_quiet_lds", "k_conv_quiet", "k_taps_active", "k_taps_quiet", "k_minmax_reduce", "k_fit",
               "k_mask_build", "k_mask_overwrite", "k_mask_wsum", "k_mask_footprint", "k_disk_init",
               "k_taps_finalize", "k_fill_u64"):
         if (name.endswith(k) or re.search(r"soap::" + k + r"[(<]", name)
