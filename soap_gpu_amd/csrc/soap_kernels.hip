@@ -2835,6 +2835,7 @@ __device__ __forceinline__ void fit_body(const double *v_, int n, const double *
   for (int a = 0; a < 4; a++) D[a] = sqrt(ev.colsq[a]) > 0 ? sqrt(ev.colsq[a]) : 1.0;
   double mu = 1.0e-3, nu = 2.0;
   int iter = 0, fit_status = 11;
+  GaussEval et;
   for (iter = 0; iter < 10000; iter++) {
     double dx[4], xt[4];
     int bad = 0;
@@ -2847,8 +2848,9 @@ __device__ __forceinline__ void fit_body(const double *v_, int n, const double *
         continue;
       }
       for (int a = 0; a < 4; a++) xt[a] = x[a] + dx[a];
-      GaussEval et;
-      gauss_eval(s_px, s_y, s_err, n, xt, false, et);
+      // the trial point with its Jacobian sums: an accepted trial's evaluation is
+      // the next iteration's (the same sums at the same point)
+      gauss_eval(s_px, s_y, s_err, n, xt, true, et);
       double pred = 0;
       for (int a = 0; a < 4; a++) {
         double Adx = 0;
@@ -2873,7 +2875,7 @@ __device__ __forceinline__ void fit_body(const double *v_, int n, const double *
       break;
     }
     for (int a = 0; a < 4; a++) x[a] = xt[a];
-    gauss_eval(s_px, s_y, s_err, n, x, true, ev);
+    ev = et;
     for (int a = 0; a < 4; a++)
       if (sqrt(ev.colsq[a]) > D[a]) D[a] = sqrt(ev.colsq[a]);
     bool small = true;
