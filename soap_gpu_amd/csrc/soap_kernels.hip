@@ -929,50 +929,63 @@ __global__ __launch_bounds__(64 * NW) void k_fir_tile(FirArgs a, const double *_
 // (v_fmac_f64 with DPP row_newbcast:n, a single rounding like fma()).  The LDS
 // holds two planes per bin: (quiet, quiet SED) and (spot, facula) pairs, so the
 // two 16-byte reads of a step are each 64 consecutive pairs.
-// Two phases of one entry in one asm block with the entry's kind (bit 0: a spot
-// tap of the group is nonzero, bit 1: a facula tap) in a scalar register: H1a x
-// quiet SED always, the spot products (F1, F4) and the facula products (F2, F3,
-// F5, F6) behind scalar branches on the kind, so an entry of one population skips
-// the other's FMAs (their taps are zero: the skipped products are exact zeros and
-// every sum is the same).  Inside one block the register allocation is untouched.
-template <int Q>
-__device__ __forceinline__ void fir_dpp_pair(double (&A)[7], double (&B)[7], double t, double vq, double vs,
-                                             double vp, double vf, int kind) {
-  asm("v_fmac_f64_dpp %0, %14, %16 row_newbcast:%[a3] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %7, %14, %16 row_newbcast:%[b3] row_mask:0xf bank_mask:0xf\n\t"
-      "s_bitcmp1_b32 %19, 0\n\t"
-      "s_cbranch_scc0 1f\n\t"
-      "v_fmac_f64_dpp %1, %14, %15 row_newbcast:%[a0] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %4, %14, %17 row_newbcast:%[a0] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %8, %14, %15 row_newbcast:%[b0] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %11, %14, %17 row_newbcast:%[b0] row_mask:0xf bank_mask:0xf\n"
-      "1:\n\t"
-      "s_bitcmp1_b32 %19, 1\n\t"
-      "s_cbranch_scc0 2f\n\t"
-      "v_fmac_f64_dpp %2, %14, %15 row_newbcast:%[a2] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %3, %14, %15 row_newbcast:%[a1] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %5, %14, %18 row_newbcast:%[a1] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %6, %14, %18 row_newbcast:%[a2] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %9, %14, %15 row_newbcast:%[b2] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %10, %14, %15 row_newbcast:%[b1] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %12, %14, %18 row_newbcast:%[b1] row_mask:0xf bank_mask:0xf\n\t"
-      "v_fmac_f64_dpp %13, %14, %18 row_newbcast:%[b2] row_mask:0xf bank_mask:0xf\n"
-      "2:"
-      : "+v"(A[0]), "+v"(A[1]), "+v"(A[2]), "+v"(A[3]), "+v"(A[4]), "+v"(A[5]), "+v"(A[6]), "+v"(B[0]), "+v"(B[1]),
-        "+v"(B[2]), "+v"(B[3]), "+v"(B[4]), "+v"(B[5]), "+v"(B[6])
-      : "v"(t), "v"(vq), "v"(vs), "v"(vp), "v"(vf), "s"(kind), [a0] "i"(4 * Q), [a1] "i"(4 * Q + 1),
-        [a2] "i"(4 * Q + 2), [a3] "i"(4 * Q + 3), [b0] "i"(4 * Q + 4), [b1] "i"(4 * Q + 5), [b2] "i"(4 * Q + 6),
-        [b3] "i"(4 * Q + 7)
-      : "scc");
-}
-
-// the FMAs of one entry (kind uniform over the wave)
+// The FMAs of one entry with the entry's kind (bit 0: a spot tap of the group is
+// nonzero, bit 1: a facula tap) in a scalar register, the four phases together:
+// H1a x quiet SED (F0) always, the spot products (F1, F4) behind one scalar branch
+// and the facula products (F2, F3, F5, F6) behind another, so an entry of one
+// population skips the other's FMAs (their taps are zero: the skipped products are
+// exact zeros and every sum is the same).  Tap lanes: phase q's (H1s, H1f, H2f, H1a)
+// in lanes 4q .. 4q + 3 of each 16-lane row.  Each accumulator takes one FMA per
+// entry, so the sums are the same as with the FMAs in any other order within it.
 __device__ __forceinline__ void fir_dpp_entry(double (&F)[kTileG][7], double t, const double2 r0, const double2 r1,
                                               int kind) {
+  static_assert(kTileG == 4, "four phases per entry");
   // two wait states between a VALU write of the tap register and its DPP read
   asm volatile("s_nop 1" ::: "memory");
-  fir_dpp_pair<0>(F[0], F[1], t, r0.x, r0.y, r1.x, r1.y, kind);
-  fir_dpp_pair<2>(F[2], F[3], t, r0.x, r0.y, r1.x, r1.y, kind);
+  asm("v_fmac_f64_dpp %0, %12, %14 row_newbcast:3 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %3, %12, %14 row_newbcast:7 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %6, %12, %14 row_newbcast:11 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %9, %12, %14 row_newbcast:15 row_mask:0xf bank_mask:0xf\n\t"
+      "s_bitcmp1_b32 %16, 0\n\t"
+      "s_cbranch_scc0 1f\n\t"
+      "v_fmac_f64_dpp %1, %12, %13 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %2, %12, %15 row_newbcast:0 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %4, %12, %13 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %5, %12, %15 row_newbcast:4 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %7, %12, %13 row_newbcast:8 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %8, %12, %15 row_newbcast:8 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %10, %12, %13 row_newbcast:12 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %11, %12, %15 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+      "1:"
+      : "+v"(F[0][0]), "+v"(F[0][1]), "+v"(F[0][4]), "+v"(F[1][0]), "+v"(F[1][1]), "+v"(F[1][4]), "+v"(F[2][0]),
+        "+v"(F[2][1]), "+v"(F[2][4]), "+v"(F[3][0]), "+v"(F[3][1]), "+v"(F[3][4])
+      : "v"(t), "v"(r0.x), "v"(r0.y), "v"(r1.x), "s"(kind)
+      : "scc");
+  asm("s_nop 1\n\t"  // the tap register may have been copied by a VALU move since the first block
+      "s_bitcmp1_b32 %19, 1\n\t"
+      "s_cbranch_scc0 2f\n\t"
+      "v_fmac_f64_dpp %0, %16, %17 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %1, %16, %17 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %2, %16, %18 row_newbcast:1 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %3, %16, %18 row_newbcast:2 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %4, %16, %17 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %5, %16, %17 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %6, %16, %18 row_newbcast:5 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %7, %16, %18 row_newbcast:6 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %8, %16, %17 row_newbcast:10 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %9, %16, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %10, %16, %18 row_newbcast:9 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %11, %16, %18 row_newbcast:10 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %12, %16, %17 row_newbcast:14 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %13, %16, %17 row_newbcast:13 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %14, %16, %18 row_newbcast:13 row_mask:0xf bank_mask:0xf\n\t"
+      "v_fmac_f64_dpp %15, %16, %18 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+      "2:"
+      : "+v"(F[0][2]), "+v"(F[0][3]), "+v"(F[0][5]), "+v"(F[0][6]), "+v"(F[1][2]), "+v"(F[1][3]), "+v"(F[1][5]),
+        "+v"(F[1][6]), "+v"(F[2][2]), "+v"(F[2][3]), "+v"(F[2][5]), "+v"(F[2][6]), "+v"(F[3][2]), "+v"(F[3][3]),
+        "+v"(F[3][5]), "+v"(F[3][6])
+      : "v"(t), "v"(r0.x), "v"(r1.y), "s"(kind)
+      : "scc");
 }
 
 constexpr int kDppAheadMax = 6;  // entries in flight per wave, at most (the stream is padded to a multiple)
