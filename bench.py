@@ -15,7 +15,9 @@ Further legs in the same line: "adjoint" (the differentiable model on a grid
 cost) and "batch" (256 uneven instances stepped by one persistent launch per
 stage, eager and as a HIP graph, at autotuned launch shapes, against the plain
 per-instance path) and "fit" (L-BFGS over 128 differentiable instances with
-256x256 filling-factor maps: time per forward + adjoint evaluation).
+256x256 filling-factor maps: time per forward + adjoint evaluation) and
+"linear" (the series on the reference's native linear 0.005 A seed format:
+production disk and activity, 100,000 pixels, 25 phases).
 
 Prints one JSON line.
 """
@@ -386,6 +388,74 @@ SERIES_LAUNCHES = {"k_disk_init": 1, "k_fill_u64": 2, "k_taps_quiet": 1, "k_taps
                    "k_fit": 1}
 
 
+def _shift_groups(keys, order):
+    """Runs of equal keys along order (the engine's cell-list order): count of runs."""
+    if order.size == 0:
+        return 0
+    new = np.zeros(order.size, dtype=bool)
+    new[0] = True
+    for k in keys:
+        v = k[order]
+        new[1:] |= v[1:] != v[:-1]
+    return int(new.sum())
+
+
+def linear_leg(steps: int, n_phase: int = 25):
+    """The series on the reference's native linear 0.005 A seed format
+    (SOAP_integration.cu:246-281): the production disk and activity, 100,000 pixels
+    over 5000-5500 A, 25 phases.  Reports the step time, the stage split, and how
+    many cells and shift groups (runs of equal shift per mu bin, one disk column
+    each under solid-body rotation) the gathers walk."""
+    from soap_gpu_amd import synth
+    from soap_gpu_amd.engine import Engine, setup_workload
+    t0 = time.time()
+    w = synth.linear_series(n_phase=n_phase)
+    eng = Engine(0)
+    setup_workload(eng, w)
+    log(f"[bench:linear] workload built in {time.time() - t0:.1f} s: n={w.n} phases={len(w.phases)}")
+    eng.run_series(w.star, w.sigma_inst)
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.run_series(w.star, w.sigma_inst)
+    eng.sync()
+    dt = time.perf_counter() - t0
+    ms_step = dt / steps * 1e3
+    eng.set_timing(True)
+    for _ in range(steps):
+        eng.run_series(w.star, w.sigma_inst)
+    st = eng.stage_ms()
+    eng.set_timing(False)
+    per = {k: round(v / steps, 4) for k, v in st.items()}
+    log(f"[bench:linear] {ms_step:.3f} ms/step; stages " + ", ".join(f"{k}={v:.3f}" for k, v in per.items()))
+    # cells and shift groups, in the engine's list order (mu bin, then map order)
+    maps = eng.get_maps()
+    region, delta = maps["region"], maps["delta"]
+    disk = np.nonzero(~((maps["limb"] == -999.0) & (delta == -999.0)))[0]
+    qord = disk[np.argsort(region[disk], kind="stable")]
+    q_cells, q_groups = int(qord.size), _shift_groups((region, delta), qord)
+    a_cells = a_groups = 0
+    for p in range(len(w.phases)):
+        temp, bi = eng.phase_maps(p)
+        act = np.nonzero(bi != 0)[0]
+        aord = act[np.argsort(region[act], kind="stable")]
+        a_cells += int(aord.size)
+        a_groups += _shift_groups((region, delta, temp), aord)
+    eng.close()
+    fa = per.get("fir_active", 0.0) * 1e-3
+    return {"workload": (f"linear 0.005 A grid {w.wave[0]:.0f}-{w.wave[-1]:.0f} A ({w.n} px), grid 1000x1000, "
+                         f"20 mu bins, 2 spots + 2 faculae overlapping, {len(w.phases)} phases, R=115000, "
+                         f"{len(w.mask_l)}-line mask, {len(w.vgrid)} velocities"),
+            "n_lambda": w.n, "phases": len(w.phases), "steps": steps,
+            "ms_per_step": round(ms_step, 3), "value": round(len(w.phases) / (ms_step * 1e-3), 1),
+            "unit": "phase-spectra/s", "stage_ms": per,
+            "quiet_cells": q_cells, "quiet_groups": q_groups,
+            "active_cells_sum": a_cells, "active_groups_sum": a_groups,
+            "active_gathers_per_s": round(a_groups * w.n / fa, 1) if fa > 0 else None,
+            "active_cell_pixels_per_s": round(a_cells * w.n / fa, 1) if fa > 0 else None,
+            "data": "synthetic, seeded (linear_series seed 1239)"}
+
+
 def series_measured_bytes(workload: str):
     """HBM bytes of one series step from the committed PMC summary (each launch's
     scaled FETCH_SIZE + WRITE_SIZE per dispatch), or None when the summary lacks a
@@ -432,7 +502,7 @@ def main():
     ap.add_argument("--cpu-frac-quiet", type=float, default=0.08, help="pixel fraction of the quiet-star CPU sample")
     ap.add_argument("--batch-instances", type=int, default=256, help="instances of the batched leg (0 skips it)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost leg")
-    ap.add_argument("--legs", default="series,adjoint,batch,fit", help="comma list of legs to run")
+    ap.add_argument("--legs", default="series,adjoint,batch,fit,linear", help="comma list of legs to run")
     ap.add_argument("--fit-instances", type=int, default=128)
     ap.add_argument("--fit-iters", type=int, default=50)
     args = ap.parse_args()
@@ -455,6 +525,8 @@ def main():
             out["batch"] = batch_leg(args.batch_instances, max(2, min(args.steps, 5)), 1)
         if "fit" in legs:
             out["fit"] = fit_leg(args.fit_instances, args.fit_iters)
+        if "linear" in legs:
+            out["linear"] = linear_leg(max(2, min(args.steps, 5)))
         print(json.dumps(out), flush=True)
         return
 
@@ -578,6 +650,12 @@ def main():
         except Exception as ex:
             out["fit"] = {"error": f"{type(ex).__name__}: {ex}"}
             log(f"[bench:fit] failed: {ex}")
+    if "linear" in legs:
+        try:
+            out["linear"] = linear_leg(max(2, min(args.steps, 5)))
+        except Exception as ex:
+            out["linear"] = {"error": f"{type(ex).__name__}: {ex}"}
+            log(f"[bench:linear] failed: {ex}")
     # the host-core oracle last, so that its threads do not share the host with the GPU legs
     if not args.no_cpu_baseline:
         nt = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

@@ -108,9 +108,11 @@ struct soap_ctx {
     DevBuf mt_ptr, mt_w, mt_col, mt_val;  // the mask in sliced pixel-major form (adjoint of the CCF)
     bool mt_ready = false;
   } diff;
-  // linear grids: staged source range per tile, bin-sorted disk cells, active cell lists
+  // linear grids: staged source range per tile, bin-sorted disk cells, active cell
+  // lists, and their shift groups with per-segment group counts
   int lin_off_lo = 0, lin_span = 0;
   DevBuf qcells, qbin_off, acells, abin_off, alist_off;
+  DevBuf qgroups, qg_n, agroups, ag_n;
   // solar mode: Lagrange nodes over the facula temperature range
   int solar_k = 0, t_norm_fac = 0;
   double solar_t[kMaxSolarNodes] = {}, solar_w[kMaxSolarNodes] = {};
@@ -772,11 +774,20 @@ int soap_quiet_star(soap_ctx *c) {
   HIPCHK(c, hipSetDevice(c->device));
   if ((rc = dalloc(c, &c->d_quiet, (size_t)2 * c->n))) return rc;
   if (c->kind == SOAP_GRID_LINEAR) {
+    size_t g2 = (size_t)c->G * c->G;
+    if ((rc = ensure(c, c->qgroups, g2 * sizeof(LinGroup))) || (rc = ensure(c, c->qg_n, (size_t)c->n_mu * 4)))
+      return rc;
+    {
+      StageTimer tm(c, ST_TAPS_QUIET);
+      HIPCHK(c, launch_lin_groups((const int32_t *)c->qcells.p, nullptr, (const int32_t *)c->qbin_off.p, 1, c->n_mu,
+                                  c->d_delta, c->d_limb, c->d_lbm, c->wave[0], c->step, 0,
+                                  (LinGroup *)c->qgroups.p, (int32_t *)c->qg_n.p, c->stream));
+    }
     {
       StageTimer tm(c, ST_FIR_QUIET);
       HIPCHK(c, launch_gather_quiet_linear(lin_args(c), c->d_cube[SOAP_CUBE_QUIET], c->d_cube[SOAP_CUBE_QUIET_SED],
-                                           (const int32_t *)c->qcells.p, (const int32_t *)c->qbin_off.p, c->d_quiet,
-                                           c->stream));
+                                           (const LinGroup *)c->qgroups.p, (const int32_t *)c->qbin_off.p,
+                                           (const int32_t *)c->qg_n.p, c->d_quiet, c->stream));
     }
     c->have_quiet = true;
     c->have_lower = c->have_ccf = c->have_fit = false;
@@ -1009,9 +1020,10 @@ static int integrate_linear(soap_ctx *c) {
     const int32_t *b = &c->box[(size_t)p * 4];
     loff[p + 1] = loff[p] + (int64_t)(b[1] - b[0]) * (b[3] - b[2]);
   }
-  if ((rc = ensure(c, c->acells, (size_t)std::max<int64_t>(1, loff[c->P]) * 4)) ||
-      (rc = ensure(c, c->abin_off, (size_t)c->P * (c->n_mu + 1) * 4)) ||
-      (rc = ensure(c, c->alist_off, (size_t)(c->P + 1) * 8)))
+  size_t cap = (size_t)std::max<int64_t>(1, loff[c->P]);
+  if ((rc = ensure(c, c->acells, cap * 4)) || (rc = ensure(c, c->abin_off, (size_t)c->P * (c->n_mu + 1) * 4)) ||
+      (rc = ensure(c, c->alist_off, (size_t)(c->P + 1) * 8)) || (rc = ensure(c, c->agroups, cap * sizeof(LinGroup))) ||
+      (rc = ensure(c, c->ag_n, (size_t)c->P * c->n_mu * 4)))
     return rc;
   HIPCHK(c, hipMemcpyAsync(c->alist_off.p, loff.data(), (c->P + 1) * 8, hipMemcpyHostToDevice, c->stream));
   {
@@ -1019,6 +1031,10 @@ static int integrate_linear(soap_ctx *c) {
     HIPCHK(c, launch_active_cells(c->G, c->n_mu, c->P, c->d_region, c->d_geo, c->d_geo_off, c->d_box,
                                   (const int64_t *)c->alist_off.p, (int32_t *)c->acells.p, (int32_t *)c->abin_off.p,
                                   c->stream));
+    HIPCHK(c, launch_lin_groups((const int32_t *)c->acells.p, (const int64_t *)c->alist_off.p,
+                                (const int32_t *)c->abin_off.p, c->P, c->n_mu, c->d_delta, c->d_limb, c->d_lbm,
+                                c->wave[0], c->step, c->act.solar_mode != 0, (LinGroup *)c->agroups.p,
+                                (int32_t *)c->ag_n.p, c->stream));
   }
   FirArgs f{};
   f.q = c->d_cube[SOAP_CUBE_QUIET];
@@ -1036,8 +1052,9 @@ static int integrate_linear(soap_ctx *c) {
   sol.t_norm = (int)(sol.t_sum + 0.9 - 216.8);
   {
     StageTimer tm(c, ST_FIR_ACTIVE);
-    HIPCHK(c, launch_gather_active_linear(lin_args(c), f, (const int32_t *)c->acells.p, (const int64_t *)c->alist_off.p,
-                                          (const int32_t *)c->abin_off.p, c->P, sol, c->stream));
+    HIPCHK(c, launch_gather_active_linear(lin_args(c), f, (const LinGroup *)c->agroups.p,
+                                          (const int64_t *)c->alist_off.p, (const int32_t *)c->abin_off.p,
+                                          (const int32_t *)c->ag_n.p, c->P, sol, c->stream));
   }
   c->have_raw = true;
   c->have_lower = c->have_ccf = c->have_fit = false;

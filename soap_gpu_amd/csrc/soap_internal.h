@@ -13,6 +13,15 @@ constexpr int kMaxSolarNodes = 12;
 constexpr double kPlanckH = 6.62606896e-34;
 constexpr double kBoltzK = 1.380e-23;
 
+// read-only for the whole kernel (the constant address space): uniform loads from it
+// go through the scalar unit even where the pointer came from memory and the kernel
+// stores elsewhere through pointers that could alias it
+#define CONSTANT __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const CONSTANT T *gc(const T *p) {
+  return (const CONSTANT T *)(uintptr_t)p;
+}
+
 // Region geometry of one (phase, region), evaluated on the host once per phase
 // (spot_init_gpu/spot_phase_gpu/spot_area + the first row of spot_inverse_rotation).
 struct RegionDev {
@@ -170,6 +179,17 @@ struct LinArgs {
   const double *wave;         // [n]
   int32_t off_lo, span;       // source indices staged per tile: [j0 + off_lo, j0 + off_lo + span)
   const double *delta, *limb, *lbm;
+};
+
+// A run of cells of one (phase, mu bin) cell list with bitwise-equal shifts and
+// one population, merged: the shift constants of the index formula, the summed
+// limb weights w = sum L and w2 = sum (1 + LB) L, and the first cell's list entry
+// (index | type << 30).  64 bytes, read with one scalar load per group.
+struct alignas(16) LinGroup {
+  double opf, fl, c1, rinv;  // 1 + fbeta, fbeta lambda_0 / step, 1 - sqrt((1+beta)/(1-beta)), 1 / opf
+  double w, w2;
+  int32_t e, ident;          // ident: delta == 0 (the row itself)
+  int32_t pad[2];
 };
 
 struct LinSolar {
@@ -369,11 +389,17 @@ hipError_t launch_disk_cells_by_bin(int ncell, int n_mu, const double *limb, con
 hipError_t launch_active_cells(int grid, int n_mu, int n_phase, const int32_t *region, const RegionDev *geo,
                                const int32_t *geo_off, const int32_t *box, const int64_t *list_off, int32_t *cells,
                                int32_t *bin_off, hipStream_t s);
-hipError_t launch_gather_quiet_linear(const LinArgs &a, const double *q, const double *qs, const int32_t *cells,
-                                      const int32_t *bin_off, double *out, hipStream_t s);
-hipError_t launch_gather_active_linear(const LinArgs &a, const FirArgs &f, const int32_t *cells,
-                                       const int64_t *list_off, const int32_t *bin_off, int n_phase,
-                                       const LinSolar &sol, hipStream_t s);
+// Groups of every (phase, bin) segment of a cell list (list_off NULL: one phase at
+// offset 0), written at the segment's own start; g_n[p * n_mu + k] = group count.
+// split_fac: facula cells stay single (solar mode's per-cell intensity).
+hipError_t launch_lin_groups(const int32_t *cells, const int64_t *list_off, const int32_t *bin_off, int n_phase,
+                             int n_mu, const double *delta, const double *limb, const double *lbm, double lambda0,
+                             double step, int split_fac, LinGroup *groups, int32_t *g_n, hipStream_t s);
+hipError_t launch_gather_quiet_linear(const LinArgs &a, const double *q, const double *qs, const LinGroup *groups,
+                                      const int32_t *bin_off, const int32_t *g_n, double *out, hipStream_t s);
+hipError_t launch_gather_active_linear(const LinArgs &a, const FirArgs &f, const LinGroup *groups,
+                                       const int64_t *list_off, const int32_t *bin_off, const int32_t *g_n,
+                                       int n_phase, const LinSolar &sol, hipStream_t s);
 hipError_t launch_conv_linear(const ConvLinArgs &c, int n_spec, const double *quiet, const double *raw, double *o_out,
                               double *part, hipStream_t s);
 int conv_linear_tile();

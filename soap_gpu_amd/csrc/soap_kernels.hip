@@ -35,14 +35,6 @@ template <class T>
 __device__ __forceinline__ GLOBAL T *gw(T *p) {
   return (GLOBAL T *)p;
 }
-// read-only for the whole kernel (the constant address space): uniform loads from it
-// go through the scalar unit even where the pointer came from memory and the kernel
-// stores elsewhere through pointers that could alias it
-#define CONSTANT __attribute__((address_space(4)))
-template <class T>
-__device__ __forceinline__ const CONSTANT T *gc(const T *p) {
-  return (const CONSTANT T *)(uintptr_t)p;
-}
 
 // ------------------------------------------------------------------ utilities
 __global__ void k_fill_u64(unsigned long long *p, size_t count) {

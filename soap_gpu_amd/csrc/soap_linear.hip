@@ -6,7 +6,10 @@
 // i(j) = max{i : ceil(i (1 + fbeta) + fbeta lambda_0 / step) <= j} and the
 // reference's interpolation expression at i.  Cells are processed per mu bin
 // from bin-sorted cell lists, with the bin's seed-row segments and the
-// wavelengths of the tile staged in LDS.  The convolution (conv_inst,
+// wavelengths of the tile staged in LDS.  The gathered value depends on a cell
+// only through its shift, so each run of cells with bitwise-equal shifts in a
+// bin's list (a disk column under solid-body rotation) is gathered once with the
+// run's summed weights.  The convolution (conv_inst,
 // SOAP_resolution.cu:76-95) has per-pixel Gaussian widths; the mask CCF uses
 // the reference's literal index arithmetic (CCF_func.py:45-91).
 #include <hip/hip_runtime.h>
@@ -124,6 +127,7 @@ __device__ __forceinline__ double cell_rcos(int idx, int grid) {
 struct LinShift {
   bool ident;            // delta == 0: the row itself (SOAP_integration.cu:262-265)
   double opf, fl, c1;    // 1 + fbeta, fbeta lambda_0 / step, 1 - sqrt((1+beta)/(1-beta))
+  double rinv;           // 1 / opf (first guess of the gather index only)
 };
 
 __device__ __forceinline__ LinShift lin_shift(double delta_kms, double lambda0, double step) {
@@ -136,6 +140,17 @@ __device__ __forceinline__ LinShift lin_shift(double delta_kms, double lambda0, 
   s.c1 = 1 - sq;
   s.opf = 1 + fbeta;
   s.fl = fbeta * lambda0 / step;
+  s.rinv = 1 / s.opf;
+  return s;
+}
+
+__device__ __forceinline__ LinShift group_shift(const CONSTANT LinGroup &g) {
+  LinShift s;
+  s.ident = g.ident != 0;
+  s.opf = g.opf;
+  s.fl = g.fl;
+  s.c1 = g.c1;
+  s.rinv = g.rinv;
   return s;
 }
 
@@ -145,10 +160,11 @@ __device__ __forceinline__ double lin_lshift(double lam, const LinShift &s) {
   return lam + -1 * lam * s.c1;
 }
 
-// max{i : ceil(i (1+fbeta) + fbeta lambda_0/step) <= j}
+// max{i : ceil(i (1+fbeta) + fbeta lambda_0/step) <= j}: a first guess from the
+// reciprocal, then the reference's own expression decides
 __device__ __forceinline__ long lin_gather_index(long j, const LinShift &s) {
 #pragma clang fp contract(off)
-  long i = (long)floor((j - s.fl) / s.opf);
+  long i = (long)floor((j - s.fl) * s.rinv);
   for (int it = 0; it < 8 && (long)ceil(i * s.opf + s.fl) > j; it++) i--;
   for (int it = 0; it < 8 && (long)ceil((i + 1) * s.opf + s.fl) <= j; it++) i++;
   return i;
@@ -198,12 +214,84 @@ __device__ void lin_stage_wave(double *dst, const double *wave, long base, int s
   }
 }
 
+// ------------------------------------------------------------ shift groups
+// Cell c of a segment starts a new group unless the previous cell has the same
+// population and a bitwise-equal shift (facula cells in solar mode always start
+// one: their intensity depends on the cell's own mu).
+__device__ __forceinline__ bool starts_group(const int32_t *cells, long c, const double *delta, int split_fac) {
+  int e = cells[c], ep = cells[c - 1];
+  int ty = e >> 30;
+  if (ty != (ep >> 30) || (split_fac && ty == 1)) return true;
+  return delta[e & 0x3fffffff] != delta[ep & 0x3fffffff];
+}
+
+// Segment (p, k) = blockIdx.x of a bin-sorted cell list: its runs of equal shift
+// become groups with w = sum L and w2 = sum (1 + LB) L over the run (in list
+// order), stored from the segment's start.  sum_c L_c v(delta) = v(delta) sum_c L_c,
+// so the gathers over groups equal the per-cell sums of SOAP_integration.cu:
+// 285-307,392-447 (and itot's, SOAP_initialize.cu:339-395) up to rounding.
+__global__ __launch_bounds__(kLTPB) void k_lin_groups(const int32_t *cells, const int64_t *list_off,
+                                                      const int32_t *bin_off, int n_mu, const double *delta,
+                                                      const double *limb, const double *lbm, double lambda0,
+                                                      double step, int split_fac, LinGroup *groups, int32_t *g_n) {
+#pragma clang fp contract(off)
+  __shared__ int s_wave[kLTPB / 64];
+  int sg = blockIdx.x, p = sg / n_mu, k = sg % n_mu;
+  long base = list_off ? list_off[p] : 0;
+  const int32_t *bo = bin_off + (long)p * (n_mu + 1);
+  long c0 = base + bo[k], c1 = base + bo[k + 1];
+  int run = 0;
+  for (long cb = c0; cb < c1; cb += kLTPB) {
+    long c = cb + threadIdx.x;
+    bool head = c < c1 && (c == c0 || starts_group(cells, c, delta, split_fac));
+    int tot;
+    int slot = block_compact(head, s_wave, &tot);
+    if (head) {
+      int e = cells[c];
+      double w = 0.0, w2 = 0.0;
+      long d = c;
+      do {
+        int id = cells[d] & 0x3fffffff;
+        double L = limb[id];
+        w += L;
+        w2 += (1 + lbm[id]) * L;
+        d++;
+      } while (d < c1 && !starts_group(cells, d, delta, split_fac));
+      LinShift sh = lin_shift(delta[e & 0x3fffffff], lambda0, step);
+      LinGroup g;
+      g.opf = sh.opf;
+      g.fl = sh.fl;
+      g.c1 = sh.c1;
+      g.rinv = sh.rinv;
+      g.w = w;
+      g.w2 = w2;
+      g.e = e;
+      g.ident = sh.ident ? 1 : 0;
+      g.pad[0] = g.pad[1] = 0;
+      groups[c0 + run + slot] = g;
+    }
+    run += tot;
+  }
+  if (threadIdx.x == 0) g_n[sg] = run;
+}
+
+hipError_t launch_lin_groups(const int32_t *cells, const int64_t *list_off, const int32_t *bin_off, int n_phase,
+                             int n_mu, const double *delta, const double *limb, const double *lbm, double lambda0,
+                             double step, int split_fac, LinGroup *groups, int32_t *g_n, hipStream_t s) {
+  long nseg = (long)n_phase * n_mu;
+  if (nseg == 0) return hipSuccess;
+  if (nseg > 0x7fffffffL) return hipErrorInvalidValue;
+  k_lin_groups<<<(unsigned)nseg, kLTPB, 0, s>>>(cells, list_off, bin_off, n_mu, delta, limb, lbm, lambda0, step,
+                                                split_fac, groups, g_n);
+  return hipGetLastError();
+}
+
 // Quiet star on a linear grid: out[0][j] = sum_cells limb Shift(Sq_k)(j), out[1] for
 // the SED cube (itot's cell loop, SOAP_initialize.cu:339-395), per tile of kLTPB
-// pixels, bins outer, cells of the bin inner.
+// pixels, bins outer, shift groups of the bin inner.
 __global__ __launch_bounds__(kLTPB) void k_gather_quiet_linear(LinArgs a, const double *q, const double *qs,
-                                                               const int32_t *cells, const int32_t *bin_off,
-                                                               double *out) {
+                                                               const LinGroup *groups, const int32_t *bin_off,
+                                                               const int32_t *g_n, double *out) {
   extern __shared__ double lds[];  // rows [2][span], wave [span]
   long j0 = (long)blockIdx.x * kLTPB;
   long j = j0 + threadIdx.x;
@@ -214,18 +302,19 @@ __global__ __launch_bounds__(kLTPB) void k_gather_quiet_linear(LinArgs a, const 
   LinTile T{rw, wl, base, span, a.n, a.wave};
   double lam_j = a.wave[j < a.n ? j : a.n - 1];
   double acc_q = 0.0, acc_s = 0.0;
+  const CONSTANT LinGroup *gp = gc(groups);
   for (int k = 0; k < a.n_mu; k++) {
-    int c0 = bin_off[k], c1 = bin_off[k + 1];
-    if (c0 == c1) continue;  // uniform
+    int c0 = bin_off[k], ng = g_n[k];
+    if (ng == 0) continue;  // uniform
     const double *gq = q + (long)k * a.n, *gs = qs + (long)k * a.n;
     __syncthreads();
     lin_stage(rw, gq, base, span, a.n);
     lin_stage(rw + span, gs, base, span, a.n);
     __syncthreads();
-    for (int c = c0; c < c1; c++) {
-      int idx = cells[c];
-      double L = a.limb[idx];
-      LinShift s = lin_shift(a.delta[idx], a.lambda0, a.step);
+    for (int c = c0; c < c0 + ng; c++) {
+      const CONSTANT LinGroup &G = gp[c];
+      double L = G.w;
+      LinShift s = group_shift(G);
       if (s.ident) {
         acc_q += T.row_at(0, gq, j) * L;
         acc_s += T.row_at(1, gs, j) * L;
@@ -246,12 +335,12 @@ __global__ __launch_bounds__(kLTPB) void k_gather_quiet_linear(LinArgs a, const 
   }
 }
 
-hipError_t launch_gather_quiet_linear(const LinArgs &a, const double *q, const double *qs, const int32_t *cells,
-                                      const int32_t *bin_off, double *out, hipStream_t s) {
+hipError_t launch_gather_quiet_linear(const LinArgs &a, const double *q, const double *qs, const LinGroup *groups,
+                                      const int32_t *bin_off, const int32_t *g_n, double *out, hipStream_t s) {
   size_t lds = (size_t)3 * a.span * sizeof(double);
   if (lds > 65536 || a.span < kLTPB) return hipErrorInvalidValue;
   unsigned nb = (unsigned)((a.n + kLTPB - 1) / kLTPB);
-  k_gather_quiet_linear<<<nb, kLTPB, lds, s>>>(a, q, qs, cells, bin_off, out);
+  k_gather_quiet_linear<<<nb, kLTPB, lds, s>>>(a, q, qs, groups, bin_off, g_n, out);
   return hipGetLastError();
 }
 
@@ -260,11 +349,14 @@ hipError_t launch_gather_quiet_linear(const LinArgs &a, const double *q, const d
 // SOAP_integration_solar.cu): per active cell, Q = Shift(Sq_k), QS = Shift(SEDq_k),
 // P = Shift(Sspot_k or Sfac_k), all x limb; intensity I = C_spot (spot),
 // (1+LB) C_fac or B(T_cell)/B(T_norm) C_fac (facula); flux += QS - I Q,
-// bconv += Q - P, tot += QS - I P.  Blocks are numbered so the phases of a tile
-// run back to back on one XCD.
-__global__ __launch_bounds__(kLTPB) void k_gather_active_linear(LinArgs a, FirArgs f, const int32_t *cells,
+// bconv += Q - P, tot += QS - I P.  Over a shift group the limb weights sum
+// (w = sum L) and the intensity weights too (spot: C_spot w, facula: C_fac
+// sum (1+LB) L; solar-mode facula groups are single cells).  Blocks are numbered
+// so the phases of a tile run back to back on one XCD.
+__global__ __launch_bounds__(kLTPB) void k_gather_active_linear(LinArgs a, FirArgs f, const LinGroup *groups,
                                                                 const int64_t *list_off, const int32_t *bin_off,
-                                                                int n_phase, int n_tiles, LinSolar sol) {
+                                                                const int32_t *g_n, int n_phase, int n_tiles,
+                                                                LinSolar sol) {
   extern __shared__ double lds[];  // rows [4][span]: q, qs, spot, facula; wave [span]
   long b = blockIdx.x;
   long slot = b / 8;
@@ -284,27 +376,29 @@ __global__ __launch_bounds__(kLTPB) void k_gather_active_linear(LinArgs a, FirAr
   double hc = kPlanckH * kC;
   double en = sol.on ? expm1(hc / (lam_j * kBoltzK * (double)sol.t_norm * 1e-10)) : 0.0;
   double flux = 0.0, bconv = 0.0, tot = 0.0;
-  const int32_t *pc = cells + list_off[p];
+  const CONSTANT LinGroup *gp = gc(groups + list_off[p]);
   const int32_t *bo = bin_off + (long)p * (a.n_mu + 1);
+  const int32_t *gn = g_n + (long)p * a.n_mu;
   for (int k = 0; k < a.n_mu; k++) {
-    int c0 = bo[k], c1 = bo[k + 1];
-    if (c0 == c1) continue;  // uniform
+    int c0 = bo[k], ng = gn[k];
+    if (ng == 0) continue;  // uniform
     const long ko = (long)k * a.n;
     const double *g[4] = {f.q + ko, f.qs + ko, f.sp ? f.sp + ko : f.q + ko, f.fa ? f.fa + ko : f.q + ko};
     __syncthreads();
     for (int r = 0; r < 4; r++) lin_stage(rw + (long)r * span, g[r], base, span, a.n);
     __syncthreads();
-    for (int c = c0; c < c1; c++) {
-      int e = pc[c];
+    for (int c = c0; c < c0 + ng; c++) {
+      const CONSTANT LinGroup &G = gp[c];
+      int e = G.e;
       int idx = e & 0x3fffffff, ty = e >> 30;
-      double L = a.limb[idx];
+      double L = G.w;
       int r_act = ty == 0 ? 2 : 3;
-      LinShift s = lin_shift(a.delta[idx], a.lambda0, a.step);
-      double vq, vs, va;
+      LinShift s = group_shift(G);
+      double iq, is, ia;  // shifted rows at pixel j
       if (s.ident) {
-        vq = T.row_at(0, g[0], j) * L;
-        vs = T.row_at(1, g[1], j) * L;
-        va = T.row_at(r_act, g[r_act], j) * L;
+        iq = T.row_at(0, g[0], j);
+        is = T.row_at(1, g[1], j);
+        ia = T.row_at(r_act, g[r_act], j);
       } else {
         long i = lin_gather_index(j, s);
         double ls0 = lin_lshift(T.wave_at(i), s), ls1 = lin_lshift(T.wave_at(i + 1), s);
@@ -312,22 +406,24 @@ __global__ __launch_bounds__(kLTPB) void k_gather_active_linear(LinArgs a, FirAr
         double q0 = T.row_at(0, g[0], i), q1 = T.row_at(0, g[0], i + 1);
         double s0 = T.row_at(1, g[1], i), s1 = T.row_at(1, g[1], i + 1);
         double a0 = T.row_at(r_act, g[r_act], i), a1 = T.row_at(r_act, g[r_act], i + 1);
-        vq = (q0 + (q1 - q0) * t) * L;
-        vs = (s0 + (s1 - s0) * t) * L;
-        va = (a0 + (a1 - a0) * t) * L;
+        iq = q0 + (q1 - q0) * t;
+        is = s0 + (s1 - s0) * t;
+        ia = a0 + (a1 - a0) * t;
       }
-      double I;
+      // IW = sum over the group of I L
+      double IW;
       if (ty == 0) {
-        I = cs;  // B(T_spot)/B(int T_spot) = 1
+        IW = cs * L;  // B(T_spot)/B(int T_spot) = 1
       } else if (sol.on) {
         double T_cell = (double)plage_temp(sol.t_sum, cell_rcos(idx, a.grid));
-        I = en / expm1(hc / (lam_j * kBoltzK * T_cell * 1e-10)) * cf;
+        IW = en / expm1(hc / (lam_j * kBoltzK * T_cell * 1e-10)) * cf * L;
       } else {
-        I = (1 + a.lbm[idx]) * cf;
+        IW = cf * G.w2;
       }
-      flux = flux + vs - I * vq;
+      double vq = iq * L, vs = is * L, va = ia * L;
+      flux = flux + vs - IW * iq;
       bconv = bconv + vq - va;
-      tot = tot + vs - va * I;
+      tot = tot + vs - IW * ia;
     }
   }
   if (j < a.n) {
@@ -338,9 +434,9 @@ __global__ __launch_bounds__(kLTPB) void k_gather_active_linear(LinArgs a, FirAr
   }
 }
 
-hipError_t launch_gather_active_linear(const LinArgs &a, const FirArgs &f, const int32_t *cells,
-                                       const int64_t *list_off, const int32_t *bin_off, int n_phase,
-                                       const LinSolar &sol, hipStream_t s) {
+hipError_t launch_gather_active_linear(const LinArgs &a, const FirArgs &f, const LinGroup *groups,
+                                       const int64_t *list_off, const int32_t *bin_off, const int32_t *g_n,
+                                       int n_phase, const LinSolar &sol, hipStream_t s) {
   if (n_phase == 0) return hipSuccess;
   size_t lds = (size_t)5 * a.span * sizeof(double);
   if (lds > 65536 || a.span < kLTPB) return hipErrorInvalidValue;
@@ -348,8 +444,8 @@ hipError_t launch_gather_active_linear(const LinArgs &a, const FirArgs &f, const
   long tiles8 = ((long)n_tiles + 7) / 8 * 8;
   long blocks = tiles8 * n_phase;
   if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
-  k_gather_active_linear<<<(unsigned)blocks, kLTPB, lds, s>>>(a, f, cells, list_off, bin_off, n_phase, n_tiles,
-                                                              sol);
+  k_gather_active_linear<<<(unsigned)blocks, kLTPB, lds, s>>>(a, f, groups, list_off, bin_off, g_n, n_phase,
+                                                              n_tiles, sol);
   return hipGetLastError();
 }
 

@@ -257,6 +257,30 @@ def linear_small(seed: int = 11, grid: int = 48, n: int = 4000, n_mu: int = 6, n
                     sigma_inst_ms(115000), vel_grid(20000.0, 500.0), meta={"step": 0.005})
 
 
+def linear_series(seed: int = 1239, n_phase: int = 25, grid: int = 1000, lmin: float = 5000.0,
+                  lmax: float = 5500.0) -> Workload:
+    """The production disk and activity (grid 1000, 20 mu bins, 2 spots + 2 faculae
+    overlapping, vsini 2 km/s, R=115000, 161 velocities) on the reference's native
+    linear 0.005 A seed format over lmin..lmax (100,000 pixels by default), with
+    lines at the production density (8000 per 3800-7000 A) and every other line in
+    the mask."""
+    rng = np.random.default_rng(seed)
+    n = int(round((lmax - lmin) / 0.005))
+    wave = linear_grid(lmin, n)
+    n_lines = int(round(8000 * (wave[-1] - lmin) / 3200.0))
+    # lines away from the edges, as in linear_small (the duplicated join point)
+    lines = draw_lines(rng, n_lines, lmin + 5.0, wave[-1] - 5.0)
+    bounds, _ = mu_bins(20)
+    cubes = make_cubes(wave, lines, 20)
+    regs = overlapping_regions(rng)
+    phases = np.arange(n_phase, dtype=np.float64) / n_phase
+    half = n_lines // 2
+    return Workload("linear_series", wave, 0, bounds, cubes, lines, lines["center"][:half].copy(),
+                    lines["depth"][:half].copy(), _star(grid, 2.0), _act(), phases,
+                    [list(regs) for _ in phases], sigma_inst_ms(115000), vel_grid(20000.0, 250.0),
+                    meta={"step": 0.005})
+
+
 def random_regions(rng: np.random.Generator, n_reg: int):
     """n_reg regions, faculae (r 0.1-0.2) first, then spots (r 0.05-0.12) each placed
     on a facula so that the two overlap (the later region wins a shared cell,

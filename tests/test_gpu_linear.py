@@ -107,3 +107,35 @@ def test_linear_series_equals_staged(linear_case):
             np.testing.assert_array_equal(fit[k], g["fit"][k])
     finally:
         e.close()
+
+
+def _linear_parity(engine, w):
+    g, o = run_engine(engine, w), run_oracle(w)
+    for q in range(2):
+        assert normwise(g["quiet"][q], o["quiet"][q]) <= SPEC_TOL, q
+    for p in range(len(w.phases)):
+        for t in range(3):
+            ref = o["raw"][p, t]
+            if np.max(np.abs(ref)) == 0:
+                assert np.max(np.abs(g["raw"][p, t])) == 0
+                continue
+            assert normwise(g["raw"][p, t], ref) <= SPEC_TOL, (p, t)
+    par = g["fit"]["par"].reshape(len(w.phases) + 1, 3, 5)
+    np.testing.assert_allclose(par[:-1, :, 3] - par[-1:, :, 3], o["rv"], rtol=0, atol=RV_TOL)
+    np.testing.assert_allclose(par[:, :, 4], o["par"][:, :, 4], rtol=0, atol=RV_TOL)
+
+
+def test_linear_grid300_shift_groups(engine):
+    """A 300 x 300 disk: runs of equal shift along whole disk columns (up to ~300
+    cells per bin segment) gathered as one group each, against the oracle's
+    per-cell sums."""
+    _linear_parity(engine, synth.linear_small(seed=12, grid=300, n=3000, n_mu=10, n_phase=2))
+
+
+def test_linear_differential_rotation(engine):
+    """Differential rotation: the shift varies along a column, so most groups are
+    single cells and the run boundaries fall inside columns."""
+    import dataclasses
+    w = synth.linear_small(seed=13, grid=64, n=3000, n_phase=2)
+    w = dataclasses.replace(w, star=synth._star(64, 3.0, incl=70.0, w_ratio=0.3))
+    _linear_parity(engine, w)
