@@ -112,7 +112,7 @@ struct soap_ctx {
   // lists, and their shift groups with per-segment group counts
   int lin_off_lo = 0, lin_span = 0;
   DevBuf qcells, qbin_off, acells, abin_off, alist_off;
-  DevBuf qgroups, qg_n, agroups, ag_n;
+  DevBuf qgroups, qg_n, agroups, ag_n, lcnt;
   // solar mode: Lagrange nodes over the facula temperature range
   int solar_k = 0, t_norm_fac = 0;
   double solar_t[kMaxSolarNodes] = {}, solar_w[kMaxSolarNodes] = {};
@@ -572,17 +572,19 @@ static int linear_prepare(soap_ctx *c, double vmax) {
     return fail(c, SOAP_ERR_LIMIT, "linear grid: shifts of up to %.1f pixels exceed the tile window", hi - lo);
   int rc;
   size_t g2 = (size_t)c->G * c->G;
-  if ((rc = ensure(c, c->qbin_off, (size_t)(c->n_mu + 1) * 4)) || (rc = ensure(c, c->qcells, g2 * 4))) return rc;
-  std::vector<int32_t> cnt(c->n_mu), off(c->n_mu + 1, 0);
-  HIPCHK(c, launch_disk_cells_by_bin((int)g2, c->n_mu, c->d_limb, c->d_delta, c->d_region, 1, (int32_t *)c->qbin_off.p,
-                                     nullptr, nullptr, c->stream));
-  HIPCHK(c, hipMemcpyAsync(cnt.data(), c->qbin_off.p, c->n_mu * 4, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (int k = 0; k < c->n_mu; k++) off[k + 1] = off[k] + cnt[k];
-  HIPCHK(c, hipMemcpyAsync(c->qbin_off.p, off.data(), (c->n_mu + 1) * 4, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, launch_disk_cells_by_bin((int)g2, c->n_mu, c->d_limb, c->d_delta, c->d_region, 0, nullptr,
-                                     (const int32_t *)c->qbin_off.p, (int32_t *)c->qcells.p, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  CellSrc src{};
+  src.grid = c->G;
+  src.n_mu = c->n_mu;
+  src.n_phase = 1;
+  src.nch = cell_list_chunks((long)g2);
+  src.region = c->d_region;
+  src.limb = c->d_limb;
+  src.delta = c->d_delta;
+  if ((rc = ensure(c, c->qbin_off, (size_t)(c->n_mu + 1) * 4)) || (rc = ensure(c, c->qcells, g2 * 4)) ||
+      (rc = ensure(c, c->lcnt, (size_t)c->n_mu * src.nch * 4)))
+    return rc;
+  HIPCHK(c, launch_cell_lists(src, nullptr, (int32_t *)c->lcnt.p, (int32_t *)c->qcells.p, (int32_t *)c->qbin_off.p,
+                              c->stream));
   return SOAP_OK;
 }
 
@@ -1016,21 +1018,35 @@ int soap_set_activity(soap_ctx *c, const soap_activity *act, int32_t n_phase, co
 static int integrate_linear(soap_ctx *c) {
   int rc;
   std::vector<int64_t> loff(c->P + 1, 0);
+  long max_box = 0;
   for (int p = 0; p < c->P; p++) {
     const int32_t *b = &c->box[(size_t)p * 4];
-    loff[p + 1] = loff[p] + (int64_t)(b[1] - b[0]) * (b[3] - b[2]);
+    long nb = (long)(b[1] - b[0]) * (b[3] - b[2]);
+    loff[p + 1] = loff[p] + nb;
+    max_box = std::max(max_box, nb);
   }
+  CellSrc src{};
+  src.grid = c->G;
+  src.n_mu = c->n_mu;
+  src.n_phase = c->P;
+  src.nch = std::max(1, cell_list_chunks(max_box));
+  src.region = c->d_region;
+  src.limb = c->d_limb;
+  src.delta = c->d_delta;
+  src.geo = c->d_geo;
+  src.geo_off = c->d_geo_off;
+  src.box = c->d_box;
   size_t cap = (size_t)std::max<int64_t>(1, loff[c->P]);
   if ((rc = ensure(c, c->acells, cap * 4)) || (rc = ensure(c, c->abin_off, (size_t)c->P * (c->n_mu + 1) * 4)) ||
       (rc = ensure(c, c->alist_off, (size_t)(c->P + 1) * 8)) || (rc = ensure(c, c->agroups, cap * sizeof(LinGroup))) ||
-      (rc = ensure(c, c->ag_n, (size_t)c->P * c->n_mu * 4)))
+      (rc = ensure(c, c->ag_n, (size_t)c->P * c->n_mu * 4)) ||
+      (rc = ensure(c, c->lcnt, (size_t)c->P * c->n_mu * src.nch * 4)))
     return rc;
   HIPCHK(c, hipMemcpyAsync(c->alist_off.p, loff.data(), (c->P + 1) * 8, hipMemcpyHostToDevice, c->stream));
   {
     StageTimer tm(c, ST_TAPS_ACTIVE);
-    HIPCHK(c, launch_active_cells(c->G, c->n_mu, c->P, c->d_region, c->d_geo, c->d_geo_off, c->d_box,
-                                  (const int64_t *)c->alist_off.p, (int32_t *)c->acells.p, (int32_t *)c->abin_off.p,
-                                  c->stream));
+    HIPCHK(c, launch_cell_lists(src, (const int64_t *)c->alist_off.p, (int32_t *)c->lcnt.p, (int32_t *)c->acells.p,
+                                (int32_t *)c->abin_off.p, c->stream));
     HIPCHK(c, launch_lin_groups((const int32_t *)c->acells.p, (const int64_t *)c->alist_off.p,
                                 (const int32_t *)c->abin_off.p, c->P, c->n_mu, c->d_delta, c->d_limb, c->d_lbm,
                                 c->wave[0], c->step, c->act.solar_mode != 0, (LinGroup *)c->agroups.p,

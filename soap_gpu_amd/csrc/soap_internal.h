@@ -192,6 +192,17 @@ struct alignas(16) LinGroup {
   int32_t pad[2];
 };
 
+// Source of the linear path's cell lists: the quiet disk (box NULL: the whole map,
+// n_phase 1) or the active regions of each phase (scan boxes + region geometry);
+// nch = chunks of 4096 box cells of the largest box (cell_list_chunks).
+struct CellSrc {
+  int32_t grid, n_mu, n_phase, nch;
+  const int32_t *region;
+  const double *limb, *delta;
+  const RegionDev *geo;
+  const int32_t *geo_off, *box;
+};
+
 struct LinSolar {
   int32_t on, t_sum, t_norm;  // Planck-ratio facula intensity (SOAP_integration_solar.cu:433)
 };
@@ -383,12 +394,12 @@ hipError_t launch_fit(const double *v, int n_v, const double *y, const double *e
                       double *model, double *depth, double *bis, int32_t *iters, hipStream_t s,
                       int32_t *status = nullptr);
 // linear grids
-hipError_t launch_disk_cells_by_bin(int ncell, int n_mu, const double *limb, const double *delta,
-                                    const int32_t *region, int count_only, int32_t *counts, const int32_t *offsets,
-                                    int32_t *cells, hipStream_t s);
-hipError_t launch_active_cells(int grid, int n_mu, int n_phase, const int32_t *region, const RegionDev *geo,
-                               const int32_t *geo_off, const int32_t *box, const int64_t *list_off, int32_t *cells,
-                               int32_t *bin_off, hipStream_t s);
+// cell lists of every (phase, bin) in map order: phase p's list from cells +
+// list_off[p] (list_off NULL: offset 0), bin k of it at bin_off[p][k];
+// cnt: n_phase * n_mu * nch ints of scratch
+hipError_t launch_cell_lists(const CellSrc &src, const int64_t *list_off, int32_t *cnt, int32_t *cells,
+                             int32_t *bin_off, hipStream_t s);
+int cell_list_chunks(long box_cells);
 // Groups of every (phase, bin) segment of a cell list (list_off NULL: one phase at
 // offset 0), written at the segment's own start; g_n[p * n_mu + k] = group count.
 // split_fac: facula cells stay single (solar mode's per-cell intensity).

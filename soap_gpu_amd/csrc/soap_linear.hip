@@ -40,79 +40,129 @@ __device__ __forceinline__ int block_compact(bool pred, int *s_wave, int *total)
   return off + before;
 }
 
-// Disk cells of mu bin k = blockIdx.x in map order.  count_only: counts[k] only;
-// else writes the indices at cells + offsets[k].
-__global__ __launch_bounds__(kLTPB) void k_disk_cells_by_bin(int ncell, const double *limb, const double *delta,
-                                                             const int32_t *region, int count_only,
-                                                             int32_t *counts, const int32_t *offsets,
-                                                             int32_t *cells) {
-  __shared__ int s_wave[kLTPB / 64];
-  int k = blockIdx.x;
-  int run = 0;
-  for (int c0 = 0; c0 < ncell; c0 += kLTPB) {
-    int idx = c0 + threadIdx.x;
-    bool on = idx < ncell && !(limb[idx] == -999.0 && delta[idx] == -999.0) && region[idx] == k;
-    int tot;
-    int slot = block_compact(on, s_wave, &tot);
-    if (on && !count_only) cells[offsets[k] + run + slot] = idx;
-    run += tot;
+// Exclusive block-wide prefix sum of v (in thread order); *total = the block's sum.
+__device__ __forceinline__ int block_scan_excl(int v, int *s_wave, int *total) {
+  int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    int y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
   }
-  if (count_only && threadIdx.x == 0) counts[k] = run;
+  if (lane == 63) s_wave[wv] = x;
+  __syncthreads();
+  int off = 0, tot = 0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) {
+    if (w < wv) off += s_wave[w];
+    tot += s_wave[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return off + x - v;
 }
 
-hipError_t launch_disk_cells_by_bin(int ncell, int n_mu, const double *limb, const double *delta,
-                                    const int32_t *region, int count_only, int32_t *counts, const int32_t *offsets,
-                                    int32_t *cells, hipStream_t s) {
-  k_disk_cells_by_bin<<<n_mu, kLTPB, 0, s>>>(ncell, limb, delta, region, count_only, counts, offsets, cells);
-  return hipGetLastError();
+// Cell lists by (phase, mu bin), each in map order: the quiet disk (one phase whose
+// box is the whole map; disk cells of bin k, entry = index) and the active regions
+// (phase p's scan box; cells of bin k that classify as spot or facula, entry =
+// index | type << 30).  Three passes over blocks (p, k, ch), ch = chunk of kChunk
+// box cells in map order: count, per-phase offsets, write.
+constexpr int kChunk = 4096;
+
+// entry of box cell c of phase p for bin k, or -1
+__device__ __forceinline__ int cell_entry(const CellSrc &s, int p, int k, long c) {
+  int y0 = 0, z0 = 0, nz = s.grid;
+  if (s.box) {
+    y0 = s.box[p * 4];
+    z0 = s.box[p * 4 + 2];
+    nz = s.box[p * 4 + 3] - z0;
+  }
+  int iy = y0 + (int)(c / nz), iz = z0 + (int)(c % nz);
+  int idx = iy * s.grid + iz;
+  if (s.region[idx] != k) return -1;
+  if (!s.box) return (s.limb[idx] == -999.0 && s.delta[idx] == -999.0) ? -1 : idx;
+  double rc;
+  int ty = classify_cell(iy, iz, 2. / s.grid, s.geo, s.geo_off[p], s.geo_off[p + 1], &rc);
+  return ty >= 0 ? (idx | (ty << 30)) : -1;
 }
 
-// Active cells of phase p = blockIdx.x, by mu bin then map order, each entry
-// idx | type << 30 (type 0 spot, 1 facula); bin boundaries in bin_off[p][n_mu+1].
-// The phase's list starts at cells + list_off[p] (capacity = its scan box).
-__global__ __launch_bounds__(kLTPB) void k_active_cells(int grid, int n_mu, const int32_t *region,
-                                                        const RegionDev *geo, const int32_t *geo_off,
-                                                        const int32_t *box, const int64_t *list_off, int32_t *cells,
-                                                        int32_t *bin_off) {
+__device__ __forceinline__ long cell_box_size(const CellSrc &s, int p) {
+  if (!s.box) return (long)s.grid * s.grid;
+  const int32_t *bx = s.box + p * 4;
+  return (long)(bx[1] - bx[0]) * (bx[3] - bx[2]);
+}
+
+__global__ __launch_bounds__(kLTPB) void k_cells_count(CellSrc s, int32_t *cnt) {
+  __shared__ int s_n;
+  long b = blockIdx.x;
+  int ch = (int)(b % s.nch);
+  long pk = b / s.nch;
+  int k = (int)(pk % s.n_mu), p = (int)(pk / s.n_mu);
+  long lo = (long)ch * kChunk, nbox = cell_box_size(s, p);
+  long hi = lo + kChunk < nbox ? lo + kChunk : nbox;
+  if (threadIdx.x == 0) s_n = 0;
+  __syncthreads();
+  int n = 0;
+  for (long c = lo + threadIdx.x; c < hi; c += kLTPB) n += cell_entry(s, p, k, c) >= 0;
+  if (n) atomicAdd(&s_n, n);
+  __syncthreads();
+  if (threadIdx.x == 0) cnt[b] = s_n;
+}
+
+// Phase p = blockIdx.x: exclusive offsets over (k, ch) within the phase, in place;
+// bin_off[p][k] = offset of (k, 0), bin_off[p][n_mu] = the phase's total.
+__global__ __launch_bounds__(kLTPB) void k_cells_offsets(int n_mu, int nch, int32_t *cnt, int32_t *bin_off) {
   __shared__ int s_wave[kLTPB / 64];
   int p = blockIdx.x;
-  const int32_t *bx = box + p * 4;
-  int y0 = bx[0], y1 = bx[1], z0 = bx[2], z1 = bx[3];
-  int nz = z1 - z0;
-  long nbox = (long)(y1 - y0) * nz;
-  int g0 = geo_off[p], g1 = geo_off[p + 1];
-  int32_t *out = cells + list_off[p];
-  double dg = 2. / grid;
-  int run = 0;
-  for (int k = 0; k < n_mu; k++) {
-    if (threadIdx.x == 0) bin_off[(long)p * (n_mu + 1) + k] = run;
-    for (long c0 = 0; c0 < nbox; c0 += kLTPB) {
-      long c = c0 + threadIdx.x;
-      int ty = -1, idx = 0;
-      if (c < nbox) {
-        int iy = y0 + (int)(c / nz), iz = z0 + (int)(c % nz);
-        idx = iy * grid + iz;
-        if (region[idx] == k) {
-          double rc;
-          ty = classify_cell(iy, iz, dg, geo, g0, g1, &rc);
-        }
-      }
-      int tot;
-      int slot = block_compact(ty >= 0, s_wave, &tot);
-      if (ty >= 0) out[run + slot] = idx | (ty << 30);
-      run += tot;
-    }
+  long ne = (long)n_mu * nch;
+  int32_t *c = cnt + (long)p * ne;
+  long per = (ne + kLTPB - 1) / kLTPB;
+  long e0 = threadIdx.x * per, e1 = e0 + per < ne ? e0 + per : ne;
+  int sum = 0;
+  for (long e = e0; e < e1; e++) sum += c[e];
+  int tot;
+  int run = block_scan_excl(sum, s_wave, &tot);
+  for (long e = e0; e < e1; e++) {
+    int v = c[e];
+    c[e] = run;
+    if (e % nch == 0) bin_off[(long)p * (n_mu + 1) + e / nch] = run;
+    run += v;
   }
-  if (threadIdx.x == 0) bin_off[(long)p * (n_mu + 1) + n_mu] = run;
+  if (threadIdx.x == 0) bin_off[(long)p * (n_mu + 1) + n_mu] = tot;
 }
 
-hipError_t launch_active_cells(int grid, int n_mu, int n_phase, const int32_t *region, const RegionDev *geo,
-                               const int32_t *geo_off, const int32_t *box, const int64_t *list_off, int32_t *cells,
-                               int32_t *bin_off, hipStream_t s) {
-  if (n_phase == 0) return hipSuccess;
-  k_active_cells<<<n_phase, kLTPB, 0, s>>>(grid, n_mu, region, geo, geo_off, box, list_off, cells, bin_off);
+__global__ __launch_bounds__(kLTPB) void k_cells_write(CellSrc s, const int32_t *off, const int64_t *list_off,
+                                                       int32_t *cells) {
+  __shared__ int s_wave[kLTPB / 64];
+  long b = blockIdx.x;
+  int ch = (int)(b % s.nch);
+  long pk = b / s.nch;
+  int k = (int)(pk % s.n_mu), p = (int)(pk / s.n_mu);
+  long lo = (long)ch * kChunk, nbox = cell_box_size(s, p);
+  if (lo >= nbox) return;  // uniform over the block
+  long hi = lo + kChunk < nbox ? lo + kChunk : nbox;
+  int32_t *out = cells + (list_off ? list_off[p] : 0) + off[b];
+  int run = 0;
+  for (long c0 = lo; c0 < hi; c0 += kLTPB) {
+    long c = c0 + threadIdx.x;
+    int e = c < hi ? cell_entry(s, p, k, c) : -1;
+    int tot;
+    int slot = block_compact(e >= 0, s_wave, &tot);
+    if (e >= 0) out[run + slot] = e;
+    run += tot;
+  }
+}
+
+hipError_t launch_cell_lists(const CellSrc &src, const int64_t *list_off, int32_t *cnt, int32_t *cells,
+                             int32_t *bin_off, hipStream_t st) {
+  long blocks = (long)src.n_phase * src.n_mu * src.nch;
+  if (blocks == 0) return hipSuccess;
+  if (blocks > 0x7fffffffL) return hipErrorInvalidValue;
+  k_cells_count<<<(unsigned)blocks, kLTPB, 0, st>>>(src, cnt);
+  k_cells_offsets<<<src.n_phase, kLTPB, 0, st>>>(src.n_mu, src.nch, cnt, bin_off);
+  k_cells_write<<<(unsigned)blocks, kLTPB, 0, st>>>(src, cnt, list_off, cells);
   return hipGetLastError();
 }
+
+int cell_list_chunks(long box_cells) { return (int)((box_cells + kChunk - 1) / kChunk); }
 
 // mu of a cell from its grid position (as classify_cell computes it)
 __device__ __forceinline__ double cell_rcos(int idx, int grid) {
@@ -226,53 +276,87 @@ __device__ __forceinline__ bool starts_group(const int32_t *cells, long c, const
 }
 
 // Segment (p, k) = blockIdx.x of a bin-sorted cell list: its runs of equal shift
-// become groups with w = sum L and w2 = sum (1 + LB) L over the run (in list
-// order), stored from the segment's start.  sum_c L_c v(delta) = v(delta) sum_c L_c,
-// so the gathers over groups equal the per-cell sums of SOAP_integration.cu:
-// 285-307,392-447 (and itot's, SOAP_initialize.cu:339-395) up to rounding.
+// become groups with w = sum L and w2 = sum (1 + LB) L over the run, stored from
+// the segment's start.  sum_c L_c v(delta) = v(delta) sum_c L_c, so the gathers
+// over groups equal the per-cell sums of SOAP_integration.cu:285-307,392-447
+// (and itot's, SOAP_initialize.cu:339-395) up to rounding.  The run sums come from
+// a segmented scan over each 256-cell chunk (restarting at run heads, in a fixed
+// order) plus the open run carried from the chunk before.
 __global__ __launch_bounds__(kLTPB) void k_lin_groups(const int32_t *cells, const int64_t *list_off,
                                                       const int32_t *bin_off, int n_mu, const double *delta,
                                                       const double *limb, const double *lbm, double lambda0,
                                                       double step, int split_fac, LinGroup *groups, int32_t *g_n) {
 #pragma clang fp contract(off)
   __shared__ int s_wave[kLTPB / 64];
+  __shared__ double s_w[kLTPB], s_w2[kLTPB];
+  __shared__ int s_f[kLTPB];
+  __shared__ double s_carry[2];
+  const int tid = threadIdx.x;
   int sg = blockIdx.x, p = sg / n_mu, k = sg % n_mu;
   long base = list_off ? list_off[p] : 0;
   const int32_t *bo = bin_off + (long)p * (n_mu + 1);
   long c0 = base + bo[k], c1 = base + bo[k + 1];
-  int run = 0;
+  LinGroup *gs = groups + c0;
+  int run = 0;                  // groups begun before this chunk
+  double cw = 0.0, cw2 = 0.0;   // sums of the run open at the chunk's start
   for (long cb = c0; cb < c1; cb += kLTPB) {
-    long c = cb + threadIdx.x;
-    bool head = c < c1 && (c == c0 || starts_group(cells, c, delta, split_fac));
+    long c = cb + tid;
+    bool in = c < c1;
+    bool head = in && (c == c0 || starts_group(cells, c, delta, split_fac));
+    bool last = in && (c + 1 == c1 || starts_group(cells, c + 1, delta, split_fac));
+    int e = in ? cells[c] : 0;
+    double v = 0.0, v2 = 0.0;
+    if (in) {
+      int id = e & 0x3fffffff;
+      v = limb[id];
+      v2 = (1 + lbm[id]) * v;
+    }
     int tot;
-    int slot = block_compact(head, s_wave, &tot);
+    int before = block_compact(head, s_wave, &tot);  // heads before this cell in the chunk
+    int f = head;
+    for (int o = 1; o < kLTPB; o <<= 1) {
+      s_w[tid] = v;
+      s_w2[tid] = v2;
+      s_f[tid] = f;
+      __syncthreads();
+      if (tid >= o && !f) {
+        v = s_w[tid - o] + v;
+        v2 = s_w2[tid - o] + v2;
+        f = s_f[tid - o];
+      }
+      __syncthreads();
+    }
+    if (!f) {  // no head at or before this cell in the chunk: the carried run
+      v = cw + v;
+      v2 = cw2 + v2;
+    }
     if (head) {
-      int e = cells[c];
-      double w = 0.0, w2 = 0.0;
-      long d = c;
-      do {
-        int id = cells[d] & 0x3fffffff;
-        double L = limb[id];
-        w += L;
-        w2 += (1 + lbm[id]) * L;
-        d++;
-      } while (d < c1 && !starts_group(cells, d, delta, split_fac));
       LinShift sh = lin_shift(delta[e & 0x3fffffff], lambda0, step);
-      LinGroup g;
+      LinGroup &g = gs[run + before];
       g.opf = sh.opf;
       g.fl = sh.fl;
       g.c1 = sh.c1;
       g.rinv = sh.rinv;
-      g.w = w;
-      g.w2 = w2;
       g.e = e;
       g.ident = sh.ident ? 1 : 0;
       g.pad[0] = g.pad[1] = 0;
-      groups[c0 + run + slot] = g;
     }
+    if (last) {  // the run's group: the last head at or before this cell
+      LinGroup &g = gs[run + before + (head ? 1 : 0) - 1];
+      g.w = v;
+      g.w2 = v2;
+    }
+    long lastc = (cb + kLTPB < c1 ? cb + kLTPB : c1) - 1;
+    if (c == lastc) {
+      s_carry[0] = last ? 0.0 : v;
+      s_carry[1] = last ? 0.0 : v2;
+    }
+    __syncthreads();
+    cw = s_carry[0];
+    cw2 = s_carry[1];
     run += tot;
   }
-  if (threadIdx.x == 0) g_n[sg] = run;
+  if (tid == 0) g_n[sg] = run;
 }
 
 hipError_t launch_lin_groups(const int32_t *cells, const int64_t *list_off, const int32_t *bin_off, int n_phase,
