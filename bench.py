@@ -436,11 +436,11 @@ def linear_leg(steps: int, n_phase: int = 25):
     q_cells, q_groups = int(qord.size), _shift_groups((region, delta), qord)
     a_cells = a_groups = 0
     for p in range(len(w.phases)):
-        temp, bi = eng.phase_maps(p)
-        act = np.nonzero(bi != 0)[0]
+        _, bi = eng.phase_maps(p)  # 0 spot, 1 facula, -9999 elsewhere
+        act = np.nonzero((bi == 0.0) | (bi == 1.0))[0]
         aord = act[np.argsort(region[act], kind="stable")]
         a_cells += int(aord.size)
-        a_groups += _shift_groups((region, delta, temp), aord)
+        a_groups += _shift_groups((region, delta, bi), aord)
     eng.close()
     fa = per.get("fir_active", 0.0) * 1e-3
     return {"workload": (f"linear 0.005 A grid {w.wave[0]:.0f}-{w.wave[-1]:.0f} A ({w.n} px), grid 1000x1000, "

@@ -112,7 +112,7 @@ struct soap_ctx {
   // lists, and their shift groups with per-segment group counts
   int lin_off_lo = 0, lin_span = 0;
   DevBuf qcells, qbin_off, acells, abin_off, alist_off;
-  DevBuf qgroups, qg_n, agroups, ag_n, lcnt;
+  DevBuf qgroups, qg_n, agroups, ag_n, lcnt, qpart;
   // solar mode: Lagrange nodes over the facula temperature range
   int solar_k = 0, t_norm_fac = 0;
   double solar_t[kMaxSolarNodes] = {}, solar_w[kMaxSolarNodes] = {};
@@ -777,7 +777,8 @@ int soap_quiet_star(soap_ctx *c) {
   if ((rc = dalloc(c, &c->d_quiet, (size_t)2 * c->n))) return rc;
   if (c->kind == SOAP_GRID_LINEAR) {
     size_t g2 = (size_t)c->G * c->G;
-    if ((rc = ensure(c, c->qgroups, g2 * sizeof(LinGroup))) || (rc = ensure(c, c->qg_n, (size_t)c->n_mu * 4)))
+    if ((rc = ensure(c, c->qgroups, g2 * sizeof(LinGroup))) || (rc = ensure(c, c->qg_n, (size_t)c->n_mu * 4)) ||
+        (rc = ensure(c, c->qpart, (size_t)quiet_linear_sets(c->n_mu) * 2 * c->n * 8)))
       return rc;
     {
       StageTimer tm(c, ST_TAPS_QUIET);
@@ -789,7 +790,8 @@ int soap_quiet_star(soap_ctx *c) {
       StageTimer tm(c, ST_FIR_QUIET);
       HIPCHK(c, launch_gather_quiet_linear(lin_args(c), c->d_cube[SOAP_CUBE_QUIET], c->d_cube[SOAP_CUBE_QUIET_SED],
                                            (const LinGroup *)c->qgroups.p, (const int32_t *)c->qbin_off.p,
-                                           (const int32_t *)c->qg_n.p, c->d_quiet, c->stream));
+                                           (const int32_t *)c->qg_n.p, (double *)c->qpart.p, c->d_quiet,
+                                           c->stream));
     }
     c->have_quiet = true;
     c->have_lower = c->have_ccf = c->have_fit = false;

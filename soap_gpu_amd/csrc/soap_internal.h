@@ -406,8 +406,11 @@ int cell_list_chunks(long box_cells);
 hipError_t launch_lin_groups(const int32_t *cells, const int64_t *list_off, const int32_t *bin_off, int n_phase,
                              int n_mu, const double *delta, const double *limb, const double *lbm, double lambda0,
                              double step, int split_fac, LinGroup *groups, int32_t *g_n, hipStream_t s);
+// part: quiet_linear_sets(n_mu) * 2 * n doubles of scratch
 hipError_t launch_gather_quiet_linear(const LinArgs &a, const double *q, const double *qs, const LinGroup *groups,
-                                      const int32_t *bin_off, const int32_t *g_n, double *out, hipStream_t s);
+                                      const int32_t *bin_off, const int32_t *g_n, double *part, double *out,
+                                      hipStream_t s);
+int quiet_linear_sets(int n_mu);
 hipError_t launch_gather_active_linear(const LinArgs &a, const FirArgs &f, const LinGroup *groups,
                                        const int64_t *list_off, const int32_t *bin_off, const int32_t *g_n,
                                        int n_phase, const LinSolar &sol, hipStream_t s);

@@ -20,6 +20,7 @@
 namespace soap {
 
 constexpr int kLTPB = 256;
+constexpr int kQuietSets = 16;  // bin sets of the quiet-star gather (blocks per tile)
 
 // ------------------------------------------------------------ cell lists
 // Stable compaction of the block's predicate over 256 candidates: returns this
@@ -372,11 +373,14 @@ hipError_t launch_lin_groups(const int32_t *cells, const int64_t *list_off, cons
 
 // Quiet star on a linear grid: out[0][j] = sum_cells limb Shift(Sq_k)(j), out[1] for
 // the SED cube (itot's cell loop, SOAP_initialize.cu:339-395), per tile of kLTPB
-// pixels, bins outer, shift groups of the bin inner.
+// pixels, bins outer, shift groups of the bin inner.  Block (tile, s) takes bins
+// s, s + ns, ... and writes its partial sums to part[s][2][n]; k_quiet_sum adds
+// the ns partials in order.
 __global__ __launch_bounds__(kLTPB) void k_gather_quiet_linear(LinArgs a, const double *q, const double *qs,
                                                                const LinGroup *groups, const int32_t *bin_off,
-                                                               const int32_t *g_n, double *out) {
+                                                               const int32_t *g_n, double *part) {
   extern __shared__ double lds[];  // rows [2][span], wave [span]
+  const int ns = gridDim.y, set = blockIdx.y;
   long j0 = (long)blockIdx.x * kLTPB;
   long j = j0 + threadIdx.x;
   long base = j0 + a.off_lo;
@@ -387,7 +391,7 @@ __global__ __launch_bounds__(kLTPB) void k_gather_quiet_linear(LinArgs a, const 
   double lam_j = a.wave[j < a.n ? j : a.n - 1];
   double acc_q = 0.0, acc_s = 0.0;
   const CONSTANT LinGroup *gp = gc(groups);
-  for (int k = 0; k < a.n_mu; k++) {
+  for (int k = set; k < a.n_mu; k += ns) {
     int c0 = bin_off[k], ng = g_n[k];
     if (ng == 0) continue;  // uniform
     const double *gq = q + (long)k * a.n, *gs = qs + (long)k * a.n;
@@ -414,17 +418,32 @@ __global__ __launch_bounds__(kLTPB) void k_gather_quiet_linear(LinArgs a, const 
     }
   }
   if (j < a.n) {
-    out[j] = acc_q;
-    out[a.n + j] = acc_s;
+    double *o = part + (long)set * 2 * a.n;
+    o[j] = acc_q;
+    o[a.n + j] = acc_s;
   }
 }
 
+__global__ __launch_bounds__(kLTPB) void k_quiet_sum(const double *part, int ns, long n2, double *out) {
+  long j = (long)blockIdx.x * kLTPB + threadIdx.x;
+  if (j >= n2) return;
+  double acc = part[j];
+  for (int t = 1; t < ns; t++) acc += part[(long)t * n2 + j];
+  out[j] = acc;
+}
+
+int quiet_linear_sets(int n_mu) { return n_mu < kQuietSets ? n_mu : kQuietSets; }
+
 hipError_t launch_gather_quiet_linear(const LinArgs &a, const double *q, const double *qs, const LinGroup *groups,
-                                      const int32_t *bin_off, const int32_t *g_n, double *out, hipStream_t s) {
+                                      const int32_t *bin_off, const int32_t *g_n, double *part, double *out,
+                                      hipStream_t s) {
   size_t lds = (size_t)3 * a.span * sizeof(double);
-  if (lds > 65536 || a.span < kLTPB) return hipErrorInvalidValue;
+  if (lds > 65536 || a.span < kLTPB || a.n_mu < 1) return hipErrorInvalidValue;
   unsigned nb = (unsigned)((a.n + kLTPB - 1) / kLTPB);
-  k_gather_quiet_linear<<<nb, kLTPB, lds, s>>>(a, q, qs, groups, bin_off, g_n, out);
+  int ns = quiet_linear_sets(a.n_mu);
+  k_gather_quiet_linear<<<dim3(nb, ns), kLTPB, lds, s>>>(a, q, qs, groups, bin_off, g_n, part);
+  long n2 = 2 * a.n;
+  k_quiet_sum<<<(unsigned)((n2 + kLTPB - 1) / kLTPB), kLTPB, 0, s>>>(part, ns, n2, out);
   return hipGetLastError();
 }
 
