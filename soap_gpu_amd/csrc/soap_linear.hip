@@ -213,36 +213,25 @@ __device__ __forceinline__ double lin_lshift(double lam, const LinShift &s) {
 
 // max{i : ceil(i (1+fbeta) + fbeta lambda_0/step) <= j}: a first guess from the
 // reciprocal, then the reference's own expression decides
-__device__ __forceinline__ long lin_gather_index(long j, const LinShift &s) {
+// (32-bit indices: the launchers refuse grids of 2^31 - 2^12 pixels or more)
+__device__ __forceinline__ int lin_gather_index(int j, const LinShift &s) {
 #pragma clang fp contract(off)
-  long i = (long)floor((j - s.fl) * s.rinv);
-  for (int it = 0; it < 8 && (long)ceil(i * s.opf + s.fl) > j; it++) i--;
-  for (int it = 0; it < 8 && (long)ceil((i + 1) * s.opf + s.fl) <= j; it++) i++;
+  int i = (int)floor((j - s.fl) * s.rinv);
+  for (int it = 0; it < 8 && (int)ceil(i * s.opf + s.fl) > j; it++) i--;
+  for (int it = 0; it < 8 && (int)ceil((i + 1) * s.opf + s.fl) <= j; it++) i++;
   return i;
 }
 
-// Staged tile: source indices [base, base + span) of up to 4 rows and the
-// wavelengths (linear continuation outside the grid), with global fallbacks.
-struct LinTile {
-  const double *rows;  // [nrow][span] in LDS
-  const double *wl;    // [span] in LDS
-  long base;
-  int span;
-  long n;
-  const double *wave;
-  __device__ double wave_at(long i) const {
-    long li = i - base;
-    if (li >= 0 && li < span) return wl[li];
-    if (i < 0) return wave[0] + (double)i * (wave[1] - wave[0]);
-    if (i >= n) return wave[n - 1] + (double)(i - (n - 1)) * (wave[n - 1] - wave[n - 2]);
-    return wave[i];
-  }
-  __device__ double row_at(int r, const double *grow, long i) const {
-    long li = i - base;
-    if (li >= 0 && li < span) return rows[(long)r * span + li];
-    return (i >= 0 && i < n) ? grow[i] : 0.0;
-  }
-};
+// Staged tile: source indices [base, base + span) of the rows and the wavelengths
+// (linear continuation outside the grid, rows 0 there).  The host sizes the window
+// from the extreme shifts on the disk with 3-pixel margins on both sides, so every
+// i, i + 1 a gather of the tile reads lies inside it; local() clamps all the same,
+// so that no index can leave the LDS arrays.
+__device__ __forceinline__ int tile_local(int i, int base, int span) {
+  int li = i - base;
+  return li < 0 ? 0 : (li > span - 2 ? span - 2 : li);
+}
+constexpr long kLinMaxN = 0x7fffffffL - 4096;
 
 __device__ void lin_stage(double *dst, const double *grow, long base, int span, long n) {
   for (int e = threadIdx.x; e < span; e += blockDim.x) {
@@ -282,15 +271,16 @@ __device__ __forceinline__ bool starts_group(const int32_t *cells, long c, const
 // over groups equal the per-cell sums of SOAP_integration.cu:285-307,392-447
 // (and itot's, SOAP_initialize.cu:339-395) up to rounding.  The run sums come from
 // a segmented scan over each 256-cell chunk (restarting at run heads, in a fixed
-// order) plus the open run carried from the chunk before.
-__global__ __launch_bounds__(kLTPB) void k_lin_groups(const int32_t *cells, const int64_t *list_off,
-                                                      const int32_t *bin_off, int n_mu, const double *delta,
-                                                      const double *limb, const double *lbm, double lambda0,
-                                                      double step, int split_fac, LinGroup *groups, int32_t *g_n) {
+// order) plus the open run carried from the chunk before; TPB cells per chunk.
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_lin_groups(const int32_t *cells, const int64_t *list_off,
+                                                    const int32_t *bin_off, int n_mu, const double *delta,
+                                                    const double *limb, const double *lbm, double lambda0,
+                                                    double step, int split_fac, LinGroup *groups, int32_t *g_n) {
 #pragma clang fp contract(off)
-  __shared__ int s_wave[kLTPB / 64];
-  __shared__ double s_w[kLTPB], s_w2[kLTPB];
-  __shared__ int s_f[kLTPB];
+  __shared__ int s_wave[TPB / 64];
+  __shared__ double s_w[TPB], s_w2[TPB];
+  __shared__ int s_f[TPB];
   __shared__ double s_carry[2];
   const int tid = threadIdx.x;
   int sg = blockIdx.x, p = sg / n_mu, k = sg % n_mu;
@@ -300,7 +290,7 @@ __global__ __launch_bounds__(kLTPB) void k_lin_groups(const int32_t *cells, cons
   LinGroup *gs = groups + c0;
   int run = 0;                  // groups begun before this chunk
   double cw = 0.0, cw2 = 0.0;   // sums of the run open at the chunk's start
-  for (long cb = c0; cb < c1; cb += kLTPB) {
+  for (long cb = c0; cb < c1; cb += TPB) {
     long c = cb + tid;
     bool in = c < c1;
     bool head = in && (c == c0 || starts_group(cells, c, delta, split_fac));
@@ -315,7 +305,7 @@ __global__ __launch_bounds__(kLTPB) void k_lin_groups(const int32_t *cells, cons
     int tot;
     int before = block_compact(head, s_wave, &tot);  // heads before this cell in the chunk
     int f = head;
-    for (int o = 1; o < kLTPB; o <<= 1) {
+    for (int o = 1; o < TPB; o <<= 1) {
       s_w[tid] = v;
       s_w2[tid] = v2;
       s_f[tid] = f;
@@ -347,7 +337,7 @@ __global__ __launch_bounds__(kLTPB) void k_lin_groups(const int32_t *cells, cons
       g.w = v;
       g.w2 = v2;
     }
-    long lastc = (cb + kLTPB < c1 ? cb + kLTPB : c1) - 1;
+    long lastc = (cb + TPB < c1 ? cb + TPB : c1) - 1;
     if (c == lastc) {
       s_carry[0] = last ? 0.0 : v;
       s_carry[1] = last ? 0.0 : v2;
@@ -366,8 +356,13 @@ hipError_t launch_lin_groups(const int32_t *cells, const int64_t *list_off, cons
   long nseg = (long)n_phase * n_mu;
   if (nseg == 0) return hipSuccess;
   if (nseg > 0x7fffffffL) return hipErrorInvalidValue;
-  k_lin_groups<<<(unsigned)nseg, kLTPB, 0, s>>>(cells, list_off, bin_off, n_mu, delta, limb, lbm, lambda0, step,
-                                                split_fac, groups, g_n);
+  // few long segments (the quiet disk: one per bin) take 1024-cell chunks
+  if (n_phase == 1)
+    k_lin_groups<1024><<<(unsigned)nseg, 1024, 0, s>>>(cells, list_off, bin_off, n_mu, delta, limb, lbm, lambda0,
+                                                       step, split_fac, groups, g_n);
+  else
+    k_lin_groups<kLTPB><<<(unsigned)nseg, kLTPB, 0, s>>>(cells, list_off, bin_off, n_mu, delta, limb, lbm, lambda0,
+                                                         step, split_fac, groups, g_n);
   return hipGetLastError();
 }
 
@@ -381,13 +376,13 @@ __global__ __launch_bounds__(kLTPB) void k_gather_quiet_linear(LinArgs a, const 
                                                                const int32_t *g_n, double *part) {
   extern __shared__ double lds[];  // rows [2][span], wave [span]
   const int ns = gridDim.y, set = blockIdx.y;
-  long j0 = (long)blockIdx.x * kLTPB;
-  long j = j0 + threadIdx.x;
-  long base = j0 + a.off_lo;
-  int span = a.span;
+  const int j0 = blockIdx.x * kLTPB;
+  const int j = j0 + threadIdx.x;
+  const int base = j0 + a.off_lo;
+  const int span = a.span;
   double *rw = lds, *wl = lds + 2 * span;
   lin_stage_wave(wl, a.wave, base, span, a.n);
-  LinTile T{rw, wl, base, span, a.n, a.wave};
+  const int lj = j - base;  // inside the window: -off_lo .. kLTPB - 1 - off_lo
   double lam_j = a.wave[j < a.n ? j : a.n - 1];
   double acc_q = 0.0, acc_s = 0.0;
   const CONSTANT LinGroup *gp = gc(groups);
@@ -404,15 +399,15 @@ __global__ __launch_bounds__(kLTPB) void k_gather_quiet_linear(LinArgs a, const 
       double L = G.w;
       LinShift s = group_shift(G);
       if (s.ident) {
-        acc_q += T.row_at(0, gq, j) * L;
-        acc_s += T.row_at(1, gs, j) * L;
+        acc_q += rw[lj] * L;
+        acc_s += rw[span + lj] * L;
         continue;
       }
-      long i = lin_gather_index(j, s);
-      double ls0 = lin_lshift(T.wave_at(i), s), ls1 = lin_lshift(T.wave_at(i + 1), s);
+      const int li = tile_local(lin_gather_index(j, s), base, span);
+      double ls0 = lin_lshift(wl[li], s), ls1 = lin_lshift(wl[li + 1], s);
       double t = (lam_j - ls0) / (ls1 - ls0);
-      double a0 = T.row_at(0, gq, i), a1 = T.row_at(0, gq, i + 1);
-      double b0 = T.row_at(1, gs, i), b1 = T.row_at(1, gs, i + 1);
+      double a0 = rw[li], a1 = rw[li + 1];
+      double b0 = rw[span + li], b1 = rw[span + li + 1];
       acc_q += (a0 + (a1 - a0) * t) * L;
       acc_s += (b0 + (b1 - b0) * t) * L;
     }
@@ -438,7 +433,7 @@ hipError_t launch_gather_quiet_linear(const LinArgs &a, const double *q, const d
                                       const int32_t *bin_off, const int32_t *g_n, double *part, double *out,
                                       hipStream_t s) {
   size_t lds = (size_t)3 * a.span * sizeof(double);
-  if (lds > 65536 || a.span < kLTPB || a.n_mu < 1) return hipErrorInvalidValue;
+  if (lds > 65536 || a.span < kLTPB || a.n_mu < 1 || a.n > kLinMaxN) return hipErrorInvalidValue;
   unsigned nb = (unsigned)((a.n + kLTPB - 1) / kLTPB);
   int ns = quiet_linear_sets(a.n_mu);
   k_gather_quiet_linear<<<dim3(nb, ns), kLTPB, lds, s>>>(a, q, qs, groups, bin_off, g_n, part);
@@ -466,14 +461,14 @@ __global__ __launch_bounds__(kLTPB) void k_gather_active_linear(LinArgs a, FirAr
   long tile = (slot / n_phase) * 8 + b % 8;
   int p = (int)(slot % n_phase);
   if (tile >= n_tiles) return;  // uniform over the block
-  long j0 = tile * kLTPB;
-  long j = j0 + threadIdx.x;
-  long jc = j < a.n ? j : a.n - 1;
-  long base = j0 + a.off_lo;
-  int span = a.span;
+  const int j0 = (int)tile * kLTPB;
+  const int j = j0 + threadIdx.x;
+  const int jc = j < a.n ? j : (int)a.n - 1;
+  const int base = j0 + a.off_lo;
+  const int span = a.span;
   double *rw = lds, *wl = lds + 4 * span;
   lin_stage_wave(wl, a.wave, base, span, a.n);
-  LinTile T{rw, wl, base, span, a.n, a.wave};
+  const int lj = j - base;  // inside the window: -off_lo .. kLTPB - 1 - off_lo
   double lam_j = a.wave[jc];
   double cs = f.sps ? f.sps[jc] : 0.0, cf = f.fas ? f.fas[jc] : 0.0;
   double hc = kPlanckH * kC;
@@ -497,18 +492,19 @@ __global__ __launch_bounds__(kLTPB) void k_gather_active_linear(LinArgs a, FirAr
       double L = G.w;
       int r_act = ty == 0 ? 2 : 3;
       LinShift s = group_shift(G);
+      const double *ra = rw + r_act * span;
       double iq, is, ia;  // shifted rows at pixel j
       if (s.ident) {
-        iq = T.row_at(0, g[0], j);
-        is = T.row_at(1, g[1], j);
-        ia = T.row_at(r_act, g[r_act], j);
+        iq = rw[lj];
+        is = rw[span + lj];
+        ia = ra[lj];
       } else {
-        long i = lin_gather_index(j, s);
-        double ls0 = lin_lshift(T.wave_at(i), s), ls1 = lin_lshift(T.wave_at(i + 1), s);
+        const int li = tile_local(lin_gather_index(j, s), base, span);
+        double ls0 = lin_lshift(wl[li], s), ls1 = lin_lshift(wl[li + 1], s);
         double t = (lam_j - ls0) / (ls1 - ls0);
-        double q0 = T.row_at(0, g[0], i), q1 = T.row_at(0, g[0], i + 1);
-        double s0 = T.row_at(1, g[1], i), s1 = T.row_at(1, g[1], i + 1);
-        double a0 = T.row_at(r_act, g[r_act], i), a1 = T.row_at(r_act, g[r_act], i + 1);
+        double q0 = rw[li], q1 = rw[li + 1];
+        double s0 = rw[span + li], s1 = rw[span + li + 1];
+        double a0 = ra[li], a1 = ra[li + 1];
         iq = q0 + (q1 - q0) * t;
         is = s0 + (s1 - s0) * t;
         ia = a0 + (a1 - a0) * t;
@@ -542,7 +538,7 @@ hipError_t launch_gather_active_linear(const LinArgs &a, const FirArgs &f, const
                                        int n_phase, const LinSolar &sol, hipStream_t s) {
   if (n_phase == 0) return hipSuccess;
   size_t lds = (size_t)5 * a.span * sizeof(double);
-  if (lds > 65536 || a.span < kLTPB) return hipErrorInvalidValue;
+  if (lds > 65536 || a.span < kLTPB || a.n > kLinMaxN) return hipErrorInvalidValue;
   int n_tiles = (int)((a.n + kLTPB - 1) / kLTPB);
   long tiles8 = ((long)n_tiles + 7) / 8 * 8;
   long blocks = tiles8 * n_phase;
