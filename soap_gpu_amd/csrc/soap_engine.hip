@@ -111,8 +111,9 @@ struct soap_ctx {
   // linear grids: staged source range per tile, bin-sorted disk cells, active cell
   // lists, and their shift groups with per-segment group counts
   int lin_off_lo = 0, lin_span = 0;
+  bool lin_solid = false;  // delta constant along each disk column (disk_init maps, omega1 = 0)
   DevBuf qcells, qbin_off, acells, abin_off, alist_off;
-  DevBuf qgroups, qg_n, agroups, ag_n, lcnt, qpart;
+  DevBuf qgroups, qg_n, agroups, ag_n, lcnt, qpart, qcolw, qcolf, qcols;
   // solar mode: Lagrange nodes over the facula temperature range
   int solar_k = 0, t_norm_fac = 0;
   double solar_t[kMaxSolarNodes] = {}, solar_w[kMaxSolarNodes] = {};
@@ -644,6 +645,7 @@ int soap_disk_init(soap_ctx *c, const soap_star *st) {
   a.theta_grid = 180.0 / st->grid;
   a.omega0 = 360.0 / st->prot_days;
   a.omega1 = a.omega0 * st->w_ratio;
+  c->lin_solid = a.omega1 == 0.0;
   a.radius = st->radius_km;
   a.sin_incl = sin(st->incl_deg * kPi / 180.);
   a.u1 = st->u1;
@@ -697,6 +699,7 @@ int soap_set_maps(soap_ctx *c, const soap_star *st, const double *delta, const d
   HIPCHK(c, hipSetDevice(c->device));
   int G = st->grid;
   size_t g2 = (size_t)G * G;
+  c->lin_solid = false;  // external maps: no column structure assumed
   int rc = alloc_maps(c, G);
   if (rc) return rc;
   c->star = *st;
@@ -777,8 +780,15 @@ int soap_quiet_star(soap_ctx *c) {
   if ((rc = dalloc(c, &c->d_quiet, (size_t)2 * c->n))) return rc;
   if (c->kind == SOAP_GRID_LINEAR) {
     size_t g2 = (size_t)c->G * c->G;
+    int ncs = quiet_column_sets(c->n_mu);
+    const bool by_cols = c->lin_solid && quiet_columns_fit(c->lin_span);
+    size_t nsets = (size_t)std::max(ncs, quiet_linear_sets(c->n_mu));
     if ((rc = ensure(c, c->qgroups, g2 * sizeof(LinGroup))) || (rc = ensure(c, c->qg_n, (size_t)c->n_mu * 4)) ||
-        (rc = ensure(c, c->qpart, (size_t)quiet_linear_sets(c->n_mu) * 2 * c->n * 8)))
+        (rc = ensure(c, c->qpart, nsets * 2 * c->n * 8)))
+      return rc;
+    if (by_cols && ((rc = ensure(c, c->qcolw, (size_t)ncs * c->G * 4 * 8)) ||
+                         (rc = ensure(c, c->qcolf, (size_t)ncs * c->G * 4)) ||
+                         (rc = ensure(c, c->qcols, (size_t)c->G * sizeof(LinGroup)))))
       return rc;
     {
       StageTimer tm(c, ST_TAPS_QUIET);
@@ -788,10 +798,16 @@ int soap_quiet_star(soap_ctx *c) {
     }
     {
       StageTimer tm(c, ST_FIR_QUIET);
-      HIPCHK(c, launch_gather_quiet_linear(lin_args(c), c->d_cube[SOAP_CUBE_QUIET], c->d_cube[SOAP_CUBE_QUIET_SED],
+      if (by_cols)
+        HIPCHK(c, launch_gather_quiet_cols(lin_args(c), c->d_cube[SOAP_CUBE_QUIET], c->d_cube[SOAP_CUBE_QUIET_SED],
                                            (const LinGroup *)c->qgroups.p, (const int32_t *)c->qbin_off.p,
-                                           (const int32_t *)c->qg_n.p, (double *)c->qpart.p, c->d_quiet,
-                                           c->stream));
+                                           (const int32_t *)c->qg_n.p, (double *)c->qcolw.p, (int32_t *)c->qcolf.p,
+                                           (LinGroup *)c->qcols.p, (double *)c->qpart.p, c->d_quiet, c->stream));
+      else
+        HIPCHK(c, launch_gather_quiet_linear(lin_args(c), c->d_cube[SOAP_CUBE_QUIET], c->d_cube[SOAP_CUBE_QUIET_SED],
+                                             (const LinGroup *)c->qgroups.p, (const int32_t *)c->qbin_off.p,
+                                             (const int32_t *)c->qg_n.p, (double *)c->qpart.p, c->d_quiet,
+                                             c->stream));
     }
     c->have_quiet = true;
     c->have_lower = c->have_ccf = c->have_fit = false;

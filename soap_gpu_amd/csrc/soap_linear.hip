@@ -442,6 +442,141 @@ hipError_t launch_gather_quiet_linear(const LinArgs &a, const double *q, const d
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------ quiet star by columns
+// Under solid-body rotation delta = y omega sin i is the same along a disk column
+// in every mu bin, so the quiet star's shift groups are (bin, column) pairs that
+// share their column's index and weight.  k_quiet_columns lays the groups out as
+// a table: per (bin set s, column iy) the weights of the set's kQColBins bins
+// (0 where the column does not meet the bin), a flag for columns with any weight,
+// and the column's shift; k_gather_quiet_cols then computes the index and weight
+// once per (pixel, column) for all bins of its set.
+constexpr int kQColBins = 4;
+
+// Column iy = thread: each bin's group of that column, found by binary search
+// over the bin's groups (in map order, so in column order).
+__global__ __launch_bounds__(kLTPB) void k_quiet_columns(const LinGroup *groups, const int32_t *bin_off,
+                                                         const int32_t *g_n, int n_mu, int grid, double *colw,
+                                                         int32_t *colf, LinGroup *cols) {
+  int iy = blockIdx.x * kLTPB + threadIdx.x;
+  if (iy >= grid) return;
+  int ns = (n_mu + kQColBins - 1) / kQColBins;
+  bool have = false;
+  for (int s = 0; s < ns; s++) {
+    int any = 0;
+    for (int kk = 0; kk < kQColBins; kk++) {
+      int k = s * kQColBins + kk;
+      double w = 0.0;
+      if (k < n_mu) {
+        const LinGroup *g = groups + bin_off[k];
+        int lo = 0, hi = g_n[k];  // first group whose column >= iy
+        while (lo < hi) {
+          int mid = (lo + hi) >> 1;
+          if ((g[mid].e & 0x3fffffff) / grid < iy) lo = mid + 1;
+          else hi = mid;
+        }
+        if (lo < g_n[k] && (g[lo].e & 0x3fffffff) / grid == iy) {
+          w = g[lo].w;
+          any = 1;
+          if (!have) {
+            cols[iy] = g[lo];
+            have = true;
+          }
+        }
+      }
+      colw[((long)s * grid + iy) * kQColBins + kk] = w;
+    }
+    colf[(long)s * grid + iy] = any;
+  }
+  if (!have) {  // no disk cell in this column: an identity shift, never used
+    LinGroup z{};
+    z.opf = 1.0;
+    z.rinv = 1.0;
+    z.ident = 1;
+    cols[iy] = z;
+  }
+}
+
+// Block (tile, s): the rows of bins s * kQColBins .. + kQColBins - 1 staged
+// together (rows of bins past n_mu staged as zeros, with weight 0); per column
+// with weight, one gather index and weight for all of them.
+__global__ __launch_bounds__(kLTPB) void k_gather_quiet_cols(LinArgs a, const double *q, const double *qs,
+                                                             const double *colw, const int32_t *colf,
+                                                             const LinGroup *cols, double *part) {
+  extern __shared__ double lds[];  // rows [kQColBins][2][span], wave [span]
+  const int set = blockIdx.y;
+  const int j0 = blockIdx.x * kLTPB;
+  const int j = j0 + threadIdx.x;
+  const int base = j0 + a.off_lo;
+  const int span = a.span;
+  double *rw = lds, *wl = lds + 2 * kQColBins * span;
+  lin_stage_wave(wl, a.wave, base, span, a.n);
+  for (int kk = 0; kk < kQColBins; kk++) {
+    int k = set * kQColBins + kk;
+    double *dq = rw + (2 * kk) * span, *ds = dq + span;
+    if (k < a.n_mu) {
+      lin_stage(dq, q + (long)k * a.n, base, span, a.n);
+      lin_stage(ds, qs + (long)k * a.n, base, span, a.n);
+    } else {
+      for (int e = threadIdx.x; e < 2 * span; e += blockDim.x) dq[e] = 0.0;
+    }
+  }
+  __syncthreads();
+  const int lj = j - base;  // inside the window: -off_lo .. kLTPB - 1 - off_lo
+  double lam_j = a.wave[j < a.n ? j : a.n - 1];
+  double acc_q = 0.0, acc_s = 0.0;
+  const CONSTANT LinGroup *cp = gc(cols);
+  const CONSTANT double *wp = gc(colw) + (long)set * a.grid * kQColBins;
+  const CONSTANT int32_t *fp = gc(colf) + (long)set * a.grid;
+  for (int iy = 0; iy < a.grid; iy++) {
+    if (!fp[iy]) continue;  // uniform
+    LinShift s = group_shift(cp[iy]);
+    const CONSTANT double *w = wp + (long)iy * kQColBins;
+    if (s.ident) {
+#pragma unroll
+      for (int kk = 0; kk < kQColBins; kk++) {
+        acc_q += rw[(2 * kk) * span + lj] * w[kk];
+        acc_s += rw[(2 * kk + 1) * span + lj] * w[kk];
+      }
+      continue;
+    }
+    const int li = tile_local(lin_gather_index(j, s), base, span);
+    double ls0 = lin_lshift(wl[li], s), ls1 = lin_lshift(wl[li + 1], s);
+    double t = (lam_j - ls0) / (ls1 - ls0);
+#pragma unroll
+    for (int kk = 0; kk < kQColBins; kk++) {
+      const double *dq = rw + (2 * kk) * span, *ds = dq + span;
+      double a0 = dq[li], a1 = dq[li + 1];
+      double b0 = ds[li], b1 = ds[li + 1];
+      acc_q += (a0 + (a1 - a0) * t) * w[kk];
+      acc_s += (b0 + (b1 - b0) * t) * w[kk];
+    }
+  }
+  if (j < a.n) {
+    double *o = part + (long)set * 2 * a.n;
+    o[j] = acc_q;
+    o[a.n + j] = acc_s;
+  }
+}
+
+int quiet_column_sets(int n_mu) { return (n_mu + kQColBins - 1) / kQColBins; }
+
+bool quiet_columns_fit(int span) { return (size_t)(2 * kQColBins + 1) * span * sizeof(double) <= 65536; }
+
+hipError_t launch_gather_quiet_cols(const LinArgs &a, const double *q, const double *qs, const LinGroup *groups,
+                                    const int32_t *bin_off, const int32_t *g_n, double *colw, int32_t *colf,
+                                    LinGroup *cols, double *part, double *out, hipStream_t s) {
+  size_t lds = (size_t)(2 * kQColBins + 1) * a.span * sizeof(double);
+  if (lds > 65536 || a.span < kLTPB || a.n_mu < 1 || a.n > kLinMaxN || a.grid < 1) return hipErrorInvalidValue;
+  k_quiet_columns<<<(unsigned)((a.grid + kLTPB - 1) / kLTPB), kLTPB, 0, s>>>(groups, bin_off, g_n, a.n_mu, a.grid,
+                                                                             colw, colf, cols);
+  unsigned nb = (unsigned)((a.n + kLTPB - 1) / kLTPB);
+  int ns = quiet_column_sets(a.n_mu);
+  k_gather_quiet_cols<<<dim3(nb, ns), kLTPB, lds, s>>>(a, q, qs, colw, colf, cols, part);
+  long n2 = 2 * a.n;
+  k_quiet_sum<<<(unsigned)((n2 + kLTPB - 1) / kLTPB), kLTPB, 0, s>>>(part, ns, n2, out);
+  return hipGetLastError();
+}
+
 // Active deltas of phase p on a linear grid (spot_integrate + fast_flux / bconv /
 // tot, SOAP_integration.cu:285-307,392-447; solar variant :433 of
 // SOAP_integration_solar.cu): per active cell, Q = Shift(Sq_k), QS = Shift(SEDq_k),

@@ -139,3 +139,10 @@ def test_linear_differential_rotation(engine):
     w = synth.linear_small(seed=13, grid=64, n=3000, n_phase=2)
     w = dataclasses.replace(w, star=synth._star(64, 3.0, incl=70.0, w_ratio=0.3))
     _linear_parity(engine, w)
+
+
+def test_linear_fast_rotator(engine):
+    """vsini 150 km/s: shifts of about 500 pixels widen the staged window past what
+    the column form of the quiet star keeps in LDS, so the quiet star takes the
+    per-group form under solid-body rotation too."""
+    _linear_parity(engine, synth.linear_small(seed=14, grid=64, n=4000, n_phase=2, vsini=150.0))
