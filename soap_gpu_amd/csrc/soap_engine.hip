@@ -567,6 +567,8 @@ static int linear_prepare(soap_ctx *c, double vmax) {
       hi = std::max(hi, off);
     }
   }
+  if (c->n > 0x7fffffffL - 4096)
+    return fail(c, SOAP_ERR_LIMIT, "linear grid: %lld pixels (the gathers index pixels in 32 bits)", (long long)c->n);
   c->lin_off_lo = (int)floor(lo) - 3;
   c->lin_span = 256 + ((int)ceil(hi) + 4 - c->lin_off_lo);
   if (c->lin_span > 1600)
