@@ -552,6 +552,10 @@ static int set_tap_range(soap_ctx *c, double vmax_kms) {
   return SOAP_OK;
 }
 
+// Linear grids, solid-body rotation: the quiet star from the disk maps by columns
+// (no quiet cell lists or groups), where a bin set's rows fit the block's LDS.
+static bool quiet_by_columns(const soap_ctx *c) { return c->lin_solid && quiet_columns_fit(c->lin_span); }
+
 // Linear grids: the staged source-index range of a pixel tile for shifts up to
 // vmax (i(j) - j = (-fbeta lambda_0/step - fbeta j)/(1 + fbeta) at the extreme
 // fbeta and j, plus a margin), and the bin-sorted lists of the disk cells.
@@ -574,6 +578,7 @@ static int linear_prepare(soap_ctx *c, double vmax) {
   if (c->lin_span > 1600)
     return fail(c, SOAP_ERR_LIMIT, "linear grid: shifts of up to %.1f pixels exceed the tile window", hi - lo);
   int rc;
+  if (quiet_by_columns(c)) return SOAP_OK;
   size_t g2 = (size_t)c->G * c->G;
   CellSrc src{};
   src.grid = c->G;
@@ -783,16 +788,16 @@ int soap_quiet_star(soap_ctx *c) {
   if (c->kind == SOAP_GRID_LINEAR) {
     size_t g2 = (size_t)c->G * c->G;
     int ncs = quiet_column_sets(c->n_mu);
-    const bool by_cols = c->lin_solid && quiet_columns_fit(c->lin_span);
+    const bool by_cols = quiet_by_columns(c);
     size_t nsets = (size_t)std::max(ncs, quiet_linear_sets(c->n_mu));
-    if ((rc = ensure(c, c->qgroups, g2 * sizeof(LinGroup))) || (rc = ensure(c, c->qg_n, (size_t)c->n_mu * 4)) ||
-        (rc = ensure(c, c->qpart, nsets * 2 * c->n * 8)))
+    if ((rc = ensure(c, c->qpart, nsets * 2 * c->n * 8))) return rc;
+    if (!by_cols && ((rc = ensure(c, c->qgroups, g2 * sizeof(LinGroup))) || (rc = ensure(c, c->qg_n, (size_t)c->n_mu * 4))))
       return rc;
     if (by_cols && ((rc = ensure(c, c->qcolw, (size_t)ncs * c->G * 4 * 8)) ||
                          (rc = ensure(c, c->qcolf, (size_t)ncs * c->G * 4)) ||
                          (rc = ensure(c, c->qcols, (size_t)c->G * sizeof(LinGroup)))))
       return rc;
-    {
+    if (!by_cols) {
       StageTimer tm(c, ST_TAPS_QUIET);
       HIPCHK(c, launch_lin_groups((const int32_t *)c->qcells.p, nullptr, (const int32_t *)c->qbin_off.p, 1, c->n_mu,
                                   c->d_delta, c->d_limb, c->d_lbm, c->wave[0], c->step, 0,
@@ -802,8 +807,7 @@ int soap_quiet_star(soap_ctx *c) {
       StageTimer tm(c, ST_FIR_QUIET);
       if (by_cols)
         HIPCHK(c, launch_gather_quiet_cols(lin_args(c), c->d_cube[SOAP_CUBE_QUIET], c->d_cube[SOAP_CUBE_QUIET_SED],
-                                           (const LinGroup *)c->qgroups.p, (const int32_t *)c->qbin_off.p,
-                                           (const int32_t *)c->qg_n.p, (double *)c->qcolw.p, (int32_t *)c->qcolf.p,
+                                           c->d_region, (double *)c->qcolw.p, (int32_t *)c->qcolf.p,
                                            (LinGroup *)c->qcols.p, (double *)c->qpart.p, c->d_quiet, c->stream));
       else
         HIPCHK(c, launch_gather_quiet_linear(lin_args(c), c->d_cube[SOAP_CUBE_QUIET], c->d_cube[SOAP_CUBE_QUIET_SED],

@@ -412,12 +412,12 @@ hipError_t launch_gather_quiet_linear(const LinArgs &a, const double *q, const d
                                       hipStream_t s);
 int quiet_linear_sets(int n_mu);
 // Solid-body rotation only (delta constant along each disk column): the quiet star
-// from the groups laid out per (bin set, column); colw: quiet_column_sets(n_mu) *
-// grid * 4 doubles, colf: quiet_column_sets(n_mu) * grid ints, cols: grid groups,
-// part: quiet_column_sets(n_mu) * 2 * n doubles
-hipError_t launch_gather_quiet_cols(const LinArgs &a, const double *q, const double *qs, const LinGroup *groups,
-                                    const int32_t *bin_off, const int32_t *g_n, double *colw, int32_t *colf,
-                                    LinGroup *cols, double *part, double *out, hipStream_t s);
+// from the disk maps' limb weights summed per (bin, column); colw:
+// quiet_column_sets(n_mu) * grid * 4 doubles, colf: quiet_column_sets(n_mu) * grid
+// ints, cols: grid groups, part: quiet_column_sets(n_mu) * 2 * n doubles
+hipError_t launch_gather_quiet_cols(const LinArgs &a, const double *q, const double *qs, const int32_t *region,
+                                    double *colw, int32_t *colf, LinGroup *cols, double *part, double *out,
+                                    hipStream_t s);
 int quiet_column_sets(int n_mu);
 bool quiet_columns_fit(int span);  // the staged rows of a bin set fit the block's LDS
 hipError_t launch_gather_active_linear(const LinArgs &a, const FirArgs &f, const LinGroup *groups,

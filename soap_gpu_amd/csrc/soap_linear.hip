@@ -445,54 +445,65 @@ hipError_t launch_gather_quiet_linear(const LinArgs &a, const double *q, const d
 // ------------------------------------------------------------ quiet star by columns
 // Under solid-body rotation delta = y omega sin i is the same along a disk column
 // in every mu bin, so the quiet star's shift groups are (bin, column) pairs that
-// share their column's index and weight.  k_quiet_columns lays the groups out as
-// a table: per (bin set s, column iy) the weights of the set's kQColBins bins
-// (0 where the column does not meet the bin), a flag for columns with any weight,
-// and the column's shift; k_gather_quiet_cols then computes the index and weight
-// once per (pixel, column) for all bins of its set.
+// share their column's index and weight.  k_quiet_columns_maps lays them out as a
+// table: per (bin set s, column iy) the weights of the set's kQColBins bins (0
+// where the column does not meet the bin), a flag for columns with any weight, and
+// the column's shift; k_gather_quiet_cols then computes the index and weight once
+// per (pixel, column) for all bins of its set.
 constexpr int kQColBins = 4;
 
-// Column iy = thread: each bin's group of that column, found by binary search
-// over the bin's groups (in map order, so in column order).
-__global__ __launch_bounds__(kLTPB) void k_quiet_columns(const LinGroup *groups, const int32_t *bin_off,
-                                                         const int32_t *g_n, int n_mu, int grid, double *colw,
-                                                         int32_t *colf, LinGroup *cols) {
-  int iy = blockIdx.x * kLTPB + threadIdx.x;
-  if (iy >= grid) return;
-  int ns = (n_mu + kQColBins - 1) / kQColBins;
-  bool have = false;
+// The same table straight from the disk maps: block = column iy (one wave), the
+// lanes over the column's cells; per bin, each lane sums its cells (z = lane,
+// lane + 64, ...) and a fixed butterfly adds the 64 partials.  The column's shift
+// comes from its first disk cell.
+__global__ __launch_bounds__(64) void k_quiet_columns_maps(const double *limb, const double *delta,
+                                                           const int32_t *region, int n_mu, int grid, double lambda0,
+                                                           double step, double *colw, int32_t *colf,
+                                                           LinGroup *cols) {
+#pragma clang fp contract(off)
+  const int iy = blockIdx.x, lane = threadIdx.x;
+  const long row = (long)iy * grid;
+  const int ns = (n_mu + kQColBins - 1) / kQColBins;
+  int first = -1;
+  for (int z0 = 0; z0 < grid && first < 0; z0 += 64) {
+    const int iz = z0 + lane;
+    const bool disk = iz < grid && !(limb[row + iz] == -999.0 && delta[row + iz] == -999.0);
+    const unsigned long long m = __ballot(disk);
+    if (m) first = z0 + __ffsll((long long)m) - 1;
+  }
   for (int s = 0; s < ns; s++) {
     int any = 0;
     for (int kk = 0; kk < kQColBins; kk++) {
-      int k = s * kQColBins + kk;
+      const int k = s * kQColBins + kk;
       double w = 0.0;
       if (k < n_mu) {
-        const LinGroup *g = groups + bin_off[k];
-        int lo = 0, hi = g_n[k];  // first group whose column >= iy
-        while (lo < hi) {
-          int mid = (lo + hi) >> 1;
-          if ((g[mid].e & 0x3fffffff) / grid < iy) lo = mid + 1;
-          else hi = mid;
+        for (int z0 = 0; z0 < grid; z0 += 64) {
+          const long idx = row + z0 + lane;
+          if (z0 + lane < grid && region[idx] == k && !(limb[idx] == -999.0 && delta[idx] == -999.0)) w += limb[idx];
         }
-        if (lo < g_n[k] && (g[lo].e & 0x3fffffff) / grid == iy) {
-          w = g[lo].w;
-          any = 1;
-          if (!have) {
-            cols[iy] = g[lo];
-            have = true;
-          }
-        }
+        for (int o = 32; o > 0; o >>= 1) w += __shfl_xor(w, o, 64);
       }
-      colw[((long)s * grid + iy) * kQColBins + kk] = w;
+      any |= w != 0.0;
+      if (lane == 0) colw[((long)s * grid + iy) * kQColBins + kk] = w;
     }
-    colf[(long)s * grid + iy] = any;
+    if (lane == 0) colf[(long)s * grid + iy] = any;
   }
-  if (!have) {  // no disk cell in this column: an identity shift, never used
-    LinGroup z{};
-    z.opf = 1.0;
-    z.rinv = 1.0;
-    z.ident = 1;
-    cols[iy] = z;
+  if (lane == 0) {
+    LinGroup g{};
+    if (first >= 0) {
+      LinShift sh = lin_shift(delta[row + first], lambda0, step);
+      g.opf = sh.opf;
+      g.fl = sh.fl;
+      g.c1 = sh.c1;
+      g.rinv = sh.rinv;
+      g.e = (int)(row + first);
+      g.ident = sh.ident ? 1 : 0;
+    } else {  // no disk cell in this column: an identity shift, never used
+      g.opf = 1.0;
+      g.rinv = 1.0;
+      g.ident = 1;
+    }
+    cols[iy] = g;
   }
 }
 
@@ -562,13 +573,13 @@ int quiet_column_sets(int n_mu) { return (n_mu + kQColBins - 1) / kQColBins; }
 
 bool quiet_columns_fit(int span) { return (size_t)(2 * kQColBins + 1) * span * sizeof(double) <= 65536; }
 
-hipError_t launch_gather_quiet_cols(const LinArgs &a, const double *q, const double *qs, const LinGroup *groups,
-                                    const int32_t *bin_off, const int32_t *g_n, double *colw, int32_t *colf,
-                                    LinGroup *cols, double *part, double *out, hipStream_t s) {
+hipError_t launch_gather_quiet_cols(const LinArgs &a, const double *q, const double *qs, const int32_t *region,
+                                    double *colw, int32_t *colf, LinGroup *cols, double *part, double *out,
+                                    hipStream_t s) {
   size_t lds = (size_t)(2 * kQColBins + 1) * a.span * sizeof(double);
   if (lds > 65536 || a.span < kLTPB || a.n_mu < 1 || a.n > kLinMaxN || a.grid < 1) return hipErrorInvalidValue;
-  k_quiet_columns<<<(unsigned)((a.grid + kLTPB - 1) / kLTPB), kLTPB, 0, s>>>(groups, bin_off, g_n, a.n_mu, a.grid,
-                                                                             colw, colf, cols);
+  k_quiet_columns_maps<<<(unsigned)a.grid, 64, 0, s>>>(a.limb, a.delta, region, a.n_mu, a.grid, a.lambda0, a.step,
+                                                       colw, colf, cols);
   unsigned nb = (unsigned)((a.n + kLTPB - 1) / kLTPB);
   int ns = quiet_column_sets(a.n_mu);
   k_gather_quiet_cols<<<dim3(nb, ns), kLTPB, lds, s>>>(a, q, qs, colw, colf, cols, part);
