@@ -72,7 +72,9 @@ def _short(name):
               "k_ccf_reduce", "k_ccf_lds",
               "k_fir_quiet_pipe", "k_fir_quiet_lds", "k_conv_quiet", "k_taps_active", "k_taps_quiet", "k_minmax_reduce", "k_fit",
               "k_mask_build", "k_mask_overwrite", "k_mask_wsum", "k_mask_footprint", "k_disk_init",
-              "k_taps_finalize", "k_fill_u64"):
+              "k_taps_finalize", "k_fill_u64",
+              "k_gather_active_linear", "k_gather_quiet_cols", "k_gather_quiet_linear", "k_quiet_columns_maps",
+              "k_lin_groups", "k_cells_count", "k_cells_write", "k_conv_linear"):
         if (name.endswith(k) or re.search(r"soap::" + k + r"[(<]", name)
                 or ("4soap" + str(len(k)) + k + "E") in name or ("4soap" + str(len(k)) + k + "I") in name):
             return k
