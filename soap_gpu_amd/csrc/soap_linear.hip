@@ -356,13 +356,10 @@ hipError_t launch_lin_groups(const int32_t *cells, const int64_t *list_off, cons
   long nseg = (long)n_phase * n_mu;
   if (nseg == 0) return hipSuccess;
   if (nseg > 0x7fffffffL) return hipErrorInvalidValue;
-  // few long segments (the quiet disk: one per bin) take 1024-cell chunks
-  if (n_phase == 1)
-    k_lin_groups<1024><<<(unsigned)nseg, 1024, 0, s>>>(cells, list_off, bin_off, n_mu, delta, limb, lbm, lambda0,
-                                                       step, split_fac, groups, g_n);
-  else
-    k_lin_groups<kLTPB><<<(unsigned)nseg, kLTPB, 0, s>>>(cells, list_off, bin_off, n_mu, delta, limb, lbm, lambda0,
-                                                         step, split_fac, groups, g_n);
+  // 1024-cell chunks: the time goes to the longest segments (the active cells of
+  // a phase crowd into the few bins its regions cover)
+  k_lin_groups<1024><<<(unsigned)nseg, 1024, 0, s>>>(cells, list_off, bin_off, n_mu, delta, limb, lbm, lambda0, step,
+                                                     split_fac, groups, g_n);
   return hipGetLastError();
 }
 
